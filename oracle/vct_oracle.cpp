@@ -1,0 +1,1219 @@
+// vct_oracle.cpp - CPU oracle of the voxel cone tracing hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+// product (voxel_cone_tracing_amd/libvct.so) never calls it.
+//
+// It restates, in plain C++ on the CPU, the reference's three GL passes:
+//   voxelize   shader/voxelize.vert:24-30, shader/voxelize.geom:25-55,
+//              shader/voxelize.frag:66-161, GL state src/renderer.cpp:316-348
+//   mip        shader/mipmap.comp:10-99, host loop src/renderer.cpp:283-314
+//   trace      shader/voxel_cone_tracing.vert:22-28,
+//              shader/voxel_cone_tracing.frag:61-275, src/renderer.cpp:355-390
+// plus the GL fixed-function parts the shaders rely on: rasterisation at
+// pixel centres (vertices snapped to 1/256 pixel, top-left fill rule),
+// barycentric attribute interpolation, GL_LESS depth, texelFetch, textureLod
+// with LINEAR_MIPMAP_LINEAR + CLAMP_TO_BORDER (src/texture_3d.cpp:10-15), and
+// unorm8 conversions.
+//
+// Two arithmetic modes:
+//  - FP32 faithful: single precision, every product and sum rounded on its
+//    own (built with -ffp-contract=off), correctly rounded / and sqrt.  The
+//    integer outputs (coverage, voxel coordinates, RGBA8 running average,
+//    mips) must match the GPU bit for bit.
+//  - FP64 ideal: the same discrete decisions (coverage, voxel addresses,
+//    visibility) but every value computed in double, the exact mean of the
+//    fragments of a voxel, no storage rounding.  Every error is stated
+//    against it.
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+extern "C" {
+
+// material_data_t (src/renderer.h:94-120): 128 bytes, packed.
+typedef struct {
+  float ambient[4], diffuse[4], specular[4], transmittance[4];
+  float emission[3];
+  float shininess, ior, dissolve;
+  int32_t illum;
+  float roughness, metallic, sheen, clearcoat_thickness, clearcoat_roughness, anisotropy, anisotropy_rotation;
+  float pad[2];
+} orc_material;
+
+// point_light_t (src/renderer.h:87-92) + kind: 0 point, 1 directional
+// (position = direction towards the light).
+typedef struct {
+  float position[3], color[3], intensity;
+  int32_t kind;
+} orc_light;
+
+}  // extern "C"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// storage conversions
+// ---------------------------------------------------------------------------
+
+// float -> IEEE binary16, round to nearest even (written out bit by bit).
+uint16_t half_from_float(float f) {
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  const uint32_t absx = x & 0x7fffffffu;
+  if (absx > 0x7f800000u) return (uint16_t)(sign | 0x7e00u);  // NaN
+  if (absx >= 0x477ff000u) {  // >= 65520 rounds to inf
+    return (uint16_t)(sign | 0x7c00u);
+  }
+  const int e = (int)(absx >> 23) - 127;
+  uint32_t mant = (absx & 0x7fffffu) | 0x800000u;
+  if (e < -25) return (uint16_t)sign;  // below half the smallest subnormal
+  if (e < -14) {
+    // subnormal: value = mant * 2^(e-23); unit = 2^-24
+    const int shift = -e - 14 + 13;  // bits to drop
+    uint32_t q = mant >> shift;
+    const uint32_t rem = mant & ((1u << shift) - 1u);
+    const uint32_t halfway = 1u << (shift - 1);
+    if (rem > halfway || (rem == halfway && (q & 1u))) ++q;
+    return (uint16_t)(sign | q);
+  }
+  uint32_t q = ((uint32_t)(e + 15) << 10) | ((mant >> 13) & 0x3ffu);
+  const uint32_t rem = mant & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) ++q;  // carries into the exponent correctly
+  return (uint16_t)(sign | q);
+}
+
+float half_to_float(uint16_t h) {
+  const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  const int e = (h >> 10) & 0x1f;
+  const uint32_t m = h & 0x3ffu;
+  float v;
+  if (e == 0) v = ldexpf((float)m, -24);
+  else if (e == 31) v = m ? NAN : INFINITY;
+  else v = ldexpf((float)(m | 0x400u), e - 25);
+  uint32_t bits;
+  memcpy(&bits, &v, 4);
+  bits |= sign;
+  memcpy(&v, &bits, 4);
+  return v;
+}
+
+uint16_t bf16_from_float(float f) {
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
+  const uint32_t lower = x & 0xffffu, upper = x >> 16;
+  if (lower > 0x8000u || (lower == 0x8000u && (upper & 1u))) return (uint16_t)(upper + 1u);
+  return (uint16_t)upper;
+}
+
+float bf16_to_float(uint16_t h) {
+  uint32_t x = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &x, 4);
+  return f;
+}
+
+// GL: unorm8 -> float is c / 255.
+float unorm8_to_float(uint32_t c) { return (float)c / 255.0f; }
+// GL: float -> unorm8 is clamp to [0,1], times 255, round to nearest
+// (ties to even here).
+uint32_t float_to_unorm8(float f) {
+  if (!(f > 0.0f)) f = 0.0f;
+  if (f > 1.0f) f = 1.0f;
+  return (uint32_t)nearbyintf(f * 255.0f);
+}
+
+// Round a float to the storage type and back (dtype: 0 rgba8, 1 fp16, 2 bf16, 3 fp32).
+float store_round(float v, int dtype) {
+  switch (dtype) {
+    case 0: return unorm8_to_float(float_to_unorm8(v));
+    case 1: return half_to_float(half_from_float(v));
+    case 2: return bf16_to_float(bf16_from_float(v));
+    default: return v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// small vector helpers (float and double)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct V3 {
+  T x, y, z;
+};
+template <typename T>
+V3<T> mk(T x, T y, T z) {
+  return V3<T>{x, y, z};
+}
+template <typename T>
+T dot(V3<T> a, V3<T> b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <typename T>
+V3<T> sub(V3<T> a, V3<T> b) {
+  return mk(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+template <typename T>
+V3<T> add(V3<T> a, V3<T> b) {
+  return mk(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+template <typename T>
+V3<T> scale(T s, V3<T> a) {
+  return mk(s * a.x, s * a.y, s * a.z);
+}
+template <typename T>
+V3<T> cross(V3<T> a, V3<T> b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+template <typename T>
+T length(V3<T> a) {
+  return sqrt(dot(a, a));
+}
+// normalize(v) = v / length(v)
+template <typename T>
+V3<T> normalize(V3<T> a) {
+  T l = length(a);
+  return mk(a.x / l, a.y / l, a.z / l);
+}
+inline float sqrt(float x) { return sqrtf(x); }
+
+// mat3(transpose(inverse(M))) for the normal (voxelize.vert:28), computed in
+// double from the upper 3x3 and rounded to float; column-major 3x3 result.
+void normal_matrix(const float* M, float* N) {
+  double a[3][3];
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) a[r][c] = M[c * 4 + r];
+  const double c00 = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+  const double c01 = a[1][2] * a[2][0] - a[1][0] * a[2][2];
+  const double c02 = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+  const double det = a[0][0] * c00 + a[0][1] * c01 + a[0][2] * c02;
+  // inverse(A)^T = cofactor(A) / det
+  double cof[3][3];
+  cof[0][0] = c00;
+  cof[0][1] = c01;
+  cof[0][2] = c02;
+  cof[1][0] = a[0][2] * a[2][1] - a[0][1] * a[2][2];
+  cof[1][1] = a[0][0] * a[2][2] - a[0][2] * a[2][0];
+  cof[1][2] = a[0][1] * a[2][0] - a[0][0] * a[2][1];
+  cof[2][0] = a[0][1] * a[1][2] - a[0][2] * a[1][1];
+  cof[2][1] = a[0][2] * a[1][0] - a[0][0] * a[1][2];
+  cof[2][2] = a[0][0] * a[1][1] - a[0][1] * a[1][0];
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) N[c * 3 + r] = (float)(cof[r][c] / det);
+}
+
+// M (column-major 4x4) applied to (p, 1): x = m0 p.x + m4 p.y + m8 p.z + m12.
+template <typename T>
+V3<T> xform_point(const float* M, const float* p) {
+  T r[3];
+  for (int k = 0; k < 3; ++k) {
+    T t = (T)M[k] * (T)p[0];
+    t = t + (T)M[4 + k] * (T)p[1];
+    t = t + (T)M[8 + k] * (T)p[2];
+    t = t + (T)M[12 + k];
+    r[k] = t;
+  }
+  return mk(r[0], r[1], r[2]);
+}
+template <typename T>
+V3<T> xform_normal(const float* N, const float* n) {
+  T r[3];
+  for (int k = 0; k < 3; ++k) {
+    T t = (T)N[k] * (T)n[0];
+    t = t + (T)N[3 + k] * (T)n[1];
+    t = t + (T)N[6 + k] * (T)n[2];
+    r[k] = t;
+  }
+  return mk(r[0], r[1], r[2]);
+}
+
+// ---------------------------------------------------------------------------
+// rasteriser: GL pixel-centre sampling, 1/256 subpixel snap, top-left rule
+// ---------------------------------------------------------------------------
+struct Tri2D {
+  int64_t X[3], Y[3];
+  int64_t area;  // > 0 after orientation fix
+  int order[3];  // CCW order of the original vertices
+  int imin, imax, jmin, jmax;
+  bool ok;
+};
+
+// Window coordinates (pixel units, y up) -> snapped fixed point with the
+// pixel centre (i + 0.5) at integer i * 256.
+int64_t snap(float w) { return (int64_t)nearbyintf((w - 0.5f) * 256.0f); }
+
+void setup2d(const float* xw, const float* yw, int W, int H, Tri2D& t) {
+  for (int v = 0; v < 3; ++v) {
+    t.X[v] = snap(xw[v]);
+    t.Y[v] = snap(yw[v]);
+  }
+  int64_t a = (t.X[1] - t.X[0]) * (t.Y[2] - t.Y[0]) - (t.Y[1] - t.Y[0]) * (t.X[2] - t.X[0]);
+  t.ok = a != 0;
+  if (a < 0) {
+    t.order[0] = 0;
+    t.order[1] = 2;
+    t.order[2] = 1;
+    a = -a;
+  } else {
+    t.order[0] = 0;
+    t.order[1] = 1;
+    t.order[2] = 2;
+  }
+  t.area = a;
+  int64_t xmin = std::min(t.X[0], std::min(t.X[1], t.X[2])), xmax = std::max(t.X[0], std::max(t.X[1], t.X[2]));
+  int64_t ymin = std::min(t.Y[0], std::min(t.Y[1], t.Y[2])), ymax = std::max(t.Y[0], std::max(t.Y[1], t.Y[2]));
+  // pixels whose centre lies inside the bounding box, clipped to the viewport
+  int64_t imin = (xmin + 255) >> 8, imax = xmax >> 8, jmin = (ymin + 255) >> 8, jmax = ymax >> 8;
+  imin = std::max<int64_t>(imin, 0);
+  jmin = std::max<int64_t>(jmin, 0);
+  imax = std::min<int64_t>(imax, W - 1);
+  jmax = std::min<int64_t>(jmax, H - 1);
+  t.imin = (int)imin;
+  t.imax = (int)imax;
+  t.jmin = (int)jmin;
+  t.jmax = (int)jmax;
+  if (imin > imax || jmin > jmax) t.ok = false;
+}
+
+// Edge a->b of a CCW triangle owns its boundary pixels if it is a top edge
+// (horizontal, going left) or a left edge (going down).
+bool owns_edge(int64_t dx, int64_t dy) { return dy < 0 || (dy == 0 && dx < 0); }
+
+// Coverage of pixel (i, j); on success the barycentric numerators of the
+// original vertices 0,1,2 (their sum is t.area).
+bool covers(const Tri2D& t, int i, int j, int64_t w[3]) {
+  const int64_t px = (int64_t)i * 256, py = (int64_t)j * 256;
+  const int a = t.order[0], b = t.order[1], c = t.order[2];
+  int64_t e[3];
+  // e[k] = orient(edge opposite vertex k of the CCW triangle, p)
+  e[0] = (t.X[c] - t.X[b]) * (py - t.Y[b]) - (t.Y[c] - t.Y[b]) * (px - t.X[b]);
+  e[1] = (t.X[a] - t.X[c]) * (py - t.Y[c]) - (t.Y[a] - t.Y[c]) * (px - t.X[c]);
+  e[2] = (t.X[b] - t.X[a]) * (py - t.Y[a]) - (t.Y[b] - t.Y[a]) * (px - t.X[a]);
+  if (e[0] < 0 || e[1] < 0 || e[2] < 0) return false;
+  if (e[0] == 0 && !owns_edge(t.X[c] - t.X[b], t.Y[c] - t.Y[b])) return false;
+  if (e[1] == 0 && !owns_edge(t.X[a] - t.X[c], t.Y[a] - t.Y[c])) return false;
+  if (e[2] == 0 && !owns_edge(t.X[b] - t.X[a], t.Y[b] - t.Y[a])) return false;
+  w[a] = e[0];
+  w[b] = e[1];
+  w[c] = e[2];
+  return true;
+}
+
+// Visit covered pixels in the canonical order: 2x2 quads aligned to even
+// pixel coordinates, quads row by row (y up, then x), pixels of a quad in
+// the order (0,0) (1,0) (0,1) (1,1).
+template <typename F>
+void for_each_pixel(const Tri2D& t, int W, int H, F&& f) {
+  if (!t.ok) return;
+  const int qx0 = t.imin & ~1, qy0 = t.jmin & ~1;
+  for (int qy = qy0; qy <= t.jmax; qy += 2)
+    for (int qx = qx0; qx <= t.imax; qx += 2)
+      for (int s = 0; s < 4; ++s) {
+        const int i = qx + (s & 1), j = qy + (s >> 1);
+        if (i >= W || j >= H) continue;
+        int64_t w[3];
+        if (covers(t, i, j, w)) f(i, j, w);
+      }
+}
+
+// ---------------------------------------------------------------------------
+// voxelize.frag lighting
+// ---------------------------------------------------------------------------
+bool transmissive(int illum) { return illum == 4 || illum == 6 || illum == 7 || illum == 9; }
+
+template <typename T>
+void frag_color(V3<T> pos, V3<T> nrm_interp, const orc_material& m, const orc_light* lights, int nlights,
+                T cube_size, T out[4]) {
+  V3<T> color = mk<T>(0, 0, 0);
+  const V3<T> n = normalize(nrm_interp);
+  const int nl = std::min(nlights, 10);
+  for (int i = 0; i < nl; ++i) {
+    const orc_light& L = lights[i];
+    V3<T> dir;
+    T att;
+    if (L.kind == 1) {
+      dir = normalize(mk<T>(L.position[0], L.position[1], L.position[2]));
+      att = (T)1;
+    } else {
+      const V3<T> lp = mk<T>((T)L.position[0] / cube_size, (T)L.position[1] / cube_size, (T)L.position[2] / cube_size);
+      const V3<T> dv = sub(lp, pos);
+      dir = normalize(dv);
+      const T d = length(dv);
+      att = (T)1 / ((T)1 + d * d);  // attenuate(1, 0, 1, d)
+    }
+    T cs = dot(n, dir);
+    cs = cs > (T)0 ? cs : (T)0;
+    const T ca = cs * att;
+    color.x = color.x + ca * (T)L.color[0] * (T)L.intensity;
+    color.y = color.y + ca * (T)L.color[1] * (T)L.intensity;
+    color.z = color.z + ca * (T)L.color[2] * (T)L.intensity;
+  }
+  T c[3] = {(T)m.diffuse[0] * color.x + (T)m.emission[0], (T)m.diffuse[1] * color.y + (T)m.emission[1],
+            (T)m.diffuse[2] * color.z + (T)m.emission[2]};
+  T tr[3] = {1, 1, 1};
+  T alpha = 1;
+  if (transmissive(m.illum)) {
+    tr[0] = m.transmittance[0];
+    tr[1] = m.transmittance[1];
+    tr[2] = m.transmittance[2];
+    alpha = m.dissolve;
+  }
+  for (int k = 0; k < 3; ++k) out[k] = tr[k] * c[k];
+  out[3] = alpha;
+  for (int k = 0; k < 4; ++k) out[k] = std::min(std::max(out[k], (T)0), (T)1);
+}
+
+struct Frag {
+  uint32_t vox;   // linear index (z * res + y) * res + x
+  uint64_t order;  // canonical sequence number
+  float c32[4];
+  double c64[4];
+};
+
+// imageAtomicRGBA8Avg helpers (voxelize.frag:66-93)
+uint32_t conv_vec4_to_rgba8(const float v[4]) {
+  return (((uint32_t)v[3] & 0xffu) << 24) | (((uint32_t)v[2] & 0xffu) << 16) | (((uint32_t)v[1] & 0xffu) << 8) |
+         ((uint32_t)v[0] & 0xffu);
+}
+uint32_t enc_unsigned_nibble(uint32_t m, uint32_t n) {
+  return (m & 0xFEFEFEFEu) | (n & 0x1u) | (n & 0x2u) << 7 | (n & 0x4u) << 14 | (n & 0x8u) << 21;
+}
+uint32_t dec_unsigned_nibble(uint32_t m) {
+  return (m & 0x1u) | (m & 0x100u) >> 7 | (m & 0x10000u) >> 14 | (m & 0x1000000u) >> 21;
+}
+
+}  // namespace
+
+extern "C" {
+
+// GLSL round() tie direction used by the running average (0: ties to even,
+// 1: ties away from zero).  GLSL leaves it to the implementation.
+static int g_round_away = 0;
+void orc_set_round_mode(int away) { g_round_away = away; }
+
+static float glsl_round(float x) { return g_round_away ? roundf(x) : nearbyintf(x); }
+
+// Sequential imageAtomicRGBA8Avg (voxelize.frag:95-120) of one fragment
+// colour into a stored value: the CAS loop of one invocation with no other
+// writer in between.
+uint32_t orc_rgba8_avg_step(uint32_t stored, const float color[4]) {
+  float val[4];
+  for (int k = 0; k < 4; ++k) val[k] = color[k] * 255.0f;
+  uint32_t newv = enc_unsigned_nibble(conv_vec4_to_rgba8(val), 1u);
+  if (stored == 0u) return newv;  // imageAtomicCompSwap(0 -> new) succeeds
+  const uint32_t cur = stored & 0xFEFEFEFEu;
+  float r[4] = {(float)(cur & 0xffu), (float)((cur >> 8) & 0xffu), (float)((cur >> 16) & 0xffu),
+                (float)((cur >> 24) & 0xffu)};
+  uint32_t n = dec_unsigned_nibble(stored);
+  for (int k = 0; k < 4; ++k) r[k] = r[k] * (float)n + val[k];
+  ++n;
+  for (int k = 0; k < 4; ++k) r[k] = r[k] / (float)n;
+  for (int k = 0; k < 4; ++k) r[k] = glsl_round(r[k] / 2.0f) * 2.0f;
+  return enc_unsigned_nibble(conv_vec4_to_rgba8(r), n);
+}
+
+// Voxelize the triangles of one draw (model matrix M) into level 0.
+//  out_ref  (res^3 u32, may be NULL): RGBA8 running average, canonical order
+//  out_q32  (res^3 x 4 float, may be NULL): exact mean through 16.16 fixed
+//           point sums, FP32 (what the GPU's exact mode computes)
+//  out_f64  (res^3 x 4 double, may be NULL): exact mean in double
+//  out_count (res^3 u32, may be NULL): fragments per voxel
+// Coordinates are linear, x fastest.  Returns the number of fragments.
+int64_t orc_voxelize(int res, float cube_size, const float* M, int ntri, const float* pos, const float* nrm,
+                     const uint32_t* tri_mat, const orc_material* mats, const orc_light* lights, int nlights,
+                     uint32_t* out_ref, float* out_q32, double* out_f64, uint32_t* out_count) {
+  float N[9];
+  normal_matrix(M, N);
+  const int W = 2 * res;  // viewport 2res x 2res (renderer.cpp:339-340)
+  std::vector<Frag> frags;
+  uint64_t seq = 0;
+  for (int t = 0; t < ntri; ++t) {
+    V3<float> w[3];
+    V3<double> wd[3];
+    V3<float> n[3];
+    V3<double> nd[3];
+    for (int v = 0; v < 3; ++v) {
+      // voxelize.vert:26
+      V3<float> p = xform_point<float>(M, pos + t * 9 + v * 3);
+      w[v] = mk(p.x / cube_size, p.y / cube_size, p.z / cube_size);
+      V3<double> pd = xform_point<double>(M, pos + t * 9 + v * 3);
+      wd[v] = mk(pd.x / cube_size, pd.y / cube_size, pd.z / cube_size);
+      // voxelize.vert:28
+      n[v] = normalize(xform_normal<float>(N, nrm + t * 9 + v * 3));
+      nd[v] = normalize(xform_normal<double>(N, nrm + t * 9 + v * 3));
+    }
+    // voxelize.geom:27-51
+    const V3<float> e1 = sub(w[1], w[0]), e2 = sub(w[2], w[0]);
+    const V3<float> cr = cross(e1, e2);
+    const float px = fabsf(cr.x), py = fabsf(cr.y), pz = fabsf(cr.z);
+    float xw[3], yw[3];
+    bool inside = true;
+    for (int v = 0; v < 3; ++v) {
+      float a, b;
+      if (pz > px && pz > py) {
+        a = w[v].x;
+        b = w[v].y;
+      } else if (px > py && px > pz) {
+        a = w[v].y;
+        b = w[v].z;
+      } else {
+        a = w[v].x;
+        b = w[v].z;
+      }
+      if (!(fabsf(a) < 64.0f && fabsf(b) < 64.0f)) inside = false;
+      xw[v] = a * (float)res + (float)res;  // viewport transform
+      yw[v] = b * (float)res + (float)res;
+    }
+    if (!inside) continue;
+    Tri2D tri;
+    setup2d(xw, yw, W, W, tri);
+    const orc_material& m = mats[tri_mat ? tri_mat[t] : 0];
+    for_each_pixel(tri, W, W, [&](int, int, const int64_t* bw) {
+      const float fa = (float)tri.area;
+      const float b[3] = {(float)bw[0] / fa, (float)bw[1] / fa, (float)bw[2] / fa};
+      V3<float> p = mk(b[0] * w[0].x + b[1] * w[1].x + b[2] * w[2].x, b[0] * w[0].y + b[1] * w[1].y + b[2] * w[2].y,
+                       b[0] * w[0].z + b[1] * w[1].z + b[2] * w[2].z);
+      // voxelize.frag:156-157, ivec3() truncates toward zero
+      int vc[3];
+      const float pc[3] = {p.x, p.y, p.z};
+      for (int k = 0; k < 3; ++k) {
+        const float f = (float)res * (0.5f * pc[k] + 0.5f);
+        if (!(f > -1.0f && f < (float)res)) return;
+        vc[k] = (int)f;
+      }
+      Frag fr;
+      fr.vox = ((uint32_t)vc[2] * (uint32_t)res + (uint32_t)vc[1]) * (uint32_t)res + (uint32_t)vc[0];
+      fr.order = seq++;
+      V3<float> ni = mk(b[0] * n[0].x + b[1] * n[1].x + b[2] * n[2].x, b[0] * n[0].y + b[1] * n[1].y + b[2] * n[2].y,
+                        b[0] * n[0].z + b[1] * n[1].z + b[2] * n[2].z);
+      frag_color<float>(p, ni, m, lights, nlights, cube_size, fr.c32);
+      // FP64: same fragment, values in double
+      const double da = (double)tri.area;
+      const double bd[3] = {(double)bw[0] / da, (double)bw[1] / da, (double)bw[2] / da};
+      V3<double> pd = mk(bd[0] * wd[0].x + bd[1] * wd[1].x + bd[2] * wd[2].x,
+                         bd[0] * wd[0].y + bd[1] * wd[1].y + bd[2] * wd[2].y,
+                         bd[0] * wd[0].z + bd[1] * wd[1].z + bd[2] * wd[2].z);
+      V3<double> nid = mk(bd[0] * nd[0].x + bd[1] * nd[1].x + bd[2] * nd[2].x,
+                          bd[0] * nd[0].y + bd[1] * nd[1].y + bd[2] * nd[2].y,
+                          bd[0] * nd[0].z + bd[1] * nd[1].z + bd[2] * nd[2].z);
+      frag_color<double>(pd, nid, m, lights, nlights, (double)cube_size, fr.c64);
+      frags.push_back(fr);
+    });
+  }
+  const size_t nvox = (size_t)res * res * res;
+  std::stable_sort(frags.begin(), frags.end(), [](const Frag& a, const Frag& b) { return a.vox < b.vox; });
+  if (out_ref) memset(out_ref, 0, nvox * 4);
+  if (out_q32) memset(out_q32, 0, nvox * 16);
+  if (out_f64) memset(out_f64, 0, nvox * 32);
+  if (out_count) memset(out_count, 0, nvox * 4);
+  for (size_t i = 0; i < frags.size();) {
+    size_t j = i;
+    while (j < frags.size() && frags[j].vox == frags[i].vox) ++j;
+    const uint32_t v = frags[i].vox;
+    if (out_ref) {
+      uint32_t s = 0;
+      for (size_t k = i; k < j; ++k) s = orc_rgba8_avg_step(s, frags[k].c32);
+      out_ref[v] = s;
+    }
+    if (out_q32) {
+      uint64_t q[4] = {0, 0, 0, 0};
+      for (size_t k = i; k < j; ++k)
+        for (int c = 0; c < 4; ++c) q[c] += (uint32_t)(frags[k].c32[c] * 65536.0f + 0.5f);
+      const float cnt = (float)(uint32_t)(j - i);
+      for (int c = 0; c < 4; ++c) out_q32[(size_t)v * 4 + c] = ((float)(uint32_t)q[c] / cnt) * (1.0f / 65536.0f);
+    }
+    if (out_f64) {
+      double s[4] = {0, 0, 0, 0};
+      for (size_t k = i; k < j; ++k)
+        for (int c = 0; c < 4; ++c) s[c] += frags[k].c64[c];
+      for (int c = 0; c < 4; ++c) out_f64[(size_t)v * 4 + c] = s[c] / (double)(j - i);
+    }
+    if (out_count) out_count[v] = (uint32_t)(j - i);
+    i = j;
+  }
+  return (int64_t)frags.size();
+}
+
+// Storage rounding of n floats in place (dtype 0..3) and packing helpers.
+void orc_store_round(float* v, int64_t n, int dtype) {
+  for (int64_t i = 0; i < n; ++i) v[i] = store_round(v[i], dtype);
+}
+void orc_encode_f16(const float* v, uint16_t* out, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) out[i] = half_from_float(v[i]);
+}
+void orc_encode_bf16(const float* v, uint16_t* out, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) out[i] = bf16_from_float(v[i]);
+}
+void orc_decode_f16(const uint16_t* v, float* out, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) out[i] = half_to_float(v[i]);
+}
+void orc_decode_bf16(const uint16_t* v, float* out, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) out[i] = bf16_to_float(v[i]);
+}
+void orc_encode_rgba8(const float* v, uint32_t* out, int64_t n) {
+  for (int64_t i = 0; i < n; ++i)
+    out[i] = float_to_unorm8(v[4 * i]) | (float_to_unorm8(v[4 * i + 1]) << 8) |
+             (float_to_unorm8(v[4 * i + 2]) << 16) | (float_to_unorm8(v[4 * i + 3]) << 24);
+}
+void orc_decode_rgba8(const uint32_t* v, float* out, int64_t n) {
+  for (int64_t i = 0; i < n; ++i)
+    for (int c = 0; c < 4; ++c) out[4 * i + c] = unorm8_to_float((v[i] >> (8 * c)) & 0xffu);
+}
+
+}  // extern "C"
+
+namespace {
+
+// voxel_offsets (mipmap.comp:10-20)
+const int kOff[8][3] = {{1, 1, 1}, {1, 1, 0}, {1, 0, 1}, {1, 0, 0}, {0, 1, 1}, {0, 1, 0}, {0, 0, 1}, {0, 0, 0}};
+// (front, back) pairs of each direction, in the order of mipmap.comp:58-98
+const int kPair[6][4][2] = {
+    {{0, 4}, {1, 5}, {2, 6}, {3, 7}}, {{4, 0}, {5, 1}, {6, 2}, {7, 3}}, {{0, 2}, {1, 3}, {5, 7}, {4, 6}},
+    {{2, 0}, {3, 1}, {7, 5}, {6, 4}}, {{0, 1}, {2, 3}, {4, 5}, {6, 7}}, {{1, 0}, {3, 2}, {5, 4}, {7, 6}}};
+
+// One level of one direction map: dst (s^3 x 4) from src ((2s)^3 x 4).
+template <typename T>
+void mip_level(const T* src, int s2, T* dst, int dir) {
+  const int s = s2 / 2;
+  for (int z = 0; z < s; ++z)
+    for (int y = 0; y < s; ++y)
+      for (int x = 0; x < s; ++x) {
+        T v[8][4];
+        for (int i = 0; i < 8; ++i) {
+          const int sx = 2 * x + kOff[i][0], sy = 2 * y + kOff[i][1], sz = 2 * z + kOff[i][2];
+          const T* p = src + (((size_t)sz * s2 + sy) * s2 + sx) * 4;
+          for (int c = 0; c < 4; ++c) v[i][c] = p[c];
+        }
+        T acc[4];
+        for (int k = 0; k < 4; ++k) {
+          const T* f = v[kPair[dir][k][0]];
+          const T* b = v[kPair[dir][k][1]];
+          const T ka = (T)1 - f[3];
+          for (int c = 0; c < 4; ++c) {
+            const T bl = f[c] + ka * b[c];  // alpha_blend (mipmap.comp:40-43)
+            acc[c] = k == 0 ? bl : acc[c] + bl;
+          }
+        }
+        T* o = dst + (((size_t)z * s + y) * s + x) * 4;
+        for (int c = 0; c < 4; ++c) o[c] = acc[c] / (T)4;
+      }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Renderer::filter (renderer.cpp:283-314) in FP32: levels 1..L-1 of the six
+// maps from level 0 (res^3 x 4 floats, already the stored values).  Each
+// level is rounded to the storage type before the next one reads it.
+// out: for l = 1..L-1, for d = 0..5: (res >> l)^3 x 4 floats, concatenated.
+void orc_mip_f32(int res, int levels, int dtype, const float* level0, float* out) {
+  size_t off = 0;
+  std::vector<size_t> loff(levels * 6, 0);
+  for (int l = 1; l < levels; ++l)
+    for (int d = 0; d < 6; ++d) {
+      loff[l * 6 + d] = off;
+      const size_t s = (size_t)(res >> l);
+      off += s * s * s * 4;
+    }
+  for (int l = 1; l < levels; ++l)
+    for (int d = 0; d < 6; ++d) {
+      const float* src = l == 1 ? level0 : out + loff[(l - 1) * 6 + d];
+      float* dst = out + loff[l * 6 + d];
+      mip_level<float>(src, res >> (l - 1), dst, d);
+      const size_t s = (size_t)(res >> l);
+      for (size_t i = 0; i < s * s * s * 4; ++i) dst[i] = store_round(dst[i], dtype);
+    }
+}
+
+// Same in double, no storage rounding (the FP64 ideal chain).
+void orc_mip_f64(int res, int levels, const double* level0, double* out) {
+  size_t off = 0;
+  std::vector<size_t> loff(levels * 6, 0);
+  for (int l = 1; l < levels; ++l)
+    for (int d = 0; d < 6; ++d) {
+      loff[l * 6 + d] = off;
+      const size_t s = (size_t)(res >> l);
+      off += s * s * s * 4;
+    }
+  for (int l = 1; l < levels; ++l)
+    for (int d = 0; d < 6; ++d) {
+      const double* src = l == 1 ? level0 : out + loff[(l - 1) * 6 + d];
+      mip_level<double>(src, res >> (l - 1), out + loff[l * 6 + d], d);
+    }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// G-buffer: voxel_cone_tracing.vert + raster + GL_LESS depth
+// ---------------------------------------------------------------------------
+extern "C" {
+
+// One draw of the scene into a W x H G-buffer (row 0 = bottom).  depth_key
+// holds the running (depth, triangle) minimum across draws: initialise it to
+// all ones before the first draw.  tri_base numbers this draw's triangles.
+// After the last draw, call orc_gbuffer_resolve for every draw.
+void orc_gbuffer_raster(int W, int H, const float* view, const float* proj, const float* M, int ntri,
+                        const float* pos, uint32_t tri_base, uint64_t* depth_key) {
+  // projection * view (the shader's `projection * view * world`), in float
+  float PV[16];
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 4; ++r) {
+      float s = proj[0 * 4 + r] * view[c * 4 + 0];
+      s = s + proj[1 * 4 + r] * view[c * 4 + 1];
+      s = s + proj[2 * 4 + r] * view[c * 4 + 2];
+      s = s + proj[3 * 4 + r] * view[c * 4 + 3];
+      PV[c * 4 + r] = s;
+    }
+  for (int t = 0; t < ntri; ++t) {
+    float clip[3][4];
+    bool ok = true;
+    for (int v = 0; v < 3; ++v) {
+      const V3<float> w = xform_point<float>(M, pos + t * 9 + v * 3);
+      for (int k = 0; k < 4; ++k) {
+        float s = PV[k] * w.x;
+        s = s + PV[4 + k] * w.y;
+        s = s + PV[8 + k] * w.z;
+        s = s + PV[12 + k];
+        clip[v][k] = s;
+      }
+      if (!(clip[v][3] > 0.0f) || clip[v][2] < -clip[v][3]) ok = false;  // behind the near plane
+    }
+    if (!ok) continue;
+    float xw[3], yw[3], z[3];
+    const float hw = 0.5f * (float)W, hh = 0.5f * (float)H;
+    for (int v = 0; v < 3; ++v) {
+      const float xn = clip[v][0] / clip[v][3], yn = clip[v][1] / clip[v][3];
+      if (!(fabsf(xn) < 256.0f && fabsf(yn) < 256.0f)) ok = false;
+      xw[v] = xn * hw + hw;
+      yw[v] = yn * hh + hh;
+      z[v] = (clip[v][2] / clip[v][3]) * 0.5f + 0.5f;
+    }
+    if (!ok) continue;
+    Tri2D tri;
+    setup2d(xw, yw, W, H, tri);
+    for_each_pixel(tri, W, H, [&](int i, int j, const int64_t* bw) {
+      const float fa = (float)tri.area;
+      const float b0 = (float)bw[0] / fa, b1 = (float)bw[1] / fa, b2 = (float)bw[2] / fa;
+      const float depth = b0 * z[0] + b1 * z[1] + b2 * z[2];
+      if (!(depth >= 0.0f && depth <= 1.0f)) return;
+      uint32_t db;
+      memcpy(&db, &depth, 4);
+      const uint64_t key = ((uint64_t)db << 32) | (uint64_t)(tri_base + (uint32_t)t);
+      uint64_t& dst = depth_key[(size_t)j * W + i];
+      if (key < dst) dst = key;  // GL_LESS; equal depth keeps the earlier triangle
+    });
+  }
+}
+
+// Fill the G-buffer pixels won by this draw's triangles:
+// pos_depth = world xyz + interpolated clip w; normal_mat = interpolated
+// normal + material id bits.  FP32 faithful into *32, FP64 into *64 (NULL ok).
+void orc_gbuffer_resolve(int W, int H, const float* view, const float* proj, const float* M, int ntri,
+                         const float* pos, const float* nrm, const uint32_t* tri_mat, uint32_t tri_base,
+                         const uint64_t* depth_key, float* pos_depth32, float* normal_mat32, double* pos_depth64,
+                         double* normal_mat64) {
+  float PV[16];
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 4; ++r) {
+      float s = proj[0 * 4 + r] * view[c * 4 + 0];
+      s = s + proj[1 * 4 + r] * view[c * 4 + 1];
+      s = s + proj[2 * 4 + r] * view[c * 4 + 2];
+      s = s + proj[3 * 4 + r] * view[c * 4 + 3];
+      PV[c * 4 + r] = s;
+    }
+  float N[9];
+  normal_matrix(M, N);
+  for (size_t idx = 0; idx < (size_t)W * H; ++idx) {
+    const uint64_t key = depth_key[idx];
+    if (key == ~0ull) continue;
+    const uint32_t gt = (uint32_t)(key & 0xffffffffu);
+    if (gt < tri_base || gt >= tri_base + (uint32_t)ntri) continue;
+    const int t = (int)(gt - tri_base);
+    const int i = (int)(idx % W), j = (int)(idx / W);
+    float clip[3][4];
+    V3<float> wv[3];
+    V3<double> wd[3];
+    for (int v = 0; v < 3; ++v) {
+      wv[v] = xform_point<float>(M, pos + t * 9 + v * 3);
+      wd[v] = xform_point<double>(M, pos + t * 9 + v * 3);
+      for (int k = 0; k < 4; ++k) {
+        float s = PV[k] * wv[v].x;
+        s = s + PV[4 + k] * wv[v].y;
+        s = s + PV[8 + k] * wv[v].z;
+        s = s + PV[12 + k];
+        clip[v][k] = s;
+      }
+    }
+    float xw[3], yw[3];
+    const float hw = 0.5f * (float)W, hh = 0.5f * (float)H;
+    for (int v = 0; v < 3; ++v) {
+      xw[v] = (clip[v][0] / clip[v][3]) * hw + hw;
+      yw[v] = (clip[v][1] / clip[v][3]) * hh + hh;
+    }
+    Tri2D tri;
+    setup2d(xw, yw, W, H, tri);
+    int64_t bw[3];
+    if (!covers(tri, i, j, bw)) continue;
+    const float fa = (float)tri.area;
+    float b[3] = {(float)bw[0] / fa, (float)bw[1] / fa, (float)bw[2] / fa};
+    // perspective-correct weights (b_k / w_k) / sum
+    float q[3], qs = 0.0f;
+    for (int v = 0; v < 3; ++v) {
+      q[v] = b[v] / clip[v][3];
+      qs = qs + q[v];
+    }
+    for (int v = 0; v < 3; ++v) q[v] = q[v] / qs;
+    float wp[3] = {0, 0, 0}, nn[3] = {0, 0, 0};
+    double bd[3], qd[3], qsd = 0.0;
+    for (int v = 0; v < 3; ++v) bd[v] = (double)bw[v] / (double)tri.area;
+    for (int v = 0; v < 3; ++v) {
+      qd[v] = bd[v] / (double)clip[v][3];
+      qsd += qd[v];
+    }
+    double wpd[3] = {0, 0, 0}, nnd[3] = {0, 0, 0};
+    for (int v = 0; v < 3; ++v) {
+      const V3<float> tn = xform_normal<float>(N, nrm + t * 9 + v * 3);
+      const float l = length(tn);
+      const V3<double> tnd = xform_normal<double>(N, nrm + t * 9 + v * 3);
+      const double ld = length(tnd);
+      const float wc[3] = {wv[v].x, wv[v].y, wv[v].z};
+      const double wcd[3] = {wd[v].x, wd[v].y, wd[v].z};
+      const float nc[3] = {tn.x / l, tn.y / l, tn.z / l};
+      const double ncd[3] = {tnd.x / ld, tnd.y / ld, tnd.z / ld};
+      for (int k = 0; k < 3; ++k) {
+        wp[k] = wp[k] + q[v] * wc[k];
+        nn[k] = nn[k] + q[v] * nc[k];
+        wpd[k] += (qd[v] / qsd) * wcd[k];
+        nnd[k] += (qd[v] / qsd) * ncd[k];
+      }
+    }
+    const uint32_t mid = tri_mat ? tri_mat[t] : 0u;
+    float midf;
+    memcpy(&midf, &mid, 4);
+    if (pos_depth32) {
+      float* o = pos_depth32 + idx * 4;
+      o[0] = wp[0];
+      o[1] = wp[1];
+      o[2] = wp[2];
+      o[3] = 1.0f / qs;
+    }
+    if (normal_mat32) {
+      float* o = normal_mat32 + idx * 4;
+      o[0] = nn[0];
+      o[1] = nn[1];
+      o[2] = nn[2];
+      o[3] = midf;
+    }
+    if (pos_depth64) {
+      double* o = pos_depth64 + idx * 4;
+      o[0] = wpd[0];
+      o[1] = wpd[1];
+      o[2] = wpd[2];
+      o[3] = 1.0 / qsd;
+    }
+    if (normal_mat64) {
+      double* o = normal_mat64 + idx * 4;
+      o[0] = nnd[0];
+      o[1] = nnd[1];
+      o[2] = nnd[2];
+      o[3] = (double)mid;
+    }
+  }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// cone tracing (voxel_cone_tracing.frag)
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename T>
+struct Volume {
+  int res, levels;
+  // level l, map d -> (res>>l)^3 x 4 values; level 0 shared
+  std::vector<const T*> lv;  // index l * 6 + d
+  const T* get(int l, int d) const { return lv[l * 6 + d]; }
+};
+
+template <typename T>
+T flr(T x) {
+  return floor(x);
+}
+inline float flr(float x) { return floorf(x); }
+
+// Trilinear filtering of one level with a zero border (CLAMP_TO_BORDER).
+template <typename T>
+void trilinear(const T* tex, int s, V3<T> p, T out[4]) {
+  const T u = p.x * (T)s - (T)0.5, v = p.y * (T)s - (T)0.5, w = p.z * (T)s - (T)0.5;
+  const T fu = flr(u), fv = flr(v), fw = flr(w);
+  const int i0 = (int)fu, j0 = (int)fv, k0 = (int)fw;
+  const T a = u - fu, b = v - fv, c = w - fw;
+  for (int ch = 0; ch < 4; ++ch) out[ch] = 0;
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx) {
+        const int x = i0 + dx, y = j0 + dy, z = k0 + dz;
+        if (x < 0 || y < 0 || z < 0 || x >= s || y >= s || z >= s) continue;  // border (0,0,0,0)
+        const T wt = (dx ? a : (T)1 - a) * (dy ? b : (T)1 - b) * (dz ? c : (T)1 - c);
+        const T* t = tex + (((size_t)z * s + y) * s + x) * 4;
+        for (int ch = 0; ch < 4; ++ch) out[ch] += wt * t[ch];
+      }
+}
+
+// textureLod with LINEAR_MIPMAP_LINEAR, lod clamped to [0, levels-1].
+template <typename T>
+void texture_lod(const Volume<T>& vol, int map, V3<T> p, T lod, T out[4]) {
+  const T lmax = (T)(vol.levels - 1);
+  if (!(lod > (T)0)) lod = 0;
+  if (lod > lmax) lod = lmax;
+  const int d1 = (int)flr(lod);
+  const T f = lod - (T)d1;
+  T a[4];
+  trilinear(vol.get(d1, map), vol.res >> d1, p, a);
+  if (f > (T)0 && d1 + 1 < vol.levels) {
+    T b[4];
+    trilinear(vol.get(d1 + 1, map), vol.res >> (d1 + 1), p, b);
+    for (int c = 0; c < 4; ++c) out[c] = ((T)1 - f) * a[c] + f * b[c];
+  } else {
+    for (int c = 0; c < 4; ++c) out[c] = a[c];
+  }
+}
+
+template <typename T>
+T log2T(T x) {
+  return log2(x);
+}
+inline float log2T(float x) { return log2f(x); }
+template <typename T>
+T tanT(T x) {
+  return tan(x);
+}
+inline float tanT(float x) { return tanf(x); }
+template <typename T>
+T powT(T x, T y) {
+  return pow(x, y);
+}
+inline float powT(float x, float y) { return powf(x, y); }
+
+struct TraceParams {
+  int W, H;
+  float cube_size;
+  int cube_res;
+  float cam[3];
+  int en_direct, en_diffuse, en_specular, en_shadow;
+  int view_dir;
+  float view_lod;
+  int cone_set;
+  int demodulate;
+};
+
+template <typename T>
+struct Tracer {
+  const Volume<T>& vol;
+  const TraceParams& P;
+  const orc_light* lights;
+  int nlights;
+  int64_t steps = 0, samples = 0;
+
+  // sample_voxel (voxel_cone_tracing.frag:80-86)
+  void sample_voxel(V3<T> pos, V3<T> dir, const int idx[3], T lod, T out[4]) {
+    const T d[3] = {fabs(dir.x), fabs(dir.y), fabs(dir.z)};
+    for (int c = 0; c < 4; ++c) out[c] = 0;
+    for (int k = 0; k < 3; ++k) {
+      T s[4];
+      texture_lod(vol, idx[k], pos, lod, s);
+      for (int c = 0; c < 4; ++c) out[c] += d[k] * s[c];
+    }
+    samples += 3;
+  }
+
+  // trace_cone (voxel_cone_tracing.frag:88-119)
+  void trace_cone(V3<T> origin, V3<T> dir, T aperture, T max_dist, T out[4]) {
+    dir = normalize(dir);
+    const int idx[3] = {dir.x < (T)0 ? 0 : 1, dir.y < (T)0 ? 2 : 3, dir.z < (T)0 ? 4 : 5};
+    T c[4] = {0, 0, 0, 0};
+    const T voxel_size = (T)1 / (T)P.cube_res;
+    T dist = (T)3 * voxel_size;
+    T diam = dist * aperture;
+    V3<T> sp = add(scale(dist, dir), origin);
+    while (c[3] < (T)1 && dist < max_dist) {
+      T mip = log2T(diam * (T)P.cube_res);
+      if (!(mip > (T)0)) mip = 0;
+      T s[4];
+      sample_voxel(sp, dir, idx, mip, s);
+      const T k = (T)1 - c[3];
+      for (int ch = 0; ch < 4; ++ch) c[ch] += k * s[ch];
+      const T step = std::max(diam / (T)2, voxel_size);
+      dist += step;
+      diam = dist * aperture;
+      sp = add(scale(dist, dir), origin);
+      ++steps;
+    }
+    for (int ch = 0; ch < 4; ++ch) out[ch] = c[ch];
+  }
+
+  // tangent (:126-134)
+  V3<T> tangent(V3<T> n) {
+    const V3<T> t1 = cross(n, mk<T>(0, 0, 1)), t2 = cross(n, mk<T>(0, 1, 0));
+    return length(t1) > length(t2) ? normalize(t1) : normalize(t2);
+  }
+
+  static V3<T> mix(V3<T> x, V3<T> y, T a) {
+    return mk(x.x * ((T)1 - a) + y.x * a, x.y * ((T)1 - a) + y.y * a, x.z * ((T)1 - a) + y.z * a);
+  }
+
+  // trace_diffuse (:140-168) and the generalised cone sets
+  V3<T> trace_diffuse(V3<T> origin, V3<T> normal) {
+    const V3<T> o1 = normalize(tangent(normal));
+    const V3<T> o2 = normalize(cross(o1, normal));
+    T acc[4] = {0, 0, 0, 0};
+    T r[4];
+    const T sqrt3 = (T)1.73205080757;
+    if (P.cone_set == 0) {
+      const T aperture = (T)0.55785173935;  // TAN_22_5
+      const V3<T> c1 = scale((T)0.5, add(o1, o2));
+      const V3<T> c2 = scale((T)0.5, sub(o1, o2));
+      const V3<T> neg_o1 = scale((T)-1, o1), neg_o2 = scale((T)-1, o2);
+      const V3<T> neg_c1 = scale((T)-1, c1), neg_c2 = scale((T)-1, c2);
+      const V3<T> dirs[9] = {normal,
+                             mix(normal, o1, (T)0.5),
+                             mix(normal, neg_o1, (T)0.5),
+                             mix(normal, o2, (T)0.5),
+                             mix(normal, neg_o2, (T)0.5),
+                             mix(normal, c1, (T)0.5),
+                             mix(normal, neg_c1, (T)0.5),
+                             mix(normal, c2, (T)0.5),
+                             mix(normal, neg_c2, (T)0.5)};
+      for (int k = 0; k < 9; ++k) {
+        trace_cone(origin, dirs[k], aperture, sqrt3, r);
+        for (int c = 0; c < 4; ++c) acc[c] += r[c];
+      }
+      return mk(acc[0] / (T)9, acc[1] / (T)9, acc[2] / (T)9);
+    }
+    // generalised sets: dir = a n + b o1 + c o2
+    T tab[16][3];
+    int n = 0;
+    T aperture;
+    if (P.cone_set == 1) {
+      const T t5[5][3] = {{1, 0, 0}, {0.5, 0.5, 0}, {0.5, -0.5, 0}, {0.5, 0, 0.5}, {0.5, 0, -0.5}};
+      for (int i = 0; i < 5; ++i)
+        for (int k = 0; k < 3; ++k) tab[i][k] = t5[i][k];
+      n = 5;
+      aperture = (T)1.15470054;  // 2 tan(30 deg)
+    } else {
+      const double d2r = 3.14159265358979323846 / 180.0;
+      tab[0][0] = 1;
+      tab[0][1] = 0;
+      tab[0][2] = 0;
+      n = 1;
+      for (int i = 0; i < 5; ++i, ++n) {
+        const double th = 28.0 * d2r, ph = 72.0 * i * d2r;
+        tab[n][0] = (T)(float)cos(th);
+        tab[n][1] = (T)(float)(sin(th) * cos(ph));
+        tab[n][2] = (T)(float)(sin(th) * sin(ph));
+      }
+      for (int i = 0; i < 10; ++i, ++n) {
+        const double th = 58.0 * d2r, ph = (36.0 * i + 18.0) * d2r;
+        tab[n][0] = (T)(float)cos(th);
+        tab[n][1] = (T)(float)(sin(th) * cos(ph));
+        tab[n][2] = (T)(float)(sin(th) * sin(ph));
+      }
+      aperture = (T)0.45;
+    }
+    for (int k = 0; k < n; ++k) {
+      const V3<T> d = add(add(scale(tab[k][0], normal), scale(tab[k][1], o1)), scale(tab[k][2], o2));
+      trace_cone(origin, d, aperture, sqrt3, r);
+      for (int c = 0; c < 4; ++c) acc[c] += r[c];
+    }
+    return mk(acc[0] / (T)n, acc[1] / (T)n, acc[2] / (T)n);
+  }
+
+  static V3<T> refract(V3<T> I, V3<T> N, T eta) {
+    const T d = dot(N, I);
+    const T k = (T)1 - eta * eta * ((T)1 - d * d);
+    if (k < (T)0) return mk<T>(0, 0, 0);
+    return sub(scale(eta, I), scale(eta * d + sqrt(k), N));
+  }
+
+  // voxel_cone_tracing.frag main (:246-275) for one pixel
+  void shade(V3<T> world, V3<T> normal, const orc_material& m, T out[4]) {
+    const T bg[4] = {(T)0.15f, (T)0.25f, (T)0.25f, (T)1};
+    const T cs = (T)P.cube_size;
+    const V3<T> pos = mk(world.x / cs * (T)0.5 + (T)0.5, world.y / cs * (T)0.5 + (T)0.5,
+                         world.z / cs * (T)0.5 + (T)0.5);
+    if (!(fabs(pos.x) < (T)1 && fabs(pos.y) < (T)1 && fabs(pos.z) < (T)1)) {  // within_cube
+      for (int c = 0; c < 4; ++c) out[c] = P.demodulate ? (T)0 : bg[c];
+      return;
+    }
+    if (P.view_dir < 7) {
+      T s[4];
+      texture_lod(vol, std::min(std::max(P.view_dir, 0), 5), pos, (T)P.view_lod, s);
+      const T a = s[3];
+      for (int c = 0; c < 3; ++c) out[c] = s[c] * a + bg[c] * ((T)1 - a);
+      out[3] = a * a + bg[3] * ((T)1 - a);
+      return;
+    }
+    const V3<T> cam = mk<T>(P.cam[0], P.cam[1], P.cam[2]);
+    const V3<T> view_dir = normalize(sub(world, cam));
+    const T sqrt3 = (T)1.73205080757;
+    V3<T> fin = mk<T>(0, 0, 0);
+    if (P.en_diffuse) {
+      const V3<T> d = trace_diffuse(pos, normal);
+      fin = add(fin, mk((T)m.diffuse[0] * d.x, (T)m.diffuse[1] * d.y, (T)m.diffuse[2] * d.z));
+    }
+    const T rough = sqrt((T)2 / ((T)m.shininess + (T)2));
+    T spec_ap = tanT((T)1.57079f * rough);
+    spec_ap = std::min(std::max(spec_ap, (T)0.0174533f), (T)3.14159265f);
+    if (P.en_direct) {
+      V3<T> res = mk<T>(0, 0, 0);
+      const int nl = std::min(nlights, 10);
+      for (int i = 0; i < nl; ++i) {
+        const orc_light& L = lights[i];
+        V3<T> ldir;
+        T d;
+        if (L.kind == 1) {
+          ldir = normalize(mk<T>(L.position[0], L.position[1], L.position[2]));
+          d = sqrt3;
+        } else {
+          const V3<T> lp = mk((T)L.position[0] / cs * (T)0.5 + (T)0.5, (T)L.position[1] / cs * (T)0.5 + (T)0.5,
+                              (T)L.position[2] / cs * (T)0.5 + (T)0.5);
+          ldir = sub(lp, pos);
+          d = length(ldir);
+          ldir = normalize(ldir);
+        }
+        const T cos_surf = std::max(dot(normal, ldir), (T)0);
+        const T att = L.kind == 1 ? (T)1 : (T)1 / ((T)1 + d * d);
+        const V3<T> lc = scale(att * cos_surf, mk<T>((T)L.color[0] * (T)L.intensity, (T)L.color[1] * (T)L.intensity,
+                                                   (T)L.color[2] * (T)L.intensity));
+        T shadow = 1;
+        if (P.en_shadow) {
+          T s[4];
+          trace_cone(pos, ldir, (T)0.1, d, s);
+          shadow = std::max((T)0, (T)1 - s[3]);
+        }
+        const T lambert = std::max(dot(ldir, normal), (T)0);
+        T refr_angle = 0;
+        if (m.dissolve <= 0.1f) {
+          const V3<T> r = refract(view_dir, normal, (T)1 / (T)m.ior);
+          refr_angle = std::max(((T)1 - (T)m.dissolve) * dot(r, ldir), (T)0);
+        }
+        const V3<T> hv = normalize(add(ldir, view_dir));
+        T sa = std::max(dot(hv, normal), (T)0);
+        sa = std::max(sa, refr_angle);
+        const T sc = powT(sa, (T)m.shininess);
+        const T f = shadow + (T)0.04f;
+        res.x += f * (lambert * (T)m.diffuse[0] + sc * (T)m.specular[0]) * lc.x;
+        res.y += f * (lambert * (T)m.diffuse[1] + sc * (T)m.specular[1]) * lc.y;
+        res.z += f * (lambert * (T)m.diffuse[2] + sc * (T)m.specular[2]) * lc.z;
+      }
+      const T e[3] = {std::min(std::max((T)m.emission[0], (T)0), (T)1),
+                      std::min(std::max((T)m.emission[1], (T)0), (T)1),
+                      std::min(std::max((T)m.emission[2], (T)0), (T)1)};
+      fin = add(fin, mk(res.x + e[0], res.y + e[1], res.z + e[2]));
+    }
+    if (P.en_specular) {
+      // normalize(reflect(-view_dir, normal))
+      const V3<T> I = scale((T)-1, view_dir);
+      const V3<T> r = sub(I, scale((T)2 * dot(normal, I), normal));
+      T s[4];
+      trace_cone(pos, normalize(r), spec_ap, sqrt3, s);
+      fin = add(fin, mk((T)m.specular[0] * s[0], (T)m.specular[1] * s[1], (T)m.specular[2] * s[2]));
+    }
+    if (transmissive(m.illum) && P.en_specular) {
+      T s[4];
+      trace_cone(pos, refract(view_dir, normal, (T)1 / (T)m.ior), spec_ap, sqrt3, s);
+      const T dv = (T)m.dissolve;
+      const T tr[3] = {(T)m.transmittance[0] * s[0], (T)m.transmittance[1] * s[1], (T)m.transmittance[2] * s[2]};
+      fin = mk(tr[0] * ((T)1 - dv) + fin.x * dv, tr[1] * ((T)1 - dv) + fin.y * dv, tr[2] * ((T)1 - dv) + fin.z * dv);
+    }
+    if (P.demodulate) {
+      const T e = (T)(1.0 / 1024.0);
+      fin = mk(fin.x / std::max((T)m.diffuse[0], e), fin.y / std::max((T)m.diffuse[1], e),
+               fin.z / std::max((T)m.diffuse[2], e));
+    }
+    out[0] = fin.x;
+    out[1] = fin.y;
+    out[2] = fin.z;
+    out[3] = 1;
+  }
+};
+
+template <typename T>
+void trace_pixels(const TraceParams& P, int res, int levels, const T* const* level_ptrs, const T* gpos,
+                  const T* gnrm, const orc_material* mats, const orc_light* lights, int nlights,
+                  const int64_t* pixels, int64_t npix, T* out, int64_t* steps) {
+  Volume<T> vol;
+  vol.res = res;
+  vol.levels = levels;
+  vol.lv.assign(level_ptrs, level_ptrs + levels * 6);
+  int64_t total_steps = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total_steps)
+  for (int64_t k = 0; k < npix; ++k) {
+    const int64_t idx = pixels ? pixels[k] : k;
+    Tracer<T> tr{vol, P, lights, nlights};
+    T* o = out + k * 4;
+    const T* gp = gpos + idx * 4;
+    const T* gn = gnrm + idx * 4;
+    if (gp[3] == (T)0) {
+      const T bg[4] = {(T)0.15f, (T)0.25f, (T)0.25f, (T)1};
+      for (int c = 0; c < 4; ++c) o[c] = P.demodulate ? (T)0 : bg[c];
+      continue;
+    }
+    uint32_t mid;
+    if (sizeof(T) == 4) {
+      memcpy(&mid, &gn[3], 4);
+    } else {
+      mid = (uint32_t)gn[3];
+    }
+    tr.shade(mk(gp[0], gp[1], gp[2]), mk(gn[0], gn[1], gn[2]), mats[mid], o);
+    total_steps += tr.steps;
+  }
+  if (steps) *steps = total_steps;
+}
+
+}  // namespace
+
+extern "C" {
+
+typedef struct {
+  int W, H;
+  float cube_size;
+  int cube_res;
+  float cam[3];
+  int en_direct, en_diffuse, en_specular, en_shadow;
+  int view_dir;
+  float view_lod;
+  int cone_set;
+  int demodulate;
+} orc_trace_params;
+
+// Shade the listed pixels (or all W*H if pixels == NULL) from a G-buffer and
+// a decoded volume.  level_ptrs[l * 6 + d] points at level l of map d
+// ((res >> l)^3 x 4 values, x fastest; level 0 pointers may all be equal).
+// out: 4 values per listed pixel.  *steps: cone steps taken.
+void orc_trace_f32(const orc_trace_params* p, int res, int levels, const float* const* level_ptrs, const float* gpos,
+                   const float* gnrm, const orc_material* mats, const orc_light* lights, int nlights,
+                   const int64_t* pixels, int64_t npix, float* out, int64_t* steps) {
+  TraceParams P;
+  memcpy(&P, p, sizeof(P));
+  trace_pixels<float>(P, res, levels, level_ptrs, gpos, gnrm, mats, lights, nlights, pixels, npix, out, steps);
+}
+
+void orc_trace_f64(const orc_trace_params* p, int res, int levels, const double* const* level_ptrs,
+                   const double* gpos, const double* gnrm, const orc_material* mats, const orc_light* lights,
+                   int nlights, const int64_t* pixels, int64_t npix, double* out, int64_t* steps) {
+  TraceParams P;
+  memcpy(&P, p, sizeof(P));
+  trace_pixels<double>(P, res, levels, level_ptrs, gpos, gnrm, mats, lights, nlights, pixels, npix, out, steps);
+}
+
+}  // extern "C"

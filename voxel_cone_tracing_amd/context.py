@@ -1,0 +1,179 @@
+"""Python handle over a native vct context (include/vct.h).
+
+Mirrors the reference Renderer's call sequence (renderer.h:122-155) for
+Python callers.  All work happens in libvct.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class Context:
+    def __init__(self, width: int, height: int):
+        self.lib = L.load()
+        h = C.c_void_p()
+        L.check(self.lib.vct_create(width, height, C.byref(h)))
+        self.h = h
+        self.width, self.height = width, height
+
+    # -- lifetime ---------------------------------------------------------------
+    def close(self) -> None:
+        if self.h:
+            self.lib.vct_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self) -> None:
+        L.check(self.lib.vct_sync(self.h))
+
+    # -- configuration ------------------------------------------------------------
+    def set_viewport(self, w: int, h: int) -> None:
+        L.check(self.lib.vct_set_viewport(self.h, w, h))
+        self.width, self.height = w, h
+
+    def set_grid(self, res: int, levels: int = 7, dtype: int = L.RGBA8, average: int | None = None) -> None:
+        if average is None:
+            average = L.AVG_REFERENCE if dtype == L.RGBA8 else L.AVG_EXACT
+        L.check(self.lib.vct_set_grid_format(self.h, res, levels, dtype, average))
+
+    def grid(self):
+        r, l, d, a = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        L.check(self.lib.vct_get_grid(self.h, C.byref(r), C.byref(l), C.byref(d), C.byref(a)))
+        return r.value, l.value, d.value, a.value
+
+    def set_grid_size(self, s: float) -> None:
+        L.check(self.lib.vct_set_grid_size(self.h, s))
+
+    def set_camera(self, view: np.ndarray, proj: np.ndarray) -> None:
+        v = np.ascontiguousarray(view, np.float32).reshape(16)
+        p = np.ascontiguousarray(proj, np.float32).reshape(16)
+        L.check(self.lib.vct_set_camera_transform(self.h, L.fptr(v), L.fptr(p)))
+
+    def set_phases(self, direct: bool, diffuse: bool, specular: bool, shadow: bool) -> None:
+        L.check(self.lib.vct_set_rendering_phases(self.h, int(direct), int(diffuse), int(specular), int(shadow)))
+
+    def set_voxel_view(self, d: int, lod: float) -> None:
+        L.check(self.lib.vct_set_voxel_view_dir(self.h, d, lod))
+
+    def set_cone_set(self, s: int) -> None:
+        L.check(self.lib.vct_set_cone_set(self.h, s))
+
+    # -- scene --------------------------------------------------------------------
+    def add_materials(self, mats) -> int:
+        arr = L.materials_array(mats)
+        first = C.c_uint32()
+        L.check(self.lib.vct_add_materials(self.h, arr, len(mats), C.byref(first)))
+        return first.value
+
+    def model_create(self, pos: np.ndarray, nrm: np.ndarray, tri_mat: np.ndarray | None = None) -> int:
+        pos = np.ascontiguousarray(pos, np.float32)
+        nrm = np.ascontiguousarray(nrm, np.float32)
+        n = pos.reshape(-1, 9).shape[0]
+        out = C.c_uint32()
+        tm = None
+        if tri_mat is not None:
+            tri_mat = np.ascontiguousarray(tri_mat, np.uint32)
+            tm = L.uptr(tri_mat)
+        L.check(self.lib.vct_model_create(self.h, L.fptr(pos), L.fptr(nrm), tm, n, C.byref(out)))
+        return out.value
+
+    def load_model(self, path: str) -> int:
+        out = C.c_uint32()
+        L.check(self.lib.vct_load_model(self.h, path.encode(), C.byref(out)))
+        return out.value
+
+    def set_model_transform(self, model: int, m: np.ndarray) -> None:
+        a = np.ascontiguousarray(m, np.float32).reshape(16)
+        L.check(self.lib.vct_set_model_transform(self.h, model, L.fptr(a)))
+
+    def queue_model(self, model: int) -> None:
+        L.check(self.lib.vct_queue_model(self.h, model))
+
+    def queue_light(self, position, color=(1, 1, 1), intensity=1.0, kind=L.LIGHT_POINT) -> None:
+        li = L.Light()
+        for k in range(3):
+            li.position[k] = position[k]
+            li.color[k] = color[k]
+        li.intensity = intensity
+        li.kind = kind
+        L.check(self.lib.vct_queue_point_light(self.h, C.byref(li)))
+
+    def clear_queues(self) -> None:
+        L.check(self.lib.vct_clear_queues(self.h))
+
+    # -- passes -------------------------------------------------------------------
+    def voxelize(self) -> None:
+        L.check(self.lib.vct_voxelize(self.h))
+
+    def mip(self) -> None:
+        L.check(self.lib.vct_mip_aniso(self.h))
+
+    def gbuffer(self) -> None:
+        L.check(self.lib.vct_gbuffer(self.h))
+
+    def trace(self) -> None:
+        L.check(self.lib.vct_trace(self.h))
+
+    def render(self) -> None:
+        L.check(self.lib.vct_render(self.h))
+
+    # -- read back ----------------------------------------------------------------
+    def download_level(self, level: int, d: int = 0) -> np.ndarray:
+        res, _, dt, _ = self.grid()
+        s = res >> level
+        nbytes = self.lib.vct_volume_level_bytes(self.h, level)
+        if dt == L.RGBA8:
+            a = np.zeros((s, s, s), np.uint32)
+        elif dt == L.FP32:
+            a = np.zeros((s, s, s, 4), np.float32)
+        else:
+            a = np.zeros((s, s, s, 4), np.uint16)
+        assert a.nbytes == nbytes
+        L.check(self.lib.vct_volume_download(self.h, level, d, L.vptr(a), nbytes))
+        return a
+
+    def upload_level(self, level: int, d: int, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        L.check(self.lib.vct_volume_upload(self.h, level, d, L.vptr(a), a.nbytes))
+
+    def download_gbuffer(self):
+        n = self.width * self.height
+        p = np.zeros((self.height, self.width, 4), np.float32)
+        q = np.zeros((self.height, self.width, 4), np.float32)
+        L.check(self.lib.vct_gbuffer_download(self.h, L.fptr(p), L.fptr(q), n))
+        return p, q
+
+    def upload_gbuffer(self, p: np.ndarray, q: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        q = np.ascontiguousarray(q, np.float32)
+        L.check(self.lib.vct_gbuffer_upload(self.h, L.fptr(p), L.fptr(q), self.width * self.height))
+
+    def download_image(self) -> np.ndarray:
+        img = np.zeros((self.height, self.width, 4), np.float32)
+        L.check(self.lib.vct_image_download(self.h, L.fptr(img), self.width * self.height))
+        return img
+
+    def pass_times(self) -> np.ndarray:
+        t = np.zeros(5, np.float32)
+        L.check(self.lib.vct_get_pass_times(self.h, L.fptr(t), 5))
+        return t
+
+    def voxelize_stats(self):
+        f, c = C.c_uint64(), C.c_uint64()
+        L.check(self.lib.vct_get_voxelize_stats(self.h, C.byref(f), C.byref(c)))
+        return f.value, c.value

@@ -1,0 +1,217 @@
+// vct_internal.h - shared definitions of the gfx950 voxel cone tracing core.
+//
+// Volume layout in HBM (all levels, all six direction maps, one allocation):
+//   level 0 (res^3) is shared by the six direction maps: the reference writes
+//   the same value into all six level-0 images (voxelize.frag:159-160), so one
+//   copy is kept.  Levels 1..L-1 exist once per direction (mipmap.comp:4-5).
+//   Each level of size s is stored in 4x4x4-texel bricks; bricks are ordered
+//   along a 3D Morton (Z-order) curve and texels inside a brick are x-fastest.
+//   Levels smaller than 4^3 occupy one padded brick.  A 2x2x2 trilinear
+//   footprint therefore touches one or two 128-byte lines in the common case,
+//   and any aligned 16^3 block is 64 consecutive bricks (one contiguous range).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/vct.h"
+
+#define VCT_HD __host__ __device__ __forceinline__
+
+namespace vct {
+
+// ---- Morton / brick addressing -------------------------------------------
+VCT_HD uint32_t spread3(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+VCT_HD uint32_t compact3(uint32_t v) {
+  v &= 0x09249249u;
+  v = (v | (v >> 2)) & 0x030C30C3u;
+  v = (v | (v >> 4)) & 0x0300F00Fu;
+  v = (v | (v >> 8)) & 0x030000FFu;
+  v = (v | (v >> 16)) & 0x3ffu;
+  return v;
+}
+VCT_HD uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
+}
+// Element index of texel (x,y,z) in a level of size s (power of two).
+VCT_HD uint32_t brick_index(uint32_t x, uint32_t y, uint32_t z) {
+  return (morton3(x >> 2, y >> 2, z >> 2) << 6) | ((z & 3u) << 4) | ((y & 3u) << 2) | (x & 3u);
+}
+// Inverse of brick_index.
+VCT_HD void brick_coords(uint32_t idx, uint32_t& x, uint32_t& y, uint32_t& z) {
+  uint32_t b = idx >> 6, t = idx & 63u;
+  x = (compact3(b) << 2) | (t & 3u);
+  y = (compact3(b >> 1) << 2) | ((t >> 2) & 3u);
+  z = (compact3(b >> 2) << 2) | (t >> 4);
+}
+VCT_HD uint64_t level_elems(uint32_t s) {
+  return s >= 4 ? (uint64_t)s * s * s : 64ull;
+}
+
+// ---- storage codecs -------------------------------------------------------
+VCT_HD uint32_t dtype_bytes(int dt) {
+  return dt == VCT_RGBA8 ? 4u : (dt == VCT_FP32 ? 16u : 8u);
+}
+
+// Round-to-nearest-even float -> bf16 bits (NaN kept quiet).
+VCT_HD uint16_t f32_to_bf16(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+VCT_HD float bf16_to_f32(uint16_t h) {
+  return __builtin_bit_cast(float, (uint32_t)h << 16);
+}
+
+VCT_HD uint16_t f32_to_f16(float f) {
+  _Float16 h = (_Float16)f;  // round to nearest even (v_cvt_f16_f32)
+  return __builtin_bit_cast(uint16_t, h);
+}
+VCT_HD float f16_to_f32(uint16_t h) {
+  return (float)__builtin_bit_cast(_Float16, h);
+}
+
+// GL unorm8 -> float: c / 255 (the GL definition; correctly rounded divide).
+VCT_HD float unorm8_exact(uint32_t c) { return (float)c / 255.0f; }
+// float -> GL unorm8: clamp to [0,1], scale by 255, round to nearest even.
+VCT_HD uint32_t to_unorm8(float f) {
+  f = f > 0.0f ? f : 0.0f;  // also maps NaN to 0
+  f = f < 1.0f ? f : 1.0f;
+  return (uint32_t)__builtin_rintf(f * 255.0f);
+}
+
+// ---- device material ---------------------------------------------------
+// 64-byte record derived from vct_material (the fields the three shaders
+// read: voxelize.frag:10-31, voxel_cone_tracing.frag:3-24).
+struct DevMaterial {
+  float diffuse[3];
+  float shininess;
+  float specular[3];
+  float ior;
+  float emission[3];
+  float dissolve;
+  float transmittance[3];
+  int32_t illum;
+};
+static_assert(sizeof(DevMaterial) == 64, "DevMaterial must be 64 bytes");
+
+VCT_HD bool illum_transmissive(int illum) {
+  return illum == 4 || illum == 6 || illum == 7 || illum == 9;
+}
+
+struct DevLight {
+  float pos[3];
+  float color[3];
+  float intensity;
+  int32_t kind;
+};
+
+struct LightSet {
+  DevLight l[VCT_MAX_LIGHTS];
+  int32_t count;
+};
+
+}  // namespace vct
+
+// ---- host side context --------------------------------------------------
+struct VctModel {
+  uint32_t ntri = 0;
+  float* d_pos = nullptr;     // 9 floats per triangle
+  float* d_nrm = nullptr;     // 9 floats per triangle
+  uint32_t* d_mat = nullptr;  // material id per triangle
+  std::vector<uint32_t> h_mat;
+  float xform[16];
+  float dims[3];
+};
+
+struct VctTex3D {
+  bool alive = false;
+  int w = 0, h = 0, d = 0, levels = 0, dtype = 0;
+  std::vector<size_t> off;  // byte offset of each level (linear layout)
+  std::vector<int> lw, lh, ld;
+  void* dptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct VctBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+struct vct_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int width = 0, height = 0;
+  unsigned res = 128;
+  int levels = 7;
+  int dtype = VCT_RGBA8;
+  int average = VCT_AVG_REFERENCE;
+  float cube_size = 1.0f;
+  float view[16], proj[16];
+  bool en_direct = true, en_diffuse = true, en_specular = true, en_shadow = true;
+  int view_voxel_dir = 0;
+  float view_voxel_lod = 0.0f;
+  int cone_set = VCT_CONES_REF9;
+
+  std::vector<vct_material> materials;
+  VctBuf d_materials;
+  bool materials_dirty = true;
+
+  std::vector<VctModel> models;
+  std::vector<uint32_t> queue;
+  std::vector<vct_light> lights;
+
+  // radiance volume
+  VctBuf volume;
+  bool volume_valid = false;
+  uint64_t lvl_off[16][6];  // byte offsets
+  uint32_t lvl_size[16];
+
+  // voxelize scratch
+  VctBuf accum;        // exact mode: 5 x u32 per voxel (kept zero between frames)
+  bool accum_zero = false;
+  VctBuf tri_count, tri_off, tri_base;  // per-triangle candidate counts / offsets
+  VctBuf frags, frags_sorted, vox_count, vox_off, scan_tmp, counters;
+  VctBuf setup;  // per-frame triangle list (model index, transform)
+  uint64_t last_fragments = 0, last_candidates = 0;
+
+  // G-buffer + image
+  VctBuf vis, g_pos, g_nrm, image;
+  VctBuf g_count, g_off;
+  size_t vis_pixels = 0;
+
+  std::vector<VctTex3D> textures;
+
+  hipEvent_t ev[12];
+  float pass_ms[5] = {0, 0, 0, 0, 0};
+  bool ev_recorded[5] = {false, false, false, false, false};
+};
+
+// ---- internal helpers (vct_runtime.hip) ---------------------------------
+int vct_set_error(int code, const std::string& msg);
+int vct_hip_check(hipError_t e, const char* what);
+int vct_ensure(VctBuf& b, size_t bytes, vct_context* ctx);
+#define VCT_HIP(call)                                  \
+  do {                                                 \
+    hipError_t _e = (call);                            \
+    if (_e != hipSuccess) return vct_hip_check(_e, #call); \
+  } while (0)
+
+// kernels launchers
+int vct_launch_mip(vct_context* ctx);
+int vct_launch_voxelize(vct_context* ctx);
+int vct_launch_gbuffer(vct_context* ctx);
+int vct_launch_trace(vct_context* ctx);
+int vct_launch_scan_u32(vct_context* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
+                        uint32_t* total_dev);
+int vct_tex3d_mip_launch(vct_context* ctx, VctTex3D& t);
+int vct_tex3d_clear_launch(vct_context* ctx, VctTex3D& t, const float rgba[4]);

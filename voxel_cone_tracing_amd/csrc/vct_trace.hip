@@ -1,0 +1,466 @@
+// vct_trace.hip - per-pixel cone marcher (kernel c).
+//
+// Shades every covered pixel as voxel_cone_tracing.frag:246-275 does:
+//   pos = scale_and_bias(world / cube_size)                      (:248)
+//   diffuse  = Kd * trace_diffuse(pos, n)                         (:140-168)
+//   direct   = direct_light(pos, view_dir) with shadow cones       (:175-218)
+//   specular = Ks * trace_specular(pos, n, view_dir)              (:229-235)
+//   refraction override for illum 4/6/7/9                         (:273-274)
+// trace_cone (:88-119) marches front to back, c += (1 - c.a) * s, with
+// mip = max(log2(diam * res), 0), step = max(diam / 2, voxel_size), and
+// sample_voxel (:80-86) = |d.x| T[x] + |d.y| T[y] + |d.z| T[z] (unnormalised).
+//
+// Sampler: a software quadrilinear textureLod over the brick/Morton volume
+// with GL semantics (texture_3d.cpp:10-15): texel centres at (i + 0.5) / s,
+// CLAMP_TO_BORDER with a zero border, LINEAR_MIPMAP_LINEAR between floor(lod)
+// and floor(lod) + 1, lod clamped to [0, L-1].  The three maps of one sample
+// share coordinates and weights, so corner addresses are computed once per
+// level; level 0 is shared by all maps, so it is fetched once per step.
+//
+// Work mapping: a 256-lane workgroup shades a 16x16 pixel tile; each wave is
+// an 8x8 sub-tile.  All lanes of a wave march the same cone index at the same
+// time over neighbouring surface points, so their texel fetches fall on the
+// same bricks and L2 lines.  Accumulation is FP32 for every storage type.
+#include <cmath>
+#include <cstring>
+
+#include "vct_internal.h"
+
+namespace vct {
+
+constexpr int kMaxCones = 16;
+
+struct ConeSet {
+  float a[kMaxCones], b[kMaxCones], c[kMaxCones];  // dir = a n + b o1 + c o2
+  float aperture;
+  int n;
+};
+
+struct TraceArgs {
+  const char* vol;
+  uint64_t off[16][6];
+  int levels;
+  int res;
+  const float4* gpos;
+  const float4* gnrm;
+  const DevMaterial* mats;
+  float4* image;
+  int W, H;
+  float cube_size;
+  float cam[3];
+  int en_direct, en_diffuse, en_specular, en_shadow;
+  int view_dir;
+  float view_lod;
+  int demodulate;  // 1: write final / max(Kd, eps) (the learned path's target)
+  ConeSet cones;
+  LightSet lights;
+};
+
+template <int DT>
+struct Tex;
+template <>
+struct Tex<VCT_RGBA8> {
+  static __device__ __forceinline__ float4 load(const char* base, uint32_t i) {
+    const uint32_t v = reinterpret_cast<const uint32_t*>(base)[i];
+    const float k = 1.0f / 255.0f;
+    return make_float4((float)(v & 255u) * k, (float)((v >> 8) & 255u) * k, (float)((v >> 16) & 255u) * k,
+                       (float)(v >> 24) * k);
+  }
+};
+template <>
+struct Tex<VCT_FP16> {
+  static __device__ __forceinline__ float4 load(const char* base, uint32_t i) {
+    const uint2 v = reinterpret_cast<const uint2*>(base)[i];
+    return make_float4(f16_to_f32(v.x & 0xffffu), f16_to_f32(v.x >> 16), f16_to_f32(v.y & 0xffffu),
+                       f16_to_f32(v.y >> 16));
+  }
+};
+template <>
+struct Tex<VCT_BF16> {
+  static __device__ __forceinline__ float4 load(const char* base, uint32_t i) {
+    const uint2 v = reinterpret_cast<const uint2*>(base)[i];
+    return make_float4(bf16_to_f32(v.x & 0xffffu), bf16_to_f32(v.x >> 16), bf16_to_f32(v.y & 0xffffu),
+                       bf16_to_f32(v.y >> 16));
+  }
+};
+template <>
+struct Tex<VCT_FP32> {
+  static __device__ __forceinline__ float4 load(const char* base, uint32_t i) {
+    return reinterpret_cast<const float4*>(base)[i];
+  }
+};
+
+__device__ __forceinline__ float4 f4(float x) { return make_float4(x, x, x, x); }
+__device__ __forceinline__ float4 operator+(float4 a, float4 b) {
+  return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+__device__ __forceinline__ float4 operator*(float s, float4 a) {
+  return make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
+}
+
+// Separable parts of brick_index: index(x,y,z) = ix(x) | iy(y) | iz(z).
+__device__ __forceinline__ uint32_t part_x(uint32_t x) { return (spread3(x >> 2) << 6) | (x & 3u); }
+__device__ __forceinline__ uint32_t part_y(uint32_t y) { return (spread3(y >> 2) << 7) | ((y & 3u) << 2); }
+__device__ __forceinline__ uint32_t part_z(uint32_t z) { return (spread3(z >> 2) << 8) | ((z & 3u) << 4); }
+
+struct Corners {
+  uint32_t ix[2], iy[2], iz[2];
+  float wx[2], wy[2], wz[2];  // zero weight for border texels
+};
+
+__device__ __forceinline__ void corners(float3 p, int s, Corners& c) {
+  const float fs = (float)s;
+  const float u = p.x * fs - 0.5f, v = p.y * fs - 0.5f, w = p.z * fs - 0.5f;
+  const float fu = floorf(u), fv = floorf(v), fw = floorf(w);
+  const float au = u - fu, av = v - fv, aw = w - fw;
+  const int x0 = (int)fu, y0 = (int)fv, z0 = (int)fw;
+  // Out-of-range texels take the zero border colour: weight 0, index clamped.
+  c.wx[0] = (x0 >= 0 && x0 < s) ? 1.0f - au : 0.0f;
+  c.wx[1] = (x0 + 1 >= 0 && x0 + 1 < s) ? au : 0.0f;
+  c.wy[0] = (y0 >= 0 && y0 < s) ? 1.0f - av : 0.0f;
+  c.wy[1] = (y0 + 1 >= 0 && y0 + 1 < s) ? av : 0.0f;
+  c.wz[0] = (z0 >= 0 && z0 < s) ? 1.0f - aw : 0.0f;
+  c.wz[1] = (z0 + 1 >= 0 && z0 + 1 < s) ? aw : 0.0f;
+  const int m = s - 1;
+  c.ix[0] = part_x((uint32_t)min(max(x0, 0), m));
+  c.ix[1] = part_x((uint32_t)min(max(x0 + 1, 0), m));
+  c.iy[0] = part_y((uint32_t)min(max(y0, 0), m));
+  c.iy[1] = part_y((uint32_t)min(max(y0 + 1, 0), m));
+  c.iz[0] = part_z((uint32_t)min(max(z0, 0), m));
+  c.iz[1] = part_z((uint32_t)min(max(z0 + 1, 0), m));
+}
+
+template <int DT>
+__device__ __forceinline__ float4 trilinear(const char* base, const Corners& c) {
+  float4 acc = f4(0.0f);
+#pragma unroll
+  for (int dz = 0; dz < 2; ++dz) {
+    if (c.wz[dz] == 0.0f) continue;
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy) {
+      const float wyz = c.wy[dy] * c.wz[dz];
+      if (wyz == 0.0f) continue;
+#pragma unroll
+      for (int dx = 0; dx < 2; ++dx) {
+        const float wt = c.wx[dx] * wyz;
+        if (wt == 0.0f) continue;
+        acc = acc + wt * Tex<DT>::load(base, c.ix[dx] | c.iy[dy] | c.iz[dz]);
+      }
+    }
+  }
+  return acc;
+}
+
+// sample_voxel with its three textureLod calls folded together.
+template <int DT>
+__device__ __forceinline__ float4 sample_voxel(const TraceArgs& A, float3 p, float3 ad, int mx, int my, int mz,
+                                               float lod) {
+  const float lmax = (float)(A.levels - 1);
+  lod = fminf(fmaxf(lod, 0.0f), lmax);
+  const int l0 = (int)floorf(lod);
+  const float t = lod - (float)l0;
+  float4 acc = f4(0.0f);
+  const int nl = (t > 0.0f && l0 + 1 < A.levels) ? 2 : 1;
+  for (int k = 0; k < nl; ++k) {
+    const int l = l0 + k;
+    const float lw = (nl == 1) ? 1.0f : (k == 0 ? 1.0f - t : t);
+    Corners c;
+    corners(p, A.res >> l, c);
+    if (l == 0) {
+      const float wsum = ad.x + ad.y + ad.z;
+      acc = acc + (lw * wsum) * trilinear<DT>(A.vol + A.off[0][0], c);
+    } else {
+      float4 s = f4(0.0f);
+      if (ad.x != 0.0f) s = s + ad.x * trilinear<DT>(A.vol + A.off[l][mx], c);
+      if (ad.y != 0.0f) s = s + ad.y * trilinear<DT>(A.vol + A.off[l][my], c);
+      if (ad.z != 0.0f) s = s + ad.z * trilinear<DT>(A.vol + A.off[l][mz], c);
+      acc = acc + lw * s;
+    }
+  }
+  return acc;
+}
+
+__device__ __forceinline__ float3 normalize3(float3 v) {
+  const float l = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+  return make_float3(v.x / l, v.y / l, v.z / l);
+}
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+  return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float len3(float3 v) { return sqrtf(dot3(v, v)); }
+
+// trace_cone (voxel_cone_tracing.frag:88-119)
+template <int DT>
+__device__ float4 trace_cone(const TraceArgs& A, float3 origin, float3 dir, float aperture, float max_dist) {
+  dir = normalize3(dir);
+  const int mx = dir.x < 0.0f ? 0 : 1, my = dir.y < 0.0f ? 2 : 3, mz = dir.z < 0.0f ? 4 : 5;
+  const float3 ad = make_float3(fabsf(dir.x), fabsf(dir.y), fabsf(dir.z));
+  const float vs = 1.0f / (float)A.res;
+  const float fres = (float)A.res;
+  float4 c = f4(0.0f);
+  float dist = 3.0f * vs;
+  float diam = dist * aperture;
+  while (c.w < 1.0f && dist < max_dist) {
+    const float3 sp = make_float3(dir.x * dist + origin.x, dir.y * dist + origin.y, dir.z * dist + origin.z);
+    const float mip = fmaxf(log2f(diam * fres), 0.0f);
+    const float4 s = sample_voxel<DT>(A, sp, ad, mx, my, mz, mip);
+    c = c + (1.0f - c.w) * s;
+    dist += fmaxf(diam * 0.5f, vs);
+    diam = dist * aperture;
+  }
+  return c;
+}
+
+// tangent (voxel_cone_tracing.frag:126-134)
+__device__ __forceinline__ float3 tangent_of(float3 n) {
+  const float3 t1 = cross3(n, make_float3(0.f, 0.f, 1.f));
+  const float3 t2 = cross3(n, make_float3(0.f, 1.f, 0.f));
+  return len3(t1) > len3(t2) ? normalize3(t1) : normalize3(t2);
+}
+
+__device__ __forceinline__ float3 refract3(float3 I, float3 N, float eta) {
+  const float d = dot3(N, I);
+  const float k = 1.0f - eta * eta * (1.0f - d * d);
+  if (k < 0.0f) return make_float3(0.f, 0.f, 0.f);
+  const float s = eta * d + sqrtf(k);
+  return make_float3(eta * I.x - s * N.x, eta * I.y - s * N.y, eta * I.z - s * N.z);
+}
+
+template <int DT>
+__global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+  const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+  if (px >= A.W || py >= A.H) return;
+  const size_t idx = (size_t)py * A.W + px;
+  const float4 gp = A.gpos[idx];
+  const float4 bg = make_float4(0.15f, 0.25f, 0.25f, 1.0f);  // renderer.cpp:398
+  if (gp.w == 0.0f) {
+    A.image[idx] = A.demodulate ? f4(0.0f) : bg;
+    return;
+  }
+  const float4 gn = A.gnrm[idx];
+  const DevMaterial& m = A.mats[__builtin_bit_cast(uint32_t, gn.w)];
+  const float3 world = make_float3(gp.x, gp.y, gp.z);
+  const float3 n = make_float3(gn.x, gn.y, gn.z);
+  const float3 pos = make_float3(0.5f * (world.x / A.cube_size) + 0.5f, 0.5f * (world.y / A.cube_size) + 0.5f,
+                                 0.5f * (world.z / A.cube_size) + 0.5f);
+  // within_cube(pos, 0) on biased coordinates (:250, :61-64)
+  if (!(fabsf(pos.x) < 1.0f && fabsf(pos.y) < 1.0f && fabsf(pos.z) < 1.0f)) {
+    A.image[idx] = A.demodulate ? f4(0.0f) : bg;
+    return;
+  }
+  if (A.view_dir < 7) {  // voxel view (:253-257), blended SRC_ALPHA / ONE_MINUS_SRC_ALPHA
+    const int d = A.view_dir < 0 ? 0 : (A.view_dir > 5 ? 5 : A.view_dir);
+    float3 ad = make_float3(0.f, 0.f, 0.f);
+    int mx = 0, my = 2, mz = 4;
+    if (d < 2) { ad.x = 1.0f; mx = d; }
+    else if (d < 4) { ad.y = 1.0f; my = d; }
+    else { ad.z = 1.0f; mz = d; }
+    const float4 s = sample_voxel<DT>(A, pos, ad, mx, my, mz, A.view_lod);
+    const float a = s.w;
+    A.image[idx] = make_float4(s.x * a + bg.x * (1.0f - a), s.y * a + bg.y * (1.0f - a),
+                               s.z * a + bg.z * (1.0f - a), a * a + bg.w * (1.0f - a));
+    return;
+  }
+  const float3 cam = make_float3(A.cam[0], A.cam[1], A.cam[2]);
+  const float3 view_dir = normalize3(make_float3(world.x - cam.x, world.y - cam.y, world.z - cam.z));
+  const float sqrt3 = 1.73205080757f;
+  float3 out = make_float3(0.f, 0.f, 0.f);
+  if (A.en_diffuse) {
+    const float3 o1 = normalize3(tangent_of(n));
+    const float3 o2 = normalize3(cross3(o1, n));
+    float4 acc = f4(0.0f);
+    for (int k = 0; k < A.cones.n; ++k) {
+      const float a = A.cones.a[k], b = A.cones.b[k], c = A.cones.c[k];
+      const float3 d = make_float3(a * n.x + b * o1.x + c * o2.x, a * n.y + b * o1.y + c * o2.y,
+                                   a * n.z + b * o1.z + c * o2.z);
+      acc = acc + trace_cone<DT>(A, pos, d, A.cones.aperture, sqrt3);
+    }
+    const float inv = 1.0f / (float)A.cones.n;
+    out.x += m.diffuse[0] * (acc.x * inv);
+    out.y += m.diffuse[1] * (acc.y * inv);
+    out.z += m.diffuse[2] * (acc.z * inv);
+  }
+  // specular aperture (:220-227)
+  const float rough = sqrtf(2.0f / (m.shininess + 2.0f));
+  const float spec_ap = fminf(fmaxf(tanf(1.57079f * rough), 0.0174533f), 3.14159265f);
+  if (A.en_direct) {
+    float3 res = make_float3(0.f, 0.f, 0.f);
+    const int nl = A.lights.count < VCT_MAX_LIGHTS ? A.lights.count : VCT_MAX_LIGHTS;
+    for (int li = 0; li < nl; ++li) {
+      const DevLight& L = A.lights.l[li];
+      float3 ldir;
+      float d;
+      if (L.kind == VCT_LIGHT_DIRECTIONAL) {
+        ldir = normalize3(make_float3(L.pos[0], L.pos[1], L.pos[2]));
+        d = sqrt3;
+      } else {
+        const float3 lp = make_float3(0.5f * (L.pos[0] / A.cube_size) + 0.5f, 0.5f * (L.pos[1] / A.cube_size) + 0.5f,
+                                      0.5f * (L.pos[2] / A.cube_size) + 0.5f);
+        ldir = make_float3(lp.x - pos.x, lp.y - pos.y, lp.z - pos.z);
+        d = len3(ldir);
+        ldir = normalize3(ldir);
+      }
+      const float cos_surf = fmaxf(dot3(n, ldir), 0.0f);
+      const float att = (L.kind == VCT_LIGHT_DIRECTIONAL) ? 1.0f : 1.0f / (1.0f + d * d);
+      const float k = att * cos_surf;
+      const float3 lc = make_float3(k * L.color[0] * L.intensity, k * L.color[1] * L.intensity,
+                                    k * L.color[2] * L.intensity);
+      float shadow = 1.0f;
+      if (A.en_shadow) shadow = fmaxf(0.0f, 1.0f - trace_cone<DT>(A, pos, ldir, 0.1f, d).w);
+      const float lambert = fmaxf(dot3(ldir, n), 0.0f);
+      float refr_angle = 0.0f;
+      if (m.dissolve <= 0.1f) {
+        const float3 r = refract3(view_dir, n, 1.0f / m.ior);
+        refr_angle = fmaxf((1.0f - m.dissolve) * dot3(r, ldir), 0.0f);
+      }
+      const float3 hv = normalize3(make_float3(ldir.x + view_dir.x, ldir.y + view_dir.y, ldir.z + view_dir.z));
+      float sa = fmaxf(dot3(hv, n), 0.0f);
+      sa = fmaxf(sa, refr_angle);
+      const float sc = powf(sa, m.shininess);
+      const float f = shadow + 0.04f;
+      res.x += f * (lambert * m.diffuse[0] + sc * m.specular[0]) * lc.x;
+      res.y += f * (lambert * m.diffuse[1] + sc * m.specular[1]) * lc.y;
+      res.z += f * (lambert * m.diffuse[2] + sc * m.specular[2]) * lc.z;
+    }
+    out.x += res.x + fminf(fmaxf(m.emission[0], 0.0f), 1.0f);
+    out.y += res.y + fminf(fmaxf(m.emission[1], 0.0f), 1.0f);
+    out.z += res.z + fminf(fmaxf(m.emission[2], 0.0f), 1.0f);
+  }
+  if (A.en_specular) {
+    // normalize(reflect(-view_dir, n)) (:231)
+    const float3 I = make_float3(-view_dir.x, -view_dir.y, -view_dir.z);
+    const float dn = 2.0f * dot3(n, I);
+    const float3 r = make_float3(I.x - dn * n.x, I.y - dn * n.y, I.z - dn * n.z);
+    const float4 s = trace_cone<DT>(A, pos, normalize3(r), spec_ap, sqrt3);
+    out.x += m.specular[0] * s.x;
+    out.y += m.specular[1] * s.y;
+    out.z += m.specular[2] * s.z;
+  }
+  if (illum_transmissive(m.illum) && A.en_specular) {
+    const float3 rd = refract3(view_dir, n, 1.0f / m.ior);
+    const float4 s = trace_cone<DT>(A, pos, rd, spec_ap, sqrt3);
+    const float3 tr = make_float3(m.transmittance[0] * s.x, m.transmittance[1] * s.y, m.transmittance[2] * s.z);
+    const float dv = m.dissolve;
+    out = make_float3(tr.x * (1.0f - dv) + out.x * dv, tr.y * (1.0f - dv) + out.y * dv,
+                      tr.z * (1.0f - dv) + out.z * dv);
+  }
+  if (A.demodulate) {
+    const float e = 1.0f / 1024.0f;
+    out = make_float3(out.x / fmaxf(m.diffuse[0], e), out.y / fmaxf(m.diffuse[1], e), out.z / fmaxf(m.diffuse[2], e));
+  }
+  A.image[idx] = make_float4(out.x, out.y, out.z, 1.0f);
+}
+
+}  // namespace vct
+
+using namespace vct;
+
+// Cone direction tables in the tangent frame (n, o1, o2).
+void vct_cone_set_table(int set, ConeSet& cs) {
+  memset(&cs, 0, sizeof(cs));
+  if (set == VCT_CONES_REF9) {
+    // trace_diffuse (voxel_cone_tracing.frag:140-168): n, mix(n, +-o1/+-o2, .5),
+    // mix(n, +-c1/+-c2, .5) with c1 = (o1 + o2) / 2, c2 = (o1 - o2) / 2.
+    const float t[9][3] = {{1.f, 0.f, 0.f},      {.5f, .5f, 0.f},     {.5f, -.5f, 0.f},
+                           {.5f, 0.f, .5f},      {.5f, 0.f, -.5f},    {.5f, .25f, .25f},
+                           {.5f, -.25f, -.25f},  {.5f, .25f, -.25f},  {.5f, -.25f, .25f}};
+    cs.n = 9;
+    for (int i = 0; i < 9; ++i) {
+      cs.a[i] = t[i][0];
+      cs.b[i] = t[i][1];
+      cs.c[i] = t[i][2];
+    }
+    cs.aperture = 0.55785173935f;  // TAN_22_5 (:136)
+  } else if (set == VCT_CONES_5_WIDE) {
+    const float t[5][3] = {{1.f, 0.f, 0.f}, {.5f, .5f, 0.f}, {.5f, -.5f, 0.f}, {.5f, 0.f, .5f}, {.5f, 0.f, -.5f}};
+    cs.n = 5;
+    for (int i = 0; i < 5; ++i) {
+      cs.a[i] = t[i][0];
+      cs.b[i] = t[i][1];
+      cs.c[i] = t[i][2];
+    }
+    cs.aperture = 1.15470054f;  // 2 tan(30 deg): 60 degree cones
+  } else {
+    // 16 cones: the normal, 5 at 28 degrees, 10 at 58 degrees from it.
+    cs.n = 16;
+    cs.a[0] = 1.f;
+    int k = 1;
+    const double d2r = 3.14159265358979323846 / 180.0;
+    for (int i = 0; i < 5; ++i, ++k) {
+      double th = 28.0 * d2r, ph = 72.0 * i * d2r;
+      cs.a[k] = (float)cos(th);
+      cs.b[k] = (float)(sin(th) * cos(ph));
+      cs.c[k] = (float)(sin(th) * sin(ph));
+    }
+    for (int i = 0; i < 10; ++i, ++k) {
+      double th = 58.0 * d2r, ph = (36.0 * i + 18.0) * d2r;
+      cs.a[k] = (float)cos(th);
+      cs.b[k] = (float)(sin(th) * cos(ph));
+      cs.c[k] = (float)(sin(th) * sin(ph));
+    }
+    cs.aperture = 0.45f;
+  }
+}
+
+int vct_trace_into(vct_context* ctx, const float4* gpos, const float4* gnrm, float4* out, int W, int H,
+                   int demodulate) {
+  TraceArgs A;
+  memset(&A, 0, sizeof(A));
+  A.vol = (const char*)ctx->volume.p;
+  for (int l = 0; l < ctx->levels; ++l)
+    for (int d = 0; d < 6; ++d) A.off[l][d] = ctx->lvl_off[l][d];
+  A.levels = ctx->levels;
+  A.res = (int)ctx->res;
+  A.gpos = gpos;
+  A.gnrm = gnrm;
+  A.mats = (const DevMaterial*)ctx->d_materials.p;
+  A.image = out;
+  A.W = W;
+  A.H = H;
+  A.cube_size = ctx->cube_size;
+  // camera_position = column 3 of the view matrix (renderer.cpp:279)
+  A.cam[0] = ctx->view[12];
+  A.cam[1] = ctx->view[13];
+  A.cam[2] = ctx->view[14];
+  A.en_direct = ctx->en_direct;
+  A.en_diffuse = ctx->en_diffuse;
+  A.en_specular = ctx->en_specular;
+  A.en_shadow = ctx->en_shadow;
+  A.view_dir = ctx->view_voxel_dir;
+  A.view_lod = ctx->view_voxel_lod;
+  A.demodulate = demodulate;
+  vct_cone_set_table(ctx->cone_set, A.cones);
+  int n = (int)ctx->lights.size();
+  if (n > VCT_MAX_LIGHTS) n = VCT_MAX_LIGHTS;
+  for (int i = 0; i < n; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      A.lights.l[i].pos[k] = ctx->lights[i].position[k];
+      A.lights.l[i].color[k] = ctx->lights[i].color[k];
+    }
+    A.lights.l[i].intensity = ctx->lights[i].intensity;
+    A.lights.l[i].kind = ctx->lights[i].kind;
+  }
+  A.lights.count = n;
+  dim3 grid((W + 15) / 16, (H + 15) / 16);
+  switch (ctx->dtype) {
+    case VCT_RGBA8: hipLaunchKernelGGL(trace_kernel<VCT_RGBA8>, grid, dim3(256), 0, ctx->stream, A); break;
+    case VCT_FP16: hipLaunchKernelGGL(trace_kernel<VCT_FP16>, grid, dim3(256), 0, ctx->stream, A); break;
+    case VCT_BF16: hipLaunchKernelGGL(trace_kernel<VCT_BF16>, grid, dim3(256), 0, ctx->stream, A); break;
+    default: hipLaunchKernelGGL(trace_kernel<VCT_FP32>, grid, dim3(256), 0, ctx->stream, A); break;
+  }
+  VCT_HIP(hipGetLastError());
+  return VCT_OK;
+}
+
+int vct_launch_trace(vct_context* ctx) {
+  const size_t npix = (size_t)ctx->width * ctx->height;
+  if (!ctx->g_pos.p || ctx->g_pos.cap < npix * 16)
+    return vct_set_error(VCT_ERR_STATE, "trace needs a G-buffer of the current viewport (run vct_gbuffer first)");
+  int rc;
+  if ((rc = vct_ensure(ctx->image, npix * 16, ctx))) return rc;
+  return vct_trace_into(ctx, (const float4*)ctx->g_pos.p, (const float4*)ctx->g_nrm.p, (float4*)ctx->image.p,
+                        ctx->width, ctx->height, 0);
+}

@@ -1,0 +1,686 @@
+// vct_voxelize.hip - voxelisation (kernel a).
+//
+// Replaces the voxelize program: voxelize.vert:24-30 (world position divided
+// by cube_size, normal by the inverse-transpose model matrix),
+// voxelize.geom:25-55 (projection onto the plane of the dominant axis of
+// |cross(e1, e2)|, ties to XZ), GL rasterisation at pixel centres of a
+// 2res x 2res viewport (renderer.cpp:339-343), and voxelize.frag:122-161
+// (per-fragment direct lighting and the collision average into the voxel at
+// ivec3(res * (0.5 p + 0.5))).
+//
+// MI355X design
+//  1. count  (one lane per triangle): dominant axis, vertices snapped to
+//     1/256 pixel, pixel bounding box in 2x2 quads -> 4 * quads candidates.
+//  2. exclusive scan of the candidate counts (tri-major canonical order).
+//  3. raster (one lane per candidate pixel, grid-stride over the device-side
+//     total, no host round trip): a workgroup finds the triangle range of its
+//     256 candidates once, each lane binary-searches inside it, recomputes
+//     the triangle setup, tests coverage with exact 64-bit edge functions
+//     (top-left rule) and shades the fragment.  The four lanes of a pixel quad
+//     belong to one triangle and usually one voxel; they are combined in
+//     registers before touching memory (the wave-level collision step).
+//  4a. exact mean (any storage): order-independent 16.16 fixed-point sums
+//     with 64-bit atomics, then a resolve pass per voxel that writes level 0
+//     and re-zeroes the sums for the next frame.
+//  4b. reference running average (RGBA8): fragments are appended, counted
+//     per voxel, bucketed by a scan over the voxels, sorted inside each
+//     bucket by canonical candidate index and replayed through the packed
+//     RGBA8 + count-nibble recurrence of imageAtomicRGBA8Avg
+//     (voxelize.frag:95-120) - the result no longer depends on scheduling.
+// Bit-exact parts (coverage, voxel coordinates, fragment colours) use no
+// fused multiply-add and correctly rounded division / square root.
+#include <cmath>
+#include <cstring>
+
+#include "vct_internal.h"
+
+namespace vct {
+
+struct Draw {
+  const float* pos;
+  const float* nrm;
+  const uint32_t* mat;
+  uint32_t tri_base;  // first triangle of this draw in the batch
+  uint32_t ntri;
+  float M[16];  // model matrix, column-major
+  float N[9];   // mat3(transpose(inverse(model))), column-major
+};
+
+constexpr int kMaxDraws = 16;
+
+struct DrawBatch {
+  Draw d[kMaxDraws];
+  int ndraw;
+  uint32_t ntri;
+};
+
+struct VoxParams {
+  uint32_t res;
+  float inv_unused;
+  float cube_size;
+  int average;   // VCT_AVG_*
+  uint32_t cand_base;  // canonical index of this batch's first candidate
+  LightSet lights;
+};
+
+struct Setup {
+  float w[3][3];   // world / cube_size
+  int64_t X[3], Y[3];
+  int axis;
+  int qx0, qy0, qw, qh;
+  int64_t area;
+  bool swap;
+  bool ok;
+};
+
+__device__ __forceinline__ int find_draw(const DrawBatch& b, uint32_t t) {
+  int d = 0;
+  for (int i = 1; i < b.ndraw; ++i)
+    if (t >= b.d[i].tri_base) d = i;
+  return d;
+}
+
+// voxelize.vert:26 + voxelize.geom:27-51 + viewport + snap.
+__device__ __forceinline__ void tri_setup(const Draw& dr, uint32_t lt, uint32_t res, float cube_size, Setup& s) {
+#pragma clang fp contract(off)
+  const float* p = dr.pos + (size_t)lt * 9;
+  for (int v = 0; v < 3; ++v) {
+    const float px = p[v * 3 + 0], py = p[v * 3 + 1], pz = p[v * 3 + 2];
+    for (int k = 0; k < 3; ++k) {
+      float t = dr.M[0 + k] * px;
+      t = t + dr.M[4 + k] * py;
+      t = t + dr.M[8 + k] * pz;
+      t = t + dr.M[12 + k];
+      s.w[v][k] = t / cube_size;
+    }
+  }
+  float e1[3], e2[3];
+  for (int k = 0; k < 3; ++k) {
+    e1[k] = s.w[1][k] - s.w[0][k];
+    e2[k] = s.w[2][k] - s.w[0][k];
+  }
+  float cx = fabsf(e1[1] * e2[2] - e1[2] * e2[1]);
+  float cy = fabsf(e1[2] * e2[0] - e1[0] * e2[2]);
+  float cz = fabsf(e1[0] * e2[1] - e1[1] * e2[0]);
+  int axis, ia, ib;
+  if (cz > cx && cz > cy) {
+    axis = 2; ia = 0; ib = 1;   // XY plane
+  } else if (cx > cy && cx > cz) {
+    axis = 0; ia = 1; ib = 2;   // YZ plane
+  } else {
+    axis = 1; ia = 0; ib = 2;   // XZ plane
+  }
+  s.axis = axis;
+  const float fres = (float)res;
+  bool ok = true;
+  for (int v = 0; v < 3; ++v) {
+    float a = s.w[v][ia], b = s.w[v][ib];
+    if (!(fabsf(a) < 64.0f && fabsf(b) < 64.0f)) ok = false;
+    float xw = a * fres + fres;  // viewport transform, scale = translate = res
+    float yw = b * fres + fres;
+    s.X[v] = (int64_t)__builtin_rintf((xw - 0.5f) * 256.0f);
+    s.Y[v] = (int64_t)__builtin_rintf((yw - 0.5f) * 256.0f);
+  }
+  int64_t area = (s.X[1] - s.X[0]) * (s.Y[2] - s.Y[0]) - (s.Y[1] - s.Y[0]) * (s.X[2] - s.X[0]);
+  s.swap = area < 0;
+  s.area = area < 0 ? -area : area;
+  if (area == 0) ok = false;
+  int64_t xmin = min(s.X[0], min(s.X[1], s.X[2])), xmax = max(s.X[0], max(s.X[1], s.X[2]));
+  int64_t ymin = min(s.Y[0], min(s.Y[1], s.Y[2])), ymax = max(s.Y[0], max(s.Y[1], s.Y[2]));
+  const int64_t lim = 2 * (int64_t)res - 1;
+  int64_t imin = (xmin + 255) >> 8, imax = xmax >> 8, jmin = (ymin + 255) >> 8, jmax = ymax >> 8;
+  imin = imin < 0 ? 0 : imin;
+  jmin = jmin < 0 ? 0 : jmin;
+  imax = imax > lim ? lim : imax;
+  jmax = jmax > lim ? lim : jmax;
+  if (imin > imax || jmin > jmax) ok = false;
+  s.ok = ok;
+  if (ok) {
+    s.qx0 = (int)(imin & ~1ll);
+    s.qy0 = (int)(jmin & ~1ll);
+    s.qw = (int)((imax - s.qx0) >> 1) + 1;
+    s.qh = (int)((jmax - s.qy0) >> 1) + 1;
+  } else {
+    s.qx0 = s.qy0 = s.qw = s.qh = 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void vox_count_kernel(DrawBatch b, VoxParams vp, uint32_t* counts) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.ntri) return;
+  const int d = find_draw(b, t);
+  Setup s;
+  tri_setup(b.d[d], t - b.d[d].tri_base, vp.res, vp.cube_size, s);
+  counts[t] = s.ok ? (uint32_t)(4 * s.qw * s.qh) : 0u;
+}
+
+__device__ __forceinline__ bool top_left(int64_t dx, int64_t dy) { return dy < 0 || (dy == 0 && dx < 0); }
+
+// Fragment of candidate `lc` of triangle `lt`: coverage, voxel and colour.
+// Returns false if the pixel is not covered or the voxel falls outside.
+__device__ __forceinline__ bool shade_fragment(const Draw& dr, uint32_t lt, const Setup& s, uint32_t lc,
+                                               const VoxParams& vp, const DevMaterial* mats, uint32_t& key,
+                                               float4& color) {
+#pragma clang fp contract(off)
+  const uint32_t quad = lc >> 2, sub = lc & 3u;
+  const int i = s.qx0 + 2 * (int)(quad % (uint32_t)s.qw) + (int)(sub & 1u);
+  const int j = s.qy0 + 2 * (int)(quad / (uint32_t)s.qw) + (int)(sub >> 1);
+  const int lim = 2 * (int)vp.res;
+  if (i >= lim || j >= lim) return false;
+  const int64_t px = (int64_t)i * 256, py = (int64_t)j * 256;
+  // CCW vertex order (a, b, c) = (0, 1, 2) or (0, 2, 1)
+  const int ia = 0, ib = s.swap ? 2 : 1, ic = s.swap ? 1 : 2;
+  const int64_t ax = s.X[ia], ay = s.Y[ia], bx = s.X[ib], by = s.Y[ib], cx = s.X[ic], cy = s.Y[ic];
+  // weights of a, b, c
+  int64_t wa = (cx - bx) * (py - by) - (cy - by) * (px - bx);
+  int64_t wb = (ax - cx) * (py - cy) - (ay - cy) * (px - cx);
+  int64_t wc = (bx - ax) * (py - ay) - (by - ay) * (px - ax);
+  if (wa < 0 || wb < 0 || wc < 0) return false;
+  if (wa == 0 && !top_left(cx - bx, cy - by)) return false;
+  if (wb == 0 && !top_left(ax - cx, ay - cy)) return false;
+  if (wc == 0 && !top_left(bx - ax, by - ay)) return false;
+  int64_t w[3];
+  w[ia] = wa;
+  w[ib] = wb;
+  w[ic] = wc;
+  const float fa = (float)s.area;
+  const float b0 = (float)w[0] / fa, b1 = (float)w[1] / fa, b2 = (float)w[2] / fa;
+  float pos[3];
+  for (int k = 0; k < 3; ++k) pos[k] = b0 * s.w[0][k] + b1 * s.w[1][k] + b2 * s.w[2][k];
+  // voxel address, voxelize.frag:156-157 (truncation toward zero)
+  int v[3];
+  const float fres = (float)vp.res;
+  for (int k = 0; k < 3; ++k) {
+    float f = fres * (0.5f * pos[k] + 0.5f);
+    if (!(f > -1.0f && f < fres)) return false;
+    v[k] = (int)f;
+  }
+  key = brick_index((uint32_t)v[0], (uint32_t)v[1], (uint32_t)v[2]);
+  // normal: voxelize.vert:28 per vertex, interpolated, normalised in the
+  // fragment shader (voxelize.frag:134)
+  const float* nr = dr.nrm + (size_t)lt * 9;
+  float nv[3][3];
+  for (int vv = 0; vv < 3; ++vv) {
+    float t[3];
+    for (int k = 0; k < 3; ++k) {
+      float a = dr.N[0 + k] * nr[vv * 3 + 0];
+      a = a + dr.N[3 + k] * nr[vv * 3 + 1];
+      a = a + dr.N[6 + k] * nr[vv * 3 + 2];
+      t[k] = a;
+    }
+    float l = sqrtf(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+    for (int k = 0; k < 3; ++k) nv[vv][k] = t[k] / l;
+  }
+  float n[3];
+  for (int k = 0; k < 3; ++k) n[k] = b0 * nv[0][k] + b1 * nv[1][k] + b2 * nv[2][k];
+  {
+    float l = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int k = 0; k < 3; ++k) n[k] = n[k] / l;
+  }
+  // lighting, voxelize.frag:127-143
+  float col[3] = {0.0f, 0.0f, 0.0f};
+  const int nl = vp.lights.count < VCT_MAX_LIGHTS ? vp.lights.count : VCT_MAX_LIGHTS;
+  for (int li = 0; li < nl; ++li) {
+    const DevLight& L = vp.lights.l[li];
+    float dir[3];
+    float att;
+    if (L.kind == VCT_LIGHT_DIRECTIONAL) {
+      float l = sqrtf(L.pos[0] * L.pos[0] + L.pos[1] * L.pos[1] + L.pos[2] * L.pos[2]);
+      for (int k = 0; k < 3; ++k) dir[k] = L.pos[k] / l;
+      att = 1.0f;
+    } else {
+      float dv[3];
+      for (int k = 0; k < 3; ++k) dv[k] = L.pos[k] / vp.cube_size - pos[k];
+      float d = sqrtf(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+      for (int k = 0; k < 3; ++k) dir[k] = dv[k] / d;
+      att = 1.0f / (1.0f + d * d);
+    }
+    float c = n[0] * dir[0] + n[1] * dir[1] + n[2] * dir[2];
+    c = c > 0.0f ? c : 0.0f;
+    float ca = c * att;
+    for (int k = 0; k < 3; ++k) col[k] = col[k] + ca * L.color[k] * L.intensity;
+  }
+  const DevMaterial& m = mats[dr.mat[lt]];
+  float tr[3] = {1.0f, 1.0f, 1.0f};
+  float alpha = 1.0f;
+  if (illum_transmissive(m.illum)) {
+    tr[0] = m.transmittance[0];
+    tr[1] = m.transmittance[1];
+    tr[2] = m.transmittance[2];
+    alpha = m.dissolve;
+  }
+  float out[4];
+  for (int k = 0; k < 3; ++k) {
+    float c = m.diffuse[k] * col[k] + m.emission[k];
+    out[k] = tr[k] * c;
+  }
+  out[3] = alpha;
+  for (int k = 0; k < 4; ++k) out[k] = fminf(fmaxf(out[k], 0.0f), 1.0f);
+  color = make_float4(out[0], out[1], out[2], out[3]);
+  return true;
+}
+
+struct RasterOut {
+  // exact mode
+  unsigned long long* sums;  // 2 per voxel: (R | G << 32), (B | A << 32)
+  uint32_t* vcount;          // fragments per voxel
+  // reference mode
+  uint32_t* f_key;
+  uint32_t* f_cand;
+  float4* f_color;
+  uint32_t* f_counter;
+  uint32_t f_cap;
+};
+
+__device__ __forceinline__ uint32_t fixq(float c) { return (uint32_t)(c * 65536.0f + 0.5f); }
+
+__global__ __launch_bounds__(256) void vox_raster_kernel(DrawBatch b, VoxParams vp, const uint32_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ d_total,
+                                                         const DevMaterial* __restrict__ mats, RasterOut o) {
+  __shared__ uint32_t s_range[2];
+  const uint32_t total = *d_total;
+  const uint32_t n = b.ntri;
+  const int lane = threadIdx.x & 63;
+  for (uint32_t base = blockIdx.x * 256u; base < total; base += gridDim.x * 256u) {
+    const uint32_t last = min(base + 255u, total - 1u);
+    if (threadIdx.x == 0 || threadIdx.x == 255) {
+      const uint32_t key = threadIdx.x == 0 ? base : last;
+      uint32_t lo = 0, hi = n;  // largest t with off[t] <= key
+      while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= key) lo = mid;
+        else hi = mid;
+      }
+      s_range[threadIdx.x == 0 ? 0 : 1] = lo;
+    }
+    __syncthreads();
+    const uint32_t c = base + threadIdx.x;
+    bool have = false;
+    uint32_t key = 0xffffffffu;
+    float4 color = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (c < total) {
+      uint32_t lo = s_range[0], hi = s_range[1] + 1;
+      while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= c) lo = mid;
+        else hi = mid;
+      }
+      const uint32_t t = lo;
+      const int d = find_draw(b, t);
+      const Draw& dr = b.d[d];
+      Setup s;
+      tri_setup(dr, t - dr.tri_base, vp.res, vp.cube_size, s);
+      have = shade_fragment(dr, t - dr.tri_base, s, c - off[t], vp, mats, key, color);
+      if (!have) key = 0xffffffffu;
+    }
+    if (vp.average == VCT_AVG_EXACT) {
+      // combine the four lanes of each pixel quad that hit the same voxel
+      const int qb = lane & ~3;
+      uint32_t q[4] = {have ? fixq(color.x) : 0u, have ? fixq(color.y) : 0u, have ? fixq(color.z) : 0u,
+                       have ? fixq(color.w) : 0u};
+      unsigned long long rg = 0, ba = 0;
+      uint32_t cnt = 0;
+      bool leader = have;
+      for (int jq = 0; jq < 4; ++jq) {
+        const uint32_t kj = __shfl(key, qb + jq, 64);
+        const uint32_t r = __shfl(q[0], qb + jq, 64), g = __shfl(q[1], qb + jq, 64);
+        const uint32_t bl = __shfl(q[2], qb + jq, 64), al = __shfl(q[3], qb + jq, 64);
+        if (have && kj == key) {
+          if (qb + jq < lane) leader = false;
+          rg += (unsigned long long)r | ((unsigned long long)g << 32);
+          ba += (unsigned long long)bl | ((unsigned long long)al << 32);
+          ++cnt;
+        }
+      }
+      if (leader) {
+        atomicAdd(&o.sums[2ull * key], rg);
+        atomicAdd(&o.sums[2ull * key + 1], ba);
+        atomicAdd(&o.vcount[key], cnt);
+      }
+    } else {
+      // append (wave-aggregated slot allocation), count per voxel
+      const unsigned long long m = __ballot(have);
+      uint32_t slot0 = 0;
+      if (lane == 0 && m) slot0 = atomicAdd(o.f_counter, (uint32_t)__popcll(m));
+      slot0 = __shfl(slot0, 0, 64);
+      if (have) {
+        const uint32_t slot = slot0 + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (slot < o.f_cap) {
+          o.f_key[slot] = key;
+          o.f_cand[slot] = vp.cand_base + c;
+          o.f_color[slot] = color;
+        }
+        atomicAdd(&o.vcount[key], 1u);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Exact mean -> level 0; zeroes the sums for the next frame.
+template <int DT>
+__global__ __launch_bounds__(256) void vox_resolve_exact_kernel(unsigned long long* __restrict__ sums,
+                                                                uint32_t* __restrict__ vcount, void* level0,
+                                                                uint32_t nvox) {
+#pragma clang fp contract(off)
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nvox) return;
+  const uint32_t cnt = vcount[v];
+  float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (cnt) {
+    const unsigned long long rg = sums[2ull * v], ba = sums[2ull * v + 1];
+    const float fc = (float)cnt;
+    const float s = 1.0f / 65536.0f;
+    m.x = ((float)(uint32_t)(rg & 0xffffffffu) / fc) * s;
+    m.y = ((float)(uint32_t)(rg >> 32) / fc) * s;
+    m.z = ((float)(uint32_t)(ba & 0xffffffffu) / fc) * s;
+    m.w = ((float)(uint32_t)(ba >> 32) / fc) * s;
+    sums[2ull * v] = 0ull;
+    sums[2ull * v + 1] = 0ull;
+    vcount[v] = 0u;
+  }
+  if (DT == VCT_RGBA8) {
+    reinterpret_cast<uint32_t*>(level0)[v] =
+        to_unorm8(m.x) | (to_unorm8(m.y) << 8) | (to_unorm8(m.z) << 16) | (to_unorm8(m.w) << 24);
+  } else if (DT == VCT_FP16) {
+    reinterpret_cast<uint2*>(level0)[v] =
+        make_uint2((uint32_t)f32_to_f16(m.x) | ((uint32_t)f32_to_f16(m.y) << 16),
+                   (uint32_t)f32_to_f16(m.z) | ((uint32_t)f32_to_f16(m.w) << 16));
+  } else if (DT == VCT_BF16) {
+    reinterpret_cast<uint2*>(level0)[v] =
+        make_uint2((uint32_t)f32_to_bf16(m.x) | ((uint32_t)f32_to_bf16(m.y) << 16),
+                   (uint32_t)f32_to_bf16(m.z) | ((uint32_t)f32_to_bf16(m.w) << 16));
+  } else {
+    reinterpret_cast<float4*>(level0)[v] = m;
+  }
+}
+
+// ---- reference running average -------------------------------------------
+__device__ __forceinline__ uint32_t conv_vec4_to_rgba8(float x, float y, float z, float w) {
+  // voxelize.frag:66-71: uint() truncation, & 0xFF
+  return (((uint32_t)w & 0xffu) << 24) | (((uint32_t)z & 0xffu) << 16) | (((uint32_t)y & 0xffu) << 8) |
+         ((uint32_t)x & 0xffu);
+}
+__device__ __forceinline__ uint32_t enc_nibble(uint32_t m, uint32_t n) {
+  return (m & 0xFEFEFEFEu) | (n & 1u) | ((n & 2u) << 7) | ((n & 4u) << 14) | ((n & 8u) << 21);
+}
+__device__ __forceinline__ uint32_t dec_nibble(uint32_t m) {
+  return (m & 1u) | ((m & 0x100u) >> 7) | ((m & 0x10000u) >> 14) | ((m & 0x1000000u) >> 21);
+}
+
+__global__ __launch_bounds__(256) void vox_bucket_kernel(const uint32_t* __restrict__ f_key,
+                                                         const uint32_t* __restrict__ f_cand,
+                                                         const float4* __restrict__ f_color,
+                                                         const uint32_t* __restrict__ d_nfrag,
+                                                         const uint32_t* __restrict__ vox_off,
+                                                         uint32_t* __restrict__ vcount, uint32_t* __restrict__ s_cand,
+                                                         float4* __restrict__ s_color) {
+  const uint32_t nf = *d_nfrag;
+  for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
+    const uint32_t k = f_key[f];
+    const uint32_t pos = vox_off[k] + atomicSub(&vcount[k], 1u) - 1u;
+    s_cand[pos] = f_cand[f];
+    s_color[pos] = f_color[f];
+  }
+}
+
+__global__ __launch_bounds__(256) void vox_replay_kernel(const uint32_t* __restrict__ vox_off,
+                                                         const uint32_t* __restrict__ d_nfrag,
+                                                         uint32_t* __restrict__ s_cand, float4* __restrict__ s_color,
+                                                         uint32_t* __restrict__ level0, uint32_t nvox) {
+#pragma clang fp contract(off)
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nvox) return;
+  const uint32_t beg = vox_off[v];
+  const uint32_t end = (v + 1 < nvox) ? vox_off[v + 1] : *d_nfrag;
+  // canonical order: insertion sort of the bucket by candidate index
+  for (uint32_t i = beg + 1; i < end; ++i) {
+    const uint32_t kc = s_cand[i];
+    const float4 kv = s_color[i];
+    uint32_t j = i;
+    while (j > beg && s_cand[j - 1] > kc) {
+      s_cand[j] = s_cand[j - 1];
+      s_color[j] = s_color[j - 1];
+      --j;
+    }
+    s_cand[j] = kc;
+    s_color[j] = kv;
+  }
+  uint32_t stored = 0u;
+  for (uint32_t i = beg; i < end; ++i) {
+    const float4 c = s_color[i];
+    const float vx = c.x * 255.0f, vy = c.y * 255.0f, vz = c.z * 255.0f, vw = c.w * 255.0f;
+    if (stored == 0u) {  // imageAtomicCompSwap(0 -> new) succeeds
+      stored = enc_nibble(conv_vec4_to_rgba8(vx, vy, vz, vw), 1u);
+      continue;
+    }
+    const uint32_t cur = stored & 0xFEFEFEFEu;
+    float rx = (float)(cur & 0xffu), ry = (float)((cur >> 8) & 0xffu), rz = (float)((cur >> 16) & 0xffu),
+          rw = (float)(cur >> 24);
+    uint32_t n = dec_nibble(stored);
+    const float fn = (float)n;
+    rx = rx * fn + vx;
+    ry = ry * fn + vy;
+    rz = rz * fn + vz;
+    rw = rw * fn + vw;
+    ++n;
+    const float fn1 = (float)n;
+    rx = rx / fn1;
+    ry = ry / fn1;
+    rz = rz / fn1;
+    rw = rw / fn1;
+    rx = __builtin_rintf(rx / 2.0f) * 2.0f;
+    ry = __builtin_rintf(ry / 2.0f) * 2.0f;
+    rz = __builtin_rintf(rz / 2.0f) * 2.0f;
+    rw = __builtin_rintf(rw / 2.0f) * 2.0f;
+    stored = enc_nibble(conv_vec4_to_rgba8(rx, ry, rz, rw), n);
+  }
+  level0[v] = stored;
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+}  // namespace vct
+
+using namespace vct;
+
+// mat3(transpose(inverse(M))) of a column-major 4x4 (voxelize.vert:28).
+static void normal_matrix(const float* M, float* N) {
+  double a[3][3];
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) a[r][c] = M[c * 4 + r];
+  double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+               a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+  double inv[3][3];
+  inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det;
+  inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
+  inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det;
+  inv[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
+  inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det;
+  inv[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
+  inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det;
+  inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
+  inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
+  // transpose(inv), column-major: N[c*3 + r] = inv^T[r][c] = inv[c][r]
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) N[c * 3 + r] = (float)inv[c][r];
+}
+
+static LightSet make_lights(const vct_context* ctx) {
+  LightSet ls;
+  memset(&ls, 0, sizeof(ls));
+  int n = (int)ctx->lights.size();
+  if (n > VCT_MAX_LIGHTS) n = VCT_MAX_LIGHTS;
+  for (int i = 0; i < n; ++i) {
+    const vct_light& L = ctx->lights[i];
+    for (int k = 0; k < 3; ++k) {
+      ls.l[i].pos[k] = L.position[k];
+      ls.l[i].color[k] = L.color[k];
+    }
+    ls.l[i].intensity = L.intensity;
+    ls.l[i].kind = L.kind;
+  }
+  ls.count = n;
+  return ls;
+}
+
+int vct_launch_voxelize(vct_context* ctx) {
+  const uint32_t res = ctx->res;
+  const uint64_t nvox = (uint64_t)res * res * res;
+  const bool exact = ctx->average == VCT_AVG_EXACT;
+  void* level0 = (char*)ctx->volume.p + ctx->lvl_off[0][0];
+  int rc;
+  if ((rc = vct_ensure(ctx->counters, 256, ctx))) return rc;
+  uint32_t* counters = (uint32_t*)ctx->counters.p;  // [0] candidates total, [1] fragments
+  if (exact) {
+    if ((rc = vct_ensure(ctx->accum, nvox * 16, ctx))) return rc;
+    if ((rc = vct_ensure(ctx->vox_count, nvox * 4, ctx))) return rc;
+    if (!ctx->accum_zero) {
+      hipLaunchKernelGGL(zero_u32_kernel, dim3(2048), dim3(256), 0, ctx->stream, (uint32_t*)ctx->accum.p, nvox * 4);
+      hipLaunchKernelGGL(zero_u32_kernel, dim3(2048), dim3(256), 0, ctx->stream, (uint32_t*)ctx->vox_count.p, nvox);
+      ctx->accum_zero = true;
+    }
+  } else {
+    if ((rc = vct_ensure(ctx->vox_count, nvox * 4, ctx))) return rc;
+    if ((rc = vct_ensure(ctx->vox_off, nvox * 4, ctx))) return rc;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(2048), dim3(256), 0, ctx->stream, (uint32_t*)ctx->vox_count.p, nvox);
+    VCT_HIP(hipMemsetAsync(counters + 1, 0, 4, ctx->stream));
+  }
+  VoxParams vp;
+  memset(&vp, 0, sizeof(vp));
+  vp.res = res;
+  vp.cube_size = ctx->cube_size;
+  vp.average = ctx->average;
+  vp.lights = make_lights(ctx);
+
+  // batches of queued draws
+  std::vector<uint32_t> q;
+  for (uint32_t id : ctx->queue)
+    if (id < ctx->models.size() && ctx->models[id].ntri) q.push_back(id);
+  uint64_t cand_base = 0;
+  ctx->last_candidates = 0;
+  ctx->last_fragments = 0;
+  for (size_t qi = 0; qi < q.size(); qi += kMaxDraws) {
+    DrawBatch b;
+    memset(&b, 0, sizeof(b));
+    uint32_t ntri = 0;
+    for (size_t k = qi; k < q.size() && k < qi + kMaxDraws; ++k) {
+      const VctModel& m = ctx->models[q[k]];
+      Draw& d = b.d[b.ndraw++];
+      d.pos = m.d_pos;
+      d.nrm = m.d_nrm;
+      d.mat = m.d_mat;
+      d.tri_base = ntri;
+      d.ntri = m.ntri;
+      memcpy(d.M, m.xform, 64);
+      normal_matrix(m.xform, d.N);
+      ntri += m.ntri;
+    }
+    b.ntri = ntri;
+    if ((rc = vct_ensure(ctx->tri_count, (size_t)ntri * 4 + 4, ctx))) return rc;
+    if ((rc = vct_ensure(ctx->tri_off, (size_t)ntri * 4 + 4, ctx))) return rc;
+    hipLaunchKernelGGL(vox_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
+                       (uint32_t*)ctx->tri_count.p);
+    VCT_HIP(hipGetLastError());
+    if ((rc = vct_launch_scan_u32(ctx, (uint32_t*)ctx->tri_count.p, (uint32_t*)ctx->tri_off.p, ntri, counters)))
+      return rc;
+    RasterOut o;
+    memset(&o, 0, sizeof(o));
+    o.sums = (unsigned long long*)ctx->accum.p;
+    o.vcount = (uint32_t*)ctx->vox_count.p;
+    vp.cand_base = (uint32_t)cand_base;
+    if (!exact) {
+      // candidates bound the fragment count: read it back to size the buffers
+      uint32_t total = 0;
+      VCT_HIP(hipMemcpyAsync(&total, counters, 4, hipMemcpyDeviceToHost, ctx->stream));
+      VCT_HIP(hipStreamSynchronize(ctx->stream));
+      if (cand_base + total > 0xffffffffull) return vct_set_error(VCT_ERR_LIMIT, "more than 2^32 candidates");
+      uint32_t nf_before = 0;
+      VCT_HIP(hipMemcpyAsync(&nf_before, counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+      VCT_HIP(hipStreamSynchronize(ctx->stream));
+      size_t cap = (size_t)nf_before + total;
+      // grow keeping existing fragments
+      if (cap * 4 > ctx->frags.cap / 6 || !ctx->frags.p) {
+        VctBuf nb;
+        size_t ncap = cap + cap / 4 + 1024;
+        if ((rc = vct_ensure(nb, ncap * 24, ctx))) return rc;
+        if (ctx->frags.p && nf_before) {
+          size_t oc = ctx->frags.cap / 24;
+          VCT_HIP(hipMemcpyAsync(nb.p, ctx->frags.p, (size_t)nf_before * 4, hipMemcpyDeviceToDevice, ctx->stream));
+          VCT_HIP(hipMemcpyAsync((uint32_t*)nb.p + ncap, (uint32_t*)ctx->frags.p + oc, (size_t)nf_before * 4,
+                                 hipMemcpyDeviceToDevice, ctx->stream));
+          VCT_HIP(hipMemcpyAsync((float4*)((uint32_t*)nb.p + 2 * ncap), (float4*)((uint32_t*)ctx->frags.p + 2 * oc),
+                                 (size_t)nf_before * 16, hipMemcpyDeviceToDevice, ctx->stream));
+          VCT_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        if (ctx->frags.p) VCT_HIP(hipFree(ctx->frags.p));
+        ctx->frags = nb;
+      }
+      const size_t fc = ctx->frags.cap / 24;
+      o.f_key = (uint32_t*)ctx->frags.p;
+      o.f_cand = (uint32_t*)ctx->frags.p + fc;
+      o.f_color = (float4*)((uint32_t*)ctx->frags.p + 2 * fc);
+      o.f_counter = counters + 1;
+      o.f_cap = (uint32_t)fc;
+      cand_base += total;
+      ctx->last_candidates += total;
+    }
+    hipLaunchKernelGGL(vox_raster_kernel, dim3(256 * 8), dim3(256), 0, ctx->stream, b, vp,
+                       (const uint32_t*)ctx->tri_off.p, (const uint32_t*)counters,
+                       (const DevMaterial*)ctx->d_materials.p, o);
+    VCT_HIP(hipGetLastError());
+  }
+  const uint32_t nb = (uint32_t)((nvox + 255) / 256);
+  if (exact) {
+    switch (ctx->dtype) {
+      case VCT_RGBA8:
+        hipLaunchKernelGGL(vox_resolve_exact_kernel<VCT_RGBA8>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (unsigned long long*)ctx->accum.p, (uint32_t*)ctx->vox_count.p, level0, (uint32_t)nvox);
+        break;
+      case VCT_FP16:
+        hipLaunchKernelGGL(vox_resolve_exact_kernel<VCT_FP16>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (unsigned long long*)ctx->accum.p, (uint32_t*)ctx->vox_count.p, level0, (uint32_t)nvox);
+        break;
+      case VCT_BF16:
+        hipLaunchKernelGGL(vox_resolve_exact_kernel<VCT_BF16>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (unsigned long long*)ctx->accum.p, (uint32_t*)ctx->vox_count.p, level0, (uint32_t)nvox);
+        break;
+      default:
+        hipLaunchKernelGGL(vox_resolve_exact_kernel<VCT_FP32>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (unsigned long long*)ctx->accum.p, (uint32_t*)ctx->vox_count.p, level0, (uint32_t)nvox);
+    }
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+  }
+  // reference mode: bucket + replay
+  uint32_t nfrag = 0;
+  VCT_HIP(hipMemcpyAsync(&nfrag, counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+  VCT_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->last_fragments = nfrag;
+  if ((rc = vct_launch_scan_u32(ctx, (uint32_t*)ctx->vox_count.p, (uint32_t*)ctx->vox_off.p, (uint32_t)nvox,
+                                counters + 2)))
+    return rc;
+  if ((rc = vct_ensure(ctx->frags_sorted, (size_t)(nfrag + 1) * 20, ctx))) return rc;
+  uint32_t* s_cand = (uint32_t*)ctx->frags_sorted.p;
+  float4* s_color = (float4*)((char*)ctx->frags_sorted.p + (((size_t)(nfrag + 1) * 4 + 15) & ~(size_t)15));
+  if (((size_t)(nfrag + 1) * 4 + 15) / 16 * 16 + (size_t)(nfrag + 1) * 16 > ctx->frags_sorted.cap) {
+    if ((rc = vct_ensure(ctx->frags_sorted, ((size_t)(nfrag + 1) * 4 + 15) / 16 * 16 + (size_t)(nfrag + 1) * 16, ctx)))
+      return rc;
+    s_cand = (uint32_t*)ctx->frags_sorted.p;
+    s_color = (float4*)((char*)ctx->frags_sorted.p + (((size_t)(nfrag + 1) * 4 + 15) & ~(size_t)15));
+  }
+  const size_t fc = ctx->frags.cap / 24;
+  if (nfrag) {
+    hipLaunchKernelGGL(vox_bucket_kernel, dim3(2048), dim3(256), 0, ctx->stream, (const uint32_t*)ctx->frags.p,
+                       (const uint32_t*)ctx->frags.p + fc, (const float4*)((uint32_t*)ctx->frags.p + 2 * fc),
+                       (const uint32_t*)(counters + 1), (const uint32_t*)ctx->vox_off.p, (uint32_t*)ctx->vox_count.p,
+                       s_cand, s_color);
+  }
+  hipLaunchKernelGGL(vox_replay_kernel, dim3(nb), dim3(256), 0, ctx->stream, (const uint32_t*)ctx->vox_off.p,
+                     (const uint32_t*)(counters + 1), s_cand, s_color, (uint32_t*)level0, (uint32_t)nvox);
+  VCT_HIP(hipGetLastError());
+  return VCT_OK;
+}
