@@ -310,3 +310,75 @@ def store_round(a: np.ndarray, dtype: int) -> np.ndarray:
 def psnr(a: np.ndarray, b: np.ndarray, peak: float = 1.0) -> float:
     mse = float(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2))
     return float("inf") if mse == 0 else 10.0 * np.log10(peak * peak / mse)
+
+
+# ---- probes of single functions (pinning tests) -----------------------------------
+def _probe_lib():
+    lib = load()
+    if not getattr(lib, "_probes", False):
+        lib.orc_sample_lod_f64.argtypes = [C.c_int, C.c_int, C.POINTER(_DP), C.c_int, _DP, C.c_double, _DP]
+        lib.orc_diffuse_dirs_f64.argtypes = [_DP, C.c_int, _DP]
+        lib.orc_diffuse_dirs_f64.restype = C.c_int
+        lib.orc_trace_cone_f32.argtypes = [C.c_int, C.c_int, C.POINTER(_FP), _FP, _FP, C.c_float, C.c_float, _FP]
+        lib.orc_trace_cone_f32.restype = C.c_int64
+        lib.orc_raster_coverage.argtypes = [_FP, _FP, C.c_int, C.c_int, C.POINTER(C.c_uint8)]
+        lib.orc_dominant_axis.argtypes = [_FP]
+        lib.orc_dominant_axis.restype = C.c_int
+        lib._probes = True
+    return lib
+
+
+def _volume_ptrs(levels_list, dt):
+    """levels_list[l][d] arrays (level 0: one array reused for all d)."""
+    P = C.POINTER(C.c_double) if dt == np.float64 else C.POINTER(C.c_float)
+    keep, ptrs = [], []
+    for l, lv in enumerate(levels_list):
+        for d in range(6):
+            a = _c(lv if l == 0 else lv[d], dt)
+            keep.append(a)
+            ptrs.append(a.ctypes.data_as(P))
+    return keep, (P * len(ptrs))(*ptrs)
+
+
+def sample_lod(levels_list, map_index, p, lod):
+    lib = _probe_lib()
+    res = levels_list[0].shape[0]
+    keep, arr = _volume_ptrs(levels_list, np.float64)
+    pp = _c(p, np.float64)
+    out = np.zeros(4, np.float64)
+    lib.orc_sample_lod_f64(res, len(levels_list), arr, map_index, _d(pp), float(lod), _d(out))
+    return out
+
+
+def diffuse_dirs(n, cone_set=0):
+    lib = _probe_lib()
+    nn = _c(n, np.float64)
+    out = np.zeros((9, 3), np.float64)
+    k = lib.orc_diffuse_dirs_f64(_d(nn), cone_set, _d(out))
+    return out[:k]
+
+
+def trace_cone(levels_list, origin, direction, aperture, max_dist):
+    lib = _probe_lib()
+    res = levels_list[0].shape[0]
+    keep, arr = _volume_ptrs(levels_list, np.float32)
+    o = _c(origin, np.float32)
+    d = _c(direction, np.float32)
+    out = np.zeros(4, np.float32)
+    steps = lib.orc_trace_cone_f32(res, len(levels_list), arr, _f(o), _f(d), aperture, max_dist, _f(out))
+    return out, int(steps)
+
+
+def raster_coverage(xw, yw, W, H):
+    lib = _probe_lib()
+    x = _c(xw, np.float32)
+    y = _c(yw, np.float32)
+    m = np.zeros((H, W), np.uint8)
+    lib.orc_raster_coverage(_f(x), _f(y), W, H, m.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return m
+
+
+def dominant_axis(tri):
+    lib = _probe_lib()
+    t = _c(tri, np.float32).reshape(9)
+    return int(lib.orc_dominant_axis(_f(t)))

@@ -1217,3 +1217,85 @@ void orc_trace_f64(const orc_trace_params* p, int res, int levels, const double*
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// probes of single oracle functions (used by the pinning tests)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+// textureLod of map `map` at p with lod (FP64), volume as in orc_trace_f64.
+void orc_sample_lod_f64(int res, int levels, const double* const* level_ptrs, int map, const double* p, double lod,
+                        double* out) {
+  Volume<double> vol;
+  vol.res = res;
+  vol.levels = levels;
+  vol.lv.assign(level_ptrs, level_ptrs + levels * 6);
+  texture_lod(vol, map, mk(p[0], p[1], p[2]), lod, out);
+}
+
+// The diffuse cone directions of trace_diffuse for a normal (cone_set 0:
+// 9 directions, unnormalised as passed to trace_cone), FP64.
+int orc_diffuse_dirs_f64(const double* n, int cone_set, double* out) {
+  Volume<double> vol;
+  vol.res = 1;
+  vol.levels = 1;
+  TraceParams P;
+  memset(&P, 0, sizeof(P));
+  P.cone_set = cone_set;
+  Tracer<double> tr{vol, P, nullptr, 0};
+  const V3<double> nn = mk(n[0], n[1], n[2]);
+  const V3<double> o1 = normalize(tr.tangent(nn));
+  const V3<double> o2 = normalize(cross(o1, nn));
+  if (cone_set != 0) return 0;
+  const V3<double> c1 = scale(0.5, add(o1, o2)), c2 = scale(0.5, sub(o1, o2));
+  const V3<double> d[9] = {nn,
+                           Tracer<double>::mix(nn, o1, 0.5),
+                           Tracer<double>::mix(nn, scale(-1.0, o1), 0.5),
+                           Tracer<double>::mix(nn, o2, 0.5),
+                           Tracer<double>::mix(nn, scale(-1.0, o2), 0.5),
+                           Tracer<double>::mix(nn, c1, 0.5),
+                           Tracer<double>::mix(nn, scale(-1.0, c1), 0.5),
+                           Tracer<double>::mix(nn, c2, 0.5),
+                           Tracer<double>::mix(nn, scale(-1.0, c2), 0.5)};
+  for (int k = 0; k < 9; ++k) {
+    out[k * 3] = d[k].x;
+    out[k * 3 + 1] = d[k].y;
+    out[k * 3 + 2] = d[k].z;
+  }
+  return 9;
+}
+
+// One trace_cone in FP32 from origin along dir; returns the steps taken.
+int64_t orc_trace_cone_f32(int res, int levels, const float* const* level_ptrs, const float* origin,
+                           const float* dir, float aperture, float max_dist, float* out) {
+  Volume<float> vol;
+  vol.res = res;
+  vol.levels = levels;
+  vol.lv.assign(level_ptrs, level_ptrs + levels * 6);
+  TraceParams P;
+  memset(&P, 0, sizeof(P));
+  P.cube_res = res;
+  Tracer<float> tr{vol, P, nullptr, 0};
+  tr.trace_cone(mk(origin[0], origin[1], origin[2]), mk(dir[0], dir[1], dir[2]), aperture, max_dist, out);
+  return tr.steps;
+}
+
+// Coverage mask (H x W bytes) of one window-space triangle.
+void orc_raster_coverage(const float* xw, const float* yw, int W, int H, uint8_t* mask) {
+  Tri2D t;
+  setup2d(xw, yw, W, H, t);
+  memset(mask, 0, (size_t)W * H);
+  for_each_pixel(t, W, H, [&](int i, int j, const int64_t*) { mask[(size_t)j * W + i] += 1; });
+}
+
+// Dominant-axis plane of voxelize.geom:40-51: 2 = XY, 0 = YZ, 1 = XZ.
+int orc_dominant_axis(const float* v) {
+  const V3<float> a = mk(v[0], v[1], v[2]), b = mk(v[3], v[4], v[5]), c = mk(v[6], v[7], v[8]);
+  const V3<float> p = cross(sub(b, a), sub(c, a));
+  const float x = fabsf(p.x), y = fabsf(p.y), z = fabsf(p.z);
+  if (z > x && z > y) return 2;
+  if (x > y && x > z) return 0;
+  return 1;
+}
+
+}  // extern "C"

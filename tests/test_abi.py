@@ -1,0 +1,105 @@
+"""The C ABI library: it loads, exports what include/*.h declares, rejects bad
+input without a GPU, and its host-side scene generators are deterministic
+and follow their stated distributions.  CPU only (no compute calls)."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from voxel_cone_tracing_amd import _lib as L
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_functions():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*(vct_[a-z0-9_]+)\s*\(", text, re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol(vctlib):
+    names = declared_functions()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(vctlib, n)]
+    assert not missing, missing
+
+
+def test_python_prototypes_cover_the_header():
+    names = set(declared_functions())
+    assert names <= set(L.PROTOTYPES) | {"vct_mlp_set_weights"} | {n for n in names if n.startswith("vct_mlp")} | \
+        {n for n in names if n.startswith("vct_upsample")} | {n for n in names if n.startswith("vct_render_learned")}
+
+
+def test_material_and_light_layouts():
+    assert C.sizeof(L.Material) == 128  # material_data_t, packed (renderer.h:94-120)
+    assert L.Material.shininess.offset == 76
+    assert L.Material.illum.offset == 88
+    assert C.sizeof(L.Light) == 32
+
+
+def test_create_without_device_reports_no_device(vctlib):
+    if L.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    h = C.c_void_p()
+    rc = vctlib.vct_create(64, 64, C.byref(h))
+    assert rc == L.VCT_ERR_NO_DEVICE
+    assert not h.value
+    assert b"no HIP device" in vctlib.vct_last_error()
+
+
+def test_null_and_bad_arguments_are_rejected(vctlib):
+    assert vctlib.vct_create(0, 64, None) == L.VCT_ERR_INVALID_ARG
+    h = C.c_void_p()
+    assert vctlib.vct_create(-1, 64, C.byref(h)) == L.VCT_ERR_INVALID_ARG
+    assert vctlib.vct_sync(None) == L.VCT_ERR_BAD_HANDLE
+    assert vctlib.vct_voxelize(None) == L.VCT_ERR_BAD_HANDLE
+    assert vctlib.vct_destroy(None) == L.VCT_OK
+
+
+def test_soup_generator_is_deterministic_and_distributed():
+    pos, nrm, ar = L.soup(42, 20000)
+    pos2, _, _ = L.soup(42, 20000)
+    pos3, _, _ = L.soup(43, 20000)
+    assert np.array_equal(pos, pos2)
+    assert not np.array_equal(pos, pos3)
+    v = pos.reshape(-1, 3, 3)
+    c = v.mean(1)
+    assert np.all(np.abs(c) <= 1.0 + 1e-6)
+    assert abs(c.mean()) < 0.02 and abs(c.std() - 1 / np.sqrt(3)) < 0.01  # U[-1, 1]
+    e = np.linalg.norm(v[:, 1] - v[:, 0], axis=1)
+    e2 = np.linalg.norm(v[:, 2] - v[:, 1], axis=1)
+    assert np.allclose(e, e2, rtol=1e-4)  # equilateral
+    le = np.log(e)
+    assert abs(le.mean() + 4.0) < 0.02 and abs(le.std() - 0.5) < 0.02  # LogNormal(-4, 0.5)
+    n = nrm.reshape(-1, 3, 3)[:, 0]
+    assert np.allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-5)
+    fn = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    fn /= np.linalg.norm(fn, axis=1, keepdims=True)
+    assert np.allclose(np.abs((fn * n).sum(1)), 1.0, atol=1e-3)
+    assert abs(n.mean(0)).max() < 0.03  # uniform orientation
+    assert np.all((ar[:, :3] >= 0) & (ar[:, :3] < 1))
+    assert np.all((ar[:, 3] >= 0.05) & (ar[:, 3] < 0.5))
+
+
+def test_cornell_generator():
+    pos, nrm, tm, mats = L.cornell(1)
+    assert pos.shape == (34, 9)
+    v = pos.reshape(-1, 3, 3)
+    assert np.all(np.abs(v) <= 0.5 + 1e-6)
+    # walls face into the room: the normal points toward the centre
+    cen = v[:10].mean(1)
+    n = nrm.reshape(-1, 3, 3)[:10, 0]
+    assert np.all((n * (-cen)).sum(1) > 0)
+    assert tm[6] == 1 and tm[8] == 2  # red -x wall, green +x wall
+    assert mats[1].diffuse[0] > mats[1].diffuse[1]
+    assert mats[2].diffuse[1] > mats[2].diffuse[0]
+    pos2, _, _, mats2 = L.cornell(2)
+    assert np.array_equal(pos, pos2) and mats2[3].diffuse[0] != mats[3].diffuse[0]

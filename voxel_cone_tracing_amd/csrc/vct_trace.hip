@@ -44,6 +44,7 @@ struct TraceArgs {
   const float4* gpos;
   const float4* gnrm;
   const DevMaterial* mats;
+  uint32_t nmat;
   float4* image;
   int W, H;
   float cube_size;
@@ -198,6 +199,9 @@ __device__ float4 trace_cone(const TraceArgs& A, float3 origin, float3 dir, floa
   const float3 ad = make_float3(fabsf(dir.x), fabsf(dir.y), fabsf(dir.z));
   const float vs = 1.0f / (float)A.res;
   const float fres = (float)A.res;
+  // A cone never needs to leave the volume by more than its diagonal; the
+  // bound also keeps a non-finite target distance from looping forever.
+  max_dist = fminf(max_dist, 3.4641016f);
   float4 c = f4(0.0f);
   float dist = 3.0f * vs;
   float diam = dist * aperture;
@@ -241,7 +245,9 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
     return;
   }
   const float4 gn = A.gnrm[idx];
-  const DevMaterial& m = A.mats[__builtin_bit_cast(uint32_t, gn.w)];
+  uint32_t mid = __builtin_bit_cast(uint32_t, gn.w);
+  mid = mid < A.nmat ? mid : 0u;  // ids of an uploaded G-buffer are not trusted
+  const DevMaterial& m = A.mats[mid];
   const float3 world = make_float3(gp.x, gp.y, gp.z);
   const float3 n = make_float3(gn.x, gn.y, gn.z);
   const float3 pos = make_float3(0.5f * (world.x / A.cube_size) + 0.5f, 0.5f * (world.y / A.cube_size) + 0.5f,
@@ -417,6 +423,7 @@ int vct_trace_into(vct_context* ctx, const float4* gpos, const float4* gnrm, flo
   A.gpos = gpos;
   A.gnrm = gnrm;
   A.mats = (const DevMaterial*)ctx->d_materials.p;
+  A.nmat = (uint32_t)ctx->materials.size();
   A.image = out;
   A.W = W;
   A.H = H;

@@ -605,7 +605,7 @@ int vct_launch_voxelize(vct_context* ctx) {
       // grow keeping existing fragments
       if (cap * 4 > ctx->frags.cap / 6 || !ctx->frags.p) {
         VctBuf nb;
-        size_t ncap = cap + cap / 4 + 1024;
+        size_t ncap = (cap + cap / 4 + 1024 + 3) & ~(size_t)3;  // keeps the float4 array 16-byte aligned
         if ((rc = vct_ensure(nb, ncap * 24, ctx))) return rc;
         if (ctx->frags.p && nf_before) {
           size_t oc = ctx->frags.cap / 24;
