@@ -137,6 +137,10 @@ def main():
     log(f"bench: {ms:.3f} ms/frame; passes voxelize {passes[0]:.3f} mip {passes[1]:.3f} "
         f"gbuffer {passes[2]:.3f} trace {passes[3]:.3f} ms")
     nfrag, ncand = ctx.voxelize_stats()
+    # one more trace with a step counter (not timed): sample count of the march
+    frame()
+    steps = ctx.trace_steps()
+    samples = 3 * steps  # sample_voxel = 3 textureLod per step
 
     bpe = 8 if dt in (L.FP16, L.BF16) else (4 if dt == L.RGBA8 else 16)
     mb = mip_bytes(wl.res, wl.levels, bpe)
@@ -174,6 +178,13 @@ def main():
         },
         "passes_ms": {n: round(float(passes[i]), 4) for i, n in enumerate(names)},
         "fragments": nfrag,
+        "trace": {
+            "cone_steps": steps,
+            "samples": samples,
+            "steps_per_pixel": round(steps / (wl.width * wl.height), 2),
+            # each quadrilinear sample reads 16 texels (2 levels x 2x2x2)
+            "effective_sample_GBps": round(samples * 16 * bpe / (passes[3] * 1e-3) / 1e9, 1) if passes[3] > 0 else 0,
+        },
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:

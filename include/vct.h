@@ -206,6 +206,11 @@ int vct_image_download(vct_ctx ctx, float* rgba, size_t pixels);
 int vct_get_pass_times(vct_ctx ctx, float* ms, int n);
 /* Fragments and candidate pixels of the last voxelize. */
 int vct_get_voxelize_stats(vct_ctx ctx, uint64_t* fragments, uint64_t* candidates);
+/* Runs vct_trace once more with a counter and returns the number of cone
+ * steps taken (each step = 3 textureLod samples of sample_voxel,
+ * voxel_cone_tracing.frag:80-86): the unit of the effective sample
+ * bandwidth.  Diagnostics; not part of a frame. */
+int vct_trace_count_steps(vct_ctx ctx, uint64_t* steps);
 
 /* ---- Texture3D (texture_3d.h:6-10) --------------------------------------- */
 /* create_tex_3d (texture_3d.cpp:3-25): levels of a w x h x d texture,
@@ -245,6 +250,36 @@ int vct_scene_cornell(uint64_t seed, float* pos, float* nrm, uint32_t* tri_mater
 #define VCT_MLP_PARAMS (VCT_MLP_IN * VCT_MLP_HIDDEN + VCT_MLP_HIDDEN + \
                         VCT_MLP_HIDDEN * VCT_MLP_HIDDEN + VCT_MLP_HIDDEN + \
                         VCT_MLP_HIDDEN * VCT_MLP_OUT + VCT_MLP_OUT)
+/* Flat FP32 parameter vector: W1[64][34], b1[64], W2[64][64], b2[64],
+ * W3[3][64], b3[3].  He-uniform initialisation from the seed. */
+int vct_mlp_init(vct_ctx ctx, uint64_t seed);
+int vct_mlp_set_params(vct_ctx ctx, const float* params, uint32_t n);
+int vct_mlp_get_params(vct_ctx ctx, float* params, uint32_t n);
+/* Network output for n feature rows (n x 34 floats, rounded to BF16 on the
+ * device) -> n x 3 floats. */
+int vct_mlp_forward(vct_ctx ctx, const float* features, uint32_t n, float* out);
+/* Loss mean((albedo * (y - target))^2) over n samples and 3 channels and its
+ * gradient w.r.t. every parameter (the training kernels, no update). */
+int vct_mlp_loss_grad(vct_ctx ctx, const float* features, const float* albedo, const float* target, uint32_t n,
+                      float* grad, float* loss);
+/* Training data from the current frame (after vct_voxelize, vct_mip_aniso
+ * and vct_gbuffer): traces the full-resolution albedo-demodulated target and
+ * the quarter-resolution input, then appends nsamples random pixels
+ * (features, albedo, target) to the device-resident training set. */
+int vct_upsample_collect(vct_ctx ctx, uint32_t nsamples, uint64_t seed);
+int vct_upsample_clear_samples(vct_ctx ctx);
+/* Replaces the training set with n host samples (features n x 34, albedo and
+ * target n x 3). */
+int vct_upsample_set_samples(vct_ctx ctx, const float* features, const float* albedo, const float* target,
+                             uint32_t n);
+int vct_upsample_sample_count(vct_ctx ctx, uint64_t* n);
+/* Adam steps (beta 0.9 / 0.999) on random windows of `batch` samples. */
+int vct_upsample_train(vct_ctx ctx, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float* last_loss);
+/* The learned form of Renderer::render: voxelize, mip, full-resolution
+ * G-buffer, quarter-resolution cone trace, MLP upsampling; pass times
+ * [voxelize, mip, gbuffer, low-res trace, upsample]. */
+int vct_render_learned(vct_ctx ctx);
+int vct_upsample_lowres_download(vct_ctx ctx, float* rgba, size_t pixels);
 
 #ifdef __cplusplus
 }

@@ -378,6 +378,29 @@ def raster_coverage(xw, yw, W, H):
     return m
 
 
+def mlp_forward(params, x, bf16=False):
+    lib = load()
+    lib.orc_mlp_forward.argtypes = [_FP, _FP, C.c_int, C.c_int, _DP]
+    p = _c(params, np.float32)
+    xx = _c(x, np.float32)
+    y = np.zeros((xx.shape[0], 3), np.float64)
+    lib.orc_mlp_forward(_f(p), _f(xx), xx.shape[0], int(bf16), _d(y))
+    return y
+
+
+def mlp_loss_grad(params, x, albedo, target, bf16=False):
+    lib = load()
+    lib.orc_mlp_loss_grad.argtypes = [_FP, _FP, _FP, _FP, C.c_int, C.c_int, _DP]
+    lib.orc_mlp_loss_grad.restype = C.c_double
+    p = _c(params, np.float32)
+    xx = _c(x, np.float32)
+    a = _c(albedo, np.float32)
+    t = _c(target, np.float32)
+    g = np.zeros(p.size, np.float64)
+    loss = lib.orc_mlp_loss_grad(_f(p), _f(xx), _f(a), _f(t), xx.shape[0], int(bf16), _d(g))
+    return float(loss), g
+
+
 def dominant_axis(tri):
     lib = _probe_lib()
     t = _c(tri, np.float32).reshape(9)

@@ -1,0 +1,134 @@
+"""The oracle against the reference's own shaders run under Mesa llvmpipe (CPU).
+
+Skipped where the reference tree or Mesa's software driver is absent (the GPU
+box): there the committed fixtures of tests/golden/ carry these outputs.
+llvmpipe runs single-threaded (LP_NUM_THREADS=0) so its fragment order, and
+hence the reference's order-dependent running average, is reproducible.
+"""
+from __future__ import annotations
+
+import os
+
+os.environ.setdefault("LP_NUM_THREADS", "0")
+
+import numpy as np  # noqa: E402
+import pytest  # noqa: E402
+
+import oracle  # noqa: E402
+from oracle import glref  # noqa: E402
+
+from helpers import SceneSpec  # noqa: E402
+
+pytestmark = pytest.mark.skipif(not glref.available(), reason="reference shaders / Mesa llvmpipe not present")
+
+
+def _bytes(a):
+    return np.ascontiguousarray(a).view(np.uint8).reshape(-1, 4).astype(int)
+
+
+@pytest.fixture(scope="module")
+def cornell():
+    sc = SceneSpec("cornell", res=64, W=128, H=96)
+    r = glref.frame(64, sc.cube_size, sc.M, sc.view, sc.proj, sc.W, sc.H, sc.pos, sc.nrm, sc.tri_mat,
+                    sc.mat_records, sc.lights)
+    return sc, r
+
+
+@pytest.fixture(scope="module")
+def soup():
+    # point light: the reference shaders have no directional light
+    sc = SceneSpec("soup", res=64, ntri=1500, W=96, H=64)
+    sc.lights = [((0.3, 1.2, 0.5), (1.0, 1.0, 1.0), 2.0, 0)]
+    r = glref.frame(64, sc.cube_size, sc.M, sc.view, sc.proj, sc.W, sc.H, sc.pos, sc.nrm, sc.tri_mat,
+                    sc.mat_records, sc.lights)
+    return sc, r
+
+
+@pytest.mark.parametrize("which", ["cornell", "soup"])
+def test_voxel_occupancy_and_coordinates_bit_exact(which, request):
+    sc, r = request.getfixturevalue(which)
+    v = oracle.voxelize(sc.res, sc.cube_size, sc.M, sc.pos, sc.nrm, sc.tri_mat, sc.mat_records, sc.lights,
+                        want=("ref", "count"))
+    occ_ref, occ_orc = r["level0"] != 0, v["ref"] != 0
+    assert occ_orc.sum() > 100
+    assert np.array_equal(occ_ref, occ_orc)
+
+
+@pytest.mark.parametrize("which", ["cornell", "soup"])
+def test_voxel_radiance_running_average(which, request):
+    sc, r = request.getfixturevalue(which)
+    v = oracle.voxelize(sc.res, sc.cube_size, sc.M, sc.pos, sc.nrm, sc.tri_mat, sc.mat_records, sc.lights,
+                        want=("ref", "count"))
+    occ = v["ref"] != 0
+    same = (r["level0"] == v["ref"])[occ]
+    d = np.abs(_bytes(r["level0"][occ]) - _bytes(v["ref"][occ]))
+    # voxels reached by a single triangle are bit-identical; voxels shared by
+    # several triangles differ only where llvmpipe's tile order differs from
+    # the canonical triangle order
+    single = (v["count"][occ] <= 2)
+    assert same[single].all()
+    assert same.mean() > 0.99
+    assert d.max() <= 2
+
+
+def test_glsl_round_ties_to_even_matches_llvmpipe(cornell):
+    sc, r = cornell
+    occ = r["level0"] != 0
+    even = oracle.voxelize(64, sc.cube_size, sc.M, sc.pos, sc.nrm, sc.tri_mat, sc.mat_records, sc.lights,
+                           want=("ref",))["ref"]
+    oracle.set_round_mode(True)
+    try:
+        away = oracle.voxelize(64, sc.cube_size, sc.M, sc.pos, sc.nrm, sc.tri_mat, sc.mat_records, sc.lights,
+                               want=("ref",))["ref"]
+    finally:
+        oracle.set_round_mode(False)
+    assert (r["level0"] == even)[occ].mean() > (r["level0"] == away)[occ].mean()
+
+
+@pytest.mark.parametrize("which", ["cornell", "soup"])
+def test_mip_within_one_lsb(which, request):
+    sc, r = request.getfixturevalue(which)
+    m = oracle.mip_f32(sc.res, 7, 0, oracle.decode_rgba8(r["level0"]))
+    worst, n_diff, n = 0, 0, 0
+    for (l, d), got in r["mips"].items():
+        want = oracle.encode_rgba8(m[(l, d)])
+        e = np.abs(_bytes(got) - _bytes(want))
+        worst = max(worst, int(e.max()))
+        n_diff += int((e > 0).any(1).sum())
+        n += e.shape[0]
+    assert worst <= 1
+    assert n_diff < 0.02 * n
+
+
+@pytest.mark.parametrize("which", ["cornell", "soup"])
+def test_gbuffer_agrees(which, request):
+    sc, r = request.getfixturevalue(which)
+    p32, n32, _, _, _ = oracle.gbuffer(sc.W, sc.H, sc.view, sc.proj, [(sc.M, sc.pos, sc.nrm, sc.tri_mat)])
+    cov_ref, cov_orc = r["gpos"][..., 3] != 0, p32[..., 3] != 0
+    assert (cov_ref == cov_orc).mean() > 0.995
+    both = cov_ref & cov_orc
+    mid_ref = r["gnrm"][..., 3].copy().view(np.uint32)
+    mid_orc = n32[..., 3].copy().view(np.uint32)
+    assert (mid_ref[both] == mid_orc[both]).mean() > 0.99
+    same = both & (mid_ref == mid_orc)
+    assert np.abs(r["gpos"][same][:, :3] - p32[same][:, :3]).max() < 2e-3
+    assert np.abs(r["gnrm"][same][:, :3] - n32[same][:, :3]).max() < 2e-3
+
+
+@pytest.mark.parametrize("which", ["cornell", "soup"])
+def test_trace_image_matches_reference(which, request):
+    """voxel_cone_tracing.frag under llvmpipe vs the oracle's FP32 trace, both
+    fed with llvmpipe's own volume and G-buffer (indirect light only)."""
+    sc, r = request.getfixturevalue(which)
+    l0 = oracle.decode_rgba8(r["level0"])
+    mips = {k: oracle.decode_rgba8(v) for k, v in r["mips"].items()}
+    cov = r["gpos"][..., 3] != 0
+    pix = np.flatnonzero(cov.reshape(-1)).astype(np.int64)
+    out, _ = oracle.trace(np.float32, sc.res, 7, l0, mips, r["gpos"], r["gnrm"], sc.mat_records, sc.lights, sc.cam,
+                          sc.cube_size, cone_set=0, pixels=pix)
+    ref = r["image"].reshape(-1, 4)[pix]
+    err = np.abs(out[:, :3] - ref[:, :3])
+    psnr = oracle.psnr(out[:, :3], ref[:, :3])
+    print(f"{which}: trace vs llvmpipe max-abs {err.max():.3e} PSNR {psnr:.1f} dB")
+    assert psnr > 45.0
+    assert np.quantile(err, 0.999) < 2e-2

@@ -1,0 +1,183 @@
+"""GPU parity of the learned upsampler's matrix-core kernels (through the C ABI).
+
+Forward outputs and the loss gradient of every parameter are compared with
+the oracle in its BF16 mode (the same BF16 rounding points, FP64
+accumulation): the remaining difference is FP32 vs FP64 accumulation and the
+odd BF16 tie.  Training must fit a smooth target, and on a rendered scene the
+trained network must beat the same quarter-resolution buffer upsampled
+bilinearly (the method without the network).
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from voxel_cone_tracing_amd import _lib as L
+from voxel_cone_tracing_amd.context import Context
+from voxel_cone_tracing_amd.scene import look_at, perspective
+
+from helpers import SceneSpec
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _params(seed=0, scale=0.2):
+    rng = np.random.default_rng(seed)
+    p = (rng.standard_normal(L.MLP_PARAMS) * scale).astype(np.float32)
+    p[34 * 64:34 * 64 + 64] = np.abs(p[34 * 64:34 * 64 + 64])
+    return p
+
+
+@pytest.mark.parametrize("n", [1, 100, 4096])
+def test_forward_matches_oracle(gpu_available, orc, n):
+    say(f"mlp forward n={n}")
+    rng = np.random.default_rng(n)
+    x = rng.random((n, 34)).astype(np.float32)
+    p = _params(1)
+    with Context(16, 16) as ctx:
+        ctx.mlp_set_params(p)
+        y = ctx.mlp_forward(x)
+    want = orc.mlp_forward(p, x, bf16=True)
+    err = np.abs(y - want)
+    say(f"  max-abs {err.max():.2e}, max |y| {np.abs(want).max():.2f}")
+    assert np.all(err <= 2e-2 + 2e-2 * np.abs(want))
+    assert np.abs(y - want).mean() < 2e-3 * max(1.0, np.abs(want).mean())
+
+
+@pytest.mark.parametrize("n", [64, 1000, 8192])
+def test_gradient_matches_oracle(gpu_available, orc, n):
+    say(f"mlp gradient n={n}")
+    rng = np.random.default_rng(100 + n)
+    x = rng.random((n, 34)).astype(np.float32)
+    a = rng.random((n, 3)).astype(np.float32)
+    t = rng.random((n, 3)).astype(np.float32)
+    p = _params(2)
+    with Context(16, 16) as ctx:
+        ctx.mlp_set_params(p)
+        loss, g = ctx.mlp_loss_grad(x, a, t)
+        loss2, g2 = ctx.mlp_loss_grad(x, a, t)
+    assert loss == loss2 and np.array_equal(g, g2), "gradient reduction must be deterministic"
+    wl, wg = orc.mlp_loss_grad(p, x, a, t, bf16=True)
+    rel = np.linalg.norm(g - wg) / np.linalg.norm(wg)
+    say(f"  loss {loss:.6f} vs {wl:.6f}; gradient relative error {rel:.2e}")
+    assert abs(loss - wl) <= 1e-3 * wl
+    assert rel < 2e-2
+    # every block of the parameter vector, not only the largest ones
+    for lo, hi in [(0, 2176), (2176, 2240), (2240, 6336), (6336, 6400), (6400, 6592), (6592, 6595)]:
+        r = np.linalg.norm(g[lo:hi] - wg[lo:hi]) / max(np.linalg.norm(wg[lo:hi]), 1e-12)
+        assert r < 5e-2, (lo, hi, r)
+
+
+def test_training_fits_a_smooth_target(gpu_available):
+    rng = np.random.default_rng(7)
+    n = 16384
+    x = rng.random((n, 34)).astype(np.float32)
+    t = np.stack([np.sin(3 * x[:, 0]) * x[:, 1], x[:, 2] * x[:, 3] + 0.5 * x[:, 4], np.cos(2 * x[:, 5])],
+                 1).astype(np.float32)
+    a = np.ones((n, 3), np.float32)
+    with Context(16, 16) as ctx:
+        ctx.mlp_init(3)
+        l0, _ = ctx.mlp_loss_grad(x, a, t)
+        ctx.upsample_set_samples(x, a, t)
+        ctx.upsample_train(400, 4096, 3e-3, seed=1)  # device Adam
+        l1, _ = ctx.mlp_loss_grad(x, a, t)
+    say(f"loss {l0:.4f} -> {l1:.5f}")
+    assert l1 < 0.1 * l0
+
+
+def _soup_ctx(W, H, ntri=30000, res=64):
+    sc = SceneSpec("soup", res=res, ntri=ntri, W=W, H=H)
+    ctx = Context(W, H)
+    sc.build(ctx, L.FP16)
+    return sc, ctx
+
+
+def _poses(k, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(k):
+        d = rng.standard_normal(3)
+        d /= np.linalg.norm(d)
+        eye = d * rng.uniform(2.6, 3.6)
+        out.append(look_at(eye, rng.uniform(-0.2, 0.2, 3), (0, 1, 0)))
+    return out
+
+
+def _plain(ctx, sc):
+    sc_q(ctx, sc)
+    ctx.render()
+    return ctx.download_image()
+
+
+def sc_q(ctx, sc):
+    ctx.clear_queues()
+    ctx.queue_model(sc.model)
+    for (p, c, i, k) in sc.lights:
+        ctx.queue_light(p, c, i, k)
+
+
+def _bilinear(low, gn_alb, W, H):
+    """The same quarter-resolution buffer upsampled without the network:
+    bilinear in low-res coordinates, times albedo."""
+    wl, hl = low.shape[1], low.shape[0]
+    px = np.arange(W)
+    py = np.arange(H)
+    u = np.clip((px - 1) * 0.25, 0, wl - 1)
+    v = np.clip((py - 1) * 0.25, 0, hl - 1)
+    u0 = np.floor(u).astype(int)
+    v0 = np.floor(v).astype(int)
+    u1 = np.minimum(u0 + 1, wl - 1)
+    v1 = np.minimum(v0 + 1, hl - 1)
+    au = (u - u0)[None, :, None]
+    av = (v - v0)[:, None, None]
+    rgb = low[..., :3]
+    top = rgb[v0][:, u0] * (1 - au) + rgb[v0][:, u1] * au
+    bot = rgb[v1][:, u0] * (1 - au) + rgb[v1][:, u1] * au
+    return (top * (1 - av) + bot * av) * gn_alb
+
+
+def test_learned_upsampler_beats_bilinear(gpu_available):
+    W, H = 256, 192
+    sc, ctx = _soup_ctx(W, H)
+    try:
+        ctx.mlp_init(11)
+        # training frames
+        for i, view in enumerate(_poses(24, 5)):
+            ctx.set_camera(view, sc.proj)
+            sc_q(ctx, sc)
+            ctx.voxelize()
+            ctx.mip()
+            ctx.gbuffer()
+            ctx.upsample_collect(8192, 1000 + i)
+        say(f"samples {ctx.upsample_count()}")
+        loss = ctx.upsample_train(600, 8192, 2e-3, seed=3)
+        say(f"final training loss {loss:.3e}")
+        # held-out pose
+        view = _poses(1, 99)[0]
+        ctx.set_camera(view, sc.proj)
+        plain = _plain(ctx, sc)
+        sc_q(ctx, sc)
+        ctx.render_learned()
+        learned = ctx.download_image()
+        low = ctx.download_lowres()
+        gp, gn = ctx.download_gbuffer()
+    finally:
+        ctx.close()
+    cov = gp[..., 3] != 0
+    mats = sc.mat_records
+    mid = gn[..., 3].copy().view(np.uint32)
+    alb = mats[np.where(cov, mid, 0)][..., 4:7]
+    bil = _bilinear(low, alb, W, H)
+    e_l = learned[..., :3][cov] - plain[..., :3][cov]
+    e_b = bil[cov] - plain[..., :3][cov]
+    psnr_l = 10 * np.log10(1.0 / np.mean(e_l ** 2))
+    psnr_b = 10 * np.log10(1.0 / np.mean(e_b ** 2))
+    say(f"PSNR vs plain full-res: learned {psnr_l:.2f} dB, bilinear {psnr_b:.2f} dB")
+    assert np.isfinite(learned).all()
+    assert psnr_l > psnr_b + 1.0

@@ -128,6 +128,7 @@ PROTOTYPES = {
     "vct_image_download": (_I, [_P, _FP, _SZ]),
     "vct_get_pass_times": (_I, [_P, _FP, _I]),
     "vct_get_voxelize_stats": (_I, [_P, _U64P, _U64P]),
+    "vct_trace_count_steps": (_I, [_P, _U64P]),
     "vct_tex3d_create": (_I, [_P, _I, _I, _I, _I, _I, _UP]),
     "vct_tex3d_destroy": (_I, [_P, _U]),
     "vct_tex3d_activate": (_I, [_P, _U, _U]),
@@ -135,6 +136,18 @@ PROTOTYPES = {
     "vct_tex3d_mip": (_I, [_P, _U]),
     "vct_tex3d_upload": (_I, [_P, _U, _I, _P, _SZ]),
     "vct_tex3d_download": (_I, [_P, _U, _I, _P, _SZ]),
+    "vct_mlp_init": (_I, [_P, C.c_uint64]),
+    "vct_mlp_set_params": (_I, [_P, _FP, _U]),
+    "vct_mlp_get_params": (_I, [_P, _FP, _U]),
+    "vct_mlp_forward": (_I, [_P, _FP, _U, _FP]),
+    "vct_mlp_loss_grad": (_I, [_P, _FP, _FP, _FP, _U, _FP, _FP]),
+    "vct_upsample_collect": (_I, [_P, _U, C.c_uint64]),
+    "vct_upsample_clear_samples": (_I, [_P]),
+    "vct_upsample_set_samples": (_I, [_P, _FP, _FP, _FP, _U]),
+    "vct_upsample_sample_count": (_I, [_P, _U64P]),
+    "vct_upsample_train": (_I, [_P, _U, _U, _F, C.c_uint64, _FP]),
+    "vct_render_learned": (_I, [_P]),
+    "vct_upsample_lowres_download": (_I, [_P, _FP, _SZ]),
     "vct_scene_soup": (_I, [C.c_uint64, _U, _F, _F, _F, _F, _F, _FP, _FP, _FP]),
     "vct_scene_cornell": (_I, [C.c_uint64, _FP, _FP, _UP, C.POINTER(Material)]),
 }

@@ -173,6 +173,71 @@ class Context:
         L.check(self.lib.vct_get_pass_times(self.h, L.fptr(t), 5))
         return t
 
+    # -- learned upsampler --------------------------------------------------------
+    def mlp_init(self, seed: int = 1) -> None:
+        L.check(self.lib.vct_mlp_init(self.h, seed))
+
+    def mlp_params(self) -> np.ndarray:
+        p = np.zeros(L.MLP_PARAMS, np.float32)
+        L.check(self.lib.vct_mlp_get_params(self.h, L.fptr(p), p.size))
+        return p
+
+    def mlp_set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        L.check(self.lib.vct_mlp_set_params(self.h, L.fptr(p), p.size))
+
+    def mlp_forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        n = x.shape[0]
+        y = np.zeros((n, 3), np.float32)
+        L.check(self.lib.vct_mlp_forward(self.h, L.fptr(x), n, L.fptr(y)))
+        return y
+
+    def mlp_loss_grad(self, x, albedo, target):
+        x = np.ascontiguousarray(x, np.float32)
+        a = np.ascontiguousarray(albedo, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        g = np.zeros(L.MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        L.check(self.lib.vct_mlp_loss_grad(self.h, L.fptr(x), L.fptr(a), L.fptr(t), x.shape[0], L.fptr(g),
+                                           C.byref(loss)))
+        return float(loss.value), g
+
+    def upsample_collect(self, n: int, seed: int) -> None:
+        L.check(self.lib.vct_upsample_collect(self.h, n, seed))
+
+    def upsample_set_samples(self, x, albedo, target) -> None:
+        x = np.ascontiguousarray(x, np.float32)
+        a = np.ascontiguousarray(albedo, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        L.check(self.lib.vct_upsample_set_samples(self.h, L.fptr(x), L.fptr(a), L.fptr(t), x.shape[0]))
+
+    def upsample_clear(self) -> None:
+        L.check(self.lib.vct_upsample_clear_samples(self.h))
+
+    def upsample_count(self) -> int:
+        n = C.c_uint64()
+        L.check(self.lib.vct_upsample_sample_count(self.h, C.byref(n)))
+        return n.value
+
+    def upsample_train(self, steps: int, batch: int, lr: float, seed: int = 0) -> float:
+        loss = C.c_float()
+        L.check(self.lib.vct_upsample_train(self.h, steps, batch, lr, seed, C.byref(loss)))
+        return float(loss.value)
+
+    def render_learned(self) -> None:
+        L.check(self.lib.vct_render_learned(self.h))
+
+    def download_lowres(self) -> np.ndarray:
+        img = np.zeros((self.height // 4, self.width // 4, 4), np.float32)
+        L.check(self.lib.vct_upsample_lowres_download(self.h, L.fptr(img), img.shape[0] * img.shape[1]))
+        return img
+
+    def trace_steps(self) -> int:
+        n = C.c_uint64()
+        L.check(self.lib.vct_trace_count_steps(self.h, C.byref(n)))
+        return n.value
+
     def voxelize_stats(self):
         f, c = C.c_uint64(), C.c_uint64()
         L.check(self.lib.vct_get_voxelize_stats(self.h, C.byref(f), C.byref(c)))

@@ -25,6 +25,8 @@ struct GDraw {
   const float* pos;
   const float* nrm;
   const uint32_t* mat;
+  const uint32_t* perm;  // stored -> original triangle index
+  const uint32_t* inv;   // original -> stored
   uint32_t tri_base;
   uint32_t ntri;
   float M[16];
@@ -145,22 +147,48 @@ __device__ __forceinline__ bool g_cover(const GSetup& s, int i, int j, float& b0
   return true;
 }
 
-__global__ __launch_bounds__(256) void g_count_kernel(GParams p, uint32_t* counts) {
+// Per-triangle record of the count pass, read by the triangle's candidate lanes.
+struct alignas(16) GRec {
+  int64_t area;
+  int32_t X[3], Y[3];
+  float z[3];  // window depth of each vertex
+  int32_t qx0, qy0, qw;
+  uint32_t swap;
+  uint32_t otri;  // original triangle index in the frame (GL draw order)
+};
+static_assert(sizeof(GRec) == 64, "GRec must be 64 bytes");
+
+__global__ __launch_bounds__(256) void g_count_kernel(GParams p, uint32_t* counts, GRec* __restrict__ recs) {
+#pragma clang fp contract(off)
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= p.ntri) return;
   const int d = g_find_draw(p, t);
   GSetup s;
   g_setup(p.draws[d], p.PV, t - p.draws[d].tri_base, p.W, p.H, s);
   counts[t] = s.ok ? (uint32_t)(4 * s.qw * s.qh) : 0u;
+  if (!s.ok) return;
+  GRec r;
+  r.area = s.area;
+  for (int v = 0; v < 3; ++v) {
+    r.X[v] = (int32_t)s.X[v];
+    r.Y[v] = (int32_t)s.Y[v];
+    r.z[v] = (s.clip[v][2] / s.clip[v][3]) * 0.5f + 0.5f;
+  }
+  r.qx0 = s.qx0;
+  r.qy0 = s.qy0;
+  r.qw = s.qw;
+  r.swap = s.swap ? 1u : 0u;
+  r.otri = p.draws[d].tri_base + p.draws[d].perm[t - p.draws[d].tri_base];
+  recs[t] = r;
 }
 
-__global__ __launch_bounds__(256) void g_raster_kernel(GParams p, const uint32_t* __restrict__ off,
+__global__ __launch_bounds__(256) void g_raster_kernel(uint32_t ntri, int W, int H, const uint32_t* __restrict__ off,
                                                        const uint32_t* __restrict__ d_total,
-                                                       unsigned long long* __restrict__ vis) {
+                                                       const GRec* __restrict__ recs, unsigned long long* vis) {
 #pragma clang fp contract(off)
   __shared__ uint32_t s_range[2];
   const uint32_t total = *d_total;
-  const uint32_t n = p.ntri;
+  const uint32_t n = ntri;
   for (uint32_t base = blockIdx.x * 256u; base < total; base += gridDim.x * 256u) {
     const uint32_t last = min(base + 255u, total - 1u);
     if (threadIdx.x == 0 || threadIdx.x == 255) {
@@ -183,23 +211,28 @@ __global__ __launch_bounds__(256) void g_raster_kernel(GParams p, const uint32_t
         else hi = mid;
       }
       const uint32_t t = lo;
-      const int d = g_find_draw(p, t);
-      const GDraw& dr = p.draws[d];
+      const GRec r = recs[t];
       GSetup s;
-      g_setup(dr, p.PV, t - dr.tri_base, p.W, p.H, s);
+      s.area = r.area;
+      s.swap = r.swap != 0u;
+      for (int v = 0; v < 3; ++v) {
+        s.X[v] = r.X[v];
+        s.Y[v] = r.Y[v];
+      }
       const uint32_t lc = c - off[t];
       const uint32_t quad = lc >> 2, sub = lc & 3u;
-      const int i = s.qx0 + 2 * (int)(quad % (uint32_t)s.qw) + (int)(sub & 1u);
-      const int j = s.qy0 + 2 * (int)(quad / (uint32_t)s.qw) + (int)(sub >> 1);
+      const int i = r.qx0 + 2 * (int)(quad % (uint32_t)r.qw) + (int)(sub & 1u);
+      const int j = r.qy0 + 2 * (int)(quad / (uint32_t)r.qw) + (int)(sub >> 1);
       float b0, b1, b2;
-      if (i < p.W && j < p.H && g_cover(s, i, j, b0, b1, b2)) {
-        float z[3];
-        for (int v = 0; v < 3; ++v) z[v] = (s.clip[v][2] / s.clip[v][3]) * 0.5f + 0.5f;
-        const float depth = b0 * z[0] + b1 * z[1] + b2 * z[2];
+      if (i < W && j < H && g_cover(s, i, j, b0, b1, b2)) {
+        const float depth = b0 * r.z[0] + b1 * r.z[1] + b2 * r.z[2];
         if (depth >= 0.0f && depth <= 1.0f) {
           const unsigned long long key =
-              ((unsigned long long)__builtin_bit_cast(uint32_t, depth) << 32) | (unsigned long long)t;
-          atomicMin(&vis[(size_t)j * p.W + i], key);
+              ((unsigned long long)__builtin_bit_cast(uint32_t, depth) << 32) | (unsigned long long)r.otri;
+          unsigned long long* dst = &vis[(size_t)j * W + i];
+          // early depth test: skip the atomic when a nearer surface is already in
+          // (a stale read only costs a redundant atomic, never a wrong result)
+          if (key < __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(dst, key);
         }
       }
     }
@@ -221,10 +254,10 @@ __global__ __launch_bounds__(256) void g_resolve_kernel(GParams p, unsigned long
     gnrm[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
     return;
   }
-  const uint32_t t = (uint32_t)(key & 0xffffffffu);
+  const uint32_t t = (uint32_t)(key & 0xffffffffu);  // original index
   const int d = g_find_draw(p, t);
   const GDraw& dr = p.draws[d];
-  const uint32_t lt = t - dr.tri_base;
+  const uint32_t lt = dr.inv[t - dr.tri_base];       // stored index
   GSetup s;
   g_setup(dr, p.PV, lt, p.W, p.H, s);
   const int i = (int)(idx % (uint32_t)p.W), j = (int)(idx / (uint32_t)p.W);
@@ -332,6 +365,8 @@ int vct_launch_gbuffer(vct_context* ctx) {
     d.pos = m.d_pos;
     d.nrm = m.d_nrm;
     d.mat = m.d_mat;
+    d.perm = m.d_perm;
+    d.inv = m.d_inv;
     d.tri_base = ntri;
     d.ntri = m.ntri;
     memcpy(d.M, m.xform, 64);
@@ -344,13 +379,15 @@ int vct_launch_gbuffer(vct_context* ctx) {
   if (ntri) {
     if ((rc = vct_ensure(ctx->g_count, (size_t)ntri * 4 + 4, ctx))) return rc;
     if ((rc = vct_ensure(ctx->g_off, (size_t)ntri * 4 + 4, ctx))) return rc;
+    if ((rc = vct_ensure(ctx->g_recs, (size_t)ntri * sizeof(GRec), ctx))) return rc;
     hipLaunchKernelGGL(g_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, p,
-                       (uint32_t*)ctx->g_count.p);
+                       (uint32_t*)ctx->g_count.p, (GRec*)ctx->g_recs.p);
     VCT_HIP(hipGetLastError());
     if ((rc = vct_launch_scan_u32(ctx, (uint32_t*)ctx->g_count.p, (uint32_t*)ctx->g_off.p, ntri, counters + 4)))
       return rc;
-    hipLaunchKernelGGL(g_raster_kernel, dim3(256 * 8), dim3(256), 0, ctx->stream, p, (const uint32_t*)ctx->g_off.p,
-                       (const uint32_t*)(counters + 4), (unsigned long long*)ctx->vis.p);
+    hipLaunchKernelGGL(g_raster_kernel, dim3(256 * 8), dim3(256), 0, ctx->stream, ntri, W, H,
+                       (const uint32_t*)ctx->g_off.p, (const uint32_t*)(counters + 4), (const GRec*)ctx->g_recs.p,
+                       (unsigned long long*)ctx->vis.p);
     VCT_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(g_resolve_kernel, dim3((uint32_t)((npix + 255) / 256)), dim3(256), 0, ctx->stream, p,

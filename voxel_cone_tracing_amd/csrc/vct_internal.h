@@ -4,11 +4,11 @@
 //   level 0 (res^3) is shared by the six direction maps: the reference writes
 //   the same value into all six level-0 images (voxelize.frag:159-160), so one
 //   copy is kept.  Levels 1..L-1 exist once per direction (mipmap.comp:4-5).
-//   Each level of size s is stored in 4x4x4-texel bricks; bricks are ordered
-//   along a 3D Morton (Z-order) curve and texels inside a brick are x-fastest.
-//   Levels smaller than 4^3 occupy one padded brick.  A 2x2x2 trilinear
-//   footprint therefore touches one or two 128-byte lines in the common case,
-//   and any aligned 16^3 block is 64 consecutive bricks (one contiguous range).
+//   Each level is stored along a 3D Morton (Z-order) curve: every aligned
+//   2^k cube of texels is one contiguous run (4x4x4 bricks are 64 elements,
+//   256 B to 1 KiB), so a 2x2x2 trilinear footprint usually touches one or two
+//   128-byte lines, and the eight texels under a mip texel m are elements
+//   [8m, 8m + 8): the mip builder streams its input and output contiguously.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -41,20 +41,17 @@ VCT_HD uint32_t compact3(uint32_t v) {
 VCT_HD uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
   return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
 }
-// Element index of texel (x,y,z) in a level of size s (power of two).
-VCT_HD uint32_t brick_index(uint32_t x, uint32_t y, uint32_t z) {
-  return (morton3(x >> 2, y >> 2, z >> 2) << 6) | ((z & 3u) << 4) | ((y & 3u) << 2) | (x & 3u);
-}
+// Element index of texel (x,y,z) in a level: the 3D Morton code.  Every
+// aligned 2^k cube is a contiguous run; in particular the 2x2x2 block under a
+// mip texel m is elements [8m, 8m + 8) and any 4x4x4 brick is 64 elements.
+VCT_HD uint32_t brick_index(uint32_t x, uint32_t y, uint32_t z) { return morton3(x, y, z); }
 // Inverse of brick_index.
 VCT_HD void brick_coords(uint32_t idx, uint32_t& x, uint32_t& y, uint32_t& z) {
-  uint32_t b = idx >> 6, t = idx & 63u;
-  x = (compact3(b) << 2) | (t & 3u);
-  y = (compact3(b >> 1) << 2) | ((t >> 2) & 3u);
-  z = (compact3(b >> 2) << 2) | (t >> 4);
+  x = compact3(idx);
+  y = compact3(idx >> 1);
+  z = compact3(idx >> 2);
 }
-VCT_HD uint64_t level_elems(uint32_t s) {
-  return s >= 4 ? (uint64_t)s * s * s : 64ull;
-}
+VCT_HD uint64_t level_elems(uint32_t s) { return (uint64_t)s * s * s; }
 
 // ---- storage codecs -------------------------------------------------------
 VCT_HD uint32_t dtype_bytes(int dt) {
@@ -123,11 +120,19 @@ struct LightSet {
 }  // namespace vct
 
 // ---- host side context --------------------------------------------------
+// A model's triangles are stored on the device in Morton order of their
+// centroids (spatially coherent rasterisation and voxel scatter); d_perm maps
+// a stored position to the triangle's original index and d_inv back.  Every
+// order-dependent rule (GL draw order for depth ties, the canonical order of
+// the running average) uses the original index.
 struct VctModel {
   uint32_t ntri = 0;
-  float* d_pos = nullptr;     // 9 floats per triangle
-  float* d_nrm = nullptr;     // 9 floats per triangle
-  uint32_t* d_mat = nullptr;  // material id per triangle
+  float* d_pos = nullptr;     // 9 floats per triangle (stored order)
+  float* d_nrm = nullptr;     // 9 floats per triangle (stored order)
+  uint32_t* d_mat = nullptr;  // material id per triangle (stored order)
+  uint32_t* d_perm = nullptr;  // stored -> original index
+  uint32_t* d_inv = nullptr;   // original -> stored index
+  std::vector<uint32_t> perm;
   std::vector<uint32_t> h_mat;
   float xform[16];
   float dims[3];
@@ -186,10 +191,18 @@ struct vct_context {
 
   // G-buffer + image
   VctBuf vis, g_pos, g_nrm, image;
-  VctBuf g_count, g_off;
+  VctBuf g_count, g_off, g_recs;
   size_t vis_pixels = 0;
 
   std::vector<VctTex3D> textures;
+
+  // learned upsampler
+  VctBuf mlp_params, mlp_m1, mlp_m2, mlp_packed, mlp_bias, mlp_grad, mlp_partial;
+  bool mlp_ready = false;
+  int mlp_step = 0;
+  VctBuf low_gpos, low_gnrm, low_img, full_target, sample_pix;
+  VctBuf train_X, train_A, train_Y, train_valid;
+  size_t train_n = 0, train_cap = 0;
 
   hipEvent_t ev[12];
   float pass_ms[5] = {0, 0, 0, 0, 0};
@@ -211,6 +224,10 @@ int vct_launch_mip(vct_context* ctx);
 int vct_launch_voxelize(vct_context* ctx);
 int vct_launch_gbuffer(vct_context* ctx);
 int vct_launch_trace(vct_context* ctx);
+// Trace of a W x H G-buffer into out; demodulate: final / max(Kd, eps) with
+// the pixel's linear depth in w (the learned path's buffers).
+int vct_trace_into(vct_context* ctx, const float4* gpos, const float4* gnrm, float4* out, int W, int H,
+                   int demodulate, unsigned long long* step_counter);
 int vct_launch_scan_u32(vct_context* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
                         uint32_t* total_dev);
 int vct_tex3d_mip_launch(vct_context* ctx, VctTex3D& t);

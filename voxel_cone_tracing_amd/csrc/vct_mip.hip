@@ -6,23 +6,27 @@
 // alpha_blend(f, b) = f + (1 - f.a) * b (mipmap.comp:40-43) of the 2x2x2 block
 // below it, paired along the axis of d, then converted to the storage type.
 //
-// MI355X design: one workgroup reads one aligned 16^3 block of its source
-// level (64 consecutive bricks, a single contiguous range, coalesced 16-byte
-// loads) into LDS, then produces up to four levels of all six maps from LDS
-// (8^3, 4^3, 2^3, 1 texels per map), writing each output brick range once.
-// Each level is computed from the stored (rounded) values of the level
-// below, exactly as the reference's per-level dispatches do, so the fused
-// result equals the level-by-level one bit for bit.  Level 0 is read once
-// from HBM; every mip texel is written once.  A second launch continues from
-// the level the first one stopped at when more levels are requested.
+// MI355X design: levels are Morton ordered, so the 2x2x2 block under output
+// texel m is the eight consecutive source elements [8m, 8m+8).  A 512-lane
+// workgroup owns 4096 consecutive source texels: every lane streams its
+// eight texels with wide loads (64 B per lane for FP16, fully coalesced),
+// produces one texel of the next level for all six maps (a shared level 0
+// is read once for the six), and writes it with coalesced stores.  The
+// following levels (64, 8 and 1 texel per map) are computed from the stored
+// values kept in LDS, exactly as the reference's per-level dispatches read
+// the level below, so the fused result is bit-identical to level-by-level.
+// Every source texel is read once from HBM and every mip texel written once.
 #include "vct_internal.h"
 
 namespace vct {
 namespace {
 
-// Pair tables of mipmap.comp:58-98.  Source texel i of the 2x2x2 block sits
-// at block + voxel_offsets[i] (mipmap.comp:10-20): i = 4*(1-x) + 2*(1-y) + (1-z).
-// For each direction: four (front, back) pairs, summed in this order.
+// Source texel i of mipmap.comp's voxel_offsets (mipmap.comp:10-20) sits at
+// (x, y, z) = (1 - (i >> 2), 1 - ((i >> 1) & 1), 1 - (i & 1)) inside the
+// block, i.e. at Morton offset x | y << 1 | z << 2.
+constexpr int kMorton[8] = {7, 3, 5, 1, 6, 2, 4, 0};
+
+// (front, back) pairs of each direction, summed in this order (mipmap.comp:58-98).
 constexpr uint8_t kPairs[6][4][2] = {
     {{0, 4}, {1, 5}, {2, 6}, {3, 7}},  // -x
     {{4, 0}, {5, 1}, {6, 2}, {7, 3}},  // +x
@@ -39,8 +43,8 @@ template <>
 struct Codec<VCT_RGBA8> {
   using T = uint32_t;
   static __device__ __forceinline__ float4 dec(T v) {
-    return make_float4(unorm8_exact(v & 255u), unorm8_exact((v >> 8) & 255u),
-                       unorm8_exact((v >> 16) & 255u), unorm8_exact(v >> 24));
+    return make_float4(unorm8_exact(v & 255u), unorm8_exact((v >> 8) & 255u), unorm8_exact((v >> 16) & 255u),
+                       unorm8_exact(v >> 24));
   }
   static __device__ __forceinline__ T enc(float4 f) {
     return to_unorm8(f.x) | (to_unorm8(f.y) << 8) | (to_unorm8(f.z) << 16) | (to_unorm8(f.w) << 24);
@@ -77,7 +81,7 @@ struct Codec<VCT_FP32> {
   static __device__ __forceinline__ T enc(float4 f) { return f; }
 };
 
-// f + (1 - f.a) * b, each product and sum rounded separately (no fused
+// f + (1 - f.a) * b, each product and sum rounded on its own (no fused
 // multiply-add), as the oracle evaluates it.
 __device__ __forceinline__ float4 alpha_blend(float4 f, float4 b) {
 #pragma clang fp contract(off)
@@ -89,8 +93,8 @@ __device__ __forceinline__ float4 add4(float4 a, float4 b) {
   return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
 }
 
-// One mip texel of direction D from the eight source texels:
-// ((b0 + b1) + b2) + b3, then / 4 (mipmap.comp:59-98).
+// One mip texel of direction D from the eight source texels, in
+// voxel_offsets order: ((b0 + b1) + b2) + b3, then / 4.
 template <int D>
 __device__ __forceinline__ float4 dir_texel(const float4 (&v)[8]) {
   float4 acc = alpha_blend(v[kPairs[D][0][0]], v[kPairs[D][0][1]]);
@@ -112,141 +116,120 @@ __device__ __forceinline__ float4 dir_texel_rt(int d, const float4 (&v)[8]) {
 }
 
 struct MipArgs {
-  const void* src[6];   // source level per direction (all equal when shared)
-  void* dst[4][6];      // up to four output levels per direction
-  uint32_t src_size;    // source level edge length
-  uint32_t block;       // edge of the source block one workgroup handles (<= 16)
-  int nout;             // output levels produced by this launch (1..4)
-  int shared_src;       // 1: one source for all six maps (level 0)
+  const void* src[6];  // source level per direction (all equal when shared)
+  void* dst[4][6];     // up to four output levels per direction
+  uint32_t n_src;      // texels in the source level
+  int nout;            // output levels produced by this launch (1..4)
+  int shared_src;      // 1: one source for all six maps (level 0)
 };
 
-// Index of local coordinate (x,y,z) inside a block of edge n stored like a
-// level: brick order when n >= 4, x-fastest inside one padded brick otherwise.
-__device__ __forceinline__ uint32_t local_index(uint32_t n, uint32_t x, uint32_t y, uint32_t z) {
-  return brick_index(x, y, z);  // same formula covers the padded case (x,y,z < 4)
-  (void)n;
+constexpr int kThreads = 512;  // one output texel of the first level per lane
+
+// Eight consecutive stored elements as decoded texels in voxel_offsets order.
+template <int DT>
+__device__ __forceinline__ void load_block(const typename Codec<DT>::T* p, float4 (&v)[8]) {
+  using T = typename Codec<DT>::T;
+  T raw[8];
+  if constexpr (sizeof(T) == 4) {
+    const uint4* q = reinterpret_cast<const uint4*>(p);
+    const uint4 a = q[0], b = q[1];
+    raw[0] = a.x; raw[1] = a.y; raw[2] = a.z; raw[3] = a.w;
+    raw[4] = b.x; raw[5] = b.y; raw[6] = b.z; raw[7] = b.w;
+  } else if constexpr (sizeof(T) == 8) {
+    const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint4 a = q[k];
+      raw[2 * k] = make_uint2(a.x, a.y);
+      raw[2 * k + 1] = make_uint2(a.z, a.w);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) raw[k] = p[k];
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v[i] = Codec<DT>::dec(raw[kMorton[i]]);
 }
 
 template <int DT>
-__global__ __launch_bounds__(256) void mip_fused_kernel(MipArgs a) {
+__global__ __launch_bounds__(kThreads) void mip_kernel(MipArgs a) {
   using C = Codec<DT>;
   using T = typename C::T;
-  // LDS: source block (4096 texels) + per direction level outputs.
-  // 8^3 + 4^3 + 2^3 + 1 = 585 texels per direction for the following levels;
-  // each output level is padded to at least one brick (64) for indexing.
-  constexpr int kSrc = 4096;
-  constexpr int kL1 = 512, kL2 = 64, kL3 = 64, kL4 = 64;
-  constexpr int kPerDir = kL1 + kL2 + kL3 + kL4;
-  __shared__ T s_src[kSrc];
-  __shared__ T s_out[6 * kPerDir];
-  const int lvl_base[4] = {0, kL1, kL1 + kL2, kL1 + kL2 + kL3};
+  // stored values of the levels produced inside the workgroup
+  __shared__ T s1[6][512];
+  __shared__ T s2[6][64];
+  __shared__ T s3[6][8];
 
-  const uint32_t B = a.block;
-  const uint32_t nblk_edge = a.src_size / B;
-  const uint32_t blk = blockIdx.x;
   const int dir_lo = a.shared_src ? 0 : (int)blockIdx.y;
-  const int dir_hi = a.shared_src ? 6 : (int)blockIdx.y + 1;
-  const int src_dir = dir_lo;
+  const int ndir = a.shared_src ? 6 : 1;
+  const uint32_t per_blk = a.n_src < 4096u ? a.n_src : 4096u;
+  const uint32_t n1 = per_blk >> 3;  // output texels of the first level in this block
+  const uint32_t t = threadIdx.x;
 
-  // Block origin: blocks are enumerated in Morton order so that block `blk`
-  // covers source elements [blk * B^3, (blk+1) * B^3) when B >= 4.
-  uint32_t bx, by, bz;
-  {
-    uint32_t m = blk;
-    bx = compact3(m);
-    by = compact3(m >> 1);
-    bz = compact3(m >> 2);
-  }
-  (void)nblk_edge;
-
-  // 1) stage the source block
-  const T* src = reinterpret_cast<const T*>(a.src[src_dir]);
-  const uint32_t n_src = B >= 4 ? B * B * B : 64u;
-  const uint64_t src_base = B >= 4 ? (uint64_t)blk * n_src : 0ull;
-  for (uint32_t i = threadIdx.x; i < n_src; i += blockDim.x) s_src[i] = src[src_base + i];
-  __syncthreads();
-
-  // 2) levels
-  uint32_t n_in = B;
-  for (int l = 0; l < a.nout; ++l) {
-    const uint32_t n_out = n_in >> 1;
-    const uint32_t cnt = n_out * n_out * n_out;
-    const uint32_t out_size = (a.src_size >> (l + 1));
-    const uint32_t ox0 = bx * n_out, oy0 = by * n_out, oz0 = bz * n_out;
-    for (uint32_t p = threadIdx.x; p < cnt; p += blockDim.x) {
-      uint32_t x, y, z;
-      if (n_out >= 4) {
-        brick_coords(p, x, y, z);
-      } else {
-        x = p % n_out;
-        y = (p / n_out) % n_out;
-        z = p / (n_out * n_out);
+  // ---- first level: straight from the source, coalesced ----------------------
+  if (t < n1) {
+    const uint32_t m = blockIdx.x * n1 + t;
+    float4 v[8];
+    if (a.shared_src) {
+      load_block<DT>(reinterpret_cast<const T*>(a.src[0]) + 8ull * m, v);
+#pragma unroll
+      for (int d = 0; d < 6; ++d) {
+        const T e = C::enc(dir_texel_rt(d, v));
+        reinterpret_cast<T*>(a.dst[0][d])[m] = e;
+        s1[d][t] = e;
       }
-      const uint32_t gi = brick_index(ox0 + x, oy0 + y, oz0 + z);
-      if (a.shared_src && l == 0) {
-        // all six maps from the same eight texels
-        float4 v[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          const uint32_t sx = 2 * x + ((i & 4) ? 0 : 1);
-          const uint32_t sy = 2 * y + ((i & 2) ? 0 : 1);
-          const uint32_t sz = 2 * z + ((i & 1) ? 0 : 1);
-          v[i] = C::dec(s_src[local_index(n_in, sx, sy, sz)]);
-        }
-#pragma unroll
-        for (int d = 0; d < 6; ++d) {
-          const T e = C::enc(dir_texel_rt(d, v));
-          s_out[d * kPerDir + lvl_base[l] + local_index(n_out, x, y, z)] = e;
-          reinterpret_cast<T*>(a.dst[l][d])[gi] = e;
-        }
-      } else {
-        for (int d = dir_lo; d < dir_hi; ++d) {
-          float4 v[8];
-#pragma unroll
-          for (int i = 0; i < 8; ++i) {
-            const uint32_t sx = 2 * x + ((i & 4) ? 0 : 1);
-            const uint32_t sy = 2 * y + ((i & 2) ? 0 : 1);
-            const uint32_t sz = 2 * z + ((i & 1) ? 0 : 1);
-            const uint32_t li = local_index(n_in, sx, sy, sz);
-            const T raw = (l == 0) ? s_src[li] : s_out[d * kPerDir + lvl_base[l - 1] + li];
-            v[i] = C::dec(raw);
-          }
-          const T e = C::enc(dir_texel_rt(d, v));
-          s_out[d * kPerDir + lvl_base[l] + local_index(n_out, x, y, z)] = e;
-          reinterpret_cast<T*>(a.dst[l][d])[gi] = e;
-        }
-      }
+    } else {
+      const int d = dir_lo;
+      load_block<DT>(reinterpret_cast<const T*>(a.src[d]) + 8ull * m, v);
+      const T e = C::enc(dir_texel_rt(d, v));
+      reinterpret_cast<T*>(a.dst[0][d])[m] = e;
+      s1[0][t] = e;
     }
-    (void)out_size;
+  }
+  // ---- further levels from LDS --------------------------------------------------
+  uint32_t n_prev = n1;
+  for (int l = 1; l < a.nout; ++l) {
     __syncthreads();
-    n_in = n_out;
+    const uint32_t n = n_prev >> 3;
+    if (t < n * (uint32_t)ndir) {
+      const int dl = (int)(t / n);  // local direction slot
+      const uint32_t q = t % n;
+      const int d = dir_lo + dl;
+      const T* src = l == 1 ? &s1[dl][8 * q] : (l == 2 ? &s2[dl][8 * q] : &s3[dl][8 * q]);
+      float4 v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = C::dec(src[kMorton[i]]);
+      const T e = C::enc(dir_texel_rt(d, v));
+      reinterpret_cast<T*>(a.dst[l][d])[blockIdx.x * n + q] = e;
+      if (l == 1) s2[dl][q] = e;
+      else if (l == 2) s3[dl][q] = e;
+    }
+    n_prev = n;
   }
 }
 
 template <int DT>
 int launch_chain(vct_context* ctx) {
-  const uint32_t res = ctx->res;
   const int L = ctx->levels;
   int src_level = 0;
   char* base = reinterpret_cast<char*>(ctx->volume.p);
   while (src_level < L - 1) {
-    const uint32_t s = res >> src_level;
-    const uint32_t B = s < 16 ? s : 16;
+    const uint32_t s = ctx->res >> src_level;
+    const uint32_t n_src = s * s * s;
+    const uint32_t per_blk = n_src < 4096u ? n_src : 4096u;
     int nout = 0;
-    for (uint32_t b = B; b > 1 && nout < 4 && src_level + nout < L - 1; b >>= 1) ++nout;
+    for (uint32_t c = per_blk; c >= 8 && nout < 4 && src_level + nout < L - 1; c >>= 3) ++nout;
     if (nout == 0) break;
     MipArgs a;
     for (int d = 0; d < 6; ++d) a.src[d] = base + ctx->lvl_off[src_level][src_level == 0 ? 0 : d];
     for (int l = 0; l < 4; ++l)
       for (int d = 0; d < 6; ++d)
         a.dst[l][d] = (l < nout) ? (void*)(base + ctx->lvl_off[src_level + 1 + l][d]) : nullptr;
-    a.src_size = s;
-    a.block = B;
+    a.n_src = n_src;
     a.nout = nout;
     a.shared_src = src_level == 0 ? 1 : 0;
-    const uint32_t nb = (s / B) * (s / B) * (s / B);
-    dim3 grid(nb, a.shared_src ? 1 : 6);
-    hipLaunchKernelGGL(mip_fused_kernel<DT>, grid, dim3(256), 0, ctx->stream, a);
+    dim3 grid(n_src / per_blk, a.shared_src ? 1 : 6);
+    hipLaunchKernelGGL(mip_kernel<DT>, grid, dim3(kThreads), 0, ctx->stream, a);
     VCT_HIP(hipGetLastError());
     src_level += nout;
   }

@@ -1,0 +1,916 @@
+// vct_mlp.hip - learned quarter-resolution upsampler (no reference counterpart).
+//
+// A per-pixel MLP 34 -> 64 -> 64 -> 3 with ReLU.  Master parameters are FP32
+// (flat: W1[64][34], b1[64], W2[64][64], b2[64], W3[3][64], b3[3]); every
+// matrix product runs on the gfx950 matrix cores as
+// v_mfma_f32_16x16x32_bf16 with BF16 operands and FP32 accumulation.
+//
+// Matrix-core tile helper: C[16x16] += A[16xK] * B[Kx16] where both operands
+// live in LDS (or global memory) with K contiguous: A as rows m, B as rows n.
+// Lane l supplies A[m = l%16][k0 + 8*(l/16) .. +7] and B[n = l%16][same k] as
+// one 16-byte load each and receives C[4*(l/16) + i][l%16], i = 0..3.
+//
+// Features of a full-resolution pixel p (index order kappa):
+//   0..26  the 3x3 neighbourhood of the quarter-resolution indirect light
+//          (albedo-demodulated), bilinearly sampled at p's own position plus
+//          offsets of -1, 0, +1 low-resolution pixels (tap-major, RGB)
+//   27..29 normal, 30 relative linear depth (d - d_low) / d_low, 31..33 albedo
+// The network predicts the demodulated indirect light; the image is
+// albedo * prediction.
+#include <cmath>
+#include <cstring>
+
+#include "vct_internal.h"
+
+namespace vct {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kIn = VCT_MLP_IN, kHid = VCT_MLP_HIDDEN, kOut = VCT_MLP_OUT;
+constexpr int kKP = 64;  // padded input width
+// flat master layout
+constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kHid * kIn, kOffW2 = kOffB1 + kHid, kOffB2 = kOffW2 + kHid * kHid,
+              kOffW3 = kOffB2 + kHid, kOffB3 = kOffW3 + kOut * kHid, kParams = kOffB3 + kOut;
+static_assert(kParams == VCT_MLP_PARAMS, "parameter count");
+
+// Packed BF16 operand copies (element offsets in one buffer, all K contiguous)
+//   W1p[64][64]  (out x in, in >= 34 zero)      A of layer 1
+//   W2p[64][64]  (out x in)                     A of layer 2
+//   W3p[16][64]  (out x in, out >= 3 zero)      A of layer 3
+//   W2T[64][64]  (in x out)                     A of dH1 = W2^T dZ2
+//   W3T[64][32]  (in x out, out >= 3 zero)      A of dH2 = W3^T dY
+constexpr int kPW1 = 0, kPW2 = kPW1 + 64 * 64, kPW3 = kPW2 + 64 * 64, kPW2T = kPW3 + 16 * 64,
+              kPW3T = kPW2T + 64 * 64, kPacked = kPW3T + 64 * 32;
+
+__device__ __forceinline__ uint16_t bf16_bits(float f) { return f32_to_bf16(f); }
+
+// One 16x16x32 MFMA step; a and b point at the lane's 8 contiguous BF16.
+__device__ __forceinline__ floatx4 mfma_step(const uint16_t* a, const uint16_t* b, floatx4 c) {
+  const bf16x8 va = *reinterpret_cast<const bf16x8*>(a);
+  const bf16x8 vb = *reinterpret_cast<const bf16x8*>(b);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, c, 0, 0, 0);
+}
+
+// C tile += A[m0:m0+16][0:K] * B[n0:n0+16][0:K]^T, K a multiple of 32.
+__device__ __forceinline__ floatx4 mfma_tile(const uint16_t* A, int lda, const uint16_t* B, int ldb, int K,
+                                             floatx4 c) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15, kq = (lane >> 4) * 8;
+  for (int k0 = 0; k0 < K; k0 += 32) c = mfma_step(A + r * lda + k0 + kq, B + r * ldb + k0 + kq, c);
+  return c;
+}
+
+struct MlpDev {
+  const uint16_t* packed;  // kPacked BF16
+  const float* b1;
+  const float* b2;
+  const float* b3;
+};
+
+// ---- features ---------------------------------------------------------------------
+struct FeatArgs {
+  const float4* gpos;  // full-res G-buffer
+  const float4* gnrm;
+  const DevMaterial* mats;
+  uint32_t nmat;
+  const float4* low;  // quarter-res demodulated indirect light (rgb) + linear depth (w)
+  int W, H, wl, hl;
+};
+
+__device__ __forceinline__ float4 low_at(const FeatArgs& F, int i, int j) {
+  i = min(max(i, 0), F.wl - 1);
+  j = min(max(j, 0), F.hl - 1);
+  return F.low[(size_t)j * F.wl + i];
+}
+
+// Returns false for background pixels.  f receives 34 features, alb the albedo.
+__device__ bool pixel_features(const FeatArgs& F, int px, int py, float* f, float* alb) {
+  const size_t idx = (size_t)py * F.W + px;
+  const float4 gp = F.gpos[idx];
+  if (gp.w == 0.0f) return false;
+  const float4 gn = F.gnrm[idx];
+  uint32_t mid = __builtin_bit_cast(uint32_t, gn.w);
+  mid = mid < F.nmat ? mid : 0u;
+  const DevMaterial& m = F.mats[mid];
+  // low-res pixel (i, j) was traced at full-res pixel (4i + 1, 4j + 1)
+  const float u = (float)(px - 1) * 0.25f, v = (float)(py - 1) * 0.25f;
+  const float fu = floorf(u), fv = floorf(v);
+  const int i0 = (int)fu, j0 = (int)fv;
+  const float au = u - fu, av = v - fv;
+  int k = 0;
+  for (int dy = -1; dy <= 1; ++dy)
+    for (int dx = -1; dx <= 1; ++dx) {
+      const float4 a = low_at(F, i0 + dx, j0 + dy), b = low_at(F, i0 + dx + 1, j0 + dy);
+      const float4 c = low_at(F, i0 + dx, j0 + dy + 1), d = low_at(F, i0 + dx + 1, j0 + dy + 1);
+      const float w00 = (1.f - au) * (1.f - av), w10 = au * (1.f - av), w01 = (1.f - au) * av, w11 = au * av;
+      f[k++] = w00 * a.x + w10 * b.x + w01 * c.x + w11 * d.x;
+      f[k++] = w00 * a.y + w10 * b.y + w01 * c.y + w11 * d.y;
+      f[k++] = w00 * a.z + w10 * b.z + w01 * c.z + w11 * d.z;
+    }
+  float nx = gn.x, ny = gn.y, nz = gn.z;
+  const float nl = sqrtf(nx * nx + ny * ny + nz * nz);
+  const float inl = nl > 0.0f ? 1.0f / nl : 0.0f;
+  f[27] = nx * inl;
+  f[28] = ny * inl;
+  f[29] = nz * inl;
+  const float dc = low_at(F, (int)floorf(u + 0.5f), (int)floorf(v + 0.5f)).w;
+  f[30] = dc > 0.0f ? fminf(fmaxf((gp.w - dc) / dc, -1.0f), 1.0f) : 0.0f;
+  f[31] = m.diffuse[0];
+  f[32] = m.diffuse[1];
+  f[33] = m.diffuse[2];
+  alb[0] = m.diffuse[0];
+  alb[1] = m.diffuse[1];
+  alb[2] = m.diffuse[2];
+  return true;
+}
+
+// ---- forward of one wave's 64 samples held in LDS ---------------------------------
+// xs: [64 samples][64] BF16 (kappa order, zero padded), hs: scratch of the same
+// size.  Returns y (C layout of the 16-row output tile) for the wave's four
+// 16-sample column tiles.
+__device__ __forceinline__ void forward_wave(const MlpDev& P, uint16_t* xs, uint16_t* hs, floatx4 (&y)[4]) {
+  const int lane = threadIdx.x & 63;
+  const int col = lane & 15, rq = (lane >> 4) * 4;
+  // layer 1: xs -> hs ; layer 2: hs -> xs ; layer 3: xs -> y
+  for (int layer = 0; layer < 2; ++layer) {
+    const uint16_t* Wp = P.packed + (layer == 0 ? kPW1 : kPW2);
+    const float* bias = layer == 0 ? P.b1 : P.b2;
+    uint16_t* src = layer == 0 ? xs : hs;
+    uint16_t* dst = layer == 0 ? hs : xs;
+    for (int nt = 0; nt < 4; ++nt) {
+      floatx4 acc[4];
+      for (int mt = 0; mt < 4; ++mt) {
+        acc[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+        acc[mt] = mfma_tile(Wp + mt * 16 * 64, 64, src + nt * 16 * 64, 64, 64, acc[mt]);
+      }
+      // the wave's own rows of src are no longer needed: overwrite dst rows
+      for (int mt = 0; mt < 4; ++mt) {
+        uint16_t o[4];
+        for (int i = 0; i < 4; ++i) {
+          const float h = fmaxf(acc[mt][i] + bias[mt * 16 + rq + i], 0.0f);
+          o[i] = bf16_bits(h);
+        }
+        uint2 pk = make_uint2((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16));
+        *reinterpret_cast<uint2*>(dst + (nt * 16 + col) * 64 + mt * 16 + rq) = pk;
+      }
+    }
+  }
+  for (int nt = 0; nt < 4; ++nt) {
+    floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+    acc = mfma_tile(P.packed + kPW3, 64, xs + nt * 16 * 64, 64, 64, acc);
+    for (int i = 0; i < 4; ++i) acc[i] += (rq + i < kOut) ? P.b3[rq + i] : 0.0f;
+    y[nt] = acc;
+  }
+}
+
+// ---- inference: features + MLP + output for a 16x16 pixel tile per workgroup ------
+struct InferArgs {
+  FeatArgs F;
+  MlpDev P;
+  float4* image;
+};
+
+__global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs A) {
+  __shared__ __attribute__((aligned(16))) uint16_t s_x[256 * 64];
+  __shared__ __attribute__((aligned(16))) uint16_t s_h[256 * 64];
+  __shared__ float s_alb[256][3];
+  __shared__ int s_cov[256];
+  const int t = threadIdx.x;
+  const int px = blockIdx.x * 16 + (t & 15), py = blockIdx.y * 16 + (t >> 4);
+  float f[kIn], alb[3] = {0.f, 0.f, 0.f};
+  bool cov = false;
+  if (px < A.F.W && py < A.F.H) cov = pixel_features(A.F, px, py, f, alb);
+  uint16_t* row = s_x + t * 64;
+  for (int k = 0; k < 64; k += 8) {
+    uint32_t w[4];
+    for (int q = 0; q < 4; ++q) {
+      const int k0 = k + 2 * q, k1 = k0 + 1;
+      const uint32_t lo = (cov && k0 < kIn) ? bf16_bits(f[k0]) : 0u;
+      const uint32_t hi = (cov && k1 < kIn) ? bf16_bits(f[k1]) : 0u;
+      w[q] = lo | (hi << 16);
+    }
+    *reinterpret_cast<uint4*>(row + k) = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+  s_alb[t][0] = alb[0];
+  s_alb[t][1] = alb[1];
+  s_alb[t][2] = alb[2];
+  s_cov[t] = cov ? 1 : 0;
+  __syncthreads();
+  const int wave = t >> 6, lane = t & 63;
+  floatx4 y[4];
+  forward_wave(A.P, s_x + wave * 64 * 64, s_h + wave * 64 * 64, y);
+  // lanes 0..15 hold rows 0..3 (outputs 0..2) of sample column lane
+  if (lane < 16) {
+    for (int nt = 0; nt < 4; ++nt) {
+      const int s = wave * 64 + nt * 16 + lane;
+      const int qx = blockIdx.x * 16 + (s & 15), qy = blockIdx.y * 16 + (s >> 4);
+      if (qx < A.F.W && qy < A.F.H) {
+        float4 o;
+        if (s_cov[s]) {
+          o = make_float4(s_alb[s][0] * y[nt][0], s_alb[s][1] * y[nt][1], s_alb[s][2] * y[nt][2], 1.0f);
+        } else {
+          o = make_float4(0.15f, 0.25f, 0.25f, 1.0f);  // clear colour, renderer.cpp:398
+        }
+        A.image[(size_t)qy * A.F.W + qx] = o;
+      }
+    }
+  }
+}
+
+// ---- training samples: features, albedo and target at listed pixels ---------------
+__global__ __launch_bounds__(256) void mlp_sample_kernel(FeatArgs F, const float4* __restrict__ target,
+                                                         const uint32_t* __restrict__ pixels, uint32_t n,
+                                                         uint16_t* __restrict__ X, float* __restrict__ Alb,
+                                                         float* __restrict__ Y, uint32_t* __restrict__ valid) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t p = pixels[k];
+  const int px = (int)(p % (uint32_t)F.W), py = (int)(p / (uint32_t)F.W);
+  float f[kIn], alb[3] = {0.f, 0.f, 0.f};
+  const bool cov = pixel_features(F, px, py, f, alb);
+  uint16_t* row = X + (size_t)k * 64;
+  for (int c = 0; c < 64; ++c) row[c] = (cov && c < kIn) ? bf16_bits(f[c]) : (uint16_t)0;
+  const float4 tg = target[p];
+  Alb[k * 3 + 0] = cov ? alb[0] : 0.f;
+  Alb[k * 3 + 1] = cov ? alb[1] : 0.f;
+  Alb[k * 3 + 2] = cov ? alb[2] : 0.f;
+  Y[k * 3 + 0] = tg.x;
+  Y[k * 3 + 1] = tg.y;
+  Y[k * 3 + 2] = tg.z;
+  valid[k] = cov ? 1u : 0u;
+}
+
+// ---- forward on given features (tests and evaluation) -------------------------------
+__global__ __launch_bounds__(256) void mlp_forward_kernel(MlpDev P, const uint16_t* __restrict__ X, uint32_t n,
+                                                          float* __restrict__ Y) {
+  __shared__ __attribute__((aligned(16))) uint16_t s_x[256 * 64];
+  __shared__ __attribute__((aligned(16))) uint16_t s_h[256 * 64];
+  const int t = threadIdx.x;
+  const uint32_t base = blockIdx.x * 256u;
+  {
+    const uint32_t k = base + t;
+    const uint4* src = reinterpret_cast<const uint4*>(X + (size_t)k * 64);
+    uint4* dst = reinterpret_cast<uint4*>(s_x + t * 64);
+    for (int q = 0; q < 8; ++q) dst[q] = k < n ? src[q] : make_uint4(0, 0, 0, 0);
+  }
+  __syncthreads();
+  const int wave = t >> 6, lane = t & 63;
+  floatx4 y[4];
+  forward_wave(P, s_x + wave * 64 * 64, s_h + wave * 64 * 64, y);
+  if (lane < 16)
+    for (int nt = 0; nt < 4; ++nt) {
+      const uint32_t k = base + wave * 64 + nt * 16 + lane;
+      if (k < n)
+        for (int o = 0; o < kOut; ++o) Y[(size_t)k * 3 + o] = y[nt][o];
+    }
+}
+
+// ---- training: forward + backward over chunks of 64 samples ------------------------
+// Per chunk (P = 64 samples, 4 waves of 16): every activation and gradient is
+// staged in LDS as BF16 in the K-contiguous layout its next product reads.
+// Gradients accumulate in FP32 matrix-core accumulators across the chunks of
+// the workgroup; partial sums go to `partial` [gridDim.x][kParams] and are
+// reduced in a fixed order by mlp_adam_kernel (deterministic).
+struct TrainArgs {
+  MlpDev P;
+  const uint16_t* X;  // [n][64]
+  const float* Alb;   // [n][3]
+  const float* Y;     // [n][3] target
+  uint32_t n;
+  float scale;        // 2 / (3 * n): d(mean squared error) / dy
+  float* partial;     // [gridDim.x][kParams + 1] (last: loss sum)
+};
+
+constexpr int kP = 64;
+
+__global__ __launch_bounds__(256) void mlp_train_kernel(TrainArgs A) {
+  __shared__ __attribute__((aligned(16))) uint16_t xs[kP * 64];    // [sample][in]
+  __shared__ __attribute__((aligned(16))) uint16_t xt[64 * kP];    // [in][sample]
+  __shared__ __attribute__((aligned(16))) uint16_t h1s[kP * 64];   // [sample][hid]
+  __shared__ __attribute__((aligned(16))) uint16_t h1t[64 * kP];   // [hid][sample]
+  __shared__ __attribute__((aligned(16))) uint16_t h2s[kP * 64];
+  __shared__ __attribute__((aligned(16))) uint16_t h2t[64 * kP];
+  __shared__ __attribute__((aligned(16))) uint16_t dys[kP * 32];   // [sample][out (pad 32)]
+  __shared__ __attribute__((aligned(16))) uint16_t dyt[16 * kP];   // [out (pad 16)][sample]
+  __shared__ __attribute__((aligned(16))) uint16_t dz2s[kP * 64];  // [sample][hid]
+  __shared__ __attribute__((aligned(16))) uint16_t dz2t[64 * kP];  // [hid][sample]
+  __shared__ __attribute__((aligned(16))) uint16_t dz1t[64 * kP];  // [hid][sample]
+  __shared__ float s_db[4][2 * 64 + 16];
+  __shared__ float s_loss[4];
+
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int col = lane & 15, rq = (lane >> 4) * 4;
+  const uint16_t* pk = A.P.packed;
+
+  // zero the padded output columns once (rows 3..31 of dY stay zero)
+  for (int i = t; i < kP * 32; i += 256) dys[i] = 0;
+  for (int i = t; i < 16 * kP; i += 256) dyt[i] = 0;
+
+  // gradient tiles owned by this wave: 4 (W3) + 16 (W2) + 12 (W1, in < 48) = 32
+  floatx4 g[8];
+  for (int q = 0; q < 8; ++q) g[q] = floatx4{0.f, 0.f, 0.f, 0.f};
+  float db1[16], db2[16], db3[3] = {0.f, 0.f, 0.f};
+  for (int q = 0; q < 16; ++q) db1[q] = db2[q] = 0.f;
+  float loss = 0.f;
+
+  const uint32_t nchunks = (A.n + kP - 1) / kP;
+  for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const uint32_t base = ch * kP;
+    __syncthreads();
+    // stage X in both layouts
+    for (int i = t; i < kP * 8; i += 256) {
+      const int s = i >> 3, q = i & 7;
+      const uint32_t k = base + s;
+      const uint4 v = k < A.n ? reinterpret_cast<const uint4*>(A.X + (size_t)k * 64)[q] : make_uint4(0, 0, 0, 0);
+      *reinterpret_cast<uint4*>(xs + s * 64 + q * 8) = v;
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+      for (int e = 0; e < 8; ++e) xt[(q * 8 + e) * kP + s] = (uint16_t)(w[e >> 1] >> ((e & 1) * 16));
+    }
+    __syncthreads();
+    // forward layers 1 and 2 on the wave's 16-sample column tile
+    floatx4 h1[4], h2[4];
+    for (int mt = 0; mt < 4; ++mt) {
+      floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+      acc = mfma_tile(pk + kPW1 + mt * 16 * 64, 64, xs + wave * 16 * 64, 64, 64, acc);
+      for (int i = 0; i < 4; ++i) acc[i] = fmaxf(acc[i] + A.P.b1[mt * 16 + rq + i], 0.f);
+      h1[mt] = acc;
+      uint16_t o[4];
+      for (int i = 0; i < 4; ++i) o[i] = bf16_bits(acc[i]);
+      const int s = wave * 16 + col;
+      *reinterpret_cast<uint2*>(h1s + s * 64 + mt * 16 + rq) =
+          make_uint2((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16));
+      for (int i = 0; i < 4; ++i) h1t[(mt * 16 + rq + i) * kP + s] = o[i];
+    }
+    __syncthreads();
+    for (int mt = 0; mt < 4; ++mt) {
+      floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+      acc = mfma_tile(pk + kPW2 + mt * 16 * 64, 64, h1s + wave * 16 * 64, 64, 64, acc);
+      for (int i = 0; i < 4; ++i) acc[i] = fmaxf(acc[i] + A.P.b2[mt * 16 + rq + i], 0.f);
+      h2[mt] = acc;
+      uint16_t o[4];
+      for (int i = 0; i < 4; ++i) o[i] = bf16_bits(acc[i]);
+      const int s = wave * 16 + col;
+      *reinterpret_cast<uint2*>(h2s + s * 64 + mt * 16 + rq) =
+          make_uint2((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16));
+      for (int i = 0; i < 4; ++i) h2t[(mt * 16 + rq + i) * kP + s] = o[i];
+    }
+    __syncthreads();
+    // layer 3 and the output gradient
+    {
+      floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+      acc = mfma_tile(pk + kPW3, 64, h2s + wave * 16 * 64, 64, 64, acc);
+      const int s = wave * 16 + col;
+      const uint32_t k = base + s;
+      if (lane < 16) {  // rows 0..3 of the output tile: outputs 0..2
+        for (int o = 0; o < kOut; ++o) {
+          float d = 0.f;
+          if (k < A.n) {
+            const float yv = acc[o] + A.P.b3[o];
+            const float a = A.Alb[(size_t)k * 3 + o];
+            const float e = a * (yv - A.Y[(size_t)k * 3 + o]);
+            loss += e * e;
+            d = A.scale * a * e;  // d(loss)/dy
+          }
+          db3[o] += d;
+          const uint16_t b = bf16_bits(d);
+          dys[s * 32 + o] = b;
+          dyt[o * kP + s] = b;
+        }
+      }
+    }
+    __syncthreads();
+    // dH2 = W3^T dY -> dZ2 = dH2 * [h2 > 0]
+    for (int mt = 0; mt < 4; ++mt) {
+      floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+      acc = mfma_tile(pk + kPW3T + mt * 16 * 32, 32, dys + wave * 16 * 32, 32, 32, acc);
+      uint16_t o[4];
+      for (int i = 0; i < 4; ++i) {
+        const float dz = h2[mt][i] > 0.f ? acc[i] : 0.f;
+        db2[mt * 4 + i] += dz;
+        o[i] = bf16_bits(dz);
+      }
+      const int s = wave * 16 + col;
+      *reinterpret_cast<uint2*>(dz2s + s * 64 + mt * 16 + rq) =
+          make_uint2((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16));
+      for (int i = 0; i < 4; ++i) dz2t[(mt * 16 + rq + i) * kP + s] = o[i];
+    }
+    __syncthreads();
+    // dH1 = W2^T dZ2 -> dZ1 = dH1 * [h1 > 0]
+    for (int mt = 0; mt < 4; ++mt) {
+      floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+      acc = mfma_tile(pk + kPW2T + mt * 16 * 64, 64, dz2s + wave * 16 * 64, 64, 64, acc);
+      const int s = wave * 16 + col;
+      for (int i = 0; i < 4; ++i) {
+        const float dz = h1[mt][i] > 0.f ? acc[i] : 0.f;
+        db1[mt * 4 + i] += dz;
+        dz1t[(mt * 16 + rq + i) * kP + s] = bf16_bits(dz);
+      }
+    }
+    __syncthreads();
+    // weight gradients over the chunk's 64 samples (K = samples)
+    for (int q = 0; q < 8; ++q) {
+      const int tile = wave * 8 + q;  // 0..31
+      if (tile < 4) {                 // dW3 [16 x 64]: n tile = tile
+        g[q] = mfma_tile(dyt, kP, h2t + tile * 16 * kP, kP, kP, g[q]);
+      } else if (tile < 20) {         // dW2 [64 x 64]
+        const int tt = tile - 4, mt = tt >> 2, nt = tt & 3;
+        g[q] = mfma_tile(dz2t + mt * 16 * kP, kP, h1t + nt * 16 * kP, kP, kP, g[q]);
+      } else {                        // dW1 [64 x 48]
+        const int tt = tile - 20, mt = tt / 3, nt = tt % 3;
+        g[q] = mfma_tile(dz1t + mt * 16 * kP, kP, xt + nt * 16 * kP, kP, kP, g[q]);
+      }
+    }
+  }
+  // ---- write this workgroup's partial gradients --------------------------------------
+  float* out = A.partial + (size_t)blockIdx.x * (kParams + 1);
+  for (int q = 0; q < 8; ++q) {
+    const int tile = wave * 8 + q;
+    for (int i = 0; i < 4; ++i) {
+      const int m = rq + i;
+      if (tile < 4) {
+        if (m < kOut) out[kOffW3 + m * 64 + tile * 16 + col] = g[q][i];
+      } else if (tile < 20) {
+        const int tt = tile - 4, mt = tt >> 2, nt = tt & 3;
+        out[kOffW2 + (mt * 16 + m) * 64 + nt * 16 + col] = g[q][i];
+      } else {
+        const int tt = tile - 20, mt = tt / 3, nt = tt % 3;
+        const int in = nt * 16 + col;
+        if (in < kIn) out[kOffW1 + (mt * 16 + m) * kIn + in] = g[q][i];
+      }
+    }
+  }
+  // bias gradients: sum the 16 sample lanes of each row group, then the waves
+  for (int q = 0; q < 16; ++q) {
+    float a = db1[q], b = db2[q];
+    for (int o = 1; o < 16; o <<= 1) {
+      a += __shfl_xor(a, o, 64);
+      b += __shfl_xor(b, o, 64);
+    }
+    db1[q] = a;
+    db2[q] = b;
+  }
+  if (col == 0) {
+    for (int mt = 0; mt < 4; ++mt)
+      for (int i = 0; i < 4; ++i) {
+        s_db[wave][mt * 16 + rq + i] = db1[mt * 4 + i];
+        s_db[wave][64 + mt * 16 + rq + i] = db2[mt * 4 + i];
+      }
+  }
+  {
+    float l = loss;
+    for (int o = 1; o < 64; o <<= 1) l += __shfl_xor(l, o, 64);
+    if (lane == 0) s_loss[wave] = l;
+  }
+  for (int o = 0; o < kOut; ++o) {
+    float a = lane < 16 ? db3[o] : 0.f;
+    for (int q = 1; q < 16; q <<= 1) a += __shfl_xor(a, q, 64);
+    if (lane == 0) s_db[wave][128 + o] = a;
+  }
+  __syncthreads();
+  if (t < 64) {
+    out[kOffB1 + t] = ((s_db[0][t] + s_db[1][t]) + s_db[2][t]) + s_db[3][t];
+    out[kOffB2 + t] = ((s_db[0][64 + t] + s_db[1][64 + t]) + s_db[2][64 + t]) + s_db[3][64 + t];
+  }
+  if (t < kOut) out[kOffB3 + t] = ((s_db[0][128 + t] + s_db[1][128 + t]) + s_db[2][128 + t]) + s_db[3][128 + t];
+  if (t == 0) out[kParams] = ((s_loss[0] + s_loss[1]) + s_loss[2]) + s_loss[3];
+}
+
+// Fixed-order reduction of the partials + Adam (FP32 master weights).
+// With lr == 0 it only reduces (gradient and loss read-back).
+__global__ __launch_bounds__(256) void mlp_adam_kernel(float* __restrict__ params, float* __restrict__ m1,
+                                                       float* __restrict__ m2, const float* __restrict__ partial,
+                                                       int nblk, float lr, float beta1, float beta2, float eps,
+                                                       float bc1, float bc2, float* __restrict__ grad_out,
+                                                       float* __restrict__ loss_out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > kParams) return;
+  float g = 0.f;
+  for (int b = 0; b < nblk; ++b) g += partial[(size_t)b * (kParams + 1) + p];
+  if (p == kParams) {
+    if (loss_out) *loss_out = g;
+    return;
+  }
+  if (grad_out) grad_out[p] = g;
+  if (lr > 0.f) {
+    const float a = beta1 * m1[p] + (1.f - beta1) * g;
+    const float b = beta2 * m2[p] + (1.f - beta2) * g * g;
+    m1[p] = a;
+    m2[p] = b;
+    params[p] -= lr * (a / bc1) / (sqrtf(b / bc2) + eps);
+  }
+}
+
+// BF16 operand copies of the master weights.
+__global__ __launch_bounds__(256) void mlp_pack_kernel(const float* __restrict__ params, uint16_t* __restrict__ pk,
+                                                       float* __restrict__ bias) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 64 * 64) {
+    const int r = i >> 6, c = i & 63;
+    pk[kPW1 + i] = c < kIn ? bf16_bits(params[kOffW1 + r * kIn + c]) : (uint16_t)0;
+    pk[kPW2 + i] = bf16_bits(params[kOffW2 + r * 64 + c]);
+    pk[kPW2T + i] = bf16_bits(params[kOffW2 + c * 64 + r]);  // [in r][out c]
+  }
+  if (i < 16 * 64) {
+    const int r = i >> 6, c = i & 63;
+    pk[kPW3 + i] = r < kOut ? bf16_bits(params[kOffW3 + r * 64 + c]) : (uint16_t)0;
+  }
+  if (i < 64 * 32) {
+    const int r = i >> 5, c = i & 31;  // [hid r][out c]
+    pk[kPW3T + i] = c < kOut ? bf16_bits(params[kOffW3 + c * 64 + r]) : (uint16_t)0;
+  }
+  if (i < 64) {
+    bias[i] = params[kOffB1 + i];
+    bias[64 + i] = params[kOffB2 + i];
+  }
+  if (i < 16) bias[128 + i] = i < kOut ? params[kOffB3 + i] : 0.f;
+}
+
+}  // namespace vct
+
+// ============================================================================
+// host side
+// ============================================================================
+using namespace vct;
+
+namespace {
+
+__global__ void lowres_gather_kernel(const float4* __restrict__ gpos, const float4* __restrict__ gnrm, int W, int H,
+                                     float4* __restrict__ lpos, float4* __restrict__ lnrm, int wl, int hl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= wl * hl) return;
+  const int x = i % wl, y = i / wl;
+  const int sx = min(4 * x + 1, W - 1), sy = min(4 * y + 1, H - 1);
+  lpos[i] = gpos[(size_t)sy * W + sx];
+  lnrm[i] = gnrm[(size_t)sy * W + sx];
+}
+
+struct HostRng {
+  uint64_t s;
+  explicit HostRng(uint64_t seed) : s(seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull) {}
+  uint64_t next() {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+int mlp_ensure(vct_context* ctx) {
+  int rc;
+  if ((rc = vct_ensure(ctx->mlp_params, kParams * 4, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->mlp_m1, kParams * 4, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->mlp_m2, kParams * 4, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->mlp_packed, kPacked * 2, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->mlp_bias, 144 * 4, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->mlp_grad, (kParams + 1) * 4, ctx))) return rc;
+  return VCT_OK;
+}
+
+int mlp_pack(vct_context* ctx) {
+  hipLaunchKernelGGL(mlp_pack_kernel, dim3(16), dim3(256), 0, ctx->stream, (const float*)ctx->mlp_params.p,
+                     (uint16_t*)ctx->mlp_packed.p, (float*)ctx->mlp_bias.p);
+  VCT_HIP(hipGetLastError());
+  return VCT_OK;
+}
+
+MlpDev mlp_dev(vct_context* ctx) {
+  MlpDev P;
+  P.packed = (const uint16_t*)ctx->mlp_packed.p;
+  P.b1 = (const float*)ctx->mlp_bias.p;
+  P.b2 = P.b1 + 64;
+  P.b3 = P.b1 + 128;
+  return P;
+}
+
+int need_mlp(vct_context* ctx) {
+  if (!ctx->mlp_ready) return vct_set_error(VCT_ERR_STATE, "upsampler parameters not set (vct_mlp_init first)");
+  return VCT_OK;
+}
+
+// Training step over X/A/Y rows [off, off + n) already on the device.
+int mlp_train_on(vct_context* ctx, const uint16_t* X, const float* Alb, const float* Y, uint32_t n, float lr,
+                 float* loss_host, float* grad_host) {
+  const uint32_t nchunks = (n + kP - 1) / kP;
+  const int G = (int)std::min<uint32_t>(nchunks, 1024u);
+  int rc;
+  if ((rc = vct_ensure(ctx->mlp_partial, (size_t)G * (kParams + 1) * 4, ctx))) return rc;
+  TrainArgs A;
+  A.P = mlp_dev(ctx);
+  A.X = X;
+  A.Alb = Alb;
+  A.Y = Y;
+  A.n = n;
+  A.scale = 2.0f / (3.0f * (float)n);
+  A.partial = (float*)ctx->mlp_partial.p;
+  hipLaunchKernelGGL(mlp_train_kernel, dim3(G), dim3(256), 0, ctx->stream, A);
+  VCT_HIP(hipGetLastError());
+  float bc1 = 1.f, bc2 = 1.f;
+  if (lr > 0.f) {
+    ctx->mlp_step += 1;
+    bc1 = 1.0f - powf(0.9f, (float)ctx->mlp_step);
+    bc2 = 1.0f - powf(0.999f, (float)ctx->mlp_step);
+  }
+  float* gout = (float*)ctx->mlp_grad.p;
+  hipLaunchKernelGGL(mlp_adam_kernel, dim3((kParams + 1 + 255) / 256), dim3(256), 0, ctx->stream,
+                     (float*)ctx->mlp_params.p, (float*)ctx->mlp_m1.p, (float*)ctx->mlp_m2.p,
+                     (const float*)ctx->mlp_partial.p, G, lr, 0.9f, 0.999f, 1e-8f, bc1, bc2, gout, gout + kParams);
+  VCT_HIP(hipGetLastError());
+  if (lr > 0.f) {
+    if ((rc = mlp_pack(ctx))) return rc;
+  }
+  if (loss_host || grad_host) {
+    std::vector<float> g(kParams + 1);
+    VCT_HIP(hipMemcpyAsync(g.data(), gout, (kParams + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    VCT_HIP(hipStreamSynchronize(ctx->stream));
+    if (grad_host) memcpy(grad_host, g.data(), kParams * 4);
+    if (loss_host) *loss_host = g[kParams] / (3.0f * (float)n);
+  }
+  return VCT_OK;
+}
+
+int upload_features(vct_context* ctx, const float* features, uint32_t n, VctBuf& buf) {
+  std::vector<uint16_t> h((size_t)n * 64, 0);
+  for (size_t i = 0; i < n; ++i)
+    for (int c = 0; c < kIn; ++c) h[i * 64 + c] = f32_to_bf16(features[i * kIn + c]);
+  int rc;
+  if ((rc = vct_ensure(buf, h.size() * 2 + 16, ctx))) return rc;
+  VCT_HIP(hipMemcpyAsync(buf.p, h.data(), h.size() * 2, hipMemcpyHostToDevice, ctx->stream));
+  VCT_HIP(hipStreamSynchronize(ctx->stream));
+  return VCT_OK;
+}
+
+FeatArgs feat_args(vct_context* ctx) {
+  FeatArgs F;
+  F.gpos = (const float4*)ctx->g_pos.p;
+  F.gnrm = (const float4*)ctx->g_nrm.p;
+  F.mats = (const DevMaterial*)ctx->d_materials.p;
+  F.nmat = (uint32_t)ctx->materials.size();
+  F.low = (const float4*)ctx->low_img.p;
+  F.W = ctx->width;
+  F.H = ctx->height;
+  F.wl = ctx->width / 4;
+  F.hl = ctx->height / 4;
+  return F;
+}
+
+// Quarter-resolution G-buffer (pixel (4i+1, 4j+1) of every 4x4 block) and the
+// albedo-demodulated indirect light traced there (w = linear depth).
+int trace_lowres(vct_context* ctx) {
+  const int W = ctx->width, H = ctx->height;
+  if (W % 4 || H % 4) return vct_set_error(VCT_ERR_STATE, "the learned path needs a viewport divisible by 4");
+  const int wl = W / 4, hl = H / 4;
+  const size_t nl = (size_t)wl * hl;
+  int rc;
+  if ((rc = vct_ensure(ctx->low_gpos, nl * 16, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->low_gnrm, nl * 16, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->low_img, nl * 16, ctx))) return rc;
+  hipLaunchKernelGGL(lowres_gather_kernel, dim3((uint32_t)((nl + 255) / 256)), dim3(256), 0, ctx->stream,
+                     (const float4*)ctx->g_pos.p, (const float4*)ctx->g_nrm.p, W, H, (float4*)ctx->low_gpos.p,
+                     (float4*)ctx->low_gnrm.p, wl, hl);
+  VCT_HIP(hipGetLastError());
+  return vct_trace_into(ctx, (const float4*)ctx->low_gpos.p, (const float4*)ctx->low_gnrm.p,
+                        (float4*)ctx->low_img.p, wl, hl, 1, nullptr);
+}
+
+int infer_full(vct_context* ctx) {
+  InferArgs A;
+  A.F = feat_args(ctx);
+  A.P = mlp_dev(ctx);
+  A.image = (float4*)ctx->image.p;
+  dim3 grid((ctx->width + 15) / 16, (ctx->height + 15) / 16);
+  hipLaunchKernelGGL(mlp_infer_kernel, grid, dim3(256), 0, ctx->stream, A);
+  VCT_HIP(hipGetLastError());
+  return VCT_OK;
+}
+
+}  // namespace
+
+#define MLP_CTX(ctx)                                                          \
+  do {                                                                        \
+    if (!(ctx)) return vct_set_error(VCT_ERR_BAD_HANDLE, "null context");     \
+    VCT_HIP(hipSetDevice((ctx)->device));                                     \
+  } while (0)
+
+extern "C" {
+
+int vct_mlp_init(vct_ctx ctx, uint64_t seed) {
+  MLP_CTX(ctx);
+  int rc;
+  if ((rc = mlp_ensure(ctx))) return rc;
+  std::vector<float> p(kParams, 0.0f);
+  HostRng rng(seed);
+  auto fill = [&](int off, int rows, int fan_in, double gain) {
+    const double b = gain * std::sqrt(6.0 / fan_in);
+    for (int i = 0; i < rows * fan_in; ++i) p[off + i] = (float)((2.0 * rng.uniform() - 1.0) * b);
+  };
+  fill(kOffW1, kHid, kIn, 1.0);
+  fill(kOffW2, kHid, kHid, 1.0);
+  fill(kOffW3, kOut, kHid, 0.5);
+  std::vector<float> z(kParams, 0.0f);
+  VCT_HIP(hipMemcpyAsync(ctx->mlp_params.p, p.data(), kParams * 4, hipMemcpyHostToDevice, ctx->stream));
+  VCT_HIP(hipMemcpyAsync(ctx->mlp_m1.p, z.data(), kParams * 4, hipMemcpyHostToDevice, ctx->stream));
+  VCT_HIP(hipMemcpyAsync(ctx->mlp_m2.p, z.data(), kParams * 4, hipMemcpyHostToDevice, ctx->stream));
+  VCT_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->mlp_step = 0;
+  ctx->mlp_ready = true;
+  return mlp_pack(ctx);
+}
+
+int vct_mlp_set_params(vct_ctx ctx, const float* params, uint32_t n) {
+  MLP_CTX(ctx);
+  if (!params || n != (uint32_t)kParams) return vct_set_error(VCT_ERR_INVALID_ARG, "need VCT_MLP_PARAMS floats");
+  int rc;
+  if ((rc = mlp_ensure(ctx))) return rc;
+  std::vector<float> z(kParams, 0.0f);
+  VCT_HIP(hipMemcpyAsync(ctx->mlp_params.p, params, kParams * 4, hipMemcpyHostToDevice, ctx->stream));
+  VCT_HIP(hipMemcpyAsync(ctx->mlp_m1.p, z.data(), kParams * 4, hipMemcpyHostToDevice, ctx->stream));
+  VCT_HIP(hipMemcpyAsync(ctx->mlp_m2.p, z.data(), kParams * 4, hipMemcpyHostToDevice, ctx->stream));
+  VCT_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->mlp_step = 0;
+  ctx->mlp_ready = true;
+  return mlp_pack(ctx);
+}
+
+int vct_mlp_get_params(vct_ctx ctx, float* params, uint32_t n) {
+  MLP_CTX(ctx);
+  if (!params || n != (uint32_t)kParams) return vct_set_error(VCT_ERR_INVALID_ARG, "need VCT_MLP_PARAMS floats");
+  int rc;
+  if ((rc = need_mlp(ctx))) return rc;
+  VCT_HIP(hipMemcpyAsync(params, ctx->mlp_params.p, kParams * 4, hipMemcpyDeviceToHost, ctx->stream));
+  VCT_HIP(hipStreamSynchronize(ctx->stream));
+  return VCT_OK;
+}
+
+int vct_mlp_forward(vct_ctx ctx, const float* features, uint32_t n, float* out) {
+  MLP_CTX(ctx);
+  if (!features || !out || n == 0) return vct_set_error(VCT_ERR_INVALID_ARG, "bad arguments");
+  int rc;
+  if ((rc = need_mlp(ctx))) return rc;
+  if ((rc = upload_features(ctx, features, n, ctx->train_X))) return rc;
+  if ((rc = vct_ensure(ctx->train_Y, (size_t)n * 12, ctx))) return rc;
+  hipLaunchKernelGGL(mlp_forward_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, mlp_dev(ctx),
+                     (const uint16_t*)ctx->train_X.p, n, (float*)ctx->train_Y.p);
+  VCT_HIP(hipGetLastError());
+  VCT_HIP(hipMemcpyAsync(out, ctx->train_Y.p, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream));
+  VCT_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->train_n = 0;  // the training set buffers were reused
+  return VCT_OK;
+}
+
+int vct_mlp_loss_grad(vct_ctx ctx, const float* features, const float* albedo, const float* target, uint32_t n,
+                      float* grad, float* loss) {
+  MLP_CTX(ctx);
+  if (!features || !albedo || !target || n == 0) return vct_set_error(VCT_ERR_INVALID_ARG, "bad arguments");
+  int rc;
+  if ((rc = need_mlp(ctx))) return rc;
+  if ((rc = upload_features(ctx, features, n, ctx->train_X))) return rc;
+  if ((rc = vct_ensure(ctx->train_A, (size_t)n * 12, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->train_Y, (size_t)n * 12, ctx))) return rc;
+  VCT_HIP(hipMemcpyAsync(ctx->train_A.p, albedo, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream));
+  VCT_HIP(hipMemcpyAsync(ctx->train_Y.p, target, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream));
+  ctx->train_n = 0;
+  return mlp_train_on(ctx, (const uint16_t*)ctx->train_X.p, (const float*)ctx->train_A.p,
+                      (const float*)ctx->train_Y.p, n, 0.0f, loss, grad);
+}
+
+int vct_upsample_set_samples(vct_ctx ctx, const float* features, const float* albedo, const float* target,
+                             uint32_t n) {
+  MLP_CTX(ctx);
+  if (!features || !albedo || !target || n == 0) return vct_set_error(VCT_ERR_INVALID_ARG, "bad arguments");
+  int rc;
+  if ((rc = upload_features(ctx, features, n, ctx->train_X))) return rc;
+  if ((rc = vct_ensure(ctx->train_A, (size_t)n * 12, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->train_Y, (size_t)n * 12, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->train_valid, (size_t)n * 4, ctx))) return rc;
+  VCT_HIP(hipMemcpyAsync(ctx->train_A.p, albedo, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream));
+  VCT_HIP(hipMemcpyAsync(ctx->train_Y.p, target, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream));
+  VCT_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->train_n = n;
+  ctx->train_cap = std::min(ctx->train_X.cap / 128, std::min(ctx->train_A.cap / 12, ctx->train_Y.cap / 12));
+  return VCT_OK;
+}
+
+int vct_upsample_clear_samples(vct_ctx ctx) {
+  MLP_CTX(ctx);
+  ctx->train_n = 0;
+  return VCT_OK;
+}
+
+int vct_upsample_sample_count(vct_ctx ctx, uint64_t* n) {
+  MLP_CTX(ctx);
+  if (!n) return vct_set_error(VCT_ERR_INVALID_ARG, "null out");
+  *n = ctx->train_n;
+  return VCT_OK;
+}
+
+int vct_upsample_collect(vct_ctx ctx, uint32_t nsamples, uint64_t seed) {
+  MLP_CTX(ctx);
+  if (nsamples == 0) return VCT_OK;
+  const int W = ctx->width, H = ctx->height;
+  const size_t npix = (size_t)W * H;
+  if (!ctx->g_pos.p || ctx->g_pos.cap < npix * 16)
+    return vct_set_error(VCT_ERR_STATE, "collect needs a G-buffer of the current viewport (vct_gbuffer first)");
+  int rc;
+  // full-resolution plain trace, demodulated: the target
+  if ((rc = vct_ensure(ctx->full_target, npix * 16, ctx))) return rc;
+  if ((rc = vct_trace_into(ctx, (const float4*)ctx->g_pos.p, (const float4*)ctx->g_nrm.p,
+                           (float4*)ctx->full_target.p, W, H, 1, nullptr)))
+    return rc;
+  if ((rc = trace_lowres(ctx))) return rc;
+  // random pixels (background ones get zero weight)
+  std::vector<uint32_t> pix(nsamples);
+  HostRng rng(seed);
+  for (uint32_t i = 0; i < nsamples; ++i) pix[i] = (uint32_t)(rng.next() % npix);
+  if ((rc = vct_ensure(ctx->sample_pix, (size_t)nsamples * 4, ctx))) return rc;
+  VCT_HIP(hipMemcpyAsync(ctx->sample_pix.p, pix.data(), (size_t)nsamples * 4, hipMemcpyHostToDevice, ctx->stream));
+  const size_t need = ctx->train_n + nsamples;
+  if (need > ctx->train_cap) {
+    size_t cap = std::max<size_t>(need, ctx->train_cap * 2);
+    VctBuf nx, na, ny, nv;
+    if ((rc = vct_ensure(nx, cap * 128, ctx))) return rc;
+    if ((rc = vct_ensure(na, cap * 12, ctx))) return rc;
+    if ((rc = vct_ensure(ny, cap * 12, ctx))) return rc;
+    if ((rc = vct_ensure(nv, cap * 4, ctx))) return rc;
+    if (ctx->train_n) {
+      VCT_HIP(hipMemcpyAsync(nx.p, ctx->train_X.p, ctx->train_n * 128, hipMemcpyDeviceToDevice, ctx->stream));
+      VCT_HIP(hipMemcpyAsync(na.p, ctx->train_A.p, ctx->train_n * 12, hipMemcpyDeviceToDevice, ctx->stream));
+      VCT_HIP(hipMemcpyAsync(ny.p, ctx->train_Y.p, ctx->train_n * 12, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    VCT_HIP(hipStreamSynchronize(ctx->stream));
+    VctBuf* olds[] = {&ctx->train_X, &ctx->train_A, &ctx->train_Y, &ctx->train_valid};
+    for (VctBuf* b : olds)
+      if (b->p) VCT_HIP(hipFree(b->p));
+    ctx->train_X = nx;
+    ctx->train_A = na;
+    ctx->train_Y = ny;
+    ctx->train_valid = nv;
+    ctx->train_cap = cap;
+  }
+  FeatArgs F = feat_args(ctx);
+  const size_t o = ctx->train_n;
+  hipLaunchKernelGGL(mlp_sample_kernel, dim3((nsamples + 255) / 256), dim3(256), 0, ctx->stream, F,
+                     (const float4*)ctx->full_target.p, (const uint32_t*)ctx->sample_pix.p, nsamples,
+                     (uint16_t*)ctx->train_X.p + o * 64, (float*)ctx->train_A.p + o * 3,
+                     (float*)ctx->train_Y.p + o * 3, (uint32_t*)ctx->train_valid.p + o);
+  VCT_HIP(hipGetLastError());
+  VCT_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->train_n = need;
+  return VCT_OK;
+}
+
+int vct_upsample_train(vct_ctx ctx, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float* last_loss) {
+  MLP_CTX(ctx);
+  int rc;
+  if ((rc = need_mlp(ctx))) return rc;
+  if (ctx->train_n < 64) return vct_set_error(VCT_ERR_STATE, "no training samples (vct_upsample_collect first)");
+  if (batch < 64 || !(lr > 0.0f)) return vct_set_error(VCT_ERR_INVALID_ARG, "batch >= 64 and lr > 0 required");
+  batch = (uint32_t)std::min<size_t>(batch, ctx->train_n) & ~63u;
+  HostRng rng(seed);
+  for (uint32_t s = 0; s < steps; ++s) {
+    const size_t span = ctx->train_n - batch + 1;
+    const size_t off = (size_t)(rng.next() % span);
+    const bool last = s + 1 == steps;
+    if ((rc = mlp_train_on(ctx, (const uint16_t*)ctx->train_X.p + off * 64, (const float*)ctx->train_A.p + off * 3,
+                           (const float*)ctx->train_Y.p + off * 3, batch, lr, last ? last_loss : nullptr, nullptr)))
+      return rc;
+  }
+  return VCT_OK;
+}
+
+// The learned form of Renderer::render: voxelize, mip, full-resolution
+// G-buffer, quarter-resolution cone trace, MLP upsampling to full resolution.
+int vct_render_learned(vct_ctx ctx) {
+  MLP_CTX(ctx);
+  int rc;
+  if ((rc = need_mlp(ctx))) return rc;
+  if ((rc = vct_voxelize(ctx))) return rc;
+  if ((rc = vct_mip_aniso(ctx))) return rc;
+  if ((rc = vct_gbuffer(ctx))) return rc;
+  const size_t npix = (size_t)ctx->width * ctx->height;
+  if ((rc = vct_ensure(ctx->image, npix * 16, ctx))) return rc;
+  VCT_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
+  if ((rc = trace_lowres(ctx))) return rc;
+  VCT_HIP(hipEventRecord(ctx->ev[7], ctx->stream));
+  ctx->ev_recorded[3] = true;
+  VCT_HIP(hipEventRecord(ctx->ev[8], ctx->stream));
+  if ((rc = infer_full(ctx))) return rc;
+  VCT_HIP(hipEventRecord(ctx->ev[9], ctx->stream));
+  ctx->ev_recorded[4] = true;
+  return vct_clear_queues(ctx);
+}
+
+// Quarter-resolution buffer of the last learned frame (tests, tools).
+int vct_upsample_lowres_download(vct_ctx ctx, float* rgba, size_t pixels) {
+  MLP_CTX(ctx);
+  if (!rgba || pixels != (size_t)(ctx->width / 4) * (ctx->height / 4))
+    return vct_set_error(VCT_ERR_INVALID_ARG, "pixel count mismatch");
+  if (!ctx->low_img.p) return vct_set_error(VCT_ERR_STATE, "no low-resolution image yet");
+  VCT_HIP(hipMemcpyAsync(rgba, ctx->low_img.p, pixels * 16, hipMemcpyDeviceToHost, ctx->stream));
+  VCT_HIP(hipStreamSynchronize(ctx->stream));
+  return VCT_OK;
+}
+
+}  // extern "C"

@@ -327,13 +327,19 @@ int vct_destroy(vct_ctx ctx) {
     if (m.d_pos) (void)hipFree(m.d_pos);
     if (m.d_nrm) (void)hipFree(m.d_nrm);
     if (m.d_mat) (void)hipFree(m.d_mat);
+    if (m.d_perm) (void)hipFree(m.d_perm);
+    if (m.d_inv) (void)hipFree(m.d_inv);
   }
   for (auto& t : ctx->textures)
     if (t.dptr) (void)hipFree(t.dptr);
   VctBuf* bufs[] = {&ctx->d_materials, &ctx->volume, &ctx->accum, &ctx->tri_count, &ctx->tri_off,
                     &ctx->tri_base, &ctx->frags, &ctx->frags_sorted, &ctx->vox_count, &ctx->vox_off,
                     &ctx->scan_tmp, &ctx->counters, &ctx->setup, &ctx->vis, &ctx->g_pos,
-                    &ctx->g_nrm, &ctx->image, &ctx->g_count, &ctx->g_off};
+                    &ctx->g_nrm, &ctx->image, &ctx->g_count, &ctx->g_off, &ctx->g_recs,
+                    &ctx->mlp_params, &ctx->mlp_m1, &ctx->mlp_m2, &ctx->mlp_packed, &ctx->mlp_bias,
+                    &ctx->mlp_grad, &ctx->mlp_partial, &ctx->low_gpos, &ctx->low_gnrm, &ctx->low_img,
+                    &ctx->full_target, &ctx->sample_pix, &ctx->train_X, &ctx->train_A, &ctx->train_Y,
+                    &ctx->train_valid};
   for (VctBuf* b : bufs) free_buf(*b);
   for (int i = 0; i < 12; ++i)
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
@@ -463,12 +469,53 @@ int vct_model_create(vct_ctx ctx, const float* positions, const float* normals, 
   }
   size_t b9 = (size_t)ntri * 9 * sizeof(float);
   if (ntri) {
+    // Morton order of the centroids (10 bits per axis over the centroid box)
+    std::vector<std::pair<uint32_t, uint32_t>> key(ntri);
+    float cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    std::vector<float> cen((size_t)ntri * 3);
+    for (uint32_t t = 0; t < ntri; ++t)
+      for (int k = 0; k < 3; ++k) {
+        const float c = (positions[t * 9 + k] + positions[t * 9 + 3 + k] + positions[t * 9 + 6 + k]) / 3.0f;
+        cen[(size_t)t * 3 + k] = c;
+        cmn[k] = std::min(cmn[k], c);
+        cmx[k] = std::max(cmx[k], c);
+      }
+    for (uint32_t t = 0; t < ntri; ++t) {
+      uint32_t q[3];
+      for (int k = 0; k < 3; ++k) {
+        const float ext = cmx[k] - cmn[k];
+        float f = ext > 0.0f ? (cen[(size_t)t * 3 + k] - cmn[k]) / ext : 0.0f;
+        f = std::isfinite(f) ? std::min(std::max(f, 0.0f), 1.0f) : 0.0f;
+        q[k] = (uint32_t)(f * 1023.0f);
+      }
+      key[t] = {vct::morton3(q[0], q[1], q[2]), t};
+    }
+    std::stable_sort(key.begin(), key.end(),
+                     [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
+                       return a.first < b.first;
+                     });
+    m.perm.resize(ntri);
+    std::vector<uint32_t> inv(ntri), mat_s(ntri);
+    std::vector<float> pos_s(b9 / 4), nrm_s(b9 / 4);
+    for (uint32_t s = 0; s < ntri; ++s) {
+      const uint32_t o = key[s].second;
+      m.perm[s] = o;
+      inv[o] = s;
+      mat_s[s] = m.h_mat[o];
+      memcpy(&pos_s[(size_t)s * 9], positions + (size_t)o * 9, 36);
+      memcpy(&nrm_s[(size_t)s * 9], normals + (size_t)o * 9, 36);
+    }
+    m.h_mat = mat_s;
     VCT_HIP(hipMalloc(&m.d_pos, b9));
     VCT_HIP(hipMalloc(&m.d_nrm, b9));
     VCT_HIP(hipMalloc(&m.d_mat, (size_t)ntri * 4));
-    VCT_HIP(hipMemcpyAsync(m.d_pos, positions, b9, hipMemcpyHostToDevice, ctx->stream));
-    VCT_HIP(hipMemcpyAsync(m.d_nrm, normals, b9, hipMemcpyHostToDevice, ctx->stream));
+    VCT_HIP(hipMalloc(&m.d_perm, (size_t)ntri * 4));
+    VCT_HIP(hipMalloc(&m.d_inv, (size_t)ntri * 4));
+    VCT_HIP(hipMemcpyAsync(m.d_pos, pos_s.data(), b9, hipMemcpyHostToDevice, ctx->stream));
+    VCT_HIP(hipMemcpyAsync(m.d_nrm, nrm_s.data(), b9, hipMemcpyHostToDevice, ctx->stream));
     VCT_HIP(hipMemcpyAsync(m.d_mat, m.h_mat.data(), (size_t)ntri * 4, hipMemcpyHostToDevice, ctx->stream));
+    VCT_HIP(hipMemcpyAsync(m.d_perm, m.perm.data(), (size_t)ntri * 4, hipMemcpyHostToDevice, ctx->stream));
+    VCT_HIP(hipMemcpyAsync(m.d_inv, inv.data(), (size_t)ntri * 4, hipMemcpyHostToDevice, ctx->stream));
     VCT_HIP(hipStreamSynchronize(ctx->stream));
   }
   ctx->models.push_back(std::move(m));

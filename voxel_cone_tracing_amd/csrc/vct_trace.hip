@@ -53,6 +53,7 @@ struct TraceArgs {
   int view_dir;
   float view_lod;
   int demodulate;  // 1: write final / max(Kd, eps) (the learned path's target)
+  unsigned long long* step_counter;  // optional: cone steps taken (diagnostics)
   ConeSet cones;
   LightSet lights;
 };
@@ -100,9 +101,9 @@ __device__ __forceinline__ float4 operator*(float s, float4 a) {
 }
 
 // Separable parts of brick_index: index(x,y,z) = ix(x) | iy(y) | iz(z).
-__device__ __forceinline__ uint32_t part_x(uint32_t x) { return (spread3(x >> 2) << 6) | (x & 3u); }
-__device__ __forceinline__ uint32_t part_y(uint32_t y) { return (spread3(y >> 2) << 7) | ((y & 3u) << 2); }
-__device__ __forceinline__ uint32_t part_z(uint32_t z) { return (spread3(z >> 2) << 8) | ((z & 3u) << 4); }
+__device__ __forceinline__ uint32_t part_x(uint32_t x) { return spread3(x); }
+__device__ __forceinline__ uint32_t part_y(uint32_t y) { return spread3(y) << 1; }
+__device__ __forceinline__ uint32_t part_z(uint32_t z) { return spread3(z) << 2; }
 
 struct Corners {
   uint32_t ix[2], iy[2], iz[2];
@@ -193,7 +194,8 @@ __device__ __forceinline__ float len3(float3 v) { return sqrtf(dot3(v, v)); }
 
 // trace_cone (voxel_cone_tracing.frag:88-119)
 template <int DT>
-__device__ float4 trace_cone(const TraceArgs& A, float3 origin, float3 dir, float aperture, float max_dist) {
+__device__ float4 trace_cone(const TraceArgs& A, float3 origin, float3 dir, float aperture, float max_dist,
+                             uint32_t& steps) {
   dir = normalize3(dir);
   const int mx = dir.x < 0.0f ? 0 : 1, my = dir.y < 0.0f ? 2 : 3, mz = dir.z < 0.0f ? 4 : 5;
   const float3 ad = make_float3(fabsf(dir.x), fabsf(dir.y), fabsf(dir.z));
@@ -211,6 +213,7 @@ __device__ float4 trace_cone(const TraceArgs& A, float3 origin, float3 dir, floa
     const float4 s = sample_voxel<DT>(A, sp, ad, mx, my, mz, mip);
     c = c + (1.0f - c.w) * s;
     dist += fmaxf(diam * 0.5f, vs);
+    ++steps;
     diam = dist * aperture;
   }
   return c;
@@ -273,25 +276,89 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
   const float3 cam = make_float3(A.cam[0], A.cam[1], A.cam[2]);
   const float3 view_dir = normalize3(make_float3(world.x - cam.x, world.y - cam.y, world.z - cam.z));
   const float sqrt3 = 1.73205080757f;
+  uint32_t steps = 0;
   float3 out = make_float3(0.f, 0.f, 0.f);
-  if (A.en_diffuse) {
-    const float3 o1 = normalize3(tangent_of(n));
-    const float3 o2 = normalize3(cross3(o1, n));
-    float4 acc = f4(0.0f);
-    for (int k = 0; k < A.cones.n; ++k) {
-      const float a = A.cones.a[k], b = A.cones.b[k], c = A.cones.c[k];
-      const float3 d = make_float3(a * n.x + b * o1.x + c * o2.x, a * n.y + b * o1.y + c * o2.y,
-                                   a * n.z + b * o1.z + c * o2.z);
-      acc = acc + trace_cone<DT>(A, pos, d, A.cones.aperture, sqrt3);
-    }
-    const float inv = 1.0f / (float)A.cones.n;
-    out.x += m.diffuse[0] * (acc.x * inv);
-    out.y += m.diffuse[1] * (acc.y * inv);
-    out.z += m.diffuse[2] * (acc.z * inv);
-  }
   // specular aperture (:220-227)
   const float rough = sqrtf(2.0f / (m.shininess + 2.0f));
   const float spec_ap = fminf(fmaxf(tanf(1.57079f * rough), 0.0174533f), 3.14159265f);
+  // All cones of the pixel in one loop: slots [0, nd) diffuse, nd specular,
+  // nd + 1 refraction.  A lane that finishes a cone starts its next one in
+  // the same iteration, so the wave runs for the longest lane's total march,
+  // not for the sum over cones of each cone's longest lane.
+  const int nd = A.en_diffuse ? A.cones.n : 0;
+  const bool do_spec = A.en_specular != 0;
+  const bool do_refr = do_spec && illum_transmissive(m.illum);
+  const int kend = nd + 2;
+  float3 o1 = make_float3(0.f, 0.f, 0.f), o2 = o1;
+  if (nd > 0) {
+    o1 = normalize3(tangent_of(n));  // trace_diffuse (:146-147)
+    o2 = normalize3(cross3(o1, n));
+  }
+  float3 diff_acc = make_float3(0.f, 0.f, 0.f), spec_rgb = diff_acc, refr_rgb = diff_acc;
+  auto next_slot = [&](int k) {
+    ++k;
+    if (k == nd && !do_spec) ++k;
+    if (k == nd + 1 && !do_refr) ++k;
+    return k;
+  };
+  const float vs = 1.0f / (float)A.res;
+  const float fres = (float)A.res;
+  int k = next_slot(-1);
+  float3 dir = make_float3(0.f, 0.f, 1.f), ad = dir;
+  float ap = 0.f, maxd = 0.f, dist = 0.f, diam = 0.f;
+  int mx = 1, my = 3, mz = 5;
+  float4 c = f4(0.0f);
+  auto setup = [&](int kk) {
+    float3 d;
+    if (kk < nd) {
+      const float a = A.cones.a[kk], b = A.cones.b[kk], cc = A.cones.c[kk];
+      d = make_float3(a * n.x + b * o1.x + cc * o2.x, a * n.y + b * o1.y + cc * o2.y, a * n.z + b * o1.z + cc * o2.z);
+      ap = A.cones.aperture;
+    } else if (kk == nd) {
+      // normalize(reflect(-view_dir, n)) (:231)
+      const float3 I = make_float3(-view_dir.x, -view_dir.y, -view_dir.z);
+      const float dn = 2.0f * dot3(n, I);
+      d = normalize3(make_float3(I.x - dn * n.x, I.y - dn * n.y, I.z - dn * n.z));
+      ap = spec_ap;
+    } else {
+      d = refract3(view_dir, n, 1.0f / m.ior);  // trace_refraction (:237-241)
+      ap = spec_ap;
+    }
+    dir = normalize3(d);  // trace_cone (:90)
+    mx = dir.x < 0.0f ? 0 : 1;
+    my = dir.y < 0.0f ? 2 : 3;
+    mz = dir.z < 0.0f ? 4 : 5;
+    ad = make_float3(fabsf(dir.x), fabsf(dir.y), fabsf(dir.z));
+    maxd = sqrt3;
+    c = f4(0.0f);
+    dist = 3.0f * vs;
+    diam = dist * ap;
+  };
+  if (k < kend) setup(k);
+  while (k < kend) {
+    if (c.w < 1.0f && dist < maxd) {
+      const float3 sp = make_float3(dir.x * dist + pos.x, dir.y * dist + pos.y, dir.z * dist + pos.z);
+      const float mip = fmaxf(log2f(diam * fres), 0.0f);
+      const float4 s = sample_voxel<DT>(A, sp, ad, mx, my, mz, mip);
+      c = c + (1.0f - c.w) * s;
+      dist += fmaxf(diam * 0.5f, vs);
+      diam = dist * ap;
+      ++steps;
+    } else {
+      const float3 r = make_float3(c.x, c.y, c.z);
+      if (k < nd) diff_acc = make_float3(diff_acc.x + r.x, diff_acc.y + r.y, diff_acc.z + r.z);
+      else if (k == nd) spec_rgb = r;
+      else refr_rgb = r;
+      k = next_slot(k);
+      if (k < kend) setup(k);
+    }
+  }
+  if (nd > 0) {
+    const float inv = 1.0f / (float)nd;
+    out.x += m.diffuse[0] * (diff_acc.x * inv);
+    out.y += m.diffuse[1] * (diff_acc.y * inv);
+    out.z += m.diffuse[2] * (diff_acc.z * inv);
+  }
   if (A.en_direct) {
     float3 res = make_float3(0.f, 0.f, 0.f);
     const int nl = A.lights.count < VCT_MAX_LIGHTS ? A.lights.count : VCT_MAX_LIGHTS;
@@ -315,7 +382,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
       const float3 lc = make_float3(k * L.color[0] * L.intensity, k * L.color[1] * L.intensity,
                                     k * L.color[2] * L.intensity);
       float shadow = 1.0f;
-      if (A.en_shadow) shadow = fmaxf(0.0f, 1.0f - trace_cone<DT>(A, pos, ldir, 0.1f, d).w);
+      if (A.en_shadow) shadow = fmaxf(0.0f, 1.0f - trace_cone<DT>(A, pos, ldir, 0.1f, d, steps).w);
       const float lambert = fmaxf(dot3(ldir, n), 0.0f);
       float refr_angle = 0.0f;
       if (m.dissolve <= 0.1f) {
@@ -335,29 +402,26 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
     out.y += res.y + fminf(fmaxf(m.emission[1], 0.0f), 1.0f);
     out.z += res.z + fminf(fmaxf(m.emission[2], 0.0f), 1.0f);
   }
-  if (A.en_specular) {
-    // normalize(reflect(-view_dir, n)) (:231)
-    const float3 I = make_float3(-view_dir.x, -view_dir.y, -view_dir.z);
-    const float dn = 2.0f * dot3(n, I);
-    const float3 r = make_float3(I.x - dn * n.x, I.y - dn * n.y, I.z - dn * n.z);
-    const float4 s = trace_cone<DT>(A, pos, normalize3(r), spec_ap, sqrt3);
-    out.x += m.specular[0] * s.x;
-    out.y += m.specular[1] * s.y;
-    out.z += m.specular[2] * s.z;
+  if (do_spec) {
+    out.x += m.specular[0] * spec_rgb.x;
+    out.y += m.specular[1] * spec_rgb.y;
+    out.z += m.specular[2] * spec_rgb.z;
   }
-  if (illum_transmissive(m.illum) && A.en_specular) {
-    const float3 rd = refract3(view_dir, n, 1.0f / m.ior);
-    const float4 s = trace_cone<DT>(A, pos, rd, spec_ap, sqrt3);
-    const float3 tr = make_float3(m.transmittance[0] * s.x, m.transmittance[1] * s.y, m.transmittance[2] * s.z);
+  if (do_refr) {  // final colour override (:273-274)
+    const float3 tr = make_float3(m.transmittance[0] * refr_rgb.x, m.transmittance[1] * refr_rgb.y,
+                                  m.transmittance[2] * refr_rgb.z);
     const float dv = m.dissolve;
     out = make_float3(tr.x * (1.0f - dv) + out.x * dv, tr.y * (1.0f - dv) + out.y * dv,
                       tr.z * (1.0f - dv) + out.z * dv);
   }
+  float ow = 1.0f;
   if (A.demodulate) {
     const float e = 1.0f / 1024.0f;
     out = make_float3(out.x / fmaxf(m.diffuse[0], e), out.y / fmaxf(m.diffuse[1], e), out.z / fmaxf(m.diffuse[2], e));
+    ow = gp.w;  // linear depth of the traced pixel
   }
-  A.image[idx] = make_float4(out.x, out.y, out.z, 1.0f);
+  A.image[idx] = make_float4(out.x, out.y, out.z, ow);
+  if (A.step_counter) atomicAdd(A.step_counter, (unsigned long long)steps);
 }
 
 }  // namespace vct
@@ -412,9 +476,10 @@ void vct_cone_set_table(int set, ConeSet& cs) {
 }
 
 int vct_trace_into(vct_context* ctx, const float4* gpos, const float4* gnrm, float4* out, int W, int H,
-                   int demodulate) {
+                   int demodulate, unsigned long long* step_counter) {
   TraceArgs A;
   memset(&A, 0, sizeof(A));
+  A.step_counter = step_counter;
   A.vol = (const char*)ctx->volume.p;
   for (int l = 0; l < ctx->levels; ++l)
     for (int d = 0; d < 6; ++d) A.off[l][d] = ctx->lvl_off[l][d];
@@ -469,5 +534,27 @@ int vct_launch_trace(vct_context* ctx) {
   int rc;
   if ((rc = vct_ensure(ctx->image, npix * 16, ctx))) return rc;
   return vct_trace_into(ctx, (const float4*)ctx->g_pos.p, (const float4*)ctx->g_nrm.p, (float4*)ctx->image.p,
-                        ctx->width, ctx->height, 0);
+                        ctx->width, ctx->height, 0, nullptr);
+}
+
+extern "C" int vct_trace_count_steps(vct_ctx ctx, uint64_t* steps) {
+  if (!ctx) return vct_set_error(VCT_ERR_BAD_HANDLE, "null context");
+  if (!steps) return vct_set_error(VCT_ERR_INVALID_ARG, "null out");
+  VCT_HIP(hipSetDevice(ctx->device));
+  const size_t npix = (size_t)ctx->width * ctx->height;
+  if (!ctx->g_pos.p || ctx->g_pos.cap < npix * 16)
+    return vct_set_error(VCT_ERR_STATE, "trace needs a G-buffer of the current viewport (run vct_gbuffer first)");
+  int rc;
+  if ((rc = vct_ensure(ctx->image, npix * 16, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->counters, 256, ctx))) return rc;
+  unsigned long long* cnt = (unsigned long long*)((char*)ctx->counters.p + 64);
+  VCT_HIP(hipMemsetAsync(cnt, 0, 8, ctx->stream));
+  if ((rc = vct_trace_into(ctx, (const float4*)ctx->g_pos.p, (const float4*)ctx->g_nrm.p, (float4*)ctx->image.p,
+                           ctx->width, ctx->height, 0, cnt)))
+    return rc;
+  unsigned long long h = 0;
+  VCT_HIP(hipMemcpyAsync(&h, cnt, 8, hipMemcpyDeviceToHost, ctx->stream));
+  VCT_HIP(hipStreamSynchronize(ctx->stream));
+  *steps = h;
+  return VCT_OK;
 }

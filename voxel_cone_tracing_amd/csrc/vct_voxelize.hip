@@ -40,7 +40,9 @@ struct Draw {
   const float* pos;
   const float* nrm;
   const uint32_t* mat;
-  uint32_t tri_base;  // first triangle of this draw in the batch
+  const uint32_t* perm;  // stored -> original triangle index
+  uint32_t tri_base;    // first triangle of this draw in the batch
+  uint32_t canon_base;  // original index of the draw's first triangle in the frame
   uint32_t ntri;
   float M[16];  // model matrix, column-major
   float N[9];   // mat3(transpose(inverse(model))), column-major
@@ -59,7 +61,7 @@ struct VoxParams {
   float inv_unused;
   float cube_size;
   int average;   // VCT_AVG_*
-  uint32_t cand_base;  // canonical index of this batch's first candidate
+  uint32_t unused;
   LightSet lights;
 };
 
@@ -72,6 +74,20 @@ struct Setup {
   bool swap;
   bool ok;
 };
+
+// Per-triangle record written once by the count pass and read by every
+// candidate lane of the triangle (the lanes of one triangle share its lines).
+struct alignas(16) TriRec {
+  int64_t area;     // doubled signed area in 1/256 pixel units, made positive
+  int32_t X[3], Y[3];
+  int32_t qx0, qy0, qw;
+  uint32_t mat;
+  uint32_t swap;
+  float w[3][3];    // world position / cube_size per vertex
+  float n[3][3];    // normalised transformed normal per vertex
+  uint32_t otri;   // original triangle index in the frame (canonical order)
+};
+static_assert(sizeof(TriRec) == 128, "TriRec must be 128 bytes");
 
 __device__ __forceinline__ int find_draw(const DrawBatch& b, uint32_t t) {
   int d = 0;
@@ -145,22 +161,53 @@ __device__ __forceinline__ void tri_setup(const Draw& dr, uint32_t lt, uint32_t 
   }
 }
 
-__global__ __launch_bounds__(256) void vox_count_kernel(DrawBatch b, VoxParams vp, uint32_t* counts) {
+__global__ __launch_bounds__(256) void vox_count_kernel(DrawBatch b, VoxParams vp, uint32_t* counts,
+                                                        TriRec* __restrict__ recs) {
+#pragma clang fp contract(off)
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= b.ntri) return;
   const int d = find_draw(b, t);
+  const Draw& dr = b.d[d];
+  const uint32_t lt = t - dr.tri_base;
   Setup s;
-  tri_setup(b.d[d], t - b.d[d].tri_base, vp.res, vp.cube_size, s);
+  tri_setup(dr, lt, vp.res, vp.cube_size, s);
   counts[t] = s.ok ? (uint32_t)(4 * s.qw * s.qh) : 0u;
+  if (!s.ok) return;
+  TriRec r;
+  r.area = s.area;
+  for (int v = 0; v < 3; ++v) {
+    r.X[v] = (int32_t)s.X[v];
+    r.Y[v] = (int32_t)s.Y[v];
+    for (int k = 0; k < 3; ++k) r.w[v][k] = s.w[v][k];
+  }
+  r.qx0 = s.qx0;
+  r.qy0 = s.qy0;
+  r.qw = s.qw;
+  r.mat = dr.mat[lt];
+  r.swap = s.swap ? 1u : 0u;
+  r.otri = dr.canon_base + dr.perm[lt];
+  // normal: voxelize.vert:28 per vertex (normalize(N * n))
+  const float* nr = dr.nrm + (size_t)lt * 9;
+  for (int v = 0; v < 3; ++v) {
+    float tt[3];
+    for (int k = 0; k < 3; ++k) {
+      float a = dr.N[0 + k] * nr[v * 3 + 0];
+      a = a + dr.N[3 + k] * nr[v * 3 + 1];
+      a = a + dr.N[6 + k] * nr[v * 3 + 2];
+      tt[k] = a;
+    }
+    const float l = sqrtf(tt[0] * tt[0] + tt[1] * tt[1] + tt[2] * tt[2]);
+    for (int k = 0; k < 3; ++k) r.n[v][k] = tt[k] / l;
+  }
+  recs[t] = r;
 }
 
 __device__ __forceinline__ bool top_left(int64_t dx, int64_t dy) { return dy < 0 || (dy == 0 && dx < 0); }
 
 // Fragment of candidate `lc` of triangle `lt`: coverage, voxel and colour.
 // Returns false if the pixel is not covered or the voxel falls outside.
-__device__ __forceinline__ bool shade_fragment(const Draw& dr, uint32_t lt, const Setup& s, uint32_t lc,
-                                               const VoxParams& vp, const DevMaterial* mats, uint32_t& key,
-                                               float4& color) {
+__device__ __forceinline__ bool shade_fragment(const TriRec& s, uint32_t lc, const VoxParams& vp,
+                                               const DevMaterial* mats, uint32_t& key, float4& color) {
 #pragma clang fp contract(off)
   const uint32_t quad = lc >> 2, sub = lc & 3u;
   const int i = s.qx0 + 2 * (int)(quad % (uint32_t)s.qw) + (int)(sub & 1u);
@@ -169,8 +216,10 @@ __device__ __forceinline__ bool shade_fragment(const Draw& dr, uint32_t lt, cons
   if (i >= lim || j >= lim) return false;
   const int64_t px = (int64_t)i * 256, py = (int64_t)j * 256;
   // CCW vertex order (a, b, c) = (0, 1, 2) or (0, 2, 1)
-  const int ia = 0, ib = s.swap ? 2 : 1, ic = s.swap ? 1 : 2;
-  const int64_t ax = s.X[ia], ay = s.Y[ia], bx = s.X[ib], by = s.Y[ib], cx = s.X[ic], cy = s.Y[ic];
+  const bool sw = s.swap != 0u;
+  const int64_t ax = s.X[0], ay = s.Y[0];
+  const int64_t bx = sw ? s.X[2] : s.X[1], by = sw ? s.Y[2] : s.Y[1];
+  const int64_t cx = sw ? s.X[1] : s.X[2], cy = sw ? s.Y[1] : s.Y[2];
   // weights of a, b, c
   int64_t wa = (cx - bx) * (py - by) - (cy - by) * (px - bx);
   int64_t wb = (ax - cx) * (py - cy) - (ay - cy) * (px - cx);
@@ -179,12 +228,10 @@ __device__ __forceinline__ bool shade_fragment(const Draw& dr, uint32_t lt, cons
   if (wa == 0 && !top_left(cx - bx, cy - by)) return false;
   if (wb == 0 && !top_left(ax - cx, ay - cy)) return false;
   if (wc == 0 && !top_left(bx - ax, by - ay)) return false;
-  int64_t w[3];
-  w[ia] = wa;
-  w[ib] = wb;
-  w[ic] = wc;
+  // weights of the original vertices 0, 1, 2
+  const int64_t w0 = wa, w1 = sw ? wc : wb, w2 = sw ? wb : wc;
   const float fa = (float)s.area;
-  const float b0 = (float)w[0] / fa, b1 = (float)w[1] / fa, b2 = (float)w[2] / fa;
+  const float b0 = (float)w0 / fa, b1 = (float)w1 / fa, b2 = (float)w2 / fa;
   float pos[3];
   for (int k = 0; k < 3; ++k) pos[k] = b0 * s.w[0][k] + b1 * s.w[1][k] + b2 * s.w[2][k];
   // voxel address, voxelize.frag:156-157 (truncation toward zero)
@@ -196,23 +243,10 @@ __device__ __forceinline__ bool shade_fragment(const Draw& dr, uint32_t lt, cons
     v[k] = (int)f;
   }
   key = brick_index((uint32_t)v[0], (uint32_t)v[1], (uint32_t)v[2]);
-  // normal: voxelize.vert:28 per vertex, interpolated, normalised in the
+  // normal: per-vertex normals (record), interpolated, normalised in the
   // fragment shader (voxelize.frag:134)
-  const float* nr = dr.nrm + (size_t)lt * 9;
-  float nv[3][3];
-  for (int vv = 0; vv < 3; ++vv) {
-    float t[3];
-    for (int k = 0; k < 3; ++k) {
-      float a = dr.N[0 + k] * nr[vv * 3 + 0];
-      a = a + dr.N[3 + k] * nr[vv * 3 + 1];
-      a = a + dr.N[6 + k] * nr[vv * 3 + 2];
-      t[k] = a;
-    }
-    float l = sqrtf(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-    for (int k = 0; k < 3; ++k) nv[vv][k] = t[k] / l;
-  }
   float n[3];
-  for (int k = 0; k < 3; ++k) n[k] = b0 * nv[0][k] + b1 * nv[1][k] + b2 * nv[2][k];
+  for (int k = 0; k < 3; ++k) n[k] = b0 * s.n[0][k] + b1 * s.n[1][k] + b2 * s.n[2][k];
   {
     float l = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
     for (int k = 0; k < 3; ++k) n[k] = n[k] / l;
@@ -240,7 +274,7 @@ __device__ __forceinline__ bool shade_fragment(const Draw& dr, uint32_t lt, cons
     float ca = c * att;
     for (int k = 0; k < 3; ++k) col[k] = col[k] + ca * L.color[k] * L.intensity;
   }
-  const DevMaterial& m = mats[dr.mat[lt]];
+  const DevMaterial& m = mats[s.mat];
   float tr[3] = {1.0f, 1.0f, 1.0f};
   float alpha = 1.0f;
   if (illum_transmissive(m.illum)) {
@@ -266,7 +300,7 @@ struct RasterOut {
   uint32_t* vcount;          // fragments per voxel
   // reference mode
   uint32_t* f_key;
-  uint32_t* f_cand;
+  unsigned long long* f_cand;  // canonical key: original triangle << 32 | candidate
   float4* f_color;
   uint32_t* f_counter;
   uint32_t f_cap;
@@ -274,12 +308,13 @@ struct RasterOut {
 
 __device__ __forceinline__ uint32_t fixq(float c) { return (uint32_t)(c * 65536.0f + 0.5f); }
 
-__global__ __launch_bounds__(256) void vox_raster_kernel(DrawBatch b, VoxParams vp, const uint32_t* __restrict__ off,
+__global__ __launch_bounds__(256) void vox_raster_kernel(uint32_t ntri, VoxParams vp, const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ d_total,
+                                                         const TriRec* __restrict__ recs,
                                                          const DevMaterial* __restrict__ mats, RasterOut o) {
   __shared__ uint32_t s_range[2];
   const uint32_t total = *d_total;
-  const uint32_t n = b.ntri;
+  const uint32_t n = ntri;
   const int lane = threadIdx.x & 63;
   for (uint32_t base = blockIdx.x * 256u; base < total; base += gridDim.x * 256u) {
     const uint32_t last = min(base + 255u, total - 1u);
@@ -297,6 +332,7 @@ __global__ __launch_bounds__(256) void vox_raster_kernel(DrawBatch b, VoxParams 
     const uint32_t c = base + threadIdx.x;
     bool have = false;
     uint32_t key = 0xffffffffu;
+    unsigned long long ckey = 0ull;
     float4 color = make_float4(0.f, 0.f, 0.f, 0.f);
     if (c < total) {
       uint32_t lo = s_range[0], hi = s_range[1] + 1;
@@ -306,11 +342,9 @@ __global__ __launch_bounds__(256) void vox_raster_kernel(DrawBatch b, VoxParams 
         else hi = mid;
       }
       const uint32_t t = lo;
-      const int d = find_draw(b, t);
-      const Draw& dr = b.d[d];
-      Setup s;
-      tri_setup(dr, t - dr.tri_base, vp.res, vp.cube_size, s);
-      have = shade_fragment(dr, t - dr.tri_base, s, c - off[t], vp, mats, key, color);
+      const TriRec r = recs[t];
+      have = shade_fragment(r, c - off[t], vp, mats, key, color);
+      ckey = ((unsigned long long)r.otri << 32) | (unsigned long long)(c - off[t]);
       if (!have) key = 0xffffffffu;
     }
     if (vp.average == VCT_AVG_EXACT) {
@@ -347,7 +381,7 @@ __global__ __launch_bounds__(256) void vox_raster_kernel(DrawBatch b, VoxParams 
         const uint32_t slot = slot0 + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         if (slot < o.f_cap) {
           o.f_key[slot] = key;
-          o.f_cand[slot] = vp.cand_base + c;
+          o.f_cand[slot] = ckey;
           o.f_color[slot] = color;
         }
         atomicAdd(&o.vcount[key], 1u);
@@ -409,11 +443,12 @@ __device__ __forceinline__ uint32_t dec_nibble(uint32_t m) {
 }
 
 __global__ __launch_bounds__(256) void vox_bucket_kernel(const uint32_t* __restrict__ f_key,
-                                                         const uint32_t* __restrict__ f_cand,
+                                                         const unsigned long long* __restrict__ f_cand,
                                                          const float4* __restrict__ f_color,
                                                          const uint32_t* __restrict__ d_nfrag,
                                                          const uint32_t* __restrict__ vox_off,
-                                                         uint32_t* __restrict__ vcount, uint32_t* __restrict__ s_cand,
+                                                         uint32_t* __restrict__ vcount,
+                                                         unsigned long long* __restrict__ s_cand,
                                                          float4* __restrict__ s_color) {
   const uint32_t nf = *d_nfrag;
   for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
@@ -426,7 +461,8 @@ __global__ __launch_bounds__(256) void vox_bucket_kernel(const uint32_t* __restr
 
 __global__ __launch_bounds__(256) void vox_replay_kernel(const uint32_t* __restrict__ vox_off,
                                                          const uint32_t* __restrict__ d_nfrag,
-                                                         uint32_t* __restrict__ s_cand, float4* __restrict__ s_color,
+                                                         unsigned long long* __restrict__ s_cand,
+                                                         float4* __restrict__ s_color,
                                                          uint32_t* __restrict__ level0, uint32_t nvox) {
 #pragma clang fp contract(off)
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -435,7 +471,7 @@ __global__ __launch_bounds__(256) void vox_replay_kernel(const uint32_t* __restr
   const uint32_t end = (v + 1 < nvox) ? vox_off[v + 1] : *d_nfrag;
   // canonical order: insertion sort of the bucket by candidate index
   for (uint32_t i = beg + 1; i < end; ++i) {
-    const uint32_t kc = s_cand[i];
+    const unsigned long long kc = s_cand[i];
     const float4 kv = s_color[i];
     uint32_t j = i;
     while (j > beg && s_cand[j - 1] > kc) {
@@ -560,7 +596,8 @@ int vct_launch_voxelize(vct_context* ctx) {
   std::vector<uint32_t> q;
   for (uint32_t id : ctx->queue)
     if (id < ctx->models.size() && ctx->models[id].ntri) q.push_back(id);
-  uint64_t cand_base = 0;
+  uint64_t cand_total = 0;
+  uint32_t canon = 0;
   ctx->last_candidates = 0;
   ctx->last_fragments = 0;
   for (size_t qi = 0; qi < q.size(); qi += kMaxDraws) {
@@ -573,7 +610,10 @@ int vct_launch_voxelize(vct_context* ctx) {
       d.pos = m.d_pos;
       d.nrm = m.d_nrm;
       d.mat = m.d_mat;
+      d.perm = m.d_perm;
       d.tri_base = ntri;
+      d.canon_base = canon;
+      canon += m.ntri;
       d.ntri = m.ntri;
       memcpy(d.M, m.xform, 64);
       normal_matrix(m.xform, d.N);
@@ -582,8 +622,9 @@ int vct_launch_voxelize(vct_context* ctx) {
     b.ntri = ntri;
     if ((rc = vct_ensure(ctx->tri_count, (size_t)ntri * 4 + 4, ctx))) return rc;
     if ((rc = vct_ensure(ctx->tri_off, (size_t)ntri * 4 + 4, ctx))) return rc;
+    if ((rc = vct_ensure(ctx->setup, (size_t)ntri * sizeof(TriRec), ctx))) return rc;
     hipLaunchKernelGGL(vox_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
-                       (uint32_t*)ctx->tri_count.p);
+                       (uint32_t*)ctx->tri_count.p, (TriRec*)ctx->setup.p);
     VCT_HIP(hipGetLastError());
     if ((rc = vct_launch_scan_u32(ctx, (uint32_t*)ctx->tri_count.p, (uint32_t*)ctx->tri_off.p, ntri, counters)))
       return rc;
@@ -591,45 +632,47 @@ int vct_launch_voxelize(vct_context* ctx) {
     memset(&o, 0, sizeof(o));
     o.sums = (unsigned long long*)ctx->accum.p;
     o.vcount = (uint32_t*)ctx->vox_count.p;
-    vp.cand_base = (uint32_t)cand_base;
     if (!exact) {
       // candidates bound the fragment count: read it back to size the buffers
       uint32_t total = 0;
       VCT_HIP(hipMemcpyAsync(&total, counters, 4, hipMemcpyDeviceToHost, ctx->stream));
       VCT_HIP(hipStreamSynchronize(ctx->stream));
-      if (cand_base + total > 0xffffffffull) return vct_set_error(VCT_ERR_LIMIT, "more than 2^32 candidates");
+      if (cand_total + total > 0xffffffffull) return vct_set_error(VCT_ERR_LIMIT, "more than 2^32 fragments");
       uint32_t nf_before = 0;
       VCT_HIP(hipMemcpyAsync(&nf_before, counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
       VCT_HIP(hipStreamSynchronize(ctx->stream));
       size_t cap = (size_t)nf_before + total;
-      // grow keeping existing fragments
-      if (cap * 4 > ctx->frags.cap / 6 || !ctx->frags.p) {
+      // grow keeping existing fragments: [key u32][canonical key u64][colour float4] x cap
+      if (cap > ctx->frags.cap / 28 || !ctx->frags.p) {
         VctBuf nb;
-        size_t ncap = (cap + cap / 4 + 1024 + 3) & ~(size_t)3;  // keeps the float4 array 16-byte aligned
-        if ((rc = vct_ensure(nb, ncap * 24, ctx))) return rc;
+        const size_t ncap = (cap + cap / 4 + 1024 + 3) & ~(size_t)3;  // 8- and 16-byte aligned arrays
+        if ((rc = vct_ensure(nb, ncap * 28, ctx))) return rc;
         if (ctx->frags.p && nf_before) {
-          size_t oc = ctx->frags.cap / 24;
-          VCT_HIP(hipMemcpyAsync(nb.p, ctx->frags.p, (size_t)nf_before * 4, hipMemcpyDeviceToDevice, ctx->stream));
-          VCT_HIP(hipMemcpyAsync((uint32_t*)nb.p + ncap, (uint32_t*)ctx->frags.p + oc, (size_t)nf_before * 4,
-                                 hipMemcpyDeviceToDevice, ctx->stream));
-          VCT_HIP(hipMemcpyAsync((float4*)((uint32_t*)nb.p + 2 * ncap), (float4*)((uint32_t*)ctx->frags.p + 2 * oc),
-                                 (size_t)nf_before * 16, hipMemcpyDeviceToDevice, ctx->stream));
+          const size_t oc = ctx->frags.cap / 28;
+          char* ob = (char*)ctx->frags.p;
+          char* nbp = (char*)nb.p;
+          VCT_HIP(hipMemcpyAsync(nbp, ob, (size_t)nf_before * 4, hipMemcpyDeviceToDevice, ctx->stream));
+          VCT_HIP(hipMemcpyAsync(nbp + ncap * 4, ob + oc * 4, (size_t)nf_before * 8, hipMemcpyDeviceToDevice,
+                                 ctx->stream));
+          VCT_HIP(hipMemcpyAsync(nbp + ncap * 12, ob + oc * 12, (size_t)nf_before * 16, hipMemcpyDeviceToDevice,
+                                 ctx->stream));
           VCT_HIP(hipStreamSynchronize(ctx->stream));
         }
         if (ctx->frags.p) VCT_HIP(hipFree(ctx->frags.p));
         ctx->frags = nb;
+        ctx->frags.cap = ncap * 28;
       }
-      const size_t fc = ctx->frags.cap / 24;
+      const size_t fc = ctx->frags.cap / 28;
       o.f_key = (uint32_t*)ctx->frags.p;
-      o.f_cand = (uint32_t*)ctx->frags.p + fc;
-      o.f_color = (float4*)((uint32_t*)ctx->frags.p + 2 * fc);
+      o.f_cand = (unsigned long long*)((char*)ctx->frags.p + fc * 4);
+      o.f_color = (float4*)((char*)ctx->frags.p + fc * 12);
       o.f_counter = counters + 1;
       o.f_cap = (uint32_t)fc;
-      cand_base += total;
+      cand_total += total;
       ctx->last_candidates += total;
     }
-    hipLaunchKernelGGL(vox_raster_kernel, dim3(256 * 8), dim3(256), 0, ctx->stream, b, vp,
-                       (const uint32_t*)ctx->tri_off.p, (const uint32_t*)counters,
+    hipLaunchKernelGGL(vox_raster_kernel, dim3(256 * 8), dim3(256), 0, ctx->stream, ntri, vp,
+                       (const uint32_t*)ctx->tri_off.p, (const uint32_t*)counters, (const TriRec*)ctx->setup.p,
                        (const DevMaterial*)ctx->d_materials.p, o);
     VCT_HIP(hipGetLastError());
   }
@@ -663,21 +706,16 @@ int vct_launch_voxelize(vct_context* ctx) {
   if ((rc = vct_launch_scan_u32(ctx, (uint32_t*)ctx->vox_count.p, (uint32_t*)ctx->vox_off.p, (uint32_t)nvox,
                                 counters + 2)))
     return rc;
-  if ((rc = vct_ensure(ctx->frags_sorted, (size_t)(nfrag + 1) * 20, ctx))) return rc;
-  uint32_t* s_cand = (uint32_t*)ctx->frags_sorted.p;
-  float4* s_color = (float4*)((char*)ctx->frags_sorted.p + (((size_t)(nfrag + 1) * 4 + 15) & ~(size_t)15));
-  if (((size_t)(nfrag + 1) * 4 + 15) / 16 * 16 + (size_t)(nfrag + 1) * 16 > ctx->frags_sorted.cap) {
-    if ((rc = vct_ensure(ctx->frags_sorted, ((size_t)(nfrag + 1) * 4 + 15) / 16 * 16 + (size_t)(nfrag + 1) * 16, ctx)))
-      return rc;
-    s_cand = (uint32_t*)ctx->frags_sorted.p;
-    s_color = (float4*)((char*)ctx->frags_sorted.p + (((size_t)(nfrag + 1) * 4 + 15) & ~(size_t)15));
-  }
-  const size_t fc = ctx->frags.cap / 24;
+  const size_t ns = ((size_t)nfrag + 4) & ~(size_t)3;
+  if ((rc = vct_ensure(ctx->frags_sorted, ns * 24, ctx))) return rc;
+  unsigned long long* s_cand = (unsigned long long*)ctx->frags_sorted.p;
+  float4* s_color = (float4*)((char*)ctx->frags_sorted.p + ns * 8);
+  const size_t fc = ctx->frags.cap / 28;
   if (nfrag) {
     hipLaunchKernelGGL(vox_bucket_kernel, dim3(2048), dim3(256), 0, ctx->stream, (const uint32_t*)ctx->frags.p,
-                       (const uint32_t*)ctx->frags.p + fc, (const float4*)((uint32_t*)ctx->frags.p + 2 * fc),
-                       (const uint32_t*)(counters + 1), (const uint32_t*)ctx->vox_off.p, (uint32_t*)ctx->vox_count.p,
-                       s_cand, s_color);
+                       (const unsigned long long*)((char*)ctx->frags.p + fc * 4),
+                       (const float4*)((char*)ctx->frags.p + fc * 12), (const uint32_t*)(counters + 1),
+                       (const uint32_t*)ctx->vox_off.p, (uint32_t*)ctx->vox_count.p, s_cand, s_color);
   }
   hipLaunchKernelGGL(vox_replay_kernel, dim3(nb), dim3(256), 0, ctx->stream, (const uint32_t*)ctx->vox_off.p,
                      (const uint32_t*)(counters + 1), s_cand, s_color, (uint32_t*)level0, (uint32_t)nvox);
