@@ -103,3 +103,37 @@ def test_cornell_generator():
     assert mats[2].diffuse[1] > mats[2].diffuse[0]
     pos2, _, _, mats2 = L.cornell(2)
     assert np.array_equal(pos, pos2) and mats2[3].diffuse[0] != mats[3].diffuse[0]
+
+
+def test_renderer_shim_exports_the_reference_entry_points(vctlib):
+    """include/vct_renderer.h: Renderer's methods and the Texture3D functions
+    (renderer.h:124-178, texture_3d.h:6-10) are defined in libvct.so."""
+    import shutil
+    import subprocess
+    if shutil.which("nm") is None:
+        pytest.skip("nm not available")
+    lib = Path(L.__file__).resolve().parent / "libvct.so"
+    out = subprocess.run(["nm", "-DC", "--defined-only", str(lib)], capture_output=True, text=True).stdout
+    for name in ["Renderer::Renderer(int, int)", "Renderer::load_model(char const*)", "Renderer::add_material(",
+                 "Renderer::upload_material_data(", "Renderer::set_model_material(", "Renderer::get_model_dimensions(",
+                 "Renderer::set_grid_resolution(unsigned int)", "Renderer::set_grid_size(float)",
+                 "Renderer::set_camera_transform(", "Renderer::set_model_transform(", "Renderer::get_material(",
+                 "Renderer::set_rendering_phases(bool, bool, bool, bool)", "Renderer::set_voxel_view_dir(int, float)",
+                 "Renderer::queue_model(", "Renderer::queue_point_light(", "Renderer::render()",
+                 "create_tex_3d(int, int, int, int)", "destroy_tex_3d(unsigned int)",
+                 "activate_tex_3d(unsigned int, unsigned int, unsigned int)", "clear_tex_3d(unsigned int, float*)",
+                 "mip_tex_3d(unsigned int)"]:
+        assert name in out, name
+
+
+def test_demo_reports_missing_device():
+    """The headless driver fails loudly (no fallback) when no GPU is visible."""
+    import subprocess
+    import torch
+    demo = Path(L.__file__).resolve().parent / "vct_demo"
+    if not demo.exists():
+        pytest.skip("vct_demo not built")
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    r = subprocess.run([str(demo), "--frames", "1"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "no HIP device" in r.stderr

@@ -19,6 +19,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OUT_DIR = CSRC / "_build"
 LIB = PKG / "libvct.so"
+DEMO = PKG / "vct_demo"
 INCLUDE = PKG.parent / "include"
 
 ARCH = "gfx950"
@@ -67,6 +68,14 @@ def build(verbose: bool = True) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # headless driver of the Renderer API (links libvct.so next to it)
+    demo_src = PKG / "apps" / "vct_demo.cpp"
+    if demo_src.exists():
+        cmd = ["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", str(demo_src), "-o", str(DEMO), f"-L{PKG}", "-lvct",
+               "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"vct_demo build failed:\n{r.stdout}\n{r.stderr}")
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources")
     return LIB
