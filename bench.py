@@ -1,21 +1,30 @@
-"""Benchmark: ms/frame of voxelize + mip + trace at 256^3, 2560x1440.
+"""Benchmark: ms/frame of voxelize + mip + trace at 256^3, 2560x1440, learned
+quarter-resolution form vs plain full-resolution form, with PSNR against the
+FP64 oracle.
 
 One step = one frame of the soup256 workload (1M-triangle soup, 256^3 grid
 with 7 levels, FP16 radiance storage with FP32 accumulation, 2560x1440,
-9 diffuse cones + 1 specular cone, indirect light only): voxelize, anisotropic
-mip, G-buffer, cone trace, all on the GPU through libvct.so.  Geometry is
-uploaded once before timing (resident in HBM).
+9 diffuse cones + 1 specular cone, indirect light only), everything on the
+GPU through libvct.so, geometry resident in HBM before timing:
+  plain   : voxelize, anisotropic mip, G-buffer, full-resolution cone trace;
+  learned : voxelize, mip, G-buffer, 640x360 cone trace, MLP upsampling to
+            2560x1440, per-tile check at probe pixels and full-resolution
+            trace of the 32x32 tiles whose estimated PSNR is below 40 dB.
+`value` is the learned form's ms/frame (the plain form's when no trained
+parameters are found); both forms are reported, each with its PSNR against
+the FP64 oracle (voxelize, mip, G-buffer and trace in double precision) over
+the covered pixels of a 512x512 centre crop.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload soup256]
-                    [--no-cpu-baseline]
+                    [--weights file.npy] [--no-quality] [--no-cpu-baseline]
 
-Prints one JSON line (see the keys below).  --gpus above 1 exits: this build
-runs on one GPU.
+Prints one JSON line.  --gpus above 1 exits: this build runs on one GPU.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -27,6 +36,9 @@ sys.path.insert(0, str(ROOT))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+WEIGHTS = ROOT / "voxel_cone_tracing_amd" / "weights" / "upsampler_soup256.npy"
+PASSES = ["voxelize", "mip", "gbuffer", "trace", "upsample", "fallback"]
 
 
 def log(*a):
@@ -79,12 +91,85 @@ def cpu_baseline(wl, dtype, sample_tri=20000, sample_pix=1500):
     }
 
 
-def mip_bytes(res, levels, bpe):
-    """Compulsory HBM bytes of the fused mip: read level 0 once, write every
-    mip texel of the six maps once."""
-    rd = res ** 3 * bpe
-    wr = sum(6 * (res >> l) ** 3 * bpe for l in range(1, levels))
-    return rd + wr
+def crop_window(wl, size=512):
+    x0 = (wl.width // 2 - size // 2) // 32 * 32
+    y0 = (wl.height // 2 - size // 2) // 32 * 32
+    return x0, y0, size
+
+
+def fp64_reference(sc, wl, crop):
+    """FP64 oracle image (indirect light) of the workload frame on the crop."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle
+    from voxel_cone_tracing_amd.scene import identity, soup_materials_fast
+    from helpers import material_table
+
+    # the oracle's table holds the renderer's default material at 0, the soup's records from 1
+    mats = material_table(soup_materials_fast(sc.albedo_rough))
+    tm = (sc.tri_mat - sc.tri_mat.min() + 1).astype(np.uint32)
+    t0 = time.perf_counter()
+    v = oracle.voxelize(wl.res, wl.cube_size, identity(), sc.pos, sc.nrm, tm, mats, wl.lights, want=("f64",))
+    mips = oracle.mip_f64(wl.res, wl.levels, v["f64"])
+    _, _, p64, n64, _ = oracle.gbuffer(wl.width, wl.height, sc.view, sc.proj, [(identity(), sc.pos, sc.nrm, tm)])
+    x0, y0, s = crop
+    yy, xx = np.mgrid[y0:y0 + s, x0:x0 + s]
+    pix = (yy * wl.width + xx).reshape(-1).astype(np.int64)
+    out, _ = oracle.trace(np.float64, wl.res, wl.levels, v["f64"], mips, p64, n64, mats, wl.lights,
+                          sc.view.reshape(16)[12:15], wl.cube_size, cone_set=wl.cone_set, pixels=pix)
+    cov = (p64[..., 3].reshape(-1)[pix] != 0).reshape(s, s)
+    log(f"bench: FP64 oracle of the {s}x{s} crop in {time.perf_counter() - t0:.1f} s")
+    return out.reshape(s, s, 4), cov
+
+
+def psnr(a, b, cov):
+    e = (a[..., :3].astype(np.float64) - b[..., :3])[cov]
+    mse = float(np.mean(e * e))
+    return (10.0 * math.log10(1.0 / mse) if mse > 0 else 999.0), float(np.abs(e).max())
+
+
+def algorithmic(name, wl, bpe, steps, fb_tiles):
+    """Algorithmic work of one pass: (amount, unit, bound, definition)."""
+    W, H = wl.width, wl.height
+    if name == "voxelize":
+        return 84 * wl.ntri + wl.res ** 3 * bpe, "B", "hbm", "84 B per triangle + level 0 written once"
+    if name == "mip":
+        rd = wl.res ** 3 * bpe + sum(6 * (wl.res >> l) ** 3 * bpe for l in range(1, wl.levels - 1))
+        wr = sum(6 * (wl.res >> l) ** 3 * bpe for l in range(1, wl.levels))
+        return rd + wr, "B", "hbm", "each source level read once, every mip texel written once"
+    if name == "gbuffer":
+        return 76 * wl.ntri + 32 * W * H, "B", "hbm", "76 B per triangle read + 32 B per pixel written"
+    if name == "trace":
+        return 3 * steps * 16 * bpe, "B", "hbm", ("3 quadrilinear samples per cone step x 16 texels x bpp "
+                                                  "(served mostly from L2 / MALL)")
+    if name == "upsample":
+        return 12928 * W * H, "FLOP", "mfma", "12,928 FLOP per output pixel (34-64-64-3 MLP)"
+    return 0, "B", "hbm", ""
+
+
+def run_form(ctx, sc, learned, steps, warmup):
+    def frame():
+        sc.queue()
+        if learned:
+            ctx.render_learned()
+        else:
+            ctx.render()
+
+    for _ in range(warmup):
+        frame()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        frame()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    # per-pass HIP event times (recorded on the context stream), in separate
+    # frames so that the timed loop above has no host synchronisation inside
+    passes = np.zeros(6)
+    for _ in range(steps):
+        frame()
+        passes += ctx.pass_times()
+    passes /= steps
+    return ms, passes
 
 
 def main():
@@ -94,6 +179,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="soup256")
     ap.add_argument("--dtype", default="fp16", choices=["rgba8", "fp16", "bf16", "fp32"])
+    ap.add_argument("--weights", default=str(WEIGHTS))
+    ap.add_argument("--fallback-db", type=float, default=40.0)
+    ap.add_argument("--no-quality", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -106,6 +194,7 @@ def main():
     from voxel_cone_tracing_amd.scene import WORKLOADS, Scene
 
     dt = {"rgba8": L.RGBA8, "fp16": L.FP16, "bf16": L.BF16, "fp32": L.FP32}[args.dtype]
+    bpe = 8 if dt in (L.FP16, L.BF16) else (4 if dt == L.RGBA8 else 16)
     wl = WORKLOADS[args.workload]
     log(f"bench: workload {wl.name} ({wl.ntri} tris, {wl.res}^3, {wl.width}x{wl.height}), dtype {args.dtype}")
     ctx = Context(wl.width, wl.height)
@@ -113,80 +202,119 @@ def main():
     sc = Scene(ctx, wl, dt)
     log(f"bench: scene uploaded in {time.perf_counter() - t0:.1f} s")
 
-    def frame():
-        sc.queue()
-        ctx.render()
-
-    for _ in range(args.warmup):
-        frame()
-    ctx.sync()
-    log("bench: warmup done")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        frame()
-    ctx.sync()
-    elapsed = time.perf_counter() - t0
-    ms = elapsed * 1000.0 / args.steps
-    # per-pass HIP event times (recorded on the context stream), separate
-    # frames so the timed loop above has no host synchronisation inside
-    passes = np.zeros(5)
-    for _ in range(args.steps):
-        frame()
-        passes += ctx.pass_times()
-    passes /= args.steps
-    log(f"bench: {ms:.3f} ms/frame; passes voxelize {passes[0]:.3f} mip {passes[1]:.3f} "
-        f"gbuffer {passes[2]:.3f} trace {passes[3]:.3f} ms")
-    nfrag, ncand = ctx.voxelize_stats()
-    # one more trace with a step counter (not timed): sample count of the march
-    frame()
+    # ---- plain form
+    plain_ms, plain_passes = run_form(ctx, sc, False, args.steps, args.warmup)
+    log(f"bench: plain {plain_ms:.3f} ms/frame; passes " +
+        " ".join(f"{n} {plain_passes[i]:.3f}" for i, n in enumerate(PASSES[:4])))
+    sc.queue()
+    ctx.render()
+    plain_img = ctx.download_image()
+    gp, _ = ctx.download_gbuffer()
+    cov_full = gp[..., 3] != 0
     steps = ctx.trace_steps()
-    samples = 3 * steps  # sample_voxel = 3 textureLod per step
 
-    bpe = 8 if dt in (L.FP16, L.BF16) else (4 if dt == L.RGBA8 else 16)
-    mb = mip_bytes(wl.res, wl.levels, bpe)
-    mip_gbs = mb / (passes[1] * 1e-3) / 1e9 if passes[1] > 0 else 0.0
-    dominant = int(np.argmax(passes[:4]))
-    names = ["voxelize", "mip", "gbuffer", "trace"]
+    # ---- learned form
+    have_weights = Path(args.weights).exists()
+    learned = None
+    if have_weights:
+        ctx.mlp_set_params(np.load(args.weights))
+        ctx.set_fallback(args.fallback_db)
+        l_ms, l_passes = run_form(ctx, sc, True, args.steps, args.warmup)
+        sc.queue()
+        ctx.render_learned()
+        l_img = ctx.download_image()
+        nt, nf = ctx.fallback_stats()
+        ctx.set_fallback(0.0)
+        sc.queue()
+        ctx.render_learned()
+        raw_img = ctx.download_image()
+        ctx.set_fallback(args.fallback_db)
+        learned = dict(ms=l_ms, passes=l_passes, img=l_img, raw=raw_img, tiles=nt, fb=nf)
+        log(f"bench: learned {l_ms:.3f} ms/frame; passes " +
+            " ".join(f"{n} {l_passes[i]:.3f}" for i, n in enumerate(PASSES)) + f"; fallback tiles {nf}/{nt}")
+    else:
+        log(f"bench: no upsampler parameters at {args.weights}: plain form only")
+
+    # ---- quality
+    quality = {}
+    if learned is not None:
+        quality["learned_vs_plain_full_frame_db"] = round(psnr(learned["img"], plain_img, cov_full)[0], 2)
+        quality["learned_no_fallback_vs_plain_full_frame_db"] = round(psnr(learned["raw"], plain_img, cov_full)[0], 2)
+    if not args.no_quality and wl.name.startswith("soup"):
+        crop = crop_window(wl)
+        ref, cov = fp64_reference(sc, wl, crop)
+        x0, y0, s = crop
+        sl = (slice(y0, y0 + s), slice(x0, x0 + s))
+        p, m = psnr(plain_img[sl], ref, cov)
+        quality["crop"] = f"{s}x{s} at ({x0},{y0}), {int(cov.sum())} covered pixels"
+        quality["plain_vs_fp64_db"] = round(p, 2)
+        quality["plain_vs_fp64_max_abs"] = float(f"{m:.3e}")
+        if learned is not None:
+            p, m = psnr(learned["img"][sl], ref, cov)
+            quality["learned_vs_fp64_db"] = round(p, 2)
+            quality["learned_vs_fp64_max_abs"] = float(f"{m:.3e}")
+            p, _ = psnr(learned["raw"][sl], ref, cov)
+            quality["learned_no_fallback_vs_fp64_db"] = round(p, 2)
+
+    # ---- roofline of the dominant kernel of the reported form
+    form_passes = learned["passes"] if learned is not None else plain_passes
+    fb_tiles = learned["fb"] if learned is not None else 0
+    dom = int(np.argmax(form_passes))
+    name = PASSES[dom]
+    if name == "fallback":
+        name = "trace"  # the fallback pass is the cone marcher on the selected tiles
+    amount, unit, bound, definition = algorithmic(name, wl, bpe, steps, fb_tiles)
+    t = float(form_passes[dom]) * 1e-3
+    if PASSES[dom] == "fallback" and learned is not None:
+        amount = amount * (fb_tiles / max(1, learned["tiles"]))
+        definition += ", scaled to the re-traced tiles"
+    if bound == "mfma":
+        achieved, peak, u = amount / t / 1e12, BF16_PEAK_TFLOPS, "TFLOP/s"
+    else:
+        achieved, peak, u = amount / t / 1e9, HBM_PEAK_GBS, "GB/s"
+    kernels = {"voxelize": "vox_brick_kernel", "mip": "mip_kernel", "gbuffer": "g_tile_kernel",
+               "trace": "trace_kernel", "upsample": "mlp_infer_kernel"}
     roofline = {
-        "kernel": "mip_fused_kernel (anisotropic mip chain)",
-        "bound": "hbm",
-        "achieved": round(mip_gbs, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(mip_gbs / HBM_PEAK_GBS, 4),
-        "traffic": None,
-        "note": f"dominant pass of the frame is {names[dominant]} ({passes[dominant]:.3f} ms)",
+        "kernel": kernels.get(name, name), "pass": PASSES[dom], "bound": bound, "achieved": round(achieved, 1),
+        "peak": peak, "unit": u, "frac": round(achieved / peak, 4), "traffic": None,
+        "algorithmic": definition, "pass_ms": round(float(form_passes[dom]), 4),
     }
+    value = learned["ms"] if learned is not None else plain_ms
     out = {
-        "metric": "ms/frame voxelize+mip+trace @256^3,1440p",
-        "value": round(ms, 4),
+        "metric": "ms/frame voxelize+mip+trace @256^3,1440p; learned vs plain at >=40 dB PSNR",
+        "value": round(value, 4),
         "unit": "ms/frame",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
+        "ms_per_step": round(value, 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{wl.name}: plain full-resolution form, {wl.ntri}-triangle soup, {wl.res}^3 grid "
-                        f"({wl.levels} levels), {wl.width}x{wl.height}, 9 diffuse + 1 specular cone",
-            "storage": args.dtype,
-            "form": "plain",
+            "workload": f"{wl.name}: {wl.ntri}-triangle soup, {wl.res}^3 grid ({wl.levels} levels, {args.dtype} "
+                        f"storage), {wl.width}x{wl.height}, 9 diffuse + 1 specular cone, indirect light",
+            "form": "learned (640x360 trace + MLP upsampling + per-tile fallback)" if learned is not None else "plain",
         },
-        "passes_ms": {n: round(float(passes[i]), 4) for i, n in enumerate(names)},
-        "fragments": nfrag,
-        "trace": {
-            "cone_steps": steps,
-            "samples": samples,
-            "steps_per_pixel": round(steps / (wl.width * wl.height), 2),
-            # each quadrilinear sample reads 16 texels (2 levels x 2x2x2)
-            "effective_sample_GBps": round(samples * 16 * bpe / (passes[3] * 1e-3) / 1e9, 1) if passes[3] > 0 else 0,
-        },
-        "roofline": roofline,
+        "plain_ms_per_frame": round(plain_ms, 4),
+        "plain_passes_ms": {n: round(float(plain_passes[i]), 4) for i, n in enumerate(PASSES[:4])},
     }
+    if learned is not None:
+        out["learned_ms_per_frame"] = round(learned["ms"], 4)
+        out["learned_passes_ms"] = {n: round(float(learned["passes"][i]), 4) for i, n in enumerate(PASSES)}
+        out["speedup_learned_vs_plain"] = round(plain_ms / learned["ms"], 3)
+        out["fallback_tiles"] = f"{learned['fb']}/{learned['tiles']}"
+    out["quality"] = quality
+    out["trace"] = {
+        "cone_steps": steps,
+        "samples": 3 * steps,
+        "steps_per_pixel": round(steps / (wl.width * wl.height), 2),
+        "effective_sample_GBps": round(3 * steps * 16 * bpe / (plain_passes[3] * 1e-3) / 1e9, 1)
+        if plain_passes[3] > 0 else 0,
+    }
+    out["roofline"] = roofline
     if not args.no_cpu_baseline:
         log("bench: cpu baseline (oracle port, bounded sample)...")
         out["cpu_baseline"] = cpu_baseline(wl, dt)

@@ -68,6 +68,7 @@ class SceneSpec:
         return self.view.reshape(16)[12:15].copy()
 
     def build(self, ctx, dtype=L.RGBA8, average=None, levels=7):
+        levels = min(levels, self.res.bit_length())  # at most log2(res) + 1 levels
         ctx.set_grid(self.res, levels, dtype, average)
         ctx.set_grid_size(self.cube_size)
         ctx.set_cone_set(self.cone_set)

@@ -80,7 +80,8 @@ def test_voxelize_reference_average_bit_exact(gpu_available, orc, kind, res, ntr
 
 
 @pytest.mark.parametrize("dtype", [L.FP32, L.FP16, L.BF16, L.RGBA8])
-@pytest.mark.parametrize("kind,res,ntri", [("cornell", 64, 34), ("soup", 128, 20000)])
+@pytest.mark.parametrize("kind,res,ntri", [("cornell", 64, 34), ("soup", 128, 20000), ("cornell", 8, 34),
+                                           ("soup", 256, 60000)])
 def test_voxelize_exact_mean_bit_exact(gpu_available, orc, dtype, kind, res, ntri):
     say(f"voxelize exact mean dtype={dtype} kind={kind} res={res}")
     sc = SceneSpec(kind, res=res, ntri=ntri)
@@ -101,13 +102,15 @@ def test_voxelize_exact_mean_bit_exact(gpu_available, orc, dtype, kind, res, ntr
 
 
 # ---- G-buffer -----------------------------------------------------------------------
-@pytest.mark.parametrize("kind,ntri,W,H", [("cornell", 34, 96, 64), ("soup", 5000, 160, 90)])
+@pytest.mark.parametrize("kind,ntri,W,H", [("cornell", 34, 96, 64), ("soup", 5000, 160, 90), ("cornell", 34, 512, 512),
+                                           ("soup", 40000, 640, 360), ("soup", 3000, 100, 70)])
 def test_gbuffer_bit_exact(gpu_available, orc, kind, ntri, W, H):
     say(f"gbuffer parity kind={kind} {W}x{H}")
     sc = SceneSpec(kind, ntri=ntri, W=W, H=H)
     with Context(W, H) as ctx:
         sc.build(ctx, L.FP16)
-        ctx.gbuffer()
+        for _ in range(2):  # the second frame reuses the tile lists and counters
+            ctx.gbuffer()
         gp, gn = ctx.download_gbuffer()
     p32, n32, _, _, _ = orc.gbuffer(W, H, sc.view, sc.proj, [(sc.M, sc.pos, sc.nrm, sc.tri_mat)])
     cov_g, cov_w = gp[..., 3] != 0, p32[..., 3] != 0

@@ -169,8 +169,9 @@ class Context:
         return img
 
     def pass_times(self) -> np.ndarray:
-        t = np.zeros(5, np.float32)
-        L.check(self.lib.vct_get_pass_times(self.h, L.fptr(t), 5))
+        """[voxelize, mip, gbuffer, trace, upsample, fallback] of the last frame (ms)."""
+        t = np.zeros(6, np.float32)
+        L.check(self.lib.vct_get_pass_times(self.h, L.fptr(t), t.shape[0]))
         return t
 
     # -- learned upsampler --------------------------------------------------------
@@ -227,6 +228,15 @@ class Context:
 
     def render_learned(self) -> None:
         L.check(self.lib.vct_render_learned(self.h))
+
+    def set_fallback(self, min_tile_psnr_db: float) -> None:
+        L.check(self.lib.vct_upsample_set_fallback(self.h, float(min_tile_psnr_db)))
+
+    def fallback_stats(self):
+        """(tiles, tiles traced again) of the last learned frame."""
+        t, f = C.c_uint32(), C.c_uint32()
+        L.check(self.lib.vct_upsample_fallback_stats(self.h, C.byref(t), C.byref(f)))
+        return t.value, f.value
 
     def download_lowres(self) -> np.ndarray:
         img = np.zeros((self.height // 4, self.width // 4, 4), np.float32)

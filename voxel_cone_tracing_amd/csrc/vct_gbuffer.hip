@@ -3,16 +3,18 @@
 // The reference gets the surface under each pixel by forward rendering with
 // depth test: voxel_cone_tracing.vert:22-28 (world = model * p, clip =
 // projection * view * world, normal by the inverse-transpose model matrix)
-// rasterised at W x H with GL_LESS depth (renderer.cpp:355-390).  Here:
-//  1. count  (lane per triangle): clip, near-plane reject, 1/256-pixel snap,
-//     pixel bounding box in 2x2 quads.
-//  2. exclusive scan of the candidate counts.
-//  3. raster (lane per candidate pixel): exact 64-bit edge functions with the
-//     top-left rule, screen-linear depth, 64-bit atomicMin of
-//     (depth bits << 32 | triangle id).  Equal depths resolve to the lower
-//     triangle id, which is the draw-order winner of GL_LESS.
-//  4. resolve (lane per pixel): perspective-correct world position and
-//     normal of the winning triangle, linear view depth, material id.
+// rasterised at W x H with GL_LESS depth (renderer.cpp:355-390).  Here, a
+// tiled rasteriser with the depth buffer in LDS:
+//  1. setup + bin count (lane per triangle): clip, near-plane reject,
+//     1/256-pixel snap, pixel bounding box; count into the 32x32 tiles the
+//     box overlaps (one atomic per distinct tile per wave).
+//  2. exclusive scan of the tile counts.
+//  3. bin write: triangle indices into per-tile lists.
+//  4. tile pass (workgroup per tile): exact 64-bit edge functions with the
+//     top-left rule, screen-linear depth, LDS atomicMin of
+//     (depth bits << 32 | triangle id); then perspective-correct world
+//     position and normal of each pixel's winner, linear view depth and
+//     material id, written once per pixel.
 // Triangles with a vertex behind the near plane are skipped (not clipped).
 #include <cmath>
 #include <cstring>
@@ -49,7 +51,7 @@ struct GSetup {
   int64_t X[3], Y[3];
   int64_t area;
   bool swap, ok;
-  int qx0, qy0, qw, qh;
+  int i0, j0, i1, j1;  // pixel bounding box, inclusive
 };
 
 __device__ __forceinline__ int g_find_draw(const GParams& p, uint32_t t) {
@@ -114,10 +116,10 @@ __device__ __forceinline__ void g_setup(const GDraw& dr, const float* PV, uint32
   imax = imax > W - 1 ? W - 1 : imax;
   jmax = jmax > H - 1 ? H - 1 : jmax;
   if (imin > imax || jmin > jmax) return;
-  s.qx0 = (int)(imin & ~1ll);
-  s.qy0 = (int)(jmin & ~1ll);
-  s.qw = (int)((imax - s.qx0) >> 1) + 1;
-  s.qh = (int)((jmax - s.qy0) >> 1) + 1;
+  s.i0 = (int)imin;
+  s.j0 = (int)jmin;
+  s.i1 = (int)imax;
+  s.j1 = (int)jmax;
   s.ok = true;
 }
 
@@ -147,108 +149,100 @@ __device__ __forceinline__ bool g_cover(const GSetup& s, int i, int j, float& b0
   return true;
 }
 
-// Per-triangle record of the count pass, read by the triangle's candidate lanes.
+// Per-triangle record of the bin pass, read by every tile the triangle overlaps.
 struct alignas(16) GRec {
   int64_t area;
   int32_t X[3], Y[3];
-  float z[3];  // window depth of each vertex
-  int32_t qx0, qy0, qw;
+  float z[3];                // window depth of each vertex
+  int16_t i0, j0, i1, j1;    // pixel bounding box, inclusive; i0 > i1 when culled
   uint32_t swap;
-  uint32_t otri;  // original triangle index in the frame (GL draw order)
+  uint32_t otri;             // original triangle index in the frame (GL draw order)
+  uint32_t pad;
 };
 static_assert(sizeof(GRec) == 64, "GRec must be 64 bytes");
 
-__global__ __launch_bounds__(256) void g_count_kernel(GParams p, uint32_t* counts, GRec* __restrict__ recs) {
+constexpr int kTile = 32;  // screen tile edge in pixels (one workgroup per tile)
+
+struct TileRange {
+  int tx0, ty0, tw, n;
+};
+
+__device__ __forceinline__ TileRange tile_range(const GRec& r) {
+  TileRange t;
+  t.n = 0;
+  t.tx0 = t.ty0 = t.tw = 0;
+  if (r.i0 > r.i1) return t;
+  t.tx0 = r.i0 / kTile;
+  t.ty0 = r.j0 / kTile;
+  t.tw = r.i1 / kTile - t.tx0 + 1;
+  t.n = t.tw * (r.j1 / kTile - t.ty0 + 1);
+  return t;
+}
+
+// 1. setup (lane per stored triangle): clip, snap, bounding box; count the
+//    triangle into every 32x32 tile its bounding box overlaps.
+__global__ __launch_bounds__(256) void g_bin_count_kernel(GParams p, GRec* __restrict__ recs, uint32_t* tile_count,
+                                                          int tiles_x) {
 #pragma clang fp contract(off)
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= p.ntri) return;
-  const int d = g_find_draw(p, t);
-  GSetup s;
-  g_setup(p.draws[d], p.PV, t - p.draws[d].tri_base, p.W, p.H, s);
-  counts[t] = s.ok ? (uint32_t)(4 * s.qw * s.qh) : 0u;
-  if (!s.ok) return;
   GRec r;
-  r.area = s.area;
-  for (int v = 0; v < 3; ++v) {
-    r.X[v] = (int32_t)s.X[v];
-    r.Y[v] = (int32_t)s.Y[v];
-    r.z[v] = (s.clip[v][2] / s.clip[v][3]) * 0.5f + 0.5f;
+  r.i0 = 1;
+  r.i1 = 0;
+  r.j0 = 1;
+  r.j1 = 0;
+  if (t < p.ntri) {
+    const int d = g_find_draw(p, t);
+    GSetup s;
+    g_setup(p.draws[d], p.PV, t - p.draws[d].tri_base, p.W, p.H, s);
+    r.area = s.area;
+    for (int v = 0; v < 3; ++v) {
+      r.X[v] = (int32_t)s.X[v];
+      r.Y[v] = (int32_t)s.Y[v];
+      r.z[v] = s.ok ? (s.clip[v][2] / s.clip[v][3]) * 0.5f + 0.5f : 0.0f;
+    }
+    if (s.ok) {
+      r.i0 = (int16_t)s.i0;
+      r.j0 = (int16_t)s.j0;
+      r.i1 = (int16_t)s.i1;
+      r.j1 = (int16_t)s.j1;
+    }
+    r.swap = s.swap ? 1u : 0u;
+    r.otri = p.draws[d].tri_base + p.draws[d].perm[t - p.draws[d].tri_base];
+    r.pad = 0;
+    recs[t] = r;
   }
-  r.qx0 = s.qx0;
-  r.qy0 = s.qy0;
-  r.qw = s.qw;
-  r.swap = s.swap ? 1u : 0u;
-  r.otri = p.draws[d].tri_base + p.draws[d].perm[t - p.draws[d].tri_base];
-  recs[t] = r;
-}
-
-__global__ __launch_bounds__(256) void g_raster_kernel(uint32_t ntri, int W, int H, const uint32_t* __restrict__ off,
-                                                       const uint32_t* __restrict__ d_total,
-                                                       const GRec* __restrict__ recs, unsigned long long* vis) {
-#pragma clang fp contract(off)
-  __shared__ uint32_t s_range[2];
-  const uint32_t total = *d_total;
-  const uint32_t n = ntri;
-  for (uint32_t base = blockIdx.x * 256u; base < total; base += gridDim.x * 256u) {
-    const uint32_t last = min(base + 255u, total - 1u);
-    if (threadIdx.x == 0 || threadIdx.x == 255) {
-      const uint32_t key = threadIdx.x == 0 ? base : last;
-      uint32_t lo = 0, hi = n;
-      while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (off[mid] <= key) lo = mid;
-        else hi = mid;
-      }
-      s_range[threadIdx.x == 0 ? 0 : 1] = lo;
-    }
-    __syncthreads();
-    const uint32_t c = base + threadIdx.x;
-    if (c < total) {
-      uint32_t lo = s_range[0], hi = s_range[1] + 1;
-      while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (off[mid] <= c) lo = mid;
-        else hi = mid;
-      }
-      const uint32_t t = lo;
-      const GRec r = recs[t];
-      GSetup s;
-      s.area = r.area;
-      s.swap = r.swap != 0u;
-      for (int v = 0; v < 3; ++v) {
-        s.X[v] = r.X[v];
-        s.Y[v] = r.Y[v];
-      }
-      const uint32_t lc = c - off[t];
-      const uint32_t quad = lc >> 2, sub = lc & 3u;
-      const int i = r.qx0 + 2 * (int)(quad % (uint32_t)r.qw) + (int)(sub & 1u);
-      const int j = r.qy0 + 2 * (int)(quad / (uint32_t)r.qw) + (int)(sub >> 1);
-      float b0, b1, b2;
-      if (i < W && j < H && g_cover(s, i, j, b0, b1, b2)) {
-        const float depth = b0 * r.z[0] + b1 * r.z[1] + b2 * r.z[2];
-        if (depth >= 0.0f && depth <= 1.0f) {
-          const unsigned long long key =
-              ((unsigned long long)__builtin_bit_cast(uint32_t, depth) << 32) | (unsigned long long)r.otri;
-          unsigned long long* dst = &vis[(size_t)j * W + i];
-          // early depth test: skip the atomic when a nearer surface is already in
-          // (a stale read only costs a redundant atomic, never a wrong result)
-          if (key < __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(dst, key);
-        }
-      }
-    }
-    __syncthreads();
+  const TileRange tr = tile_range(r);
+  for (int k = 0; __ballot(k < tr.n) != 0ull; ++k) {
+    const bool act = k < tr.n;
+    const uint32_t key = act ? (uint32_t)((tr.ty0 + k / tr.tw) * tiles_x + tr.tx0 + k % tr.tw) : 0u;
+    wave_add_one(tile_count, key, act);
   }
 }
 
-// Reads the winner of each pixel and resets the entry to empty for the next
-// frame (so the visibility buffer never needs a separate clear).
-__global__ __launch_bounds__(256) void g_resolve_kernel(GParams p, unsigned long long* vis,
-                                                        float4* __restrict__ gpos, float4* __restrict__ gnrm) {
+// 3. list write: the stored triangle index into each overlapped tile's list.
+__global__ __launch_bounds__(256) void g_bin_write_kernel(uint32_t ntri, const GRec* __restrict__ recs,
+                                                          const uint32_t* __restrict__ tile_off, uint32_t* cursor,
+                                                          int tiles_x, uint32_t* __restrict__ list) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  GRec r;
+  r.i0 = 1;
+  r.i1 = 0;
+  if (t < ntri) r = recs[t];
+  const TileRange tr = tile_range(r);
+  for (int k = 0; __ballot(k < tr.n) != 0ull; ++k) {
+    const bool act = k < tr.n;
+    const uint32_t key = act ? (uint32_t)((tr.ty0 + k / tr.tw) * tiles_x + tr.tx0 + k % tr.tw) : 0u;
+    const uint32_t slot = wave_add_one(cursor, key, act);
+    if (act) list[tile_off[key] + slot] = t;
+  }
+}
+
+// Perspective-correct world position, normal, linear depth and material id of
+// pixel (i, j) from its winning key; writes the G-buffer texel.
+__device__ __forceinline__ void g_resolve_pixel(const GParams& p, unsigned long long key, int i, int j,
+                                                float4* __restrict__ gpos, float4* __restrict__ gnrm) {
 #pragma clang fp contract(off)
-  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (uint32_t)(p.W * p.H)) return;
-  const unsigned long long key = vis[idx];
-  vis[idx] = ~0ull;
+  const size_t idx = (size_t)j * p.W + i;
   if (key == ~0ull) {
     gpos[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
     gnrm[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -260,7 +254,6 @@ __global__ __launch_bounds__(256) void g_resolve_kernel(GParams p, unsigned long
   const uint32_t lt = dr.inv[t - dr.tri_base];       // stored index
   GSetup s;
   g_setup(dr, p.PV, lt, p.W, p.H, s);
-  const int i = (int)(idx % (uint32_t)p.W), j = (int)(idx / (uint32_t)p.W);
   float b[3];
   if (!g_cover(s, i, j, b[0], b[1], b[2])) {  // cannot happen: same arithmetic as the raster
     b[0] = b[1] = b[2] = 1.0f / 3.0f;
@@ -296,9 +289,114 @@ __global__ __launch_bounds__(256) void g_resolve_kernel(GParams p, unsigned long
   gnrm[idx] = make_float4(n[0], n[1], n[2], __builtin_bit_cast(float, dr.mat[lt]));
 }
 
-__global__ void fill_u64_kernel(unsigned long long* p, uint64_t n, unsigned long long v) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    p[i] = v;
+// 4. one workgroup per 32x32 tile: the depth test runs on a tile depth buffer
+//    in LDS ((depth bits << 32 | original triangle id), 64-bit LDS atomicMin;
+//    equal depths resolve to the lower triangle id, the draw-order winner of
+//    GL_LESS).  The tile's triangles are taken 256 at a time; the pixels of
+//    their bounding boxes (clipped to the tile) are spread over the 256 lanes.
+//    Then every pixel of the tile is resolved and written once.
+__global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __restrict__ recs,
+                                                     const uint32_t* __restrict__ tile_count,
+                                                     const uint32_t* __restrict__ tile_off,
+                                                     const uint32_t* __restrict__ list, int tiles_x,
+                                                     float4* __restrict__ gpos, float4* __restrict__ gnrm) {
+#pragma clang fp contract(off)
+  __shared__ unsigned long long s_depth[kTile * kTile];
+  __shared__ GRec s_rec[256];
+  __shared__ uint32_t s_end[256];   // inclusive prefix of the candidate counts
+  __shared__ uint32_t s_box[256];   // clipped box: x | y << 8 | width << 16 (tile-relative)
+  __shared__ uint32_t s_rcp[256];   // ceil(65536 / width)
+  __shared__ uint32_t s_wsum[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tile = blockIdx.x;
+  const int x0 = (tile % tiles_x) * kTile, y0 = (tile / tiles_x) * kTile;
+  for (int q = tid; q < kTile * kTile; q += 256) s_depth[q] = ~0ull;
+  const uint32_t n = tile_count[tile], base = tile_off[tile];
+  __syncthreads();
+  for (uint32_t b0 = 0; b0 < n; b0 += 256) {
+    uint32_t cnt = 0;
+    if (b0 + tid < n) {
+      const GRec r = recs[list[base + b0 + tid]];
+      s_rec[tid] = r;
+      const int ci0 = max((int)r.i0, x0) - x0, ci1 = min((int)r.i1, x0 + kTile - 1) - x0;
+      const int cj0 = max((int)r.j0, y0) - y0, cj1 = min((int)r.j1, y0 + kTile - 1) - y0;
+      if (ci1 >= ci0 && cj1 >= cj0) {
+        const uint32_t cw = (uint32_t)(ci1 - ci0 + 1);
+        cnt = cw * (uint32_t)(cj1 - cj0 + 1);
+        s_box[tid] = (uint32_t)ci0 | ((uint32_t)cj0 << 8) | (cw << 16);
+        s_rcp[tid] = (65536u + cw - 1u) / cw;
+      }
+    }
+    // block-wide inclusive scan of cnt
+    uint32_t x = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wsum[wave] = x;
+    __syncthreads();
+    for (int w = 0; w < wave; ++w) x += s_wsum[w];
+    s_end[tid] = x;
+    __syncthreads();
+    const uint32_t total = s_end[255];
+    // every lane takes one contiguous run of candidates: one search, then a
+    // walk through the boxes with the triangle held in registers
+    const uint32_t per = (total + 255u) / 256u;
+    uint32_t c = (uint32_t)tid * per;
+    const uint32_t cend = min(c + per, total);
+    if (c < cend) {
+      uint32_t slot = 0, hi = 255;
+      while (slot < hi) {
+        const uint32_t mid = (slot + hi) >> 1;
+        if (s_end[mid] > c) hi = mid;
+        else slot = mid + 1;
+      }
+      while (c < cend) {
+        const uint32_t end = s_end[slot];
+        const uint32_t local = c - (slot ? s_end[slot - 1] : 0u);
+        const uint32_t box = s_box[slot], cw = box >> 16;
+        uint32_t row = (local * s_rcp[slot]) >> 16;  // exact: local < 1024, cw <= 32
+        uint32_t col = local - row * cw;
+        const GRec& r = s_rec[slot];
+        GSetup s;
+        s.area = r.area;
+        s.swap = r.swap != 0u;
+        for (int v = 0; v < 3; ++v) {
+          s.X[v] = r.X[v];
+          s.Y[v] = r.Y[v];
+        }
+        const float z0 = r.z[0], z1 = r.z[1], z2 = r.z[2];
+        const unsigned long long otri = r.otri;
+        const int bx = (int)(box & 255u), by = (int)((box >> 8) & 255u);
+        const uint32_t stop = min(end, cend);
+        for (; c < stop; ++c) {
+          const int li = bx + (int)col, lj = by + (int)row;
+          float b0w, b1w, b2w;
+          if (g_cover(s, x0 + li, y0 + lj, b0w, b1w, b2w)) {
+            const float depth = b0w * z0 + b1w * z1 + b2w * z2;
+            if (depth >= 0.0f && depth <= 1.0f) {
+              const unsigned long long key = ((unsigned long long)__builtin_bit_cast(uint32_t, depth) << 32) | otri;
+              atomicMin(&s_depth[lj * kTile + li], key);
+            }
+          }
+          if (++col == cw) {
+            col = 0;
+            ++row;
+          }
+        }
+        ++slot;  // candidate counts of listed triangles are positive: the next run starts at the next slot
+      }
+    }
+    __syncthreads();
+  }
+  for (int q = tid; q < kTile * kTile; q += 256) {
+    const int i = x0 + (q % kTile), j = y0 + (q / kTile);
+    if (i < p.W && j < p.H) g_resolve_pixel(p, s_depth[q], i, j, gpos, gnrm);
+  }
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
 }  // namespace vct
@@ -339,17 +437,11 @@ static void normal_matrix3(const float* M, float* N) {
 
 int vct_launch_gbuffer(vct_context* ctx) {
   const int W = ctx->width, H = ctx->height;
+  if (W > 32767 || H > 32767) return vct_set_error(VCT_ERR_LIMIT, "viewport above 32767 pixels");
   const size_t npix = (size_t)W * H;
   int rc;
   if ((rc = vct_ensure(ctx->g_pos, npix * 16, ctx))) return rc;
   if ((rc = vct_ensure(ctx->g_nrm, npix * 16, ctx))) return rc;
-  const bool fresh_vis = !ctx->vis.p || ctx->vis.cap < npix * 8;
-  if ((rc = vct_ensure(ctx->vis, npix * 8, ctx))) return rc;
-  if (fresh_vis || ctx->vis_pixels != npix) {
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1024), dim3(256), 0, ctx->stream, (unsigned long long*)ctx->vis.p,
-                       (uint64_t)npix, ~0ull);
-    ctx->vis_pixels = npix;
-  }
   if ((rc = vct_ensure(ctx->counters, 256, ctx))) return rc;
   uint32_t* counters = (uint32_t*)ctx->counters.p;
 
@@ -376,22 +468,45 @@ int vct_launch_gbuffer(vct_context* ctx) {
   p.ntri = ntri;
   p.W = W;
   p.H = H;
-  if (ntri) {
-    if ((rc = vct_ensure(ctx->g_count, (size_t)ntri * 4 + 4, ctx))) return rc;
-    if ((rc = vct_ensure(ctx->g_off, (size_t)ntri * 4 + 4, ctx))) return rc;
-    if ((rc = vct_ensure(ctx->g_recs, (size_t)ntri * sizeof(GRec), ctx))) return rc;
-    hipLaunchKernelGGL(g_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, p,
-                       (uint32_t*)ctx->g_count.p, (GRec*)ctx->g_recs.p);
-    VCT_HIP(hipGetLastError());
-    if ((rc = vct_launch_scan_u32(ctx, (uint32_t*)ctx->g_count.p, (uint32_t*)ctx->g_off.p, ntri, counters + 4)))
-      return rc;
-    hipLaunchKernelGGL(g_raster_kernel, dim3(256 * 8), dim3(256), 0, ctx->stream, ntri, W, H,
-                       (const uint32_t*)ctx->g_off.p, (const uint32_t*)(counters + 4), (const GRec*)ctx->g_recs.p,
-                       (unsigned long long*)ctx->vis.p);
-    VCT_HIP(hipGetLastError());
-  }
-  hipLaunchKernelGGL(g_resolve_kernel, dim3((uint32_t)((npix + 255) / 256)), dim3(256), 0, ctx->stream, p,
-                     (unsigned long long*)ctx->vis.p, (float4*)ctx->g_pos.p, (float4*)ctx->g_nrm.p);
+  const int tiles_x = (W + kTile - 1) / kTile, tiles_y = (H + kTile - 1) / kTile;
+  const uint32_t ntiles = (uint32_t)(tiles_x * tiles_y);
+  // [0, ntiles): counts, [ntiles, 2 ntiles): write cursors
+  if ((rc = vct_ensure(ctx->g_count, (size_t)ntiles * 8 + 16, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->g_off, (size_t)ntiles * 4 + 16, ctx))) return rc;
+  uint32_t* tile_count = (uint32_t*)ctx->g_count.p;
+  uint32_t* cursor = tile_count + ntiles;
+  uint32_t* tile_off = (uint32_t*)ctx->g_off.p;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((2 * ntiles + 255) / 256), dim3(256), 0, ctx->stream, tile_count,
+                     2 * ntiles, 0u);
   VCT_HIP(hipGetLastError());
+  uint32_t total = 0;
+  if (ntri) {
+    if ((rc = vct_ensure(ctx->g_recs, (size_t)ntri * sizeof(GRec), ctx))) return rc;
+    hipLaunchKernelGGL(g_bin_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, p,
+                       (GRec*)ctx->g_recs.p, tile_count, tiles_x);
+    VCT_HIP(hipGetLastError());
+    if ((rc = vct_launch_scan_u32(ctx, tile_count, tile_off, ntiles, counters + 4))) return rc;
+    // the list size is known on the device only
+    VCT_HIP(hipMemcpyAsync(&total, counters + 4, 4, hipMemcpyDeviceToHost, ctx->stream));
+    VCT_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  if (total) {
+    if ((rc = vct_ensure(ctx->g_list, (size_t)total * 4 + (size_t)total / 4 * 4, ctx))) return rc;
+    hipLaunchKernelGGL(g_bin_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, ntri,
+                       (const GRec*)ctx->g_recs.p, (const uint32_t*)tile_off, cursor, tiles_x,
+                       (uint32_t*)ctx->g_list.p);
+    VCT_HIP(hipGetLastError());
+  } else {
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((ntiles + 255) / 256), dim3(256), 0, ctx->stream, tile_off, ntiles,
+                       0u);
+    VCT_HIP(hipGetLastError());
+    if ((rc = vct_ensure(ctx->g_list, 16, ctx))) return rc;
+  }
+  if ((rc = vct_ensure(ctx->g_recs, 64, ctx))) return rc;
+  hipLaunchKernelGGL(g_tile_kernel, dim3(ntiles), dim3(256), 0, ctx->stream, p, (const GRec*)ctx->g_recs.p,
+                     (const uint32_t*)tile_count, (const uint32_t*)tile_off, (const uint32_t*)ctx->g_list.p,
+                     tiles_x, (float4*)ctx->g_pos.p, (float4*)ctx->g_nrm.p);
+  VCT_HIP(hipGetLastError());
+  ctx->last_gbuffer_refs = total;
   return VCT_OK;
 }

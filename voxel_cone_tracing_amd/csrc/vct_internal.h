@@ -21,6 +21,29 @@
 
 namespace vct {
 
+// ---- wave-aggregated counters ----------------------------------------------
+#ifdef __HIPCC__
+// Adds 1 to counter[key] for every active lane, one atomic per distinct key
+// in the wave; returns the lane's slot (the counter's old value + its rank
+// among the lanes with the same key).
+__device__ __forceinline__ uint32_t wave_add_one(uint32_t* counter, uint32_t key, bool active) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long pending = __ballot(active);
+  uint32_t slot = 0;
+  while (pending) {
+    const int leader = __ffsll(pending) - 1;
+    const uint32_t k = __shfl(key, leader, 64);
+    const unsigned long long same = __ballot(active && key == k);
+    uint32_t old = 0;
+    if (lane == leader) old = atomicAdd(counter + k, (uint32_t)__popcll(same));
+    old = __shfl(old, leader, 64);
+    if (active && key == k) slot = old + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+    pending &= ~same;
+  }
+  return slot;
+}
+#endif
+
 // ---- Morton / brick addressing -------------------------------------------
 VCT_HD uint32_t spread3(uint32_t v) {
   v &= 0x3ffu;
@@ -182,17 +205,15 @@ struct vct_context {
   uint32_t lvl_size[16];
 
   // voxelize scratch
-  VctBuf accum;        // exact mode: 5 x u32 per voxel (kept zero between frames)
-  bool accum_zero = false;
   VctBuf tri_count, tri_off, tri_base;  // per-triangle candidate counts / offsets
   VctBuf frags, frags_sorted, vox_count, vox_off, scan_tmp, counters;
   VctBuf setup;  // per-frame triangle list (model index, transform)
   uint64_t last_fragments = 0, last_candidates = 0;
 
   // G-buffer + image
-  VctBuf vis, g_pos, g_nrm, image;
-  VctBuf g_count, g_off, g_recs;
-  size_t vis_pixels = 0;
+  VctBuf g_list, g_pos, g_nrm, image;
+  VctBuf g_count, g_off, g_recs;  // per-tile counts + cursors, per-tile offsets, per-triangle records
+  uint64_t last_gbuffer_refs = 0;  // (triangle, tile) pairs of the last G-buffer
 
   std::vector<VctTex3D> textures;
 
@@ -203,10 +224,15 @@ struct vct_context {
   VctBuf low_gpos, low_gnrm, low_img, full_target, sample_pix;
   VctBuf train_X, train_A, train_Y, train_valid;
   size_t train_n = 0, train_cap = 0;
+  // per-tile fallback of the learned frame
+  VctBuf probe_pix, probe_val, fb_list;  // fb_list: [count][tiles...]
+  int probe_w = 0, probe_h = 0;
+  float fb_psnr_db = 40.0f;  // minimum estimated tile PSNR; <= 0 disables the fallback
+  bool fb_ran = false;
 
-  hipEvent_t ev[12];
-  float pass_ms[5] = {0, 0, 0, 0, 0};
-  bool ev_recorded[5] = {false, false, false, false, false};
+  hipEvent_t ev[12];  // start / end of 6 passes
+  float pass_ms[6] = {0, 0, 0, 0, 0, 0};
+  bool ev_recorded[6] = {false, false, false, false, false, false};
 };
 
 // ---- internal helpers (vct_runtime.hip) ---------------------------------
@@ -226,6 +252,17 @@ int vct_launch_gbuffer(vct_context* ctx);
 int vct_launch_trace(vct_context* ctx);
 // Trace of a W x H G-buffer into out; demodulate: final / max(Kd, eps) with
 // the pixel's linear depth in w (the learned path's buffers).
+// Pixel selection of a trace launch (zero-initialised: every pixel).
+struct TraceSelect {
+  const uint32_t* pix_list;    // device list of pixels (y * W + x); output k = pixel k
+  uint32_t npix_list;
+  const uint32_t* tile_list;   // device list of 32x32 tiles; output indexed like the G-buffer
+  const uint32_t* tile_count;  // device-side count of listed tiles
+  uint32_t max_tiles;          // grid bound (host side)
+  int tiles_x;
+};
+int vct_trace_select(vct_context* ctx, const float4* gpos, const float4* gnrm, float4* out, int W, int H,
+                     int demodulate, unsigned long long* step_counter, const TraceSelect& sel);
 int vct_trace_into(vct_context* ctx, const float4* gpos, const float4* gnrm, float4* out, int W, int H,
                    int demodulate, unsigned long long* step_counter);
 int vct_launch_scan_u32(vct_context* ctx, const uint32_t* in, uint32_t* out, uint32_t n,

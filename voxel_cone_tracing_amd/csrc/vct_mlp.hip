@@ -208,7 +208,13 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs A) {
       if (qx < A.F.W && qy < A.F.H) {
         float4 o;
         if (s_cov[s]) {
-          o = make_float4(s_alb[s][0] * y[nt][0], s_alb[s][1] * y[nt][1], s_alb[s][2] * y[nt][2], 1.0f);
+          float3 v = make_float3(y[nt][0], y[nt][1], y[nt][2]);
+          if ((qx & 3) == 1 && (qy & 3) == 1 && (qx >> 2) < A.F.wl && (qy >> 2) < A.F.hl) {
+            // this pixel was traced at quarter resolution: keep the traced value
+            const float4 t = A.F.low[(size_t)(qy >> 2) * A.F.wl + (qx >> 2)];
+            v = make_float3(t.x, t.y, t.z);
+          }
+          o = make_float4(s_alb[s][0] * v.x, s_alb[s][1] * v.y, s_alb[s][2] * v.z, 1.0f);
         } else {
           o = make_float4(0.15f, 0.25f, 0.25f, 1.0f);  // clear colour, renderer.cpp:398
         }
@@ -528,6 +534,62 @@ __global__ __launch_bounds__(256) void mlp_pack_kernel(const float* __restrict__
 
 }  // namespace vct
 
+// ---- per-tile fallback ------------------------------------------------------------
+// Every 32x32 tile of the upsampled frame is checked at 16 probe pixels (a
+// 4x4 grid at offsets 3, 11, 19, 27: the pixels farthest from the traced
+// quarter-resolution samples), traced at full resolution beforehand.  A tile
+// whose mean squared error over its covered probes exceeds the bound is
+// listed and traced again in full.  The probes' exact values are written into
+// the image.  One workgroup; the list comes out in tile order.
+constexpr int kFbTile = 32;
+constexpr int kProbes = 16;
+
+__global__ __launch_bounds__(1024) void fallback_select_kernel(const float4* __restrict__ probe_val,
+                                                               const uint32_t* __restrict__ probe_pix,
+                                                               const float4* __restrict__ gpos,
+                                                               float4* __restrict__ image, uint32_t ntiles,
+                                                               float max_mse, uint32_t* __restrict__ fb) {
+  __shared__ uint32_t s_wsum[16];
+  __shared__ uint32_t s_base;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) s_base = 0;
+  __syncthreads();
+  uint32_t* list = fb + 1;
+  for (uint32_t t0 = 0; t0 < ntiles; t0 += 1024) {
+    const uint32_t t = t0 + tid;
+    bool flag = false;
+    if (t < ntiles) {
+      float sse = 0.0f;
+      int nc = 0;
+      for (int k = 0; k < kProbes; ++k) {
+        const uint32_t q = t * kProbes + k;
+        const uint32_t p = probe_pix[q];
+        if (gpos[p].w == 0.0f) continue;  // background
+        const float4 v = probe_val[q], m = image[p];
+        const float dx = v.x - m.x, dy = v.y - m.y, dz = v.z - m.z;
+        sse += dx * dx + dy * dy + dz * dz;
+        ++nc;
+      }
+      flag = nc > 0 && sse > max_mse * 3.0f * (float)nc;
+      for (int k = 0; k < kProbes; ++k) image[probe_pix[t * kProbes + k]] = probe_val[t * kProbes + k];
+    }
+    const unsigned long long m = __ballot(flag);
+    if (lane == 0) s_wsum[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t off = s_base;
+    for (int w = 0; w < wave; ++w) off += s_wsum[w];
+    if (flag) list[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = t;
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t tot = 0;
+      for (int w = 0; w < 16; ++w) tot += s_wsum[w];
+      s_base += tot;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) fb[0] = s_base;
+}
+
 // ============================================================================
 // host side
 // ============================================================================
@@ -684,6 +746,50 @@ int infer_full(vct_context* ctx) {
   hipLaunchKernelGGL(mlp_infer_kernel, grid, dim3(256), 0, ctx->stream, A);
   VCT_HIP(hipGetLastError());
   return VCT_OK;
+}
+
+// Probe trace, tile selection and full-resolution trace of the selected tiles.
+int fallback(vct_context* ctx) {
+  const int W = ctx->width, H = ctx->height;
+  const int tx = (W + kFbTile - 1) / kFbTile, ty = (H + kFbTile - 1) / kFbTile;
+  const uint32_t ntiles = (uint32_t)(tx * ty);
+  int rc;
+  if ((rc = vct_ensure(ctx->probe_val, (size_t)ntiles * kProbes * 16, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->fb_list, (size_t)(ntiles + 1) * 4, ctx))) return rc;
+  if (ctx->probe_w != W || ctx->probe_h != H || !ctx->probe_pix.p) {
+    std::vector<uint32_t> pix((size_t)ntiles * kProbes);
+    for (int j = 0; j < ty; ++j)
+      for (int i = 0; i < tx; ++i)
+        for (int b = 0; b < 4; ++b)
+          for (int a = 0; a < 4; ++a) {
+            const int x = std::min(i * kFbTile + 8 * a + 3, W - 1), y = std::min(j * kFbTile + 8 * b + 3, H - 1);
+            pix[((size_t)(j * tx + i)) * kProbes + b * 4 + a] = (uint32_t)(y * W + x);
+          }
+    if ((rc = vct_ensure(ctx->probe_pix, pix.size() * 4, ctx))) return rc;
+    VCT_HIP(hipMemcpyAsync(ctx->probe_pix.p, pix.data(), pix.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    VCT_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->probe_w = W;
+    ctx->probe_h = H;
+  }
+  TraceSelect sel;
+  memset(&sel, 0, sizeof(sel));
+  sel.pix_list = (const uint32_t*)ctx->probe_pix.p;
+  sel.npix_list = ntiles * kProbes;
+  if ((rc = vct_trace_select(ctx, (const float4*)ctx->g_pos.p, (const float4*)ctx->g_nrm.p,
+                             (float4*)ctx->probe_val.p, W, H, 0, nullptr, sel)))
+    return rc;
+  const float max_mse = powf(10.0f, -ctx->fb_psnr_db / 10.0f);
+  hipLaunchKernelGGL(fallback_select_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const float4*)ctx->probe_val.p,
+                     (const uint32_t*)ctx->probe_pix.p, (const float4*)ctx->g_pos.p, (float4*)ctx->image.p, ntiles,
+                     max_mse, (uint32_t*)ctx->fb_list.p);
+  VCT_HIP(hipGetLastError());
+  memset(&sel, 0, sizeof(sel));
+  sel.tile_list = (const uint32_t*)ctx->fb_list.p + 1;
+  sel.tile_count = (const uint32_t*)ctx->fb_list.p;
+  sel.max_tiles = ntiles;
+  sel.tiles_x = tx;
+  return vct_trace_select(ctx, (const float4*)ctx->g_pos.p, (const float4*)ctx->g_nrm.p, (float4*)ctx->image.p, W,
+                          H, 0, nullptr, sel);
 }
 
 }  // namespace
@@ -881,7 +987,8 @@ int vct_upsample_train(vct_ctx ctx, uint32_t steps, uint32_t batch, float lr, ui
 }
 
 // The learned form of Renderer::render: voxelize, mip, full-resolution
-// G-buffer, quarter-resolution cone trace, MLP upsampling to full resolution.
+// G-buffer, quarter-resolution cone trace, MLP upsampling to full resolution,
+// per-tile check and full-resolution trace of the tiles that fail it.
 int vct_render_learned(vct_ctx ctx) {
   MLP_CTX(ctx);
   int rc;
@@ -899,7 +1006,40 @@ int vct_render_learned(vct_ctx ctx) {
   if ((rc = infer_full(ctx))) return rc;
   VCT_HIP(hipEventRecord(ctx->ev[9], ctx->stream));
   ctx->ev_recorded[4] = true;
+  ctx->fb_ran = false;
+  if (ctx->fb_psnr_db > 0.0f) {
+    VCT_HIP(hipEventRecord(ctx->ev[10], ctx->stream));
+    if ((rc = fallback(ctx))) return rc;
+    VCT_HIP(hipEventRecord(ctx->ev[11], ctx->stream));
+    ctx->ev_recorded[5] = true;
+    ctx->fb_ran = true;
+  } else {
+    ctx->pass_ms[5] = 0.0f;
+    ctx->ev_recorded[5] = false;
+  }
   return vct_clear_queues(ctx);
+}
+
+int vct_upsample_set_fallback(vct_ctx ctx, float min_tile_psnr_db) {
+  MLP_CTX(ctx);
+  if (!(min_tile_psnr_db == min_tile_psnr_db)) return vct_set_error(VCT_ERR_INVALID_ARG, "NaN threshold");
+  ctx->fb_psnr_db = min_tile_psnr_db;
+  return VCT_OK;
+}
+
+int vct_upsample_fallback_stats(vct_ctx ctx, uint32_t* tiles, uint32_t* fallback_tiles) {
+  MLP_CTX(ctx);
+  const uint32_t nt = (uint32_t)(((ctx->width + kFbTile - 1) / kFbTile) * ((ctx->height + kFbTile - 1) / kFbTile));
+  if (tiles) *tiles = nt;
+  if (fallback_tiles) {
+    uint32_t n = 0;
+    if (ctx->fb_ran && ctx->fb_list.p) {
+      VCT_HIP(hipMemcpyAsync(&n, ctx->fb_list.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+      VCT_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    *fallback_tiles = n;
+  }
+  return VCT_OK;
 }
 
 // Quarter-resolution buffer of the last learned frame (tests, tools).

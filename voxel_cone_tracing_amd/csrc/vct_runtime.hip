@@ -185,7 +185,6 @@ static int alloc_volume(vct_context* ctx) {
   if (rc) return rc;
   VCT_HIP(hipMemsetAsync(ctx->volume.p, 0, off, ctx->stream));
   ctx->volume_valid = true;
-  ctx->accum_zero = false;
   return VCT_OK;
 }
 
@@ -332,14 +331,14 @@ int vct_destroy(vct_ctx ctx) {
   }
   for (auto& t : ctx->textures)
     if (t.dptr) (void)hipFree(t.dptr);
-  VctBuf* bufs[] = {&ctx->d_materials, &ctx->volume, &ctx->accum, &ctx->tri_count, &ctx->tri_off,
+  VctBuf* bufs[] = {&ctx->d_materials, &ctx->volume, &ctx->tri_count, &ctx->tri_off,
                     &ctx->tri_base, &ctx->frags, &ctx->frags_sorted, &ctx->vox_count, &ctx->vox_off,
-                    &ctx->scan_tmp, &ctx->counters, &ctx->setup, &ctx->vis, &ctx->g_pos,
+                    &ctx->scan_tmp, &ctx->counters, &ctx->setup, &ctx->g_list, &ctx->g_pos,
                     &ctx->g_nrm, &ctx->image, &ctx->g_count, &ctx->g_off, &ctx->g_recs,
                     &ctx->mlp_params, &ctx->mlp_m1, &ctx->mlp_m2, &ctx->mlp_packed, &ctx->mlp_bias,
                     &ctx->mlp_grad, &ctx->mlp_partial, &ctx->low_gpos, &ctx->low_gnrm, &ctx->low_img,
                     &ctx->full_target, &ctx->sample_pix, &ctx->train_X, &ctx->train_A, &ctx->train_Y,
-                    &ctx->train_valid};
+                    &ctx->train_valid, &ctx->probe_pix, &ctx->probe_val, &ctx->fb_list};
   for (VctBuf* b : bufs) free_buf(*b);
   for (int i = 0; i < 12; ++i)
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
@@ -748,14 +747,14 @@ int vct_get_pass_times(vct_ctx ctx, float* ms, int n) {
   CTX_CHECK(ctx);
   if (!ms || n < 0) return vct_set_error(VCT_ERR_INVALID_ARG, "bad output");
   VCT_HIP(hipStreamSynchronize(ctx->stream));
-  for (int i = 0; i < 5; ++i) {
+  for (int i = 0; i < 6; ++i) {
     if (ctx->ev_recorded[i]) {
       float t = 0;
       if (hipEventElapsedTime(&t, ctx->ev[2 * i], ctx->ev[2 * i + 1]) == hipSuccess) ctx->pass_ms[i] = t;
       else (void)hipGetLastError();
     }
   }
-  for (int i = 0; i < n && i < 5; ++i) ms[i] = ctx->pass_ms[i];
+  for (int i = 0; i < n && i < 6; ++i) ms[i] = ctx->pass_ms[i];
   return VCT_OK;
 }
 

@@ -54,6 +54,12 @@ struct TraceArgs {
   float view_lod;
   int demodulate;  // 1: write final / max(Kd, eps) (the learned path's target)
   unsigned long long* step_counter;  // optional: cone steps taken (diagnostics)
+  // pixel selection (both null: every pixel, image indexed like the G-buffer)
+  const uint32_t* pix_list;    // listed pixels (y * W + x); image[k] receives pixel k
+  uint32_t npix_list;
+  const uint32_t* tile_list;   // listed 32x32 tiles (ty * tiles_x + tx); 4 workgroups per tile
+  const uint32_t* tile_count;  // device-side number of listed tiles
+  int tiles_x;
   ConeSet cones;
   LightSet lights;
 };
@@ -237,14 +243,34 @@ __device__ __forceinline__ float3 refract3(float3 I, float3 N, float eta) {
 template <int DT>
 __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-  const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+  int px, py;
+  size_t oidx;
+  if (A.pix_list) {
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= A.npix_list) return;
+    const uint32_t p = A.pix_list[k];
+    px = (int)(p % (uint32_t)A.W);
+    py = (int)(p / (uint32_t)A.W);
+    oidx = k;
+  } else {
+    int bx = blockIdx.x, by = blockIdx.y;
+    if (A.tile_list) {
+      const uint32_t k = blockIdx.x >> 2;
+      if (k >= *A.tile_count) return;
+      const uint32_t t = A.tile_list[k];
+      bx = (int)(t % (uint32_t)A.tiles_x) * 2 + (int)(blockIdx.x & 1u);
+      by = (int)(t / (uint32_t)A.tiles_x) * 2 + (int)((blockIdx.x >> 1) & 1u);
+    }
+    px = bx * 16 + (wave & 1) * 8 + (lane & 7);
+    py = by * 16 + (wave >> 1) * 8 + (lane >> 3);
+    oidx = (size_t)py * A.W + px;
+  }
   if (px >= A.W || py >= A.H) return;
   const size_t idx = (size_t)py * A.W + px;
   const float4 gp = A.gpos[idx];
   const float4 bg = make_float4(0.15f, 0.25f, 0.25f, 1.0f);  // renderer.cpp:398
   if (gp.w == 0.0f) {
-    A.image[idx] = A.demodulate ? f4(0.0f) : bg;
+    A.image[oidx] = A.demodulate ? f4(0.0f) : bg;
     return;
   }
   const float4 gn = A.gnrm[idx];
@@ -257,7 +283,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
                                  0.5f * (world.z / A.cube_size) + 0.5f);
   // within_cube(pos, 0) on biased coordinates (:250, :61-64)
   if (!(fabsf(pos.x) < 1.0f && fabsf(pos.y) < 1.0f && fabsf(pos.z) < 1.0f)) {
-    A.image[idx] = A.demodulate ? f4(0.0f) : bg;
+    A.image[oidx] = A.demodulate ? f4(0.0f) : bg;
     return;
   }
   if (A.view_dir < 7) {  // voxel view (:253-257), blended SRC_ALPHA / ONE_MINUS_SRC_ALPHA
@@ -269,7 +295,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
     else { ad.z = 1.0f; mz = d; }
     const float4 s = sample_voxel<DT>(A, pos, ad, mx, my, mz, A.view_lod);
     const float a = s.w;
-    A.image[idx] = make_float4(s.x * a + bg.x * (1.0f - a), s.y * a + bg.y * (1.0f - a),
+    A.image[oidx] = make_float4(s.x * a + bg.x * (1.0f - a), s.y * a + bg.y * (1.0f - a),
                                s.z * a + bg.z * (1.0f - a), a * a + bg.w * (1.0f - a));
     return;
   }
@@ -281,77 +307,36 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
   // specular aperture (:220-227)
   const float rough = sqrtf(2.0f / (m.shininess + 2.0f));
   const float spec_ap = fminf(fmaxf(tanf(1.57079f * rough), 0.0174533f), 3.14159265f);
-  // All cones of the pixel in one loop: slots [0, nd) diffuse, nd specular,
-  // nd + 1 refraction.  A lane that finishes a cone starts its next one in
-  // the same iteration, so the wave runs for the longest lane's total march,
-  // not for the sum over cones of each cone's longest lane.
+  // Cones one after another (each wave marches a cone until its last lane is
+  // done).  Interleaving the cones of a lane in one loop was measured slower:
+  // lanes then sample different cones in the same iteration and the set-up of
+  // a new cone diverges from the marching lanes.
   const int nd = A.en_diffuse ? A.cones.n : 0;
   const bool do_spec = A.en_specular != 0;
   const bool do_refr = do_spec && illum_transmissive(m.illum);
-  const int kend = nd + 2;
-  float3 o1 = make_float3(0.f, 0.f, 0.f), o2 = o1;
-  if (nd > 0) {
-    o1 = normalize3(tangent_of(n));  // trace_diffuse (:146-147)
-    o2 = normalize3(cross3(o1, n));
-  }
   float3 diff_acc = make_float3(0.f, 0.f, 0.f), spec_rgb = diff_acc, refr_rgb = diff_acc;
-  auto next_slot = [&](int k) {
-    ++k;
-    if (k == nd && !do_spec) ++k;
-    if (k == nd + 1 && !do_refr) ++k;
-    return k;
-  };
-  const float vs = 1.0f / (float)A.res;
-  const float fres = (float)A.res;
-  int k = next_slot(-1);
-  float3 dir = make_float3(0.f, 0.f, 1.f), ad = dir;
-  float ap = 0.f, maxd = 0.f, dist = 0.f, diam = 0.f;
-  int mx = 1, my = 3, mz = 5;
-  float4 c = f4(0.0f);
-  auto setup = [&](int kk) {
-    float3 d;
-    if (kk < nd) {
+  if (nd > 0) {
+    const float3 o1 = normalize3(tangent_of(n));  // trace_diffuse (:146-147)
+    const float3 o2 = normalize3(cross3(o1, n));
+    for (int kk = 0; kk < nd; ++kk) {
       const float a = A.cones.a[kk], b = A.cones.b[kk], cc = A.cones.c[kk];
-      d = make_float3(a * n.x + b * o1.x + cc * o2.x, a * n.y + b * o1.y + cc * o2.y, a * n.z + b * o1.z + cc * o2.z);
-      ap = A.cones.aperture;
-    } else if (kk == nd) {
-      // normalize(reflect(-view_dir, n)) (:231)
-      const float3 I = make_float3(-view_dir.x, -view_dir.y, -view_dir.z);
-      const float dn = 2.0f * dot3(n, I);
-      d = normalize3(make_float3(I.x - dn * n.x, I.y - dn * n.y, I.z - dn * n.z));
-      ap = spec_ap;
-    } else {
-      d = refract3(view_dir, n, 1.0f / m.ior);  // trace_refraction (:237-241)
-      ap = spec_ap;
+      const float3 d =
+          make_float3(a * n.x + b * o1.x + cc * o2.x, a * n.y + b * o1.y + cc * o2.y, a * n.z + b * o1.z + cc * o2.z);
+      const float4 r = trace_cone<DT>(A, pos, d, A.cones.aperture, sqrt3, steps);
+      diff_acc = make_float3(diff_acc.x + r.x, diff_acc.y + r.y, diff_acc.z + r.z);
     }
-    dir = normalize3(d);  // trace_cone (:90)
-    mx = dir.x < 0.0f ? 0 : 1;
-    my = dir.y < 0.0f ? 2 : 3;
-    mz = dir.z < 0.0f ? 4 : 5;
-    ad = make_float3(fabsf(dir.x), fabsf(dir.y), fabsf(dir.z));
-    maxd = sqrt3;
-    c = f4(0.0f);
-    dist = 3.0f * vs;
-    diam = dist * ap;
-  };
-  if (k < kend) setup(k);
-  while (k < kend) {
-    if (c.w < 1.0f && dist < maxd) {
-      const float3 sp = make_float3(dir.x * dist + pos.x, dir.y * dist + pos.y, dir.z * dist + pos.z);
-      const float mip = fmaxf(log2f(diam * fres), 0.0f);
-      const float4 s = sample_voxel<DT>(A, sp, ad, mx, my, mz, mip);
-      c = c + (1.0f - c.w) * s;
-      dist += fmaxf(diam * 0.5f, vs);
-      diam = dist * ap;
-      ++steps;
-    } else {
-      const float3 r = make_float3(c.x, c.y, c.z);
-      if (k < nd) diff_acc = make_float3(diff_acc.x + r.x, diff_acc.y + r.y, diff_acc.z + r.z);
-      else if (k == nd) spec_rgb = r;
-      else refr_rgb = r;
-      k = next_slot(k);
-      if (k < kend) setup(k);
-    }
+  }
+  if (do_spec) {
+    // normalize(reflect(-view_dir, n)) (:231)
+    const float3 I = make_float3(-view_dir.x, -view_dir.y, -view_dir.z);
+    const float dn = 2.0f * dot3(n, I);
+    const float3 rd = normalize3(make_float3(I.x - dn * n.x, I.y - dn * n.y, I.z - dn * n.z));
+    const float4 r = trace_cone<DT>(A, pos, rd, spec_ap, sqrt3, steps);  // normalised again inside (:90)
+    spec_rgb = make_float3(r.x, r.y, r.z);
+  }
+  if (do_refr) {  // trace_refraction (:237-241)
+    const float4 r = trace_cone<DT>(A, pos, refract3(view_dir, n, 1.0f / m.ior), spec_ap, sqrt3, steps);
+    refr_rgb = make_float3(r.x, r.y, r.z);
   }
   if (nd > 0) {
     const float inv = 1.0f / (float)nd;
@@ -420,7 +405,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
     out = make_float3(out.x / fmaxf(m.diffuse[0], e), out.y / fmaxf(m.diffuse[1], e), out.z / fmaxf(m.diffuse[2], e));
     ow = gp.w;  // linear depth of the traced pixel
   }
-  A.image[idx] = make_float4(out.x, out.y, out.z, ow);
+  A.image[oidx] = make_float4(out.x, out.y, out.z, ow);
   if (A.step_counter) atomicAdd(A.step_counter, (unsigned long long)steps);
 }
 
@@ -477,9 +462,21 @@ void vct_cone_set_table(int set, ConeSet& cs) {
 
 int vct_trace_into(vct_context* ctx, const float4* gpos, const float4* gnrm, float4* out, int W, int H,
                    int demodulate, unsigned long long* step_counter) {
+  TraceSelect sel;
+  memset(&sel, 0, sizeof(sel));
+  return vct_trace_select(ctx, gpos, gnrm, out, W, H, demodulate, step_counter, sel);
+}
+
+int vct_trace_select(vct_context* ctx, const float4* gpos, const float4* gnrm, float4* out, int W, int H,
+                     int demodulate, unsigned long long* step_counter, const TraceSelect& sel) {
   TraceArgs A;
   memset(&A, 0, sizeof(A));
   A.step_counter = step_counter;
+  A.pix_list = sel.pix_list;
+  A.npix_list = sel.npix_list;
+  A.tile_list = sel.tile_list;
+  A.tile_count = sel.tile_count;
+  A.tiles_x = sel.tiles_x;
   A.vol = (const char*)ctx->volume.p;
   for (int l = 0; l < ctx->levels; ++l)
     for (int d = 0; d < 6; ++d) A.off[l][d] = ctx->lvl_off[l][d];
@@ -517,6 +514,13 @@ int vct_trace_into(vct_context* ctx, const float4* gpos, const float4* gnrm, flo
   }
   A.lights.count = n;
   dim3 grid((W + 15) / 16, (H + 15) / 16);
+  if (sel.pix_list) {
+    if (sel.npix_list == 0) return VCT_OK;
+    grid = dim3((sel.npix_list + 255) / 256);
+  } else if (sel.tile_list) {
+    if (sel.max_tiles == 0) return VCT_OK;
+    grid = dim3(4 * sel.max_tiles);
+  }
   switch (ctx->dtype) {
     case VCT_RGBA8: hipLaunchKernelGGL(trace_kernel<VCT_RGBA8>, grid, dim3(256), 0, ctx->stream, A); break;
     case VCT_FP16: hipLaunchKernelGGL(trace_kernel<VCT_FP16>, grid, dim3(256), 0, ctx->stream, A); break;

@@ -82,7 +82,7 @@ struct alignas(16) TriRec {
   int32_t X[3], Y[3];
   int32_t qx0, qy0, qw;
   uint32_t mat;
-  uint32_t swap;
+  uint32_t flags;   // bit 0: vertices 1, 2 swapped (CW); bits 1-2: dominant axis; bits 8-31: qh
   float w[3][3];    // world position / cube_size per vertex
   float n[3][3];    // normalised transformed normal per vertex
   uint32_t otri;   // original triangle index in the frame (canonical order)
@@ -161,18 +161,9 @@ __device__ __forceinline__ void tri_setup(const Draw& dr, uint32_t lt, uint32_t 
   }
 }
 
-__global__ __launch_bounds__(256) void vox_count_kernel(DrawBatch b, VoxParams vp, uint32_t* counts,
-                                                        TriRec* __restrict__ recs) {
+// Record of a triangle that passed setup (s.ok).
+__device__ __forceinline__ TriRec make_rec(const Draw& dr, uint32_t lt, const Setup& s) {
 #pragma clang fp contract(off)
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= b.ntri) return;
-  const int d = find_draw(b, t);
-  const Draw& dr = b.d[d];
-  const uint32_t lt = t - dr.tri_base;
-  Setup s;
-  tri_setup(dr, lt, vp.res, vp.cube_size, s);
-  counts[t] = s.ok ? (uint32_t)(4 * s.qw * s.qh) : 0u;
-  if (!s.ok) return;
   TriRec r;
   r.area = s.area;
   for (int v = 0; v < 3; ++v) {
@@ -184,7 +175,7 @@ __global__ __launch_bounds__(256) void vox_count_kernel(DrawBatch b, VoxParams v
   r.qy0 = s.qy0;
   r.qw = s.qw;
   r.mat = dr.mat[lt];
-  r.swap = s.swap ? 1u : 0u;
+  r.flags = (s.swap ? 1u : 0u) | ((uint32_t)s.axis << 1) | ((uint32_t)s.qh << 8);
   r.otri = dr.canon_base + dr.perm[lt];
   // normal: voxelize.vert:28 per vertex (normalize(N * n))
   const float* nr = dr.nrm + (size_t)lt * 9;
@@ -199,24 +190,38 @@ __global__ __launch_bounds__(256) void vox_count_kernel(DrawBatch b, VoxParams v
     const float l = sqrtf(tt[0] * tt[0] + tt[1] * tt[1] + tt[2] * tt[2]);
     for (int k = 0; k < 3; ++k) r.n[v][k] = tt[k] / l;
   }
-  recs[t] = r;
+  return r;
+}
+
+__global__ __launch_bounds__(256) void vox_count_kernel(DrawBatch b, VoxParams vp, uint32_t* counts,
+                                                        TriRec* __restrict__ recs) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.ntri) return;
+  const int d = find_draw(b, t);
+  const Draw& dr = b.d[d];
+  const uint32_t lt = t - dr.tri_base;
+  Setup s;
+  tri_setup(dr, lt, vp.res, vp.cube_size, s);
+  counts[t] = s.ok ? (uint32_t)(4 * s.qw * s.qh) : 0u;
+  if (s.ok) recs[t] = make_rec(dr, lt, s);
 }
 
 __device__ __forceinline__ bool top_left(int64_t dx, int64_t dy) { return dy < 0 || (dy == 0 && dx < 0); }
 
-// Fragment of candidate `lc` of triangle `lt`: coverage, voxel and colour.
-// Returns false if the pixel is not covered or the voxel falls outside.
-__device__ __forceinline__ bool shade_fragment(const TriRec& s, uint32_t lc, const VoxParams& vp,
-                                               const DevMaterial* mats, uint32_t& key, float4& color) {
+// Fragment at pixel (i, j) of the triangle's 2res x 2res raster: coverage,
+// voxel (Morton index) and colour.  Returns false if the pixel is not covered
+// or the voxel falls outside the grid (with kBrick: also when the voxel's
+// Morton index >> bshift is not bwant, before any shading).
+template <bool kBrick = false>
+__device__ __forceinline__ bool shade_pixel(const TriRec& s, int i, int j, const VoxParams& vp,
+                                            const DevMaterial* mats, uint32_t& key, float4& color,
+                                            uint32_t bshift = 0, uint32_t bwant = 0) {
 #pragma clang fp contract(off)
-  const uint32_t quad = lc >> 2, sub = lc & 3u;
-  const int i = s.qx0 + 2 * (int)(quad % (uint32_t)s.qw) + (int)(sub & 1u);
-  const int j = s.qy0 + 2 * (int)(quad / (uint32_t)s.qw) + (int)(sub >> 1);
   const int lim = 2 * (int)vp.res;
   if (i >= lim || j >= lim) return false;
   const int64_t px = (int64_t)i * 256, py = (int64_t)j * 256;
   // CCW vertex order (a, b, c) = (0, 1, 2) or (0, 2, 1)
-  const bool sw = s.swap != 0u;
+  const bool sw = (s.flags & 1u) != 0u;
   const int64_t ax = s.X[0], ay = s.Y[0];
   const int64_t bx = sw ? s.X[2] : s.X[1], by = sw ? s.Y[2] : s.Y[1];
   const int64_t cx = sw ? s.X[1] : s.X[2], cy = sw ? s.Y[1] : s.Y[2];
@@ -243,6 +248,7 @@ __device__ __forceinline__ bool shade_fragment(const TriRec& s, uint32_t lc, con
     v[k] = (int)f;
   }
   key = brick_index((uint32_t)v[0], (uint32_t)v[1], (uint32_t)v[2]);
+  if (kBrick && (key >> bshift) != bwant) return false;
   // normal: per-vertex normals (record), interpolated, normalised in the
   // fragment shader (voxelize.frag:134)
   float n[3];
@@ -294,19 +300,25 @@ __device__ __forceinline__ bool shade_fragment(const TriRec& s, uint32_t lc, con
   return true;
 }
 
+// Candidate `lc` of the triangle's quad-ordered bounding box (the canonical
+// candidate order of the reference running average).
+__device__ __forceinline__ bool shade_fragment(const TriRec& s, uint32_t lc, const VoxParams& vp,
+                                               const DevMaterial* mats, uint32_t& key, float4& color) {
+  const uint32_t quad = lc >> 2, sub = lc & 3u;
+  const int i = s.qx0 + 2 * (int)(quad % (uint32_t)s.qw) + (int)(sub & 1u);
+  const int j = s.qy0 + 2 * (int)(quad / (uint32_t)s.qw) + (int)(sub >> 1);
+  return shade_pixel(s, i, j, vp, mats, key, color);
+}
+
+// Output of the reference-mode raster: appended fragments + per-voxel counts.
 struct RasterOut {
-  // exact mode
-  unsigned long long* sums;  // 2 per voxel: (R | G << 32), (B | A << 32)
-  uint32_t* vcount;          // fragments per voxel
-  // reference mode
+  uint32_t* vcount;  // fragments per voxel
   uint32_t* f_key;
   unsigned long long* f_cand;  // canonical key: original triangle << 32 | candidate
   float4* f_color;
   uint32_t* f_counter;
   uint32_t f_cap;
 };
-
-__device__ __forceinline__ uint32_t fixq(float c) { return (uint32_t)(c * 65536.0f + 0.5f); }
 
 __global__ __launch_bounds__(256) void vox_raster_kernel(uint32_t ntri, VoxParams vp, const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ d_total,
@@ -347,31 +359,7 @@ __global__ __launch_bounds__(256) void vox_raster_kernel(uint32_t ntri, VoxParam
       ckey = ((unsigned long long)r.otri << 32) | (unsigned long long)(c - off[t]);
       if (!have) key = 0xffffffffu;
     }
-    if (vp.average == VCT_AVG_EXACT) {
-      // combine the four lanes of each pixel quad that hit the same voxel
-      const int qb = lane & ~3;
-      uint32_t q[4] = {have ? fixq(color.x) : 0u, have ? fixq(color.y) : 0u, have ? fixq(color.z) : 0u,
-                       have ? fixq(color.w) : 0u};
-      unsigned long long rg = 0, ba = 0;
-      uint32_t cnt = 0;
-      bool leader = have;
-      for (int jq = 0; jq < 4; ++jq) {
-        const uint32_t kj = __shfl(key, qb + jq, 64);
-        const uint32_t r = __shfl(q[0], qb + jq, 64), g = __shfl(q[1], qb + jq, 64);
-        const uint32_t bl = __shfl(q[2], qb + jq, 64), al = __shfl(q[3], qb + jq, 64);
-        if (have && kj == key) {
-          if (qb + jq < lane) leader = false;
-          rg += (unsigned long long)r | ((unsigned long long)g << 32);
-          ba += (unsigned long long)bl | ((unsigned long long)al << 32);
-          ++cnt;
-        }
-      }
-      if (leader) {
-        atomicAdd(&o.sums[2ull * key], rg);
-        atomicAdd(&o.sums[2ull * key + 1], ba);
-        atomicAdd(&o.vcount[key], cnt);
-      }
-    } else {
+    {
       // append (wave-aggregated slot allocation), count per voxel
       const unsigned long long m = __ballot(have);
       uint32_t slot0 = 0;
@@ -391,27 +379,133 @@ __global__ __launch_bounds__(256) void vox_raster_kernel(uint32_t ntri, VoxParam
   }
 }
 
-// Exact mean -> level 0; zeroes the sums for the next frame.
-template <int DT>
-__global__ __launch_bounds__(256) void vox_resolve_exact_kernel(unsigned long long* __restrict__ sums,
-                                                                uint32_t* __restrict__ vcount, void* level0,
-                                                                uint32_t nvox) {
+// ---- exact mean: bricks of the grid accumulated in LDS ----------------------
+// The grid is cut into cubes of kBrick^3 voxels (the whole grid when smaller);
+// a brick is one contiguous run of the Morton-ordered level 0, and the low
+// 3 * log2(kBrick) bits of a voxel's Morton index are its index inside the
+// brick.  Triangles are binned to the bricks their voxel bounding box
+// overlaps; one workgroup per brick rasterises its triangles' pixels inside
+// the brick's footprint on the projection plane, keeps the fragments whose
+// voxel lies in the brick, and sums them in LDS as 16.16 fixed point (exact,
+// so the order of the additions does not matter).  It then writes the
+// brick's level-0 texels (mean = sum / count, 0 where empty) once.
+constexpr int kBrick = 16;
+constexpr int kBrickThreads = 1024;
+
+struct BrickGeom {
+  uint32_t bits;    // log2(brick edge)
+  uint32_t per;     // bricks per grid edge
+};
+
+__device__ __forceinline__ BrickGeom brick_geom(uint32_t res) {
+  BrickGeom g;
+  const uint32_t edge = res < (uint32_t)kBrick ? res : (uint32_t)kBrick;
+  g.bits = 31u - __clz(edge);
+  g.per = res >> g.bits;
+  return g;
+}
+
+struct BrickBox {
+  int b0[3], nb[3];
+  int n;
+};
+
+// Bricks overlapped by the voxel bounding box of a set-up triangle.  Fragment
+// positions are convex combinations of the vertices up to rounding; 0.01
+// voxel of margin covers it.
+__device__ __forceinline__ BrickBox brick_box(const Setup& s, uint32_t res, BrickGeom g) {
 #pragma clang fp contract(off)
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nvox) return;
-  const uint32_t cnt = vcount[v];
+  BrickBox bb;
+  bb.n = 0;
+  if (!s.ok) return bb;
+  const float fres = (float)res;
+  bb.n = 1;
+  for (int k = 0; k < 3; ++k) {
+    float lo = 3.0e38f, hi = -3.0e38f;
+    for (int v = 0; v < 3; ++v) {
+      const float f = fres * (0.5f * s.w[v][k] + 0.5f);
+      lo = fminf(lo, f);
+      hi = fmaxf(hi, f);
+    }
+    lo -= 0.01f;
+    hi += 0.01f;
+    if (!(hi > -1.0f && lo < fres)) {  // every fragment would be dropped
+      bb.n = 0;
+      return bb;
+    }
+    const int vlo = lo <= 0.0f ? 0 : (int)lo;
+    const int vhi = hi >= fres - 1.0f ? (int)res - 1 : (int)hi;
+    bb.b0[k] = vlo >> g.bits;
+    bb.nb[k] = (vhi >> g.bits) - bb.b0[k] + 1;
+    bb.n *= bb.nb[k];
+  }
+  return bb;
+}
+
+__device__ __forceinline__ uint32_t brick_of(const BrickBox& bb, int k) {
+  const int x = bb.b0[0] + k % bb.nb[0];
+  const int y = bb.b0[1] + (k / bb.nb[0]) % bb.nb[1];
+  const int z = bb.b0[2] + k / (bb.nb[0] * bb.nb[1]);
+  return morton3((uint32_t)x, (uint32_t)y, (uint32_t)z);
+}
+
+// 1. setup (lane per stored triangle): record + count into overlapped bricks.
+__global__ __launch_bounds__(256) void vox_brick_count_kernel(DrawBatch b, VoxParams vp, TriRec* __restrict__ recs,
+                                                              uint32_t* brick_count) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const BrickGeom g = brick_geom(vp.res);
+  Setup s;
+  s.ok = false;
+  if (t < b.ntri) {
+    const int d = find_draw(b, t);
+    const Draw& dr = b.d[d];
+    const uint32_t lt = t - dr.tri_base;
+    tri_setup(dr, lt, vp.res, vp.cube_size, s);
+    if (s.ok) recs[t] = make_rec(dr, lt, s);
+  }
+  const BrickBox bb = brick_box(s, vp.res, g);
+  for (int k = 0; __ballot(k < bb.n) != 0ull; ++k) {
+    const bool act = k < bb.n;
+    wave_add_one(brick_count, act ? brick_of(bb, k) : 0u, act);
+  }
+}
+
+// 3. list write (recomputes the same brick box from the same setup).
+__global__ __launch_bounds__(256) void vox_brick_write_kernel(DrawBatch b, VoxParams vp,
+                                                              const uint32_t* __restrict__ brick_off,
+                                                              uint32_t* cursor, uint32_t* __restrict__ list) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const BrickGeom g = brick_geom(vp.res);
+  Setup s;
+  s.ok = false;
+  if (t < b.ntri) {
+    const int d = find_draw(b, t);
+    tri_setup(b.d[d], t - b.d[d].tri_base, vp.res, vp.cube_size, s);
+  }
+  const BrickBox bb = brick_box(s, vp.res, g);
+  for (int k = 0; __ballot(k < bb.n) != 0ull; ++k) {
+    const bool act = k < bb.n;
+    const uint32_t key = act ? brick_of(bb, k) : 0u;
+    const uint32_t slot = wave_add_one(cursor, key, act);
+    if (act) list[brick_off[key] + slot] = t;
+  }
+}
+
+__device__ __forceinline__ uint32_t fixq(float c) { return (uint32_t)(c * 65536.0f + 0.5f); }
+
+// mean of 16.16 fixed-point sums -> stored texel
+template <int DT>
+__device__ __forceinline__ void store_mean(void* level0, uint32_t v, uint32_t cnt, uint32_t r, uint32_t g, uint32_t b,
+                                           uint32_t a) {
+#pragma clang fp contract(off)
   float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
   if (cnt) {
-    const unsigned long long rg = sums[2ull * v], ba = sums[2ull * v + 1];
     const float fc = (float)cnt;
     const float s = 1.0f / 65536.0f;
-    m.x = ((float)(uint32_t)(rg & 0xffffffffu) / fc) * s;
-    m.y = ((float)(uint32_t)(rg >> 32) / fc) * s;
-    m.z = ((float)(uint32_t)(ba & 0xffffffffu) / fc) * s;
-    m.w = ((float)(uint32_t)(ba >> 32) / fc) * s;
-    sums[2ull * v] = 0ull;
-    sums[2ull * v + 1] = 0ull;
-    vcount[v] = 0u;
+    m.x = ((float)r / fc) * s;
+    m.y = ((float)g / fc) * s;
+    m.z = ((float)b / fc) * s;
+    m.w = ((float)a / fc) * s;
   }
   if (DT == VCT_RGBA8) {
     reinterpret_cast<uint32_t*>(level0)[v] =
@@ -427,6 +521,117 @@ __global__ __launch_bounds__(256) void vox_resolve_exact_kernel(unsigned long lo
   } else {
     reinterpret_cast<float4*>(level0)[v] = m;
   }
+}
+
+// 4. one workgroup per brick (blockIdx.x = the brick's Morton index).
+template <int DT>
+__global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, const TriRec* __restrict__ recs,
+                                                                  const DevMaterial* __restrict__ mats,
+                                                                  const uint32_t* __restrict__ brick_count,
+                                                                  const uint32_t* __restrict__ brick_off,
+                                                                  const uint32_t* __restrict__ list,
+                                                                  void* level0) {
+  constexpr int kVox = kBrick * kBrick * kBrick;
+  __shared__ uint32_t s_sum[4][kVox];
+  __shared__ uint32_t s_cnt[kVox];
+  __shared__ uint32_t s_end[kBrickThreads];  // inclusive prefix of candidate counts
+  __shared__ uint32_t s_org[kBrickThreads];  // clipped box origin: i | j << 16
+  __shared__ uint32_t s_div[kBrickThreads];  // ceil(65536 / width) | width << 20
+  __shared__ uint32_t s_tri[kBrickThreads];
+  __shared__ uint32_t s_wsum[kBrickThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const BrickGeom g = brick_geom(vp.res);
+  const uint32_t brick = blockIdx.x;
+  const uint32_t nvox = 1u << (3 * g.bits);
+  const uint32_t bc[3] = {compact3(brick), compact3(brick >> 1), compact3(brick >> 2)};
+  for (uint32_t v = tid; v < nvox; v += kBrickThreads) {
+    s_sum[0][v] = 0u;
+    s_sum[1][v] = 0u;
+    s_sum[2][v] = 0u;
+    s_sum[3][v] = 0u;
+    s_cnt[v] = 0u;
+  }
+  const uint32_t n = brick_count[brick], base = brick_off[brick];
+  const int lim = 2 * (int)vp.res - 1;
+  const int span = 2 << g.bits;  // footprint of the brick on the raster, pixels per axis
+  __syncthreads();
+  for (uint32_t b0 = 0; b0 < n; b0 += kBrickThreads) {
+    uint32_t cnt = 0;
+    if (b0 + tid < n) {
+      const uint32_t t = list[base + b0 + tid];
+      const TriRec& r = recs[t];
+      const uint32_t fl = r.flags;
+      const int axis = (int)((fl >> 1) & 3u);
+      const int ia = axis == 0 ? 1 : 0, ib = axis == 2 ? 1 : 2;
+      const int qh = (int)(fl >> 8);
+      // footprint with one pixel of margin, clipped to the triangle's box
+      const int fx = (int)bc[ia] * span, fy = (int)bc[ib] * span;
+      const int i0 = max(r.qx0, fx - 1), i1 = min(min(r.qx0 + 2 * r.qw - 1, fx + span), lim);
+      const int j0 = max(r.qy0, fy - 1), j1 = min(min(r.qy0 + 2 * qh - 1, fy + span), lim);
+      if (i1 >= i0 && j1 >= j0) {
+        const uint32_t cw = (uint32_t)(i1 - i0 + 1);
+        cnt = cw * (uint32_t)(j1 - j0 + 1);
+        s_org[tid] = (uint32_t)i0 | ((uint32_t)j0 << 16);
+        s_div[tid] = ((65536u + cw - 1u) / cw) | (cw << 20);
+        s_tri[tid] = t;
+      }
+    }
+    uint32_t x = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wsum[wave] = x;
+    __syncthreads();
+    for (int w = 0; w < wave; ++w) x += s_wsum[w];
+    s_end[tid] = x;
+    __syncthreads();
+    const uint32_t total = s_end[kBrickThreads - 1];
+    // every lane takes one contiguous run of candidates: one search, then a
+    // walk through the boxes with the triangle record held in registers
+    const uint32_t per = (total + kBrickThreads - 1u) / kBrickThreads;
+    uint32_t c = (uint32_t)tid * per;
+    const uint32_t cend = min(c + per, total);
+    if (c < cend) {
+      uint32_t slot = 0, hi = kBrickThreads - 1;
+      while (slot < hi) {
+        const uint32_t mid = (slot + hi) >> 1;
+        if (s_end[mid] > c) hi = mid;
+        else slot = mid + 1;
+      }
+      while (c < cend) {
+        const uint32_t end = s_end[slot];
+        const uint32_t local = c - (slot ? s_end[slot - 1] : 0u);
+        const uint32_t dv = s_div[slot], cw = dv >> 20;
+        uint32_t row = (local * (dv & 0xfffffu)) >> 16;  // exact: local < 35 * 35, cw <= 35
+        uint32_t col = local - row * cw;
+        const uint32_t org = s_org[slot];
+        const int oi = (int)(org & 0xffffu), oj = (int)(org >> 16);
+        const TriRec r = recs[s_tri[slot]];
+        const uint32_t stop = min(end, cend);
+        for (; c < stop; ++c) {
+          uint32_t key;
+          float4 color;
+          if (shade_pixel<true>(r, oi + (int)col, oj + (int)row, vp, mats, key, color, 3 * g.bits, brick)) {
+            const uint32_t v = key & (nvox - 1u);
+            atomicAdd(&s_sum[0][v], fixq(color.x));
+            atomicAdd(&s_sum[1][v], fixq(color.y));
+            atomicAdd(&s_sum[2][v], fixq(color.z));
+            atomicAdd(&s_sum[3][v], fixq(color.w));
+            atomicAdd(&s_cnt[v], 1u);
+          }
+          if (++col == cw) {
+            col = 0;
+            ++row;
+          }
+        }
+        ++slot;  // listed triangles have positive candidate counts
+      }
+    }
+    __syncthreads();
+  }
+  for (uint32_t v = tid; v < nvox; v += kBrickThreads)
+    store_mean<DT>(level0, brick * nvox + v, s_cnt[v], s_sum[0][v], s_sum[1][v], s_sum[2][v], s_sum[3][v]);
 }
 
 // ---- reference running average -------------------------------------------
@@ -563,63 +768,121 @@ static LightSet make_lights(const vct_context* ctx) {
   return ls;
 }
 
+static void fill_batch(vct_context* ctx, const std::vector<uint32_t>& q, size_t qi, uint32_t& canon, DrawBatch& b) {
+  memset(&b, 0, sizeof(b));
+  uint32_t ntri = 0;
+  for (size_t k = qi; k < q.size() && k < qi + kMaxDraws; ++k) {
+    const VctModel& m = ctx->models[q[k]];
+    Draw& d = b.d[b.ndraw++];
+    d.pos = m.d_pos;
+    d.nrm = m.d_nrm;
+    d.mat = m.d_mat;
+    d.perm = m.d_perm;
+    d.tri_base = ntri;
+    d.canon_base = canon;
+    canon += m.ntri;
+    d.ntri = m.ntri;
+    memcpy(d.M, m.xform, 64);
+    normal_matrix(m.xform, d.N);
+    ntri += m.ntri;
+  }
+  b.ntri = ntri;
+}
+
+template <int DT>
+static void launch_brick(vct_context* ctx, uint32_t nbricks, const VoxParams& vp, const uint32_t* count,
+                         const uint32_t* off, const uint32_t* list, void* level0) {
+  hipLaunchKernelGGL(vox_brick_kernel<DT>, dim3(nbricks), dim3(kBrickThreads), 0, ctx->stream, vp,
+                     (const TriRec*)ctx->setup.p, (const DevMaterial*)ctx->d_materials.p, count, off, list, level0);
+}
+
+// Exact mean: all queued draws in one batch (brick lists span every draw).
+static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, const VoxParams& vp, void* level0) {
+  const uint32_t res = ctx->res;
+  const uint32_t edge = res < (uint32_t)kBrick ? res : (uint32_t)kBrick;
+  const uint32_t per = res / edge;
+  const uint32_t nbricks = per * per * per;
+  uint32_t* counters = (uint32_t*)ctx->counters.p;
+  int rc;
+  // [0, nbricks): counts, [nbricks, 2 nbricks): cursors; then offsets
+  if ((rc = vct_ensure(ctx->vox_count, (size_t)nbricks * 8 + 16, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->vox_off, (size_t)nbricks * 4 + 16, ctx))) return rc;
+  uint32_t* bcount = (uint32_t*)ctx->vox_count.p;
+  uint32_t* cursor = bcount + nbricks;
+  uint32_t* boff = (uint32_t*)ctx->vox_off.p;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3((2 * nbricks + 255) / 256), dim3(256), 0, ctx->stream, bcount,
+                     (uint64_t)2 * nbricks);
+  VCT_HIP(hipGetLastError());
+  if (q.size() > (size_t)kMaxDraws) return vct_set_error(VCT_ERR_LIMIT, "at most 16 queued models per frame");
+  DrawBatch b;
+  uint32_t canon = 0;
+  fill_batch(ctx, q, 0, canon, b);
+  const uint32_t ntri = b.ntri;
+  uint32_t total = 0;
+  if (ntri) {
+    if ((rc = vct_ensure(ctx->setup, (size_t)ntri * sizeof(TriRec), ctx))) return rc;
+    hipLaunchKernelGGL(vox_brick_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
+                       (TriRec*)ctx->setup.p, bcount);
+    VCT_HIP(hipGetLastError());
+    if ((rc = vct_launch_scan_u32(ctx, bcount, boff, nbricks, counters + 2))) return rc;
+    VCT_HIP(hipMemcpyAsync(&total, counters + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
+    VCT_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  if (total) {
+    if ((rc = vct_ensure(ctx->tri_off, (size_t)total * 4 + (size_t)total / 4 * 4, ctx))) return rc;
+    hipLaunchKernelGGL(vox_brick_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
+                       (const uint32_t*)boff, cursor, (uint32_t*)ctx->tri_off.p);
+    VCT_HIP(hipGetLastError());
+  } else {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((nbricks + 255) / 256), dim3(256), 0, ctx->stream, boff,
+                       (uint64_t)nbricks);
+    VCT_HIP(hipGetLastError());
+    if ((rc = vct_ensure(ctx->tri_off, 16, ctx))) return rc;
+  }
+  if ((rc = vct_ensure(ctx->setup, sizeof(TriRec), ctx))) return rc;
+  ctx->last_candidates = total;  // (triangle, brick) pairs
+  const uint32_t* list = (const uint32_t*)ctx->tri_off.p;
+  switch (ctx->dtype) {
+    case VCT_RGBA8: launch_brick<VCT_RGBA8>(ctx, nbricks, vp, bcount, boff, list, level0); break;
+    case VCT_FP16: launch_brick<VCT_FP16>(ctx, nbricks, vp, bcount, boff, list, level0); break;
+    case VCT_BF16: launch_brick<VCT_BF16>(ctx, nbricks, vp, bcount, boff, list, level0); break;
+    default: launch_brick<VCT_FP32>(ctx, nbricks, vp, bcount, boff, list, level0); break;
+  }
+  VCT_HIP(hipGetLastError());
+  return VCT_OK;
+}
+
 int vct_launch_voxelize(vct_context* ctx) {
   const uint32_t res = ctx->res;
   const uint64_t nvox = (uint64_t)res * res * res;
-  const bool exact = ctx->average == VCT_AVG_EXACT;
   void* level0 = (char*)ctx->volume.p + ctx->lvl_off[0][0];
   int rc;
   if ((rc = vct_ensure(ctx->counters, 256, ctx))) return rc;
-  uint32_t* counters = (uint32_t*)ctx->counters.p;  // [0] candidates total, [1] fragments
-  if (exact) {
-    if ((rc = vct_ensure(ctx->accum, nvox * 16, ctx))) return rc;
-    if ((rc = vct_ensure(ctx->vox_count, nvox * 4, ctx))) return rc;
-    if (!ctx->accum_zero) {
-      hipLaunchKernelGGL(zero_u32_kernel, dim3(2048), dim3(256), 0, ctx->stream, (uint32_t*)ctx->accum.p, nvox * 4);
-      hipLaunchKernelGGL(zero_u32_kernel, dim3(2048), dim3(256), 0, ctx->stream, (uint32_t*)ctx->vox_count.p, nvox);
-      ctx->accum_zero = true;
-    }
-  } else {
-    if ((rc = vct_ensure(ctx->vox_count, nvox * 4, ctx))) return rc;
-    if ((rc = vct_ensure(ctx->vox_off, nvox * 4, ctx))) return rc;
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(2048), dim3(256), 0, ctx->stream, (uint32_t*)ctx->vox_count.p, nvox);
-    VCT_HIP(hipMemsetAsync(counters + 1, 0, 4, ctx->stream));
-  }
+  uint32_t* counters = (uint32_t*)ctx->counters.p;  // [0] candidates total, [1] fragments, [2] scan totals
   VoxParams vp;
   memset(&vp, 0, sizeof(vp));
   vp.res = res;
   vp.cube_size = ctx->cube_size;
   vp.average = ctx->average;
   vp.lights = make_lights(ctx);
-
-  // batches of queued draws
   std::vector<uint32_t> q;
   for (uint32_t id : ctx->queue)
     if (id < ctx->models.size() && ctx->models[id].ntri) q.push_back(id);
-  uint64_t cand_total = 0;
-  uint32_t canon = 0;
   ctx->last_candidates = 0;
   ctx->last_fragments = 0;
+  if (ctx->average == VCT_AVG_EXACT) return voxelize_exact(ctx, q, vp, level0);
+
+  // reference running average: append, bucket per voxel, canonical replay
+  if ((rc = vct_ensure(ctx->vox_count, nvox * 4, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->vox_off, nvox * 4, ctx))) return rc;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(2048), dim3(256), 0, ctx->stream, (uint32_t*)ctx->vox_count.p, nvox);
+  VCT_HIP(hipMemsetAsync(counters + 1, 0, 4, ctx->stream));
+  uint64_t cand_total = 0;
+  uint32_t canon = 0;
   for (size_t qi = 0; qi < q.size(); qi += kMaxDraws) {
     DrawBatch b;
-    memset(&b, 0, sizeof(b));
-    uint32_t ntri = 0;
-    for (size_t k = qi; k < q.size() && k < qi + kMaxDraws; ++k) {
-      const VctModel& m = ctx->models[q[k]];
-      Draw& d = b.d[b.ndraw++];
-      d.pos = m.d_pos;
-      d.nrm = m.d_nrm;
-      d.mat = m.d_mat;
-      d.perm = m.d_perm;
-      d.tri_base = ntri;
-      d.canon_base = canon;
-      canon += m.ntri;
-      d.ntri = m.ntri;
-      memcpy(d.M, m.xform, 64);
-      normal_matrix(m.xform, d.N);
-      ntri += m.ntri;
-    }
-    b.ntri = ntri;
+    fill_batch(ctx, q, qi, canon, b);
+    const uint32_t ntri = b.ntri;
     if ((rc = vct_ensure(ctx->tri_count, (size_t)ntri * 4 + 4, ctx))) return rc;
     if ((rc = vct_ensure(ctx->tri_off, (size_t)ntri * 4 + 4, ctx))) return rc;
     if ((rc = vct_ensure(ctx->setup, (size_t)ntri * sizeof(TriRec), ctx))) return rc;
@@ -630,74 +893,50 @@ int vct_launch_voxelize(vct_context* ctx) {
       return rc;
     RasterOut o;
     memset(&o, 0, sizeof(o));
-    o.sums = (unsigned long long*)ctx->accum.p;
     o.vcount = (uint32_t*)ctx->vox_count.p;
-    if (!exact) {
-      // candidates bound the fragment count: read it back to size the buffers
-      uint32_t total = 0;
-      VCT_HIP(hipMemcpyAsync(&total, counters, 4, hipMemcpyDeviceToHost, ctx->stream));
-      VCT_HIP(hipStreamSynchronize(ctx->stream));
-      if (cand_total + total > 0xffffffffull) return vct_set_error(VCT_ERR_LIMIT, "more than 2^32 fragments");
-      uint32_t nf_before = 0;
-      VCT_HIP(hipMemcpyAsync(&nf_before, counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
-      VCT_HIP(hipStreamSynchronize(ctx->stream));
-      size_t cap = (size_t)nf_before + total;
-      // grow keeping existing fragments: [key u32][canonical key u64][colour float4] x cap
-      if (cap > ctx->frags.cap / 28 || !ctx->frags.p) {
-        VctBuf nb;
-        const size_t ncap = (cap + cap / 4 + 1024 + 3) & ~(size_t)3;  // 8- and 16-byte aligned arrays
-        if ((rc = vct_ensure(nb, ncap * 28, ctx))) return rc;
-        if (ctx->frags.p && nf_before) {
-          const size_t oc = ctx->frags.cap / 28;
-          char* ob = (char*)ctx->frags.p;
-          char* nbp = (char*)nb.p;
-          VCT_HIP(hipMemcpyAsync(nbp, ob, (size_t)nf_before * 4, hipMemcpyDeviceToDevice, ctx->stream));
-          VCT_HIP(hipMemcpyAsync(nbp + ncap * 4, ob + oc * 4, (size_t)nf_before * 8, hipMemcpyDeviceToDevice,
-                                 ctx->stream));
-          VCT_HIP(hipMemcpyAsync(nbp + ncap * 12, ob + oc * 12, (size_t)nf_before * 16, hipMemcpyDeviceToDevice,
-                                 ctx->stream));
-          VCT_HIP(hipStreamSynchronize(ctx->stream));
-        }
-        if (ctx->frags.p) VCT_HIP(hipFree(ctx->frags.p));
-        ctx->frags = nb;
-        ctx->frags.cap = ncap * 28;
+    // candidates bound the fragment count: read it back to size the buffers
+    uint32_t total = 0;
+    VCT_HIP(hipMemcpyAsync(&total, counters, 4, hipMemcpyDeviceToHost, ctx->stream));
+    VCT_HIP(hipStreamSynchronize(ctx->stream));
+    if (cand_total + total > 0xffffffffull) return vct_set_error(VCT_ERR_LIMIT, "more than 2^32 fragments");
+    uint32_t nf_before = 0;
+    VCT_HIP(hipMemcpyAsync(&nf_before, counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+    VCT_HIP(hipStreamSynchronize(ctx->stream));
+    size_t cap = (size_t)nf_before + total;
+    // grow keeping existing fragments: [key u32][canonical key u64][colour float4] x cap
+    if (cap > ctx->frags.cap / 28 || !ctx->frags.p) {
+      VctBuf nb;
+      const size_t ncap = (cap + cap / 4 + 1024 + 3) & ~(size_t)3;  // 8- and 16-byte aligned arrays
+      if ((rc = vct_ensure(nb, ncap * 28, ctx))) return rc;
+      if (ctx->frags.p && nf_before) {
+        const size_t oc = ctx->frags.cap / 28;
+        char* ob = (char*)ctx->frags.p;
+        char* nbp = (char*)nb.p;
+        VCT_HIP(hipMemcpyAsync(nbp, ob, (size_t)nf_before * 4, hipMemcpyDeviceToDevice, ctx->stream));
+        VCT_HIP(hipMemcpyAsync(nbp + ncap * 4, ob + oc * 4, (size_t)nf_before * 8, hipMemcpyDeviceToDevice,
+                               ctx->stream));
+        VCT_HIP(hipMemcpyAsync(nbp + ncap * 12, ob + oc * 12, (size_t)nf_before * 16, hipMemcpyDeviceToDevice,
+                               ctx->stream));
+        VCT_HIP(hipStreamSynchronize(ctx->stream));
       }
-      const size_t fc = ctx->frags.cap / 28;
-      o.f_key = (uint32_t*)ctx->frags.p;
-      o.f_cand = (unsigned long long*)((char*)ctx->frags.p + fc * 4);
-      o.f_color = (float4*)((char*)ctx->frags.p + fc * 12);
-      o.f_counter = counters + 1;
-      o.f_cap = (uint32_t)fc;
-      cand_total += total;
-      ctx->last_candidates += total;
+      if (ctx->frags.p) VCT_HIP(hipFree(ctx->frags.p));
+      ctx->frags = nb;
+      ctx->frags.cap = ncap * 28;
     }
+    const size_t fc = ctx->frags.cap / 28;
+    o.f_key = (uint32_t*)ctx->frags.p;
+    o.f_cand = (unsigned long long*)((char*)ctx->frags.p + fc * 4);
+    o.f_color = (float4*)((char*)ctx->frags.p + fc * 12);
+    o.f_counter = counters + 1;
+    o.f_cap = (uint32_t)fc;
+    cand_total += total;
+    ctx->last_candidates += total;
     hipLaunchKernelGGL(vox_raster_kernel, dim3(256 * 8), dim3(256), 0, ctx->stream, ntri, vp,
                        (const uint32_t*)ctx->tri_off.p, (const uint32_t*)counters, (const TriRec*)ctx->setup.p,
                        (const DevMaterial*)ctx->d_materials.p, o);
     VCT_HIP(hipGetLastError());
   }
   const uint32_t nb = (uint32_t)((nvox + 255) / 256);
-  if (exact) {
-    switch (ctx->dtype) {
-      case VCT_RGBA8:
-        hipLaunchKernelGGL(vox_resolve_exact_kernel<VCT_RGBA8>, dim3(nb), dim3(256), 0, ctx->stream,
-                           (unsigned long long*)ctx->accum.p, (uint32_t*)ctx->vox_count.p, level0, (uint32_t)nvox);
-        break;
-      case VCT_FP16:
-        hipLaunchKernelGGL(vox_resolve_exact_kernel<VCT_FP16>, dim3(nb), dim3(256), 0, ctx->stream,
-                           (unsigned long long*)ctx->accum.p, (uint32_t*)ctx->vox_count.p, level0, (uint32_t)nvox);
-        break;
-      case VCT_BF16:
-        hipLaunchKernelGGL(vox_resolve_exact_kernel<VCT_BF16>, dim3(nb), dim3(256), 0, ctx->stream,
-                           (unsigned long long*)ctx->accum.p, (uint32_t*)ctx->vox_count.p, level0, (uint32_t)nvox);
-        break;
-      default:
-        hipLaunchKernelGGL(vox_resolve_exact_kernel<VCT_FP32>, dim3(nb), dim3(256), 0, ctx->stream,
-                           (unsigned long long*)ctx->accum.p, (uint32_t*)ctx->vox_count.p, level0, (uint32_t)nvox);
-    }
-    VCT_HIP(hipGetLastError());
-    return VCT_OK;
-  }
   // reference mode: bucket + replay
   uint32_t nfrag = 0;
   VCT_HIP(hipMemcpyAsync(&nfrag, counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
