@@ -146,6 +146,26 @@ def algorithmic(name, wl, bpe, steps, fb_tiles):
     return 0, "B", "hbm", ""
 
 
+def copy_calibration(nbytes: int) -> None:
+    """Two device-to-device copies of nbytes (the runtime's copy kernel), so
+    that a counter run over this program also measures a copy of known size:
+    its FETCH_SIZE / WRITE_SIZE per dispatch give the counters' scale."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    for _ in range(2):
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"bench: calibration copies of {nbytes} bytes done")
+
+
 def run_form(ctx, sc, learned, steps, warmup):
     def frame():
         sc.queue()
@@ -183,6 +203,9 @@ def main():
     ap.add_argument("--fallback-db", type=float, default=40.0)
     ap.add_argument("--no-quality", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="run two device copies of BYTES first (scale of the HBM byte counters)")
+    ap.add_argument("--plain-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
@@ -198,6 +221,8 @@ def main():
     wl = WORKLOADS[args.workload]
     log(f"bench: workload {wl.name} ({wl.ntri} tris, {wl.res}^3, {wl.width}x{wl.height}), dtype {args.dtype}")
     ctx = Context(wl.width, wl.height)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     t0 = time.perf_counter()
     sc = Scene(ctx, wl, dt)
     log(f"bench: scene uploaded in {time.perf_counter() - t0:.1f} s")
@@ -214,7 +239,7 @@ def main():
     steps = ctx.trace_steps()
 
     # ---- learned form
-    have_weights = Path(args.weights).exists()
+    have_weights = Path(args.weights).exists() and not args.plain_only
     learned = None
     if have_weights:
         ctx.mlp_set_params(np.load(args.weights))
@@ -224,12 +249,13 @@ def main():
         ctx.render_learned()
         l_img = ctx.download_image()
         nt, nf = ctx.fallback_stats()
+        traced = ctx.traced_pixels()
         ctx.set_fallback(0.0)
         sc.queue()
         ctx.render_learned()
         raw_img = ctx.download_image()
         ctx.set_fallback(args.fallback_db)
-        learned = dict(ms=l_ms, passes=l_passes, img=l_img, raw=raw_img, tiles=nt, fb=nf)
+        learned = dict(ms=l_ms, passes=l_passes, img=l_img, raw=raw_img, tiles=nt, fb=nf, traced=traced)
         log(f"bench: learned {l_ms:.3f} ms/frame; passes " +
             " ".join(f"{n} {l_passes[i]:.3f}" for i, n in enumerate(PASSES)) + f"; fallback tiles {nf}/{nt}")
     else:
@@ -306,6 +332,7 @@ def main():
         out["learned_passes_ms"] = {n: round(float(learned["passes"][i]), 4) for i, n in enumerate(PASSES)}
         out["speedup_learned_vs_plain"] = round(plain_ms / learned["ms"], 3)
         out["fallback_tiles"] = f"{learned['fb']}/{learned['tiles']}"
+        out["traced_pixels"] = f"{learned['traced']}/{int(cov_full.sum())} covered"
     out["quality"] = quality
     out["trace"] = {
         "cone_steps": steps,

@@ -277,17 +277,21 @@ int vct_upsample_sample_count(vct_ctx ctx, uint64_t* n);
 /* Adam steps (beta 0.9 / 0.999) on random windows of `batch` samples. */
 int vct_upsample_train(vct_ctx ctx, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float* last_loss);
 /* The learned form of Renderer::render: voxelize, mip, full-resolution
- * G-buffer, quarter-resolution cone trace, MLP upsampling, then the per-tile
- * fallback: every 32x32 tile is compared with a full-resolution trace at 16
- * probe pixels and traced again in full when its estimated PSNR is below the
- * bound.  Pass times [voxelize, mip, gbuffer, low-res trace, upsample,
- * fallback]. */
+ * G-buffer, quarter-resolution cone trace, MLP upsampling (surface-guided:
+ * a pixel with no quarter-resolution sample of its own surface nearby is
+ * traced at full resolution instead), then the per-tile fallback: every
+ * 32x32 tile is compared with a full-resolution trace at 16 probe pixels and
+ * traced again in full when its estimated PSNR is below the bound.  Pass
+ * times [voxelize, mip, gbuffer, low-res trace, upsample, fallback]. */
 int vct_render_learned(vct_ctx ctx);
 /* Minimum estimated tile PSNR in dB (peak 1, over the tile's covered probe
  * pixels, RGB); default 40; <= 0 turns the fallback off. */
 int vct_upsample_set_fallback(vct_ctx ctx, float min_tile_psnr_db);
 /* Tiles of the viewport and tiles traced again by the last learned frame. */
 int vct_upsample_fallback_stats(vct_ctx ctx, uint32_t* tiles, uint32_t* fallback_tiles);
+/* Pixels of the last learned frame traced at full resolution because no
+ * quarter-resolution sample of their surface was near. */
+int vct_upsample_traced_pixels(vct_ctx ctx, uint32_t* pixels);
 int vct_upsample_lowres_download(vct_ctx ctx, float* rgba, size_t pixels);
 
 #ifdef __cplusplus

@@ -142,12 +142,16 @@ def _bilinear(low, gn_alb, W, H):
     return (top * (1 - av) + bot * av) * gn_alb
 
 
-def test_learned_upsampler_beats_bilinear(gpu_available):
+def test_learned_upsampler_beats_its_interpolation_and_bilinear(gpu_available):
+    """What the trained network adds: the learned frame (per-tile fallback off)
+    against the same frame with a zero network (the surface-guided
+    interpolation alone, same per-pixel tracing) and against bilinear
+    upsampling of the same quarter-resolution buffer; then the per-tile
+    fallback at 40 dB."""
     W, H = 256, 192
     sc, ctx = _soup_ctx(W, H)
     try:
         ctx.mlp_init(11)
-        # training frames
         for i, view in enumerate(_poses(24, 5)):
             ctx.set_camera(view, sc.proj)
             sc_q(ctx, sc)
@@ -156,17 +160,29 @@ def test_learned_upsampler_beats_bilinear(gpu_available):
             ctx.gbuffer()
             ctx.upsample_collect(8192, 1000 + i)
         say(f"samples {ctx.upsample_count()}")
-        loss = ctx.upsample_train(600, 8192, 2e-3, seed=3)
+        loss = ctx.upsample_train(1500, 8192, 2e-3, seed=3)
         say(f"final training loss {loss:.3e}")
-        # held-out pose
-        view = _poses(1, 99)[0]
+        trained = ctx.mlp_params()
+        view = _poses(1, 99)[0]  # held-out pose
         ctx.set_camera(view, sc.proj)
         plain = _plain(ctx, sc)
+        ctx.set_fallback(0.0)
         sc_q(ctx, sc)
         ctx.render_learned()
         learned = ctx.download_image()
+        traced = ctx.traced_pixels()
         low = ctx.download_lowres()
         gp, gn = ctx.download_gbuffer()
+        ctx.mlp_set_params(np.zeros_like(trained))
+        sc_q(ctx, sc)
+        ctx.render_learned()
+        guided = ctx.download_image()
+        ctx.mlp_set_params(trained)
+        ctx.set_fallback(40.0)
+        sc_q(ctx, sc)
+        ctx.render_learned()
+        checked = ctx.download_image()
+        tiles, fb = ctx.fallback_stats()
     finally:
         ctx.close()
     cov = gp[..., 3] != 0
@@ -174,10 +190,15 @@ def test_learned_upsampler_beats_bilinear(gpu_available):
     mid = gn[..., 3].copy().view(np.uint32)
     alb = mats[np.where(cov, mid, 0)][..., 4:7]
     bil = _bilinear(low, alb, W, H)
-    e_l = learned[..., :3][cov] - plain[..., :3][cov]
-    e_b = bil[cov] - plain[..., :3][cov]
-    psnr_l = 10 * np.log10(1.0 / np.mean(e_l ** 2))
-    psnr_b = 10 * np.log10(1.0 / np.mean(e_b ** 2))
-    say(f"PSNR vs plain full-res: learned {psnr_l:.2f} dB, bilinear {psnr_b:.2f} dB")
+
+    def psnr(img):
+        e = img[..., :3][cov].astype(np.float64) - plain[..., :3][cov]
+        return 10 * np.log10(1.0 / np.mean(e ** 2))
+
+    p_l, p_g, p_b, p_c = psnr(learned), psnr(guided), psnr(bil), psnr(checked)
+    say(f"PSNR vs plain full-res: learned {p_l:.2f} dB, guided interpolation {p_g:.2f} dB, bilinear {p_b:.2f} dB; "
+        f"traced pixels {traced}/{int(cov.sum())}; with tile fallback {p_c:.2f} dB ({fb}/{tiles} tiles)")
     assert np.isfinite(learned).all()
-    assert psnr_l > psnr_b + 1.0
+    assert p_l > p_g
+    assert p_l > p_b + 1.0
+    assert p_c >= 40.0

@@ -150,6 +150,7 @@ PROTOTYPES = {
     "vct_upsample_lowres_download": (_I, [_P, _FP, _SZ]),
     "vct_upsample_set_fallback": (_I, [_P, _F]),
     "vct_upsample_fallback_stats": (_I, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "vct_upsample_traced_pixels": (_I, [_P, C.POINTER(C.c_uint32)]),
     "vct_scene_soup": (_I, [C.c_uint64, _U, _F, _F, _F, _F, _F, _FP, _FP, _FP]),
     "vct_scene_cornell": (_I, [C.c_uint64, _FP, _FP, _UP, C.POINTER(Material)]),
 }

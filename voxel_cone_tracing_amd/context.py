@@ -238,6 +238,13 @@ class Context:
         L.check(self.lib.vct_upsample_fallback_stats(self.h, C.byref(t), C.byref(f)))
         return t.value, f.value
 
+    def traced_pixels(self) -> int:
+        """Pixels of the last learned frame traced at full resolution (no
+        quarter-resolution sample of their surface nearby)."""
+        n = C.c_uint32()
+        L.check(self.lib.vct_upsample_traced_pixels(self.h, C.byref(n)))
+        return n.value
+
     def download_lowres(self) -> np.ndarray:
         img = np.zeros((self.height // 4, self.width // 4, 4), np.float32)
         L.check(self.lib.vct_upsample_lowres_download(self.h, L.fptr(img), img.shape[0] * img.shape[1]))

@@ -125,28 +125,59 @@ __device__ __forceinline__ void g_setup(const GDraw& dr, const float* PV, uint32
 
 __device__ __forceinline__ bool g_top_left(int64_t dx, int64_t dy) { return dy < 0 || (dy == 0 && dx < 0); }
 
-// Barycentric weights (of vertices 0,1,2) of pixel (i,j); false if outside.
-__device__ __forceinline__ bool g_cover(const GSetup& s, int i, int j, float& b0, float& b1, float& b2) {
-#pragma clang fp contract(off)
+// Edge functions of a set-up triangle in CCW order (a, b, c) = (0, 1, 2) or
+// (0, 2, 1): w_a(i, j) = wa0 + i * dia + j * dja (exact 64-bit integers at
+// pixel centres, coordinates in 1/256 pixel), likewise for b and c.
+struct GEdges {
+  int64_t w[3];    // at the start pixel
+  int64_t di[3];   // per column
+  int64_t dj[3];   // per row
+  int64_t ex[3], ey[3];  // edge vectors (for the top-left rule)
+};
+
+__device__ __forceinline__ void g_edges(const GSetup& s, int i, int j, GEdges& e) {
   const int64_t px = (int64_t)i * 256, py = (int64_t)j * 256;
   const int ia = 0, ib = s.swap ? 2 : 1, ic = s.swap ? 1 : 2;
   const int64_t ax = s.X[ia], ay = s.Y[ia], bx = s.X[ib], by = s.Y[ib], cx = s.X[ic], cy = s.Y[ic];
-  int64_t wa = (cx - bx) * (py - by) - (cy - by) * (px - bx);
-  int64_t wb = (ax - cx) * (py - cy) - (ay - cy) * (px - cx);
-  int64_t wc = (bx - ax) * (py - ay) - (by - ay) * (px - ax);
+  e.w[0] = (cx - bx) * (py - by) - (cy - by) * (px - bx);
+  e.w[1] = (ax - cx) * (py - cy) - (ay - cy) * (px - cx);
+  e.w[2] = (bx - ax) * (py - ay) - (by - ay) * (px - ax);
+  e.di[0] = -(cy - by) * 256;
+  e.di[1] = -(ay - cy) * 256;
+  e.di[2] = -(by - ay) * 256;
+  e.dj[0] = (cx - bx) * 256;
+  e.dj[1] = (ax - cx) * 256;
+  e.dj[2] = (bx - ax) * 256;
+  e.ex[0] = cx - bx;
+  e.ey[0] = cy - by;
+  e.ex[1] = ax - cx;
+  e.ey[1] = ay - cy;
+  e.ex[2] = bx - ax;
+  e.ey[2] = by - ay;
+}
+
+// Coverage (top-left rule) and barycentric weights of vertices 0, 1, 2 from
+// the edge values (wa, wb, wc) of a pixel.
+__device__ __forceinline__ bool g_cover_w(const GSetup& s, const GEdges& e, int64_t wa, int64_t wb, int64_t wc,
+                                          float& b0, float& b1, float& b2) {
+#pragma clang fp contract(off)
   if (wa < 0 || wb < 0 || wc < 0) return false;
-  if (wa == 0 && !g_top_left(cx - bx, cy - by)) return false;
-  if (wb == 0 && !g_top_left(ax - cx, ay - cy)) return false;
-  if (wc == 0 && !g_top_left(bx - ax, by - ay)) return false;
-  int64_t w[3];
-  w[ia] = wa;
-  w[ib] = wb;
-  w[ic] = wc;
+  if (wa == 0 && !g_top_left(e.ex[0], e.ey[0])) return false;
+  if (wb == 0 && !g_top_left(e.ex[1], e.ey[1])) return false;
+  if (wc == 0 && !g_top_left(e.ex[2], e.ey[2])) return false;
+  const int64_t w0 = wa, w1 = s.swap ? wc : wb, w2 = s.swap ? wb : wc;
   const float fa = (float)s.area;
-  b0 = (float)w[0] / fa;
-  b1 = (float)w[1] / fa;
-  b2 = (float)w[2] / fa;
+  b0 = (float)w0 / fa;
+  b1 = (float)w1 / fa;
+  b2 = (float)w2 / fa;
   return true;
+}
+
+// Barycentric weights (of vertices 0,1,2) of pixel (i,j); false if outside.
+__device__ __forceinline__ bool g_cover(const GSetup& s, int i, int j, float& b0, float& b1, float& b2) {
+  GEdges e;
+  g_edges(s, i, j, e);
+  return g_cover_w(s, e, e.w[0], e.w[1], e.w[2], b0, b1, b2);
 }
 
 // Per-triangle record of the bin pass, read by every tile the triangle overlaps.
@@ -292,9 +323,35 @@ __device__ __forceinline__ void g_resolve_pixel(const GParams& p, unsigned long 
 // 4. one workgroup per 32x32 tile: the depth test runs on a tile depth buffer
 //    in LDS ((depth bits << 32 | original triangle id), 64-bit LDS atomicMin;
 //    equal depths resolve to the lower triangle id, the draw-order winner of
-//    GL_LESS).  The tile's triangles are taken 256 at a time; the pixels of
-//    their bounding boxes (clipped to the tile) are spread over the 256 lanes.
-//    Then every pixel of the tile is resolved and written once.
+//    GL_LESS).  The tile's triangles are taken 256 at a time, one per lane:
+//    a lane walks its triangle's box (clipped to the tile) with incremental
+//    edge functions; triangles whose clipped box exceeds kLaneBox pixels are
+//    queued and rasterised by the whole workgroup, one pixel per lane.  Then
+//    every pixel of the tile is resolved and written once.
+constexpr int kLaneBox = 128;
+
+__device__ __forceinline__ GSetup g_setup_of(const GRec& r) {
+  GSetup s;
+  s.area = r.area;
+  s.swap = r.swap != 0u;
+  for (int v = 0; v < 3; ++v) {
+    s.X[v] = r.X[v];
+    s.Y[v] = r.Y[v];
+  }
+  return s;
+}
+
+__device__ __forceinline__ void g_depth_test(unsigned long long* s_depth, int li, int lj, float b0, float b1,
+                                             float b2, const GRec& r) {
+#pragma clang fp contract(off)
+  const float depth = b0 * r.z[0] + b1 * r.z[1] + b2 * r.z[2];
+  if (depth >= 0.0f && depth <= 1.0f) {
+    const unsigned long long key =
+        ((unsigned long long)__builtin_bit_cast(uint32_t, depth) << 32) | (unsigned long long)r.otri;
+    atomicMin(&s_depth[lj * kTile + li], key);
+  }
+}
+
 __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __restrict__ recs,
                                                      const uint32_t* __restrict__ tile_count,
                                                      const uint32_t* __restrict__ tile_off,
@@ -302,91 +359,62 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
                                                      float4* __restrict__ gpos, float4* __restrict__ gnrm) {
 #pragma clang fp contract(off)
   __shared__ unsigned long long s_depth[kTile * kTile];
-  __shared__ GRec s_rec[256];
-  __shared__ uint32_t s_end[256];   // inclusive prefix of the candidate counts
-  __shared__ uint32_t s_box[256];   // clipped box: x | y << 8 | width << 16 (tile-relative)
-  __shared__ uint32_t s_rcp[256];   // ceil(65536 / width)
-  __shared__ uint32_t s_wsum[4];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ GRec s_big[256];
+  __shared__ uint32_t s_nbig;
+  const int tid = threadIdx.x;
   const int tile = blockIdx.x;
   const int x0 = (tile % tiles_x) * kTile, y0 = (tile / tiles_x) * kTile;
   for (int q = tid; q < kTile * kTile; q += 256) s_depth[q] = ~0ull;
+  if (tid == 0) s_nbig = 0;
   const uint32_t n = tile_count[tile], base = tile_off[tile];
   __syncthreads();
   for (uint32_t b0 = 0; b0 < n; b0 += 256) {
-    uint32_t cnt = 0;
     if (b0 + tid < n) {
       const GRec r = recs[list[base + b0 + tid]];
-      s_rec[tid] = r;
-      const int ci0 = max((int)r.i0, x0) - x0, ci1 = min((int)r.i1, x0 + kTile - 1) - x0;
-      const int cj0 = max((int)r.j0, y0) - y0, cj1 = min((int)r.j1, y0 + kTile - 1) - y0;
-      if (ci1 >= ci0 && cj1 >= cj0) {
-        const uint32_t cw = (uint32_t)(ci1 - ci0 + 1);
-        cnt = cw * (uint32_t)(cj1 - cj0 + 1);
-        s_box[tid] = (uint32_t)ci0 | ((uint32_t)cj0 << 8) | (cw << 16);
-        s_rcp[tid] = (65536u + cw - 1u) / cw;
-      }
-    }
-    // block-wide inclusive scan of cnt
-    uint32_t x = cnt;
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) s_wsum[wave] = x;
-    __syncthreads();
-    for (int w = 0; w < wave; ++w) x += s_wsum[w];
-    s_end[tid] = x;
-    __syncthreads();
-    const uint32_t total = s_end[255];
-    // every lane takes one contiguous run of candidates: one search, then a
-    // walk through the boxes with the triangle held in registers
-    const uint32_t per = (total + 255u) / 256u;
-    uint32_t c = (uint32_t)tid * per;
-    const uint32_t cend = min(c + per, total);
-    if (c < cend) {
-      uint32_t slot = 0, hi = 255;
-      while (slot < hi) {
-        const uint32_t mid = (slot + hi) >> 1;
-        if (s_end[mid] > c) hi = mid;
-        else slot = mid + 1;
-      }
-      while (c < cend) {
-        const uint32_t end = s_end[slot];
-        const uint32_t local = c - (slot ? s_end[slot - 1] : 0u);
-        const uint32_t box = s_box[slot], cw = box >> 16;
-        uint32_t row = (local * s_rcp[slot]) >> 16;  // exact: local < 1024, cw <= 32
-        uint32_t col = local - row * cw;
-        const GRec& r = s_rec[slot];
-        GSetup s;
-        s.area = r.area;
-        s.swap = r.swap != 0u;
-        for (int v = 0; v < 3; ++v) {
-          s.X[v] = r.X[v];
-          s.Y[v] = r.Y[v];
-        }
-        const float z0 = r.z[0], z1 = r.z[1], z2 = r.z[2];
-        const unsigned long long otri = r.otri;
-        const int bx = (int)(box & 255u), by = (int)((box >> 8) & 255u);
-        const uint32_t stop = min(end, cend);
-        for (; c < stop; ++c) {
-          const int li = bx + (int)col, lj = by + (int)row;
-          float b0w, b1w, b2w;
-          if (g_cover(s, x0 + li, y0 + lj, b0w, b1w, b2w)) {
-            const float depth = b0w * z0 + b1w * z1 + b2w * z2;
-            if (depth >= 0.0f && depth <= 1.0f) {
-              const unsigned long long key = ((unsigned long long)__builtin_bit_cast(uint32_t, depth) << 32) | otri;
-              atomicMin(&s_depth[lj * kTile + li], key);
+      const int ci0 = max((int)r.i0, x0), ci1 = min((int)r.i1, x0 + kTile - 1);
+      const int cj0 = max((int)r.j0, y0), cj1 = min((int)r.j1, y0 + kTile - 1);
+      const int cw = ci1 - ci0 + 1, ch = cj1 - cj0 + 1;
+      if (cw > 0 && ch > 0) {
+        if (cw * ch > kLaneBox) {
+          s_big[atomicAdd(&s_nbig, 1u)] = r;
+        } else {
+          const GSetup s = g_setup_of(r);
+          GEdges e;
+          g_edges(s, ci0, cj0, e);
+          int64_t ra = e.w[0], rb = e.w[1], rc = e.w[2];
+          for (int j = cj0; j <= cj1; ++j) {
+            int64_t wa = ra, wb = rb, wc = rc;
+            for (int i = ci0; i <= ci1; ++i) {
+              float bw0, bw1, bw2;
+              if (g_cover_w(s, e, wa, wb, wc, bw0, bw1, bw2)) g_depth_test(s_depth, i - x0, j - y0, bw0, bw1, bw2, r);
+              wa += e.di[0];
+              wb += e.di[1];
+              wc += e.di[2];
             }
-          }
-          if (++col == cw) {
-            col = 0;
-            ++row;
+            ra += e.dj[0];
+            rb += e.dj[1];
+            rc += e.dj[2];
           }
         }
-        ++slot;  // candidate counts of listed triangles are positive: the next run starts at the next slot
       }
     }
+    __syncthreads();
+    // large triangles: the whole workgroup, one pixel per lane
+    const uint32_t nbig = s_nbig;
+    for (uint32_t k = 0; k < nbig; ++k) {
+      const GRec& r = s_big[k];
+      const int ci0 = max((int)r.i0, x0), ci1 = min((int)r.i1, x0 + kTile - 1);
+      const int cj0 = max((int)r.j0, y0), cj1 = min((int)r.j1, y0 + kTile - 1);
+      const int cw = ci1 - ci0 + 1, npx = cw * (cj1 - cj0 + 1);
+      const GSetup s = g_setup_of(r);
+      for (int q = tid; q < npx; q += 256) {
+        const int i = ci0 + q % cw, j = cj0 + q / cw;
+        float bw0, bw1, bw2;
+        if (g_cover(s, i, j, bw0, bw1, bw2)) g_depth_test(s_depth, i - x0, j - y0, bw0, bw1, bw2, r);
+      }
+    }
+    __syncthreads();
+    if (tid == 0) s_nbig = 0;
     __syncthreads();
   }
   for (int q = tid; q < kTile * kTile; q += 256) {

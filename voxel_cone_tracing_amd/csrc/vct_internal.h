@@ -226,6 +226,7 @@ struct vct_context {
   size_t train_n = 0, train_cap = 0;
   // per-tile fallback of the learned frame
   VctBuf probe_pix, probe_val, fb_list;  // fb_list: [count][tiles...]
+  VctBuf fb_pix;                          // [count][pixels...] traced at full resolution
   int probe_w = 0, probe_h = 0;
   float fb_psnr_db = 40.0f;  // minimum estimated tile PSNR; <= 0 disables the fallback
   bool fb_ran = false;
@@ -255,7 +256,9 @@ int vct_launch_trace(vct_context* ctx);
 // Pixel selection of a trace launch (zero-initialised: every pixel).
 struct TraceSelect {
   const uint32_t* pix_list;    // device list of pixels (y * W + x); output k = pixel k
-  uint32_t npix_list;
+  uint32_t npix_list;          // count, or the bound of *pix_count (grid size)
+  const uint32_t* pix_count;   // optional device-side count
+  int pix_scatter;             // 1: output indexed like the G-buffer
   const uint32_t* tile_list;   // device list of 32x32 tiles; output indexed like the G-buffer
   const uint32_t* tile_count;  // device-side count of listed tiles
   uint32_t max_tiles;          // grid bound (host side)

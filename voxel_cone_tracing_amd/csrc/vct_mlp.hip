@@ -74,55 +74,82 @@ struct FeatArgs {
   const float4* gnrm;
   const DevMaterial* mats;
   uint32_t nmat;
-  const float4* low;  // quarter-res demodulated indirect light (rgb) + linear depth (w)
+  const float4* low;   // quarter-res demodulated indirect light (rgb) + linear depth (w)
+  const float4* lnrm;  // quarter-res normals + material id (the G-buffer at the traced pixels)
   int W, H, wl, hl;
 };
 
-__device__ __forceinline__ float4 low_at(const FeatArgs& F, int i, int j) {
-  i = min(max(i, 0), F.wl - 1);
-  j = min(max(j, 0), F.hl - 1);
-  return F.low[(size_t)j * F.wl + i];
-}
+enum : int { kBackground = 0, kFeatures = 1, kTraceIt = 2 };
 
-// Returns false for background pixels.  f receives 34 features, alb the albedo.
-__device__ bool pixel_features(const FeatArgs& F, int px, int py, float* f, float* alb) {
+// Features of full-resolution pixel (px, py).  Low-resolution pixel (i, j)
+// was traced at full-resolution pixel (4i + 1, 4j + 1).  The 3x3 taps around
+// the nearest low-resolution pixel are weighted by a tent of radius 2
+// low-resolution pixels and by whether the tap lies on the same surface as
+// the pixel (same material, normals within ~18 degrees, linear depths within
+// 5 %); f[0..26] are the taps' RGB times their normalised weights, `base`
+// their sum (the surface-guided interpolation the network corrects), f[27..29]
+// the unit normal, f[30] the relative depth to the taps' weighted depth, and
+// f[31..33] the albedo.  Returns kBackground, kFeatures, or kTraceIt when no
+// tap lies on the pixel's surface (the pixel is traced instead).
+__device__ int pixel_features(const FeatArgs& F, int px, int py, float* f, float* alb, float3& base) {
   const size_t idx = (size_t)py * F.W + px;
   const float4 gp = F.gpos[idx];
-  if (gp.w == 0.0f) return false;
+  if (gp.w == 0.0f) return kBackground;
   const float4 gn = F.gnrm[idx];
-  uint32_t mid = __builtin_bit_cast(uint32_t, gn.w);
-  mid = mid < F.nmat ? mid : 0u;
+  const uint32_t mbits = __builtin_bit_cast(uint32_t, gn.w);
+  const uint32_t mid = mbits < F.nmat ? mbits : 0u;
   const DevMaterial& m = F.mats[mid];
-  // low-res pixel (i, j) was traced at full-res pixel (4i + 1, 4j + 1)
-  const float u = (float)(px - 1) * 0.25f, v = (float)(py - 1) * 0.25f;
-  const float fu = floorf(u), fv = floorf(v);
-  const int i0 = (int)fu, j0 = (int)fv;
-  const float au = u - fu, av = v - fv;
-  int k = 0;
-  for (int dy = -1; dy <= 1; ++dy)
-    for (int dx = -1; dx <= 1; ++dx) {
-      const float4 a = low_at(F, i0 + dx, j0 + dy), b = low_at(F, i0 + dx + 1, j0 + dy);
-      const float4 c = low_at(F, i0 + dx, j0 + dy + 1), d = low_at(F, i0 + dx + 1, j0 + dy + 1);
-      const float w00 = (1.f - au) * (1.f - av), w10 = au * (1.f - av), w01 = (1.f - au) * av, w11 = au * av;
-      f[k++] = w00 * a.x + w10 * b.x + w01 * c.x + w11 * d.x;
-      f[k++] = w00 * a.y + w10 * b.y + w01 * c.y + w11 * d.y;
-      f[k++] = w00 * a.z + w10 * b.z + w01 * c.z + w11 * d.z;
-    }
-  float nx = gn.x, ny = gn.y, nz = gn.z;
-  const float nl = sqrtf(nx * nx + ny * ny + nz * nz);
+  const float nl = sqrtf(gn.x * gn.x + gn.y * gn.y + gn.z * gn.z);
   const float inl = nl > 0.0f ? 1.0f / nl : 0.0f;
-  f[27] = nx * inl;
-  f[28] = ny * inl;
-  f[29] = nz * inl;
-  const float dc = low_at(F, (int)floorf(u + 0.5f), (int)floorf(v + 0.5f)).w;
-  f[30] = dc > 0.0f ? fminf(fmaxf((gp.w - dc) / dc, -1.0f), 1.0f) : 0.0f;
-  f[31] = m.diffuse[0];
-  f[32] = m.diffuse[1];
-  f[33] = m.diffuse[2];
+  const float nx = gn.x * inl, ny = gn.y * inl, nz = gn.z * inl;
   alb[0] = m.diffuse[0];
   alb[1] = m.diffuse[1];
   alb[2] = m.diffuse[2];
-  return true;
+  const float u = (float)(px - 1) * 0.25f, v = (float)(py - 1) * 0.25f;
+  const int ic = (int)floorf(u + 0.5f), jc = (int)floorf(v + 0.5f);
+  float w[9], wsum = 0.0f, dsum = 0.0f;
+  float4 c[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int i = ic + (k % 3) - 1, j = jc + (k / 3) - 1;
+    w[k] = 0.0f;
+    c[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i < 0 || i >= F.wl || j < 0 || j >= F.hl) continue;
+    const size_t q = (size_t)j * F.wl + i;
+    const float4 ln = F.lnrm[q];
+    const float4 lv = F.low[q];
+    if (__builtin_bit_cast(uint32_t, ln.w) != mbits || !(lv.w > 0.0f)) continue;
+    const float d = nx * ln.x + ny * ln.y + nz * ln.z;
+    const float ll = sqrtf(ln.x * ln.x + ln.y * ln.y + ln.z * ln.z);
+    if (!(d > 0.95f * ll) || !(fabsf(lv.w - gp.w) < 0.05f * gp.w)) continue;
+    const float wk = fmaxf(0.0f, 1.0f - 0.5f * fabsf(u - (float)i)) * fmaxf(0.0f, 1.0f - 0.5f * fabsf(v - (float)j));
+    w[k] = wk;
+    c[k] = lv;
+    wsum += wk;
+    dsum += wk * lv.w;
+  }
+  if (!(wsum > 0.0f)) return kTraceIt;
+  const float iw = 1.0f / wsum;
+  base = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const float t = w[k] * iw;
+    f[3 * k + 0] = c[k].x * t;
+    f[3 * k + 1] = c[k].y * t;
+    f[3 * k + 2] = c[k].z * t;
+    base.x += f[3 * k + 0];
+    base.y += f[3 * k + 1];
+    base.z += f[3 * k + 2];
+  }
+  f[27] = nx;
+  f[28] = ny;
+  f[29] = nz;
+  const float dw = dsum * iw;
+  f[30] = fminf(fmaxf((gp.w - dw) / gp.w, -1.0f), 1.0f);
+  f[31] = m.diffuse[0];
+  f[32] = m.diffuse[1];
+  f[33] = m.diffuse[2];
+  return kFeatures;
 }
 
 // ---- forward of one wave's 64 samples held in LDS ---------------------------------
@@ -165,29 +192,44 @@ __device__ __forceinline__ void forward_wave(const MlpDev& P, uint16_t* xs, uint
 }
 
 // ---- inference: features + MLP + output for a 16x16 pixel tile per workgroup ------
+// Output = albedo * (guided interpolation + network correction).  Pixels with
+// no same-surface tap are appended to `trace_list` (traced afterwards at full
+// resolution); pixels that were traced at quarter resolution keep that value.
 struct InferArgs {
   FeatArgs F;
   MlpDev P;
   float4* image;
+  uint32_t* trace_list;  // [0]: count (zero before the launch), then pixel indices
 };
 
 __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs A) {
   __shared__ __attribute__((aligned(16))) uint16_t s_x[256 * 64];
   __shared__ __attribute__((aligned(16))) uint16_t s_h[256 * 64];
+  __shared__ float s_base[256][3];
   __shared__ float s_alb[256][3];
-  __shared__ int s_cov[256];
-  const int t = threadIdx.x;
+  __shared__ int s_st[256];
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   const int px = blockIdx.x * 16 + (t & 15), py = blockIdx.y * 16 + (t >> 4);
   float f[kIn], alb[3] = {0.f, 0.f, 0.f};
-  bool cov = false;
-  if (px < A.F.W && py < A.F.H) cov = pixel_features(A.F, px, py, f, alb);
+  float3 base = make_float3(0.f, 0.f, 0.f);
+  int st = kBackground;
+  if (px < A.F.W && py < A.F.H) st = pixel_features(A.F, px, py, f, alb, base);
+  {  // pixels to trace: one atomic per wave
+    const unsigned long long m = __ballot(st == kTraceIt);
+    uint32_t b0 = 0;
+    if (lane == 0 && m) b0 = atomicAdd(A.trace_list, (uint32_t)__popcll(m));
+    b0 = __shfl(b0, 0, 64);
+    if (st == kTraceIt)
+      A.trace_list[1 + b0 + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)(py * A.F.W + px);
+  }
   uint16_t* row = s_x + t * 64;
+  const bool feat = st == kFeatures;
   for (int k = 0; k < 64; k += 8) {
     uint32_t w[4];
     for (int q = 0; q < 4; ++q) {
       const int k0 = k + 2 * q, k1 = k0 + 1;
-      const uint32_t lo = (cov && k0 < kIn) ? bf16_bits(f[k0]) : 0u;
-      const uint32_t hi = (cov && k1 < kIn) ? bf16_bits(f[k1]) : 0u;
+      const uint32_t lo = (feat && k0 < kIn) ? bf16_bits(f[k0]) : 0u;
+      const uint32_t hi = (feat && k1 < kIn) ? bf16_bits(f[k1]) : 0u;
       w[q] = lo | (hi << 16);
     }
     *reinterpret_cast<uint4*>(row + k) = make_uint4(w[0], w[1], w[2], w[3]);
@@ -195,9 +237,11 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs A) {
   s_alb[t][0] = alb[0];
   s_alb[t][1] = alb[1];
   s_alb[t][2] = alb[2];
-  s_cov[t] = cov ? 1 : 0;
+  s_base[t][0] = base.x;
+  s_base[t][1] = base.y;
+  s_base[t][2] = base.z;
+  s_st[t] = st;
   __syncthreads();
-  const int wave = t >> 6, lane = t & 63;
   floatx4 y[4];
   forward_wave(A.P, s_x + wave * 64 * 64, s_h + wave * 64 * 64, y);
   // lanes 0..15 hold rows 0..3 (outputs 0..2) of sample column lane
@@ -205,26 +249,27 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs A) {
     for (int nt = 0; nt < 4; ++nt) {
       const int s = wave * 64 + nt * 16 + lane;
       const int qx = blockIdx.x * 16 + (s & 15), qy = blockIdx.y * 16 + (s >> 4);
-      if (qx < A.F.W && qy < A.F.H) {
-        float4 o;
-        if (s_cov[s]) {
-          float3 v = make_float3(y[nt][0], y[nt][1], y[nt][2]);
-          if ((qx & 3) == 1 && (qy & 3) == 1 && (qx >> 2) < A.F.wl && (qy >> 2) < A.F.hl) {
-            // this pixel was traced at quarter resolution: keep the traced value
-            const float4 t = A.F.low[(size_t)(qy >> 2) * A.F.wl + (qx >> 2)];
-            v = make_float3(t.x, t.y, t.z);
-          }
-          o = make_float4(s_alb[s][0] * v.x, s_alb[s][1] * v.y, s_alb[s][2] * v.z, 1.0f);
-        } else {
-          o = make_float4(0.15f, 0.25f, 0.25f, 1.0f);  // clear colour, renderer.cpp:398
+      if (qx >= A.F.W || qy >= A.F.H || s_st[s] == kTraceIt) continue;
+      float4 o;
+      if (s_st[s] == kFeatures) {
+        float3 v = make_float3(s_base[s][0] + y[nt][0], s_base[s][1] + y[nt][1], s_base[s][2] + y[nt][2]);
+        if ((qx & 3) == 1 && (qy & 3) == 1 && (qx >> 2) < A.F.wl && (qy >> 2) < A.F.hl) {
+          // this pixel was traced at quarter resolution: keep the traced value
+          const float4 tv = A.F.low[(size_t)(qy >> 2) * A.F.wl + (qx >> 2)];
+          v = make_float3(tv.x, tv.y, tv.z);
         }
-        A.image[(size_t)qy * A.F.W + qx] = o;
+        o = make_float4(s_alb[s][0] * v.x, s_alb[s][1] * v.y, s_alb[s][2] * v.z, 1.0f);
+      } else {
+        o = make_float4(0.15f, 0.25f, 0.25f, 1.0f);  // clear colour, renderer.cpp:398
       }
+      A.image[(size_t)qy * A.F.W + qx] = o;
     }
   }
 }
 
-// ---- training samples: features, albedo and target at listed pixels ---------------
+// ---- training samples: features, albedo and residual target at listed pixels ------
+// Y = target - guided interpolation (the network learns the correction);
+// background pixels and pixels without a same-surface tap get zero weight.
 __global__ __launch_bounds__(256) void mlp_sample_kernel(FeatArgs F, const float4* __restrict__ target,
                                                          const uint32_t* __restrict__ pixels, uint32_t n,
                                                          uint16_t* __restrict__ X, float* __restrict__ Alb,
@@ -234,17 +279,18 @@ __global__ __launch_bounds__(256) void mlp_sample_kernel(FeatArgs F, const float
   const uint32_t p = pixels[k];
   const int px = (int)(p % (uint32_t)F.W), py = (int)(p / (uint32_t)F.W);
   float f[kIn], alb[3] = {0.f, 0.f, 0.f};
-  const bool cov = pixel_features(F, px, py, f, alb);
+  float3 base = make_float3(0.f, 0.f, 0.f);
+  const bool ok = pixel_features(F, px, py, f, alb, base) == kFeatures;
   uint16_t* row = X + (size_t)k * 64;
-  for (int c = 0; c < 64; ++c) row[c] = (cov && c < kIn) ? bf16_bits(f[c]) : (uint16_t)0;
+  for (int c = 0; c < 64; ++c) row[c] = (ok && c < kIn) ? bf16_bits(f[c]) : (uint16_t)0;
   const float4 tg = target[p];
-  Alb[k * 3 + 0] = cov ? alb[0] : 0.f;
-  Alb[k * 3 + 1] = cov ? alb[1] : 0.f;
-  Alb[k * 3 + 2] = cov ? alb[2] : 0.f;
-  Y[k * 3 + 0] = tg.x;
-  Y[k * 3 + 1] = tg.y;
-  Y[k * 3 + 2] = tg.z;
-  valid[k] = cov ? 1u : 0u;
+  Alb[k * 3 + 0] = ok ? alb[0] : 0.f;
+  Alb[k * 3 + 1] = ok ? alb[1] : 0.f;
+  Alb[k * 3 + 2] = ok ? alb[2] : 0.f;
+  Y[k * 3 + 0] = ok ? tg.x - base.x : 0.f;
+  Y[k * 3 + 1] = ok ? tg.y - base.y : 0.f;
+  Y[k * 3 + 2] = ok ? tg.z - base.z : 0.f;
+  valid[k] = ok ? 1u : 0u;
 }
 
 // ---- forward on given features (tests and evaluation) -------------------------------
@@ -711,6 +757,7 @@ FeatArgs feat_args(vct_context* ctx) {
   F.mats = (const DevMaterial*)ctx->d_materials.p;
   F.nmat = (uint32_t)ctx->materials.size();
   F.low = (const float4*)ctx->low_img.p;
+  F.lnrm = (const float4*)ctx->low_gnrm.p;
   F.W = ctx->width;
   F.H = ctx->height;
   F.wl = ctx->width / 4;
@@ -737,15 +784,34 @@ int trace_lowres(vct_context* ctx) {
                         (float4*)ctx->low_img.p, wl, hl, 1, nullptr);
 }
 
+__global__ void zero_count_kernel(uint32_t* p) { *p = 0u; }
+
+// Upsampling, then the full-resolution trace of the pixels it listed.
 int infer_full(vct_context* ctx) {
+  const size_t npix = (size_t)ctx->width * ctx->height;
+  int rc;
+  if ((rc = vct_ensure(ctx->fb_pix, (npix + 1) * 4, ctx))) return rc;
+  hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, ctx->stream, (uint32_t*)ctx->fb_pix.p);
   InferArgs A;
   A.F = feat_args(ctx);
   A.P = mlp_dev(ctx);
   A.image = (float4*)ctx->image.p;
+  A.trace_list = (uint32_t*)ctx->fb_pix.p;
   dim3 grid((ctx->width + 15) / 16, (ctx->height + 15) / 16);
   hipLaunchKernelGGL(mlp_infer_kernel, grid, dim3(256), 0, ctx->stream, A);
   VCT_HIP(hipGetLastError());
   return VCT_OK;
+}
+
+int trace_listed_pixels(vct_context* ctx) {
+  TraceSelect sel;
+  memset(&sel, 0, sizeof(sel));
+  sel.pix_list = (const uint32_t*)ctx->fb_pix.p + 1;
+  sel.pix_count = (const uint32_t*)ctx->fb_pix.p;
+  sel.npix_list = (uint32_t)((size_t)ctx->width * ctx->height);
+  sel.pix_scatter = 1;
+  return vct_trace_select(ctx, (const float4*)ctx->g_pos.p, (const float4*)ctx->g_nrm.p, (float4*)ctx->image.p,
+                          ctx->width, ctx->height, 0, nullptr, sel);
 }
 
 // Probe trace, tile selection and full-resolution trace of the selected tiles.
@@ -814,7 +880,7 @@ int vct_mlp_init(vct_ctx ctx, uint64_t seed) {
   };
   fill(kOffW1, kHid, kIn, 1.0);
   fill(kOffW2, kHid, kHid, 1.0);
-  fill(kOffW3, kOut, kHid, 0.5);
+  fill(kOffW3, kOut, kHid, 0.1);  // the network starts near the guided interpolation
   std::vector<float> z(kParams, 0.0f);
   VCT_HIP(hipMemcpyAsync(ctx->mlp_params.p, p.data(), kParams * 4, hipMemcpyHostToDevice, ctx->stream));
   VCT_HIP(hipMemcpyAsync(ctx->mlp_m1.p, z.data(), kParams * 4, hipMemcpyHostToDevice, ctx->stream));
@@ -1006,17 +1072,15 @@ int vct_render_learned(vct_ctx ctx) {
   if ((rc = infer_full(ctx))) return rc;
   VCT_HIP(hipEventRecord(ctx->ev[9], ctx->stream));
   ctx->ev_recorded[4] = true;
+  VCT_HIP(hipEventRecord(ctx->ev[10], ctx->stream));
+  if ((rc = trace_listed_pixels(ctx))) return rc;
   ctx->fb_ran = false;
   if (ctx->fb_psnr_db > 0.0f) {
-    VCT_HIP(hipEventRecord(ctx->ev[10], ctx->stream));
     if ((rc = fallback(ctx))) return rc;
-    VCT_HIP(hipEventRecord(ctx->ev[11], ctx->stream));
-    ctx->ev_recorded[5] = true;
     ctx->fb_ran = true;
-  } else {
-    ctx->pass_ms[5] = 0.0f;
-    ctx->ev_recorded[5] = false;
   }
+  VCT_HIP(hipEventRecord(ctx->ev[11], ctx->stream));
+  ctx->ev_recorded[5] = true;
   return vct_clear_queues(ctx);
 }
 
@@ -1024,6 +1088,17 @@ int vct_upsample_set_fallback(vct_ctx ctx, float min_tile_psnr_db) {
   MLP_CTX(ctx);
   if (!(min_tile_psnr_db == min_tile_psnr_db)) return vct_set_error(VCT_ERR_INVALID_ARG, "NaN threshold");
   ctx->fb_psnr_db = min_tile_psnr_db;
+  return VCT_OK;
+}
+
+int vct_upsample_traced_pixels(vct_ctx ctx, uint32_t* pixels) {
+  MLP_CTX(ctx);
+  if (!pixels) return vct_set_error(VCT_ERR_INVALID_ARG, "null out");
+  *pixels = 0;
+  if (ctx->fb_pix.p) {
+    VCT_HIP(hipMemcpyAsync(pixels, ctx->fb_pix.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    VCT_HIP(hipStreamSynchronize(ctx->stream));
+  }
   return VCT_OK;
 }
 

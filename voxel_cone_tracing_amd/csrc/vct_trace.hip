@@ -56,7 +56,9 @@ struct TraceArgs {
   unsigned long long* step_counter;  // optional: cone steps taken (diagnostics)
   // pixel selection (both null: every pixel, image indexed like the G-buffer)
   const uint32_t* pix_list;    // listed pixels (y * W + x); image[k] receives pixel k
-  uint32_t npix_list;
+  uint32_t npix_list;          // count (host), or the bound of *pix_count when that is set
+  const uint32_t* pix_count;   // optional device-side count of listed pixels
+  int pix_scatter;             // 1: image[pixel] receives the pixel instead of image[k]
   const uint32_t* tile_list;   // listed 32x32 tiles (ty * tiles_x + tx); 4 workgroups per tile
   const uint32_t* tile_count;  // device-side number of listed tiles
   int tiles_x;
@@ -247,11 +249,11 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
   size_t oidx;
   if (A.pix_list) {
     const uint32_t k = blockIdx.x * 256u + threadIdx.x;
-    if (k >= A.npix_list) return;
+    if (k >= A.npix_list || (A.pix_count && k >= *A.pix_count)) return;
     const uint32_t p = A.pix_list[k];
     px = (int)(p % (uint32_t)A.W);
     py = (int)(p / (uint32_t)A.W);
-    oidx = k;
+    oidx = A.pix_scatter ? (size_t)p : (size_t)k;
   } else {
     int bx = blockIdx.x, by = blockIdx.y;
     if (A.tile_list) {
@@ -474,6 +476,8 @@ int vct_trace_select(vct_context* ctx, const float4* gpos, const float4* gnrm, f
   A.step_counter = step_counter;
   A.pix_list = sel.pix_list;
   A.npix_list = sel.npix_list;
+  A.pix_count = sel.pix_count;
+  A.pix_scatter = sel.pix_scatter;
   A.tile_list = sel.tile_list;
   A.tile_count = sel.tile_count;
   A.tiles_x = sel.tiles_x;

@@ -212,27 +212,53 @@ __device__ __forceinline__ bool top_left(int64_t dx, int64_t dy) { return dy < 0
 // voxel (Morton index) and colour.  Returns false if the pixel is not covered
 // or the voxel falls outside the grid (with kBrick: also when the voxel's
 // Morton index >> bshift is not bwant, before any shading).
-template <bool kBrick = false>
-__device__ __forceinline__ bool shade_pixel(const TriRec& s, int i, int j, const VoxParams& vp,
-                                            const DevMaterial* mats, uint32_t& key, float4& color,
-                                            uint32_t bshift = 0, uint32_t bwant = 0) {
-#pragma clang fp contract(off)
-  const int lim = 2 * (int)vp.res;
-  if (i >= lim || j >= lim) return false;
+// Edge functions of a record in CCW order (a, b, c) = (0, 1, 2) or (0, 2, 1)
+// at pixel (i, j), and their exact per-column / per-row increments.
+struct VEdges {
+  int64_t w[3], di[3], dj[3];
+  int64_t ex[3], ey[3];
+};
+
+__device__ __forceinline__ void vox_edges(const TriRec& s, int i, int j, VEdges& e) {
   const int64_t px = (int64_t)i * 256, py = (int64_t)j * 256;
-  // CCW vertex order (a, b, c) = (0, 1, 2) or (0, 2, 1)
   const bool sw = (s.flags & 1u) != 0u;
   const int64_t ax = s.X[0], ay = s.Y[0];
   const int64_t bx = sw ? s.X[2] : s.X[1], by = sw ? s.Y[2] : s.Y[1];
   const int64_t cx = sw ? s.X[1] : s.X[2], cy = sw ? s.Y[1] : s.Y[2];
-  // weights of a, b, c
-  int64_t wa = (cx - bx) * (py - by) - (cy - by) * (px - bx);
-  int64_t wb = (ax - cx) * (py - cy) - (ay - cy) * (px - cx);
-  int64_t wc = (bx - ax) * (py - ay) - (by - ay) * (px - ax);
+  e.w[0] = (cx - bx) * (py - by) - (cy - by) * (px - bx);
+  e.w[1] = (ax - cx) * (py - cy) - (ay - cy) * (px - cx);
+  e.w[2] = (bx - ax) * (py - ay) - (by - ay) * (px - ax);
+  e.di[0] = -(cy - by) * 256;
+  e.di[1] = -(ay - cy) * 256;
+  e.di[2] = -(by - ay) * 256;
+  e.dj[0] = (cx - bx) * 256;
+  e.dj[1] = (ax - cx) * 256;
+  e.dj[2] = (bx - ax) * 256;
+  e.ex[0] = cx - bx;
+  e.ey[0] = cy - by;
+  e.ex[1] = ax - cx;
+  e.ey[1] = ay - cy;
+  e.ex[2] = bx - ax;
+  e.ey[2] = by - ay;
+}
+
+// Fragment at pixel (i, j) of the triangle's 2res x 2res raster given its
+// edge values (wa, wb, wc): coverage, voxel (Morton index) and colour.
+// Returns false if the pixel is not covered or the voxel falls outside the
+// grid (with kBrick: also when the voxel's Morton index >> bshift is not
+// bwant, before any shading).
+template <bool kBrick = false>
+__device__ __forceinline__ bool shade_w(const TriRec& s, const VEdges& e, int i, int j, int64_t wa, int64_t wb,
+                                        int64_t wc, const VoxParams& vp, const DevMaterial* mats, uint32_t& key,
+                                        float4& color, uint32_t bshift = 0, uint32_t bwant = 0) {
+#pragma clang fp contract(off)
+  const int lim = 2 * (int)vp.res;
+  if (i >= lim || j >= lim) return false;
+  const bool sw = (s.flags & 1u) != 0u;
   if (wa < 0 || wb < 0 || wc < 0) return false;
-  if (wa == 0 && !top_left(cx - bx, cy - by)) return false;
-  if (wb == 0 && !top_left(ax - cx, ay - cy)) return false;
-  if (wc == 0 && !top_left(bx - ax, by - ay)) return false;
+  if (wa == 0 && !top_left(e.ex[0], e.ey[0])) return false;
+  if (wb == 0 && !top_left(e.ex[1], e.ey[1])) return false;
+  if (wc == 0 && !top_left(e.ex[2], e.ey[2])) return false;
   // weights of the original vertices 0, 1, 2
   const int64_t w0 = wa, w1 = sw ? wc : wb, w2 = sw ? wb : wc;
   const float fa = (float)s.area;
@@ -298,6 +324,16 @@ __device__ __forceinline__ bool shade_pixel(const TriRec& s, int i, int j, const
   for (int k = 0; k < 4; ++k) out[k] = fminf(fmaxf(out[k], 0.0f), 1.0f);
   color = make_float4(out[0], out[1], out[2], out[3]);
   return true;
+}
+
+// Fragment at pixel (i, j) (edge values computed directly).
+template <bool kBrick = false>
+__device__ __forceinline__ bool shade_pixel(const TriRec& s, int i, int j, const VoxParams& vp,
+                                            const DevMaterial* mats, uint32_t& key, float4& color,
+                                            uint32_t bshift = 0, uint32_t bwant = 0) {
+  VEdges e;
+  vox_edges(s, i, j, e);
+  return shade_w<kBrick>(s, e, i, j, e.w[0], e.w[1], e.w[2], vp, mats, key, color, bshift, bwant);
 }
 
 // Candidate `lc` of the triangle's quad-ordered bounding box (the canonical
@@ -523,7 +559,37 @@ __device__ __forceinline__ void store_mean(void* level0, uint32_t v, uint32_t cn
   }
 }
 
-// 4. one workgroup per brick (blockIdx.x = the brick's Morton index).
+// 4. one workgroup per brick (blockIdx.x = the brick's Morton index).  The
+//    brick's triangles are taken 1024 at a time, one per lane: a lane walks the
+//    triangle's box inside the brick's footprint (one pixel of margin) with
+//    incremental edge functions; boxes above kLaneBox pixels are queued and
+//    rasterised by the whole workgroup, one pixel per lane.
+constexpr int kLaneBox = 128;
+
+__device__ __forceinline__ void brick_add(uint32_t (*s_sum)[kBrick * kBrick * kBrick], uint32_t* s_cnt, uint32_t v,
+                                          float4 color) {
+  atomicAdd(&s_sum[0][v], fixq(color.x));
+  atomicAdd(&s_sum[1][v], fixq(color.y));
+  atomicAdd(&s_sum[2][v], fixq(color.z));
+  atomicAdd(&s_sum[3][v], fixq(color.w));
+  atomicAdd(&s_cnt[v], 1u);
+}
+
+// Candidate box of a record inside the footprint of brick coordinates bc.
+__device__ __forceinline__ bool brick_box_px(const TriRec& r, const uint32_t* bc, int span, int lim, int& i0, int& i1,
+                                             int& j0, int& j1) {
+  const uint32_t fl = r.flags;
+  const int axis = (int)((fl >> 1) & 3u);
+  const int ia = axis == 0 ? 1 : 0, ib = axis == 2 ? 1 : 2;
+  const int qh = (int)(fl >> 8);
+  const int fx = (int)bc[ia] * span, fy = (int)bc[ib] * span;
+  i0 = max(r.qx0, fx - 1);
+  i1 = min(min(r.qx0 + 2 * r.qw - 1, fx + span), lim);
+  j0 = max(r.qy0, fy - 1);
+  j1 = min(min(r.qy0 + 2 * qh - 1, fy + span), lim);
+  return i1 >= i0 && j1 >= j0;
+}
+
 template <int DT>
 __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, const TriRec* __restrict__ recs,
                                                                   const DevMaterial* __restrict__ mats,
@@ -534,15 +600,13 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
   constexpr int kVox = kBrick * kBrick * kBrick;
   __shared__ uint32_t s_sum[4][kVox];
   __shared__ uint32_t s_cnt[kVox];
-  __shared__ uint32_t s_end[kBrickThreads];  // inclusive prefix of candidate counts
-  __shared__ uint32_t s_org[kBrickThreads];  // clipped box origin: i | j << 16
-  __shared__ uint32_t s_div[kBrickThreads];  // ceil(65536 / width) | width << 20
-  __shared__ uint32_t s_tri[kBrickThreads];
-  __shared__ uint32_t s_wsum[kBrickThreads / 64];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ uint32_t s_big[kBrickThreads];
+  __shared__ uint32_t s_nbig;
+  const int tid = threadIdx.x;
   const BrickGeom g = brick_geom(vp.res);
   const uint32_t brick = blockIdx.x;
   const uint32_t nvox = 1u << (3 * g.bits);
+  const uint32_t bshift = 3 * g.bits;
   const uint32_t bc[3] = {compact3(brick), compact3(brick >> 1), compact3(brick >> 2)};
   for (uint32_t v = tid; v < nvox; v += kBrickThreads) {
     s_sum[0][v] = 0u;
@@ -551,83 +615,57 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
     s_sum[3][v] = 0u;
     s_cnt[v] = 0u;
   }
+  if (tid == 0) s_nbig = 0;
   const uint32_t n = brick_count[brick], base = brick_off[brick];
   const int lim = 2 * (int)vp.res - 1;
   const int span = 2 << g.bits;  // footprint of the brick on the raster, pixels per axis
   __syncthreads();
   for (uint32_t b0 = 0; b0 < n; b0 += kBrickThreads) {
-    uint32_t cnt = 0;
     if (b0 + tid < n) {
       const uint32_t t = list[base + b0 + tid];
-      const TriRec& r = recs[t];
-      const uint32_t fl = r.flags;
-      const int axis = (int)((fl >> 1) & 3u);
-      const int ia = axis == 0 ? 1 : 0, ib = axis == 2 ? 1 : 2;
-      const int qh = (int)(fl >> 8);
-      // footprint with one pixel of margin, clipped to the triangle's box
-      const int fx = (int)bc[ia] * span, fy = (int)bc[ib] * span;
-      const int i0 = max(r.qx0, fx - 1), i1 = min(min(r.qx0 + 2 * r.qw - 1, fx + span), lim);
-      const int j0 = max(r.qy0, fy - 1), j1 = min(min(r.qy0 + 2 * qh - 1, fy + span), lim);
-      if (i1 >= i0 && j1 >= j0) {
-        const uint32_t cw = (uint32_t)(i1 - i0 + 1);
-        cnt = cw * (uint32_t)(j1 - j0 + 1);
-        s_org[tid] = (uint32_t)i0 | ((uint32_t)j0 << 16);
-        s_div[tid] = ((65536u + cw - 1u) / cw) | (cw << 20);
-        s_tri[tid] = t;
-      }
-    }
-    uint32_t x = cnt;
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) s_wsum[wave] = x;
-    __syncthreads();
-    for (int w = 0; w < wave; ++w) x += s_wsum[w];
-    s_end[tid] = x;
-    __syncthreads();
-    const uint32_t total = s_end[kBrickThreads - 1];
-    // every lane takes one contiguous run of candidates: one search, then a
-    // walk through the boxes with the triangle record held in registers
-    const uint32_t per = (total + kBrickThreads - 1u) / kBrickThreads;
-    uint32_t c = (uint32_t)tid * per;
-    const uint32_t cend = min(c + per, total);
-    if (c < cend) {
-      uint32_t slot = 0, hi = kBrickThreads - 1;
-      while (slot < hi) {
-        const uint32_t mid = (slot + hi) >> 1;
-        if (s_end[mid] > c) hi = mid;
-        else slot = mid + 1;
-      }
-      while (c < cend) {
-        const uint32_t end = s_end[slot];
-        const uint32_t local = c - (slot ? s_end[slot - 1] : 0u);
-        const uint32_t dv = s_div[slot], cw = dv >> 20;
-        uint32_t row = (local * (dv & 0xfffffu)) >> 16;  // exact: local < 35 * 35, cw <= 35
-        uint32_t col = local - row * cw;
-        const uint32_t org = s_org[slot];
-        const int oi = (int)(org & 0xffffu), oj = (int)(org >> 16);
-        const TriRec r = recs[s_tri[slot]];
-        const uint32_t stop = min(end, cend);
-        for (; c < stop; ++c) {
-          uint32_t key;
-          float4 color;
-          if (shade_pixel<true>(r, oi + (int)col, oj + (int)row, vp, mats, key, color, 3 * g.bits, brick)) {
-            const uint32_t v = key & (nvox - 1u);
-            atomicAdd(&s_sum[0][v], fixq(color.x));
-            atomicAdd(&s_sum[1][v], fixq(color.y));
-            atomicAdd(&s_sum[2][v], fixq(color.z));
-            atomicAdd(&s_sum[3][v], fixq(color.w));
-            atomicAdd(&s_cnt[v], 1u);
-          }
-          if (++col == cw) {
-            col = 0;
-            ++row;
+      const TriRec r = recs[t];
+      int i0, i1, j0, j1;
+      if (brick_box_px(r, bc, span, lim, i0, i1, j0, j1)) {
+        if ((i1 - i0 + 1) * (j1 - j0 + 1) > kLaneBox) {
+          s_big[atomicAdd(&s_nbig, 1u)] = t;
+        } else {
+          VEdges e;
+          vox_edges(r, i0, j0, e);
+          int64_t ra = e.w[0], rb = e.w[1], rc = e.w[2];
+          for (int j = j0; j <= j1; ++j) {
+            int64_t wa = ra, wb = rb, wc = rc;
+            for (int i = i0; i <= i1; ++i) {
+              uint32_t key;
+              float4 color;
+              if (shade_w<true>(r, e, i, j, wa, wb, wc, vp, mats, key, color, bshift, brick))
+                brick_add(s_sum, s_cnt, key & (nvox - 1u), color);
+              wa += e.di[0];
+              wb += e.di[1];
+              wc += e.di[2];
+            }
+            ra += e.dj[0];
+            rb += e.dj[1];
+            rc += e.dj[2];
           }
         }
-        ++slot;  // listed triangles have positive candidate counts
       }
     }
+    __syncthreads();
+    const uint32_t nbig = s_nbig;
+    for (uint32_t k = 0; k < nbig; ++k) {
+      const TriRec r = recs[s_big[k]];
+      int i0, i1, j0, j1;
+      brick_box_px(r, bc, span, lim, i0, i1, j0, j1);
+      const int cw = i1 - i0 + 1, npx = cw * (j1 - j0 + 1);
+      for (int q = tid; q < npx; q += kBrickThreads) {
+        uint32_t key;
+        float4 color;
+        if (shade_pixel<true>(r, i0 + q % cw, j0 + q / cw, vp, mats, key, color, bshift, brick))
+          brick_add(s_sum, s_cnt, key & (nvox - 1u), color);
+      }
+    }
+    __syncthreads();
+    if (tid == 0) s_nbig = 0;
     __syncthreads();
   }
   for (uint32_t v = tid; v < nvox; v += kBrickThreads)

@@ -113,10 +113,18 @@ def evaluate(ctx, sc, wl, poses):
         sc.queue()
         ctx.render_learned()
         learned = ctx.download_image()
+        traced = ctx.traced_pixels()
         low = ctx.download_lowres()
         alb = albedo_image(ctx, sc)
         bil = bilinear_upsample(low, wl.width, wl.height) * alb
+        params = ctx.mlp_params()
+        ctx.mlp_set_params(np.zeros_like(params))
+        sc.queue()
+        ctx.render_learned()
+        guided = ctx.download_image()
+        ctx.mlp_set_params(params)
         p_l, _ = psnr_covered(learned, plain, cov)
+        p_g, _ = psnr_covered(guided, plain, cov)
         p_b, _ = psnr_covered(bil, plain, cov)
         fail = tile_fail_share(learned, plain, cov)
         ctx.set_fallback(40.0)
@@ -126,10 +134,13 @@ def evaluate(ctx, sc, wl, poses):
         p_f, _ = psnr_covered(fb, plain, cov)
         nt, nf = ctx.fallback_stats()
         rows.append({"eye": [round(float(x), 4) for x in eye], "psnr_learned_db": round(p_l, 2),
-                     "psnr_bilinear_db": round(p_b, 2), "tiles_below_40db": round(fail, 4),
-                     "psnr_with_fallback_db": round(p_f, 2), "fallback_tiles": nf, "tiles": nt})
-        log(f"  eval eye={rows[-1]['eye']}: learned {p_l:.2f} dB, bilinear {p_b:.2f} dB, "
-            f"tiles<40dB {fail:.3f}, with fallback {p_f:.2f} dB ({nf}/{nt} tiles)")
+                     "psnr_guided_interpolation_db": round(p_g, 2), "psnr_bilinear_db": round(p_b, 2),
+                     "traced_pixels": traced, "covered_pixels": int(cov.sum()),
+                     "tiles_below_40db": round(fail, 4), "psnr_with_fallback_db": round(p_f, 2),
+                     "fallback_tiles": nf, "tiles": nt})
+        log(f"  eval eye={rows[-1]['eye']}: learned {p_l:.2f} dB, guided {p_g:.2f} dB, bilinear {p_b:.2f} dB, "
+            f"traced px {traced}/{int(cov.sum())}, tiles<40dB {fail:.3f}, with fallback {p_f:.2f} dB "
+            f"({nf}/{nt} tiles)")
     return rows
 
 
