@@ -153,25 +153,23 @@ __device__ int pixel_features(const FeatArgs& F, int px, int py, float* f, float
 }
 
 // ---- forward of one wave's 64 samples held in LDS ---------------------------------
-// xs: [64 samples][64] BF16 (kappa order, zero padded), hs: scratch of the same
-// size.  Returns y (C layout of the 16-row output tile) for the wave's four
-// 16-sample column tiles.
-__device__ __forceinline__ void forward_wave(const MlpDev& P, uint16_t* xs, uint16_t* hs, floatx4 (&y)[4]) {
+// xs: [64 samples][64] BF16 (features zero padded).  Layers 1 and 2 are
+// computed per 16-sample column tile and written back over that tile's own
+// input rows, which no other tile reads (the tile's MFMAs have consumed them
+// before the write; one wave, in order).  Returns y (C layout of the 16-row
+// output tile) for the wave's four 16-sample column tiles.
+__device__ __forceinline__ void forward_wave(const MlpDev& P, uint16_t* xs, floatx4 (&y)[4]) {
   const int lane = threadIdx.x & 63;
   const int col = lane & 15, rq = (lane >> 4) * 4;
-  // layer 1: xs -> hs ; layer 2: hs -> xs ; layer 3: xs -> y
   for (int layer = 0; layer < 2; ++layer) {
     const uint16_t* Wp = P.packed + (layer == 0 ? kPW1 : kPW2);
     const float* bias = layer == 0 ? P.b1 : P.b2;
-    uint16_t* src = layer == 0 ? xs : hs;
-    uint16_t* dst = layer == 0 ? hs : xs;
     for (int nt = 0; nt < 4; ++nt) {
       floatx4 acc[4];
       for (int mt = 0; mt < 4; ++mt) {
         acc[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
-        acc[mt] = mfma_tile(Wp + mt * 16 * 64, 64, src + nt * 16 * 64, 64, 64, acc[mt]);
+        acc[mt] = mfma_tile(Wp + mt * 16 * 64, 64, xs + nt * 16 * 64, 64, 64, acc[mt]);
       }
-      // the wave's own rows of src are no longer needed: overwrite dst rows
       for (int mt = 0; mt < 4; ++mt) {
         uint16_t o[4];
         for (int i = 0; i < 4; ++i) {
@@ -179,7 +177,7 @@ __device__ __forceinline__ void forward_wave(const MlpDev& P, uint16_t* xs, uint
           o[i] = bf16_bits(h);
         }
         uint2 pk = make_uint2((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16));
-        *reinterpret_cast<uint2*>(dst + (nt * 16 + col) * 64 + mt * 16 + rq) = pk;
+        *reinterpret_cast<uint2*>(xs + (nt * 16 + col) * 64 + mt * 16 + rq) = pk;
       }
     }
   }
@@ -204,7 +202,6 @@ struct InferArgs {
 
 __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs A) {
   __shared__ __attribute__((aligned(16))) uint16_t s_x[256 * 64];
-  __shared__ __attribute__((aligned(16))) uint16_t s_h[256 * 64];
   __shared__ float s_base[256][3];
   __shared__ float s_alb[256][3];
   __shared__ int s_st[256];
@@ -243,7 +240,7 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs A) {
   s_st[t] = st;
   __syncthreads();
   floatx4 y[4];
-  forward_wave(A.P, s_x + wave * 64 * 64, s_h + wave * 64 * 64, y);
+  forward_wave(A.P, s_x + wave * 64 * 64, y);
   // lanes 0..15 hold rows 0..3 (outputs 0..2) of sample column lane
   if (lane < 16) {
     for (int nt = 0; nt < 4; ++nt) {
@@ -297,7 +294,6 @@ __global__ __launch_bounds__(256) void mlp_sample_kernel(FeatArgs F, const float
 __global__ __launch_bounds__(256) void mlp_forward_kernel(MlpDev P, const uint16_t* __restrict__ X, uint32_t n,
                                                           float* __restrict__ Y) {
   __shared__ __attribute__((aligned(16))) uint16_t s_x[256 * 64];
-  __shared__ __attribute__((aligned(16))) uint16_t s_h[256 * 64];
   const int t = threadIdx.x;
   const uint32_t base = blockIdx.x * 256u;
   {
@@ -309,7 +305,7 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(MlpDev P, const uint16
   __syncthreads();
   const int wave = t >> 6, lane = t & 63;
   floatx4 y[4];
-  forward_wave(P, s_x + wave * 64 * 64, s_h + wave * 64 * 64, y);
+  forward_wave(P, s_x + wave * 64 * 64, y);
   if (lane < 16)
     for (int nt = 0; nt < 4; ++nt) {
       const uint32_t k = base + wave * 64 + nt * 16 + lane;

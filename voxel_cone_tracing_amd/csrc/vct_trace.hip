@@ -59,7 +59,7 @@ struct TraceArgs {
   uint32_t npix_list;          // count (host), or the bound of *pix_count when that is set
   const uint32_t* pix_count;   // optional device-side count of listed pixels
   int pix_scatter;             // 1: image[pixel] receives the pixel instead of image[k]
-  const uint32_t* tile_list;   // listed 32x32 tiles (ty * tiles_x + tx); 4 workgroups per tile
+  const uint32_t* tile_list;   // listed 32x32 tiles (ty * tiles_x + tx); 16 workgroups per tile
   const uint32_t* tile_count;  // device-side number of listed tiles
   int tiles_x;
   ConeSet cones;
@@ -242,39 +242,47 @@ __device__ __forceinline__ float3 refract3(float3 I, float3 N, float eta) {
   return make_float3(eta * I.x - s * N.x, eta * I.y - s * N.y, eta * I.z - s * N.z);
 }
 
+// One workgroup = the 8x8 pixels of a block (lane = pixel) x 4 cone groups
+// (wave = group).  The cone slots of a pixel (diffuse 0..nd-1, specular nd,
+// refraction nd+1) are dealt round-robin to the groups, so every wave still
+// marches the same cone for 64 neighbouring pixels (coherent gathers) while a
+// pixel's cones run in 4 waves at once; group 3 also does the direct light
+// and its shadow cones.  Group 0 then sums the diffuse cones in their
+// original order and composes the pixel (voxel_cone_tracing.frag:246-275).
+constexpr int kMaxSlots = 18;  // 16 diffuse + specular + refraction
+
 template <int DT>
 __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ float s_cone[kMaxSlots][3][64];
+  __shared__ float s_direct[3][64];
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
   int px, py;
   size_t oidx;
+  bool valid = true;
   if (A.pix_list) {
-    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
-    if (k >= A.npix_list || (A.pix_count && k >= *A.pix_count)) return;
-    const uint32_t p = A.pix_list[k];
+    const uint32_t k = blockIdx.x * 64u + (uint32_t)lane;
+    valid = k < A.npix_list && !(A.pix_count && k >= *A.pix_count);
+    const uint32_t p = valid ? A.pix_list[k] : 0u;
     px = (int)(p % (uint32_t)A.W);
     py = (int)(p / (uint32_t)A.W);
     oidx = A.pix_scatter ? (size_t)p : (size_t)k;
   } else {
     int bx = blockIdx.x, by = blockIdx.y;
     if (A.tile_list) {
-      const uint32_t k = blockIdx.x >> 2;
-      if (k >= *A.tile_count) return;
+      const uint32_t k = blockIdx.x >> 4;
+      if (k >= *A.tile_count) return;  // the whole workgroup leaves
       const uint32_t t = A.tile_list[k];
-      bx = (int)(t % (uint32_t)A.tiles_x) * 2 + (int)(blockIdx.x & 1u);
-      by = (int)(t / (uint32_t)A.tiles_x) * 2 + (int)((blockIdx.x >> 1) & 1u);
+      bx = (int)(t % (uint32_t)A.tiles_x) * 4 + (int)(blockIdx.x & 3u);
+      by = (int)(t / (uint32_t)A.tiles_x) * 4 + (int)((blockIdx.x >> 2) & 3u);
     }
-    px = bx * 16 + (wave & 1) * 8 + (lane & 7);
-    py = by * 16 + (wave >> 1) * 8 + (lane >> 3);
+    px = bx * 8 + (lane & 7);
+    py = by * 8 + (lane >> 3);
     oidx = (size_t)py * A.W + px;
   }
-  if (px >= A.W || py >= A.H) return;
-  const size_t idx = (size_t)py * A.W + px;
+  valid = valid && px < A.W && py < A.H;
+  const size_t idx = valid ? (size_t)py * A.W + px : 0;
   const float4 gp = A.gpos[idx];
   const float4 bg = make_float4(0.15f, 0.25f, 0.25f, 1.0f);  // renderer.cpp:398
-  if (gp.w == 0.0f) {
-    A.image[oidx] = A.demodulate ? f4(0.0f) : bg;
-    return;
-  }
   const float4 gn = A.gnrm[idx];
   uint32_t mid = __builtin_bit_cast(uint32_t, gn.w);
   mid = mid < A.nmat ? mid : 0u;  // ids of an uploaded G-buffer are not trusted
@@ -283,12 +291,103 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
   const float3 n = make_float3(gn.x, gn.y, gn.z);
   const float3 pos = make_float3(0.5f * (world.x / A.cube_size) + 0.5f, 0.5f * (world.y / A.cube_size) + 0.5f,
                                  0.5f * (world.z / A.cube_size) + 0.5f);
-  // within_cube(pos, 0) on biased coordinates (:250, :61-64)
-  if (!(fabsf(pos.x) < 1.0f && fabsf(pos.y) < 1.0f && fabsf(pos.z) < 1.0f)) {
+  // 0: nothing to do, 1: background / outside the cube, 2: voxel view, 3: shade
+  int mode = 0;
+  if (valid) {
+    if (gp.w == 0.0f || !(fabsf(pos.x) < 1.0f && fabsf(pos.y) < 1.0f && fabsf(pos.z) < 1.0f)) mode = 1;
+    else mode = A.view_dir < 7 ? 2 : 3;  // within_cube on biased coordinates (:250, :61-64)
+  }
+  const float3 cam = make_float3(A.cam[0], A.cam[1], A.cam[2]);
+  const float3 view_dir = normalize3(make_float3(world.x - cam.x, world.y - cam.y, world.z - cam.z));
+  const float sqrt3 = 1.73205080757f;
+  uint32_t steps = 0;
+  // specular aperture (:220-227)
+  const float rough = sqrtf(2.0f / (m.shininess + 2.0f));
+  const float spec_ap = fminf(fmaxf(tanf(1.57079f * rough), 0.0174533f), 3.14159265f);
+  const int nd = A.en_diffuse ? A.cones.n : 0;
+  const bool do_spec = A.en_specular != 0;
+  const bool do_refr = do_spec && illum_transmissive(m.illum);
+  if (mode == 3) {
+    float3 o1 = make_float3(0.f, 0.f, 0.f), o2 = o1;
+    if (nd > 0) {
+      o1 = normalize3(tangent_of(n));  // trace_diffuse (:146-147)
+      o2 = normalize3(cross3(o1, n));
+    }
+    for (int s = grp; s < nd + 2; s += 4) {
+      float3 d;
+      float ap = spec_ap;
+      if (s < nd) {
+        const float a = A.cones.a[s], b = A.cones.b[s], cc = A.cones.c[s];
+        d = make_float3(a * n.x + b * o1.x + cc * o2.x, a * n.y + b * o1.y + cc * o2.y, a * n.z + b * o1.z + cc * o2.z);
+        ap = A.cones.aperture;
+      } else if (s == nd) {
+        if (!do_spec) continue;
+        // normalize(reflect(-view_dir, n)) (:231), normalised again in trace_cone (:90)
+        const float3 I = make_float3(-view_dir.x, -view_dir.y, -view_dir.z);
+        const float dn = 2.0f * dot3(n, I);
+        d = normalize3(make_float3(I.x - dn * n.x, I.y - dn * n.y, I.z - dn * n.z));
+      } else {
+        if (!do_refr) continue;
+        d = refract3(view_dir, n, 1.0f / m.ior);  // trace_refraction (:237-241)
+      }
+      const float4 r = trace_cone<DT>(A, pos, d, ap, sqrt3, steps);
+      s_cone[s][0][lane] = r.x;
+      s_cone[s][1][lane] = r.y;
+      s_cone[s][2][lane] = r.z;
+    }
+    if (grp == 3 && A.en_direct) {
+      float3 res = make_float3(0.f, 0.f, 0.f);
+      const int nl = A.lights.count < VCT_MAX_LIGHTS ? A.lights.count : VCT_MAX_LIGHTS;
+      for (int li = 0; li < nl; ++li) {
+        const DevLight& L = A.lights.l[li];
+        float3 ldir;
+        float d;
+        if (L.kind == VCT_LIGHT_DIRECTIONAL) {
+          ldir = normalize3(make_float3(L.pos[0], L.pos[1], L.pos[2]));
+          d = sqrt3;
+        } else {
+          const float3 lp = make_float3(0.5f * (L.pos[0] / A.cube_size) + 0.5f,
+                                        0.5f * (L.pos[1] / A.cube_size) + 0.5f,
+                                        0.5f * (L.pos[2] / A.cube_size) + 0.5f);
+          ldir = make_float3(lp.x - pos.x, lp.y - pos.y, lp.z - pos.z);
+          d = len3(ldir);
+          ldir = normalize3(ldir);
+        }
+        const float cos_surf = fmaxf(dot3(n, ldir), 0.0f);
+        const float att = (L.kind == VCT_LIGHT_DIRECTIONAL) ? 1.0f : 1.0f / (1.0f + d * d);
+        const float k = att * cos_surf;
+        const float3 lc = make_float3(k * L.color[0] * L.intensity, k * L.color[1] * L.intensity,
+                                      k * L.color[2] * L.intensity);
+        float shadow = 1.0f;
+        if (A.en_shadow) shadow = fmaxf(0.0f, 1.0f - trace_cone<DT>(A, pos, ldir, 0.1f, d, steps).w);
+        const float lambert = fmaxf(dot3(ldir, n), 0.0f);
+        float refr_angle = 0.0f;
+        if (m.dissolve <= 0.1f) {
+          const float3 r = refract3(view_dir, n, 1.0f / m.ior);
+          refr_angle = fmaxf((1.0f - m.dissolve) * dot3(r, ldir), 0.0f);
+        }
+        const float3 hv = normalize3(make_float3(ldir.x + view_dir.x, ldir.y + view_dir.y, ldir.z + view_dir.z));
+        float sa = fmaxf(dot3(hv, n), 0.0f);
+        sa = fmaxf(sa, refr_angle);
+        const float sc = powf(sa, m.shininess);
+        const float f = shadow + 0.04f;
+        res.x += f * (lambert * m.diffuse[0] + sc * m.specular[0]) * lc.x;
+        res.y += f * (lambert * m.diffuse[1] + sc * m.specular[1]) * lc.y;
+        res.z += f * (lambert * m.diffuse[2] + sc * m.specular[2]) * lc.z;
+      }
+      s_direct[0][lane] = res.x + fminf(fmaxf(m.emission[0], 0.0f), 1.0f);
+      s_direct[1][lane] = res.y + fminf(fmaxf(m.emission[1], 0.0f), 1.0f);
+      s_direct[2][lane] = res.z + fminf(fmaxf(m.emission[2], 0.0f), 1.0f);
+    }
+  }
+  if (A.step_counter && steps) atomicAdd(A.step_counter, (unsigned long long)steps);
+  __syncthreads();
+  if (grp != 0 || mode == 0) return;
+  if (mode == 1) {
     A.image[oidx] = A.demodulate ? f4(0.0f) : bg;
     return;
   }
-  if (A.view_dir < 7) {  // voxel view (:253-257), blended SRC_ALPHA / ONE_MINUS_SRC_ALPHA
+  if (mode == 2) {  // voxel view (:253-257), blended SRC_ALPHA / ONE_MINUS_SRC_ALPHA
     const int d = A.view_dir < 0 ? 0 : (A.view_dir > 5 ? 5 : A.view_dir);
     float3 ad = make_float3(0.f, 0.f, 0.f);
     int mx = 0, my = 2, mz = 4;
@@ -298,105 +397,34 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
     const float4 s = sample_voxel<DT>(A, pos, ad, mx, my, mz, A.view_lod);
     const float a = s.w;
     A.image[oidx] = make_float4(s.x * a + bg.x * (1.0f - a), s.y * a + bg.y * (1.0f - a),
-                               s.z * a + bg.z * (1.0f - a), a * a + bg.w * (1.0f - a));
+                                s.z * a + bg.z * (1.0f - a), a * a + bg.w * (1.0f - a));
     return;
   }
-  const float3 cam = make_float3(A.cam[0], A.cam[1], A.cam[2]);
-  const float3 view_dir = normalize3(make_float3(world.x - cam.x, world.y - cam.y, world.z - cam.z));
-  const float sqrt3 = 1.73205080757f;
-  uint32_t steps = 0;
   float3 out = make_float3(0.f, 0.f, 0.f);
-  // specular aperture (:220-227)
-  const float rough = sqrtf(2.0f / (m.shininess + 2.0f));
-  const float spec_ap = fminf(fmaxf(tanf(1.57079f * rough), 0.0174533f), 3.14159265f);
-  // Cones one after another (each wave marches a cone until its last lane is
-  // done).  Interleaving the cones of a lane in one loop was measured slower:
-  // lanes then sample different cones in the same iteration and the set-up of
-  // a new cone diverges from the marching lanes.
-  const int nd = A.en_diffuse ? A.cones.n : 0;
-  const bool do_spec = A.en_specular != 0;
-  const bool do_refr = do_spec && illum_transmissive(m.illum);
-  float3 diff_acc = make_float3(0.f, 0.f, 0.f), spec_rgb = diff_acc, refr_rgb = diff_acc;
   if (nd > 0) {
-    const float3 o1 = normalize3(tangent_of(n));  // trace_diffuse (:146-147)
-    const float3 o2 = normalize3(cross3(o1, n));
-    for (int kk = 0; kk < nd; ++kk) {
-      const float a = A.cones.a[kk], b = A.cones.b[kk], cc = A.cones.c[kk];
-      const float3 d =
-          make_float3(a * n.x + b * o1.x + cc * o2.x, a * n.y + b * o1.y + cc * o2.y, a * n.z + b * o1.z + cc * o2.z);
-      const float4 r = trace_cone<DT>(A, pos, d, A.cones.aperture, sqrt3, steps);
-      diff_acc = make_float3(diff_acc.x + r.x, diff_acc.y + r.y, diff_acc.z + r.z);
-    }
-  }
-  if (do_spec) {
-    // normalize(reflect(-view_dir, n)) (:231)
-    const float3 I = make_float3(-view_dir.x, -view_dir.y, -view_dir.z);
-    const float dn = 2.0f * dot3(n, I);
-    const float3 rd = normalize3(make_float3(I.x - dn * n.x, I.y - dn * n.y, I.z - dn * n.z));
-    const float4 r = trace_cone<DT>(A, pos, rd, spec_ap, sqrt3, steps);  // normalised again inside (:90)
-    spec_rgb = make_float3(r.x, r.y, r.z);
-  }
-  if (do_refr) {  // trace_refraction (:237-241)
-    const float4 r = trace_cone<DT>(A, pos, refract3(view_dir, n, 1.0f / m.ior), spec_ap, sqrt3, steps);
-    refr_rgb = make_float3(r.x, r.y, r.z);
-  }
-  if (nd > 0) {
+    float3 diff_acc = make_float3(0.f, 0.f, 0.f);
+    for (int k = 0; k < nd; ++k)
+      diff_acc = make_float3(diff_acc.x + s_cone[k][0][lane], diff_acc.y + s_cone[k][1][lane],
+                             diff_acc.z + s_cone[k][2][lane]);
     const float inv = 1.0f / (float)nd;
     out.x += m.diffuse[0] * (diff_acc.x * inv);
     out.y += m.diffuse[1] * (diff_acc.y * inv);
     out.z += m.diffuse[2] * (diff_acc.z * inv);
   }
   if (A.en_direct) {
-    float3 res = make_float3(0.f, 0.f, 0.f);
-    const int nl = A.lights.count < VCT_MAX_LIGHTS ? A.lights.count : VCT_MAX_LIGHTS;
-    for (int li = 0; li < nl; ++li) {
-      const DevLight& L = A.lights.l[li];
-      float3 ldir;
-      float d;
-      if (L.kind == VCT_LIGHT_DIRECTIONAL) {
-        ldir = normalize3(make_float3(L.pos[0], L.pos[1], L.pos[2]));
-        d = sqrt3;
-      } else {
-        const float3 lp = make_float3(0.5f * (L.pos[0] / A.cube_size) + 0.5f, 0.5f * (L.pos[1] / A.cube_size) + 0.5f,
-                                      0.5f * (L.pos[2] / A.cube_size) + 0.5f);
-        ldir = make_float3(lp.x - pos.x, lp.y - pos.y, lp.z - pos.z);
-        d = len3(ldir);
-        ldir = normalize3(ldir);
-      }
-      const float cos_surf = fmaxf(dot3(n, ldir), 0.0f);
-      const float att = (L.kind == VCT_LIGHT_DIRECTIONAL) ? 1.0f : 1.0f / (1.0f + d * d);
-      const float k = att * cos_surf;
-      const float3 lc = make_float3(k * L.color[0] * L.intensity, k * L.color[1] * L.intensity,
-                                    k * L.color[2] * L.intensity);
-      float shadow = 1.0f;
-      if (A.en_shadow) shadow = fmaxf(0.0f, 1.0f - trace_cone<DT>(A, pos, ldir, 0.1f, d, steps).w);
-      const float lambert = fmaxf(dot3(ldir, n), 0.0f);
-      float refr_angle = 0.0f;
-      if (m.dissolve <= 0.1f) {
-        const float3 r = refract3(view_dir, n, 1.0f / m.ior);
-        refr_angle = fmaxf((1.0f - m.dissolve) * dot3(r, ldir), 0.0f);
-      }
-      const float3 hv = normalize3(make_float3(ldir.x + view_dir.x, ldir.y + view_dir.y, ldir.z + view_dir.z));
-      float sa = fmaxf(dot3(hv, n), 0.0f);
-      sa = fmaxf(sa, refr_angle);
-      const float sc = powf(sa, m.shininess);
-      const float f = shadow + 0.04f;
-      res.x += f * (lambert * m.diffuse[0] + sc * m.specular[0]) * lc.x;
-      res.y += f * (lambert * m.diffuse[1] + sc * m.specular[1]) * lc.y;
-      res.z += f * (lambert * m.diffuse[2] + sc * m.specular[2]) * lc.z;
-    }
-    out.x += res.x + fminf(fmaxf(m.emission[0], 0.0f), 1.0f);
-    out.y += res.y + fminf(fmaxf(m.emission[1], 0.0f), 1.0f);
-    out.z += res.z + fminf(fmaxf(m.emission[2], 0.0f), 1.0f);
+    out.x += s_direct[0][lane];
+    out.y += s_direct[1][lane];
+    out.z += s_direct[2][lane];
   }
   if (do_spec) {
-    out.x += m.specular[0] * spec_rgb.x;
-    out.y += m.specular[1] * spec_rgb.y;
-    out.z += m.specular[2] * spec_rgb.z;
+    out.x += m.specular[0] * s_cone[nd][0][lane];
+    out.y += m.specular[1] * s_cone[nd][1][lane];
+    out.z += m.specular[2] * s_cone[nd][2][lane];
   }
   if (do_refr) {  // final colour override (:273-274)
-    const float3 tr = make_float3(m.transmittance[0] * refr_rgb.x, m.transmittance[1] * refr_rgb.y,
-                                  m.transmittance[2] * refr_rgb.z);
+    const float3 tr = make_float3(m.transmittance[0] * s_cone[nd + 1][0][lane],
+                                  m.transmittance[1] * s_cone[nd + 1][1][lane],
+                                  m.transmittance[2] * s_cone[nd + 1][2][lane]);
     const float dv = m.dissolve;
     out = make_float3(tr.x * (1.0f - dv) + out.x * dv, tr.y * (1.0f - dv) + out.y * dv,
                       tr.z * (1.0f - dv) + out.z * dv);
@@ -408,7 +436,6 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
     ow = gp.w;  // linear depth of the traced pixel
   }
   A.image[oidx] = make_float4(out.x, out.y, out.z, ow);
-  if (A.step_counter) atomicAdd(A.step_counter, (unsigned long long)steps);
 }
 
 }  // namespace vct
@@ -517,13 +544,13 @@ int vct_trace_select(vct_context* ctx, const float4* gpos, const float4* gnrm, f
     A.lights.l[i].kind = ctx->lights[i].kind;
   }
   A.lights.count = n;
-  dim3 grid((W + 15) / 16, (H + 15) / 16);
+  dim3 grid((W + 7) / 8, (H + 7) / 8);  // 8x8 pixels per workgroup
   if (sel.pix_list) {
     if (sel.npix_list == 0) return VCT_OK;
-    grid = dim3((sel.npix_list + 255) / 256);
+    grid = dim3((sel.npix_list + 63) / 64);
   } else if (sel.tile_list) {
     if (sel.max_tiles == 0) return VCT_OK;
-    grid = dim3(4 * sel.max_tiles);
+    grid = dim3(16 * sel.max_tiles);  // 16 blocks of 8x8 per 32x32 tile
   }
   switch (ctx->dtype) {
     case VCT_RGBA8: hipLaunchKernelGGL(trace_kernel<VCT_RGBA8>, grid, dim3(256), 0, ctx->stream, A); break;

@@ -219,7 +219,8 @@ struct VEdges {
   int64_t ex[3], ey[3];
 };
 
-__device__ __forceinline__ void vox_edges(const TriRec& s, int i, int j, VEdges& e) {
+template <typename Rec>
+__device__ __forceinline__ void vox_edges(const Rec& s, int i, int j, VEdges& e) {
   const int64_t px = (int64_t)i * 256, py = (int64_t)j * 256;
   const bool sw = (s.flags & 1u) != 0u;
   const int64_t ax = s.X[0], ay = s.Y[0];
@@ -247,22 +248,25 @@ __device__ __forceinline__ void vox_edges(const TriRec& s, int i, int j, VEdges&
 // Returns false if the pixel is not covered or the voxel falls outside the
 // grid (with kBrick: also when the voxel's Morton index >> bshift is not
 // bwant, before any shading).
-template <bool kBrick = false>
-__device__ __forceinline__ bool shade_w(const TriRec& s, const VEdges& e, int i, int j, int64_t wa, int64_t wb,
-                                        int64_t wc, const VoxParams& vp, const DevMaterial* mats, uint32_t& key,
-                                        float4& color, uint32_t bshift = 0, uint32_t bwant = 0) {
+// `c` holds the coverage fields (X, Y, area, flags), `sp` the whole record
+// (read only for covered pixels).
+template <bool kBrick = false, typename Rec>
+__device__ __forceinline__ bool shade_w(const Rec& c, const TriRec* sp, const VEdges& e, int i, int j, int64_t wa,
+                                        int64_t wb, int64_t wc, const VoxParams& vp, const DevMaterial* mats,
+                                        uint32_t& key, float4& color, uint32_t bshift = 0, uint32_t bwant = 0) {
 #pragma clang fp contract(off)
   const int lim = 2 * (int)vp.res;
   if (i >= lim || j >= lim) return false;
-  const bool sw = (s.flags & 1u) != 0u;
+  const bool sw = (c.flags & 1u) != 0u;
   if (wa < 0 || wb < 0 || wc < 0) return false;
   if (wa == 0 && !top_left(e.ex[0], e.ey[0])) return false;
   if (wb == 0 && !top_left(e.ex[1], e.ey[1])) return false;
   if (wc == 0 && !top_left(e.ex[2], e.ey[2])) return false;
   // weights of the original vertices 0, 1, 2
   const int64_t w0 = wa, w1 = sw ? wc : wb, w2 = sw ? wb : wc;
-  const float fa = (float)s.area;
+  const float fa = (float)c.area;
   const float b0 = (float)w0 / fa, b1 = (float)w1 / fa, b2 = (float)w2 / fa;
+  const TriRec& s = *sp;
   float pos[3];
   for (int k = 0; k < 3; ++k) pos[k] = b0 * s.w[0][k] + b1 * s.w[1][k] + b2 * s.w[2][k];
   // voxel address, voxelize.frag:156-157 (truncation toward zero)
@@ -333,7 +337,7 @@ __device__ __forceinline__ bool shade_pixel(const TriRec& s, int i, int j, const
                                             uint32_t bshift = 0, uint32_t bwant = 0) {
   VEdges e;
   vox_edges(s, i, j, e);
-  return shade_w<kBrick>(s, e, i, j, e.w[0], e.w[1], e.w[2], vp, mats, key, color, bshift, bwant);
+  return shade_w<kBrick>(s, &s, e, i, j, e.w[0], e.w[1], e.w[2], vp, mats, key, color, bshift, bwant);
 }
 
 // Candidate `lc` of the triangle's quad-ordered bounding box (the canonical
@@ -560,11 +564,17 @@ __device__ __forceinline__ void store_mean(void* level0, uint32_t v, uint32_t cn
 }
 
 // 4. one workgroup per brick (blockIdx.x = the brick's Morton index).  The
-//    brick's triangles are taken 1024 at a time, one per lane: a lane walks the
-//    triangle's box inside the brick's footprint (one pixel of margin) with
-//    incremental edge functions; boxes above kLaneBox pixels are queued and
-//    rasterised by the whole workgroup, one pixel per lane.
-constexpr int kLaneBox = 128;
+//    brick's triangles are taken 1024 at a time; the pixels of their boxes
+//    inside the brick's footprint (one pixel of margin) are spread over the
+//    lanes (a lane finds its triangle by a binary search over the prefix of
+//    the box sizes), coverage is tested from a compact copy of the records in
+//    LDS, and only covered pixels read the rest of the record.
+struct CovRec {  // the coverage fields of a TriRec
+  int64_t area;
+  int32_t X[3], Y[3];
+  uint32_t flags;
+  uint32_t tri;
+};
 
 __device__ __forceinline__ void brick_add(uint32_t (*s_sum)[kBrick * kBrick * kBrick], uint32_t* s_cnt, uint32_t v,
                                           float4 color) {
@@ -573,21 +583,6 @@ __device__ __forceinline__ void brick_add(uint32_t (*s_sum)[kBrick * kBrick * kB
   atomicAdd(&s_sum[2][v], fixq(color.z));
   atomicAdd(&s_sum[3][v], fixq(color.w));
   atomicAdd(&s_cnt[v], 1u);
-}
-
-// Candidate box of a record inside the footprint of brick coordinates bc.
-__device__ __forceinline__ bool brick_box_px(const TriRec& r, const uint32_t* bc, int span, int lim, int& i0, int& i1,
-                                             int& j0, int& j1) {
-  const uint32_t fl = r.flags;
-  const int axis = (int)((fl >> 1) & 3u);
-  const int ia = axis == 0 ? 1 : 0, ib = axis == 2 ? 1 : 2;
-  const int qh = (int)(fl >> 8);
-  const int fx = (int)bc[ia] * span, fy = (int)bc[ib] * span;
-  i0 = max(r.qx0, fx - 1);
-  i1 = min(min(r.qx0 + 2 * r.qw - 1, fx + span), lim);
-  j0 = max(r.qy0, fy - 1);
-  j1 = min(min(r.qy0 + 2 * qh - 1, fy + span), lim);
-  return i1 >= i0 && j1 >= j0;
 }
 
 template <int DT>
@@ -600,9 +595,12 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
   constexpr int kVox = kBrick * kBrick * kBrick;
   __shared__ uint32_t s_sum[4][kVox];
   __shared__ uint32_t s_cnt[kVox];
-  __shared__ uint32_t s_big[kBrickThreads];
-  __shared__ uint32_t s_nbig;
-  const int tid = threadIdx.x;
+  __shared__ CovRec s_cov[kBrickThreads];
+  __shared__ uint32_t s_end[kBrickThreads];  // inclusive prefix of candidate counts
+  __shared__ uint32_t s_org[kBrickThreads];  // clipped box origin: i | j << 16
+  __shared__ uint32_t s_div[kBrickThreads];  // ceil(65536 / width) | width << 20
+  __shared__ uint32_t s_wsum[kBrickThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const BrickGeom g = brick_geom(vp.res);
   const uint32_t brick = blockIdx.x;
   const uint32_t nvox = 1u << (3 * g.bits);
@@ -615,57 +613,72 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
     s_sum[3][v] = 0u;
     s_cnt[v] = 0u;
   }
-  if (tid == 0) s_nbig = 0;
   const uint32_t n = brick_count[brick], base = brick_off[brick];
   const int lim = 2 * (int)vp.res - 1;
   const int span = 2 << g.bits;  // footprint of the brick on the raster, pixels per axis
   __syncthreads();
   for (uint32_t b0 = 0; b0 < n; b0 += kBrickThreads) {
+    uint32_t cnt = 0;
     if (b0 + tid < n) {
       const uint32_t t = list[base + b0 + tid];
-      const TriRec r = recs[t];
-      int i0, i1, j0, j1;
-      if (brick_box_px(r, bc, span, lim, i0, i1, j0, j1)) {
-        if ((i1 - i0 + 1) * (j1 - j0 + 1) > kLaneBox) {
-          s_big[atomicAdd(&s_nbig, 1u)] = t;
-        } else {
-          VEdges e;
-          vox_edges(r, i0, j0, e);
-          int64_t ra = e.w[0], rb = e.w[1], rc = e.w[2];
-          for (int j = j0; j <= j1; ++j) {
-            int64_t wa = ra, wb = rb, wc = rc;
-            for (int i = i0; i <= i1; ++i) {
-              uint32_t key;
-              float4 color;
-              if (shade_w<true>(r, e, i, j, wa, wb, wc, vp, mats, key, color, bshift, brick))
-                brick_add(s_sum, s_cnt, key & (nvox - 1u), color);
-              wa += e.di[0];
-              wb += e.di[1];
-              wc += e.di[2];
-            }
-            ra += e.dj[0];
-            rb += e.dj[1];
-            rc += e.dj[2];
-          }
+      const TriRec* rp = recs + t;
+      const uint32_t fl = rp->flags;
+      const int axis = (int)((fl >> 1) & 3u);
+      const int ia = axis == 0 ? 1 : 0, ib = axis == 2 ? 1 : 2;
+      const int qh = (int)(fl >> 8);
+      const int qx0 = rp->qx0, qy0 = rp->qy0, qw = rp->qw;
+      // footprint with one pixel of margin, clipped to the triangle's box
+      const int fx = (int)bc[ia] * span, fy = (int)bc[ib] * span;
+      const int i0 = max(qx0, fx - 1), i1 = min(min(qx0 + 2 * qw - 1, fx + span), lim);
+      const int j0 = max(qy0, fy - 1), j1 = min(min(qy0 + 2 * qh - 1, fy + span), lim);
+      if (i1 >= i0 && j1 >= j0) {
+        const uint32_t cw = (uint32_t)(i1 - i0 + 1);
+        cnt = cw * (uint32_t)(j1 - j0 + 1);
+        s_org[tid] = (uint32_t)i0 | ((uint32_t)j0 << 16);
+        s_div[tid] = ((65536u + cw - 1u) / cw) | (cw << 20);
+        CovRec cr;
+        cr.area = rp->area;
+        for (int v = 0; v < 3; ++v) {
+          cr.X[v] = rp->X[v];
+          cr.Y[v] = rp->Y[v];
         }
+        cr.flags = fl;
+        cr.tri = t;
+        s_cov[tid] = cr;
       }
     }
-    __syncthreads();
-    const uint32_t nbig = s_nbig;
-    for (uint32_t k = 0; k < nbig; ++k) {
-      const TriRec r = recs[s_big[k]];
-      int i0, i1, j0, j1;
-      brick_box_px(r, bc, span, lim, i0, i1, j0, j1);
-      const int cw = i1 - i0 + 1, npx = cw * (j1 - j0 + 1);
-      for (int q = tid; q < npx; q += kBrickThreads) {
-        uint32_t key;
-        float4 color;
-        if (shade_pixel<true>(r, i0 + q % cw, j0 + q / cw, vp, mats, key, color, bshift, brick))
-          brick_add(s_sum, s_cnt, key & (nvox - 1u), color);
-      }
+    uint32_t x = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
     }
+    if (lane == 63) s_wsum[wave] = x;
     __syncthreads();
-    if (tid == 0) s_nbig = 0;
+    for (int w = 0; w < wave; ++w) x += s_wsum[w];
+    s_end[tid] = x;
+    __syncthreads();
+    const uint32_t total = s_end[kBrickThreads - 1];
+    for (uint32_t c = tid; c < total; c += kBrickThreads) {
+      uint32_t lo = 0, hi = kBrickThreads - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_end[mid] > c) hi = mid;
+        else lo = mid + 1;
+      }
+      const uint32_t local = c - (lo ? s_end[lo - 1] : 0u);
+      const uint32_t dv = s_div[lo], cw = dv >> 20;
+      const uint32_t row = (local * (dv & 0xfffffu)) >> 16;  // exact: local < 35 * 35, cw <= 35
+      const uint32_t col = local - row * cw;
+      const uint32_t org = s_org[lo];
+      const int i = (int)(org & 0xffffu) + (int)col, j = (int)(org >> 16) + (int)row;
+      const CovRec cr = s_cov[lo];
+      VEdges e;
+      vox_edges(cr, i, j, e);
+      uint32_t key;
+      float4 color;
+      if (shade_w<true>(cr, recs + cr.tri, e, i, j, e.w[0], e.w[1], e.w[2], vp, mats, key, color, bshift, brick))
+        brick_add(s_sum, s_cnt, key & (nvox - 1u), color);
+    }
     __syncthreads();
   }
   for (uint32_t v = tid; v < nvox; v += kBrickThreads)

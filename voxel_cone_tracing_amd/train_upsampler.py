@@ -10,9 +10,10 @@ then fitted with the library's own BF16 forward/backward kernels and Adam.
 Held-out evaluation: the soup256 benchmark scene (a scene seed that is not
 trained on) from its benchmark camera and from random poses with pose seeds
 >= 10000; PSNR over covered pixels against the plain full-resolution GPU
-frame, for the learned frame without fallback, for bilinear upsampling of
-the same quarter-resolution trace, and the share of 32x32 tiles whose
-estimated PSNR is below 40 dB.
+frame, for the learned frame without the tile fallback, for the same frame
+with a zero network (the surface-guided interpolation alone), for bilinear
+upsampling of the same quarter-resolution trace; the pixels traced at full
+resolution, and the share of 32x32 tiles whose PSNR is below 40 dB.
 
     python -m voxel_cone_tracing_amd.train_upsampler --frames 4096 --out weights.npy --record train.json
 """
