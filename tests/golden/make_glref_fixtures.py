@@ -7,7 +7,7 @@ tests/helpers.SceneSpec, and stores the outputs only: level 0 and the mip
 chain (RGBA8), the indirect-light image and the G-buffer (RGBA32F), with
 the llvmpipe pass times.
 
-    LP_NUM_THREADS=0 python tests/golden/make_glref_fixtures.py
+    LP_NUM_THREADS=0 python tests/golden/make_glref_fixtures.py [scene ...]
 """
 from __future__ import annotations
 
@@ -28,6 +28,9 @@ from helpers import SceneSpec  # noqa: E402
 SCENES = {
     "cornell64": dict(kind="cornell", res=64, W=128, H=96),
     "soup64": dict(kind="soup", res=64, ntri=1500, W=96, H=64),
+    # every phase (direct light, shadow cones, diffuse and specular cones), a
+    # transmissive glossy box and an emissive one: the full compose
+    "cornell64_full": dict(kind="cornell", res=64, W=128, H=96, glass=True, phases=(True, True, True, True)),
 }
 
 
@@ -36,12 +39,15 @@ def soup_point_light(sc):
 
 
 def main():
+    only = sys.argv[1:]
     for name, kw in SCENES.items():
+        if only and name not in only:
+            continue
         sc = SceneSpec(**kw)
         if kw["kind"] == "soup":
             soup_point_light(sc)
         r = glref.frame(sc.res, sc.cube_size, sc.M, sc.view, sc.proj, sc.W, sc.H, sc.pos, sc.nrm, sc.tri_mat,
-                        sc.mat_records, sc.lights)
+                        sc.mat_records, sc.lights, phases=sc.phases)
         mips = np.concatenate([r["mips"][(l, d)].reshape(-1) for l in range(1, 7) for d in range(6)])
         out = HERE / f"glref_{name}.npz"
         np.savez_compressed(out, level0=r["level0"], mips=mips, image=r["image"], gpos=r["gpos"], gnrm=r["gnrm"],

@@ -147,3 +147,48 @@ def test_trace_matches_oracle(gpu_available, orc, kind, dtype, res, ntri):
     err = np.abs(got[:, :3] - want[:, :3]).max()
     say(f"  max-abs vs FP32 oracle = {err:.3e}")
     assert err <= 2e-3
+
+
+@pytest.mark.parametrize("dtype", [L.RGBA8, L.FP16])
+def test_full_compose_matches_oracle(gpu_available, orc, dtype):
+    """All phases on (direct light + shadow cones, diffuse, specular and
+    refraction cones, emission, the transmissive override), on the GPU's own
+    volume and G-buffer."""
+    say(f"full compose parity dtype={dtype}")
+    W, H, res = 128, 96, 64
+    sc = SceneSpec("cornell", res=res, W=W, H=H, glass=True, phases=(True, True, True, True))
+    with Context(W, H) as ctx:
+        sc.build(ctx, dtype)
+        ctx.render()
+        img = ctx.download_image()
+        gp, gn = ctx.download_gbuffer()
+        l0 = decode_level(ctx.download_level(0), dtype)
+        mips = {(l, d): decode_level(ctx.download_level(l, d), dtype) for l in range(1, 7) for d in range(6)}
+    cov = gp[..., 3].reshape(-1) != 0
+    pix = np.flatnonzero(cov).astype(np.int64)
+    want, _ = orc.trace(np.float32, res, 7, l0, mips, gp, gn, sc.mat_records, sc.lights, sc.cam, sc.cube_size,
+                        cone_set=sc.cone_set, phases=sc.phases, pixels=pix)
+    err = np.abs(img.reshape(-1, 4)[pix, :3] - want[:, :3]).max()
+    say(f"  max-abs vs FP32 oracle = {err:.3e}")
+    assert err <= 2e-3
+
+
+@pytest.mark.parametrize("view_dir,lod", [(0, 0.0), (3, 1.5), (5, 4.0)])
+def test_voxel_view_matches_oracle(gpu_available, orc, view_dir, lod):
+    """The voxel-view debug path (voxel_cone_tracing.frag:253-257)."""
+    say(f"voxel view dir={view_dir} lod={lod}")
+    W, H, res = 96, 64, 64
+    sc = SceneSpec("cornell", res=res, W=W, H=H)
+    with Context(W, H) as ctx:
+        sc.build(ctx, L.RGBA8)
+        ctx.set_voxel_view(view_dir, lod)
+        ctx.render()
+        img = ctx.download_image()
+        gp, gn = ctx.download_gbuffer()
+        l0 = decode_level(ctx.download_level(0), L.RGBA8)
+        mips = {(l, d): decode_level(ctx.download_level(l, d), L.RGBA8) for l in range(1, 7) for d in range(6)}
+    want, _ = orc.trace(np.float32, res, 7, l0, mips, gp, gn, sc.mat_records, sc.lights, sc.cam, sc.cube_size,
+                        cone_set=sc.cone_set, phases=sc.phases, view_dir=view_dir, view_lod=lod)
+    err = np.abs(img.reshape(-1, 4) - want).max()
+    say(f"  max-abs vs FP32 oracle = {err:.3e}")
+    assert err <= 1e-4
