@@ -152,6 +152,18 @@ int vct_model_create(vct_ctx ctx, const float* positions, const float* normals,
  * triangulated faces, per-face materials from the MTL.  Returns
  * VCT_INVALID_ID in *out_id on a read failure, like the reference. */
 int vct_load_model(vct_ctx ctx, const char* filename, uint32_t* out_id);
+/* The parsing half of load_model (renderer.cpp:407-537) without a context
+ * or a GPU: vct_obj_open reads the OBJ and the MTL files it names and
+ * reports the triangle and material counts; vct_obj_read copies positions
+ * and normals (9 floats per triangle), each triangle's index into the file's
+ * materials (-1: none), the materials, and the model dimensions as the
+ * reference computes them (renderer.cpp:457-486, 517). */
+typedef struct vct_obj_data* vct_obj;
+#define VCT_ERR_NOT_FOUND -8
+int vct_obj_open(const char* filename, vct_obj* out, uint32_t* ntri, uint32_t* nmat);
+int vct_obj_read(vct_obj obj, float* positions, float* normals, int32_t* tri_material, vct_material* materials,
+                 float dims[3]);
+void vct_obj_close(vct_obj obj);
 int vct_set_model_material(vct_ctx ctx, uint32_t material_id, uint32_t model_id);
 int vct_set_model_transform(vct_ctx ctx, uint32_t model_id, const float m[16]);
 int vct_get_model_dimensions(vct_ctx ctx, uint32_t model_id, float out[3]);

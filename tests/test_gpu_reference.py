@@ -133,3 +133,119 @@ def test_full_compose_against_reference_shader(gpu_available):
     psnr = 10 * np.log10(1.0 / np.mean(e.astype(np.float64) ** 2))
     print(f"  max-abs {np.abs(e).max():.3e}, PSNR {psnr:.1f} dB")
     assert psnr > 60.0 and np.abs(e).max() < 5e-3
+
+
+# ---- the reference application's own scene (CornellBox-Glossy + suzanne) ----------
+APP_LIGHT = [((0.0, 1.4, 0.0), (1.0, 1.0, 1.0), 1.0, L.LIGHT_POINT)]
+
+
+def _app():
+    return np.load(GOLD / "glref_reference_app.npz")
+
+
+def _app_ctx(g, baked=True):
+    """Builds the scene through the C ABI as the application does: materials,
+    models, grid size 3, 128^3, camera, ceiling light, every phase on."""
+    from voxel_cone_tracing_amd.scene import C_add_material_records
+
+    H, W = g["image"].shape[:2]
+    ctx = Context(W, H)
+    ctx.set_grid(128, 7, L.RGBA8, L.AVG_REFERENCE)
+    ctx.set_grid_size(3.0)
+    ctx.set_cone_set(L.CONES_REF9)
+    ctx.set_phases(True, True, True, True)
+    ctx.set_voxel_view(7, 0.0)
+    first = C_add_material_records(ctx, g["mat_records"][1:])
+    assert first == 1
+    ctx.clear_queues()
+    if baked:
+        model = ctx.model_create(g["pos"], g["nrm"], g["tri_mat"])
+        ctx.queue_model(model)
+    else:
+        nb = int(g["nbox"])
+        box = ctx.model_create(g["pos"][:nb], g["nrm"][:nb], g["tri_mat"][:nb])
+        suz = ctx.model_create(g["suzanne_local"], g["nrm"][nb:], None)
+        ctx.set_model_material(int(g["tri_mat"][nb]), suz)  # main.cpp:100
+        ctx.set_model_transform(suz, np.ascontiguousarray(g["suzanne_M"].T))  # column-major
+        ctx.queue_model(box)
+        ctx.queue_model(suz)
+    ctx.set_camera(g["view"], g["proj"])
+    for (p, c, i, k) in APP_LIGHT:
+        ctx.queue_light(p, c, i, k)
+    return ctx
+
+
+def test_reference_app_voxelize_and_mip(gpu_available):
+    g = _app()
+    print("reference app scene: voxelize + mip vs llvmpipe", flush=True)
+    ctx = _app_ctx(g)
+    try:
+        ctx.voxelize()
+        got = ctx.download_level(0)
+        ctx.upload_level(0, 0, g["level0"])
+        ctx.mip()
+        mips = _split_mips(g["mips"], 128)
+        worst_mip = max(int(np.abs(_bytes(ctx.download_level(l, d)) - _bytes(r)).max()) for (l, d), r in mips.items())
+    finally:
+        ctx.close()
+    ref = g["level0"]
+    assert np.array_equal(got != 0, ref != 0)
+    occ = ref != 0
+    same = (got == ref)[occ].mean()
+    worst = np.abs(_bytes(got[occ]) - _bytes(ref[occ])).max()
+    print(f"  voxels {int(occ.sum())}, bit-identical {same:.4f}, max byte difference {worst}, mip max LSB {worst_mip}")
+    assert same > 0.99 and worst <= 2 and worst_mip <= 1
+
+
+def test_reference_app_trace(gpu_available):
+    """voxel_cone_tracing.frag under llvmpipe vs the GPU on llvmpipe's volume
+    and G-buffer, every phase on, the glass suzanne included."""
+    g = _app()
+    print("reference app scene: trace vs llvmpipe", flush=True)
+    ctx = _app_ctx(g)
+    try:
+        ctx.upload_level(0, 0, g["level0"])
+        for (l, d), a in _split_mips(g["mips"], 128).items():
+            ctx.upload_level(l, d, a)
+        ctx.upload_gbuffer(g["gpos"], g["gnrm"])
+        ctx.trace()
+        img = ctx.download_image()
+        ctx.gbuffer()
+        gp, _ = ctx.download_gbuffer()
+    finally:
+        ctx.close()
+    cov = g["gpos"][..., 3] != 0
+    e = img[cov][:, :3] - g["image"][cov][:, :3]
+    psnr = 10 * np.log10(1.0 / np.mean(e.astype(np.float64) ** 2))
+    print(f"  max-abs {np.abs(e).max():.3e}, PSNR {psnr:.1f} dB; G-buffer coverage agreement "
+          f"{((gp[..., 3] != 0) == cov).mean():.5f}")
+    assert psnr > 60.0 and np.abs(e).max() < 1e-2
+    assert ((gp[..., 3] != 0) == cov).mean() > 0.995
+
+
+def test_reference_app_frame_through_the_drop_in(gpu_available):
+    """The whole frame on the GPU (its own voxelize, mip, G-buffer, trace):
+    once with the scene as one pre-transformed model, once as the application
+    builds it (two models, set_model_transform, set_model_material); both
+    against the reference frame under llvmpipe."""
+    g = _app()
+    print("reference app scene: full frame vs llvmpipe", flush=True)
+    imgs = []
+    for baked in (True, False):
+        ctx = _app_ctx(g, baked)
+        try:
+            ctx.render()
+            imgs.append(ctx.download_image())
+        finally:
+            ctx.close()
+    ref = g["image"]
+    cov = g["gpos"][..., 3] != 0
+
+    def psnr(a, b):
+        e = a[cov][:, :3].astype(np.float64) - b[cov][:, :3]
+        return 10 * np.log10(1.0 / np.mean(e ** 2))
+
+    p_baked, p_api, p_pair = psnr(imgs[0], ref), psnr(imgs[1], ref), psnr(imgs[1], imgs[0])
+    print(f"  PSNR vs llvmpipe: one model {p_baked:.1f} dB, application API {p_api:.1f} dB; "
+          f"between the two {p_pair:.1f} dB")
+    assert p_baked > 40.0 and p_api > 40.0 and p_pair > 40.0

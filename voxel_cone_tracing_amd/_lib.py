@@ -22,6 +22,7 @@ VCT_ERR_HIP = -4
 VCT_ERR_NO_DEVICE = -5
 VCT_ERR_STATE = -6
 VCT_ERR_LIMIT = -7
+VCT_ERR_NOT_FOUND = -8
 VCT_INVALID_ID = 0xFFFFFFFF
 
 RGBA8, FP16, BF16, FP32 = 0, 1, 2, 3
@@ -151,6 +152,9 @@ PROTOTYPES = {
     "vct_upsample_set_fallback": (_I, [_P, _F]),
     "vct_upsample_fallback_stats": (_I, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "vct_upsample_traced_pixels": (_I, [_P, C.POINTER(C.c_uint32)]),
+    "vct_obj_open": (_I, [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "vct_obj_read": (_I, [C.c_void_p, _FP, _FP, C.POINTER(C.c_int32), C.c_void_p, _FP]),
+    "vct_obj_close": (None, [C.c_void_p]),
     "vct_scene_soup": (_I, [C.c_uint64, _U, _F, _F, _F, _F, _F, _FP, _FP, _FP]),
     "vct_scene_cornell": (_I, [C.c_uint64, _FP, _FP, _UP, C.POINTER(Material)]),
 }
@@ -234,6 +238,26 @@ def soup(seed: int, ntri: int, extent: float = 1.0, mu: float = -4.0, sigma: flo
     ar = np.zeros((ntri, 4), np.float32)
     check(lib.vct_scene_soup(seed, ntri, extent, mu, sigma, rmin, rmax, fptr(pos), fptr(nrm), fptr(ar)))
     return pos, nrm, ar
+
+
+def read_obj(path: str):
+    """Parses an OBJ/MTL file as load_model does (no GPU needed): positions and
+    normals (n x 9), per-triangle local material index (-1: none), the list of
+    Materials and the reference's model dimensions."""
+    lib = load()
+    h = C.c_void_p()
+    nt, nm = C.c_uint32(), C.c_uint32()
+    check(lib.vct_obj_open(str(path).encode(), C.byref(h), C.byref(nt), C.byref(nm)))
+    try:
+        pos = np.zeros((nt.value, 9), np.float32)
+        nrm = np.zeros((nt.value, 9), np.float32)
+        tm = np.zeros(nt.value, np.int32)
+        mats = (Material * max(1, nm.value))()
+        dims = np.zeros(3, np.float32)
+        check(lib.vct_obj_read(h, fptr(pos), fptr(nrm), tm.ctypes.data_as(C.POINTER(C.c_int32)), mats, fptr(dims)))
+    finally:
+        lib.vct_obj_close(h)
+    return pos, nrm, tm, [mats[i] for i in range(nm.value)], dims
 
 
 def cornell(seed: int):

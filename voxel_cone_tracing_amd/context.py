@@ -101,6 +101,15 @@ class Context:
         a = np.ascontiguousarray(m, np.float32).reshape(16)
         L.check(self.lib.vct_set_model_transform(self.h, model, L.fptr(a)))
 
+    def set_model_material(self, material: int, model: int) -> None:
+        """Every triangle of the model takes the material (renderer.cpp:225-233)."""
+        L.check(self.lib.vct_set_model_material(self.h, material, model))
+
+    def model_dimensions(self, model: int) -> np.ndarray:
+        d = np.zeros(3, np.float32)
+        L.check(self.lib.vct_get_model_dimensions(self.h, model, L.fptr(d)))
+        return d
+
     def queue_model(self, model: int) -> None:
         L.check(self.lib.vct_queue_model(self.h, model))
 

@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -27,6 +28,15 @@ namespace {
 std::string dir_of(const std::string& f) {
   size_t p = f.find_last_of("\\/");
   return p == std::string::npos ? std::string() : f.substr(0, p + 1);
+}
+
+// Name field of a statement: the rest of the line without the whitespace
+// around it (tinyobjloader trims each line's trailing whitespace).
+std::string name_of(const char* rest) {
+  while (*rest == ' ' || *rest == '\t') ++rest;
+  std::string n(rest);
+  const size_t e = n.find_last_not_of(" \t\r\n");
+  return e == std::string::npos ? std::string() : n.substr(0, e + 1);
 }
 
 void init_material(vct_material& m) {
@@ -50,11 +60,9 @@ bool parse_mtl(const std::string& path, std::vector<vct_material>& mats, std::ma
     if (sscanf(s, "%63s%n", key, &n) != 1) continue;
     const char* rest = s + n;
     if (!strcmp(key, "newmtl")) {
-      char name[512] = {0};
-      sscanf(rest, " %511[^\r\n]", name);
       mats.emplace_back();
       init_material(mats.back());
-      names[name] = (int)mats.size() - 1;
+      names[name_of(rest)] = (int)mats.size() - 1;
       cur = &mats.back();
       continue;
     }
@@ -95,20 +103,21 @@ int fix_index(int i, int n) { return i > 0 ? i - 1 : (i < 0 ? n + i : -1); }
 extern "C" __attribute__((visibility("hidden"))) int vct_model_set_dims(vct_ctx ctx, uint32_t model_id,
                                                                       const float dims[3]);
 
-extern "C" int vct_load_model(vct_ctx ctx, const char* filename, uint32_t* out_id) {
-  if (!out_id) return VCT_ERR_INVALID_ARG;
-  *out_id = VCT_INVALID_ID;
-  if (!ctx || !filename) return VCT_ERR_INVALID_ARG;
-  FILE* f = fopen(filename, "r");
-  if (!f) {
-    fprintf(stderr, "Error loading file %s\n", filename);  // renderer.cpp:419
-    return VCT_OK;
-  }
-  std::vector<float> V, N;
+// Parsed content of one OBJ file (and the MTL files it names).
+struct vct_obj_data {
+  std::vector<float> pos, nrm;       // 9 floats per triangle
+  std::vector<int> tri_local_mat;    // index into mats, -1 = none
   std::vector<vct_material> mats;
+  float dims[3] = {0, 0, 0};
+};
+
+namespace {
+
+int parse_obj(const char* filename, vct_obj_data& o) {
+  FILE* f = fopen(filename, "r");
+  if (!f) return -1;
+  std::vector<float> V, N;
   std::map<std::string, int> names;
-  std::vector<float> pos, nrm;
-  std::vector<int> tri_local_mat;
   int cur_mat = -1;
   const std::string base = dir_of(filename);
   float mn[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f}, mx[3] = {0, 0, 0};
@@ -129,13 +138,9 @@ extern "C" int vct_load_model(vct_ctx ctx, const char* filename, uint32_t* out_i
       N.push_back(b);
       N.push_back(c);
     } else if (!strncmp(s, "mtllib", 6)) {
-      char name[1024] = {0};
-      sscanf(s + 6, " %1023[^\r\n]", name);
-      parse_mtl(base + name, mats, names);
+      parse_mtl(base + name_of(s + 6), o.mats, names);
     } else if (!strncmp(s, "usemtl", 6)) {
-      char name[512] = {0};
-      sscanf(s + 6, " %511[^\r\n]", name);
-      auto it = names.find(name);
+      auto it = names.find(name_of(s + 6));
       cur_mat = it == names.end() ? -1 : it->second;
     } else if (s[0] == 'f' && (s[1] == ' ' || s[1] == '\t')) {
       std::vector<Idx> face;
@@ -174,8 +179,8 @@ extern "C" int vct_load_model(vct_ctx ctx, const char* filename, uint32_t* out_i
         for (int c = 0; c < 3; ++c) fn[c] = l > 0 ? fn[c] / l : 0.0f;
         for (int v = 0; v < 3; ++v) {
           for (int c = 0; c < 3; ++c) {
-            pos.push_back(fp[v][c]);
-            nrm.push_back(tri[v].vn >= 0 && tri[v].vn < nn ? N[tri[v].vn * 3 + c] : fn[c]);
+            o.pos.push_back(fp[v][c]);
+            o.nrm.push_back(tri[v].vn >= 0 && tri[v].vn < nn ? N[tri[v].vn * 3 + c] : fn[c]);
           }
           // reference min/max bookkeeping, renderer.cpp:475-486
           if (fp[v][0] > mx[0]) mx[0] = fp[v][0];
@@ -185,27 +190,72 @@ extern "C" int vct_load_model(vct_ctx ctx, const char* filename, uint32_t* out_i
           if (fp[v][2] > mx[2]) mx[2] = fp[v][2];
           if (fp[v][2] < mn[1]) mn[2] = fp[v][2];
         }
-        tri_local_mat.push_back(cur_mat);
+        o.tri_local_mat.push_back(cur_mat);
       }
     }
   }
   fclose(f);
+  for (int c = 0; c < 3; ++c) o.dims[c] = mx[c] - mn[c];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int vct_obj_open(const char* filename, vct_obj* out, uint32_t* ntri, uint32_t* nmat) {
+  if (!filename || !out) return VCT_ERR_INVALID_ARG;
+  *out = nullptr;
+  vct_obj_data* o = new (std::nothrow) vct_obj_data;
+  if (!o) return VCT_ERR_OUT_OF_MEMORY;
+  if (parse_obj(filename, *o) != 0) {
+    delete o;
+    return VCT_ERR_NOT_FOUND;
+  }
+  *out = o;
+  if (ntri) *ntri = (uint32_t)o->tri_local_mat.size();
+  if (nmat) *nmat = (uint32_t)o->mats.size();
+  return VCT_OK;
+}
+
+extern "C" int vct_obj_read(vct_obj o, float* positions, float* normals, int32_t* tri_material,
+                            vct_material* materials, float dims[3]) {
+  if (!o) return VCT_ERR_INVALID_ARG;
+  const size_t nt = o->tri_local_mat.size();
+  if (positions && nt) memcpy(positions, o->pos.data(), nt * 9 * sizeof(float));
+  if (normals && nt) memcpy(normals, o->nrm.data(), nt * 9 * sizeof(float));
+  if (tri_material)
+    for (size_t i = 0; i < nt; ++i) tri_material[i] = o->tri_local_mat[i];
+  if (materials && !o->mats.empty()) memcpy(materials, o->mats.data(), o->mats.size() * sizeof(vct_material));
+  if (dims)
+    for (int c = 0; c < 3; ++c) dims[c] = o->dims[c];
+  return VCT_OK;
+}
+
+extern "C" void vct_obj_close(vct_obj o) { delete o; }
+
+extern "C" int vct_load_model(vct_ctx ctx, const char* filename, uint32_t* out_id) {
+  if (!out_id) return VCT_ERR_INVALID_ARG;
+  *out_id = VCT_INVALID_ID;
+  if (!ctx || !filename) return VCT_ERR_INVALID_ARG;
+  vct_obj_data o;
+  if (parse_obj(filename, o) != 0) {
+    fprintf(stderr, "Error loading file %s\n", filename);  // renderer.cpp:419
+    return VCT_OK;
+  }
   uint32_t base_id = 0;
-  if (!mats.empty()) {
-    int rc = vct_add_materials(ctx, mats.data(), (uint32_t)mats.size(), &base_id);
+  if (!o.mats.empty()) {
+    int rc = vct_add_materials(ctx, o.mats.data(), (uint32_t)o.mats.size(), &base_id);
     if (rc) return rc;
   }
-  const uint32_t ntri = (uint32_t)tri_local_mat.size();
+  const uint32_t ntri = (uint32_t)o.tri_local_mat.size();
   std::vector<uint32_t> tm(ntri);
   for (uint32_t i = 0; i < ntri; ++i) {
-    int lm = tri_local_mat[i] < 0 ? 0 : tri_local_mat[i];
-    tm[i] = mats.empty() ? 0u : base_id + (uint32_t)lm;
+    int lm = o.tri_local_mat[i] < 0 ? 0 : o.tri_local_mat[i];
+    tm[i] = o.mats.empty() ? 0u : base_id + (uint32_t)lm;
   }
   uint32_t id = VCT_INVALID_ID;
-  int rc = vct_model_create(ctx, pos.data(), nrm.data(), tm.data(), ntri, &id);
+  int rc = vct_model_create(ctx, o.pos.data(), o.nrm.data(), tm.data(), ntri, &id);
   if (rc) return rc;
-  float dims[3] = {mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
-  rc = vct_model_set_dims(ctx, id, dims);
+  rc = vct_model_set_dims(ctx, id, o.dims);
   if (rc) return rc;
   *out_id = id;
   return VCT_OK;
