@@ -250,12 +250,13 @@ def main():
         l_img = ctx.download_image()
         nt, nf = ctx.fallback_stats()
         traced = ctx.traced_pixels()
+        stats = ctx.upsample_stats()
         ctx.set_fallback(0.0)
         sc.queue()
         ctx.render_learned()
         raw_img = ctx.download_image()
         ctx.set_fallback(args.fallback_db)
-        learned = dict(ms=l_ms, passes=l_passes, img=l_img, raw=raw_img, tiles=nt, fb=nf, traced=traced)
+        learned = dict(ms=l_ms, passes=l_passes, img=l_img, raw=raw_img, tiles=nt, fb=nf, traced=traced, stats=stats)
         log(f"bench: learned {l_ms:.3f} ms/frame; passes " +
             " ".join(f"{n} {l_passes[i]:.3f}" for i, n in enumerate(PASSES)) + f"; fallback tiles {nf}/{nt}")
     else:
@@ -333,6 +334,7 @@ def main():
         out["speedup_learned_vs_plain"] = round(plain_ms / learned["ms"], 3)
         out["fallback_tiles"] = f"{learned['fb']}/{learned['tiles']}"
         out["traced_pixels"] = f"{learned['traced']}/{int(cov_full.sum())} covered"
+        out["learned_stats"] = learned["stats"]
     out["quality"] = quality
     out["trace"] = {
         "cone_steps": steps,

@@ -299,8 +299,18 @@ int vct_render_learned(vct_ctx ctx);
 /* Minimum estimated tile PSNR in dB (peak 1, over the tile's covered probe
  * pixels, RGB); default 40; <= 0 turns the fallback off. */
 int vct_upsample_set_fallback(vct_ctx ctx, float min_tile_psnr_db);
-/* Tiles of the viewport and tiles traced again by the last learned frame. */
+/* Tiles of the viewport and tiles traced in full by the last learned frame. */
 int vct_upsample_fallback_stats(vct_ctx ctx, uint32_t* tiles, uint32_t* fallback_tiles);
+/* Refinement of a tile that fails the check: 1 (default) first traces the
+ * tile's odd-odd pixels (half resolution), interpolates from them with the
+ * same surface guidance and checks again at 16 even-even probes, tracing the
+ * tile in full only if it fails again; 0 traces every failing tile in full. */
+int vct_upsample_set_refine(vct_ctx ctx, int half_resolution);
+/* Counters of the last learned frame: [tiles, tiles refined at half
+ * resolution (or traced in full when that stage is off), tiles traced in
+ * full, pixels traced after the quarter-resolution stage, pixels traced
+ * after the half-resolution stage, probe pixels traced]. */
+int vct_upsample_stats(vct_ctx ctx, uint32_t out[6]);
 /* Pixels of the last learned frame traced at full resolution because no
  * quarter-resolution sample of their surface was near. */
 int vct_upsample_traced_pixels(vct_ctx ctx, uint32_t* pixels);

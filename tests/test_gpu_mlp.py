@@ -283,3 +283,32 @@ def test_guided_interpolation_and_per_pixel_tracing(gpu_available):
     err = np.abs(got[~exact] - want[yy[~exact], xx[~exact]])
     assert err.max() < 1e-4, err.max()
     assert np.allclose(got[exact], plain[yy[exact], xx[exact], :3], atol=1e-5)
+
+
+@pytest.mark.parametrize("half", [True, False])
+def test_tile_check_and_refinement_hold_40db(gpu_available, half):
+    """The learned frame with the per-tile check at 40 dB: against the plain
+    frame the result holds 40 dB, with (half=True) or without the
+    half-resolution stage; the counters are consistent."""
+    W, H = 320, 192
+    sc, ctx = _soup_ctx(W, H, ntri=20000)
+    try:
+        ctx.mlp_set_params(np.zeros(L.MLP_PARAMS, np.float32))  # guided interpolation alone
+        ctx.set_fallback(40.0)
+        ctx.set_refine(half)
+        plain = _plain(ctx, sc)
+        sc_q(ctx, sc)
+        ctx.render_learned()
+        img = ctx.download_image()
+        st = ctx.upsample_stats()
+        gp, _ = ctx.download_gbuffer()
+    finally:
+        ctx.close()
+    cov = gp[..., 3] != 0
+    e = img[..., :3][cov].astype(np.float64) - plain[..., :3][cov]
+    p = 10 * np.log10(1.0 / np.mean(e ** 2))
+    say(f"half={half}: PSNR {p:.2f} dB, {st}")
+    assert np.isfinite(img).all() and (img[..., 3] >= 0).all()  # no pixel left unresolved
+    assert p >= 40.0
+    assert st["tiles"] == 10 * 6 and 0 <= st["tiles_full"] <= st["tiles_refined"] <= st["tiles"]
+    assert st["probes"] >= 16 * st["tiles"]

@@ -152,6 +152,8 @@ PROTOTYPES = {
     "vct_upsample_set_fallback": (_I, [_P, _F]),
     "vct_upsample_fallback_stats": (_I, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "vct_upsample_traced_pixels": (_I, [_P, C.POINTER(C.c_uint32)]),
+    "vct_upsample_set_refine": (_I, [_P, _I]),
+    "vct_upsample_stats": (_I, [_P, C.POINTER(C.c_uint32)]),
     "vct_obj_open": (_I, [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "vct_obj_read": (_I, [C.c_void_p, _FP, _FP, C.POINTER(C.c_int32), C.c_void_p, _FP]),
     "vct_obj_close": (None, [C.c_void_p]),

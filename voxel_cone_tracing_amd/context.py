@@ -247,6 +247,16 @@ class Context:
         L.check(self.lib.vct_upsample_fallback_stats(self.h, C.byref(t), C.byref(f)))
         return t.value, f.value
 
+    def set_refine(self, half_resolution: bool) -> None:
+        L.check(self.lib.vct_upsample_set_refine(self.h, int(bool(half_resolution))))
+
+    def upsample_stats(self) -> dict:
+        """Counters of the last learned frame."""
+        a = (C.c_uint32 * 6)()
+        L.check(self.lib.vct_upsample_stats(self.h, a))
+        keys = ["tiles", "tiles_refined", "tiles_full", "pixels_traced_quarter", "pixels_traced_half", "probes"]
+        return dict(zip(keys, [int(x) for x in a]))
+
     def traced_pixels(self) -> int:
         """Pixels of the last learned frame traced at full resolution (no
         quarter-resolution sample of their surface nearby)."""

@@ -227,6 +227,10 @@ struct vct_context {
   // per-tile fallback of the learned frame
   VctBuf probe_pix, probe_val, fb_list;  // fb_list: [count][tiles...]
   VctBuf fb_pix;                          // [count][pixels...] traced at full resolution
+  VctBuf fb_list2, fb_pix2, fb_flag;      // tiles refined again, pixels listed by the half-res pass, tile states
+  VctBuf half_img, probe_pix2, probe_val2;
+  bool fb_half = true;                    // refine failing tiles at half resolution before full
+  bool fb_half_ran = false;
   int probe_w = 0, probe_h = 0;
   float fb_psnr_db = 40.0f;  // minimum estimated tile PSNR; <= 0 disables the fallback
   bool fb_ran = false;
@@ -263,6 +267,10 @@ struct TraceSelect {
   const uint32_t* tile_count;  // device-side count of listed tiles
   uint32_t max_tiles;          // grid bound (host side)
   int tiles_x;
+  const uint32_t* tile_flag;   // pixel lists: trace only where tile_flag[tile] == flag_want
+  uint32_t flag_want;
+  int grid_stride;             // tile lists: 2 = the odd-odd pixels, written to a half-resolution image
+  int half_w;
 };
 int vct_trace_select(vct_context* ctx, const float4* gpos, const float4* gnrm, float4* out, int W, int H,
                      int demodulate, unsigned long long* step_counter, const TraceSelect& sel);

@@ -246,8 +246,12 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs A) {
     for (int nt = 0; nt < 4; ++nt) {
       const int s = wave * 64 + nt * 16 + lane;
       const int qx = blockIdx.x * 16 + (s & 15), qy = blockIdx.y * 16 + (s >> 4);
-      if (qx >= A.F.W || qy >= A.F.H || s_st[s] == kTraceIt) continue;
+      if (qx >= A.F.W || qy >= A.F.H) continue;
       float4 o;
+      if (s_st[s] == kTraceIt) {  // left for tracing
+        A.image[(size_t)qy * A.F.W + qx] = make_float4(0.f, 0.f, 0.f, -1.0f);
+        continue;
+      }
       if (s_st[s] == kFeatures) {
         float3 v = make_float3(s_base[s][0] + y[nt][0], s_base[s][1] + y[nt][1], s_base[s][2] + y[nt][2]);
         if ((qx & 3) == 1 && (qy & 3) == 1 && (qx >> 2) < A.F.wl && (qy >> 2) < A.F.hl) {
@@ -576,44 +580,60 @@ __global__ __launch_bounds__(256) void mlp_pack_kernel(const float* __restrict__
 
 }  // namespace vct
 
-// ---- per-tile fallback ------------------------------------------------------------
-// Every 32x32 tile of the upsampled frame is checked at 16 probe pixels (a
-// 4x4 grid at offsets 3, 11, 19, 27: the pixels farthest from the traced
-// quarter-resolution samples), traced at full resolution beforehand.  A tile
-// whose mean squared error over its covered probes exceeds the bound is
-// listed and traced again in full.  The probes' exact values are written into
-// the image.  One workgroup; the list comes out in tile order.
+namespace vct {
+
+// ---- per-tile check and refinement -------------------------------------------------
+// Every 32x32 tile is checked at 16 probe pixels traced at full resolution.
+// Stage 1 probes sit at offsets 3, 11, 19, 27 (the odd-odd pixels farthest
+// from the quarter-resolution samples); a tile whose mean squared error over
+// its covered probes exceeds the bound is refined: with the half-resolution
+// stage on, its odd-odd pixels (a 16x16 grid) are traced and interpolated
+// with the same surface guidance, then checked again at even-even probes
+// (offsets 4, 12, 20, 28, the pixels farthest from that grid); tiles that
+// still fail, or every failing tile with the stage off, are traced in full.
+// Probe values are written into the image; a probe on a pixel that the
+// interpolation left for tracing (w < 0) counts as exact.
 constexpr int kFbTile = 32;
 constexpr int kProbes = 16;
 
-__global__ __launch_bounds__(1024) void fallback_select_kernel(const float4* __restrict__ probe_val,
-                                                               const uint32_t* __restrict__ probe_pix,
-                                                               const float4* __restrict__ gpos,
-                                                               float4* __restrict__ image, uint32_t ntiles,
-                                                               float max_mse, uint32_t* __restrict__ fb) {
+// One workgroup.  Checks every tile (in_list null) or the listed tiles,
+// lists the failing ones in tile order and sets flags[t] to fail_flag or
+// pass_flag.  Probe k of the i-th checked tile is probe_val / probe_pix [16 i + k].
+__global__ __launch_bounds__(1024) void tile_select_kernel(const float4* __restrict__ probe_val,
+                                                           const uint32_t* __restrict__ probe_pix,
+                                                           const float4* __restrict__ gpos, float4* image,
+                                                           const uint32_t* __restrict__ in_list, uint32_t ntiles,
+                                                           float max_mse, uint32_t* __restrict__ out,
+                                                           uint32_t* __restrict__ flags, uint32_t pass_flag,
+                                                           uint32_t fail_flag) {
   __shared__ uint32_t s_wsum[16];
   __shared__ uint32_t s_base;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (tid == 0) s_base = 0;
   __syncthreads();
-  uint32_t* list = fb + 1;
-  for (uint32_t t0 = 0; t0 < ntiles; t0 += 1024) {
-    const uint32_t t = t0 + tid;
+  const uint32_t n = in_list ? in_list[0] : ntiles;
+  uint32_t* list = out + 1;
+  for (uint32_t i0 = 0; i0 < n; i0 += 1024) {
+    const uint32_t i = i0 + tid;
     bool flag = false;
-    if (t < ntiles) {
+    uint32_t t = 0;
+    if (i < n) {
+      t = in_list ? in_list[1 + i] : i;
       float sse = 0.0f;
       int nc = 0;
       for (int k = 0; k < kProbes; ++k) {
-        const uint32_t q = t * kProbes + k;
+        const uint32_t q = i * kProbes + k;
         const uint32_t p = probe_pix[q];
         if (gpos[p].w == 0.0f) continue;  // background
         const float4 v = probe_val[q], m = image[p];
+        if (m.w < 0.0f) continue;  // left for tracing: exact
         const float dx = v.x - m.x, dy = v.y - m.y, dz = v.z - m.z;
         sse += dx * dx + dy * dy + dz * dz;
         ++nc;
       }
       flag = nc > 0 && sse > max_mse * 3.0f * (float)nc;
-      for (int k = 0; k < kProbes; ++k) image[probe_pix[t * kProbes + k]] = probe_val[t * kProbes + k];
+      for (int k = 0; k < kProbes; ++k) image[probe_pix[i * kProbes + k]] = probe_val[i * kProbes + k];
+      flags[t] = flag ? fail_flag : pass_flag;
     }
     const unsigned long long m = __ballot(flag);
     if (lane == 0) s_wsum[wave] = (uint32_t)__popcll(m);
@@ -629,8 +649,100 @@ __global__ __launch_bounds__(1024) void fallback_select_kernel(const float4* __r
     }
     __syncthreads();
   }
-  if (tid == 0) fb[0] = s_base;
+  if (tid == 0) out[0] = s_base;
 }
+
+// Stage-2 probe pixels of the listed tiles: pix[0] = count, then 16 per tile.
+__global__ void stage2_probes_kernel(const uint32_t* __restrict__ list, int tiles_x, int W, int H,
+                                     uint32_t* __restrict__ pix) {
+  const uint32_t n = list[0];
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n * kProbes; q += gridDim.x * blockDim.x) {
+    const uint32_t t = list[1 + q / kProbes], k = q % kProbes;
+    const int x = min((int)(t % (uint32_t)tiles_x) * kFbTile + 8 * (int)(k & 3u) + 4, W - 1);
+    const int y = min((int)(t / (uint32_t)tiles_x) * kFbTile + 8 * (int)(k >> 2) + 4, H - 1);
+    pix[1 + q] = (uint32_t)(y * W + x);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) pix[0] = n * kProbes;
+}
+
+// Half-resolution interpolation of the listed tiles (one workgroup per tile):
+// the 3x3 odd-odd grid points around the pixel's nearest one, weighted by a
+// tent of radius 2 grid points and by the same-surface test of the
+// quarter-resolution stage; grid points of tiles without half-resolution
+// data do not count.  Grid points keep their traced value; pixels without a
+// same-surface grid point are listed in trace_list and marked w = -1.
+__global__ __launch_bounds__(256) void half_upsample_kernel(const uint32_t* __restrict__ list,
+                                                            const uint32_t* __restrict__ flags, int tiles_x,
+                                                            const float4* __restrict__ gpos,
+                                                            const float4* __restrict__ gnrm,
+                                                            const DevMaterial* __restrict__ mats, uint32_t nmat,
+                                                            const float4* __restrict__ half, int hw, int hh, int W,
+                                                            int H, float4* __restrict__ image,
+                                                            uint32_t* __restrict__ trace_list) {
+  const uint32_t k = blockIdx.x;
+  if (k >= list[0]) return;
+  const uint32_t t = list[1 + k];
+  const int x0 = (int)(t % (uint32_t)tiles_x) * kFbTile, y0 = (int)(t / (uint32_t)tiles_x) * kFbTile;
+  const int lane = threadIdx.x & 63;
+  for (int q = threadIdx.x; q < kFbTile * kFbTile; q += 256) {
+    const int px = x0 + (q & 31), py = y0 + (q >> 5);
+    const bool inside = px < W && py < H;
+    const size_t idx = inside ? (size_t)py * W + px : 0;
+    const float4 gp = gpos[idx];
+    int st = inside ? (gp.w == 0.0f ? kBackground : kFeatures) : -1;
+    float4 o = make_float4(0.15f, 0.25f, 0.25f, 1.0f);  // clear colour, renderer.cpp:398
+    if (st == kFeatures) {
+      const float4 gn = gnrm[idx];
+      const uint32_t mbits = __builtin_bit_cast(uint32_t, gn.w);
+      const DevMaterial& m = mats[mbits < nmat ? mbits : 0u];
+      const float nl = sqrtf(gn.x * gn.x + gn.y * gn.y + gn.z * gn.z);
+      const float inl = nl > 0.0f ? 1.0f / nl : 0.0f;
+      const float nx = gn.x * inl, ny = gn.y * inl, nz = gn.z * inl;
+      const float u = (float)(px - 1) * 0.5f, v = (float)(py - 1) * 0.5f;
+      float3 acc = make_float3(0.f, 0.f, 0.f);
+      float wsum = 0.0f;
+      if ((px & 1) && (py & 1)) {  // a grid point: its traced value
+        const float4 h = half[(size_t)(py >> 1) * hw + (px >> 1)];
+        acc = make_float3(h.x, h.y, h.z);
+        wsum = 1.0f;
+      } else {
+        const int ic = (int)floorf(u + 0.5f), jc = (int)floorf(v + 0.5f);
+        for (int kk = 0; kk < 9; ++kk) {
+          const int i = ic + (kk % 3) - 1, j = jc + (kk / 3) - 1;
+          if (i < 0 || i >= hw || j < 0 || j >= hh) continue;
+          const int sx = 2 * i + 1, sy = 2 * j + 1;
+          if (sx >= W || sy >= H || flags[(sy >> 5) * tiles_x + (sx >> 5)] == 0u) continue;
+          const size_t sidx = (size_t)sy * W + sx;
+          const float4 ln = gnrm[sidx];
+          const float4 hv = half[(size_t)j * hw + i];
+          if (__builtin_bit_cast(uint32_t, ln.w) != mbits || !(hv.w > 0.0f)) continue;
+          const float d = nx * ln.x + ny * ln.y + nz * ln.z;
+          const float ll = sqrtf(ln.x * ln.x + ln.y * ln.y + ln.z * ln.z);
+          if (!(d > 0.95f * ll) || !(fabsf(hv.w - gp.w) < 0.05f * gp.w)) continue;
+          const float wk =
+              fmaxf(0.0f, 1.0f - 0.5f * fabsf(u - (float)i)) * fmaxf(0.0f, 1.0f - 0.5f * fabsf(v - (float)j));
+          acc = make_float3(acc.x + wk * hv.x, acc.y + wk * hv.y, acc.z + wk * hv.z);
+          wsum += wk;
+        }
+      }
+      if (wsum > 0.0f) {
+        const float iw = 1.0f / wsum;
+        o = make_float4(m.diffuse[0] * acc.x * iw, m.diffuse[1] * acc.y * iw, m.diffuse[2] * acc.z * iw, 1.0f);
+      } else {
+        st = kTraceIt;
+        o = make_float4(0.f, 0.f, 0.f, -1.0f);
+      }
+    }
+    const unsigned long long bm = __ballot(st == kTraceIt);
+    uint32_t b0 = 0;
+    if (lane == 0 && bm) b0 = atomicAdd(trace_list, (uint32_t)__popcll(bm));
+    b0 = __shfl(b0, 0, 64);
+    if (st == kTraceIt) trace_list[1 + b0 + (uint32_t)__popcll(bm & ((1ull << lane) - 1ull))] = (uint32_t)idx;
+    if (st >= 0) image[idx] = o;
+  }
+}
+
+}  // namespace vct
 
 // ============================================================================
 // host side
@@ -810,14 +922,19 @@ int trace_listed_pixels(vct_context* ctx) {
                           ctx->width, ctx->height, 0, nullptr, sel);
 }
 
-// Probe trace, tile selection and full-resolution trace of the selected tiles.
-int fallback(vct_context* ctx) {
+// Per-tile check and refinement of the learned frame (see tile_select_kernel).
+int refine(vct_context* ctx) {
   const int W = ctx->width, H = ctx->height;
   const int tx = (W + kFbTile - 1) / kFbTile, ty = (H + kFbTile - 1) / kFbTile;
   const uint32_t ntiles = (uint32_t)(tx * ty);
+  const size_t npix = (size_t)W * H;
+  const float4* gpos = (const float4*)ctx->g_pos.p;
+  const float4* gnrm = (const float4*)ctx->g_nrm.p;
+  float4* image = (float4*)ctx->image.p;
   int rc;
   if ((rc = vct_ensure(ctx->probe_val, (size_t)ntiles * kProbes * 16, ctx))) return rc;
   if ((rc = vct_ensure(ctx->fb_list, (size_t)(ntiles + 1) * 4, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->fb_flag, (size_t)ntiles * 4, ctx))) return rc;
   if (ctx->probe_w != W || ctx->probe_h != H || !ctx->probe_pix.p) {
     std::vector<uint32_t> pix((size_t)ntiles * kProbes);
     for (int j = 0; j < ty; ++j)
@@ -833,25 +950,87 @@ int fallback(vct_context* ctx) {
     ctx->probe_w = W;
     ctx->probe_h = H;
   }
+  const float max_mse = powf(10.0f, -ctx->fb_psnr_db / 10.0f);
+  uint32_t* flags = (uint32_t*)ctx->fb_flag.p;
+  uint32_t* l1 = (uint32_t*)ctx->fb_list.p;
+  // stage 1: probes of every tile, failing tiles -> l1 (flag 1), others flag 0
   TraceSelect sel;
   memset(&sel, 0, sizeof(sel));
   sel.pix_list = (const uint32_t*)ctx->probe_pix.p;
   sel.npix_list = ntiles * kProbes;
-  if ((rc = vct_trace_select(ctx, (const float4*)ctx->g_pos.p, (const float4*)ctx->g_nrm.p,
-                             (float4*)ctx->probe_val.p, W, H, 0, nullptr, sel)))
-    return rc;
-  const float max_mse = powf(10.0f, -ctx->fb_psnr_db / 10.0f);
-  hipLaunchKernelGGL(fallback_select_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const float4*)ctx->probe_val.p,
-                     (const uint32_t*)ctx->probe_pix.p, (const float4*)ctx->g_pos.p, (float4*)ctx->image.p, ntiles,
-                     max_mse, (uint32_t*)ctx->fb_list.p);
+  if ((rc = vct_trace_select(ctx, gpos, gnrm, (float4*)ctx->probe_val.p, W, H, 0, nullptr, sel))) return rc;
+  hipLaunchKernelGGL(tile_select_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const float4*)ctx->probe_val.p,
+                     (const uint32_t*)ctx->probe_pix.p, gpos, image, (const uint32_t*)nullptr, ntiles, max_mse, l1,
+                     flags, 0u, 1u);
   VCT_HIP(hipGetLastError());
+  // pixels the quarter-resolution stage left, in tiles that passed
   memset(&sel, 0, sizeof(sel));
-  sel.tile_list = (const uint32_t*)ctx->fb_list.p + 1;
-  sel.tile_count = (const uint32_t*)ctx->fb_list.p;
+  sel.pix_list = (const uint32_t*)ctx->fb_pix.p + 1;
+  sel.pix_count = (const uint32_t*)ctx->fb_pix.p;
+  sel.npix_list = (uint32_t)npix;
+  sel.pix_scatter = 1;
+  sel.tile_flag = flags;
+  sel.flag_want = 0u;
+  sel.tiles_x = tx;
+  if ((rc = vct_trace_select(ctx, gpos, gnrm, image, W, H, 0, nullptr, sel))) return rc;
+  const uint32_t* full_list = l1;
+  ctx->fb_half_ran = false;
+  if (ctx->fb_half) {
+    const int hw = (W + 1) / 2, hh = (H + 1) / 2;
+    if ((rc = vct_ensure(ctx->half_img, (size_t)hw * hh * 16, ctx))) return rc;
+    if ((rc = vct_ensure(ctx->fb_pix2, (npix + 1) * 4, ctx))) return rc;
+    if ((rc = vct_ensure(ctx->fb_list2, (size_t)(ntiles + 1) * 4, ctx))) return rc;
+    if ((rc = vct_ensure(ctx->probe_pix2, ((size_t)ntiles * kProbes + 1) * 4, ctx))) return rc;
+    if ((rc = vct_ensure(ctx->probe_val2, (size_t)ntiles * kProbes * 16, ctx))) return rc;
+    // odd-odd grid of the l1 tiles, demodulated (w = depth)
+    memset(&sel, 0, sizeof(sel));
+    sel.tile_list = l1 + 1;
+    sel.tile_count = l1;
+    sel.max_tiles = ntiles;
+    sel.tiles_x = tx;
+    sel.grid_stride = 2;
+    sel.half_w = hw;
+    if ((rc = vct_trace_select(ctx, gpos, gnrm, (float4*)ctx->half_img.p, W, H, 1, nullptr, sel))) return rc;
+    uint32_t* p2 = (uint32_t*)ctx->fb_pix2.p;
+    hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, ctx->stream, p2);
+    hipLaunchKernelGGL(half_upsample_kernel, dim3(ntiles), dim3(256), 0, ctx->stream, (const uint32_t*)l1,
+                       (const uint32_t*)flags, tx, gpos, gnrm, (const DevMaterial*)ctx->d_materials.p,
+                       (uint32_t)ctx->materials.size(), (const float4*)ctx->half_img.p, hw, hh, W, H, image, p2);
+    VCT_HIP(hipGetLastError());
+    // stage 2: even-even probes of the l1 tiles -> l2 (flag 2), others stay 1
+    uint32_t* pp2 = (uint32_t*)ctx->probe_pix2.p;
+    hipLaunchKernelGGL(stage2_probes_kernel, dim3(64), dim3(256), 0, ctx->stream, (const uint32_t*)l1, tx, W, H, pp2);
+    VCT_HIP(hipGetLastError());
+    memset(&sel, 0, sizeof(sel));
+    sel.pix_list = pp2 + 1;
+    sel.pix_count = pp2;
+    sel.npix_list = ntiles * kProbes;
+    if ((rc = vct_trace_select(ctx, gpos, gnrm, (float4*)ctx->probe_val2.p, W, H, 0, nullptr, sel))) return rc;
+    uint32_t* l2 = (uint32_t*)ctx->fb_list2.p;
+    hipLaunchKernelGGL(tile_select_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const float4*)ctx->probe_val2.p,
+                       (const uint32_t*)(pp2 + 1), gpos, image, (const uint32_t*)l1, ntiles, max_mse, l2, flags, 1u,
+                       2u);
+    VCT_HIP(hipGetLastError());
+    // pixels the half-resolution stage left, in tiles that passed stage 2
+    memset(&sel, 0, sizeof(sel));
+    sel.pix_list = p2 + 1;
+    sel.pix_count = p2;
+    sel.npix_list = (uint32_t)npix;
+    sel.pix_scatter = 1;
+    sel.tile_flag = flags;
+    sel.flag_want = 1u;
+    sel.tiles_x = tx;
+    if ((rc = vct_trace_select(ctx, gpos, gnrm, image, W, H, 0, nullptr, sel))) return rc;
+    full_list = l2;
+    ctx->fb_half_ran = true;
+  }
+  // tiles traced in full
+  memset(&sel, 0, sizeof(sel));
+  sel.tile_list = full_list + 1;
+  sel.tile_count = full_list;
   sel.max_tiles = ntiles;
   sel.tiles_x = tx;
-  return vct_trace_select(ctx, (const float4*)ctx->g_pos.p, (const float4*)ctx->g_nrm.p, (float4*)ctx->image.p, W,
-                          H, 0, nullptr, sel);
+  return vct_trace_select(ctx, gpos, gnrm, image, W, H, 0, nullptr, sel);
 }
 
 }  // namespace
@@ -1069,11 +1248,12 @@ int vct_render_learned(vct_ctx ctx) {
   VCT_HIP(hipEventRecord(ctx->ev[9], ctx->stream));
   ctx->ev_recorded[4] = true;
   VCT_HIP(hipEventRecord(ctx->ev[10], ctx->stream));
-  if ((rc = trace_listed_pixels(ctx))) return rc;
   ctx->fb_ran = false;
   if (ctx->fb_psnr_db > 0.0f) {
-    if ((rc = fallback(ctx))) return rc;
+    if ((rc = refine(ctx))) return rc;
     ctx->fb_ran = true;
+  } else if ((rc = trace_listed_pixels(ctx))) {
+    return rc;
   }
   VCT_HIP(hipEventRecord(ctx->ev[11], ctx->stream));
   ctx->ev_recorded[5] = true;
@@ -1098,18 +1278,43 @@ int vct_upsample_traced_pixels(vct_ctx ctx, uint32_t* pixels) {
   return VCT_OK;
 }
 
-int vct_upsample_fallback_stats(vct_ctx ctx, uint32_t* tiles, uint32_t* fallback_tiles) {
+int vct_upsample_set_refine(vct_ctx ctx, int half_resolution) {
   MLP_CTX(ctx);
-  const uint32_t nt = (uint32_t)(((ctx->width + kFbTile - 1) / kFbTile) * ((ctx->height + kFbTile - 1) / kFbTile));
-  if (tiles) *tiles = nt;
-  if (fallback_tiles) {
-    uint32_t n = 0;
-    if (ctx->fb_ran && ctx->fb_list.p) {
-      VCT_HIP(hipMemcpyAsync(&n, ctx->fb_list.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-      VCT_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->fb_half = half_resolution != 0;
+  return VCT_OK;
+}
+
+int vct_upsample_stats(vct_ctx ctx, uint32_t out[6]) {
+  MLP_CTX(ctx);
+  if (!out) return vct_set_error(VCT_ERR_INVALID_ARG, "null out");
+  for (int i = 0; i < 6; ++i) out[i] = 0;
+  out[0] = (uint32_t)(((ctx->width + kFbTile - 1) / kFbTile) * ((ctx->height + kFbTile - 1) / kFbTile));
+  auto rd = [&](const VctBuf& b, uint32_t& v) -> int {
+    if (!b.p) return VCT_OK;
+    VCT_HIP(hipMemcpyAsync(&v, b.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    return VCT_OK;
+  };
+  int rc;
+  if ((rc = rd(ctx->fb_pix, out[3]))) return rc;
+  if (ctx->fb_ran) {
+    if ((rc = rd(ctx->fb_list, out[1]))) return rc;
+    if (ctx->fb_half_ran) {
+      if ((rc = rd(ctx->fb_list2, out[2]))) return rc;
+      if ((rc = rd(ctx->fb_pix2, out[4]))) return rc;
     }
-    *fallback_tiles = n;
   }
+  VCT_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->fb_ran && !ctx->fb_half_ran) out[2] = out[1];  // every failing tile was traced in full
+  if (ctx->fb_ran) out[5] = out[0] * kProbes + (ctx->fb_half_ran ? out[1] * kProbes : 0u);
+  return VCT_OK;
+}
+
+int vct_upsample_fallback_stats(vct_ctx ctx, uint32_t* tiles, uint32_t* fallback_tiles) {
+  uint32_t st[6];
+  int rc = vct_upsample_stats(ctx, st);
+  if (rc) return rc;
+  if (tiles) *tiles = st[0];
+  if (fallback_tiles) *fallback_tiles = st[2];
   return VCT_OK;
 }
 

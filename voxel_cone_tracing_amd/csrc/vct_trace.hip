@@ -62,6 +62,11 @@ struct TraceArgs {
   const uint32_t* tile_list;   // listed 32x32 tiles (ty * tiles_x + tx); 16 workgroups per tile
   const uint32_t* tile_count;  // device-side number of listed tiles
   int tiles_x;
+  const uint32_t* tile_flag;   // optional: listed pixels are traced only where tile_flag[tile] == flag_want
+  uint32_t flag_want;
+  int grid_stride;             // tile-list mode: 2 = the odd-odd pixels of the tiles (a 16x16 grid per tile),
+                               // written to image[(y >> 1) * half_w + (x >> 1)]
+  int half_w;
   ConeSet cones;
   LightSet lights;
 };
@@ -266,18 +271,31 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
     px = (int)(p % (uint32_t)A.W);
     py = (int)(p / (uint32_t)A.W);
     oidx = A.pix_scatter ? (size_t)p : (size_t)k;
+    if (valid && A.tile_flag) valid = A.tile_flag[(py >> 5) * A.tiles_x + (px >> 5)] == A.flag_want;
   } else {
     int bx = blockIdx.x, by = blockIdx.y;
-    if (A.tile_list) {
-      const uint32_t k = blockIdx.x >> 4;
+    if (A.tile_list && A.grid_stride == 2) {
+      // 4 workgroups of 8x8 grid points per tile; grid point (u, v) is pixel (2u + 1, 2v + 1)
+      const uint32_t k = blockIdx.x >> 2;
       if (k >= *A.tile_count) return;  // the whole workgroup leaves
       const uint32_t t = A.tile_list[k];
-      bx = (int)(t % (uint32_t)A.tiles_x) * 4 + (int)(blockIdx.x & 3u);
-      by = (int)(t / (uint32_t)A.tiles_x) * 4 + (int)((blockIdx.x >> 2) & 3u);
+      const int gu = (int)(t % (uint32_t)A.tiles_x) * 16 + (int)(blockIdx.x & 1u) * 8 + (lane & 7);
+      const int gv = (int)(t / (uint32_t)A.tiles_x) * 16 + (int)((blockIdx.x >> 1) & 1u) * 8 + (lane >> 3);
+      px = 2 * gu + 1;
+      py = 2 * gv + 1;
+      oidx = (size_t)gv * A.half_w + gu;
+    } else {
+      if (A.tile_list) {
+        const uint32_t k = blockIdx.x >> 4;
+        if (k >= *A.tile_count) return;  // the whole workgroup leaves
+        const uint32_t t = A.tile_list[k];
+        bx = (int)(t % (uint32_t)A.tiles_x) * 4 + (int)(blockIdx.x & 3u);
+        by = (int)(t / (uint32_t)A.tiles_x) * 4 + (int)((blockIdx.x >> 2) & 3u);
+      }
+      px = bx * 8 + (lane & 7);
+      py = by * 8 + (lane >> 3);
+      oidx = (size_t)py * A.W + px;
     }
-    px = bx * 8 + (lane & 7);
-    py = by * 8 + (lane >> 3);
-    oidx = (size_t)py * A.W + px;
   }
   valid = valid && px < A.W && py < A.H;
   const size_t idx = valid ? (size_t)py * A.W + px : 0;
@@ -508,6 +526,10 @@ int vct_trace_select(vct_context* ctx, const float4* gpos, const float4* gnrm, f
   A.tile_list = sel.tile_list;
   A.tile_count = sel.tile_count;
   A.tiles_x = sel.tiles_x;
+  A.tile_flag = sel.tile_flag;
+  A.flag_want = sel.flag_want;
+  A.grid_stride = sel.grid_stride;
+  A.half_w = sel.half_w;
   A.vol = (const char*)ctx->volume.p;
   for (int l = 0; l < ctx->levels; ++l)
     for (int d = 0; d < 6; ++d) A.off[l][d] = ctx->lvl_off[l][d];
@@ -550,7 +572,8 @@ int vct_trace_select(vct_context* ctx, const float4* gpos, const float4* gnrm, f
     grid = dim3((sel.npix_list + 63) / 64);
   } else if (sel.tile_list) {
     if (sel.max_tiles == 0) return VCT_OK;
-    grid = dim3(16 * sel.max_tiles);  // 16 blocks of 8x8 per 32x32 tile
+    // 16 blocks of 8x8 pixels per 32x32 tile, or 4 blocks of 8x8 grid points
+    grid = dim3((sel.grid_stride == 2 ? 4 : 16) * sel.max_tiles);
   }
   switch (ctx->dtype) {
     case VCT_RGBA8: hipLaunchKernelGGL(trace_kernel<VCT_RGBA8>, grid, dim3(256), 0, ctx->stream, A); break;
