@@ -300,7 +300,7 @@ def main():
     else:
         achieved, peak, u = amount / t / 1e9, HBM_PEAK_GBS, "GB/s"
     kernels = {"voxelize": "vox_brick_kernel", "mip": "mip_kernel", "gbuffer": "g_tile_kernel",
-               "trace": "trace_kernel", "upsample": "mlp_infer_kernel"}
+               "trace": "trace_frame_kernel", "upsample": "mlp_infer_kernel"}
     roofline = {
         "kernel": kernels.get(name, name), "pass": PASSES[dom], "bound": bound, "achieved": round(achieved, 1),
         "peak": peak, "unit": u, "frac": round(achieved / peak, 4), "traffic": None,

@@ -667,7 +667,8 @@ __global__ void stage2_probes_kernel(const uint32_t* __restrict__ list, int tile
 
 // Half-resolution interpolation of the listed tiles (one workgroup per tile):
 // the 3x3 odd-odd grid points around the pixel's nearest one, weighted by a
-// tent of radius 2 grid points and by the same-surface test of the
+// tent of radius 1.25 grid points (2.5 pixels; wider tents blur the voxel-scale
+// detail of the indirect light) and by the same-surface test of the
 // quarter-resolution stage; grid points of tiles without half-resolution
 // data do not count.  Grid points keep their traced value; pixels without a
 // same-surface grid point are listed in trace_list and marked w = -1.
@@ -720,7 +721,7 @@ __global__ __launch_bounds__(256) void half_upsample_kernel(const uint32_t* __re
           const float ll = sqrtf(ln.x * ln.x + ln.y * ln.y + ln.z * ln.z);
           if (!(d > 0.95f * ll) || !(fabsf(hv.w - gp.w) < 0.05f * gp.w)) continue;
           const float wk =
-              fmaxf(0.0f, 1.0f - 0.5f * fabsf(u - (float)i)) * fmaxf(0.0f, 1.0f - 0.5f * fabsf(v - (float)j));
+              fmaxf(0.0f, 1.0f - 0.8f * fabsf(u - (float)i)) * fmaxf(0.0f, 1.0f - 0.8f * fabsf(v - (float)j));
           acc = make_float3(acc.x + wk * hv.x, acc.y + wk * hv.y, acc.z + wk * hv.z);
           wsum += wk;
         }

@@ -21,6 +21,7 @@
 // an 8x8 sub-tile.  All lanes of a wave march the same cone index at the same
 // time over neighbouring surface points, so their texel fetches fall on the
 // same bricks and L2 lines.  Accumulation is FP32 for every storage type.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -113,17 +114,18 @@ __device__ __forceinline__ float4 operator*(float s, float4 a) {
   return make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
 }
 
-// Separable parts of brick_index: index(x,y,z) = ix(x) | iy(y) | iz(z).
-__device__ __forceinline__ uint32_t part_x(uint32_t x) { return spread3(x); }
-__device__ __forceinline__ uint32_t part_y(uint32_t y) { return spread3(y) << 1; }
-__device__ __forceinline__ uint32_t part_z(uint32_t z) { return spread3(z) << 2; }
+// Separable parts of brick_index: index(x,y,z) = ix(x) | iy(y) | iz(z), with
+// spread3 read from a table in LDS (sp, 1024 entries) instead of computed.
+__device__ __forceinline__ uint32_t part_x(const uint32_t* sp, uint32_t x) { return sp[x]; }
+__device__ __forceinline__ uint32_t part_y(const uint32_t* sp, uint32_t y) { return sp[y] << 1; }
+__device__ __forceinline__ uint32_t part_z(const uint32_t* sp, uint32_t z) { return sp[z] << 2; }
 
 struct Corners {
   uint32_t ix[2], iy[2], iz[2];
   float wx[2], wy[2], wz[2];  // zero weight for border texels
 };
 
-__device__ __forceinline__ void corners(float3 p, int s, Corners& c) {
+__device__ __forceinline__ void corners(const uint32_t* sp, float3 p, int s, Corners& c) {
   const float fs = (float)s;
   const float u = p.x * fs - 0.5f, v = p.y * fs - 0.5f, w = p.z * fs - 0.5f;
   const float fu = floorf(u), fv = floorf(v), fw = floorf(w);
@@ -137,30 +139,27 @@ __device__ __forceinline__ void corners(float3 p, int s, Corners& c) {
   c.wz[0] = (z0 >= 0 && z0 < s) ? 1.0f - aw : 0.0f;
   c.wz[1] = (z0 + 1 >= 0 && z0 + 1 < s) ? aw : 0.0f;
   const int m = s - 1;
-  c.ix[0] = part_x((uint32_t)min(max(x0, 0), m));
-  c.ix[1] = part_x((uint32_t)min(max(x0 + 1, 0), m));
-  c.iy[0] = part_y((uint32_t)min(max(y0, 0), m));
-  c.iy[1] = part_y((uint32_t)min(max(y0 + 1, 0), m));
-  c.iz[0] = part_z((uint32_t)min(max(z0, 0), m));
-  c.iz[1] = part_z((uint32_t)min(max(z0 + 1, 0), m));
+  c.ix[0] = part_x(sp, (uint32_t)min(max(x0, 0), m));
+  c.ix[1] = part_x(sp, (uint32_t)min(max(x0 + 1, 0), m));
+  c.iy[0] = part_y(sp, (uint32_t)min(max(y0, 0), m));
+  c.iy[1] = part_y(sp, (uint32_t)min(max(y0 + 1, 0), m));
+  c.iz[0] = part_z(sp, (uint32_t)min(max(z0, 0), m));
+  c.iz[1] = part_z(sp, (uint32_t)min(max(z0 + 1, 0), m));
 }
 
+// All eight corners are fetched (border corners have weight 0 and a clamped,
+// valid index): no per-corner branches, which cost more than the loads they
+// would save.
 template <int DT>
 __device__ __forceinline__ float4 trilinear(const char* base, const Corners& c) {
   float4 acc = f4(0.0f);
 #pragma unroll
   for (int dz = 0; dz < 2; ++dz) {
-    if (c.wz[dz] == 0.0f) continue;
 #pragma unroll
     for (int dy = 0; dy < 2; ++dy) {
       const float wyz = c.wy[dy] * c.wz[dz];
-      if (wyz == 0.0f) continue;
 #pragma unroll
-      for (int dx = 0; dx < 2; ++dx) {
-        const float wt = c.wx[dx] * wyz;
-        if (wt == 0.0f) continue;
-        acc = acc + wt * Tex<DT>::load(base, c.ix[dx] | c.iy[dy] | c.iz[dz]);
-      }
+      for (int dx = 0; dx < 2; ++dx) acc = acc + (c.wx[dx] * wyz) * Tex<DT>::load(base, c.ix[dx] | c.iy[dy] | c.iz[dz]);
     }
   }
   return acc;
@@ -168,7 +167,8 @@ __device__ __forceinline__ float4 trilinear(const char* base, const Corners& c) 
 
 // sample_voxel with its three textureLod calls folded together.
 template <int DT>
-__device__ __forceinline__ float4 sample_voxel(const TraceArgs& A, float3 p, float3 ad, int mx, int my, int mz,
+__device__ __forceinline__ float4 sample_voxel(const TraceArgs& A, const uint32_t* sp, float3 p, float3 ad, int mx,
+                                               int my, int mz,
                                                float lod) {
   const float lmax = (float)(A.levels - 1);
   lod = fminf(fmaxf(lod, 0.0f), lmax);
@@ -180,7 +180,7 @@ __device__ __forceinline__ float4 sample_voxel(const TraceArgs& A, float3 p, flo
     const int l = l0 + k;
     const float lw = (nl == 1) ? 1.0f : (k == 0 ? 1.0f - t : t);
     Corners c;
-    corners(p, A.res >> l, c);
+    corners(sp, p, A.res >> l, c);
     if (l == 0) {
       const float wsum = ad.x + ad.y + ad.z;
       acc = acc + (lw * wsum) * trilinear<DT>(A.vol + A.off[0][0], c);
@@ -207,7 +207,8 @@ __device__ __forceinline__ float len3(float3 v) { return sqrtf(dot3(v, v)); }
 
 // trace_cone (voxel_cone_tracing.frag:88-119)
 template <int DT>
-__device__ float4 trace_cone(const TraceArgs& A, float3 origin, float3 dir, float aperture, float max_dist,
+__device__ float4 trace_cone(const TraceArgs& A, const uint32_t* spt, float3 origin, float3 dir, float aperture,
+                             float max_dist,
                              uint32_t& steps) {
   dir = normalize3(dir);
   const int mx = dir.x < 0.0f ? 0 : 1, my = dir.y < 0.0f ? 2 : 3, mz = dir.z < 0.0f ? 4 : 5;
@@ -223,7 +224,7 @@ __device__ float4 trace_cone(const TraceArgs& A, float3 origin, float3 dir, floa
   while (c.w < 1.0f && dist < max_dist) {
     const float3 sp = make_float3(dir.x * dist + origin.x, dir.y * dist + origin.y, dir.z * dist + origin.z);
     const float mip = fmaxf(log2f(diam * fres), 0.0f);
-    const float4 s = sample_voxel<DT>(A, sp, ad, mx, my, mz, mip);
+    const float4 s = sample_voxel<DT>(A, spt, sp, ad, mx, my, mz, mip);
     c = c + (1.0f - c.w) * s;
     dist += fmaxf(diam * 0.5f, vs);
     ++steps;
@@ -256,47 +257,13 @@ __device__ __forceinline__ float3 refract3(float3 I, float3 N, float eta) {
 // original order and composes the pixel (voxel_cone_tracing.frag:246-275).
 constexpr int kMaxSlots = 18;  // 16 diffuse + specular + refraction
 
+// Shades the block's 64 pixels (lane = pixel, wave = cone group); returns
+// after the workgroup barrier of the composition.
 template <int DT>
-__global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
-  __shared__ float s_cone[kMaxSlots][3][64];
-  __shared__ float s_direct[3][64];
+__device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, size_t oidx, bool valid,
+                                            float (*s_cone)[3][64], float (*s_direct)[64],
+                                            const uint32_t* s_spread) {
   const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
-  int px, py;
-  size_t oidx;
-  bool valid = true;
-  if (A.pix_list) {
-    const uint32_t k = blockIdx.x * 64u + (uint32_t)lane;
-    valid = k < A.npix_list && !(A.pix_count && k >= *A.pix_count);
-    const uint32_t p = valid ? A.pix_list[k] : 0u;
-    px = (int)(p % (uint32_t)A.W);
-    py = (int)(p / (uint32_t)A.W);
-    oidx = A.pix_scatter ? (size_t)p : (size_t)k;
-    if (valid && A.tile_flag) valid = A.tile_flag[(py >> 5) * A.tiles_x + (px >> 5)] == A.flag_want;
-  } else {
-    int bx = blockIdx.x, by = blockIdx.y;
-    if (A.tile_list && A.grid_stride == 2) {
-      // 4 workgroups of 8x8 grid points per tile; grid point (u, v) is pixel (2u + 1, 2v + 1)
-      const uint32_t k = blockIdx.x >> 2;
-      if (k >= *A.tile_count) return;  // the whole workgroup leaves
-      const uint32_t t = A.tile_list[k];
-      const int gu = (int)(t % (uint32_t)A.tiles_x) * 16 + (int)(blockIdx.x & 1u) * 8 + (lane & 7);
-      const int gv = (int)(t / (uint32_t)A.tiles_x) * 16 + (int)((blockIdx.x >> 1) & 1u) * 8 + (lane >> 3);
-      px = 2 * gu + 1;
-      py = 2 * gv + 1;
-      oidx = (size_t)gv * A.half_w + gu;
-    } else {
-      if (A.tile_list) {
-        const uint32_t k = blockIdx.x >> 4;
-        if (k >= *A.tile_count) return;  // the whole workgroup leaves
-        const uint32_t t = A.tile_list[k];
-        bx = (int)(t % (uint32_t)A.tiles_x) * 4 + (int)(blockIdx.x & 3u);
-        by = (int)(t / (uint32_t)A.tiles_x) * 4 + (int)((blockIdx.x >> 2) & 3u);
-      }
-      px = bx * 8 + (lane & 7);
-      py = by * 8 + (lane >> 3);
-      oidx = (size_t)py * A.W + px;
-    }
-  }
   valid = valid && px < A.W && py < A.H;
   const size_t idx = valid ? (size_t)py * A.W + px : 0;
   const float4 gp = A.gpos[idx];
@@ -348,7 +315,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
         if (!do_refr) continue;
         d = refract3(view_dir, n, 1.0f / m.ior);  // trace_refraction (:237-241)
       }
-      const float4 r = trace_cone<DT>(A, pos, d, ap, sqrt3, steps);
+      const float4 r = trace_cone<DT>(A, s_spread, pos, d, ap, sqrt3, steps);
       s_cone[s][0][lane] = r.x;
       s_cone[s][1][lane] = r.y;
       s_cone[s][2][lane] = r.z;
@@ -377,7 +344,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
         const float3 lc = make_float3(k * L.color[0] * L.intensity, k * L.color[1] * L.intensity,
                                       k * L.color[2] * L.intensity);
         float shadow = 1.0f;
-        if (A.en_shadow) shadow = fmaxf(0.0f, 1.0f - trace_cone<DT>(A, pos, ldir, 0.1f, d, steps).w);
+        if (A.en_shadow) shadow = fmaxf(0.0f, 1.0f - trace_cone<DT>(A, s_spread, pos, ldir, 0.1f, d, steps).w);
         const float lambert = fmaxf(dot3(ldir, n), 0.0f);
         float refr_angle = 0.0f;
         if (m.dissolve <= 0.1f) {
@@ -412,7 +379,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
     if (d < 2) { ad.x = 1.0f; mx = d; }
     else if (d < 4) { ad.y = 1.0f; my = d; }
     else { ad.z = 1.0f; mz = d; }
-    const float4 s = sample_voxel<DT>(A, pos, ad, mx, my, mz, A.view_lod);
+    const float4 s = sample_voxel<DT>(A, s_spread, pos, ad, mx, my, mz, A.view_lod);
     const float a = s.w;
     A.image[oidx] = make_float4(s.x * a + bg.x * (1.0f - a), s.y * a + bg.y * (1.0f - a),
                                 s.z * a + bg.z * (1.0f - a), a * a + bg.w * (1.0f - a));
@@ -454,6 +421,77 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs A) {
     ow = gp.w;  // linear depth of the traced pixel
   }
   A.image[oidx] = make_float4(out.x, out.y, out.z, ow);
+}
+
+// Work of a launch: the whole viewport (trace_frame_kernel, one workgroup per
+// 8x8 block), or a device-side list of pixels (64 per chunk) or tiles (16
+// chunks of 8x8 pixels per tile, or 4 chunks of 8x8 odd-odd grid points) that
+// trace_list_kernel's fixed grid of workgroups takes in strides (no workgroups
+// launched for entries that are not there).
+template <int DT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void trace_frame_kernel(TraceArgs A) {
+  __shared__ float s_cone[kMaxSlots][3][64];
+  __shared__ float s_direct[3][64];
+  __shared__ uint32_t s_spread[1024];
+  const int lane = threadIdx.x & 63;
+  for (int q = threadIdx.x; q < 1024; q += 256) s_spread[q] = spread3((uint32_t)q);
+  __syncthreads();
+  const int px = blockIdx.x * 8 + (lane & 7), py = blockIdx.y * 8 + (lane >> 3);
+  shade_block<DT>(A, px, py, (size_t)py * A.W + px, true, s_cone, s_direct, s_spread);
+}
+
+template <int DT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void trace_list_kernel(TraceArgs A) {
+  __shared__ float s_cone[kMaxSlots][3][64];
+  __shared__ float s_direct[3][64];
+  __shared__ uint32_t s_spread[1024];
+  const int lane = threadIdx.x & 63;
+  for (int q = threadIdx.x; q < 1024; q += 256) s_spread[q] = spread3((uint32_t)q);
+  __syncthreads();
+  // chunks of a list launch
+  uint32_t n = 1, first = 0, stride = 1;
+  if (A.pix_list) {
+    n = ((A.pix_count ? min(*A.pix_count, A.npix_list) : A.npix_list) + 63u) / 64u;
+    first = blockIdx.x;
+    stride = gridDim.x;
+  } else if (A.tile_list) {
+    n = *A.tile_count * (A.grid_stride == 2 ? 4u : 16u);
+    first = blockIdx.x;
+    stride = gridDim.x;
+  }
+  for (uint32_t c = first; c < n; c += stride) {
+    int px, py;
+    size_t oidx;
+    bool valid = true;
+    if (A.pix_list) {
+      const uint32_t cnt = A.pix_count ? min(*A.pix_count, A.npix_list) : A.npix_list;
+      const uint32_t k = c * 64u + (uint32_t)lane;
+      valid = k < cnt;
+      const uint32_t p = valid ? A.pix_list[k] : 0u;
+      px = (int)(p % (uint32_t)A.W);
+      py = (int)(p / (uint32_t)A.W);
+      oidx = A.pix_scatter ? (size_t)p : (size_t)k;
+      if (valid && A.tile_flag) valid = A.tile_flag[(py >> 5) * A.tiles_x + (px >> 5)] == A.flag_want;
+    } else if (A.tile_list) {
+      const uint32_t per = A.grid_stride == 2 ? 4u : 16u;
+      const uint32_t t = A.tile_list[c / per], b = c % per;
+      if (A.grid_stride == 2) {  // grid point (u, v) is pixel (2u + 1, 2v + 1)
+        const int gu = (int)(t % (uint32_t)A.tiles_x) * 16 + (int)(b & 1u) * 8 + (lane & 7);
+        const int gv = (int)(t / (uint32_t)A.tiles_x) * 16 + (int)(b >> 1) * 8 + (lane >> 3);
+        px = 2 * gu + 1;
+        py = 2 * gv + 1;
+        oidx = (size_t)gv * A.half_w + gu;
+      } else {
+        px = (int)(t % (uint32_t)A.tiles_x) * 32 + (int)(b & 3u) * 8 + (lane & 7);
+        py = (int)(t / (uint32_t)A.tiles_x) * 32 + (int)(b >> 2) * 8 + (lane >> 3);
+        oidx = (size_t)py * A.W + px;
+      }
+    } else {
+      break;
+    }
+    shade_block<DT>(A, px, py, oidx, valid, s_cone, s_direct, s_spread);
+    __syncthreads();
+  }
 }
 
 }  // namespace vct
@@ -567,19 +605,35 @@ int vct_trace_select(vct_context* ctx, const float4* gpos, const float4* gnrm, f
   }
   A.lights.count = n;
   dim3 grid((W + 7) / 8, (H + 7) / 8);  // 8x8 pixels per workgroup
+  // lists: a fixed grid (4 workgroups of the list kernel are resident per CU
+  // at once, 256 CUs) strides over the chunks; never more workgroups than the
+  // largest list has chunks
+  const uint32_t kListGrid = 256 * 4;
   if (sel.pix_list) {
     if (sel.npix_list == 0) return VCT_OK;
-    grid = dim3((sel.npix_list + 63) / 64);
+    grid = dim3(std::min<uint32_t>((sel.npix_list + 63) / 64, kListGrid));
   } else if (sel.tile_list) {
     if (sel.max_tiles == 0) return VCT_OK;
-    // 16 blocks of 8x8 pixels per 32x32 tile, or 4 blocks of 8x8 grid points
-    grid = dim3((sel.grid_stride == 2 ? 4 : 16) * sel.max_tiles);
+    grid = dim3(std::min<uint32_t>((sel.grid_stride == 2 ? 4u : 16u) * sel.max_tiles, kListGrid));
   }
+  const bool list = sel.pix_list || sel.tile_list;
   switch (ctx->dtype) {
-    case VCT_RGBA8: hipLaunchKernelGGL(trace_kernel<VCT_RGBA8>, grid, dim3(256), 0, ctx->stream, A); break;
-    case VCT_FP16: hipLaunchKernelGGL(trace_kernel<VCT_FP16>, grid, dim3(256), 0, ctx->stream, A); break;
-    case VCT_BF16: hipLaunchKernelGGL(trace_kernel<VCT_BF16>, grid, dim3(256), 0, ctx->stream, A); break;
-    default: hipLaunchKernelGGL(trace_kernel<VCT_FP32>, grid, dim3(256), 0, ctx->stream, A); break;
+    case VCT_RGBA8:
+      if (list) hipLaunchKernelGGL(trace_list_kernel<VCT_RGBA8>, grid, dim3(256), 0, ctx->stream, A);
+      else hipLaunchKernelGGL(trace_frame_kernel<VCT_RGBA8>, grid, dim3(256), 0, ctx->stream, A);
+      break;
+    case VCT_FP16:
+      if (list) hipLaunchKernelGGL(trace_list_kernel<VCT_FP16>, grid, dim3(256), 0, ctx->stream, A);
+      else hipLaunchKernelGGL(trace_frame_kernel<VCT_FP16>, grid, dim3(256), 0, ctx->stream, A);
+      break;
+    case VCT_BF16:
+      if (list) hipLaunchKernelGGL(trace_list_kernel<VCT_BF16>, grid, dim3(256), 0, ctx->stream, A);
+      else hipLaunchKernelGGL(trace_frame_kernel<VCT_BF16>, grid, dim3(256), 0, ctx->stream, A);
+      break;
+    default:
+      if (list) hipLaunchKernelGGL(trace_list_kernel<VCT_FP32>, grid, dim3(256), 0, ctx->stream, A);
+      else hipLaunchKernelGGL(trace_frame_kernel<VCT_FP32>, grid, dim3(256), 0, ctx->stream, A);
+      break;
   }
   VCT_HIP(hipGetLastError());
   return VCT_OK;
