@@ -15,12 +15,14 @@
 // CLAMP_TO_BORDER with a zero border, LINEAR_MIPMAP_LINEAR between floor(lod)
 // and floor(lod) + 1, lod clamped to [0, L-1].  The three maps of one sample
 // share coordinates and weights, so corner addresses are computed once per
-// level; level 0 is shared by all maps, so it is fetched once per step.
+// level and all 24 corner loads of a level are issued before the first is
+// used; level 0 is shared by all maps, so it is fetched once per step.
 //
-// Work mapping: a 256-lane workgroup shades a 16x16 pixel tile; each wave is
-// an 8x8 sub-tile.  All lanes of a wave march the same cone index at the same
-// time over neighbouring surface points, so their texel fetches fall on the
-// same bricks and L2 lines.  Accumulation is FP32 for every storage type.
+// Work mapping: a workgroup of 4 waves shades an 8x8 pixel block (lane =
+// pixel); the cones of a pixel are dealt over the 4 waves, and all lanes of
+// a wave march the same cone at the same time over neighbouring surface
+// points, so their texel fetches fall on the same bricks and L2 lines.
+// Accumulation is FP32 for every storage type.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -72,12 +74,17 @@ struct TraceArgs {
   LightSet lights;
 };
 
+// Storage types: raw texel as loaded (Raw) and its decode to float4; the
+// decode folds into the blend (v_fma_mix_f32 for the 16-bit types).
 template <int DT>
 struct Tex;
 template <>
 struct Tex<VCT_RGBA8> {
-  static __device__ __forceinline__ float4 load(const char* base, uint32_t i) {
-    const uint32_t v = reinterpret_cast<const uint32_t*>(base)[i];
+  using Raw = uint32_t;
+  static __device__ __forceinline__ Raw load(const char* base, uint32_t i) {
+    return reinterpret_cast<const uint32_t*>(base)[i];
+  }
+  static __device__ __forceinline__ float4 decode(Raw v) {
     const float k = 1.0f / 255.0f;
     return make_float4((float)(v & 255u) * k, (float)((v >> 8) & 255u) * k, (float)((v >> 16) & 255u) * k,
                        (float)(v >> 24) * k);
@@ -85,25 +92,33 @@ struct Tex<VCT_RGBA8> {
 };
 template <>
 struct Tex<VCT_FP16> {
-  static __device__ __forceinline__ float4 load(const char* base, uint32_t i) {
-    const uint2 v = reinterpret_cast<const uint2*>(base)[i];
+  using Raw = uint2;
+  static __device__ __forceinline__ Raw load(const char* base, uint32_t i) {
+    return reinterpret_cast<const uint2*>(base)[i];
+  }
+  static __device__ __forceinline__ float4 decode(Raw v) {
     return make_float4(f16_to_f32(v.x & 0xffffu), f16_to_f32(v.x >> 16), f16_to_f32(v.y & 0xffffu),
                        f16_to_f32(v.y >> 16));
   }
 };
 template <>
 struct Tex<VCT_BF16> {
-  static __device__ __forceinline__ float4 load(const char* base, uint32_t i) {
-    const uint2 v = reinterpret_cast<const uint2*>(base)[i];
+  using Raw = uint2;
+  static __device__ __forceinline__ Raw load(const char* base, uint32_t i) {
+    return reinterpret_cast<const uint2*>(base)[i];
+  }
+  static __device__ __forceinline__ float4 decode(Raw v) {
     return make_float4(bf16_to_f32(v.x & 0xffffu), bf16_to_f32(v.x >> 16), bf16_to_f32(v.y & 0xffffu),
                        bf16_to_f32(v.y >> 16));
   }
 };
 template <>
 struct Tex<VCT_FP32> {
-  static __device__ __forceinline__ float4 load(const char* base, uint32_t i) {
+  using Raw = float4;
+  static __device__ __forceinline__ Raw load(const char* base, uint32_t i) {
     return reinterpret_cast<const float4*>(base)[i];
   }
+  static __device__ __forceinline__ float4 decode(Raw v) { return v; }
 };
 
 __device__ __forceinline__ float4 f4(float x) { return make_float4(x, x, x, x); }
@@ -114,84 +129,105 @@ __device__ __forceinline__ float4 operator*(float s, float4 a) {
   return make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
 }
 
-// Separable parts of brick_index: index(x,y,z) = ix(x) | iy(y) | iz(z), with
-// spread3 read from a table in LDS (sp, 1024 entries) instead of computed.
-__device__ __forceinline__ uint32_t part_x(const uint32_t* sp, uint32_t x) { return sp[x]; }
-__device__ __forceinline__ uint32_t part_y(const uint32_t* sp, uint32_t y) { return sp[y] << 1; }
-__device__ __forceinline__ uint32_t part_z(const uint32_t* sp, uint32_t z) { return sp[z] << 2; }
+// Per-workgroup sampler tables in LDS: spread3 (brick_index(x, y, z) =
+// spread[x] | spread[y] << 1 | spread[z] << 2) and the byte offset of every
+// (level, map) of the volume (a lane's level varies, so the table is indexed
+// per lane; LDS reads are far cheaper than per-lane reads of kernel arguments).
+struct SamplerLds {
+  uint32_t spread[1024];
+  uint64_t off[16][6];
+};
+
+__device__ __forceinline__ void sampler_init(const TraceArgs& A, SamplerLds* S) {
+  for (int q = threadIdx.x; q < 1024; q += blockDim.x) S->spread[q] = spread3((uint32_t)q);
+  for (int q = threadIdx.x; q < 16 * 6; q += blockDim.x) S->off[q / 6][q % 6] = A.off[q / 6][q % 6];
+}
 
 struct Corners {
   uint32_t ix[2], iy[2], iz[2];
-  float wx[2], wy[2], wz[2];  // zero weight for border texels
+  float w[8];  // corner weights (dx + 2 dy + 4 dz); zero for border texels
 };
 
-__device__ __forceinline__ void corners(const uint32_t* sp, float3 p, int s, Corners& c) {
+__device__ __forceinline__ void corners(const SamplerLds* S, float3 p, int s, Corners& c) {
   const float fs = (float)s;
   const float u = p.x * fs - 0.5f, v = p.y * fs - 0.5f, w = p.z * fs - 0.5f;
   const float fu = floorf(u), fv = floorf(v), fw = floorf(w);
   const float au = u - fu, av = v - fv, aw = w - fw;
   const int x0 = (int)fu, y0 = (int)fv, z0 = (int)fw;
   // Out-of-range texels take the zero border colour: weight 0, index clamped.
-  c.wx[0] = (x0 >= 0 && x0 < s) ? 1.0f - au : 0.0f;
-  c.wx[1] = (x0 + 1 >= 0 && x0 + 1 < s) ? au : 0.0f;
-  c.wy[0] = (y0 >= 0 && y0 < s) ? 1.0f - av : 0.0f;
-  c.wy[1] = (y0 + 1 >= 0 && y0 + 1 < s) ? av : 0.0f;
-  c.wz[0] = (z0 >= 0 && z0 < s) ? 1.0f - aw : 0.0f;
-  c.wz[1] = (z0 + 1 >= 0 && z0 + 1 < s) ? aw : 0.0f;
+  const float wx0 = (x0 >= 0 && x0 < s) ? 1.0f - au : 0.0f;
+  const float wx1 = (x0 + 1 >= 0 && x0 + 1 < s) ? au : 0.0f;
+  const float wy0 = (y0 >= 0 && y0 < s) ? 1.0f - av : 0.0f;
+  const float wy1 = (y0 + 1 >= 0 && y0 + 1 < s) ? av : 0.0f;
+  const float wz0 = (z0 >= 0 && z0 < s) ? 1.0f - aw : 0.0f;
+  const float wz1 = (z0 + 1 >= 0 && z0 + 1 < s) ? aw : 0.0f;
+  const float w00 = wy0 * wz0, w10 = wy1 * wz0, w01 = wy0 * wz1, w11 = wy1 * wz1;
+  c.w[0] = wx0 * w00;
+  c.w[1] = wx1 * w00;
+  c.w[2] = wx0 * w10;
+  c.w[3] = wx1 * w10;
+  c.w[4] = wx0 * w01;
+  c.w[5] = wx1 * w01;
+  c.w[6] = wx0 * w11;
+  c.w[7] = wx1 * w11;
   const int m = s - 1;
-  c.ix[0] = part_x(sp, (uint32_t)min(max(x0, 0), m));
-  c.ix[1] = part_x(sp, (uint32_t)min(max(x0 + 1, 0), m));
-  c.iy[0] = part_y(sp, (uint32_t)min(max(y0, 0), m));
-  c.iy[1] = part_y(sp, (uint32_t)min(max(y0 + 1, 0), m));
-  c.iz[0] = part_z(sp, (uint32_t)min(max(z0, 0), m));
-  c.iz[1] = part_z(sp, (uint32_t)min(max(z0 + 1, 0), m));
+  c.ix[0] = S->spread[min(max(x0, 0), m)];
+  c.ix[1] = S->spread[min(max(x0 + 1, 0), m)];
+  c.iy[0] = S->spread[min(max(y0, 0), m)] << 1;
+  c.iy[1] = S->spread[min(max(y0 + 1, 0), m)] << 1;
+  c.iz[0] = S->spread[min(max(z0, 0), m)] << 2;
+  c.iz[1] = S->spread[min(max(z0 + 1, 0), m)] << 2;
 }
 
-// All eight corners are fetched (border corners have weight 0 and a clamped,
-// valid index): no per-corner branches, which cost more than the loads they
-// would save.
+// The eight corners of one map, all loads issued before any is used (border
+// corners have weight 0 and a clamped, valid index: no per-corner branches).
 template <int DT>
-__device__ __forceinline__ float4 trilinear(const char* base, const Corners& c) {
+__device__ __forceinline__ void gather8(const char* base, const Corners& c, typename Tex<DT>::Raw (&r)[8]) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r[k] = Tex<DT>::load(base, c.ix[k & 1] | c.iy[(k >> 1) & 1] | c.iz[k >> 2]);
+}
+
+template <int DT>
+__device__ __forceinline__ float4 blend8(const Corners& c, const typename Tex<DT>::Raw (&r)[8], float scale) {
   float4 acc = f4(0.0f);
 #pragma unroll
-  for (int dz = 0; dz < 2; ++dz) {
-#pragma unroll
-    for (int dy = 0; dy < 2; ++dy) {
-      const float wyz = c.wy[dy] * c.wz[dz];
-#pragma unroll
-      for (int dx = 0; dx < 2; ++dx) acc = acc + (c.wx[dx] * wyz) * Tex<DT>::load(base, c.ix[dx] | c.iy[dy] | c.iz[dz]);
-    }
-  }
+  for (int k = 0; k < 8; ++k) acc = acc + (scale * c.w[k]) * Tex<DT>::decode(r[k]);
   return acc;
 }
 
-// sample_voxel with its three textureLod calls folded together.
+// One level of sample_voxel: level 0 is the one map shared by all directions
+// (|d.x| + |d.y| + |d.z| times one trilinear fetch), a higher level blends
+// the three directional maps mx, my, mz.
 template <int DT>
-__device__ __forceinline__ float4 sample_voxel(const TraceArgs& A, const uint32_t* sp, float3 p, float3 ad, int mx,
-                                               int my, int mz,
-                                               float lod) {
+__device__ __forceinline__ float4 sample_level(const TraceArgs& A, const SamplerLds* S, float3 p, float3 ad, int mx,
+                                               int my, int mz, int l) {
+  Corners c;
+  corners(S, p, A.res >> l, c);
+  using Raw = typename Tex<DT>::Raw;
+  if (l == 0) {
+    Raw r[8];
+    gather8<DT>(A.vol, c, r);
+    return blend8<DT>(c, r, ad.x + ad.y + ad.z);
+  }
+  Raw rx[8], ry[8], rz[8];
+  gather8<DT>(A.vol + S->off[l][mx], c, rx);
+  gather8<DT>(A.vol + S->off[l][my], c, ry);
+  gather8<DT>(A.vol + S->off[l][mz], c, rz);
+  return blend8<DT>(c, rx, ad.x) + blend8<DT>(c, ry, ad.y) + blend8<DT>(c, rz, ad.z);
+}
+
+// sample_voxel (voxel_cone_tracing.frag:80-86) with its three textureLod
+// calls folded together: lod clamped to [0, L-1], levels floor(lod) and
+// floor(lod) + 1 blended by the fraction (the second only when it is > 0).
+template <int DT>
+__device__ __forceinline__ float4 sample_voxel(const TraceArgs& A, const SamplerLds* S, float3 p, float3 ad, int mx,
+                                               int my, int mz, float lod) {
   const float lmax = (float)(A.levels - 1);
   lod = fminf(fmaxf(lod, 0.0f), lmax);
   const int l0 = (int)floorf(lod);
   const float t = lod - (float)l0;
-  float4 acc = f4(0.0f);
-  const int nl = (t > 0.0f && l0 + 1 < A.levels) ? 2 : 1;
-  for (int k = 0; k < nl; ++k) {
-    const int l = l0 + k;
-    const float lw = (nl == 1) ? 1.0f : (k == 0 ? 1.0f - t : t);
-    Corners c;
-    corners(sp, p, A.res >> l, c);
-    if (l == 0) {
-      const float wsum = ad.x + ad.y + ad.z;
-      acc = acc + (lw * wsum) * trilinear<DT>(A.vol + A.off[0][0], c);
-    } else {
-      float4 s = f4(0.0f);
-      if (ad.x != 0.0f) s = s + ad.x * trilinear<DT>(A.vol + A.off[l][mx], c);
-      if (ad.y != 0.0f) s = s + ad.y * trilinear<DT>(A.vol + A.off[l][my], c);
-      if (ad.z != 0.0f) s = s + ad.z * trilinear<DT>(A.vol + A.off[l][mz], c);
-      acc = acc + lw * s;
-    }
-  }
+  float4 acc = sample_level<DT>(A, S, p, ad, mx, my, mz, l0);
+  if (t > 0.0f) acc = (1.0f - t) * acc + t * sample_level<DT>(A, S, p, ad, mx, my, mz, l0 + 1);
   return acc;
 }
 
@@ -207,7 +243,7 @@ __device__ __forceinline__ float len3(float3 v) { return sqrtf(dot3(v, v)); }
 
 // trace_cone (voxel_cone_tracing.frag:88-119)
 template <int DT>
-__device__ float4 trace_cone(const TraceArgs& A, const uint32_t* spt, float3 origin, float3 dir, float aperture,
+__device__ float4 trace_cone(const TraceArgs& A, const SamplerLds* spt, float3 origin, float3 dir, float aperture,
                              float max_dist,
                              uint32_t& steps) {
   dir = normalize3(dir);
@@ -262,7 +298,7 @@ constexpr int kMaxSlots = 18;  // 16 diffuse + specular + refraction
 template <int DT>
 __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, size_t oidx, bool valid,
                                             float (*s_cone)[3][64], float (*s_direct)[64],
-                                            const uint32_t* s_spread) {
+                                            const SamplerLds* s_spread) {
   const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
   valid = valid && px < A.W && py < A.H;
   const size_t idx = valid ? (size_t)py * A.W + px : 0;
@@ -425,73 +461,73 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
 
 // Work of a launch: the whole viewport (trace_frame_kernel, one workgroup per
 // 8x8 block), or a device-side list of pixels (64 per chunk) or tiles (16
-// chunks of 8x8 pixels per tile, or 4 chunks of 8x8 odd-odd grid points) that
-// trace_list_kernel's fixed grid of workgroups takes in strides (no workgroups
-// launched for entries that are not there).
+// chunks of 8x8 pixels per tile, or 4 chunks of 8x8 odd-odd grid points) for
+// trace_list_kernel.  123 VGPRs at 4 waves per SIMD: the 24 loads of a
+// three-map level are in flight together.
 template <int DT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void trace_frame_kernel(TraceArgs A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void trace_frame_kernel(TraceArgs A) {
   __shared__ float s_cone[kMaxSlots][3][64];
   __shared__ float s_direct[3][64];
-  __shared__ uint32_t s_spread[1024];
+  __shared__ SamplerLds s_sampler;
   const int lane = threadIdx.x & 63;
-  for (int q = threadIdx.x; q < 1024; q += 256) s_spread[q] = spread3((uint32_t)q);
+  sampler_init(A, &s_sampler);
   __syncthreads();
   const int px = blockIdx.x * 8 + (lane & 7), py = blockIdx.y * 8 + (lane >> 3);
-  shade_block<DT>(A, px, py, (size_t)py * A.W + px, true, s_cone, s_direct, s_spread);
+  shade_block<DT>(A, px, py, (size_t)py * A.W + px, true, s_cone, s_direct, &s_sampler);
 }
 
+// Chunks (64 pixels each) of a list launch.
+__device__ __forceinline__ uint32_t list_chunks(const TraceArgs& A) {
+  if (A.pix_list) return ((A.pix_count ? min(*A.pix_count, A.npix_list) : A.npix_list) + 63u) / 64u;
+  return *A.tile_count * (A.grid_stride == 2 ? 4u : 16u);
+}
+
+// The pixel of `lane` in chunk c, where its result goes, and whether it is traced.
+__device__ __forceinline__ void list_chunk(const TraceArgs& A, uint32_t c, int lane, int& px, int& py, size_t& oidx,
+                                           bool& valid) {
+  valid = true;
+  if (A.pix_list) {
+    const uint32_t cnt = A.pix_count ? min(*A.pix_count, A.npix_list) : A.npix_list;
+    const uint32_t k = c * 64u + (uint32_t)lane;
+    valid = k < cnt;
+    const uint32_t p = valid ? A.pix_list[k] : 0u;
+    px = (int)(p % (uint32_t)A.W);
+    py = (int)(p / (uint32_t)A.W);
+    oidx = A.pix_scatter ? (size_t)p : (size_t)k;
+    if (valid && A.tile_flag) valid = A.tile_flag[(py >> 5) * A.tiles_x + (px >> 5)] == A.flag_want;
+    return;
+  }
+  const uint32_t per = A.grid_stride == 2 ? 4u : 16u;
+  const uint32_t t = A.tile_list[c / per], b = c % per;
+  if (A.grid_stride == 2) {  // grid point (u, v) is pixel (2u + 1, 2v + 1)
+    const int gu = (int)(t % (uint32_t)A.tiles_x) * 16 + (int)(b & 1u) * 8 + (lane & 7);
+    const int gv = (int)(t / (uint32_t)A.tiles_x) * 16 + (int)(b >> 1) * 8 + (lane >> 3);
+    px = 2 * gu + 1;
+    py = 2 * gv + 1;
+    oidx = (size_t)gv * A.half_w + gu;
+  } else {
+    px = (int)(t % (uint32_t)A.tiles_x) * 32 + (int)(b & 3u) * 8 + (lane & 7);
+    py = (int)(t / (uint32_t)A.tiles_x) * 32 + (int)(b >> 2) * 8 + (lane >> 3);
+    oidx = (size_t)py * A.W + px;
+  }
+}
+
+// One chunk per workgroup; workgroups past the device-side count leave at once.
+// (A fixed grid striding over the chunks measured slower: the loop costs the
+// kernel registers and spills at 4 waves per SIMD.)
 template <int DT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void trace_list_kernel(TraceArgs A) {
   __shared__ float s_cone[kMaxSlots][3][64];
   __shared__ float s_direct[3][64];
-  __shared__ uint32_t s_spread[1024];
-  const int lane = threadIdx.x & 63;
-  for (int q = threadIdx.x; q < 1024; q += 256) s_spread[q] = spread3((uint32_t)q);
+  __shared__ SamplerLds s_sampler;
+  if (blockIdx.x >= list_chunks(A)) return;
+  sampler_init(A, &s_sampler);
   __syncthreads();
-  // chunks of a list launch
-  uint32_t n = 1, first = 0, stride = 1;
-  if (A.pix_list) {
-    n = ((A.pix_count ? min(*A.pix_count, A.npix_list) : A.npix_list) + 63u) / 64u;
-    first = blockIdx.x;
-    stride = gridDim.x;
-  } else if (A.tile_list) {
-    n = *A.tile_count * (A.grid_stride == 2 ? 4u : 16u);
-    first = blockIdx.x;
-    stride = gridDim.x;
-  }
-  for (uint32_t c = first; c < n; c += stride) {
-    int px, py;
-    size_t oidx;
-    bool valid = true;
-    if (A.pix_list) {
-      const uint32_t cnt = A.pix_count ? min(*A.pix_count, A.npix_list) : A.npix_list;
-      const uint32_t k = c * 64u + (uint32_t)lane;
-      valid = k < cnt;
-      const uint32_t p = valid ? A.pix_list[k] : 0u;
-      px = (int)(p % (uint32_t)A.W);
-      py = (int)(p / (uint32_t)A.W);
-      oidx = A.pix_scatter ? (size_t)p : (size_t)k;
-      if (valid && A.tile_flag) valid = A.tile_flag[(py >> 5) * A.tiles_x + (px >> 5)] == A.flag_want;
-    } else if (A.tile_list) {
-      const uint32_t per = A.grid_stride == 2 ? 4u : 16u;
-      const uint32_t t = A.tile_list[c / per], b = c % per;
-      if (A.grid_stride == 2) {  // grid point (u, v) is pixel (2u + 1, 2v + 1)
-        const int gu = (int)(t % (uint32_t)A.tiles_x) * 16 + (int)(b & 1u) * 8 + (lane & 7);
-        const int gv = (int)(t / (uint32_t)A.tiles_x) * 16 + (int)(b >> 1) * 8 + (lane >> 3);
-        px = 2 * gu + 1;
-        py = 2 * gv + 1;
-        oidx = (size_t)gv * A.half_w + gu;
-      } else {
-        px = (int)(t % (uint32_t)A.tiles_x) * 32 + (int)(b & 3u) * 8 + (lane & 7);
-        py = (int)(t / (uint32_t)A.tiles_x) * 32 + (int)(b >> 2) * 8 + (lane >> 3);
-        oidx = (size_t)py * A.W + px;
-      }
-    } else {
-      break;
-    }
-    shade_block<DT>(A, px, py, oidx, valid, s_cone, s_direct, s_spread);
-    __syncthreads();
-  }
+  int px, py;
+  size_t oidx;
+  bool valid;
+  list_chunk(A, blockIdx.x, threadIdx.x & 63, px, py, oidx, valid);
+  shade_block<DT>(A, px, py, oidx, valid, s_cone, s_direct, &s_sampler);
 }
 
 }  // namespace vct
@@ -605,33 +641,30 @@ int vct_trace_select(vct_context* ctx, const float4* gpos, const float4* gnrm, f
   }
   A.lights.count = n;
   dim3 grid((W + 7) / 8, (H + 7) / 8);  // 8x8 pixels per workgroup
-  // lists: a fixed grid (4 workgroups of the list kernel are resident per CU
-  // at once, 256 CUs) strides over the chunks; never more workgroups than the
-  // largest list has chunks
-  const uint32_t kListGrid = 256 * 4;
+  // lists: one workgroup per chunk of the largest the list can be
   if (sel.pix_list) {
     if (sel.npix_list == 0) return VCT_OK;
-    grid = dim3(std::min<uint32_t>((sel.npix_list + 63) / 64, kListGrid));
+    grid = dim3((sel.npix_list + 63) / 64);
   } else if (sel.tile_list) {
     if (sel.max_tiles == 0) return VCT_OK;
-    grid = dim3(std::min<uint32_t>((sel.grid_stride == 2 ? 4u : 16u) * sel.max_tiles, kListGrid));
+    grid = dim3((sel.grid_stride == 2 ? 4u : 16u) * sel.max_tiles);
   }
   const bool list = sel.pix_list || sel.tile_list;
   switch (ctx->dtype) {
     case VCT_RGBA8:
-      if (list) hipLaunchKernelGGL(trace_list_kernel<VCT_RGBA8>, grid, dim3(256), 0, ctx->stream, A);
+      if (list) hipLaunchKernelGGL((trace_list_kernel<VCT_RGBA8>), grid, dim3(256), 0, ctx->stream, A);
       else hipLaunchKernelGGL(trace_frame_kernel<VCT_RGBA8>, grid, dim3(256), 0, ctx->stream, A);
       break;
     case VCT_FP16:
-      if (list) hipLaunchKernelGGL(trace_list_kernel<VCT_FP16>, grid, dim3(256), 0, ctx->stream, A);
+      if (list) hipLaunchKernelGGL((trace_list_kernel<VCT_FP16>), grid, dim3(256), 0, ctx->stream, A);
       else hipLaunchKernelGGL(trace_frame_kernel<VCT_FP16>, grid, dim3(256), 0, ctx->stream, A);
       break;
     case VCT_BF16:
-      if (list) hipLaunchKernelGGL(trace_list_kernel<VCT_BF16>, grid, dim3(256), 0, ctx->stream, A);
+      if (list) hipLaunchKernelGGL((trace_list_kernel<VCT_BF16>), grid, dim3(256), 0, ctx->stream, A);
       else hipLaunchKernelGGL(trace_frame_kernel<VCT_BF16>, grid, dim3(256), 0, ctx->stream, A);
       break;
     default:
-      if (list) hipLaunchKernelGGL(trace_list_kernel<VCT_FP32>, grid, dim3(256), 0, ctx->stream, A);
+      if (list) hipLaunchKernelGGL((trace_list_kernel<VCT_FP32>), grid, dim3(256), 0, ctx->stream, A);
       else hipLaunchKernelGGL(trace_frame_kernel<VCT_FP32>, grid, dim3(256), 0, ctx->stream, A);
       break;
   }
