@@ -164,23 +164,26 @@ int vct_launch_scan_u32(vct_context* ctx, const uint32_t* in, uint32_t* out, uin
 }
 
 // ---- volume ------------------------------------------------------------------
-static int alloc_volume(vct_context* ctx) {
-  const uint32_t bpe = vct::dtype_bytes(ctx->dtype);
-  uint64_t off = 0;
-  for (int l = 0; l < ctx->levels; ++l) {
-    uint32_t s = ctx->res >> l;
-    ctx->lvl_size[l] = s;
-    uint64_t bytes = vct::level_elems(s) * bpe;
-    if (l == 0) {
-      for (int d = 0; d < 6; ++d) ctx->lvl_off[0][d] = 0;
-      off += (bytes + 255) & ~255ull;
-    } else {
-      for (int d = 0; d < 6; ++d) {
-        ctx->lvl_off[l][d] = off;
-        off += (bytes + 255) & ~255ull;
-      }
+// Byte layout of the volume: level 0 (one map shared by all directions),
+// then 6 directional maps per level, each 256-byte aligned.  Returns the
+// total size; fills off[l][d] when it is given.
+static uint64_t volume_layout(unsigned res, int levels, int dtype, uint64_t (*off)[6]) {
+  const uint32_t bpe = vct::dtype_bytes(dtype);
+  uint64_t total = 0;
+  for (int l = 0; l < levels; ++l) {
+    const uint64_t bytes = (vct::level_elems(res >> l) * bpe + 255) & ~255ull;
+    for (int d = 0; d < 6; ++d) {
+      if (off) off[l][d] = total;
+      if (l > 0) total += bytes;
     }
+    if (l == 0) total += bytes;
   }
+  return total;
+}
+
+static int alloc_volume(vct_context* ctx) {
+  for (int l = 0; l < ctx->levels; ++l) ctx->lvl_size[l] = ctx->res >> l;
+  const uint64_t off = volume_layout(ctx->res, ctx->levels, ctx->dtype, ctx->lvl_off);
   int rc = vct_ensure(ctx->volume, off, ctx);
   if (rc) return rc;
   VCT_HIP(hipMemsetAsync(ctx->volume.p, 0, off, ctx->stream));
@@ -200,6 +203,10 @@ static int validate_grid(unsigned res, int levels, int dtype, int average) {
     return vct_set_error(VCT_ERR_INVALID_ARG, "bad averaging mode");
   if (average == VCT_AVG_REFERENCE && dtype != VCT_RGBA8)
     return vct_set_error(VCT_ERR_INVALID_ARG, "the reference running average needs RGBA8 storage");
+  // the cone marcher addresses texels by 32-bit byte offsets (512^3 with FP32
+  // storage, 3.9 GiB, is the largest grid of that type that fits)
+  if (volume_layout(res, levels, dtype, nullptr) > 0xffffffffull)
+    return vct_set_error(VCT_ERR_INVALID_ARG, "radiance volume above 4 GiB (use a smaller grid or 16-bit storage)");
   return VCT_OK;
 }
 

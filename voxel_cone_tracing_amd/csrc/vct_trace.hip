@@ -81,9 +81,6 @@ struct Tex;
 template <>
 struct Tex<VCT_RGBA8> {
   using Raw = uint32_t;
-  static __device__ __forceinline__ Raw load(const char* base, uint32_t i) {
-    return reinterpret_cast<const uint32_t*>(base)[i];
-  }
   static __device__ __forceinline__ float4 decode(Raw v) {
     const float k = 1.0f / 255.0f;
     return make_float4((float)(v & 255u) * k, (float)((v >> 8) & 255u) * k, (float)((v >> 16) & 255u) * k,
@@ -93,9 +90,6 @@ struct Tex<VCT_RGBA8> {
 template <>
 struct Tex<VCT_FP16> {
   using Raw = uint2;
-  static __device__ __forceinline__ Raw load(const char* base, uint32_t i) {
-    return reinterpret_cast<const uint2*>(base)[i];
-  }
   static __device__ __forceinline__ float4 decode(Raw v) {
     return make_float4(f16_to_f32(v.x & 0xffffu), f16_to_f32(v.x >> 16), f16_to_f32(v.y & 0xffffu),
                        f16_to_f32(v.y >> 16));
@@ -104,9 +98,6 @@ struct Tex<VCT_FP16> {
 template <>
 struct Tex<VCT_BF16> {
   using Raw = uint2;
-  static __device__ __forceinline__ Raw load(const char* base, uint32_t i) {
-    return reinterpret_cast<const uint2*>(base)[i];
-  }
   static __device__ __forceinline__ float4 decode(Raw v) {
     return make_float4(bf16_to_f32(v.x & 0xffffu), bf16_to_f32(v.x >> 16), bf16_to_f32(v.y & 0xffffu),
                        bf16_to_f32(v.y >> 16));
@@ -115,9 +106,6 @@ struct Tex<VCT_BF16> {
 template <>
 struct Tex<VCT_FP32> {
   using Raw = float4;
-  static __device__ __forceinline__ Raw load(const char* base, uint32_t i) {
-    return reinterpret_cast<const float4*>(base)[i];
-  }
   static __device__ __forceinline__ float4 decode(Raw v) { return v; }
 };
 
@@ -132,23 +120,33 @@ __device__ __forceinline__ float4 operator*(float s, float4 a) {
 // Per-workgroup sampler tables in LDS: spread3 (brick_index(x, y, z) =
 // spread[x] | spread[y] << 1 | spread[z] << 2) and the byte offset of every
 // (level, map) of the volume (a lane's level varies, so the table is indexed
-// per lane; LDS reads are far cheaper than per-lane reads of kernel arguments).
+// per lane; LDS reads are far cheaper than per-lane reads of kernel
+// arguments).  Offsets are 32-bit: the volume stays below 4 GiB (checked
+// when it is allocated), so every texel load is the uniform volume pointer
+// plus one 32-bit lane offset.
 struct SamplerLds {
   uint32_t spread[1024];
-  uint64_t off[16][6];
+  uint32_t off[16][6];
 };
 
 __device__ __forceinline__ void sampler_init(const TraceArgs& A, SamplerLds* S) {
   for (int q = threadIdx.x; q < 1024; q += blockDim.x) S->spread[q] = spread3((uint32_t)q);
-  for (int q = threadIdx.x; q < 16 * 6; q += blockDim.x) S->off[q / 6][q % 6] = A.off[q / 6][q % 6];
+  for (int q = threadIdx.x; q < 16 * 6; q += blockDim.x) S->off[q / 6][q % 6] = (uint32_t)A.off[q / 6][q % 6];
 }
 
+template <int DT>
+constexpr int kTexShift = sizeof(typename Tex<DT>::Raw) == 16 ? 4 : (sizeof(typename Tex<DT>::Raw) == 8 ? 3 : 2);
+
+// Corner byte offsets within a map: spread parts have disjoint bits, so
+// brick_index(x, y, z) << shift = ix[dx] + iyz[dy + 2 dz].
 struct Corners {
-  uint32_t ix[2], iy[2], iz[2];
+  uint32_t ix[2], iyz[4];
   float w[8];  // corner weights (dx + 2 dy + 4 dz); zero for border texels
 };
 
+template <int DT>
 __device__ __forceinline__ void corners(const SamplerLds* S, float3 p, int s, Corners& c) {
+  constexpr int sh = kTexShift<DT>;
   const float fs = (float)s;
   const float u = p.x * fs - 0.5f, v = p.y * fs - 0.5f, w = p.z * fs - 0.5f;
   const float fu = floorf(u), fv = floorf(v), fw = floorf(w);
@@ -171,27 +169,34 @@ __device__ __forceinline__ void corners(const SamplerLds* S, float3 p, int s, Co
   c.w[6] = wx0 * w11;
   c.w[7] = wx1 * w11;
   const int m = s - 1;
-  c.ix[0] = S->spread[min(max(x0, 0), m)];
-  c.ix[1] = S->spread[min(max(x0 + 1, 0), m)];
-  c.iy[0] = S->spread[min(max(y0, 0), m)] << 1;
-  c.iy[1] = S->spread[min(max(y0 + 1, 0), m)] << 1;
-  c.iz[0] = S->spread[min(max(z0, 0), m)] << 2;
-  c.iz[1] = S->spread[min(max(z0 + 1, 0), m)] << 2;
+  c.ix[0] = S->spread[min(max(x0, 0), m)] << sh;
+  c.ix[1] = S->spread[min(max(x0 + 1, 0), m)] << sh;
+  const uint32_t y0p = S->spread[min(max(y0, 0), m)] << (sh + 1);
+  const uint32_t y1p = S->spread[min(max(y0 + 1, 0), m)] << (sh + 1);
+  const uint32_t z0p = S->spread[min(max(z0, 0), m)] << (sh + 2);
+  const uint32_t z1p = S->spread[min(max(z0 + 1, 0), m)] << (sh + 2);
+  c.iyz[0] = y0p + z0p;
+  c.iyz[1] = y1p + z0p;
+  c.iyz[2] = y0p + z1p;
+  c.iyz[3] = y1p + z1p;
 }
 
-// The eight corners of one map, all loads issued before any is used (border
-// corners have weight 0 and a clamped, valid index: no per-corner branches).
+// The eight corners of one map (byte offset `base`), all loads issued before
+// any is used (border corners have weight 0 and a clamped, valid index: no
+// per-corner branches).
 template <int DT>
-__device__ __forceinline__ void gather8(const char* base, const Corners& c, typename Tex<DT>::Raw (&r)[8]) {
+__device__ __forceinline__ void gather8(const char* __restrict__ vol, uint32_t base, const Corners& c,
+                                        typename Tex<DT>::Raw (&r)[8]) {
+  using Raw = typename Tex<DT>::Raw;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) r[k] = Tex<DT>::load(base, c.ix[k & 1] | c.iy[(k >> 1) & 1] | c.iz[k >> 2]);
+  for (int k = 0; k < 8; ++k) r[k] = *reinterpret_cast<const Raw*>(vol + (base + c.ix[k & 1] + c.iyz[k >> 1]));
 }
 
 template <int DT>
-__device__ __forceinline__ float4 blend8(const Corners& c, const typename Tex<DT>::Raw (&r)[8], float scale) {
+__device__ __forceinline__ float4 blend8(const Corners& c, const typename Tex<DT>::Raw (&r)[8]) {
   float4 acc = f4(0.0f);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) acc = acc + (scale * c.w[k]) * Tex<DT>::decode(r[k]);
+  for (int k = 0; k < 8; ++k) acc = acc + c.w[k] * Tex<DT>::decode(r[k]);
   return acc;
 }
 
@@ -202,18 +207,18 @@ template <int DT>
 __device__ __forceinline__ float4 sample_level(const TraceArgs& A, const SamplerLds* S, float3 p, float3 ad, int mx,
                                                int my, int mz, int l) {
   Corners c;
-  corners(S, p, A.res >> l, c);
+  corners<DT>(S, p, A.res >> l, c);
   using Raw = typename Tex<DT>::Raw;
   if (l == 0) {
     Raw r[8];
-    gather8<DT>(A.vol, c, r);
-    return blend8<DT>(c, r, ad.x + ad.y + ad.z);
+    gather8<DT>(A.vol, 0u, c, r);
+    return (ad.x + ad.y + ad.z) * blend8<DT>(c, r);
   }
   Raw rx[8], ry[8], rz[8];
-  gather8<DT>(A.vol + S->off[l][mx], c, rx);
-  gather8<DT>(A.vol + S->off[l][my], c, ry);
-  gather8<DT>(A.vol + S->off[l][mz], c, rz);
-  return blend8<DT>(c, rx, ad.x) + blend8<DT>(c, ry, ad.y) + blend8<DT>(c, rz, ad.z);
+  gather8<DT>(A.vol, S->off[l][mx], c, rx);
+  gather8<DT>(A.vol, S->off[l][my], c, ry);
+  gather8<DT>(A.vol, S->off[l][mz], c, rz);
+  return ad.x * blend8<DT>(c, rx) + ad.y * blend8<DT>(c, ry) + ad.z * blend8<DT>(c, rz);
 }
 
 // sample_voxel (voxel_cone_tracing.frag:80-86) with its three textureLod
@@ -284,22 +289,32 @@ __device__ __forceinline__ float3 refract3(float3 I, float3 N, float eta) {
   return make_float3(eta * I.x - s * N.x, eta * I.y - s * N.y, eta * I.z - s * N.z);
 }
 
-// One workgroup = the 8x8 pixels of a block (lane = pixel) x 4 cone groups
-// (wave = group).  The cone slots of a pixel (diffuse 0..nd-1, specular nd,
-// refraction nd+1) are dealt round-robin to the groups, so every wave still
-// marches the same cone for 64 neighbouring pixels (coherent gathers) while a
-// pixel's cones run in 4 waves at once; group 3 also does the direct light
-// and its shadow cones.  Group 0 then sums the diffuse cones in their
+// One workgroup = the 8x8 pixels of a block (lane = pixel) x 4 waves.  The
+// jobs of a pixel - direct light with its shadow cones, the specular cone,
+// the refraction cone, then the diffuse cones - are taken by the waves from a
+// workgroup counter in LDS, longest first, so the four waves finish close
+// together; every wave still marches one cone for 64 neighbouring pixels at a
+// time (coherent gathers).  Wave 0 then sums the diffuse cones in their
 // original order and composes the pixel (voxel_cone_tracing.frag:246-275).
 constexpr int kMaxSlots = 18;  // 16 diffuse + specular + refraction
+constexpr int kJobDirect = 0, kJobSpecular = 1, kJobRefraction = 2, kJobDiffuse = 3;
 
-// Shades the block's 64 pixels (lane = pixel, wave = cone group); returns
-// after the workgroup barrier of the composition.
+// LDS of one workgroup of the cone marcher.
+struct ShadeLds {
+  float cone[kMaxSlots][3][64];  // per slot (diffuse 0..nd-1, specular nd, refraction nd+1) and pixel
+  float direct[3][64];
+  int next_job;
+};
+
+// Shades the block's 64 pixels (lane = pixel, wave = job taker); returns
+// after the workgroup barrier of the composition.  next_job must be 0 and
+// visible to the workgroup on entry.
 template <int DT>
 __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, size_t oidx, bool valid,
-                                            float (*s_cone)[3][64], float (*s_direct)[64],
-                                            const SamplerLds* s_spread) {
+                                            ShadeLds* L, const SamplerLds* s_spread) {
   const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  float (*s_cone)[3][64] = L->cone;
+  float (*s_direct)[64] = L->direct;
   valid = valid && px < A.W && py < A.H;
   const size_t idx = valid ? (size_t)py * A.W + px : 0;
   const float4 gp = A.gpos[idx];
@@ -328,77 +343,89 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
   const int nd = A.en_diffuse ? A.cones.n : 0;
   const bool do_spec = A.en_specular != 0;
   const bool do_refr = do_spec && illum_transmissive(m.illum);
-  if (mode == 3) {
-    float3 o1 = make_float3(0.f, 0.f, 0.f), o2 = o1;
-    if (nd > 0) {
-      o1 = normalize3(tangent_of(n));  // trace_diffuse (:146-147)
-      o2 = normalize3(cross3(o1, n));
-    }
-    for (int s = grp; s < nd + 2; s += 4) {
-      float3 d;
-      float ap = spec_ap;
-      if (s < nd) {
-        const float a = A.cones.a[s], b = A.cones.b[s], cc = A.cones.c[s];
-        d = make_float3(a * n.x + b * o1.x + cc * o2.x, a * n.y + b * o1.y + cc * o2.y, a * n.z + b * o1.z + cc * o2.z);
-        ap = A.cones.aperture;
-      } else if (s == nd) {
-        if (!do_spec) continue;
-        // normalize(reflect(-view_dir, n)) (:231), normalised again in trace_cone (:90)
-        const float3 I = make_float3(-view_dir.x, -view_dir.y, -view_dir.z);
-        const float dn = 2.0f * dot3(n, I);
-        d = normalize3(make_float3(I.x - dn * n.x, I.y - dn * n.y, I.z - dn * n.z));
-      } else {
-        if (!do_refr) continue;
-        d = refract3(view_dir, n, 1.0f / m.ior);  // trace_refraction (:237-241)
-      }
-      const float4 r = trace_cone<DT>(A, s_spread, pos, d, ap, sqrt3, steps);
-      s_cone[s][0][lane] = r.x;
-      s_cone[s][1][lane] = r.y;
-      s_cone[s][2][lane] = r.z;
-    }
-    if (grp == 3 && A.en_direct) {
+  float3 o1 = make_float3(0.f, 0.f, 0.f), o2 = o1;
+  if (mode == 3 && nd > 0) {
+    o1 = normalize3(tangent_of(n));  // trace_diffuse (:146-147)
+    o2 = normalize3(cross3(o1, n));
+  }
+  const int njobs = kJobDiffuse + nd;
+  for (;;) {
+    int job = 0;
+    if (lane == 0) job = atomicAdd(&L->next_job, 1);
+    job = __builtin_amdgcn_readfirstlane(job);  // one job per wave
+    if (job >= njobs) break;
+    if (job == kJobDirect) {
+      if (!A.en_direct) continue;
       float3 res = make_float3(0.f, 0.f, 0.f);
-      const int nl = A.lights.count < VCT_MAX_LIGHTS ? A.lights.count : VCT_MAX_LIGHTS;
-      for (int li = 0; li < nl; ++li) {
-        const DevLight& L = A.lights.l[li];
-        float3 ldir;
-        float d;
-        if (L.kind == VCT_LIGHT_DIRECTIONAL) {
-          ldir = normalize3(make_float3(L.pos[0], L.pos[1], L.pos[2]));
-          d = sqrt3;
-        } else {
-          const float3 lp = make_float3(0.5f * (L.pos[0] / A.cube_size) + 0.5f,
-                                        0.5f * (L.pos[1] / A.cube_size) + 0.5f,
-                                        0.5f * (L.pos[2] / A.cube_size) + 0.5f);
-          ldir = make_float3(lp.x - pos.x, lp.y - pos.y, lp.z - pos.z);
-          d = len3(ldir);
-          ldir = normalize3(ldir);
+      if (mode == 3) {
+        const int nl = A.lights.count < VCT_MAX_LIGHTS ? A.lights.count : VCT_MAX_LIGHTS;
+        for (int li = 0; li < nl; ++li) {
+          const DevLight& Lt = A.lights.l[li];
+          float3 ldir;
+          float d;
+          if (Lt.kind == VCT_LIGHT_DIRECTIONAL) {
+            ldir = normalize3(make_float3(Lt.pos[0], Lt.pos[1], Lt.pos[2]));
+            d = sqrt3;
+          } else {
+            const float3 lp = make_float3(0.5f * (Lt.pos[0] / A.cube_size) + 0.5f,
+                                          0.5f * (Lt.pos[1] / A.cube_size) + 0.5f,
+                                          0.5f * (Lt.pos[2] / A.cube_size) + 0.5f);
+            ldir = make_float3(lp.x - pos.x, lp.y - pos.y, lp.z - pos.z);
+            d = len3(ldir);
+            ldir = normalize3(ldir);
+          }
+          const float cos_surf = fmaxf(dot3(n, ldir), 0.0f);
+          const float att = (Lt.kind == VCT_LIGHT_DIRECTIONAL) ? 1.0f : 1.0f / (1.0f + d * d);
+          const float k = att * cos_surf;
+          const float3 lc = make_float3(k * Lt.color[0] * Lt.intensity, k * Lt.color[1] * Lt.intensity,
+                                        k * Lt.color[2] * Lt.intensity);
+          float shadow = 1.0f;
+          if (A.en_shadow) shadow = fmaxf(0.0f, 1.0f - trace_cone<DT>(A, s_spread, pos, ldir, 0.1f, d, steps).w);
+          const float lambert = fmaxf(dot3(ldir, n), 0.0f);
+          float refr_angle = 0.0f;
+          if (m.dissolve <= 0.1f) {
+            const float3 r = refract3(view_dir, n, 1.0f / m.ior);
+            refr_angle = fmaxf((1.0f - m.dissolve) * dot3(r, ldir), 0.0f);
+          }
+          const float3 hv = normalize3(make_float3(ldir.x + view_dir.x, ldir.y + view_dir.y, ldir.z + view_dir.z));
+          float sa = fmaxf(dot3(hv, n), 0.0f);
+          sa = fmaxf(sa, refr_angle);
+          const float sc = powf(sa, m.shininess);
+          const float f = shadow + 0.04f;
+          res.x += f * (lambert * m.diffuse[0] + sc * m.specular[0]) * lc.x;
+          res.y += f * (lambert * m.diffuse[1] + sc * m.specular[1]) * lc.y;
+          res.z += f * (lambert * m.diffuse[2] + sc * m.specular[2]) * lc.z;
         }
-        const float cos_surf = fmaxf(dot3(n, ldir), 0.0f);
-        const float att = (L.kind == VCT_LIGHT_DIRECTIONAL) ? 1.0f : 1.0f / (1.0f + d * d);
-        const float k = att * cos_surf;
-        const float3 lc = make_float3(k * L.color[0] * L.intensity, k * L.color[1] * L.intensity,
-                                      k * L.color[2] * L.intensity);
-        float shadow = 1.0f;
-        if (A.en_shadow) shadow = fmaxf(0.0f, 1.0f - trace_cone<DT>(A, s_spread, pos, ldir, 0.1f, d, steps).w);
-        const float lambert = fmaxf(dot3(ldir, n), 0.0f);
-        float refr_angle = 0.0f;
-        if (m.dissolve <= 0.1f) {
-          const float3 r = refract3(view_dir, n, 1.0f / m.ior);
-          refr_angle = fmaxf((1.0f - m.dissolve) * dot3(r, ldir), 0.0f);
-        }
-        const float3 hv = normalize3(make_float3(ldir.x + view_dir.x, ldir.y + view_dir.y, ldir.z + view_dir.z));
-        float sa = fmaxf(dot3(hv, n), 0.0f);
-        sa = fmaxf(sa, refr_angle);
-        const float sc = powf(sa, m.shininess);
-        const float f = shadow + 0.04f;
-        res.x += f * (lambert * m.diffuse[0] + sc * m.specular[0]) * lc.x;
-        res.y += f * (lambert * m.diffuse[1] + sc * m.specular[1]) * lc.y;
-        res.z += f * (lambert * m.diffuse[2] + sc * m.specular[2]) * lc.z;
       }
       s_direct[0][lane] = res.x + fminf(fmaxf(m.emission[0], 0.0f), 1.0f);
       s_direct[1][lane] = res.y + fminf(fmaxf(m.emission[1], 0.0f), 1.0f);
       s_direct[2][lane] = res.z + fminf(fmaxf(m.emission[2], 0.0f), 1.0f);
+      continue;
+    }
+    int slot;
+    float3 d;
+    float ap = spec_ap;
+    if (job >= kJobDiffuse) {
+      slot = job - kJobDiffuse;
+      const float a = A.cones.a[slot], b = A.cones.b[slot], cc = A.cones.c[slot];
+      d = make_float3(a * n.x + b * o1.x + cc * o2.x, a * n.y + b * o1.y + cc * o2.y, a * n.z + b * o1.z + cc * o2.z);
+      ap = A.cones.aperture;
+    } else if (job == kJobSpecular) {
+      if (!do_spec) continue;
+      slot = nd;
+      // normalize(reflect(-view_dir, n)) (:231), normalised again in trace_cone (:90)
+      const float3 I = make_float3(-view_dir.x, -view_dir.y, -view_dir.z);
+      const float dn = 2.0f * dot3(n, I);
+      d = normalize3(make_float3(I.x - dn * n.x, I.y - dn * n.y, I.z - dn * n.z));
+    } else {
+      slot = nd + 1;
+      d = refract3(view_dir, n, 1.0f / m.ior);  // trace_refraction (:237-241)
+    }
+    if (mode == 3 && (job != kJobRefraction || do_refr)) {
+      const float4 r = trace_cone<DT>(A, s_spread, pos, d, ap, sqrt3, steps);
+      s_cone[slot][0][lane] = r.x;
+      s_cone[slot][1][lane] = r.y;
+      s_cone[slot][2][lane] = r.z;
     }
   }
   if (A.step_counter && steps) atomicAdd(A.step_counter, (unsigned long long)steps);
@@ -466,14 +493,14 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
 // three-map level are in flight together.
 template <int DT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void trace_frame_kernel(TraceArgs A) {
-  __shared__ float s_cone[kMaxSlots][3][64];
-  __shared__ float s_direct[3][64];
+  __shared__ ShadeLds s_shade;
   __shared__ SamplerLds s_sampler;
   const int lane = threadIdx.x & 63;
   sampler_init(A, &s_sampler);
+  if (threadIdx.x == 0) s_shade.next_job = 0;
   __syncthreads();
   const int px = blockIdx.x * 8 + (lane & 7), py = blockIdx.y * 8 + (lane >> 3);
-  shade_block<DT>(A, px, py, (size_t)py * A.W + px, true, s_cone, s_direct, &s_sampler);
+  shade_block<DT>(A, px, py, (size_t)py * A.W + px, true, &s_shade, &s_sampler);
 }
 
 // Chunks (64 pixels each) of a list launch.
@@ -517,17 +544,17 @@ __device__ __forceinline__ void list_chunk(const TraceArgs& A, uint32_t c, int l
 // kernel registers and spills at 4 waves per SIMD.)
 template <int DT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void trace_list_kernel(TraceArgs A) {
-  __shared__ float s_cone[kMaxSlots][3][64];
-  __shared__ float s_direct[3][64];
+  __shared__ ShadeLds s_shade;
   __shared__ SamplerLds s_sampler;
   if (blockIdx.x >= list_chunks(A)) return;
   sampler_init(A, &s_sampler);
+  if (threadIdx.x == 0) s_shade.next_job = 0;
   __syncthreads();
   int px, py;
   size_t oidx;
   bool valid;
   list_chunk(A, blockIdx.x, threadIdx.x & 63, px, py, oidx, valid);
-  shade_block<DT>(A, px, py, oidx, valid, s_cone, s_direct, &s_sampler);
+  shade_block<DT>(A, px, py, oidx, valid, &s_shade, &s_sampler);
 }
 
 }  // namespace vct
