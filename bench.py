@@ -208,7 +208,7 @@ def main():
     ap.add_argument("--plain-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
+        print("bench.py runs on one GPU only: the voxel cone tracing core has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
 

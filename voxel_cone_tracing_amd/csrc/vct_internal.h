@@ -203,6 +203,11 @@ struct vct_context {
   bool volume_valid = false;
   uint64_t lvl_off[16][6];  // byte offsets
   uint32_t lvl_size[16];
+  uint64_t volume_version = 1;  // advanced by every write of the volume
+  // x-paired copy of the volume for the cone marcher (16-bit and RGBA8 storage)
+  VctBuf pairs;
+  uint64_t pair_off[16][6];
+  uint64_t pairs_version = 0;  // volume_version the copy was built from
 
   // voxelize scratch
   VctBuf tri_count, tri_off, tri_base;  // per-triangle candidate counts / offsets

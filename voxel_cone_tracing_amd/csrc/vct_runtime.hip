@@ -184,6 +184,7 @@ static uint64_t volume_layout(unsigned res, int levels, int dtype, uint64_t (*of
 static int alloc_volume(vct_context* ctx) {
   for (int l = 0; l < ctx->levels; ++l) ctx->lvl_size[l] = ctx->res >> l;
   const uint64_t off = volume_layout(ctx->res, ctx->levels, ctx->dtype, ctx->lvl_off);
+  ++ctx->volume_version;
   int rc = vct_ensure(ctx->volume, off, ctx);
   if (rc) return rc;
   VCT_HIP(hipMemsetAsync(ctx->volume.p, 0, off, ctx->stream));
@@ -204,8 +205,10 @@ static int validate_grid(unsigned res, int levels, int dtype, int average) {
   if (average == VCT_AVG_REFERENCE && dtype != VCT_RGBA8)
     return vct_set_error(VCT_ERR_INVALID_ARG, "the reference running average needs RGBA8 storage");
   // the cone marcher addresses texels by 32-bit byte offsets (512^3 with FP32
-  // storage, 3.9 GiB, is the largest grid of that type that fits)
-  if (volume_layout(res, levels, dtype, nullptr) > 0xffffffffull)
+  // or with 16-bit storage, whose x-paired copy is twice the volume, is the
+  // largest grid that fits)
+  const uint64_t paired = dtype == VCT_FP32 ? 1 : 2;  // the cone marcher's x-paired copy
+  if (paired * volume_layout(res, levels, dtype, nullptr) > 0xffffffffull)
     return vct_set_error(VCT_ERR_INVALID_ARG, "radiance volume above 4 GiB (use a smaller grid or 16-bit storage)");
   return VCT_OK;
 }
@@ -347,7 +350,7 @@ int vct_destroy(vct_ctx ctx) {
                     &ctx->full_target, &ctx->sample_pix, &ctx->train_X, &ctx->train_A, &ctx->train_Y,
                     &ctx->train_valid, &ctx->probe_pix, &ctx->probe_val, &ctx->fb_list, &ctx->fb_pix,
                     &ctx->fb_list2, &ctx->fb_pix2, &ctx->fb_flag, &ctx->half_img, &ctx->probe_pix2,
-                    &ctx->probe_val2};
+                    &ctx->probe_val2, &ctx->pairs};
   for (VctBuf* b : bufs) free_buf(*b);
   for (int i = 0; i < 12; ++i)
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
@@ -634,11 +637,13 @@ int vct_voxelize(vct_ctx ctx) {
   CTX_CHECK(ctx);
   int rc = sync_materials(ctx);
   if (rc) return rc;
+  ++ctx->volume_version;
   return timed(ctx, 0, vct_launch_voxelize);
 }
 
 int vct_mip_aniso(vct_ctx ctx) {
   CTX_CHECK(ctx);
+  ++ctx->volume_version;
   return timed(ctx, 1, vct_launch_mip);
 }
 
@@ -711,6 +716,7 @@ int vct_volume_upload(vct_ctx ctx, int level, int dir, const void* host, size_t 
   size_t dev_bytes = vct::level_elems(s) * bpe;
   std::vector<char> tmp(dev_bytes, 0);
   to_brick((const char*)host, tmp.data(), s, bpe);
+  ++ctx->volume_version;
   VCT_HIP(hipMemcpyAsync((char*)ctx->volume.p + ctx->lvl_off[level][level == 0 ? 0 : dir], tmp.data(), dev_bytes,
                          hipMemcpyHostToDevice, ctx->stream));
   VCT_HIP(hipStreamSynchronize(ctx->stream));

@@ -109,6 +109,44 @@ struct Tex<VCT_FP32> {
   static __device__ __forceinline__ float4 decode(Raw v) { return v; }
 };
 
+// x-paired copy of the volume read by the marcher: entry i of a map holds
+// texel i and its +x neighbour (zero past the last column), so one load
+// returns both x-corners of a trilinear fetch - half the load instructions,
+// which the texture address units otherwise bound.  16-bit storage pairs
+// channel by channel (four words of (c(x), c(x+1))), RGBA8 as two words;
+// FP32 texels are already 16 bytes and are not paired.
+template <int DT>
+struct Pair {
+  static constexpr bool kOn = false;
+};
+template <>
+struct Pair<VCT_RGBA8> {
+  static constexpr bool kOn = true;
+  using Raw = uint2;
+  static __device__ __forceinline__ Raw make(uint32_t a, uint32_t b) { return make_uint2(a, b); }
+  static __device__ __forceinline__ float4 first(Raw v) { return Tex<VCT_RGBA8>::decode(v.x); }
+  static __device__ __forceinline__ float4 second(Raw v) { return Tex<VCT_RGBA8>::decode(v.y); }
+};
+template <int DT>
+struct Pair16 {
+  static constexpr bool kOn = true;
+  using Raw = uint4;
+  static __device__ __forceinline__ Raw make(uint2 a, uint2 b) {
+    return make_uint4((a.x & 0xffffu) | (b.x << 16), (a.x >> 16) | (b.x & 0xffff0000u), (a.y & 0xffffu) | (b.y << 16),
+                      (a.y >> 16) | (b.y & 0xffff0000u));
+  }
+  static __device__ __forceinline__ float4 first(Raw v) {
+    return Tex<DT>::decode(make_uint2((v.x & 0xffffu) | (v.y << 16), (v.z & 0xffffu) | (v.w << 16)));
+  }
+  static __device__ __forceinline__ float4 second(Raw v) {
+    return Tex<DT>::decode(make_uint2((v.x >> 16) | (v.y & 0xffff0000u), (v.z >> 16) | (v.w & 0xffff0000u)));
+  }
+};
+template <>
+struct Pair<VCT_FP16> : Pair16<VCT_FP16> {};
+template <>
+struct Pair<VCT_BF16> : Pair16<VCT_BF16> {};
+
 __device__ __forceinline__ float4 f4(float x) { return make_float4(x, x, x, x); }
 __device__ __forceinline__ float4 operator+(float4 a, float4 b) {
   return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
@@ -135,10 +173,28 @@ __device__ __forceinline__ void sampler_init(const TraceArgs& A, SamplerLds* S) 
 }
 
 template <int DT>
-constexpr int kTexShift = sizeof(typename Tex<DT>::Raw) == 16 ? 4 : (sizeof(typename Tex<DT>::Raw) == 8 ? 3 : 2);
+struct Fetch {  // the unit one texel load returns: a pair where the volume is paired
+  using Raw = typename Tex<DT>::Raw;
+};
+template <>
+struct Fetch<VCT_RGBA8> {
+  using Raw = Pair<VCT_RGBA8>::Raw;
+};
+template <>
+struct Fetch<VCT_FP16> {
+  using Raw = Pair<VCT_FP16>::Raw;
+};
+template <>
+struct Fetch<VCT_BF16> {
+  using Raw = Pair<VCT_BF16>::Raw;
+};
+template <int DT>
+constexpr int kTexShift = sizeof(typename Fetch<DT>::Raw) == 16 ? 4 : (sizeof(typename Fetch<DT>::Raw) == 8 ? 3 : 2);
 
 // Corner byte offsets within a map: spread parts have disjoint bits, so
-// brick_index(x, y, z) << shift = ix[dx] + iyz[dy + 2 dz].
+// brick_index(x, y, z) << shift = ix[dx] + iyz[dy + 2 dz].  Paired volumes
+// use ix[0] only: the entry of the clamped x0 holds both x-corners, weighted
+// w[2k] (first texel) and w[2k + 1] (second) for the (y, z) corner k.
 struct Corners {
   uint32_t ix[2], iyz[4];
   float w[8];  // corner weights (dx + 2 dy + 4 dz); zero for border texels
@@ -153,8 +209,12 @@ __device__ __forceinline__ void corners(const SamplerLds* S, float3 p, int s, Co
   const float au = u - fu, av = v - fv, aw = w - fw;
   const int x0 = (int)fu, y0 = (int)fv, z0 = (int)fw;
   // Out-of-range texels take the zero border colour: weight 0, index clamped.
-  const float wx0 = (x0 >= 0 && x0 < s) ? 1.0f - au : 0.0f;
-  const float wx1 = (x0 + 1 >= 0 && x0 + 1 < s) ? au : 0.0f;
+  float wx0 = (x0 >= 0 && x0 < s) ? 1.0f - au : 0.0f;
+  float wx1 = (x0 + 1 >= 0 && x0 + 1 < s) ? au : 0.0f;
+  if (Pair<DT>::kOn && x0 == -1) {  // the pair at column 0 holds x1 = 0 first
+    wx0 = au;
+    wx1 = 0.0f;
+  }
   const float wy0 = (y0 >= 0 && y0 < s) ? 1.0f - av : 0.0f;
   const float wy1 = (y0 + 1 >= 0 && y0 + 1 < s) ? av : 0.0f;
   const float wz0 = (z0 >= 0 && z0 < s) ? 1.0f - aw : 0.0f;
@@ -193,6 +253,23 @@ __device__ __forceinline__ void gather8(const char* __restrict__ vol, uint32_t b
 }
 
 template <int DT>
+__device__ __forceinline__ void gather4(const char* __restrict__ vol, uint32_t base, const Corners& c,
+                                        typename Pair<DT>::Raw (&r)[4]) {
+  using Raw = typename Pair<DT>::Raw;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r[k] = *reinterpret_cast<const Raw*>(vol + (base + c.ix[0] + c.iyz[k]));
+}
+
+template <int DT>
+__device__ __forceinline__ float4 blend4(const Corners& c, const typename Pair<DT>::Raw (&r)[4]) {
+  float4 acc = f4(0.0f);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    acc = acc + c.w[2 * k] * Pair<DT>::first(r[k]) + c.w[2 * k + 1] * Pair<DT>::second(r[k]);
+  return acc;
+}
+
+template <int DT>
 __device__ __forceinline__ float4 blend8(const Corners& c, const typename Tex<DT>::Raw (&r)[8]) {
   float4 acc = f4(0.0f);
 #pragma unroll
@@ -208,6 +285,19 @@ __device__ __forceinline__ float4 sample_level(const TraceArgs& A, const Sampler
                                                int my, int mz, int l) {
   Corners c;
   corners<DT>(S, p, A.res >> l, c);
+  if constexpr (Pair<DT>::kOn) {
+    using PRaw = typename Pair<DT>::Raw;
+    if (l == 0) {
+      PRaw r[4];
+      gather4<DT>(A.vol, 0u, c, r);
+      return (ad.x + ad.y + ad.z) * blend4<DT>(c, r);
+    }
+    PRaw rx[4], ry[4], rz[4];
+    gather4<DT>(A.vol, S->off[l][mx], c, rx);
+    gather4<DT>(A.vol, S->off[l][my], c, ry);
+    gather4<DT>(A.vol, S->off[l][mz], c, rz);
+    return ad.x * blend4<DT>(c, rx) + ad.y * blend4<DT>(c, ry) + ad.z * blend4<DT>(c, rz);
+  }
   using Raw = typename Tex<DT>::Raw;
   if (l == 0) {
     Raw r[8];
@@ -557,6 +647,50 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void t
   shade_block<DT>(A, px, py, oidx, valid, &s_shade, &s_sampler);
 }
 
+// Builds the x-paired copy of every (level, map) of the volume in one launch:
+// thread g covers texel g of the concatenated maps (level 0, then the 6 maps
+// of each higher level).  The +x neighbour of Morton index m is the Morton
+// increment of its x bits; the last column pairs with zero.
+struct PairArgs {
+  const char* src;
+  char* dst;
+  uint64_t src_off[16][6], dst_off[16][6];
+  int levels;
+  uint32_t res;
+  uint64_t total;  // texels over all maps
+};
+
+template <int DT>
+__global__ __launch_bounds__(256) void pair_kernel(PairArgs P) {
+  using T = typename Tex<DT>::Raw;
+  using PR = typename Pair<DT>::Raw;
+  for (uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x; g < P.total; g += (uint64_t)gridDim.x * 256u) {
+    uint64_t r = g;
+    int l = 0, d = 0;
+    uint32_t s = P.res;
+    uint64_t n = (uint64_t)s * s * s;
+    if (r >= n) {
+      r -= n;
+      for (l = 1; l < P.levels; ++l) {
+        s = P.res >> l;
+        n = (uint64_t)s * s * s;
+        if (r < 6 * n) break;
+        r -= 6 * n;
+      }
+      d = (int)(r / n);
+      r -= (uint64_t)d * n;
+    }
+    const uint32_t m = (uint32_t)r;
+    const uint32_t mx = spread3(s - 1u);  // x bits of an index within this level
+    const T* src = reinterpret_cast<const T*>(P.src + P.src_off[l][d]);
+    const T a = src[m];
+    T b;
+    memset(&b, 0, sizeof(b));
+    if ((m & mx) != mx) b = src[(((m | ~mx) + 1u) & mx) | (m & ~mx)];
+    reinterpret_cast<PR*>(P.dst + P.dst_off[l][d])[m] = Pair<DT>::make(a, b);
+  }
+}
+
 }  // namespace vct
 
 using namespace vct;
@@ -608,6 +742,45 @@ void vct_cone_set_table(int set, ConeSet& cs) {
   }
 }
 
+// Rebuilds the x-paired copy of the volume when the volume has changed since
+// it was last built (16-bit and RGBA8 storage).
+static int ensure_pairs(vct_context* ctx) {
+  if (ctx->dtype == VCT_FP32 || ctx->pairs_version == ctx->volume_version) return VCT_OK;
+  PairArgs P;
+  memset(&P, 0, sizeof(P));
+  const uint64_t bpe = dtype_bytes(ctx->dtype);
+  uint64_t bytes = 0, total = 0;
+  for (int l = 0; l < ctx->levels; ++l) {
+    const uint32_t s = ctx->res >> l;
+    const uint64_t n = level_elems(s);
+    const uint64_t map_bytes = (n * 2 * bpe + 255) & ~255ull;
+    for (int d = 0; d < 6; ++d) {
+      P.src_off[l][d] = ctx->lvl_off[l][d];
+      P.dst_off[l][d] = bytes;
+      ctx->pair_off[l][d] = bytes;
+      if (l > 0) bytes += map_bytes;
+    }
+    if (l == 0) bytes += map_bytes;
+    total += (l == 0 ? 1 : 6) * n;
+  }
+  int rc;
+  if ((rc = vct_ensure(ctx->pairs, bytes, ctx))) return rc;
+  P.src = (const char*)ctx->volume.p;
+  P.dst = (char*)ctx->pairs.p;
+  P.levels = ctx->levels;
+  P.res = ctx->res;
+  P.total = total;
+  const dim3 grid((unsigned)std::min<uint64_t>((total + 255) / 256, 256ull * 64));
+  switch (ctx->dtype) {
+    case VCT_RGBA8: hipLaunchKernelGGL(pair_kernel<VCT_RGBA8>, grid, dim3(256), 0, ctx->stream, P); break;
+    case VCT_FP16: hipLaunchKernelGGL(pair_kernel<VCT_FP16>, grid, dim3(256), 0, ctx->stream, P); break;
+    default: hipLaunchKernelGGL(pair_kernel<VCT_BF16>, grid, dim3(256), 0, ctx->stream, P); break;
+  }
+  VCT_HIP(hipGetLastError());
+  ctx->pairs_version = ctx->volume_version;
+  return VCT_OK;
+}
+
 int vct_trace_into(vct_context* ctx, const float4* gpos, const float4* gnrm, float4* out, int W, int H,
                    int demodulate, unsigned long long* step_counter) {
   TraceSelect sel;
@@ -631,9 +804,12 @@ int vct_trace_select(vct_context* ctx, const float4* gpos, const float4* gnrm, f
   A.flag_want = sel.flag_want;
   A.grid_stride = sel.grid_stride;
   A.half_w = sel.half_w;
-  A.vol = (const char*)ctx->volume.p;
+  int rc;
+  if ((rc = ensure_pairs(ctx))) return rc;
+  const bool paired = ctx->dtype != VCT_FP32;
+  A.vol = (const char*)(paired ? ctx->pairs.p : ctx->volume.p);
   for (int l = 0; l < ctx->levels; ++l)
-    for (int d = 0; d < 6; ++d) A.off[l][d] = ctx->lvl_off[l][d];
+    for (int d = 0; d < 6; ++d) A.off[l][d] = paired ? ctx->pair_off[l][d] : ctx->lvl_off[l][d];
   A.levels = ctx->levels;
   A.res = (int)ctx->res;
   A.gpos = gpos;
