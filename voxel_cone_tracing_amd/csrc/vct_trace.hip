@@ -260,12 +260,64 @@ __device__ __forceinline__ void gather4(const char* __restrict__ vol, uint32_t b
   for (int k = 0; k < 4; ++k) r[k] = *reinterpret_cast<const Raw*>(vol + (base + c.ix[0] + c.iyz[k]));
 }
 
+// Weights of the four (y, z) corner pairs, as the packed 16-bit operand of a
+// two-term dot product for 16-bit storage (rounded to nearest in the storage
+// type: the filter weights carry the precision of the stored texels).
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf162_t __attribute__((ext_vector_type(2)));
+
 template <int DT>
-__device__ __forceinline__ float4 blend4(const Corners& c, const typename Pair<DT>::Raw (&r)[4]) {
-  float4 acc = f4(0.0f);
+struct PairWeights {
+  float w[8];
+  __device__ __forceinline__ explicit PairWeights(const Corners& c) {
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
-    acc = acc + c.w[2 * k] * Pair<DT>::first(r[k]) + c.w[2 * k + 1] * Pair<DT>::second(r[k]);
+    for (int k = 0; k < 8; ++k) w[k] = c.w[k];
+  }
+};
+template <>
+struct PairWeights<VCT_FP16> {
+  half2_t w[4];
+  __device__ __forceinline__ explicit PairWeights(const Corners& c) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = half2_t{(_Float16)c.w[2 * k], (_Float16)c.w[2 * k + 1]};
+  }
+};
+template <>
+struct PairWeights<VCT_BF16> {
+  bf162_t w[4];
+  __device__ __forceinline__ explicit PairWeights(const Corners& c) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = bf162_t{(__bf16)c.w[2 * k], (__bf16)c.w[2 * k + 1]};
+  }
+};
+
+// Blend of the eight corners of one paired map: per channel, one FP32
+// two-term dot product (v_dot2_f32_f16 / _bf16) per corner pair for 16-bit
+// storage; RGBA8 decodes both texels and accumulates in FP32.
+template <int DT>
+__device__ __forceinline__ float4 blend4(const PairWeights<DT>& W, const typename Pair<DT>::Raw (&r)[4]) {
+  float4 acc = f4(0.0f);
+  if constexpr (DT == VCT_FP16) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc.x = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, r[k].x), W.w[k], acc.x, false);
+      acc.y = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, r[k].y), W.w[k], acc.y, false);
+      acc.z = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, r[k].z), W.w[k], acc.z, false);
+      acc.w = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, r[k].w), W.w[k], acc.w, false);
+    }
+  } else if constexpr (DT == VCT_BF16) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc.x = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf162_t, r[k].x), W.w[k], acc.x, false);
+      acc.y = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf162_t, r[k].y), W.w[k], acc.y, false);
+      acc.z = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf162_t, r[k].z), W.w[k], acc.z, false);
+      acc.w = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf162_t, r[k].w), W.w[k], acc.w, false);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      acc = acc + W.w[2 * k] * Pair<DT>::first(r[k]) + W.w[2 * k + 1] * Pair<DT>::second(r[k]);
+  }
   return acc;
 }
 
@@ -287,16 +339,17 @@ __device__ __forceinline__ float4 sample_level(const TraceArgs& A, const Sampler
   corners<DT>(S, p, A.res >> l, c);
   if constexpr (Pair<DT>::kOn) {
     using PRaw = typename Pair<DT>::Raw;
+    const PairWeights<DT> W(c);
     if (l == 0) {
       PRaw r[4];
       gather4<DT>(A.vol, 0u, c, r);
-      return (ad.x + ad.y + ad.z) * blend4<DT>(c, r);
+      return (ad.x + ad.y + ad.z) * blend4<DT>(W, r);
     }
     PRaw rx[4], ry[4], rz[4];
     gather4<DT>(A.vol, S->off[l][mx], c, rx);
     gather4<DT>(A.vol, S->off[l][my], c, ry);
     gather4<DT>(A.vol, S->off[l][mz], c, rz);
-    return ad.x * blend4<DT>(c, rx) + ad.y * blend4<DT>(c, ry) + ad.z * blend4<DT>(c, rz);
+    return ad.x * blend4<DT>(W, rx) + ad.y * blend4<DT>(W, ry) + ad.z * blend4<DT>(W, rz);
   }
   using Raw = typename Tex<DT>::Raw;
   if (l == 0) {
@@ -304,11 +357,14 @@ __device__ __forceinline__ float4 sample_level(const TraceArgs& A, const Sampler
     gather8<DT>(A.vol, 0u, c, r);
     return (ad.x + ad.y + ad.z) * blend8<DT>(c, r);
   }
-  Raw rx[8], ry[8], rz[8];
-  gather8<DT>(A.vol, S->off[l][mx], c, rx);
-  gather8<DT>(A.vol, S->off[l][my], c, ry);
-  gather8<DT>(A.vol, S->off[l][mz], c, rz);
-  return ad.x * blend8<DT>(c, rx) + ad.y * blend8<DT>(c, ry) + ad.z * blend8<DT>(c, rz);
+  // 16-byte FP32 texels: one map in flight at a time (24 would spill)
+  Raw r[8];
+  gather8<DT>(A.vol, S->off[l][mx], c, r);
+  float4 acc = ad.x * blend8<DT>(c, r);
+  gather8<DT>(A.vol, S->off[l][my], c, r);
+  acc = acc + ad.y * blend8<DT>(c, r);
+  gather8<DT>(A.vol, S->off[l][mz], c, r);
+  return acc + ad.z * blend8<DT>(c, r);
 }
 
 // sample_voxel (voxel_cone_tracing.frag:80-86) with its three textureLod
