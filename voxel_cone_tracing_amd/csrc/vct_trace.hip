@@ -405,11 +405,19 @@ __device__ float4 trace_cone(const TraceArgs& A, const SamplerLds* spt, float3 o
   // A cone never needs to leave the volume by more than its diagonal; the
   // bound also keeps a non-finite target distance from looping forever.
   max_dist = fminf(max_dist, 3.4641016f);
+  // A sample more than half a texel of the coarsest level outside the unit
+  // cube reads only the zero border; once the cone is that far out and moving
+  // away on that axis, every later sample is zero and c no longer changes.
+  const float margin = 0.5f * (float)(1u << (A.levels - 1)) * vs;
   float4 c = f4(0.0f);
   float dist = 3.0f * vs;
   float diam = dist * aperture;
   while (c.w < 1.0f && dist < max_dist) {
     const float3 sp = make_float3(dir.x * dist + origin.x, dir.y * dist + origin.y, dir.z * dist + origin.z);
+    if ((sp.x < -margin && dir.x <= 0.0f) || (sp.x >= 1.0f + margin && dir.x >= 0.0f) ||
+        (sp.y < -margin && dir.y <= 0.0f) || (sp.y >= 1.0f + margin && dir.y >= 0.0f) ||
+        (sp.z < -margin && dir.z <= 0.0f) || (sp.z >= 1.0f + margin && dir.z >= 0.0f))
+      break;
     const float mip = fmaxf(log2f(diam * fres), 0.0f);
     const float4 s = sample_voxel<DT>(A, spt, sp, ad, mx, my, mz, mip);
     c = c + (1.0f - c.w) * s;
