@@ -292,19 +292,23 @@ int vct_upsample_train(vct_ctx ctx, uint32_t steps, uint32_t batch, float lr, ui
  * G-buffer, quarter-resolution cone trace, MLP upsampling (surface-guided:
  * a pixel with no quarter-resolution sample of its own surface nearby is
  * traced at full resolution instead), then the per-tile fallback: every
- * 32x32 tile is compared with a full-resolution trace at 16 probe pixels and
- * traced again in full when its estimated PSNR is below the bound.  Pass
- * times [voxelize, mip, gbuffer, low-res trace, upsample, fallback]. */
+ * 32x32 tile is compared with a full-resolution trace at 16 probe pixels,
+ * which estimates its error, and the tiles with the largest estimated errors
+ * are traced again in full until the error left in the others is within 30%
+ * of the frame's budget for the bound (with the half-resolution refinement on,
+ * every tile whose own estimate is below the bound is refined instead).
+ * Pass times [voxelize, mip, gbuffer, low-res trace, upsample, fallback]. */
 int vct_render_learned(vct_ctx ctx);
-/* Minimum estimated tile PSNR in dB (peak 1, over the tile's covered probe
- * pixels, RGB); default 40; <= 0 turns the fallback off. */
+/* The PSNR bound in dB (peak 1, RGB over covered pixels, against the plain
+ * full-resolution frame); default 40; <= 0 turns the fallback off. */
 int vct_upsample_set_fallback(vct_ctx ctx, float min_tile_psnr_db);
 /* Tiles of the viewport and tiles traced in full by the last learned frame. */
 int vct_upsample_fallback_stats(vct_ctx ctx, uint32_t* tiles, uint32_t* fallback_tiles);
-/* Refinement of a tile that fails the check: 1 (default) first traces the
- * tile's odd-odd pixels (half resolution), interpolates from them with the
- * same surface guidance and checks again at 16 even-even probes, tracing the
- * tile in full only if it fails again; 0 traces every failing tile in full. */
+/* 0 (default): the frame budget above, picked tiles traced in full.  1:
+ * every tile whose estimated PSNR is below the bound first has its odd-odd
+ * pixels traced (half resolution) and interpolated from with the same
+ * surface guidance, is checked again at 16 even-even probes, and is traced in
+ * full only if it fails again. */
 int vct_upsample_set_refine(vct_ctx ctx, int half_resolution);
 /* Counters of the last learned frame: [tiles, tiles refined at half
  * resolution (or traced in full when that stage is off), tiles traced in

@@ -234,7 +234,8 @@ struct vct_context {
   VctBuf fb_pix;                          // [count][pixels...] traced at full resolution
   VctBuf fb_list2, fb_pix2, fb_flag;      // tiles refined again, pixels listed by the half-res pass, tile states
   VctBuf half_img, probe_pix2, probe_val2;
-  bool fb_half = true;                    // refine failing tiles at half resolution before full
+  VctBuf fb_est;                          // per-tile error estimate + covered probes
+  bool fb_half = false;                   // per-tile bound + half-resolution stage instead of the frame budget
   bool fb_half_ran = false;
   int probe_w = 0, probe_h = 0;
   float fb_psnr_db = 40.0f;  // minimum estimated tile PSNR; <= 0 disables the fallback
@@ -257,6 +258,7 @@ int vct_ensure(VctBuf& b, size_t bytes, vct_context* ctx);
 
 // kernels launchers
 int vct_launch_mip(vct_context* ctx);
+int vct_ensure_pairs(vct_context* ctx);  // x-paired copy of the volume for the marcher
 int vct_launch_voxelize(vct_context* ctx);
 int vct_launch_gbuffer(vct_context* ctx);
 int vct_launch_trace(vct_context* ctx);

@@ -239,12 +239,16 @@ int launch_chain(vct_context* ctx) {
 }  // namespace
 }  // namespace vct
 
+// The mip chain, then the cone marcher's x-paired copy of the finished volume.
 int vct_launch_mip(vct_context* ctx) {
+  int rc;
   switch (ctx->dtype) {
-    case VCT_RGBA8: return vct::launch_chain<VCT_RGBA8>(ctx);
-    case VCT_FP16: return vct::launch_chain<VCT_FP16>(ctx);
-    case VCT_BF16: return vct::launch_chain<VCT_BF16>(ctx);
-    case VCT_FP32: return vct::launch_chain<VCT_FP32>(ctx);
+    case VCT_RGBA8: rc = vct::launch_chain<VCT_RGBA8>(ctx); break;
+    case VCT_FP16: rc = vct::launch_chain<VCT_FP16>(ctx); break;
+    case VCT_BF16: rc = vct::launch_chain<VCT_BF16>(ctx); break;
+    case VCT_FP32: rc = vct::launch_chain<VCT_FP32>(ctx); break;
+    default: return vct_set_error(VCT_ERR_INVALID_ARG, "bad dtype");
   }
-  return vct_set_error(VCT_ERR_INVALID_ARG, "bad dtype");
+  if (rc) return rc;
+  return vct_ensure_pairs(ctx);
 }

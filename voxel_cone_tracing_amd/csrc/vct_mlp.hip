@@ -585,12 +585,14 @@ namespace vct {
 // ---- per-tile check and refinement -------------------------------------------------
 // Every 32x32 tile is checked at 16 probe pixels traced at full resolution.
 // Stage 1 probes sit at offsets 3, 11, 19, 27 (the odd-odd pixels farthest
-// from the quarter-resolution samples); a tile whose mean squared error over
-// its covered probes exceeds the bound is refined: with the half-resolution
-// stage on, its odd-odd pixels (a 16x16 grid) are traced and interpolated
-// with the same surface guidance, then checked again at even-even probes
-// (offsets 4, 12, 20, 28, the pixels farthest from that grid); tiles that
-// still fail, or every failing tile with the stage off, are traced in full.
+// from the quarter-resolution samples).  By default the probes estimate each
+// tile's error and the tiles with the largest estimates are traced in full
+// until the estimated error of the rest fits 30% of the frame's budget for
+// the bound (tile_budget_kernel).  With the half-resolution stage on, a tile
+// whose mean squared error over its covered probes exceeds the bound has its
+// odd-odd pixels (a 16x16 grid) traced and interpolated with the same surface
+// guidance, is checked again at even-even probes (offsets 4, 12, 20, 28, the
+// pixels farthest from that grid), and is traced in full if it still fails.
 // Probe values are written into the image; a probe on a pixel that the
 // interpolation left for tracing (w < 0) counts as exact.
 constexpr int kFbTile = 32;
@@ -650,6 +652,107 @@ __global__ __launch_bounds__(1024) void tile_select_kernel(const float4* __restr
     __syncthreads();
   }
   if (tid == 0) out[0] = s_base;
+}
+
+// Stage-1 estimate, one thread per tile: est[t] = sum over the tile's covered
+// probes of the squared error (3 channels) of the learned value, ncov[t] =
+// covered probes; the probe values go into the image.  A probe on a pixel
+// the interpolation left for tracing (w < 0) counts as exact.
+__global__ __launch_bounds__(256) void tile_estimate_kernel(const float4* __restrict__ probe_val,
+                                                            const uint32_t* __restrict__ probe_pix,
+                                                            const float4* __restrict__ gpos, float4* image,
+                                                            uint32_t ntiles, float* __restrict__ est,
+                                                            uint32_t* __restrict__ ncov) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  float sse = 0.0f;
+  uint32_t nc = 0;
+#pragma unroll 4
+  for (int k = 0; k < kProbes; ++k) {
+    const uint32_t q = t * kProbes + k;
+    const uint32_t p = probe_pix[q];
+    const float4 v = probe_val[q], m = image[p];
+    if (gpos[p].w == 0.0f) continue;  // background
+    ++nc;
+    if (m.w < 0.0f) continue;  // left for tracing: exact
+    const float dx = v.x - m.x, dy = v.y - m.y, dz = v.z - m.z;
+    sse += dx * dx + dy * dy + dz * dz;
+  }
+  for (int k = 0; k < kProbes; ++k) image[probe_pix[t * kProbes + k]] = probe_val[t * kProbes + k];
+  est[t] = sse;
+  ncov[t] = nc;
+}
+
+// One workgroup: the tiles to trace in full, largest estimated error first,
+// until the error left in the others fits the frame's budget,
+//   sum_{not picked} est <= 3 * max_mse * kBudgetShare * sum ncov
+// (3 channels; each covered probe stands for the same share of the tile's
+// covered pixels).  The estimates are binned by their float exponent and two
+// mantissa bits (4 bins per octave) and summed in 2^-40 fixed point, so the
+// pick is deterministic; every tile of the threshold bin is picked.
+// flags[t] = 1 picked, 0 not; out = [count, picked tiles...].
+constexpr int kBudgetBins = 256;
+constexpr float kBudgetShare = 0.3f;  // 16 probes per tile estimate the error loosely: most of the budget is margin
+
+__device__ __forceinline__ int budget_bin(float e) {
+  if (!(e > 0.0f)) return -1;  // no error: never picked
+  const int b = (int)(__float_as_uint(e) >> 21) - 300;  // 2^-52 .. 2^12
+  return b < 0 ? 0 : (b >= kBudgetBins ? kBudgetBins - 1 : b);
+}
+
+__global__ __launch_bounds__(1024) void tile_budget_kernel(const float* __restrict__ est,
+                                                           const uint32_t* __restrict__ ncov, uint32_t ntiles,
+                                                           float max_mse, uint32_t* __restrict__ out,
+                                                           uint32_t* __restrict__ flags) {
+  __shared__ unsigned long long s_mass[kBudgetBins];
+  __shared__ unsigned long long s_total;
+  __shared__ unsigned int s_cov, s_count;
+  __shared__ int s_thr;
+  const int tid = threadIdx.x;
+  const float kFix = 1099511627776.0f;  // 2^40
+  for (int b = tid; b < kBudgetBins; b += 1024) s_mass[b] = 0ull;
+  if (tid == 0) {
+    s_total = 0ull;
+    s_cov = 0u;
+    s_count = 0u;
+  }
+  __syncthreads();
+  for (uint32_t t = tid; t < ntiles; t += 1024) {
+    const float e = est[t];
+    const int b = budget_bin(e);
+    atomicAdd(&s_cov, ncov[t]);
+    if (b >= 0) {
+      const unsigned long long q = (unsigned long long)(e * kFix);
+      atomicAdd(&s_mass[b], q);
+      atomicAdd(&s_total, q);
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const double budget = 3.0 * (double)max_mse * (double)kBudgetShare * (double)s_cov * (double)kFix;
+    double left = (double)s_total;
+    int thr = kBudgetBins;  // pick bins >= thr
+    while (thr > 0 && left > budget) {
+      --thr;
+      left -= (double)s_mass[thr];
+    }
+    s_thr = thr;
+  }
+  __syncthreads();
+  const int thr = s_thr;
+  const int lane = tid & 63;
+  for (uint32_t t0 = 0; t0 < ntiles; t0 += 1024) {
+    const uint32_t t = t0 + tid;
+    const bool pick = t < ntiles && budget_bin(est[t]) >= thr;
+    if (t < ntiles) flags[t] = pick ? 1u : 0u;
+    const unsigned long long m = __ballot(pick);
+    uint32_t base = 0;
+    if (lane == 0 && m) base = atomicAdd(&s_count, (uint32_t)__popcll(m));
+    base = __shfl(base, 0);
+    if (pick) out[1 + base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = t;
+  }
+  __syncthreads();
+  if (tid == 0) out[0] = s_count;
 }
 
 // Stage-2 probe pixels of the listed tiles: pix[0] = count, then 16 per tile.
@@ -954,15 +1057,26 @@ int refine(vct_context* ctx) {
   const float max_mse = powf(10.0f, -ctx->fb_psnr_db / 10.0f);
   uint32_t* flags = (uint32_t*)ctx->fb_flag.p;
   uint32_t* l1 = (uint32_t*)ctx->fb_list.p;
-  // stage 1: probes of every tile, failing tiles -> l1 (flag 1), others flag 0
+  // stage 1: probes of every tile; picked tiles -> l1 (flag 1), others flag 0
   TraceSelect sel;
   memset(&sel, 0, sizeof(sel));
   sel.pix_list = (const uint32_t*)ctx->probe_pix.p;
   sel.npix_list = ntiles * kProbes;
   if ((rc = vct_trace_select(ctx, gpos, gnrm, (float4*)ctx->probe_val.p, W, H, 0, nullptr, sel))) return rc;
-  hipLaunchKernelGGL(tile_select_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const float4*)ctx->probe_val.p,
-                     (const uint32_t*)ctx->probe_pix.p, gpos, image, (const uint32_t*)nullptr, ntiles, max_mse, l1,
-                     flags, 0u, 1u);
+  if (ctx->fb_half) {  // per-tile bound, then the half-resolution stage
+    hipLaunchKernelGGL(tile_select_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const float4*)ctx->probe_val.p,
+                       (const uint32_t*)ctx->probe_pix.p, gpos, image, (const uint32_t*)nullptr, ntiles, max_mse, l1,
+                       flags, 0u, 1u);
+  } else {  // frame budget: largest estimated errors first
+    if ((rc = vct_ensure(ctx->fb_est, (size_t)ntiles * 8, ctx))) return rc;
+    float* est = (float*)ctx->fb_est.p;
+    uint32_t* ncov = (uint32_t*)(est + ntiles);
+    hipLaunchKernelGGL(tile_estimate_kernel, dim3((ntiles + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const float4*)ctx->probe_val.p, (const uint32_t*)ctx->probe_pix.p, gpos, image, ntiles, est,
+                       ncov);
+    hipLaunchKernelGGL(tile_budget_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const float*)est,
+                       (const uint32_t*)ncov, ntiles, max_mse, l1, flags);
+  }
   VCT_HIP(hipGetLastError());
   // pixels the quarter-resolution stage left, in tiles that passed
   memset(&sel, 0, sizeof(sel));

@@ -350,7 +350,7 @@ int vct_destroy(vct_ctx ctx) {
                     &ctx->full_target, &ctx->sample_pix, &ctx->train_X, &ctx->train_A, &ctx->train_Y,
                     &ctx->train_valid, &ctx->probe_pix, &ctx->probe_val, &ctx->fb_list, &ctx->fb_pix,
                     &ctx->fb_list2, &ctx->fb_pix2, &ctx->fb_flag, &ctx->half_img, &ctx->probe_pix2,
-                    &ctx->probe_val2, &ctx->pairs};
+                    &ctx->probe_val2, &ctx->pairs, &ctx->fb_est};
   for (VctBuf* b : bufs) free_buf(*b);
   for (int i = 0; i < 12; ++i)
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);

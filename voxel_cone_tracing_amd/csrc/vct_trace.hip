@@ -807,8 +807,9 @@ void vct_cone_set_table(int set, ConeSet& cs) {
 }
 
 // Rebuilds the x-paired copy of the volume when the volume has changed since
-// it was last built (16-bit and RGBA8 storage).
-static int ensure_pairs(vct_context* ctx) {
+// it was last built (16-bit and RGBA8 storage).  The mip pass calls it last;
+// the marcher calls it again for volumes written since (uploads).
+int vct_ensure_pairs(vct_context* ctx) {
   if (ctx->dtype == VCT_FP32 || ctx->pairs_version == ctx->volume_version) return VCT_OK;
   PairArgs P;
   memset(&P, 0, sizeof(P));
@@ -869,7 +870,7 @@ int vct_trace_select(vct_context* ctx, const float4* gpos, const float4* gnrm, f
   A.grid_stride = sel.grid_stride;
   A.half_w = sel.half_w;
   int rc;
-  if ((rc = ensure_pairs(ctx))) return rc;
+  if ((rc = vct_ensure_pairs(ctx))) return rc;
   const bool paired = ctx->dtype != VCT_FP32;
   A.vol = (const char*)(paired ? ctx->pairs.p : ctx->volume.p);
   for (int l = 0; l < ctx->levels; ++l)
