@@ -275,7 +275,7 @@ def main():
         quality["learned_vs_plain_full_frame_db"] = round(psnr(learned["img"], plain_img, cov_full)[0], 2)
         quality["learned_no_fallback_vs_plain_full_frame_db"] = round(psnr(learned["raw"], plain_img, cov_full)[0], 2)
     if not args.no_quality and wl.name.startswith("soup"):
-        crop = crop_window(wl)
+        crop = crop_window(wl, 256 if wl.res >= 512 else 512)  # 256x256 crop at 512^3
         ref, cov = fp64_reference(sc, wl, crop)
         x0, y0, s = crop
         sl = (slice(y0, y0 + s), slice(x0, x0 + s))
