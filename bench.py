@@ -11,9 +11,9 @@ GPU through libvct.so, geometry resident in HBM before timing:
             2560x1440, per-tile error estimate at probe pixels and
             full-resolution trace of the 32x32 tiles with the largest
             estimates until the frame holds 40 dB.
-`value` is the learned form's ms/frame when it holds 40 dB (the plain
-form's otherwise, or when no trained parameters are found); both forms are
-reported, each with its PSNR against the FP64 oracle (voxelize, mip,
+`value` is the ms/frame of the faster form that holds 40 dB against the
+FP64 oracle (the learned form must also hold 40 dB against the plain frame);
+both forms are reported, each with its PSNR against the FP64 oracle (voxelize, mip,
 G-buffer and trace in double precision) over the covered pixels of a
 512x512 centre crop.
 
@@ -321,10 +321,12 @@ def main():
             gbs = amt / (float(plain_passes[i]) * 1e-3) / 1e9
             hbm_passes[n] = {"achieved_GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
                              "bytes": int(amt), "algorithmic": dfn, "pass_ms": round(float(plain_passes[i]), 4)}
-    # the learned form is the reported value when it holds the 40 dB bound
+    # value: the faster form that holds the 40 dB bound (the plain form holds it
+    # by construction: it is the full-resolution trace)
     learned_ok = learned is not None and quality.get("learned_vs_plain_full_frame_db", 0.0) >= 40.0 and \
         quality.get("learned_vs_fp64_db", 99.0) >= 40.0
-    value = learned["ms"] if learned_ok else plain_ms
+    use_learned = learned_ok and learned["ms"] < plain_ms
+    value = learned["ms"] if use_learned else plain_ms
     out = {
         "metric": "ms/frame voxelize+mip+trace @256^3,1440p; learned vs plain at >=40 dB PSNR",
         "value": round(value, 4),
@@ -341,7 +343,8 @@ def main():
         "config": {
             "workload": f"{wl.name}: {wl.ntri}-triangle soup, {wl.res}^3 grid ({wl.levels} levels, {args.dtype} "
                         f"storage), {wl.width}x{wl.height}, 9 diffuse + 1 specular cone, indirect light",
-            "form": "learned (640x360 trace + MLP upsampling + per-tile fallback)" if learned_ok else "plain",
+            "form": "learned (640x360 trace + MLP upsampling + per-tile fallback)" if use_learned else
+                    "plain (full-resolution trace; the faster form at >= 40 dB)",
         },
         "plain_ms_per_frame": round(plain_ms, 4),
         "plain_passes_ms": {n: round(float(plain_passes[i]), 4) for i, n in enumerate(PASSES[:4])},

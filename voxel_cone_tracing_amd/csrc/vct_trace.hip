@@ -392,42 +392,6 @@ __device__ __forceinline__ float3 cross3(float3 a, float3 b) {
 }
 __device__ __forceinline__ float len3(float3 v) { return sqrtf(dot3(v, v)); }
 
-// trace_cone (voxel_cone_tracing.frag:88-119)
-template <int DT>
-__device__ float4 trace_cone(const TraceArgs& A, const SamplerLds* spt, float3 origin, float3 dir, float aperture,
-                             float max_dist,
-                             uint32_t& steps) {
-  dir = normalize3(dir);
-  const int mx = dir.x < 0.0f ? 0 : 1, my = dir.y < 0.0f ? 2 : 3, mz = dir.z < 0.0f ? 4 : 5;
-  const float3 ad = make_float3(fabsf(dir.x), fabsf(dir.y), fabsf(dir.z));
-  const float vs = 1.0f / (float)A.res;
-  const float fres = (float)A.res;
-  // A cone never needs to leave the volume by more than its diagonal; the
-  // bound also keeps a non-finite target distance from looping forever.
-  max_dist = fminf(max_dist, 3.4641016f);
-  // A sample more than half a texel of the coarsest level outside the unit
-  // cube reads only the zero border; once the cone is that far out and moving
-  // away on that axis, every later sample is zero and c no longer changes.
-  const float margin = 0.5f * (float)(1u << (A.levels - 1)) * vs;
-  float4 c = f4(0.0f);
-  float dist = 3.0f * vs;
-  float diam = dist * aperture;
-  while (c.w < 1.0f && dist < max_dist) {
-    const float3 sp = make_float3(dir.x * dist + origin.x, dir.y * dist + origin.y, dir.z * dist + origin.z);
-    if ((sp.x < -margin && dir.x <= 0.0f) || (sp.x >= 1.0f + margin && dir.x >= 0.0f) ||
-        (sp.y < -margin && dir.y <= 0.0f) || (sp.y >= 1.0f + margin && dir.y >= 0.0f) ||
-        (sp.z < -margin && dir.z <= 0.0f) || (sp.z >= 1.0f + margin && dir.z >= 0.0f))
-      break;
-    const float mip = fmaxf(log2f(diam * fres), 0.0f);
-    const float4 s = sample_voxel<DT>(A, spt, sp, ad, mx, my, mz, mip);
-    c = c + (1.0f - c.w) * s;
-    dist += fmaxf(diam * 0.5f, vs);
-    ++steps;
-    diam = dist * aperture;
-  }
-  return c;
-}
-
 // tangent (voxel_cone_tracing.frag:126-134)
 __device__ __forceinline__ float3 tangent_of(float3 n) {
   const float3 t1 = cross3(n, make_float3(0.f, 0.f, 1.f));
@@ -443,148 +407,258 @@ __device__ __forceinline__ float3 refract3(float3 I, float3 N, float eta) {
   return make_float3(eta * I.x - s * N.x, eta * I.y - s * N.y, eta * I.z - s * N.z);
 }
 
-// One workgroup = the 8x8 pixels of a block (lane = pixel) x 4 waves.  The
-// jobs of a pixel - direct light with its shadow cones, the specular cone,
-// the refraction cone, then the diffuse cones - are taken by the waves from a
-// workgroup counter in LDS, longest first, so the four waves finish close
-// together; every wave still marches one cone for 64 neighbouring pixels at a
-// time (coherent gathers).  Wave 0 then sums the diffuse cones in their
-// original order and composes the pixel (voxel_cone_tracing.frag:246-275).
+// One workgroup = the 8x8 pixels of a block x 4 waves.  Wave 0 sets the
+// block's shading pixels up (lane = pixel) into LDS.  The cones of the block
+// - every pixel's specular, refraction and shadow cones, then its diffuse
+// cones, longest kinds first - form one queue of (job, pixel) items, job-major
+// so that neighbouring lanes start on the same cone of neighbouring pixels
+// (coherent gathers).  Every lane marches its own item and takes the next one
+// from a workgroup counter in LDS as soon as its cone ends, so lanes do not
+// idle while the longest cone of a wave finishes.  Wave 0 then composes the
+// pixels in the reference's order (voxel_cone_tracing.frag:246-275).
 constexpr int kMaxSlots = 18;  // 16 diffuse + specular + refraction
-constexpr int kJobDirect = 0, kJobSpecular = 1, kJobRefraction = 2, kJobDiffuse = 3;
+// job codes: 0..15 diffuse cone k, then:
+constexpr int kJobSpecular = 16, kJobRefraction = 17, kJobShadow = 18;  // kJobShadow + light
+constexpr int kMaxJobs = kMaxSlots + VCT_MAX_LIGHTS;
+constexpr int kRefillMin = 16;  // idle lanes of a wave that trigger a refill (measured: 1 -> 3.15 ms, 16 -> 2.97 ms)
 
 // LDS of one workgroup of the cone marcher.
 struct ShadeLds {
-  float cone[kMaxSlots][3][64];  // per slot (diffuse 0..nd-1, specular nd, refraction nd+1) and pixel
-  float direct[3][64];
-  int next_job;
+  float cone[kMaxSlots][3][64];      // slot (diffuse 0..nd-1, specular nd, refraction nd+1) x pixel
+  float shadow[VCT_MAX_LIGHTS][64];  // max(0, 1 - alpha) of the shadow cone per light and pixel
+  float pos[3][64], nrm[3][64], o1[3][64], o2[3][64], vdir[3][64];
+  float spec_ap[64], eta[64];
+  unsigned long long refr_mask;  // shading pixels with a transmissive material
+  int npx, njobs, next_item;
+  uint8_t px_of[64];  // the block's shading pixels, compacted
+  int8_t jobs[kMaxJobs];
 };
 
-// Shades the block's 64 pixels (lane = pixel, wave = job taker); returns
-// after the workgroup barrier of the composition.  next_job must be 0 and
-// visible to the workgroup on entry.
+// The march state of one lane's cone (trace_cone, voxel_cone_tracing.frag:88-119).
+struct Cone {
+  float3 origin, dir, ad;
+  int mx, my, mz;
+  float ap, max_dist, dist, diam;
+  float4 c;
+};
+
+__device__ __forceinline__ void cone_begin(Cone& k, float3 origin, float3 dir, float aperture, float max_dist,
+                                           float vs) {
+  dir = normalize3(dir);
+  k.origin = origin;
+  k.dir = dir;
+  k.mx = dir.x < 0.0f ? 0 : 1;
+  k.my = dir.y < 0.0f ? 2 : 3;
+  k.mz = dir.z < 0.0f ? 4 : 5;
+  k.ad = make_float3(fabsf(dir.x), fabsf(dir.y), fabsf(dir.z));
+  // A cone never needs to leave the volume by more than its diagonal; the
+  // bound also keeps a non-finite target distance from looping forever.
+  k.max_dist = fminf(max_dist, 3.4641016f);
+  k.ap = aperture;
+  k.c = f4(0.0f);
+  k.dist = 3.0f * vs;
+  k.diam = k.dist * aperture;
+}
+
+// The loop condition of trace_cone, plus the exact early exit: a sample more
+// than half a texel of the coarsest level outside the unit cube reads only
+// the zero border, so once the cone is that far out and moving away on that
+// axis every later sample is zero and c no longer changes.
+__device__ __forceinline__ bool cone_alive(const Cone& k, float margin) {
+  if (!(k.c.w < 1.0f && k.dist < k.max_dist)) return false;
+  const float3 sp = make_float3(k.dir.x * k.dist + k.origin.x, k.dir.y * k.dist + k.origin.y,
+                                k.dir.z * k.dist + k.origin.z);
+  return !((sp.x < -margin && k.dir.x <= 0.0f) || (sp.x >= 1.0f + margin && k.dir.x >= 0.0f) ||
+           (sp.y < -margin && k.dir.y <= 0.0f) || (sp.y >= 1.0f + margin && k.dir.y >= 0.0f) ||
+           (sp.z < -margin && k.dir.z <= 0.0f) || (sp.z >= 1.0f + margin && k.dir.z >= 0.0f));
+}
+
+template <int DT>
+__device__ __forceinline__ void cone_step(const TraceArgs& A, const SamplerLds* S, Cone& k, float vs, float fres) {
+  const float3 sp = make_float3(k.dir.x * k.dist + k.origin.x, k.dir.y * k.dist + k.origin.y,
+                                k.dir.z * k.dist + k.origin.z);
+  const float mip = fmaxf(log2f(k.diam * fres), 0.0f);
+  const float4 s = sample_voxel<DT>(A, S, sp, k.ad, k.mx, k.my, k.mz, mip);
+  k.c = k.c + (1.0f - k.c.w) * s;
+  k.dist += fmaxf(k.diam * 0.5f, vs);
+  k.diam = k.dist * k.ap;
+}
+
+// Direction of light li seen from pos (biased coordinates) and the distance
+// the shadow cone may travel (direct_light, voxel_cone_tracing.frag:175-218).
+__device__ __forceinline__ float3 light_dir(const TraceArgs& A, int li, float3 pos, float& d) {
+  const DevLight& Lt = A.lights.l[li];
+  if (Lt.kind == VCT_LIGHT_DIRECTIONAL) {
+    d = 1.73205080757f;
+    return normalize3(make_float3(Lt.pos[0], Lt.pos[1], Lt.pos[2]));
+  }
+  const float3 lp = make_float3(0.5f * (Lt.pos[0] / A.cube_size) + 0.5f, 0.5f * (Lt.pos[1] / A.cube_size) + 0.5f,
+                                0.5f * (Lt.pos[2] / A.cube_size) + 0.5f);
+  const float3 v = make_float3(lp.x - pos.x, lp.y - pos.y, lp.z - pos.z);
+  d = len3(v);
+  return normalize3(v);
+}
+
+__device__ __forceinline__ float3 ld3(const float (*a)[64], int p) { return make_float3(a[0][p], a[1][p], a[2][p]); }
+
+// Shades the block's 64 pixels; returns after the workgroup barrier of the
+// composition.  L->next_item must be 0 and visible to the workgroup on entry.
 template <int DT>
 __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, size_t oidx, bool valid,
-                                            ShadeLds* L, const SamplerLds* s_spread) {
+                                            ShadeLds* L, const SamplerLds* S) {
   const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
-  float (*s_cone)[3][64] = L->cone;
-  float (*s_direct)[64] = L->direct;
   valid = valid && px < A.W && py < A.H;
   const size_t idx = valid ? (size_t)py * A.W + px : 0;
-  const float4 gp = A.gpos[idx];
-  const float4 bg = make_float4(0.15f, 0.25f, 0.25f, 1.0f);  // renderer.cpp:398
-  const float4 gn = A.gnrm[idx];
-  uint32_t mid = __builtin_bit_cast(uint32_t, gn.w);
-  mid = mid < A.nmat ? mid : 0u;  // ids of an uploaded G-buffer are not trusted
-  const DevMaterial& m = A.mats[mid];
-  const float3 world = make_float3(gp.x, gp.y, gp.z);
-  const float3 n = make_float3(gn.x, gn.y, gn.z);
-  const float3 pos = make_float3(0.5f * (world.x / A.cube_size) + 0.5f, 0.5f * (world.y / A.cube_size) + 0.5f,
-                                 0.5f * (world.z / A.cube_size) + 0.5f);
-  // 0: nothing to do, 1: background / outside the cube, 2: voxel view, 3: shade
-  int mode = 0;
-  if (valid) {
-    if (gp.w == 0.0f || !(fabsf(pos.x) < 1.0f && fabsf(pos.y) < 1.0f && fabsf(pos.z) < 1.0f)) mode = 1;
-    else mode = A.view_dir < 7 ? 2 : 3;  // within_cube on biased coordinates (:250, :61-64)
-  }
-  const float3 cam = make_float3(A.cam[0], A.cam[1], A.cam[2]);
-  const float3 view_dir = normalize3(make_float3(world.x - cam.x, world.y - cam.y, world.z - cam.z));
   const float sqrt3 = 1.73205080757f;
-  uint32_t steps = 0;
-  // specular aperture (:220-227)
-  const float rough = sqrtf(2.0f / (m.shininess + 2.0f));
-  const float spec_ap = fminf(fmaxf(tanf(1.57079f * rough), 0.0174533f), 3.14159265f);
   const int nd = A.en_diffuse ? A.cones.n : 0;
   const bool do_spec = A.en_specular != 0;
-  const bool do_refr = do_spec && illum_transmissive(m.illum);
-  float3 o1 = make_float3(0.f, 0.f, 0.f), o2 = o1;
-  if (mode == 3 && nd > 0) {
-    o1 = normalize3(tangent_of(n));  // trace_diffuse (:146-147)
-    o2 = normalize3(cross3(o1, n));
+  const int nl = A.lights.count < VCT_MAX_LIGHTS ? A.lights.count : VCT_MAX_LIGHTS;
+  // ---- set-up of the block's pixels (wave 0, lane = pixel) -------------------
+  auto pixel = [&](float4& gp, float4& gn, const DevMaterial*& m, float3& pos, int& mode) {
+    gp = A.gpos[idx];
+    gn = A.gnrm[idx];
+    uint32_t mid = __builtin_bit_cast(uint32_t, gn.w);
+    mid = mid < A.nmat ? mid : 0u;  // ids of an uploaded G-buffer are not trusted
+    m = &A.mats[mid];
+    pos = make_float3(0.5f * (gp.x / A.cube_size) + 0.5f, 0.5f * (gp.y / A.cube_size) + 0.5f,
+                      0.5f * (gp.z / A.cube_size) + 0.5f);
+    // 0: nothing to do, 1: background / outside the cube, 2: voxel view, 3: shade
+    mode = 0;
+    if (valid) {
+      if (gp.w == 0.0f || !(fabsf(pos.x) < 1.0f && fabsf(pos.y) < 1.0f && fabsf(pos.z) < 1.0f)) mode = 1;
+      else mode = A.view_dir < 7 ? 2 : 3;  // within_cube on biased coordinates (:250, :61-64)
+    }
+  };
+  if (grp == 0) {
+    float4 gp, gn;
+    const DevMaterial* m;
+    float3 pos;
+    int mode;
+    pixel(gp, gn, m, pos, mode);
+    const float3 n = make_float3(gn.x, gn.y, gn.z);
+    const float3 cam = make_float3(A.cam[0], A.cam[1], A.cam[2]);
+    const float3 vd = normalize3(make_float3(gp.x - cam.x, gp.y - cam.y, gp.z - cam.z));
+    // specular aperture (:220-227)
+    const float rough = sqrtf(2.0f / (m->shininess + 2.0f));
+    L->spec_ap[lane] = fminf(fmaxf(tanf(1.57079f * rough), 0.0174533f), 3.14159265f);
+    L->eta[lane] = 1.0f / m->ior;
+    float3 o1 = make_float3(0.f, 0.f, 0.f), o2 = o1;
+    if (mode == 3 && nd > 0) {
+      o1 = normalize3(tangent_of(n));  // trace_diffuse (:146-147)
+      o2 = normalize3(cross3(o1, n));
+    }
+    const float3 vals[5] = {pos, n, o1, o2, vd};
+    float (*dst[5])[64] = {L->pos, L->nrm, L->o1, L->o2, L->vdir};
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      dst[q][0][lane] = vals[q].x;
+      dst[q][1][lane] = vals[q].y;
+      dst[q][2][lane] = vals[q].z;
+    }
+    const unsigned long long shade = __ballot(mode == 3);
+    const unsigned long long refr = __ballot(mode == 3 && do_spec && illum_transmissive(m->illum));
+    if (mode == 3) L->px_of[__popcll(shade & ((1ull << lane) - 1ull))] = (uint8_t)lane;
+    if (lane == 0) {
+      L->npx = __popcll(shade);
+      L->refr_mask = refr;
+      int nj = 0;  // longest kinds first
+      if (do_spec) L->jobs[nj++] = kJobSpecular;
+      if (refr) L->jobs[nj++] = kJobRefraction;
+      if (A.en_direct && A.en_shadow)
+        for (int li = 0; li < nl; ++li) L->jobs[nj++] = (int8_t)(kJobShadow + li);
+      for (int k = 0; k < nd; ++k) L->jobs[nj++] = (int8_t)k;
+      L->njobs = nj;
+    }
   }
-  const int njobs = kJobDiffuse + nd;
-  for (;;) {
-    int job = 0;
-    if (lane == 0) job = atomicAdd(&L->next_job, 1);
-    job = __builtin_amdgcn_readfirstlane(job);  // one job per wave
-    if (job >= njobs) break;
-    if (job == kJobDirect) {
-      if (!A.en_direct) continue;
-      float3 res = make_float3(0.f, 0.f, 0.f);
-      if (mode == 3) {
-        const int nl = A.lights.count < VCT_MAX_LIGHTS ? A.lights.count : VCT_MAX_LIGHTS;
-        for (int li = 0; li < nl; ++li) {
-          const DevLight& Lt = A.lights.l[li];
-          float3 ldir;
-          float d;
-          if (Lt.kind == VCT_LIGHT_DIRECTIONAL) {
-            ldir = normalize3(make_float3(Lt.pos[0], Lt.pos[1], Lt.pos[2]));
-            d = sqrt3;
-          } else {
-            const float3 lp = make_float3(0.5f * (Lt.pos[0] / A.cube_size) + 0.5f,
-                                          0.5f * (Lt.pos[1] / A.cube_size) + 0.5f,
-                                          0.5f * (Lt.pos[2] / A.cube_size) + 0.5f);
-            ldir = make_float3(lp.x - pos.x, lp.y - pos.y, lp.z - pos.z);
-            d = len3(ldir);
-            ldir = normalize3(ldir);
-          }
-          const float cos_surf = fmaxf(dot3(n, ldir), 0.0f);
-          const float att = (Lt.kind == VCT_LIGHT_DIRECTIONAL) ? 1.0f : 1.0f / (1.0f + d * d);
-          const float k = att * cos_surf;
-          const float3 lc = make_float3(k * Lt.color[0] * Lt.intensity, k * Lt.color[1] * Lt.intensity,
-                                        k * Lt.color[2] * Lt.intensity);
-          float shadow = 1.0f;
-          if (A.en_shadow) shadow = fmaxf(0.0f, 1.0f - trace_cone<DT>(A, s_spread, pos, ldir, 0.1f, d, steps).w);
-          const float lambert = fmaxf(dot3(ldir, n), 0.0f);
-          float refr_angle = 0.0f;
-          if (m.dissolve <= 0.1f) {
-            const float3 r = refract3(view_dir, n, 1.0f / m.ior);
-            refr_angle = fmaxf((1.0f - m.dissolve) * dot3(r, ldir), 0.0f);
-          }
-          const float3 hv = normalize3(make_float3(ldir.x + view_dir.x, ldir.y + view_dir.y, ldir.z + view_dir.z));
-          float sa = fmaxf(dot3(hv, n), 0.0f);
-          sa = fmaxf(sa, refr_angle);
-          const float sc = powf(sa, m.shininess);
-          const float f = shadow + 0.04f;
-          res.x += f * (lambert * m.diffuse[0] + sc * m.specular[0]) * lc.x;
-          res.y += f * (lambert * m.diffuse[1] + sc * m.specular[1]) * lc.y;
-          res.z += f * (lambert * m.diffuse[2] + sc * m.specular[2]) * lc.z;
-        }
-      }
-      s_direct[0][lane] = res.x + fminf(fmaxf(m.emission[0], 0.0f), 1.0f);
-      s_direct[1][lane] = res.y + fminf(fmaxf(m.emission[1], 0.0f), 1.0f);
-      s_direct[2][lane] = res.z + fminf(fmaxf(m.emission[2], 0.0f), 1.0f);
-      continue;
-    }
-    int slot;
-    float3 d;
-    float ap = spec_ap;
-    if (job >= kJobDiffuse) {
-      slot = job - kJobDiffuse;
-      const float a = A.cones.a[slot], b = A.cones.b[slot], cc = A.cones.c[slot];
-      d = make_float3(a * n.x + b * o1.x + cc * o2.x, a * n.y + b * o1.y + cc * o2.y, a * n.z + b * o1.z + cc * o2.z);
-      ap = A.cones.aperture;
-    } else if (job == kJobSpecular) {
-      if (!do_spec) continue;
-      slot = nd;
-      // normalize(reflect(-view_dir, n)) (:231), normalised again in trace_cone (:90)
-      const float3 I = make_float3(-view_dir.x, -view_dir.y, -view_dir.z);
-      const float dn = 2.0f * dot3(n, I);
-      d = normalize3(make_float3(I.x - dn * n.x, I.y - dn * n.y, I.z - dn * n.z));
+  __syncthreads();
+  // ---- the cone queue (every lane) -----------------------------------------------
+  const float vs = 1.0f / (float)A.res, fres = (float)A.res;
+  const float margin = 0.5f * (float)(1u << (A.levels - 1)) * vs;
+  const int npx = L->npx;
+  const int nitems = npx * L->njobs;
+  uint32_t steps = 0;
+  int item = -1, job = 0, pix = 0;
+  bool drained = nitems == 0;
+  Cone k;
+  auto finish = [&]() {
+    if (job >= kJobShadow) {
+      L->shadow[job - kJobShadow][pix] = fmaxf(0.0f, 1.0f - k.c.w);
     } else {
-      slot = nd + 1;
-      d = refract3(view_dir, n, 1.0f / m.ior);  // trace_refraction (:237-241)
+      const int slot = job < kJobSpecular ? job : (job == kJobSpecular ? nd : nd + 1);
+      L->cone[slot][0][pix] = k.c.x;
+      L->cone[slot][1][pix] = k.c.y;
+      L->cone[slot][2][pix] = k.c.z;
     }
-    if (mode == 3 && (job != kJobRefraction || do_refr)) {
-      const float4 r = trace_cone<DT>(A, s_spread, pos, d, ap, sqrt3, steps);
-      s_cone[slot][0][lane] = r.x;
-      s_cone[slot][1][lane] = r.y;
-      s_cone[slot][2][lane] = r.z;
+  };
+  for (;;) {
+    // lanes without a cone take the next items of the queue, once enough of
+    // them idle (or none is busy): a refill costs the whole wave its set-up
+    const unsigned long long busy = __ballot(item >= 0);
+    for (;;) {
+      const bool need = item < 0 && !drained;
+      const unsigned long long want = __ballot(need);
+      if (!want || (busy && __popcll(want) < kRefillMin)) break;
+      const int first = __ffsll((long long)want) - 1;
+      int base = 0;
+      if (lane == first) base = atomicAdd(&L->next_item, __popcll(want));
+      base = __shfl(base, first);
+      if (!need) continue;
+      const int it = base + __popcll(want & ((1ull << lane) - 1ull));
+      if (it >= nitems) {
+        drained = true;
+        continue;
+      }
+      const int j = it / npx;
+      job = L->jobs[j];
+      pix = L->px_of[it - j * npx];
+      if (job == kJobRefraction && !((L->refr_mask >> pix) & 1ull)) continue;  // opaque pixel: no item
+      const float3 pos = ld3(L->pos, pix), n = ld3(L->nrm, pix);
+      float3 d;
+      float ap, md = sqrt3;
+      if (job < kJobSpecular) {
+        const float a = A.cones.a[job], b = A.cones.b[job], cc = A.cones.c[job];
+        const float3 o1 = ld3(L->o1, pix), o2 = ld3(L->o2, pix);
+        d = make_float3(a * n.x + b * o1.x + cc * o2.x, a * n.y + b * o1.y + cc * o2.y, a * n.z + b * o1.z + cc * o2.z);
+        ap = A.cones.aperture;
+      } else if (job == kJobSpecular) {
+        // normalize(reflect(-view_dir, n)) (:231), normalised again in trace_cone (:90)
+        const float3 vd = ld3(L->vdir, pix);
+        const float3 I = make_float3(-vd.x, -vd.y, -vd.z);
+        const float dn = 2.0f * dot3(n, I);
+        d = normalize3(make_float3(I.x - dn * n.x, I.y - dn * n.y, I.z - dn * n.z));
+        ap = L->spec_ap[pix];
+      } else if (job == kJobRefraction) {
+        d = refract3(ld3(L->vdir, pix), n, L->eta[pix]);  // trace_refraction (:237-241)
+        ap = L->spec_ap[pix];
+      } else {
+        d = light_dir(A, job - kJobShadow, pos, md);  // shadow cone of direct_light
+        ap = 0.1f;
+      }
+      cone_begin(k, pos, d, ap, md, vs);
+      if (cone_alive(k, margin)) item = it;
+      else finish();  // no step at all
+    }
+    if (!__ballot(item >= 0)) break;
+    if (item >= 0) {
+      cone_step<DT>(A, S, k, vs, fres);
+      ++steps;
+      if (!cone_alive(k, margin)) {
+        finish();
+        item = -1;
+      }
     }
   }
   if (A.step_counter && steps) atomicAdd(A.step_counter, (unsigned long long)steps);
   __syncthreads();
-  if (grp != 0 || mode == 0) return;
+  // ---- composition (wave 0, lane = pixel) --------------------------------------------
+  if (grp != 0) return;
+  float4 gp, gn;
+  const DevMaterial* mp;
+  float3 pos;
+  int mode;
+  pixel(gp, gn, mp, pos, mode);
+  if (mode == 0) return;
+  const float4 bg = make_float4(0.15f, 0.25f, 0.25f, 1.0f);  // renderer.cpp:398
   if (mode == 1) {
     A.image[oidx] = A.demodulate ? f4(0.0f) : bg;
     return;
@@ -596,37 +670,67 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
     if (d < 2) { ad.x = 1.0f; mx = d; }
     else if (d < 4) { ad.y = 1.0f; my = d; }
     else { ad.z = 1.0f; mz = d; }
-    const float4 s = sample_voxel<DT>(A, s_spread, pos, ad, mx, my, mz, A.view_lod);
+    const float4 s = sample_voxel<DT>(A, S, pos, ad, mx, my, mz, A.view_lod);
     const float a = s.w;
     A.image[oidx] = make_float4(s.x * a + bg.x * (1.0f - a), s.y * a + bg.y * (1.0f - a),
                                 s.z * a + bg.z * (1.0f - a), a * a + bg.w * (1.0f - a));
     return;
   }
+  const DevMaterial& m = *mp;
+  const float3 n = make_float3(gn.x, gn.y, gn.z);
+  const float3 view_dir = ld3(L->vdir, lane);
+  const bool do_refr = (L->refr_mask >> lane) & 1ull;
   float3 out = make_float3(0.f, 0.f, 0.f);
   if (nd > 0) {
     float3 diff_acc = make_float3(0.f, 0.f, 0.f);
-    for (int k = 0; k < nd; ++k)
-      diff_acc = make_float3(diff_acc.x + s_cone[k][0][lane], diff_acc.y + s_cone[k][1][lane],
-                             diff_acc.z + s_cone[k][2][lane]);
+    for (int q = 0; q < nd; ++q)
+      diff_acc = make_float3(diff_acc.x + L->cone[q][0][lane], diff_acc.y + L->cone[q][1][lane],
+                             diff_acc.z + L->cone[q][2][lane]);
     const float inv = 1.0f / (float)nd;
     out.x += m.diffuse[0] * (diff_acc.x * inv);
     out.y += m.diffuse[1] * (diff_acc.y * inv);
     out.z += m.diffuse[2] * (diff_acc.z * inv);
   }
-  if (A.en_direct) {
-    out.x += s_direct[0][lane];
-    out.y += s_direct[1][lane];
-    out.z += s_direct[2][lane];
+  if (A.en_direct) {  // direct_light (:175-218) with the shadow cones' results
+    float3 res = make_float3(0.f, 0.f, 0.f);
+    for (int li = 0; li < nl; ++li) {
+      const DevLight& Lt = A.lights.l[li];
+      float d;
+      const float3 ldir = light_dir(A, li, pos, d);
+      const float cos_surf = fmaxf(dot3(n, ldir), 0.0f);
+      const float att = (Lt.kind == VCT_LIGHT_DIRECTIONAL) ? 1.0f : 1.0f / (1.0f + d * d);
+      const float kk = att * cos_surf;
+      const float3 lc = make_float3(kk * Lt.color[0] * Lt.intensity, kk * Lt.color[1] * Lt.intensity,
+                                    kk * Lt.color[2] * Lt.intensity);
+      const float shadow = A.en_shadow ? L->shadow[li][lane] : 1.0f;
+      const float lambert = fmaxf(dot3(ldir, n), 0.0f);
+      float refr_angle = 0.0f;
+      if (m.dissolve <= 0.1f) {
+        const float3 r = refract3(view_dir, n, 1.0f / m.ior);
+        refr_angle = fmaxf((1.0f - m.dissolve) * dot3(r, ldir), 0.0f);
+      }
+      const float3 hv = normalize3(make_float3(ldir.x + view_dir.x, ldir.y + view_dir.y, ldir.z + view_dir.z));
+      float sa = fmaxf(dot3(hv, n), 0.0f);
+      sa = fmaxf(sa, refr_angle);
+      const float sc = powf(sa, m.shininess);
+      const float f = shadow + 0.04f;
+      res.x += f * (lambert * m.diffuse[0] + sc * m.specular[0]) * lc.x;
+      res.y += f * (lambert * m.diffuse[1] + sc * m.specular[1]) * lc.y;
+      res.z += f * (lambert * m.diffuse[2] + sc * m.specular[2]) * lc.z;
+    }
+    out.x += res.x + fminf(fmaxf(m.emission[0], 0.0f), 1.0f);
+    out.y += res.y + fminf(fmaxf(m.emission[1], 0.0f), 1.0f);
+    out.z += res.z + fminf(fmaxf(m.emission[2], 0.0f), 1.0f);
   }
   if (do_spec) {
-    out.x += m.specular[0] * s_cone[nd][0][lane];
-    out.y += m.specular[1] * s_cone[nd][1][lane];
-    out.z += m.specular[2] * s_cone[nd][2][lane];
+    out.x += m.specular[0] * L->cone[nd][0][lane];
+    out.y += m.specular[1] * L->cone[nd][1][lane];
+    out.z += m.specular[2] * L->cone[nd][2][lane];
   }
   if (do_refr) {  // final colour override (:273-274)
-    const float3 tr = make_float3(m.transmittance[0] * s_cone[nd + 1][0][lane],
-                                  m.transmittance[1] * s_cone[nd + 1][1][lane],
-                                  m.transmittance[2] * s_cone[nd + 1][2][lane]);
+    const float3 tr = make_float3(m.transmittance[0] * L->cone[nd + 1][0][lane],
+                                  m.transmittance[1] * L->cone[nd + 1][1][lane],
+                                  m.transmittance[2] * L->cone[nd + 1][2][lane]);
     const float dv = m.dissolve;
     out = make_float3(tr.x * (1.0f - dv) + out.x * dv, tr.y * (1.0f - dv) + out.y * dv,
                       tr.z * (1.0f - dv) + out.z * dv);
@@ -651,7 +755,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void t
   __shared__ SamplerLds s_sampler;
   const int lane = threadIdx.x & 63;
   sampler_init(A, &s_sampler);
-  if (threadIdx.x == 0) s_shade.next_job = 0;
+  if (threadIdx.x == 0) s_shade.next_item = 0;
   __syncthreads();
   const int px = blockIdx.x * 8 + (lane & 7), py = blockIdx.y * 8 + (lane >> 3);
   shade_block<DT>(A, px, py, (size_t)py * A.W + px, true, &s_shade, &s_sampler);
@@ -702,7 +806,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void t
   __shared__ SamplerLds s_sampler;
   if (blockIdx.x >= list_chunks(A)) return;
   sampler_init(A, &s_sampler);
-  if (threadIdx.x == 0) s_shade.next_job = 0;
+  if (threadIdx.x == 0) s_shade.next_item = 0;
   __syncthreads();
   int px, py;
   size_t oidx;
