@@ -355,6 +355,23 @@ __device__ __forceinline__ GSetup g_setup_of(const GRec& r) {
   return s;
 }
 
+// Early depth rejection: a covered pixel's depth is a convex combination of
+// the vertex depths, computed with b0 + b1 + b2 within 4 * 2^-23 of 1, so it
+// is never below min(z) - 2e-6 (depths lie in [0, 1]).  A pixel whose tile
+// depth is already below that bound keeps its winner whatever this triangle
+// computes, so its exact barycentrics and atomic are skipped.  The key's high
+// word is the depth's bit pattern (order-preserving for depths >= 0; the
+// empty key ~0 never compares below).
+__device__ __forceinline__ uint32_t g_zmin_bits(const GRec& r) {
+  const float zmin = fminf(r.z[0], fminf(r.z[1], r.z[2])) - 2e-6f;
+  return zmin > 0.0f ? __builtin_bit_cast(uint32_t, zmin) : 0u;
+}
+
+__device__ __forceinline__ bool g_hidden(const unsigned long long* s_depth, int li, int lj, uint32_t zmin_bits) {
+  const uint32_t cur = (uint32_t)(s_depth[lj * kTile + li] >> 32);
+  return zmin_bits > cur;
+}
+
 __device__ __forceinline__ void g_depth_test(unsigned long long* s_depth, int li, int lj, float b0, float b1,
                                              float b2, const GRec& r) {
 #pragma clang fp contract(off)
@@ -393,6 +410,7 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
           s_big[atomicAdd(&s_nbig, 1u)] = r;
         } else {
           const GSetup s = g_setup_of(r);
+          const uint32_t zb = g_zmin_bits(r);
           GEdges e;
           g_edges(s, ci0, cj0, e);
           int64_t ra = e.w[0], rb = e.w[1], rc = e.w[2];
@@ -408,7 +426,8 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
               int64_t wa = ra + e.di[0] * k, wb = rb + e.di[1] * k, wc = rc + e.di[2] * k;
               for (int i = lo; i <= hi; ++i) {
                 float bw0, bw1, bw2;
-                if (g_cover_w(s, e, wa, wb, wc, bw0, bw1, bw2))
+                if (wa >= 0 && wb >= 0 && wc >= 0 && !g_hidden(s_depth, i - x0, j - y0, zb) &&
+                    g_cover_w(s, e, wa, wb, wc, bw0, bw1, bw2))
                   g_depth_test(s_depth, i - x0, j - y0, bw0, bw1, bw2, r);
                 wa += e.di[0];
                 wb += e.di[1];
@@ -431,10 +450,12 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
       const int cj0 = max((int)r.j0, y0), cj1 = min((int)r.j1, y0 + kTile - 1);
       const int cw = ci1 - ci0 + 1, npx = cw * (cj1 - cj0 + 1);
       const GSetup s = g_setup_of(r);
+      const uint32_t zb = g_zmin_bits(r);
       for (int q = tid; q < npx; q += 256) {
         const int i = ci0 + q % cw, j = cj0 + q / cw;
         float bw0, bw1, bw2;
-        if (g_cover(s, i, j, bw0, bw1, bw2)) g_depth_test(s_depth, i - x0, j - y0, bw0, bw1, bw2, r);
+        if (!g_hidden(s_depth, i - x0, j - y0, zb) && g_cover(s, i, j, bw0, bw1, bw2))
+          g_depth_test(s_depth, i - x0, j - y0, bw0, bw1, bw2, r);
       }
     }
     __syncthreads();
