@@ -63,6 +63,10 @@ struct VoxParams {
   int average;   // VCT_AVG_*
   uint32_t unused;
   LightSet lights;
+  // normalize(position) of each directional light, computed on the host with
+  // the same IEEE float operations (sqrt, then three divisions), so the bits
+  // equal what every fragment would compute
+  float ldir[VCT_MAX_LIGHTS][3];
 };
 
 struct Setup {
@@ -295,8 +299,7 @@ __device__ __forceinline__ bool shade_w(const Rec& c, const TriRec* sp, const VE
     float dir[3];
     float att;
     if (L.kind == VCT_LIGHT_DIRECTIONAL) {
-      float l = sqrtf(L.pos[0] * L.pos[0] + L.pos[1] * L.pos[1] + L.pos[2] * L.pos[2]);
-      for (int k = 0; k < 3; ++k) dir[k] = L.pos[k] / l;
+      for (int k = 0; k < 3; ++k) dir[k] = vp.ldir[li][k];
       att = 1.0f;
     } else {
       float dv[3];
@@ -659,12 +662,17 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
     __syncthreads();
     const uint32_t total = s_end[kBrickThreads - 1];
     for (uint32_t c = tid; c < total; c += kBrickThreads) {
+      // the triangle of the wave's first candidate by binary search (the same
+      // for every lane), then each lane steps forward over the few triangles
+      // its candidate lies past
+      const uint32_t c0 = c - (uint32_t)lane;
       uint32_t lo = 0, hi = kBrickThreads - 1;
       while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (s_end[mid] > c) hi = mid;
+        if (s_end[mid] > c0) hi = mid;
         else lo = mid + 1;
       }
+      while (s_end[lo] <= c) ++lo;
       const uint32_t local = c - (lo ? s_end[lo - 1] : 0u);
       const uint32_t dv = s_div[lo], cw = dv >> 20;
       const uint32_t row = (local * (dv & 0xfffffu)) >> 16;  // exact: local < 35 * 35, cw <= 35
@@ -903,6 +911,17 @@ static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, cons
   return VCT_OK;
 }
 
+// normalize(position) of the directional lights, operation for operation as
+// the fragment shader computes it (voxelize.frag:127-131), without contraction.
+static void light_dirs(VoxParams& vp) {
+#pragma clang fp contract(off)
+  for (int li = 0; li < VCT_MAX_LIGHTS; ++li) {
+    const DevLight& L = vp.lights.l[li];
+    const float l = sqrtf(L.pos[0] * L.pos[0] + L.pos[1] * L.pos[1] + L.pos[2] * L.pos[2]);
+    for (int k = 0; k < 3; ++k) vp.ldir[li][k] = L.pos[k] / l;
+  }
+}
+
 int vct_launch_voxelize(vct_context* ctx) {
   const uint32_t res = ctx->res;
   const uint64_t nvox = (uint64_t)res * res * res;
@@ -916,6 +935,7 @@ int vct_launch_voxelize(vct_context* ctx) {
   vp.cube_size = ctx->cube_size;
   vp.average = ctx->average;
   vp.lights = make_lights(ctx);
+  light_dirs(vp);
   std::vector<uint32_t> q;
   for (uint32_t id : ctx->queue)
     if (id < ctx->models.size() && ctx->models[id].ntri) q.push_back(id);
