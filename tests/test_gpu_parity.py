@@ -192,3 +192,47 @@ def test_voxel_view_matches_oracle(gpu_available, orc, view_dir, lod):
     err = np.abs(img.reshape(-1, 4) - want).max()
     say(f"  max-abs vs FP32 oracle = {err:.3e}")
     assert err <= 1e-4
+
+
+@pytest.mark.parametrize("dtype", [L.RGBA8, L.FP16, L.BF16, L.FP32])
+@pytest.mark.parametrize("view_dir,lod", [(1, 0.0), (2, 0.6), (4, 2.3)])
+def test_sampler_at_the_volume_faces(gpu_available, orc, dtype, view_dir, lod):
+    """The marcher's sampler (x-paired texels for 8- and 16-bit storage) on a
+    random volume at points spread over the whole cube and up to half a texel
+    past its faces, where the zero border and the first / last columns meet
+    (the voxel view samples exactly one textureLod per pixel)."""
+    say(f"sampler at faces dtype={dtype} dir={view_dir} lod={lod}")
+    W, H, res, levels = 64, 32, 8, 4
+    sc = SceneSpec("cornell", res=res, W=W, H=H)
+    rng = np.random.default_rng(100 + dtype)
+    with Context(W, H) as ctx:
+        sc.build(ctx, dtype, levels=levels)
+        want_levels = {}
+        for lvl in range(levels):
+            s = res >> lvl
+            for d in range(1 if lvl == 0 else 6):
+                f = rng.random((s, s, s, 4)).astype(np.float32)
+                enc = encode_level(f, dtype)
+                ctx.upload_level(lvl, d, enc)
+                want_levels[(lvl, d)] = decode_level(enc, dtype)
+        # biased coordinates in [-0.06, 0.999]: columns -1 and s-1 of every level are hit
+        b = rng.uniform(-0.06, 0.999, (H, W, 3)).astype(np.float32)
+        b[0, :, 0] = -0.5 / res + 1e-3  # the first texel's left border, level 0
+        b[1, :, 0] = 1.0 - 0.5 / res + 1e-3  # past the last texel centre, level 0
+        world = (b - 0.5) * 2.0 * sc.cube_size
+        gp = np.concatenate([world, np.ones((H, W, 1), np.float32)], -1).astype(np.float32)
+        gn = np.zeros((H, W, 4), np.float32)
+        gn[..., 1] = 1.0
+        gn[..., 3] = np.uint32(1).view(np.float32)
+        ctx.upload_gbuffer(gp, gn)
+        ctx.set_voxel_view(view_dir, lod)
+        ctx.trace()
+        img = ctx.download_image()
+    l0 = want_levels[(0, 0)]
+    mips = {(lv, d): want_levels[(lv, d)] for lv in range(1, levels) for d in range(6)}
+    want, _ = orc.trace(np.float32, res, levels, l0, mips, gp, gn, sc.mat_records, sc.lights, sc.cam, sc.cube_size,
+                        cone_set=sc.cone_set, phases=sc.phases, view_dir=view_dir, view_lod=lod)
+    err = np.abs(img.reshape(-1, 4) - want).max()
+    say(f"  max-abs vs FP32 oracle = {err:.3e}")
+    # 16-bit pairs blend with weights rounded to the storage type (2^-11 / 2^-8)
+    assert err <= {L.RGBA8: 1e-5, L.FP16: 2e-3, L.BF16: 1.6e-2, L.FP32: 1e-5}[dtype]

@@ -330,18 +330,21 @@ __device__ __forceinline__ void g_resolve_pixel(const GParams& p, unsigned long 
 //    every pixel of the tile is resolved and written once.
 constexpr int kLaneBox = 128;
 
-// Narrows [lo, hi] to the columns where w + d * (i - i0) >= 0 may hold.  The
-// bound is computed in float with a margin of 1e-3 pixel, so it never drops a
-// pixel the exact integer test would keep.
-__device__ __forceinline__ void g_span(int64_t w, int64_t d, int i0, int& lo, int& hi) {
+// Narrows [lo, hi] to the columns where w + d * (i - i0) >= 0 may hold, with
+// inv_d = 1 / d computed once per triangle (0 when d == 0).  The crossing
+// t = -w / d is bounded in float with a margin of 1e-3 pixel plus 2^-19 |t|
+// (far above the rounding of the conversions, the reciprocal and the
+// product), so it never drops a pixel the exact integer test would keep.
+__device__ __forceinline__ void g_span(int64_t w, int64_t d, float inv_d, int i0, int& lo, int& hi) {
   if (d == 0) {
     if (w < 0) hi = lo - 1;  // the whole row is outside this edge
     return;
   }
-  float t = (float)(-w) / (float)d;  // w(i) = 0 at i = i0 + t
+  float t = (float)(-w) * inv_d;  // w(i) = 0 at i = i0 + t
   t = fminf(fmaxf(t, -4096.0f), 4096.0f);
-  if (d > 0) lo = max(lo, i0 + (int)ceilf(t - 1e-3f));
-  else hi = min(hi, i0 + (int)floorf(t + 1e-3f));
+  const float m = 1e-3f + fabsf(t) * 1.9073486e-6f;
+  if (d > 0) lo = max(lo, i0 + (int)ceilf(t - m));
+  else hi = min(hi, i0 + (int)floorf(t + m));
 }
 
 __device__ __forceinline__ GSetup g_setup_of(const GRec& r) {
@@ -414,13 +417,16 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
           GEdges e;
           g_edges(s, ci0, cj0, e);
           int64_t ra = e.w[0], rb = e.w[1], rc = e.w[2];
+          const float ia = e.di[0] ? 1.0f / (float)e.di[0] : 0.0f;
+          const float ib = e.di[1] ? 1.0f / (float)e.di[1] : 0.0f;
+          const float ic = e.di[2] ? 1.0f / (float)e.di[2] : 0.0f;
           for (int j = cj0; j <= cj1; ++j) {
             // the row's span: for each edge w(i) = w_row + di * (i - ci0) >= 0,
             // bounded in float with a margin (the exact test below decides)
             int lo = ci0, hi = ci1;
-            g_span(ra, e.di[0], ci0, lo, hi);
-            g_span(rb, e.di[1], ci0, lo, hi);
-            g_span(rc, e.di[2], ci0, lo, hi);
+            g_span(ra, e.di[0], ia, ci0, lo, hi);
+            g_span(rb, e.di[1], ib, ci0, lo, hi);
+            g_span(rc, e.di[2], ic, ci0, lo, hi);
             if (lo <= hi) {
               const int64_t k = lo - ci0;
               int64_t wa = ra + e.di[0] * k, wb = rb + e.di[1] * k, wc = rc + e.di[2] * k;

@@ -576,6 +576,8 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
   const float margin = 0.5f * (float)(1u << (A.levels - 1)) * vs;
   const int npx = L->npx;
   const int nitems = npx * L->njobs;
+  // it / npx as (it * ceil(2^20 / npx)) >> 20: exact for it < kMaxJobs * 64 and npx <= 64
+  const uint32_t npx_inv = npx ? (1u << 20) / (uint32_t)npx + (((1u << 20) % (uint32_t)npx) ? 1u : 0u) : 0u;
   uint32_t steps = 0;
   int item = -1, job = 0, pix = 0;
   bool drained = nitems == 0;
@@ -608,7 +610,7 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
         drained = true;
         continue;
       }
-      const int j = it / npx;
+      const int j = (int)(((uint32_t)it * npx_inv) >> 20);
       job = L->jobs[j];
       pix = L->px_of[it - j * npx];
       if (job == kJobRefraction && !((L->refr_mask >> pix) & 1ull)) continue;  // opaque pixel: no item
