@@ -178,6 +178,8 @@ struct VctBuf {
 struct vct_context {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;          // second stream (default priority): the G-buffer beside voxelize + mip
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int width = 0, height = 0;
   unsigned res = 128;
   int levels = 7;
@@ -259,6 +261,7 @@ int vct_ensure(VctBuf& b, size_t bytes, vct_context* ctx);
 // kernels launchers
 int vct_launch_mip(vct_context* ctx);
 int vct_ensure_pairs(vct_context* ctx);  // x-paired copy of the volume for the marcher
+int vct_frame_front(vct_context* ctx);   // voxelize + mip beside the G-buffer, joined
 int vct_launch_voxelize(vct_context* ctx);
 int vct_launch_gbuffer(vct_context* ctx);
 int vct_launch_trace(vct_context* ctx);

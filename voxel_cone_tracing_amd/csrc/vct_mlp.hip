@@ -1026,15 +1026,13 @@ int trace_listed_pixels(vct_context* ctx) {
                           ctx->width, ctx->height, 0, nullptr, sel);
 }
 
-// Per-tile check and refinement of the learned frame (see tile_select_kernel).
-int refine(vct_context* ctx) {
+// Stage-1 probes of every tile, traced at full resolution into probe_val.
+// They need only the G-buffer and the volume, so the learned frame traces
+// them on the second stream beside the quarter-resolution trace and the MLP.
+int refine_probes(vct_context* ctx) {
   const int W = ctx->width, H = ctx->height;
   const int tx = (W + kFbTile - 1) / kFbTile, ty = (H + kFbTile - 1) / kFbTile;
   const uint32_t ntiles = (uint32_t)(tx * ty);
-  const size_t npix = (size_t)W * H;
-  const float4* gpos = (const float4*)ctx->g_pos.p;
-  const float4* gnrm = (const float4*)ctx->g_nrm.p;
-  float4* image = (float4*)ctx->image.p;
   int rc;
   if ((rc = vct_ensure(ctx->probe_val, (size_t)ntiles * kProbes * 16, ctx))) return rc;
   if ((rc = vct_ensure(ctx->fb_list, (size_t)(ntiles + 1) * 4, ctx))) return rc;
@@ -1054,15 +1052,32 @@ int refine(vct_context* ctx) {
     ctx->probe_w = W;
     ctx->probe_h = H;
   }
-  const float max_mse = powf(10.0f, -ctx->fb_psnr_db / 10.0f);
-  uint32_t* flags = (uint32_t*)ctx->fb_flag.p;
-  uint32_t* l1 = (uint32_t*)ctx->fb_list.p;
-  // stage 1: probes of every tile; picked tiles -> l1 (flag 1), others flag 0
   TraceSelect sel;
   memset(&sel, 0, sizeof(sel));
   sel.pix_list = (const uint32_t*)ctx->probe_pix.p;
   sel.npix_list = ntiles * kProbes;
-  if ((rc = vct_trace_select(ctx, gpos, gnrm, (float4*)ctx->probe_val.p, W, H, 0, nullptr, sel))) return rc;
+  return vct_trace_select(ctx, (const float4*)ctx->g_pos.p, (const float4*)ctx->g_nrm.p,
+                          (float4*)ctx->probe_val.p, W, H, 0, nullptr, sel);
+}
+
+// Per-tile check and refinement of the learned frame, after refine_probes
+// (see tile_select_kernel and tile_budget_kernel).
+int refine(vct_context* ctx) {
+  const int W = ctx->width, H = ctx->height;
+  const int tx = (W + kFbTile - 1) / kFbTile, ty = (H + kFbTile - 1) / kFbTile;
+  const uint32_t ntiles = (uint32_t)(tx * ty);
+  const size_t npix = (size_t)W * H;
+  const float4* gpos = (const float4*)ctx->g_pos.p;
+  const float4* gnrm = (const float4*)ctx->g_nrm.p;
+  float4* image = (float4*)ctx->image.p;
+  int rc;
+  if ((rc = vct_ensure(ctx->fb_list, (size_t)(ntiles + 1) * 4, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->fb_flag, (size_t)ntiles * 4, ctx))) return rc;
+  const float max_mse = powf(10.0f, -ctx->fb_psnr_db / 10.0f);
+  uint32_t* flags = (uint32_t*)ctx->fb_flag.p;
+  uint32_t* l1 = (uint32_t*)ctx->fb_list.p;
+  // stage 1: picked tiles -> l1 (flag 1), others flag 0
+  TraceSelect sel;
   if (ctx->fb_half) {  // per-tile bound, then the half-resolution stage
     hipLaunchKernelGGL(tile_select_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const float4*)ctx->probe_val.p,
                        (const uint32_t*)ctx->probe_pix.p, gpos, image, (const uint32_t*)nullptr, ntiles, max_mse, l1,
@@ -1078,7 +1093,8 @@ int refine(vct_context* ctx) {
                        (const uint32_t*)ncov, ntiles, max_mse, l1, flags);
   }
   VCT_HIP(hipGetLastError());
-  // pixels the quarter-resolution stage left, in tiles that passed
+  // pixels the quarter-resolution stage left, in tiles that passed: on the
+  // second stream, beside the tiles traced in full (disjoint pixels)
   memset(&sel, 0, sizeof(sel));
   sel.pix_list = (const uint32_t*)ctx->fb_pix.p + 1;
   sel.pix_count = (const uint32_t*)ctx->fb_pix.p;
@@ -1087,7 +1103,13 @@ int refine(vct_context* ctx) {
   sel.tile_flag = flags;
   sel.flag_want = 0u;
   sel.tiles_x = tx;
-  if ((rc = vct_trace_select(ctx, gpos, gnrm, image, W, H, 0, nullptr, sel))) return rc;
+  VCT_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+  VCT_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+  std::swap(ctx->stream, ctx->side);
+  rc = vct_trace_select(ctx, gpos, gnrm, image, W, H, 0, nullptr, sel);
+  std::swap(ctx->stream, ctx->side);
+  if (rc) return rc;
+  VCT_HIP(hipEventRecord(ctx->ev_join, ctx->side));
   const uint32_t* full_list = l1;
   ctx->fb_half_ran = false;
   if (ctx->fb_half) {
@@ -1145,7 +1167,9 @@ int refine(vct_context* ctx) {
   sel.tile_count = full_list;
   sel.max_tiles = ntiles;
   sel.tiles_x = tx;
-  return vct_trace_select(ctx, gpos, gnrm, image, W, H, 0, nullptr, sel);
+  if ((rc = vct_trace_select(ctx, gpos, gnrm, image, W, H, 0, nullptr, sel))) return rc;
+  VCT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+  return VCT_OK;
 }
 
 }  // namespace
@@ -1349,11 +1373,19 @@ int vct_render_learned(vct_ctx ctx) {
   MLP_CTX(ctx);
   int rc;
   if ((rc = need_mlp(ctx))) return rc;
-  if ((rc = vct_voxelize(ctx))) return rc;
-  if ((rc = vct_mip_aniso(ctx))) return rc;
-  if ((rc = vct_gbuffer(ctx))) return rc;
+  if ((rc = vct_frame_front(ctx))) return rc;
   const size_t npix = (size_t)ctx->width * ctx->height;
   if ((rc = vct_ensure(ctx->image, npix * 16, ctx))) return rc;
+  const bool fallback = ctx->fb_psnr_db > 0.0f;
+  if (fallback) {  // the probes beside the quarter-resolution trace and the MLP
+    VCT_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+    VCT_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+    std::swap(ctx->stream, ctx->side);
+    rc = refine_probes(ctx);
+    std::swap(ctx->stream, ctx->side);
+    if (rc) return rc;
+    VCT_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+  }
   VCT_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
   if ((rc = trace_lowres(ctx))) return rc;
   VCT_HIP(hipEventRecord(ctx->ev[7], ctx->stream));
@@ -1364,7 +1396,8 @@ int vct_render_learned(vct_ctx ctx) {
   ctx->ev_recorded[4] = true;
   VCT_HIP(hipEventRecord(ctx->ev[10], ctx->stream));
   ctx->fb_ran = false;
-  if (ctx->fb_psnr_db > 0.0f) {
+  if (fallback) {
+    VCT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
     if ((rc = refine(ctx))) return rc;
     ctx->fb_ran = true;
   } else if ((rc = trace_listed_pixels(ctx))) {

@@ -253,6 +253,25 @@ static void identity(float* m) {
     if (_se != hipSuccess) return vct_hip_check(_se, "hipSetDevice");           \
   } while (0)
 
+// The passes before the trace of a frame: the G-buffer on the second stream
+// beside voxelize and mip on the context stream (they read the same geometry
+// and write disjoint buffers; each pass is timed on its own stream, so the
+// pass times overlap); the context stream then waits for the G-buffer.
+int vct_frame_front(vct_context* ctx) {
+  int rc;
+  VCT_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+  VCT_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+  std::swap(ctx->stream, ctx->side);
+  rc = vct_gbuffer(ctx);
+  std::swap(ctx->stream, ctx->side);
+  if (rc) return rc;
+  if ((rc = vct_voxelize(ctx))) return rc;
+  VCT_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+  if ((rc = vct_mip_aniso(ctx))) return rc;
+  VCT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+  return VCT_OK;
+}
+
 extern "C" {
 
 const char* vct_last_error(void) { return g_last_error.c_str(); }
@@ -296,16 +315,17 @@ int vct_create(int width, int height, vct_ctx* out) {
   identity(ctx->view);
   identity(ctx->proj);
   e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
   if (e != hipSuccess) {
-    delete ctx;
+    vct_destroy(ctx);
     return vct_hip_check(e, "hipStreamCreate");
   }
-  for (int i = 0; i < 12; ++i) {
-    e = hipEventCreate(&ctx->ev[i]);
-    if (e != hipSuccess) {
-      delete ctx;
-      return vct_hip_check(e, "hipEventCreate");
-    }
+  for (int i = 0; i < 12 && e == hipSuccess; ++i) e = hipEventCreate(&ctx->ev[i]);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    vct_destroy(ctx);
+    return vct_hip_check(e, "hipEventCreate");
   }
   // default material (renderer.cpp:85-98,115-128)
   vct_material dm;
@@ -331,6 +351,7 @@ int vct_create(int width, int height, vct_ctx* out) {
 int vct_destroy(vct_ctx ctx) {
   if (!ctx) return VCT_OK;
   (void)hipSetDevice(ctx->device);
+  if (ctx->side) (void)hipStreamSynchronize(ctx->side);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto& m : ctx->models) {
     if (m.d_pos) (void)hipFree(m.d_pos);
@@ -354,6 +375,9 @@ int vct_destroy(vct_ctx ctx) {
   for (VctBuf* b : bufs) free_buf(*b);
   for (int i = 0; i < 12; ++i)
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+  if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return VCT_OK;
@@ -361,6 +385,7 @@ int vct_destroy(vct_ctx ctx) {
 
 int vct_sync(vct_ctx ctx) {
   CTX_CHECK(ctx);
+  VCT_HIP(hipStreamSynchronize(ctx->side));
   VCT_HIP(hipStreamSynchronize(ctx->stream));
   return VCT_OK;
 }
@@ -662,10 +687,9 @@ int vct_trace(vct_ctx ctx) {
 }
 
 int vct_render(vct_ctx ctx) {
+  CTX_CHECK(ctx);
   int rc;
-  if ((rc = vct_voxelize(ctx))) return rc;
-  if ((rc = vct_mip_aniso(ctx))) return rc;
-  if ((rc = vct_gbuffer(ctx))) return rc;
+  if ((rc = vct_frame_front(ctx))) return rc;
   if ((rc = vct_trace(ctx))) return rc;
   return vct_clear_queues(ctx);
 }
