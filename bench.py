@@ -173,6 +173,22 @@ def copy_calibration(nbytes: int) -> None:
     log(f"bench: calibration copies of {nbytes} bytes done")
 
 
+def sequential_passes(ctx, sc, steps):
+    """Per-pass HIP event times of plain frames whose passes run one after the
+    other on one stream (a frame overlaps the G-buffer with voxelize + mip, so
+    its own pass times overlap)."""
+    passes = np.zeros(6)
+    for _ in range(steps):
+        sc.queue()
+        ctx.voxelize()
+        ctx.mip()
+        ctx.gbuffer()
+        ctx.trace()
+        passes += ctx.pass_times()
+    ctx.clear_queues()
+    return passes / steps
+
+
 def run_form(ctx, sc, learned, steps, warmup):
     def frame():
         sc.queue()
@@ -313,14 +329,16 @@ def main():
         "note": ("effective sample bandwidth: the gathers are served from L1/L2 (see traffic, the HBM bytes per "
                  "launch from rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a copy-kernel calibration)"),
     }
-    # HBM-bound passes of the frame against the HBM peak (algorithmic bytes / pass time)
+    # HBM-bound passes against the HBM peak (algorithmic bytes / pass time), timed
+    # in frames whose passes do not overlap
+    seq = sequential_passes(ctx, sc, max(3, args.steps // 4))
     hbm_passes = {}
     for i, n in ((0, "voxelize"), (1, "mip")):
         amt, _, _, dfn = algorithmic(n, wl, bpe, steps, 0)
-        if plain_passes[i] > 0:
-            gbs = amt / (float(plain_passes[i]) * 1e-3) / 1e9
+        if seq[i] > 0:
+            gbs = amt / (float(seq[i]) * 1e-3) / 1e9
             hbm_passes[n] = {"achieved_GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
-                             "bytes": int(amt), "algorithmic": dfn, "pass_ms": round(float(plain_passes[i]), 4)}
+                             "bytes": int(amt), "algorithmic": dfn, "pass_ms": round(float(seq[i]), 4)}
     # value: the faster form that holds the 40 dB bound (the plain form holds it
     # by construction: it is the full-resolution trace)
     learned_ok = learned is not None and quality.get("learned_vs_plain_full_frame_db", 0.0) >= 40.0 and \
@@ -348,6 +366,7 @@ def main():
         },
         "plain_ms_per_frame": round(plain_ms, 4),
         "plain_passes_ms": {n: round(float(plain_passes[i]), 4) for i, n in enumerate(PASSES[:4])},
+        "plain_passes_sequential_ms": {n: round(float(seq[i]), 4) for i, n in enumerate(PASSES[:4])},
     }
     if learned is not None:
         out["learned_ms_per_frame"] = round(learned["ms"], 4)

@@ -749,10 +749,11 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
 // Work of a launch: the whole viewport (trace_frame_kernel, one workgroup per
 // 8x8 block), or a device-side list of pixels (64 per chunk) or tiles (16
 // chunks of 8x8 pixels per tile, or 4 chunks of 8x8 odd-odd grid points) for
-// trace_list_kernel.  123 VGPRs at 4 waves per SIMD: the 24 loads of a
-// three-map level are in flight together.
+// trace_list_kernel.  5 waves per SIMD (96 VGPRs; a few values of the 16-bit
+// paths spill, which measured 4 % faster than 4 waves without spills): the 12
+// paired loads of a three-map level are in flight together.
 template <int DT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void trace_frame_kernel(TraceArgs A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void trace_frame_kernel(TraceArgs A) {
   __shared__ ShadeLds s_shade;
   __shared__ SamplerLds s_sampler;
   const int lane = threadIdx.x & 63;
@@ -803,7 +804,7 @@ __device__ __forceinline__ void list_chunk(const TraceArgs& A, uint32_t c, int l
 // (A fixed grid striding over the chunks measured slower: the loop costs the
 // kernel registers and spills at 4 waves per SIMD.)
 template <int DT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void trace_list_kernel(TraceArgs A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void trace_list_kernel(TraceArgs A) {
   __shared__ ShadeLds s_shade;
   __shared__ SamplerLds s_sampler;
   if (blockIdx.x >= list_chunks(A)) return;
