@@ -596,7 +596,8 @@ namespace vct {
 // Probe values are written into the image; a probe on a pixel that the
 // interpolation left for tracing (w < 0) counts as exact.
 constexpr int kFbTile = 32;
-constexpr int kProbes = 16;
+constexpr int kProbes = 16;  // per tile: one or two 4x4 grids
+static_assert(kProbes == 16 || kProbes == 32, "probe grids");
 
 // One workgroup.  Checks every tile (in_list null) or the listed tiles,
 // lists the failing ones in tile order and sets flags[t] to fail_flag or
@@ -761,8 +762,9 @@ __global__ void stage2_probes_kernel(const uint32_t* __restrict__ list, int tile
   const uint32_t n = list[0];
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n * kProbes; q += gridDim.x * blockDim.x) {
     const uint32_t t = list[1 + q / kProbes], k = q % kProbes;
-    const int x = min((int)(t % (uint32_t)tiles_x) * kFbTile + 8 * (int)(k & 3u) + 4, W - 1);
-    const int y = min((int)(t / (uint32_t)tiles_x) * kFbTile + 8 * (int)(k >> 2) + 4, H - 1);
+    const int o = (k >> 4) ? 0 : 4;  // 4x4 grids at offsets 4, then 0 (even-even pixels)
+    const int x = min((int)(t % (uint32_t)tiles_x) * kFbTile + 8 * (int)(k & 3u) + o, W - 1);
+    const int y = min((int)(t / (uint32_t)tiles_x) * kFbTile + 8 * (int)((k >> 2) & 3u) + o, H - 1);
     pix[1 + q] = (uint32_t)(y * W + x);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) pix[0] = n * kProbes;
@@ -1041,11 +1043,13 @@ int refine_probes(vct_context* ctx) {
     std::vector<uint32_t> pix((size_t)ntiles * kProbes);
     for (int j = 0; j < ty; ++j)
       for (int i = 0; i < tx; ++i)
-        for (int b = 0; b < 4; ++b)
-          for (int a = 0; a < 4; ++a) {
-            const int x = std::min(i * kFbTile + 8 * a + 3, W - 1), y = std::min(j * kFbTile + 8 * b + 3, H - 1);
-            pix[((size_t)(j * tx + i)) * kProbes + b * 4 + a] = (uint32_t)(y * W + x);
-          }
+        for (int g = 0; g < kProbes / 16; ++g)  // 4x4 grids at offsets 3, then 7
+          for (int b = 0; b < 4; ++b)
+            for (int a = 0; a < 4; ++a) {
+              const int o = g ? 7 : 3;
+              const int x = std::min(i * kFbTile + 8 * a + o, W - 1), y = std::min(j * kFbTile + 8 * b + o, H - 1);
+              pix[((size_t)(j * tx + i)) * kProbes + g * 16 + b * 4 + a] = (uint32_t)(y * W + x);
+            }
     if ((rc = vct_ensure(ctx->probe_pix, pix.size() * 4, ctx))) return rc;
     VCT_HIP(hipMemcpyAsync(ctx->probe_pix.p, pix.data(), pix.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     VCT_HIP(hipStreamSynchronize(ctx->stream));
