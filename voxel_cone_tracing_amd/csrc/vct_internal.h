@@ -29,18 +29,25 @@ namespace vct {
 __device__ __forceinline__ uint32_t wave_add_one(uint32_t* counter, uint32_t key, bool active) {
   const int lane = threadIdx.x & 63;
   unsigned long long pending = __ballot(active);
-  uint32_t slot = 0;
+  // group the active lanes by key (ALU only), then one atomic per group, all
+  // groups' atomics in one instruction: a single memory round trip
+  int my_leader = lane;
+  uint32_t my_rank = 0, my_count = 0;
   while (pending) {
     const int leader = __ffsll(pending) - 1;
     const uint32_t k = __shfl(key, leader, 64);
     const unsigned long long same = __ballot(active && key == k);
-    uint32_t old = 0;
-    if (lane == leader) old = atomicAdd(counter + k, (uint32_t)__popcll(same));
-    old = __shfl(old, leader, 64);
-    if (active && key == k) slot = old + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+    if (active && key == k) {
+      my_leader = leader;
+      my_rank = (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+    }
+    if (lane == leader) my_count = (uint32_t)__popcll(same);
     pending &= ~same;
   }
-  return slot;
+  uint32_t old = 0;
+  if (active && lane == my_leader) old = atomicAdd(counter + key, my_count);
+  old = __shfl(old, my_leader, 64);
+  return old + my_rank;
 }
 #endif
 
