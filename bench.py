@@ -282,6 +282,8 @@ def main():
         learned = dict(ms=l_ms, passes=l_passes, img=l_img, raw=raw_img, tiles=nt, fb=nf, traced=traced, stats=stats)
         log(f"bench: learned {l_ms:.3f} ms/frame; passes " +
             " ".join(f"{n} {l_passes[i]:.3f}" for i, n in enumerate(PASSES)) + f"; fallback tiles {nf}/{nt}")
+    elif args.plain_only:
+        log("bench: plain form only (--plain-only)")
     else:
         log(f"bench: no upsampler parameters at {args.weights}: plain form only")
 

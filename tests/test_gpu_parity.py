@@ -79,6 +79,31 @@ def test_voxelize_reference_average_bit_exact(gpu_available, orc, kind, res, ntr
     assert np.array_equal(got, want["ref"]), f"{np.count_nonzero(got != want['ref'])} voxels differ"
 
 
+def test_voxelize_reference_average_many_models(gpu_available, orc):
+    """37 queued models (more than one batch of 16 draws, so fragments of later
+    batches join voxels filled by earlier ones): the texels equal the oracle's
+    for the same triangles as one list, since the canonical order runs over
+    the queue."""
+    say("voxelize reference average over 37 models")
+    sc = SceneSpec("soup", res=64, ntri=3000)
+    cuts = np.linspace(0, 3000, 38).astype(int)
+    with Context(32, 32) as ctx:
+        sc.build(ctx, L.RGBA8, L.AVG_REFERENCE)
+        ctx.clear_queues()
+        for a, b in zip(cuts[:-1], cuts[1:]):
+            ctx.queue_model(ctx.model_create(sc.pos[a:b], sc.nrm[a:b], sc.tri_mat[a:b]))
+        for (p, c, i, k) in sc.lights:
+            ctx.queue_light(p, c, i, k)
+        for _ in range(2):  # the second frame checks that the per-voxel counts start again from zero
+            ctx.voxelize()
+            got = ctx.download_level(0)
+        nfrag, _ = ctx.voxelize_stats()
+    want = orc.voxelize(sc.res, sc.cube_size, sc.M, sc.pos, sc.nrm, sc.tri_mat, sc.mat_records, sc.lights,
+                        want=("ref", "count"))
+    assert nfrag == want["fragments"]
+    assert np.array_equal(got, want["ref"]), f"{np.count_nonzero(got != want['ref'])} voxels differ"
+
+
 @pytest.mark.parametrize("dtype", [L.FP32, L.FP16, L.BF16, L.RGBA8])
 @pytest.mark.parametrize("kind,res,ntri", [("cornell", 64, 34), ("soup", 128, 20000), ("cornell", 8, 34),
                                            ("soup", 256, 60000)])

@@ -219,8 +219,8 @@ struct vct_context {
   uint64_t pairs_version = 0;  // volume_version the copy was built from
 
   // voxelize scratch
-  VctBuf tri_count, tri_off, tri_base;  // per-triangle candidate counts / offsets
-  VctBuf frags, frags_sorted, vox_count, vox_off, scan_tmp, counters;
+  VctBuf tri_count, tri_off, tri_base;  // per-brick triangle counts + cursors, triangle lists, per-brick offsets
+  VctBuf frags_sorted, vox_count, vox_off, scan_tmp, counters;
   VctBuf setup;  // per-frame triangle list (model index, transform)
   uint64_t last_fragments = 0, last_candidates = 0;
 

@@ -363,7 +363,7 @@ int vct_destroy(vct_ctx ctx) {
   for (auto& t : ctx->textures)
     if (t.dptr) (void)hipFree(t.dptr);
   VctBuf* bufs[] = {&ctx->d_materials, &ctx->volume, &ctx->tri_count, &ctx->tri_off,
-                    &ctx->tri_base, &ctx->frags, &ctx->frags_sorted, &ctx->vox_count, &ctx->vox_off,
+                    &ctx->tri_base, &ctx->frags_sorted, &ctx->vox_count, &ctx->vox_off,
                     &ctx->scan_tmp, &ctx->counters, &ctx->setup, &ctx->g_list, &ctx->g_pos,
                     &ctx->g_nrm, &ctx->image, &ctx->g_count, &ctx->g_off, &ctx->g_recs,
                     &ctx->mlp_params, &ctx->mlp_m1, &ctx->mlp_m2, &ctx->mlp_packed, &ctx->mlp_bias,

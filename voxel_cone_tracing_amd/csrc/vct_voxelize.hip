@@ -9,24 +9,24 @@
 // ivec3(res * (0.5 p + 0.5))).
 //
 // MI355X design
-//  1. count  (one lane per triangle): dominant axis, vertices snapped to
-//     1/256 pixel, pixel bounding box in 2x2 quads -> 4 * quads candidates.
-//  2. exclusive scan of the candidate counts (tri-major canonical order).
-//  3. raster (one lane per candidate pixel, grid-stride over the device-side
-//     total, no host round trip): a workgroup finds the triangle range of its
-//     256 candidates once, each lane binary-searches inside it, recomputes
-//     the triangle setup, tests coverage with exact 64-bit edge functions
-//     (top-left rule) and shades the fragment.  The four lanes of a pixel quad
-//     belong to one triangle and usually one voxel; they are combined in
-//     registers before touching memory (the wave-level collision step).
-//  4a. exact mean (any storage): order-independent 16.16 fixed-point sums
-//     with 64-bit atomics, then a resolve pass per voxel that writes level 0
-//     and re-zeroes the sums for the next frame.
-//  4b. reference running average (RGBA8): fragments are appended, counted
-//     per voxel, bucketed by a scan over the voxels, sorted inside each
-//     bucket by canonical candidate index and replayed through the packed
-//     RGBA8 + count-nibble recurrence of imageAtomicRGBA8Avg
+//  1. setup (one lane per triangle): dominant axis, vertices snapped to
+//     1/256 pixel, pixel bounding box in 2x2 quads, a 128-byte record; each
+//     triangle is counted into the 8^3-voxel bricks its voxel box overlaps.
+//  2. exclusive scan of the brick counts, then per-brick triangle lists.
+//  3. one workgroup per brick: the pixels of its triangles' boxes inside the
+//     brick's footprint are spread over the lanes, coverage is tested with
+//     exact 64-bit edge functions (top-left rule) from records in LDS, and
+//     only fragments whose voxel lies in the brick are shaded, so every
+//     fragment is found exactly once.
+//  4a. exact mean (any storage): 16.16 fixed-point sums in LDS (exact, so
+//     order-independent); the workgroup writes its brick's texels once.
+//  4b. reference running average (RGBA8): a count pass sizes every voxel's
+//     slots (scan over the voxels), a place pass stores (canonical candidate
+//     key, colour) in them and then sorts each voxel's slots and replays the
+//     packed RGBA8 + count-nibble recurrence of imageAtomicRGBA8Avg
 //     (voxelize.frag:95-120) - the result no longer depends on scheduling.
+//     No counter is shared between workgroups (a single append counter for
+//     all fragments serialises at one L2 channel).
 // Bit-exact parts (coverage, voxel coordinates, fragment colours) use no
 // fused multiply-add and correctly rounded division / square root.
 #include <cmath>
@@ -79,8 +79,8 @@ struct Setup {
   bool ok;
 };
 
-// Per-triangle record written once by the count pass and read by every
-// candidate lane of the triangle (the lanes of one triangle share its lines).
+// Per-triangle record written once by the setup pass and read by the brick
+// workgroups (coverage fields through LDS, the rest for covered pixels).
 struct alignas(16) TriRec {
   int64_t area;     // doubled signed area in 1/256 pixel units, made positive
   int32_t X[3], Y[3];
@@ -197,25 +197,8 @@ __device__ __forceinline__ TriRec make_rec(const Draw& dr, uint32_t lt, const Se
   return r;
 }
 
-__global__ __launch_bounds__(256) void vox_count_kernel(DrawBatch b, VoxParams vp, uint32_t* counts,
-                                                        TriRec* __restrict__ recs) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= b.ntri) return;
-  const int d = find_draw(b, t);
-  const Draw& dr = b.d[d];
-  const uint32_t lt = t - dr.tri_base;
-  Setup s;
-  tri_setup(dr, lt, vp.res, vp.cube_size, s);
-  counts[t] = s.ok ? (uint32_t)(4 * s.qw * s.qh) : 0u;
-  if (s.ok) recs[t] = make_rec(dr, lt, s);
-}
-
 __device__ __forceinline__ bool top_left(int64_t dx, int64_t dy) { return dy < 0 || (dy == 0 && dx < 0); }
 
-// Fragment at pixel (i, j) of the triangle's 2res x 2res raster: coverage,
-// voxel (Morton index) and colour.  Returns false if the pixel is not covered
-// or the voxel falls outside the grid (with kBrick: also when the voxel's
-// Morton index >> bshift is not bwant, before any shading).
 // Edge functions of a record in CCW order (a, b, c) = (0, 1, 2) or (0, 2, 1)
 // at pixel (i, j), and their exact per-column / per-row increments.
 struct VEdges {
@@ -251,10 +234,10 @@ __device__ __forceinline__ void vox_edges(const Rec& s, int i, int j, VEdges& e)
 // edge values (wa, wb, wc): coverage, voxel (Morton index) and colour.
 // Returns false if the pixel is not covered or the voxel falls outside the
 // grid (with kBrick: also when the voxel's Morton index >> bshift is not
-// bwant, before any shading).
+// bwant, before any shading).  kKeyOnly stops once the voxel is known.
 // `c` holds the coverage fields (X, Y, area, flags), `sp` the whole record
 // (read only for covered pixels).
-template <bool kBrick = false, typename Rec>
+template <bool kBrick = false, bool kKeyOnly = false, typename Rec>
 __device__ __forceinline__ bool shade_w(const Rec& c, const TriRec* sp, const VEdges& e, int i, int j, int64_t wa,
                                         int64_t wb, int64_t wc, const VoxParams& vp, const DevMaterial* mats,
                                         uint32_t& key, float4& color, uint32_t bshift = 0, uint32_t bwant = 0) {
@@ -283,6 +266,7 @@ __device__ __forceinline__ bool shade_w(const Rec& c, const TriRec* sp, const VE
   }
   key = brick_index((uint32_t)v[0], (uint32_t)v[1], (uint32_t)v[2]);
   if (kBrick && (key >> bshift) != bwant) return false;
+  if (kKeyOnly) return true;
   // normal: per-vertex normals (record), interpolated, normalised in the
   // fragment shader (voxelize.frag:134)
   float n[3];
@@ -333,94 +317,17 @@ __device__ __forceinline__ bool shade_w(const Rec& c, const TriRec* sp, const VE
   return true;
 }
 
-// Fragment at pixel (i, j) (edge values computed directly).
-template <bool kBrick = false>
-__device__ __forceinline__ bool shade_pixel(const TriRec& s, int i, int j, const VoxParams& vp,
-                                            const DevMaterial* mats, uint32_t& key, float4& color,
-                                            uint32_t bshift = 0, uint32_t bwant = 0) {
-  VEdges e;
-  vox_edges(s, i, j, e);
-  return shade_w<kBrick>(s, &s, e, i, j, e.w[0], e.w[1], e.w[2], vp, mats, key, color, bshift, bwant);
-}
-
-// Candidate `lc` of the triangle's quad-ordered bounding box (the canonical
-// candidate order of the reference running average).
-__device__ __forceinline__ bool shade_fragment(const TriRec& s, uint32_t lc, const VoxParams& vp,
-                                               const DevMaterial* mats, uint32_t& key, float4& color) {
-  const uint32_t quad = lc >> 2, sub = lc & 3u;
-  const int i = s.qx0 + 2 * (int)(quad % (uint32_t)s.qw) + (int)(sub & 1u);
-  const int j = s.qy0 + 2 * (int)(quad / (uint32_t)s.qw) + (int)(sub >> 1);
-  return shade_pixel(s, i, j, vp, mats, key, color);
-}
-
-// Output of the reference-mode raster: appended fragments + per-voxel counts.
-struct RasterOut {
-  uint32_t* vcount;  // fragments per voxel
-  uint32_t* f_key;
-  unsigned long long* f_cand;  // canonical key: original triangle << 32 | candidate
-  float4* f_color;
-  uint32_t* f_counter;
-  uint32_t f_cap;
+// Buffers of the reference running average's two brick passes.
+struct RefPass {
+  uint32_t* vcount;              // count pass: fragments per voxel; place pass: fragments placed so far
+  const uint32_t* vox_off;       // first slot of each voxel's fragments (exclusive scan of the counts)
+  unsigned long long* s_cand;    // canonical key per slot: original triangle << 32 | candidate
+  float4* s_color;               // colour per slot
+  uint32_t nfrag;                // slots
+  uint32_t nvox;                 // voxels of level 0
+  uint32_t replay;               // place pass: replay the brick's voxels into level 0 afterwards
+  uint32_t carry;                // place pass: earlier batches placed fragments, counted in vcount
 };
-
-__global__ __launch_bounds__(256) void vox_raster_kernel(uint32_t ntri, VoxParams vp, const uint32_t* __restrict__ off,
-                                                         const uint32_t* __restrict__ d_total,
-                                                         const TriRec* __restrict__ recs,
-                                                         const DevMaterial* __restrict__ mats, RasterOut o) {
-  __shared__ uint32_t s_range[2];
-  const uint32_t total = *d_total;
-  const uint32_t n = ntri;
-  const int lane = threadIdx.x & 63;
-  for (uint32_t base = blockIdx.x * 256u; base < total; base += gridDim.x * 256u) {
-    const uint32_t last = min(base + 255u, total - 1u);
-    if (threadIdx.x == 0 || threadIdx.x == 255) {
-      const uint32_t key = threadIdx.x == 0 ? base : last;
-      uint32_t lo = 0, hi = n;  // largest t with off[t] <= key
-      while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (off[mid] <= key) lo = mid;
-        else hi = mid;
-      }
-      s_range[threadIdx.x == 0 ? 0 : 1] = lo;
-    }
-    __syncthreads();
-    const uint32_t c = base + threadIdx.x;
-    bool have = false;
-    uint32_t key = 0xffffffffu;
-    unsigned long long ckey = 0ull;
-    float4 color = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (c < total) {
-      uint32_t lo = s_range[0], hi = s_range[1] + 1;
-      while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (off[mid] <= c) lo = mid;
-        else hi = mid;
-      }
-      const uint32_t t = lo;
-      const TriRec r = recs[t];
-      have = shade_fragment(r, c - off[t], vp, mats, key, color);
-      ckey = ((unsigned long long)r.otri << 32) | (unsigned long long)(c - off[t]);
-      if (!have) key = 0xffffffffu;
-    }
-    {
-      // append (wave-aggregated slot allocation), count per voxel
-      const unsigned long long m = __ballot(have);
-      uint32_t slot0 = 0;
-      if (lane == 0 && m) slot0 = atomicAdd(o.f_counter, (uint32_t)__popcll(m));
-      slot0 = __shfl(slot0, 0, 64);
-      if (have) {
-        const uint32_t slot = slot0 + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        if (slot < o.f_cap) {
-          o.f_key[slot] = key;
-          o.f_cand[slot] = ckey;
-          o.f_color[slot] = color;
-        }
-        atomicAdd(&o.vcount[key], 1u);
-      }
-    }
-    __syncthreads();
-  }
-}
 
 // ---- exact mean: bricks of the grid accumulated in LDS ----------------------
 // The grid is cut into cubes of kBrick^3 voxels (the whole grid when smaller);
@@ -592,16 +499,91 @@ __device__ __forceinline__ void brick_add(uint32_t (*s_sum)[kBrick * kBrick * kB
   atomicAdd(&s_cnt[v], 1u);
 }
 
-template <int DT>
+// ---- reference running average -------------------------------------------
+__device__ __forceinline__ uint32_t conv_vec4_to_rgba8(float x, float y, float z, float w) {
+  // voxelize.frag:66-71: uint() truncation, & 0xFF
+  return (((uint32_t)w & 0xffu) << 24) | (((uint32_t)z & 0xffu) << 16) | (((uint32_t)y & 0xffu) << 8) |
+         ((uint32_t)x & 0xffu);
+}
+__device__ __forceinline__ uint32_t enc_nibble(uint32_t m, uint32_t n) {
+  return (m & 0xFEFEFEFEu) | (n & 1u) | ((n & 2u) << 7) | ((n & 4u) << 14) | ((n & 8u) << 21);
+}
+__device__ __forceinline__ uint32_t dec_nibble(uint32_t m) {
+  return (m & 1u) | ((m & 0x100u) >> 7) | ((m & 0x10000u) >> 14) | ((m & 0x1000000u) >> 21);
+}
+
+// Level-0 texel of one voxel from its fragments in slots [beg, end): sorted
+// into canonical order (insertion sort by candidate key), then the packed
+// RGBA8 + count-nibble recurrence of imageAtomicRGBA8Avg, one fragment after
+// another (voxelize.frag:95-120).
+__device__ __forceinline__ uint32_t replay_voxel(unsigned long long* __restrict__ s_cand,
+                                                 float4* __restrict__ s_color, uint32_t beg, uint32_t end) {
+#pragma clang fp contract(off)
+  // canonical order: insertion sort of the bucket by candidate index
+  for (uint32_t i = beg + 1; i < end; ++i) {
+    const unsigned long long kc = s_cand[i];
+    const float4 kv = s_color[i];
+    uint32_t j = i;
+    while (j > beg && s_cand[j - 1] > kc) {
+      s_cand[j] = s_cand[j - 1];
+      s_color[j] = s_color[j - 1];
+      --j;
+    }
+    s_cand[j] = kc;
+    s_color[j] = kv;
+  }
+  uint32_t stored = 0u;
+  for (uint32_t i = beg; i < end; ++i) {
+    const float4 c = s_color[i];
+    const float vx = c.x * 255.0f, vy = c.y * 255.0f, vz = c.z * 255.0f, vw = c.w * 255.0f;
+    if (stored == 0u) {  // imageAtomicCompSwap(0 -> new) succeeds
+      stored = enc_nibble(conv_vec4_to_rgba8(vx, vy, vz, vw), 1u);
+      continue;
+    }
+    const uint32_t cur = stored & 0xFEFEFEFEu;
+    float rx = (float)(cur & 0xffu), ry = (float)((cur >> 8) & 0xffu), rz = (float)((cur >> 16) & 0xffu),
+          rw = (float)(cur >> 24);
+    uint32_t n = dec_nibble(stored);
+    const float fn = (float)n;
+    rx = rx * fn + vx;
+    ry = ry * fn + vy;
+    rz = rz * fn + vz;
+    rw = rw * fn + vw;
+    ++n;
+    const float fn1 = (float)n;
+    rx = rx / fn1;
+    ry = ry / fn1;
+    rz = rz / fn1;
+    rw = rw / fn1;
+    rx = __builtin_rintf(rx / 2.0f) * 2.0f;
+    ry = __builtin_rintf(ry / 2.0f) * 2.0f;
+    rz = __builtin_rintf(rz / 2.0f) * 2.0f;
+    rw = __builtin_rintf(rw / 2.0f) * 2.0f;
+    stored = enc_nibble(conv_vec4_to_rgba8(rx, ry, rz, rw), n);
+  }
+  return stored;
+}
+
+// kMode: kExactMean sums the brick's fragments and writes its texels.  The
+// reference running average takes two passes over the same candidates:
+// kRefCount counts the fragments per voxel (into r.vcount), and after the
+// counts are scanned into r.vox_off, kRefPlace shades them and stores
+// (canonical key, colour) in the voxel's slots, in no particular order.  With
+// r.replay (the last batch of draws, when every earlier fragment is placed)
+// the workgroup then replays its brick's voxels and writes their texels.  No
+// global counter is shared between workgroups.
+enum { kExactMean = 0, kRefCount = 1, kRefPlace = 2 };
+template <int DT, int kMode>
 __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, const TriRec* __restrict__ recs,
                                                                   const DevMaterial* __restrict__ mats,
                                                                   const uint32_t* __restrict__ brick_count,
                                                                   const uint32_t* __restrict__ brick_off,
                                                                   const uint32_t* __restrict__ list,
-                                                                  void* level0) {
+                                                                  void* level0, RefPass r) {
   constexpr int kVox = kBrick * kBrick * kBrick;
   __shared__ uint32_t s_sum[4][kVox];
   __shared__ uint32_t s_cnt[kVox];
+  __shared__ uint32_t s_base[kVox];
   __shared__ CovRec s_cov[kBrickThreads];
   __shared__ uint32_t s_end[kBrickThreads];  // inclusive prefix of candidate counts
   __shared__ uint32_t s_org[kBrickThreads];  // clipped box origin: i | j << 16
@@ -614,10 +596,14 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
   const uint32_t bshift = 3 * g.bits;
   const uint32_t bc[3] = {compact3(brick), compact3(brick >> 1), compact3(brick >> 2)};
   for (uint32_t v = tid; v < nvox; v += kBrickThreads) {
-    s_sum[0][v] = 0u;
-    s_sum[1][v] = 0u;
-    s_sum[2][v] = 0u;
-    s_sum[3][v] = 0u;
+    if constexpr (kMode == kExactMean) {
+      s_sum[0][v] = 0u;
+      s_sum[1][v] = 0u;
+      s_sum[2][v] = 0u;
+      s_sum[3][v] = 0u;
+    }
+    if constexpr (kMode == kRefPlace)
+      s_base[v] = r.vox_off[brick * nvox + v] + (r.carry ? r.vcount[brick * nvox + v] : 0u);
     s_cnt[v] = 0u;
   }
   const uint32_t n = brick_count[brick], base = brick_off[brick];
@@ -688,98 +674,46 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
       vox_edges(cr, i, j, e);
       uint32_t key;
       float4 color;
-      if (shade_w<true>(cr, recs + cr.tri, e, i, j, e.w[0], e.w[1], e.w[2], vp, mats, key, color, bshift, brick))
-        brick_add(s_sum, s_cnt, key & (nvox - 1u), color);
+      if (!shade_w<true, kMode == kRefCount>(cr, recs + cr.tri, e, i, j, e.w[0], e.w[1], e.w[2], vp, mats, key,
+                                             color, bshift, brick))
+        continue;
+      const uint32_t lv = key & (nvox - 1u);
+      if constexpr (kMode == kExactMean) {
+        brick_add(s_sum, s_cnt, lv, color);
+      } else if constexpr (kMode == kRefCount) {
+        atomicAdd(&s_cnt[lv], 1u);
+      } else {
+        // canonical key: the candidate's place in the triangle's quad-ordered
+        // box (2x2 quads row by row, each quad x then y)
+        const TriRec* rp = recs + cr.tri;
+        const uint32_t dx = (uint32_t)(i - rp->qx0), dy = (uint32_t)(j - rp->qy0);
+        const uint32_t lc = (((dy >> 1) * (uint32_t)rp->qw + (dx >> 1)) << 2) | ((dy & 1u) << 1) | (dx & 1u);
+        const uint32_t slot = s_base[lv] + atomicAdd(&s_cnt[lv], 1u);
+        if (slot < r.nfrag) {  // always: the count pass found the same fragments
+          r.s_cand[slot] = ((unsigned long long)rp->otri << 32) | (unsigned long long)lc;
+          r.s_color[slot] = color;
+        }
+      }
     }
     __syncthreads();
   }
-  for (uint32_t v = tid; v < nvox; v += kBrickThreads)
-    store_mean<DT>(level0, brick * nvox + v, s_cnt[v], s_sum[0][v], s_sum[1][v], s_sum[2][v], s_sum[3][v]);
-}
-
-// ---- reference running average -------------------------------------------
-__device__ __forceinline__ uint32_t conv_vec4_to_rgba8(float x, float y, float z, float w) {
-  // voxelize.frag:66-71: uint() truncation, & 0xFF
-  return (((uint32_t)w & 0xffu) << 24) | (((uint32_t)z & 0xffu) << 16) | (((uint32_t)y & 0xffu) << 8) |
-         ((uint32_t)x & 0xffu);
-}
-__device__ __forceinline__ uint32_t enc_nibble(uint32_t m, uint32_t n) {
-  return (m & 0xFEFEFEFEu) | (n & 1u) | ((n & 2u) << 7) | ((n & 4u) << 14) | ((n & 8u) << 21);
-}
-__device__ __forceinline__ uint32_t dec_nibble(uint32_t m) {
-  return (m & 1u) | ((m & 0x100u) >> 7) | ((m & 0x10000u) >> 14) | ((m & 0x1000000u) >> 21);
-}
-
-__global__ __launch_bounds__(256) void vox_bucket_kernel(const uint32_t* __restrict__ f_key,
-                                                         const unsigned long long* __restrict__ f_cand,
-                                                         const float4* __restrict__ f_color,
-                                                         const uint32_t* __restrict__ d_nfrag,
-                                                         const uint32_t* __restrict__ vox_off,
-                                                         uint32_t* __restrict__ vcount,
-                                                         unsigned long long* __restrict__ s_cand,
-                                                         float4* __restrict__ s_color) {
-  const uint32_t nf = *d_nfrag;
-  for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
-    const uint32_t k = f_key[f];
-    const uint32_t pos = vox_off[k] + atomicSub(&vcount[k], 1u) - 1u;
-    s_cand[pos] = f_cand[f];
-    s_color[pos] = f_color[f];
-  }
-}
-
-__global__ __launch_bounds__(256) void vox_replay_kernel(const uint32_t* __restrict__ vox_off,
-                                                         const uint32_t* __restrict__ d_nfrag,
-                                                         unsigned long long* __restrict__ s_cand,
-                                                         float4* __restrict__ s_color,
-                                                         uint32_t* __restrict__ level0, uint32_t nvox) {
-#pragma clang fp contract(off)
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nvox) return;
-  const uint32_t beg = vox_off[v];
-  const uint32_t end = (v + 1 < nvox) ? vox_off[v + 1] : *d_nfrag;
-  // canonical order: insertion sort of the bucket by candidate index
-  for (uint32_t i = beg + 1; i < end; ++i) {
-    const unsigned long long kc = s_cand[i];
-    const float4 kv = s_color[i];
-    uint32_t j = i;
-    while (j > beg && s_cand[j - 1] > kc) {
-      s_cand[j] = s_cand[j - 1];
-      s_color[j] = s_color[j - 1];
-      --j;
+  if constexpr (kMode == kExactMean) {
+    for (uint32_t v = tid; v < nvox; v += kBrickThreads)
+      store_mean<DT>(level0, brick * nvox + v, s_cnt[v], s_sum[0][v], s_sum[1][v], s_sum[2][v], s_sum[3][v]);
+  } else if (kMode == kRefCount || !r.replay) {
+    // one workgroup per brick per launch: a plain read-modify-write suffices
+    for (uint32_t v = tid; v < nvox; v += kBrickThreads)
+      if (s_cnt[v]) r.vcount[brick * nvox + v] += s_cnt[v];
+  } else {
+    // the barrier makes the workgroup's slot stores visible to all its lanes
+    // (the loop above ended with one)
+    for (uint32_t v = tid; v < nvox; v += kBrickThreads) {
+      const uint32_t gv = brick * nvox + v;
+      const uint32_t beg = r.vox_off[gv];
+      const uint32_t end = gv + 1u < r.nvox ? r.vox_off[gv + 1u] : r.nfrag;
+      reinterpret_cast<uint32_t*>(level0)[gv] = replay_voxel(r.s_cand, r.s_color, beg, end);
     }
-    s_cand[j] = kc;
-    s_color[j] = kv;
   }
-  uint32_t stored = 0u;
-  for (uint32_t i = beg; i < end; ++i) {
-    const float4 c = s_color[i];
-    const float vx = c.x * 255.0f, vy = c.y * 255.0f, vz = c.z * 255.0f, vw = c.w * 255.0f;
-    if (stored == 0u) {  // imageAtomicCompSwap(0 -> new) succeeds
-      stored = enc_nibble(conv_vec4_to_rgba8(vx, vy, vz, vw), 1u);
-      continue;
-    }
-    const uint32_t cur = stored & 0xFEFEFEFEu;
-    float rx = (float)(cur & 0xffu), ry = (float)((cur >> 8) & 0xffu), rz = (float)((cur >> 16) & 0xffu),
-          rw = (float)(cur >> 24);
-    uint32_t n = dec_nibble(stored);
-    const float fn = (float)n;
-    rx = rx * fn + vx;
-    ry = ry * fn + vy;
-    rz = rz * fn + vz;
-    rw = rw * fn + vw;
-    ++n;
-    const float fn1 = (float)n;
-    rx = rx / fn1;
-    ry = ry / fn1;
-    rz = rz / fn1;
-    rw = rw / fn1;
-    rx = __builtin_rintf(rx / 2.0f) * 2.0f;
-    ry = __builtin_rintf(ry / 2.0f) * 2.0f;
-    rz = __builtin_rintf(rz / 2.0f) * 2.0f;
-    rw = __builtin_rintf(rw / 2.0f) * 2.0f;
-    stored = enc_nibble(conv_vec4_to_rgba8(rx, ry, rz, rw), n);
-  }
-  level0[v] = stored;
 }
 
 __global__ void zero_u32_kernel(uint32_t* p, uint64_t n) {
@@ -852,11 +786,12 @@ static void fill_batch(vct_context* ctx, const std::vector<uint32_t>& q, size_t 
   b.ntri = ntri;
 }
 
-template <int DT>
+template <int DT, int kMode = kExactMean>
 static void launch_brick(vct_context* ctx, uint32_t nbricks, const VoxParams& vp, const uint32_t* count,
-                         const uint32_t* off, const uint32_t* list, void* level0) {
-  hipLaunchKernelGGL(vox_brick_kernel<DT>, dim3(nbricks), dim3(kBrickThreads), 0, ctx->stream, vp,
-                     (const TriRec*)ctx->setup.p, (const DevMaterial*)ctx->d_materials.p, count, off, list, level0);
+                         const uint32_t* off, const uint32_t* list, void* level0, const RefPass& r = RefPass{}) {
+  hipLaunchKernelGGL((vox_brick_kernel<DT, kMode>), dim3(nbricks), dim3(kBrickThreads), 0, ctx->stream, vp,
+                     (const TriRec*)ctx->setup.p, (const DevMaterial*)ctx->d_materials.p, count, off, list, level0,
+                     r);
 }
 
 // Exact mean: all queued draws in one batch (brick lists span every draw).
@@ -915,6 +850,114 @@ static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, cons
   return VCT_OK;
 }
 
+// Brick lists of one batch of draws for the reference running average:
+// records, per-brick triangle counts (bcount), offsets (boff) and lists
+// (tri_off).  Returns the number of (triangle, brick) pairs in `pairs`.
+static int ref_batch_lists(vct_context* ctx, const DrawBatch& b, const VoxParams& vp, uint32_t nbricks,
+                           uint32_t& pairs) {
+  uint32_t* counters = (uint32_t*)ctx->counters.p;
+  uint32_t* bcount = (uint32_t*)ctx->tri_count.p;
+  uint32_t* cursor = bcount + nbricks;
+  uint32_t* boff = (uint32_t*)ctx->tri_base.p;
+  const uint32_t ntri = b.ntri;
+  int rc;
+  pairs = 0;
+  if ((rc = vct_ensure(ctx->setup, (size_t)(ntri ? ntri : 1) * sizeof(TriRec), ctx))) return rc;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3((2 * nbricks + 255) / 256), dim3(256), 0, ctx->stream, bcount,
+                     (uint64_t)2 * nbricks);
+  VCT_HIP(hipGetLastError());
+  if (!ntri) return VCT_OK;
+  hipLaunchKernelGGL(vox_brick_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
+                     (TriRec*)ctx->setup.p, bcount);
+  VCT_HIP(hipGetLastError());
+  if ((rc = vct_launch_scan_u32(ctx, bcount, boff, nbricks, counters + 2))) return rc;
+  VCT_HIP(hipMemcpyAsync(&pairs, counters + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
+  VCT_HIP(hipStreamSynchronize(ctx->stream));
+  if (!pairs) return VCT_OK;
+  if ((rc = vct_ensure(ctx->tri_off, (size_t)pairs * 4 + (size_t)pairs / 4 * 4, ctx))) return rc;
+  hipLaunchKernelGGL(vox_brick_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
+                     (const uint32_t*)boff, cursor, (uint32_t*)ctx->tri_off.p);
+  VCT_HIP(hipGetLastError());
+  return VCT_OK;
+}
+
+// Reference running average: the brick pipeline of the exact mean finds the
+// same fragments (each covered pixel of a triangle once, in the brick of its
+// voxel).  A count pass adds them up per voxel, a scan gives every voxel its
+// slots, a place pass stores (canonical key, colour) in them, and the replay
+// sorts each voxel's slots and runs the RGBA8 recurrence.  Batches of more
+// than kMaxDraws draws rebuild their brick lists for the place pass.
+static int voxelize_reference(vct_context* ctx, const std::vector<uint32_t>& q, const VoxParams& vp, void* level0) {
+  const uint32_t res = ctx->res;
+  const uint64_t nvox = (uint64_t)res * res * res;
+  const uint32_t edge = res < (uint32_t)kBrick ? res : (uint32_t)kBrick;
+  const uint32_t per = res / edge;
+  const uint32_t nbricks = per * per * per;
+  uint32_t* counters = (uint32_t*)ctx->counters.p;  // [1] fragments, [2] scan totals
+  int rc;
+  if ((rc = vct_ensure(ctx->vox_count, nvox * 4, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->vox_off, nvox * 4, ctx))) return rc;
+  // per-brick counts + cursors, per-brick offsets
+  if ((rc = vct_ensure(ctx->tri_count, (size_t)nbricks * 8 + 16, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->tri_base, (size_t)nbricks * 4 + 16, ctx))) return rc;
+  uint32_t* vcount = (uint32_t*)ctx->vox_count.p;
+  const uint32_t* bcount = (const uint32_t*)ctx->tri_count.p;
+  const uint32_t* boff = (const uint32_t*)ctx->tri_base.p;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(2048), dim3(256), 0, ctx->stream, vcount, nvox);
+  VCT_HIP(hipGetLastError());
+  const size_t nbatch = (q.size() + kMaxDraws - 1) / kMaxDraws;
+  RefPass r;
+  memset(&r, 0, sizeof(r));
+  r.vcount = vcount;
+  uint32_t canon = 0, pairs = 0;
+  for (size_t qi = 0; qi < q.size(); qi += kMaxDraws) {
+    DrawBatch b;
+    fill_batch(ctx, q, qi, canon, b);
+    if ((rc = ref_batch_lists(ctx, b, vp, nbricks, pairs))) return rc;
+    ctx->last_candidates += pairs;
+    if (pairs)
+      launch_brick<VCT_RGBA8, kRefCount>(ctx, nbricks, vp, bcount, boff, (const uint32_t*)ctx->tri_off.p, nullptr, r);
+    VCT_HIP(hipGetLastError());
+  }
+  if ((rc = vct_launch_scan_u32(ctx, vcount, (uint32_t*)ctx->vox_off.p, (uint32_t)nvox, counters + 1))) return rc;
+  uint32_t nfrag = 0;
+  VCT_HIP(hipMemcpyAsync(&nfrag, counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+  VCT_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->last_fragments = nfrag;
+  const size_t ns = ((size_t)nfrag + 4) & ~(size_t)3;
+  if ((rc = vct_ensure(ctx->frags_sorted, ns * 24, ctx))) return rc;
+  r.vox_off = (const uint32_t*)ctx->vox_off.p;
+  r.s_cand = (unsigned long long*)ctx->frags_sorted.p;
+  r.s_color = (float4*)((char*)ctx->frags_sorted.p + ns * 8);
+  r.nfrag = nfrag;
+  r.nvox = (uint32_t)nvox;
+  if (!nfrag) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(2048), dim3(256), 0, ctx->stream, (uint32_t*)level0, nvox);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+  }
+  if (nbatch > 1) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(2048), dim3(256), 0, ctx->stream, vcount, nvox);
+    VCT_HIP(hipGetLastError());
+  }
+  canon = 0;
+  for (size_t qi = 0; qi < q.size(); qi += kMaxDraws) {
+    if (nbatch > 1) {
+      DrawBatch b;
+      fill_batch(ctx, q, qi, canon, b);
+      if ((rc = ref_batch_lists(ctx, b, vp, nbricks, pairs))) return rc;
+    }
+    // the last batch runs even without pairs: its workgroups write level 0
+    r.replay = qi + kMaxDraws >= q.size() ? 1u : 0u;
+    r.carry = qi > 0 ? 1u : 0u;
+    if (pairs || r.replay)
+      launch_brick<VCT_RGBA8, kRefPlace>(ctx, nbricks, vp, bcount, boff, (const uint32_t*)ctx->tri_off.p, level0,
+                                         r);
+    VCT_HIP(hipGetLastError());
+  }
+  return VCT_OK;
+}
+
 // normalize(position) of the directional lights, operation for operation as
 // the fragment shader computes it (voxelize.frag:127-131), without contraction.
 static void light_dirs(VoxParams& vp) {
@@ -928,11 +971,9 @@ static void light_dirs(VoxParams& vp) {
 
 int vct_launch_voxelize(vct_context* ctx) {
   const uint32_t res = ctx->res;
-  const uint64_t nvox = (uint64_t)res * res * res;
   void* level0 = (char*)ctx->volume.p + ctx->lvl_off[0][0];
   int rc;
   if ((rc = vct_ensure(ctx->counters, 256, ctx))) return rc;
-  uint32_t* counters = (uint32_t*)ctx->counters.p;  // [0] candidates total, [1] fragments, [2] scan totals
   VoxParams vp;
   memset(&vp, 0, sizeof(vp));
   vp.res = res;
@@ -946,93 +987,5 @@ int vct_launch_voxelize(vct_context* ctx) {
   ctx->last_candidates = 0;
   ctx->last_fragments = 0;
   if (ctx->average == VCT_AVG_EXACT) return voxelize_exact(ctx, q, vp, level0);
-
-  // reference running average: append, bucket per voxel, canonical replay
-  if ((rc = vct_ensure(ctx->vox_count, nvox * 4, ctx))) return rc;
-  if ((rc = vct_ensure(ctx->vox_off, nvox * 4, ctx))) return rc;
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(2048), dim3(256), 0, ctx->stream, (uint32_t*)ctx->vox_count.p, nvox);
-  VCT_HIP(hipMemsetAsync(counters + 1, 0, 4, ctx->stream));
-  uint64_t cand_total = 0;
-  uint32_t canon = 0;
-  for (size_t qi = 0; qi < q.size(); qi += kMaxDraws) {
-    DrawBatch b;
-    fill_batch(ctx, q, qi, canon, b);
-    const uint32_t ntri = b.ntri;
-    if ((rc = vct_ensure(ctx->tri_count, (size_t)ntri * 4 + 4, ctx))) return rc;
-    if ((rc = vct_ensure(ctx->tri_off, (size_t)ntri * 4 + 4, ctx))) return rc;
-    if ((rc = vct_ensure(ctx->setup, (size_t)ntri * sizeof(TriRec), ctx))) return rc;
-    hipLaunchKernelGGL(vox_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
-                       (uint32_t*)ctx->tri_count.p, (TriRec*)ctx->setup.p);
-    VCT_HIP(hipGetLastError());
-    if ((rc = vct_launch_scan_u32(ctx, (uint32_t*)ctx->tri_count.p, (uint32_t*)ctx->tri_off.p, ntri, counters)))
-      return rc;
-    RasterOut o;
-    memset(&o, 0, sizeof(o));
-    o.vcount = (uint32_t*)ctx->vox_count.p;
-    // candidates bound the fragment count: read it back to size the buffers
-    uint32_t total = 0;
-    VCT_HIP(hipMemcpyAsync(&total, counters, 4, hipMemcpyDeviceToHost, ctx->stream));
-    VCT_HIP(hipStreamSynchronize(ctx->stream));
-    if (cand_total + total > 0xffffffffull) return vct_set_error(VCT_ERR_LIMIT, "more than 2^32 fragments");
-    uint32_t nf_before = 0;
-    VCT_HIP(hipMemcpyAsync(&nf_before, counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
-    VCT_HIP(hipStreamSynchronize(ctx->stream));
-    size_t cap = (size_t)nf_before + total;
-    // grow keeping existing fragments: [key u32][canonical key u64][colour float4] x cap
-    if (cap > ctx->frags.cap / 28 || !ctx->frags.p) {
-      VctBuf nb;
-      const size_t ncap = (cap + cap / 4 + 1024 + 3) & ~(size_t)3;  // 8- and 16-byte aligned arrays
-      if ((rc = vct_ensure(nb, ncap * 28, ctx))) return rc;
-      if (ctx->frags.p && nf_before) {
-        const size_t oc = ctx->frags.cap / 28;
-        char* ob = (char*)ctx->frags.p;
-        char* nbp = (char*)nb.p;
-        VCT_HIP(hipMemcpyAsync(nbp, ob, (size_t)nf_before * 4, hipMemcpyDeviceToDevice, ctx->stream));
-        VCT_HIP(hipMemcpyAsync(nbp + ncap * 4, ob + oc * 4, (size_t)nf_before * 8, hipMemcpyDeviceToDevice,
-                               ctx->stream));
-        VCT_HIP(hipMemcpyAsync(nbp + ncap * 12, ob + oc * 12, (size_t)nf_before * 16, hipMemcpyDeviceToDevice,
-                               ctx->stream));
-        VCT_HIP(hipStreamSynchronize(ctx->stream));
-      }
-      if (ctx->frags.p) VCT_HIP(hipFree(ctx->frags.p));
-      ctx->frags = nb;
-      ctx->frags.cap = ncap * 28;
-    }
-    const size_t fc = ctx->frags.cap / 28;
-    o.f_key = (uint32_t*)ctx->frags.p;
-    o.f_cand = (unsigned long long*)((char*)ctx->frags.p + fc * 4);
-    o.f_color = (float4*)((char*)ctx->frags.p + fc * 12);
-    o.f_counter = counters + 1;
-    o.f_cap = (uint32_t)fc;
-    cand_total += total;
-    ctx->last_candidates += total;
-    hipLaunchKernelGGL(vox_raster_kernel, dim3(256 * 8), dim3(256), 0, ctx->stream, ntri, vp,
-                       (const uint32_t*)ctx->tri_off.p, (const uint32_t*)counters, (const TriRec*)ctx->setup.p,
-                       (const DevMaterial*)ctx->d_materials.p, o);
-    VCT_HIP(hipGetLastError());
-  }
-  const uint32_t nb = (uint32_t)((nvox + 255) / 256);
-  // reference mode: bucket + replay
-  uint32_t nfrag = 0;
-  VCT_HIP(hipMemcpyAsync(&nfrag, counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
-  VCT_HIP(hipStreamSynchronize(ctx->stream));
-  ctx->last_fragments = nfrag;
-  if ((rc = vct_launch_scan_u32(ctx, (uint32_t*)ctx->vox_count.p, (uint32_t*)ctx->vox_off.p, (uint32_t)nvox,
-                                counters + 2)))
-    return rc;
-  const size_t ns = ((size_t)nfrag + 4) & ~(size_t)3;
-  if ((rc = vct_ensure(ctx->frags_sorted, ns * 24, ctx))) return rc;
-  unsigned long long* s_cand = (unsigned long long*)ctx->frags_sorted.p;
-  float4* s_color = (float4*)((char*)ctx->frags_sorted.p + ns * 8);
-  const size_t fc = ctx->frags.cap / 28;
-  if (nfrag) {
-    hipLaunchKernelGGL(vox_bucket_kernel, dim3(2048), dim3(256), 0, ctx->stream, (const uint32_t*)ctx->frags.p,
-                       (const unsigned long long*)((char*)ctx->frags.p + fc * 4),
-                       (const float4*)((char*)ctx->frags.p + fc * 12), (const uint32_t*)(counters + 1),
-                       (const uint32_t*)ctx->vox_off.p, (uint32_t*)ctx->vox_count.p, s_cand, s_color);
-  }
-  hipLaunchKernelGGL(vox_replay_kernel, dim3(nb), dim3(256), 0, ctx->stream, (const uint32_t*)ctx->vox_off.p,
-                     (const uint32_t*)(counters + 1), s_cand, s_color, (uint32_t*)level0, (uint32_t)nvox);
-  VCT_HIP(hipGetLastError());
-  return VCT_OK;
+  return voxelize_reference(ctx, q, vp, level0);
 }
