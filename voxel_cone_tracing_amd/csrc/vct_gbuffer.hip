@@ -514,7 +514,9 @@ static void normal_matrix3(const float* M, float* N) {
     for (int r = 0; r < 3; ++r) N[c * 3 + r] = (float)inv[c][r];
 }
 
-int vct_launch_gbuffer(vct_context* ctx) {
+int vct_launch_gbuffer(vct_context* ctx) { return vct_launch_gbuffer_phase(ctx, 3); }
+
+int vct_launch_gbuffer_phase(vct_context* ctx, int phase) {
   const int W = ctx->width, H = ctx->height;
   if (W > 32767 || H > 32767) return vct_set_error(VCT_ERR_LIMIT, "viewport above 32767 pixels");
   const size_t npix = (size_t)W * H;
@@ -555,20 +557,24 @@ int vct_launch_gbuffer(vct_context* ctx) {
   uint32_t* tile_count = (uint32_t*)ctx->g_count.p;
   uint32_t* cursor = tile_count + ntiles;
   uint32_t* tile_off = (uint32_t*)ctx->g_off.p;
-  hipLaunchKernelGGL(fill_u32_kernel, dim3((2 * ntiles + 255) / 256), dim3(256), 0, ctx->stream, tile_count,
-                     2 * ntiles, 0u);
-  VCT_HIP(hipGetLastError());
-  uint32_t total = 0;
-  if (ntri) {
-    if ((rc = vct_ensure(ctx->g_recs, (size_t)ntri * sizeof(GRec), ctx))) return rc;
-    hipLaunchKernelGGL(g_bin_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, p,
-                       (GRec*)ctx->g_recs.p, tile_count, tiles_x);
+  if (phase & 1) {
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((2 * ntiles + 255) / 256), dim3(256), 0, ctx->stream, tile_count,
+                       2 * ntiles, 0u);
     VCT_HIP(hipGetLastError());
-    if ((rc = vct_launch_scan_u32(ctx, tile_count, tile_off, ntiles, counters + 4))) return rc;
-    // the list size is known on the device only
-    VCT_HIP(hipMemcpyAsync(&total, counters + 4, 4, hipMemcpyDeviceToHost, ctx->stream));
-    VCT_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->h_readback[0] = 0u;
+    if (ntri) {
+      if ((rc = vct_ensure(ctx->g_recs, (size_t)ntri * sizeof(GRec), ctx))) return rc;
+      hipLaunchKernelGGL(g_bin_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, p,
+                         (GRec*)ctx->g_recs.p, tile_count, tiles_x);
+      VCT_HIP(hipGetLastError());
+      if ((rc = vct_launch_scan_u32(ctx, tile_count, tile_off, ntiles, counters + 4, &ctx->g_scan_tmp))) return rc;
+      // the list size is known on the device only
+      VCT_HIP(hipMemcpyAsync(ctx->h_readback, counters + 4, 4, hipMemcpyDeviceToHost, ctx->stream));
+    }
   }
+  if (!(phase & 2)) return VCT_OK;
+  if (ntri) VCT_HIP(hipStreamSynchronize(ctx->stream));
+  const uint32_t total = ctx->h_readback[0];
   if (total) {
     if ((rc = vct_ensure(ctx->g_list, (size_t)total * 4 + (size_t)total / 4 * 4, ctx))) return rc;
     hipLaunchKernelGGL(g_bin_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, ntri,

@@ -187,6 +187,9 @@ struct vct_context {
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;          // second stream (default priority): the G-buffer beside voxelize + mip
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // pinned host words the list sizes are copied into: [0] G-buffer
+  // (triangle, tile) pairs, [1] voxelize (triangle, brick) pairs
+  uint32_t* h_readback = nullptr;
   int width = 0, height = 0;
   unsigned res = 128;
   int levels = 7;
@@ -220,7 +223,7 @@ struct vct_context {
 
   // voxelize scratch
   VctBuf tri_count, tri_off, tri_base;  // per-brick triangle counts + cursors, triangle lists, per-brick offsets
-  VctBuf frags_sorted, vox_count, vox_off, scan_tmp, counters;
+  VctBuf frags_sorted, vox_count, vox_off, scan_tmp, g_scan_tmp, counters;
   VctBuf setup;  // per-frame triangle list (model index, transform)
   uint64_t last_fragments = 0, last_candidates = 0;
 
@@ -271,6 +274,12 @@ int vct_ensure_pairs(vct_context* ctx);  // x-paired copy of the volume for the 
 int vct_frame_front(vct_context* ctx);   // voxelize + mip beside the G-buffer, joined
 int vct_launch_voxelize(vct_context* ctx);
 int vct_launch_gbuffer(vct_context* ctx);
+// The two passes that size a triangle list on the device, in two phases:
+// phase 1 launches the work up to the list size and its copy to pinned host
+// memory, phase 2 waits for that copy and launches the rest (3 = both).  The
+// frame front runs phase 1 of both passes before either waits.
+int vct_launch_voxelize_phase(vct_context* ctx, int phase);
+int vct_launch_gbuffer_phase(vct_context* ctx, int phase);
 int vct_launch_trace(vct_context* ctx);
 // Trace of a W x H G-buffer into out; demodulate: final / max(Kd, eps) with
 // the pixel's linear depth in w (the learned path's buffers).
@@ -294,6 +303,6 @@ int vct_trace_select(vct_context* ctx, const float4* gpos, const float4* gnrm, f
 int vct_trace_into(vct_context* ctx, const float4* gpos, const float4* gnrm, float4* out, int W, int H,
                    int demodulate, unsigned long long* step_counter);
 int vct_launch_scan_u32(vct_context* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
-                        uint32_t* total_dev);
+                        uint32_t* total_dev, VctBuf* tmp = nullptr);
 int vct_tex3d_mip_launch(vct_context* ctx, VctTex3D& t);
 int vct_tex3d_clear_launch(vct_context* ctx, VctTex3D& t, const float rgba[4]);

@@ -115,8 +115,12 @@ __global__ __launch_bounds__(256) void scan_apply_kernel(const uint32_t* in, uin
 }  // namespace
 
 // Exclusive scan of n u32 values; *total_dev (device) receives the sum.
-// Scratch comes from ctx->scan_tmp.  Sums must fit in 32 bits.
-int vct_launch_scan_u32(vct_context* ctx, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total_dev) {
+// Scratch comes from *tmp, by default ctx->scan_tmp (scans that may run at
+// the same time on two streams need scratch of their own).  Sums must fit in
+// 32 bits.
+int vct_launch_scan_u32(vct_context* ctx, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total_dev,
+                        VctBuf* tmp_buf) {
+  VctBuf& scratch = tmp_buf ? *tmp_buf : ctx->scan_tmp;
   if (n == 0) {
     if (total_dev) VCT_HIP(hipMemsetAsync(total_dev, 0, 4, ctx->stream));
     return VCT_OK;
@@ -132,9 +136,9 @@ int vct_launch_scan_u32(vct_context* ctx, const uint32_t* in, uint32_t* out, uin
   // scratch: for each level >0 a sums array and an offsets array
   size_t need = 0;
   for (size_t i = 1; i < sizes.size(); ++i) need += 2 * ((size_t)sizes[i] + 64);
-  int rc = vct_ensure(ctx->scan_tmp, need * 4 + 256, ctx);
+  int rc = vct_ensure(scratch, need * 4 + 256, ctx);
   if (rc) return rc;
-  uint32_t* tmp = reinterpret_cast<uint32_t*>(ctx->scan_tmp.p);
+  uint32_t* tmp = reinterpret_cast<uint32_t*>(scratch.p);
   std::vector<uint32_t*> sums(sizes.size(), nullptr), offs(sizes.size(), nullptr);
   size_t pos = 0;
   for (size_t i = 1; i < sizes.size(); ++i) {
@@ -256,16 +260,33 @@ static void identity(float* m) {
 // The passes before the trace of a frame: the G-buffer on the second stream
 // beside voxelize and mip on the context stream (they read the same geometry
 // and write disjoint buffers; each pass is timed on its own stream, so the
-// pass times overlap); the context stream then waits for the G-buffer.
+// pass times overlap); the context stream then waits for the G-buffer.  Both
+// the G-buffer and voxelize size a triangle list on the device and read it
+// back: their first phases are launched before either waits, so neither
+// stream idles while the host waits for the other's count.
 int vct_frame_front(vct_context* ctx) {
   int rc;
+  if ((rc = sync_materials(ctx))) return rc;
+  // runs fn with the side stream as the context stream
+  auto on_side = [ctx](auto&& fn) {
+    std::swap(ctx->stream, ctx->side);
+    const int r = fn();
+    std::swap(ctx->stream, ctx->side);
+    return r;
+  };
   VCT_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
   VCT_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-  std::swap(ctx->stream, ctx->side);
-  rc = vct_gbuffer(ctx);
-  std::swap(ctx->stream, ctx->side);
-  if (rc) return rc;
-  if ((rc = vct_voxelize(ctx))) return rc;
+  VCT_HIP(hipEventRecord(ctx->ev[4], ctx->side));
+  if ((rc = on_side([ctx] { return vct_launch_gbuffer_phase(ctx, 1); }))) return rc;
+  ++ctx->volume_version;
+  VCT_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+  if ((rc = vct_launch_voxelize_phase(ctx, 1))) return rc;
+  if ((rc = on_side([ctx] { return vct_launch_gbuffer_phase(ctx, 2); }))) return rc;
+  VCT_HIP(hipEventRecord(ctx->ev[5], ctx->side));
+  ctx->ev_recorded[2] = true;
+  if ((rc = vct_launch_voxelize_phase(ctx, 2))) return rc;
+  VCT_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+  ctx->ev_recorded[0] = true;
   VCT_HIP(hipEventRecord(ctx->ev_join, ctx->side));
   if ((rc = vct_mip_aniso(ctx))) return rc;
   VCT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
@@ -327,6 +348,12 @@ int vct_create(int width, int height, vct_ctx* out) {
     vct_destroy(ctx);
     return vct_hip_check(e, "hipEventCreate");
   }
+  e = hipHostMalloc((void**)&ctx->h_readback, 64, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    ctx->h_readback = nullptr;
+    vct_destroy(ctx);
+    return vct_hip_check(e, "hipHostMalloc");
+  }
   // default material (renderer.cpp:85-98,115-128)
   vct_material dm;
   memset(&dm, 0, sizeof(dm));
@@ -364,7 +391,7 @@ int vct_destroy(vct_ctx ctx) {
     if (t.dptr) (void)hipFree(t.dptr);
   VctBuf* bufs[] = {&ctx->d_materials, &ctx->volume, &ctx->tri_count, &ctx->tri_off,
                     &ctx->tri_base, &ctx->frags_sorted, &ctx->vox_count, &ctx->vox_off,
-                    &ctx->scan_tmp, &ctx->counters, &ctx->setup, &ctx->g_list, &ctx->g_pos,
+                    &ctx->scan_tmp, &ctx->g_scan_tmp, &ctx->counters, &ctx->setup, &ctx->g_list, &ctx->g_pos,
                     &ctx->g_nrm, &ctx->image, &ctx->g_count, &ctx->g_off, &ctx->g_recs,
                     &ctx->mlp_params, &ctx->mlp_m1, &ctx->mlp_m2, &ctx->mlp_packed, &ctx->mlp_bias,
                     &ctx->mlp_grad, &ctx->mlp_partial, &ctx->low_gpos, &ctx->low_gnrm, &ctx->low_img,
@@ -379,6 +406,7 @@ int vct_destroy(vct_ctx ctx) {
   if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->h_readback) (void)hipHostFree(ctx->h_readback);
   delete ctx;
   return VCT_OK;
 }

@@ -795,7 +795,10 @@ static void launch_brick(vct_context* ctx, uint32_t nbricks, const VoxParams& vp
 }
 
 // Exact mean: all queued draws in one batch (brick lists span every draw).
-static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, const VoxParams& vp, void* level0) {
+// Phase 1 launches up to the copy of the list size to pinned host memory,
+// phase 2 waits for it and launches the rest.
+static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, const VoxParams& vp, void* level0,
+                          int phase) {
   const uint32_t res = ctx->res;
   const uint32_t edge = res < (uint32_t)kBrick ? res : (uint32_t)kBrick;
   const uint32_t per = res / edge;
@@ -808,24 +811,28 @@ static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, cons
   uint32_t* bcount = (uint32_t*)ctx->vox_count.p;
   uint32_t* cursor = bcount + nbricks;
   uint32_t* boff = (uint32_t*)ctx->vox_off.p;
-  hipLaunchKernelGGL(zero_u32_kernel, dim3((2 * nbricks + 255) / 256), dim3(256), 0, ctx->stream, bcount,
-                     (uint64_t)2 * nbricks);
-  VCT_HIP(hipGetLastError());
   if (q.size() > (size_t)kMaxDraws) return vct_set_error(VCT_ERR_LIMIT, "at most 16 queued models per frame");
   DrawBatch b;
   uint32_t canon = 0;
   fill_batch(ctx, q, 0, canon, b);
   const uint32_t ntri = b.ntri;
-  uint32_t total = 0;
-  if (ntri) {
-    if ((rc = vct_ensure(ctx->setup, (size_t)ntri * sizeof(TriRec), ctx))) return rc;
-    hipLaunchKernelGGL(vox_brick_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
-                       (TriRec*)ctx->setup.p, bcount);
+  if (phase & 1) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((2 * nbricks + 255) / 256), dim3(256), 0, ctx->stream, bcount,
+                       (uint64_t)2 * nbricks);
     VCT_HIP(hipGetLastError());
-    if ((rc = vct_launch_scan_u32(ctx, bcount, boff, nbricks, counters + 2))) return rc;
-    VCT_HIP(hipMemcpyAsync(&total, counters + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
-    VCT_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->h_readback[1] = 0u;
+    if (ntri) {
+      if ((rc = vct_ensure(ctx->setup, (size_t)ntri * sizeof(TriRec), ctx))) return rc;
+      hipLaunchKernelGGL(vox_brick_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
+                         (TriRec*)ctx->setup.p, bcount);
+      VCT_HIP(hipGetLastError());
+      if ((rc = vct_launch_scan_u32(ctx, bcount, boff, nbricks, counters + 2))) return rc;
+      VCT_HIP(hipMemcpyAsync(ctx->h_readback + 1, counters + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
+    }
   }
+  if (!(phase & 2)) return VCT_OK;
+  if (ntri) VCT_HIP(hipStreamSynchronize(ctx->stream));
+  const uint32_t total = ctx->h_readback[1];
   if (total) {
     if ((rc = vct_ensure(ctx->tri_off, (size_t)total * 4 + (size_t)total / 4 * 4, ctx))) return rc;
     hipLaunchKernelGGL(vox_brick_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
@@ -969,7 +976,9 @@ static void light_dirs(VoxParams& vp) {
   }
 }
 
-int vct_launch_voxelize(vct_context* ctx) {
+int vct_launch_voxelize(vct_context* ctx) { return vct_launch_voxelize_phase(ctx, 3); }
+
+int vct_launch_voxelize_phase(vct_context* ctx, int phase) {
   const uint32_t res = ctx->res;
   void* level0 = (char*)ctx->volume.p + ctx->lvl_off[0][0];
   int rc;
@@ -984,8 +993,10 @@ int vct_launch_voxelize(vct_context* ctx) {
   std::vector<uint32_t> q;
   for (uint32_t id : ctx->queue)
     if (id < ctx->models.size() && ctx->models[id].ntri) q.push_back(id);
-  ctx->last_candidates = 0;
-  ctx->last_fragments = 0;
-  if (ctx->average == VCT_AVG_EXACT) return voxelize_exact(ctx, q, vp, level0);
-  return voxelize_reference(ctx, q, vp, level0);
+  if (phase & 1) {
+    ctx->last_candidates = 0;
+    ctx->last_fragments = 0;
+  }
+  if (ctx->average == VCT_AVG_EXACT) return voxelize_exact(ctx, q, vp, level0, phase);
+  return (phase & 2) ? voxelize_reference(ctx, q, vp, level0) : VCT_OK;
 }
