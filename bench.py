@@ -315,8 +315,9 @@ def main():
     amount, unit, bound, definition = algorithmic("trace", wl, bpe, steps, 0)
     t_launch = float(plain_passes[3]) * 1e-3
     achieved = amount / t_launch / 1e9
-    kname = f"trace_frame_kernel<{dt}>"
-    grid = (wl.width + 7) // 8 * 256 * ((wl.height + 7) // 8)
+    # up to 9 diffuse cones without shadow lights: the 128-pixel (16x8) block shape
+    kname = f"trace_frame_kernel<{dt},Shape<128,11,2>>"
+    grid = (wl.width + 15) // 16 * 256 * ((wl.height + 7) // 8)
     traffic = l2_hit = None
     if wl.name == "soup256" and args.dtype == "fp16":
         from voxel_cone_tracing_amd.profile_summary import read_summary, read_traffic

@@ -174,6 +174,37 @@ def test_trace_matches_oracle(gpu_available, orc, kind, dtype, res, ntri):
     assert err <= 2e-3
 
 
+@pytest.mark.parametrize("case", ["cones16", "lights3"])
+def test_trace_full_block_shape_matches_oracle(gpu_available, orc, case):
+    """Configurations beyond the 128-pixel block shape (more than 9 diffuse
+    cones, or shadow cones of more than 2 lights) run the 64-pixel shape with
+    every LDS slot; same oracle bar as the other trace tests."""
+    say(f"trace parity, full block shape: {case}")
+    W, H, res = 96, 64, 64
+    if case == "cones16":
+        sc = SceneSpec("soup", res=res, ntri=4000, W=W, H=H)
+        sc.cone_set = L.CONES_16
+    else:
+        sc = SceneSpec("cornell", res=res, W=W, H=H, glass=True, phases=(True, True, True, True))
+        sc.lights = sc.lights + [((0.3, 0.2, 0.35), (0.9, 0.6, 0.4), 0.7, L.LIGHT_POINT),
+                                 ((-0.6, 1.0, 0.8), (0.5, 0.6, 1.0), 0.8, L.LIGHT_DIRECTIONAL)]
+    with Context(W, H) as ctx:
+        sc.build(ctx, L.FP16)
+        ctx.render()
+        img = ctx.download_image()
+        gp, gn = ctx.download_gbuffer()
+        l0 = decode_level(ctx.download_level(0), L.FP16)
+        mips = {(l, d): decode_level(ctx.download_level(l, d), L.FP16) for l in range(1, 7) for d in range(6)}
+    cov = gp[..., 3].reshape(-1) != 0
+    pix = np.flatnonzero(cov).astype(np.int64)
+    assert pix.size > 0
+    want, _ = orc.trace(np.float32, res, 7, l0, mips, gp, gn, sc.mat_records, sc.lights, sc.cam, sc.cube_size,
+                        cone_set=sc.cone_set, phases=sc.phases, pixels=pix)
+    err = np.abs(img.reshape(-1, 4)[pix, :3] - want[:, :3]).max()
+    say(f"  max-abs vs FP32 oracle = {err:.3e}")
+    assert err <= 2e-3
+
+
 @pytest.mark.parametrize("dtype", [L.RGBA8, L.FP16])
 def test_full_compose_matches_oracle(gpu_available, orc, dtype):
     """All phases on (direct light + shadow cones, diffuse, specular and

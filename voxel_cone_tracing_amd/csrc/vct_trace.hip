@@ -407,30 +407,49 @@ __device__ __forceinline__ float3 refract3(float3 I, float3 N, float eta) {
   return make_float3(eta * I.x - s * N.x, eta * I.y - s * N.y, eta * I.z - s * N.z);
 }
 
-// One workgroup = the 8x8 pixels of a block x 4 waves.  Wave 0 sets the
-// block's shading pixels up (lane = pixel) into LDS.  The cones of the block
-// - every pixel's specular, refraction and shadow cones, then its diffuse
-// cones, longest kinds first - form one queue of (job, pixel) items, job-major
-// so that neighbouring lanes start on the same cone of neighbouring pixels
-// (coherent gathers).  Every lane marches its own item and takes the next one
-// from a workgroup counter in LDS as soon as its cone ends, so lanes do not
-// idle while the longest cone of a wave finishes.  Wave 0 then composes the
-// pixels in the reference's order (voxel_cone_tracing.frag:246-275).
-constexpr int kMaxSlots = 18;  // 16 diffuse + specular + refraction
+// One workgroup = a block of Shape::NPIX pixels (NSUB sub-blocks of 8x8) x 4
+// waves.  Waves 0..NSUB-1 set the shading pixels of their sub-block up (lane =
+// pixel) into LDS.  The cones of the block - every pixel's specular,
+// refraction and shadow cones, then its diffuse cones, longest kinds first -
+// form one queue of (job, pixel) items, job-major so that neighbouring lanes
+// start on the same cone of neighbouring pixels (coherent gathers).  Every
+// lane marches its own item and takes the next one from a workgroup counter
+// in LDS as soon as its cone ends, so lanes do not idle while the longest cone
+// of a wave finishes.  The same waves then compose their pixels in the
+// reference's order (voxel_cone_tracing.frag:246-275).
 // job codes: 0..15 diffuse cone k, then:
 constexpr int kJobSpecular = 16, kJobRefraction = 17, kJobShadow = 18;  // kJobShadow + light
-constexpr int kMaxJobs = kMaxSlots + VCT_MAX_LIGHTS;
-constexpr int kRefillMin = 16;  // idle lanes of a wave that trigger a refill (measured: 1 -> 3.15 ms, 16 -> 2.97 ms)
+constexpr int kMaxJobs = kJobShadow + VCT_MAX_LIGHTS;
+// idle lanes of a wave that trigger a refill: a refill costs the whole wave
+// its set-up; measured on the 1M-triangle soup with 128-pixel blocks: 8 ->
+// 2.69 ms, 16 -> 2.60 ms, 24 -> 2.55 ms, 32 -> 2.59 ms, 48 -> 2.77 ms
+constexpr int kRefillMin = 24;
+
+// Block shapes: pixels per workgroup and the LDS slots for cone results.
+// Cones are short in cluttered scenes (3 steps on average on a 1M-triangle
+// soup), so a queue of 64 pixels x 10 cones gives each of the 256 lanes only
+// 2.5 items and the drain at its end leaves a third of the lanes idle; 128
+// pixels per workgroup halve that.  The wide shape keeps LDS at 5 workgroups
+// per CU by holding only the slots of up to 9 diffuse cones and 2 shadow
+// lights; other configurations use the full 64-pixel shape.
+template <int NPIX_, int NSLOT_, int NSHADOW_>
+struct Shape {
+  static constexpr int NPIX = NPIX_, NSLOT = NSLOT_, NSHADOW = NSHADOW_, NSUB = NPIX_ / 64;
+};
+using ShapeWide = Shape<128, 11, 2>;                 // <= 9 diffuse + specular + refraction
+using ShapeFull = Shape<64, 18, VCT_MAX_LIGHTS>;     // 16 diffuse + specular + refraction
 
 // LDS of one workgroup of the cone marcher.
+template <class SH>
 struct ShadeLds {
-  float cone[kMaxSlots][3][64];      // slot (diffuse 0..nd-1, specular nd, refraction nd+1) x pixel
-  float shadow[VCT_MAX_LIGHTS][64];  // max(0, 1 - alpha) of the shadow cone per light and pixel
-  float pos[3][64], nrm[3][64], o1[3][64], o2[3][64], vdir[3][64];
-  float spec_ap[64], eta[64];
-  unsigned long long refr_mask;  // shading pixels with a transmissive material
+  float cone[SH::NSLOT][3][SH::NPIX];  // slot (diffuse 0..nd-1, specular nd, refraction nd+1) x pixel
+  float shadow[SH::NSHADOW][SH::NPIX];  // max(0, 1 - alpha) of the shadow cone per light and pixel
+  float pos[3][SH::NPIX], nrm[3][SH::NPIX], o1[3][SH::NPIX], o2[3][SH::NPIX], vdir[3][SH::NPIX];
+  float spec_ap[SH::NPIX], eta[SH::NPIX];
+  unsigned long long refr_mask[SH::NSUB];  // shading pixels with a transmissive material, per sub-block
+  int sub_count[SH::NSUB];                 // shading pixels per sub-block
   int npx, njobs, next_item;
-  uint8_t px_of[64];  // the block's shading pixels, compacted
+  uint8_t px_of[SH::NPIX];  // the block's shading pixels, compacted
   int8_t jobs[kMaxJobs];
 };
 
@@ -499,14 +518,17 @@ __device__ __forceinline__ float3 light_dir(const TraceArgs& A, int li, float3 p
   return normalize3(v);
 }
 
-__device__ __forceinline__ float3 ld3(const float (*a)[64], int p) { return make_float3(a[0][p], a[1][p], a[2][p]); }
+template <int N>
+__device__ __forceinline__ float3 ld3(const float (*a)[N], int p) { return make_float3(a[0][p], a[1][p], a[2][p]); }
 
-// Shades the block's 64 pixels; returns after the workgroup barrier of the
-// composition.  L->next_item must be 0 and visible to the workgroup on entry.
-template <int DT>
+// Shades the block's pixels; (px, py, oidx, valid) are the thread's pixel in
+// sub-block (wave index mod NSUB).  L->next_item must be 0 and visible to the
+// workgroup on entry.
+template <int DT, class SH>
 __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, size_t oidx, bool valid,
-                                            ShadeLds* L, const SamplerLds* S) {
+                                            ShadeLds<SH>* L, const SamplerLds* S) {
   const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const int me = grp * 64 + lane;  // the thread's pixel in the block (set-up and composition waves)
   valid = valid && px < A.W && py < A.H;
   const size_t idx = valid ? (size_t)py * A.W + px : 0;
   const float sqrt3 = 1.73205080757f;
@@ -529,38 +551,56 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
       else mode = A.view_dir < 7 ? 2 : 3;  // within_cube on biased coordinates (:250, :61-64)
     }
   };
-  if (grp == 0) {
+  unsigned long long shade = 0ull;
+  int smode = 0;
+  if (grp < SH::NSUB) {
     float4 gp, gn;
     const DevMaterial* m;
     float3 pos;
     int mode;
     pixel(gp, gn, m, pos, mode);
+    smode = mode;
     const float3 n = make_float3(gn.x, gn.y, gn.z);
     const float3 cam = make_float3(A.cam[0], A.cam[1], A.cam[2]);
     const float3 vd = normalize3(make_float3(gp.x - cam.x, gp.y - cam.y, gp.z - cam.z));
     // specular aperture (:220-227)
     const float rough = sqrtf(2.0f / (m->shininess + 2.0f));
-    L->spec_ap[lane] = fminf(fmaxf(tanf(1.57079f * rough), 0.0174533f), 3.14159265f);
-    L->eta[lane] = 1.0f / m->ior;
+    L->spec_ap[me] = fminf(fmaxf(tanf(1.57079f * rough), 0.0174533f), 3.14159265f);
+    L->eta[me] = 1.0f / m->ior;
     float3 o1 = make_float3(0.f, 0.f, 0.f), o2 = o1;
     if (mode == 3 && nd > 0) {
       o1 = normalize3(tangent_of(n));  // trace_diffuse (:146-147)
       o2 = normalize3(cross3(o1, n));
     }
     const float3 vals[5] = {pos, n, o1, o2, vd};
-    float (*dst[5])[64] = {L->pos, L->nrm, L->o1, L->o2, L->vdir};
+    float (*dst[5])[SH::NPIX] = {L->pos, L->nrm, L->o1, L->o2, L->vdir};
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-      dst[q][0][lane] = vals[q].x;
-      dst[q][1][lane] = vals[q].y;
-      dst[q][2][lane] = vals[q].z;
+      dst[q][0][me] = vals[q].x;
+      dst[q][1][me] = vals[q].y;
+      dst[q][2][me] = vals[q].z;
     }
-    const unsigned long long shade = __ballot(mode == 3);
+    shade = __ballot(mode == 3);
     const unsigned long long refr = __ballot(mode == 3 && do_spec && illum_transmissive(m->illum));
-    if (mode == 3) L->px_of[__popcll(shade & ((1ull << lane) - 1ull))] = (uint8_t)lane;
     if (lane == 0) {
-      L->npx = __popcll(shade);
-      L->refr_mask = refr;
+      L->sub_count[grp] = __popcll(shade);
+      L->refr_mask[grp] = refr;
+    }
+  }
+  __syncthreads();
+  if (grp < SH::NSUB) {
+    // the block's shading pixels, compacted in sub-block order
+    int base = 0, total = 0;
+    bool refr = false;
+#pragma unroll
+    for (int w = 0; w < SH::NSUB; ++w) {
+      base += w < grp ? L->sub_count[w] : 0;
+      total += L->sub_count[w];
+      refr = refr || L->refr_mask[w] != 0ull;
+    }
+    if (smode == 3) L->px_of[base + __popcll(shade & ((1ull << lane) - 1ull))] = (uint8_t)me;
+    if (me == 0) {
+      L->npx = total;
       int nj = 0;  // longest kinds first
       if (do_spec) L->jobs[nj++] = kJobSpecular;
       if (refr) L->jobs[nj++] = kJobRefraction;
@@ -576,7 +616,7 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
   const float margin = 0.5f * (float)(1u << (A.levels - 1)) * vs;
   const int npx = L->npx;
   const int nitems = npx * L->njobs;
-  // it / npx as (it * ceil(2^20 / npx)) >> 20: exact for it < kMaxJobs * 64 and npx <= 64
+  // it / npx as (it * ceil(2^20 / npx)) >> 20: exact for it < kMaxJobs * 128 and npx <= 128
   const uint32_t npx_inv = npx ? (1u << 20) / (uint32_t)npx + (((1u << 20) % (uint32_t)npx) ? 1u : 0u) : 0u;
   uint32_t steps = 0;
   int item = -1, job = 0, pix = 0;
@@ -586,7 +626,7 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
     if (job >= kJobShadow) {
       L->shadow[job - kJobShadow][pix] = fmaxf(0.0f, 1.0f - k.c.w);
     } else {
-      const int slot = job < kJobSpecular ? job : (job == kJobSpecular ? nd : nd + 1);
+      const int slot = job < kJobSpecular ? job : (job == kJobSpecular ? nd : nd + 1);  // < SH::NSLOT
       L->cone[slot][0][pix] = k.c.x;
       L->cone[slot][1][pix] = k.c.y;
       L->cone[slot][2][pix] = k.c.z;
@@ -613,7 +653,7 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
       const int j = (int)(((uint32_t)it * npx_inv) >> 20);
       job = L->jobs[j];
       pix = L->px_of[it - j * npx];
-      if (job == kJobRefraction && !((L->refr_mask >> pix) & 1ull)) continue;  // opaque pixel: no item
+      if (job == kJobRefraction && !((L->refr_mask[pix >> 6] >> (pix & 63)) & 1ull)) continue;  // opaque: no item
       const float3 pos = ld3(L->pos, pix), n = ld3(L->nrm, pix);
       float3 d;
       float ap, md = sqrt3;
@@ -652,8 +692,8 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
   }
   if (A.step_counter && steps) atomicAdd(A.step_counter, (unsigned long long)steps);
   __syncthreads();
-  // ---- composition (wave 0, lane = pixel) --------------------------------------------
-  if (grp != 0) return;
+  // ---- composition (waves 0..NSUB-1, lane = pixel) -----------------------------------
+  if (grp >= SH::NSUB) return;
   float4 gp, gn;
   const DevMaterial* mp;
   float3 pos;
@@ -680,14 +720,14 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
   }
   const DevMaterial& m = *mp;
   const float3 n = make_float3(gn.x, gn.y, gn.z);
-  const float3 view_dir = ld3(L->vdir, lane);
-  const bool do_refr = (L->refr_mask >> lane) & 1ull;
+  const float3 view_dir = ld3(L->vdir, me);
+  const bool do_refr = (L->refr_mask[grp] >> lane) & 1ull;
   float3 out = make_float3(0.f, 0.f, 0.f);
   if (nd > 0) {
     float3 diff_acc = make_float3(0.f, 0.f, 0.f);
     for (int q = 0; q < nd; ++q)
-      diff_acc = make_float3(diff_acc.x + L->cone[q][0][lane], diff_acc.y + L->cone[q][1][lane],
-                             diff_acc.z + L->cone[q][2][lane]);
+      diff_acc = make_float3(diff_acc.x + L->cone[q][0][me], diff_acc.y + L->cone[q][1][me],
+                             diff_acc.z + L->cone[q][2][me]);
     const float inv = 1.0f / (float)nd;
     out.x += m.diffuse[0] * (diff_acc.x * inv);
     out.y += m.diffuse[1] * (diff_acc.y * inv);
@@ -704,7 +744,7 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
       const float kk = att * cos_surf;
       const float3 lc = make_float3(kk * Lt.color[0] * Lt.intensity, kk * Lt.color[1] * Lt.intensity,
                                     kk * Lt.color[2] * Lt.intensity);
-      const float shadow = A.en_shadow ? L->shadow[li][lane] : 1.0f;
+      const float shadow = A.en_shadow ? L->shadow[li][me] : 1.0f;
       const float lambert = fmaxf(dot3(ldir, n), 0.0f);
       float refr_angle = 0.0f;
       if (m.dissolve <= 0.1f) {
@@ -725,14 +765,14 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
     out.z += res.z + fminf(fmaxf(m.emission[2], 0.0f), 1.0f);
   }
   if (do_spec) {
-    out.x += m.specular[0] * L->cone[nd][0][lane];
-    out.y += m.specular[1] * L->cone[nd][1][lane];
-    out.z += m.specular[2] * L->cone[nd][2][lane];
+    out.x += m.specular[0] * L->cone[nd][0][me];
+    out.y += m.specular[1] * L->cone[nd][1][me];
+    out.z += m.specular[2] * L->cone[nd][2][me];
   }
   if (do_refr) {  // final colour override (:273-274)
-    const float3 tr = make_float3(m.transmittance[0] * L->cone[nd + 1][0][lane],
-                                  m.transmittance[1] * L->cone[nd + 1][1][lane],
-                                  m.transmittance[2] * L->cone[nd + 1][2][lane]);
+    const float3 tr = make_float3(m.transmittance[0] * L->cone[nd + 1][0][me],
+                                  m.transmittance[1] * L->cone[nd + 1][1][me],
+                                  m.transmittance[2] * L->cone[nd + 1][2][me]);
     const float dv = m.dissolve;
     out = make_float3(tr.x * (1.0f - dv) + out.x * dv, tr.y * (1.0f - dv) + out.y * dv,
                       tr.z * (1.0f - dv) + out.z * dv);
@@ -747,36 +787,38 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
 }
 
 // Work of a launch: the whole viewport (trace_frame_kernel, one workgroup per
-// 8x8 block), or a device-side list of pixels (64 per chunk) or tiles (16
-// chunks of 8x8 pixels per tile, or 4 chunks of 8x8 odd-odd grid points) for
-// trace_list_kernel.  5 waves per SIMD (96 VGPRs; a few values of the 16-bit
-// paths spill, which measured 4 % faster than 4 waves without spills): the 12
-// paired loads of a three-map level are in flight together.
-template <int DT>
+// block of NSUB 8x8 sub-blocks side by side), or a device-side list of pixels
+// (NPIX per chunk) or tiles (16 sub-blocks of 8x8 pixels per tile, or 4 of
+// 8x8 odd-odd grid points, NSUB per chunk) for trace_list_kernel.  5 waves per
+// SIMD (96 VGPRs; a few values of the 16-bit paths spill, which measured 4 %
+// faster than 4 waves without spills): the 12 paired loads of a three-map
+// level are in flight together.
+template <int DT, class SH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void trace_frame_kernel(TraceArgs A) {
-  __shared__ ShadeLds s_shade;
+  __shared__ ShadeLds<SH> s_shade;
   __shared__ SamplerLds s_sampler;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, sub = (int)(threadIdx.x >> 6) % SH::NSUB;
   sampler_init(A, &s_sampler);
   if (threadIdx.x == 0) s_shade.next_item = 0;
   __syncthreads();
-  const int px = blockIdx.x * 8 + (lane & 7), py = blockIdx.y * 8 + (lane >> 3);
-  shade_block<DT>(A, px, py, (size_t)py * A.W + px, true, &s_shade, &s_sampler);
+  const int px = blockIdx.x * (8 * SH::NSUB) + sub * 8 + (lane & 7), py = blockIdx.y * 8 + (lane >> 3);
+  shade_block<DT, SH>(A, px, py, (size_t)py * A.W + px, true, &s_shade, &s_sampler);
 }
 
-// Chunks (64 pixels each) of a list launch.
-__device__ __forceinline__ uint32_t list_chunks(const TraceArgs& A) {
-  if (A.pix_list) return ((A.pix_count ? min(*A.pix_count, A.npix_list) : A.npix_list) + 63u) / 64u;
-  return *A.tile_count * (A.grid_stride == 2 ? 4u : 16u);
+// Chunks (npix pixels each) of a list launch.
+__device__ __forceinline__ uint32_t list_chunks(const TraceArgs& A, uint32_t npix) {
+  if (A.pix_list) return ((A.pix_count ? min(*A.pix_count, A.npix_list) : A.npix_list) + npix - 1u) / npix;
+  return *A.tile_count * (A.grid_stride == 2 ? 4u : 16u) / (npix / 64u);
 }
 
-// The pixel of `lane` in chunk c, where its result goes, and whether it is traced.
-__device__ __forceinline__ void list_chunk(const TraceArgs& A, uint32_t c, int lane, int& px, int& py, size_t& oidx,
-                                           bool& valid) {
+// The pixel of `lane` of sub-block `sub` in chunk c (nsub sub-blocks of 64
+// pixels per chunk), where its result goes, and whether it is traced.
+__device__ __forceinline__ void list_chunk(const TraceArgs& A, uint32_t c, uint32_t nsub, uint32_t sub, int lane,
+                                           int& px, int& py, size_t& oidx, bool& valid) {
   valid = true;
   if (A.pix_list) {
     const uint32_t cnt = A.pix_count ? min(*A.pix_count, A.npix_list) : A.npix_list;
-    const uint32_t k = c * 64u + (uint32_t)lane;
+    const uint32_t k = (c * nsub + sub) * 64u + (uint32_t)lane;
     valid = k < cnt;
     const uint32_t p = valid ? A.pix_list[k] : 0u;
     px = (int)(p % (uint32_t)A.W);
@@ -785,8 +827,9 @@ __device__ __forceinline__ void list_chunk(const TraceArgs& A, uint32_t c, int l
     if (valid && A.tile_flag) valid = A.tile_flag[(py >> 5) * A.tiles_x + (px >> 5)] == A.flag_want;
     return;
   }
-  const uint32_t per = A.grid_stride == 2 ? 4u : 16u;
-  const uint32_t t = A.tile_list[c / per], b = c % per;
+  const uint32_t per = A.grid_stride == 2 ? 4u : 16u;  // sub-blocks per tile
+  const uint32_t g = c * nsub + sub;                    // sub-block index over the list
+  const uint32_t t = A.tile_list[g / per], b = g % per;
   if (A.grid_stride == 2) {  // grid point (u, v) is pixel (2u + 1, 2v + 1)
     const int gu = (int)(t % (uint32_t)A.tiles_x) * 16 + (int)(b & 1u) * 8 + (lane & 7);
     const int gv = (int)(t / (uint32_t)A.tiles_x) * 16 + (int)(b >> 1) * 8 + (lane >> 3);
@@ -803,19 +846,19 @@ __device__ __forceinline__ void list_chunk(const TraceArgs& A, uint32_t c, int l
 // One chunk per workgroup; workgroups past the device-side count leave at once.
 // (A fixed grid striding over the chunks measured slower: the loop costs the
 // kernel registers and spills at 4 waves per SIMD.)
-template <int DT>
+template <int DT, class SH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void trace_list_kernel(TraceArgs A) {
-  __shared__ ShadeLds s_shade;
+  __shared__ ShadeLds<SH> s_shade;
   __shared__ SamplerLds s_sampler;
-  if (blockIdx.x >= list_chunks(A)) return;
+  if (blockIdx.x >= list_chunks(A, SH::NPIX)) return;
   sampler_init(A, &s_sampler);
   if (threadIdx.x == 0) s_shade.next_item = 0;
   __syncthreads();
   int px, py;
   size_t oidx;
   bool valid;
-  list_chunk(A, blockIdx.x, threadIdx.x & 63, px, py, oidx, valid);
-  shade_block<DT>(A, px, py, oidx, valid, &s_shade, &s_sampler);
+  list_chunk(A, blockIdx.x, SH::NSUB, (threadIdx.x >> 6) % SH::NSUB, threadIdx.x & 63, px, py, oidx, valid);
+  shade_block<DT, SH>(A, px, py, oidx, valid, &s_shade, &s_sampler);
 }
 
 // Builds the x-paired copy of every (level, map) of the volume in one launch:
@@ -960,6 +1003,30 @@ int vct_trace_into(vct_context* ctx, const float4* gpos, const float4* gnrm, flo
   return vct_trace_select(ctx, gpos, gnrm, out, W, H, demodulate, step_counter, sel);
 }
 
+// Launches the marcher of one block shape: the viewport, or one workgroup
+// per chunk of the largest the list can be.
+template <int DT, class SH>
+static void launch_shape(vct_context* ctx, const TraceArgs& A, const TraceSelect& sel, int W, int H) {
+  dim3 grid((W + 8 * SH::NSUB - 1) / (8 * SH::NSUB), (H + 7) / 8);
+  if (sel.pix_list) {
+    grid = dim3((sel.npix_list + SH::NPIX - 1) / SH::NPIX);
+    hipLaunchKernelGGL((trace_list_kernel<DT, SH>), grid, dim3(256), 0, ctx->stream, A);
+  } else if (sel.tile_list) {
+    grid = dim3((sel.grid_stride == 2 ? 4u : 16u) / SH::NSUB * sel.max_tiles);
+    hipLaunchKernelGGL((trace_list_kernel<DT, SH>), grid, dim3(256), 0, ctx->stream, A);
+  } else {
+    hipLaunchKernelGGL((trace_frame_kernel<DT, SH>), grid, dim3(256), 0, ctx->stream, A);
+  }
+}
+
+template <int DT>
+static int launch_trace(vct_context* ctx, const TraceArgs& A, const TraceSelect& sel, int W, int H, bool wide) {
+  if ((sel.pix_list && sel.npix_list == 0) || (!sel.pix_list && sel.tile_list && sel.max_tiles == 0)) return VCT_OK;
+  if (wide) launch_shape<DT, ShapeWide>(ctx, A, sel, W, H);
+  else launch_shape<DT, ShapeFull>(ctx, A, sel, W, H);
+  return VCT_OK;
+}
+
 int vct_trace_select(vct_context* ctx, const float4* gpos, const float4* gnrm, float4* out, int W, int H,
                      int demodulate, unsigned long long* step_counter, const TraceSelect& sel) {
   TraceArgs A;
@@ -1015,34 +1082,16 @@ int vct_trace_select(vct_context* ctx, const float4* gpos, const float4* gnrm, f
     A.lights.l[i].kind = ctx->lights[i].kind;
   }
   A.lights.count = n;
-  dim3 grid((W + 7) / 8, (H + 7) / 8);  // 8x8 pixels per workgroup
-  // lists: one workgroup per chunk of the largest the list can be
-  if (sel.pix_list) {
-    if (sel.npix_list == 0) return VCT_OK;
-    grid = dim3((sel.npix_list + 63) / 64);
-  } else if (sel.tile_list) {
-    if (sel.max_tiles == 0) return VCT_OK;
-    grid = dim3((sel.grid_stride == 2 ? 4u : 16u) * sel.max_tiles);
-  }
-  const bool list = sel.pix_list || sel.tile_list;
+  // the wide shape holds up to 9 diffuse cones and shadows of up to 2 lights
+  const int nshadow = (ctx->en_direct && ctx->en_shadow) ? n : 0;
+  const bool wide = A.cones.n <= ShapeWide::NSLOT - 2 && nshadow <= ShapeWide::NSHADOW;
   switch (ctx->dtype) {
-    case VCT_RGBA8:
-      if (list) hipLaunchKernelGGL((trace_list_kernel<VCT_RGBA8>), grid, dim3(256), 0, ctx->stream, A);
-      else hipLaunchKernelGGL(trace_frame_kernel<VCT_RGBA8>, grid, dim3(256), 0, ctx->stream, A);
-      break;
-    case VCT_FP16:
-      if (list) hipLaunchKernelGGL((trace_list_kernel<VCT_FP16>), grid, dim3(256), 0, ctx->stream, A);
-      else hipLaunchKernelGGL(trace_frame_kernel<VCT_FP16>, grid, dim3(256), 0, ctx->stream, A);
-      break;
-    case VCT_BF16:
-      if (list) hipLaunchKernelGGL((trace_list_kernel<VCT_BF16>), grid, dim3(256), 0, ctx->stream, A);
-      else hipLaunchKernelGGL(trace_frame_kernel<VCT_BF16>, grid, dim3(256), 0, ctx->stream, A);
-      break;
-    default:
-      if (list) hipLaunchKernelGGL((trace_list_kernel<VCT_FP32>), grid, dim3(256), 0, ctx->stream, A);
-      else hipLaunchKernelGGL(trace_frame_kernel<VCT_FP32>, grid, dim3(256), 0, ctx->stream, A);
-      break;
+    case VCT_RGBA8: rc = launch_trace<VCT_RGBA8>(ctx, A, sel, W, H, wide); break;
+    case VCT_FP16: rc = launch_trace<VCT_FP16>(ctx, A, sel, W, H, wide); break;
+    case VCT_BF16: rc = launch_trace<VCT_BF16>(ctx, A, sel, W, H, wide); break;
+    default: rc = launch_trace<VCT_FP32>(ctx, A, sel, W, H, wide); break;
   }
+  if (rc) return rc;
   VCT_HIP(hipGetLastError());
   return VCT_OK;
 }

@@ -29,12 +29,12 @@ from pathlib import Path
 
 
 def short_name(name: str) -> str:
-    """`void vct::trace_frame_kernel<1>(vct::TraceArgs)` -> `trace_frame_kernel<1>`."""
-    n = name.strip().replace("(anonymous namespace)::", "")
+    """`void vct::trace_frame_kernel<1, vct::Shape<128, 11, 2> >(vct::TraceArgs)` ->
+    `trace_frame_kernel<1,Shape<128,11,2>>` (no namespaces, no spaces)."""
+    n = name.strip().replace("(anonymous namespace)::", "").replace("vct::", "")
     n = re.sub(r"\(.*$", "", n)
     n = re.sub(r"^void ", "", n)
-    n = n.split("::")[-1]
-    return n
+    return n.replace(" ", "")
 
 
 def _times(trace_csv: Path):
