@@ -316,7 +316,7 @@ def main():
     t_launch = float(plain_passes[3]) * 1e-3
     achieved = amount / t_launch / 1e9
     # up to 9 diffuse cones without shadow lights: the 128-pixel (16x8) block shape
-    kname = f"trace_frame_kernel<{dt},Shape<128,11,2>>"
+    kname = f"trace_frame_kernel<{dt},Shape<128,2>>"
     grid = (wl.width + 15) // 16 * 256 * ((wl.height + 7) // 8)
     traffic = l2_hit = None
     if wl.name == "soup256" and args.dtype == "fp16":

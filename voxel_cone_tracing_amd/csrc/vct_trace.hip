@@ -425,25 +425,30 @@ constexpr int kMaxJobs = kJobShadow + VCT_MAX_LIGHTS;
 // 2.69 ms, 16 -> 2.60 ms, 24 -> 2.55 ms, 32 -> 2.59 ms, 48 -> 2.77 ms
 constexpr int kRefillMin = 24;
 
-// Block shapes: pixels per workgroup and the LDS slots for cone results.
+// Block shapes: pixels per workgroup and the shadow-light slots in LDS.
 // Cones are short in cluttered scenes (3 steps on average on a 1M-triangle
 // soup), so a queue of 64 pixels x 10 cones gives each of the 256 lanes only
-// 2.5 items and the drain at its end leaves a third of the lanes idle; 128
-// pixels per workgroup halve that.  The wide shape keeps LDS at 5 workgroups
-// per CU by holding only the slots of up to 9 diffuse cones and 2 shadow
-// lights; other configurations use the full 64-pixel shape.
-template <int NPIX_, int NSLOT_, int NSHADOW_>
+// 2.5 items and the drain at its end leaves a third of the lanes idle; more
+// pixels per workgroup shorten the drain relative to the work.  The wide
+// shape holds shadow results of up to 2 lights; more lights use the full one.
+template <int NPIX_, int NSHADOW_>
 struct Shape {
-  static constexpr int NPIX = NPIX_, NSLOT = NSLOT_, NSHADOW = NSHADOW_, NSUB = NPIX_ / 64;
+  static constexpr int NPIX = NPIX_, NSHADOW = NSHADOW_, NSUB = NPIX_ / 64;
+  static constexpr int BX = NSUB >= 2 ? 2 : 1;  // sub-blocks side by side (BX x NSUB/BX of 8x8)
 };
-using ShapeWide = Shape<128, 11, 2>;                 // <= 9 diffuse + specular + refraction
-using ShapeFull = Shape<64, 18, VCT_MAX_LIGHTS>;     // 16 diffuse + specular + refraction
+using ShapeWide = Shape<128, 2>;
+using ShapeFull = Shape<128, VCT_MAX_LIGHTS>;
+
+// 32.32 fixed point of a non-negative cone colour channel (< 2^31): sums of
+// these are exact, so the diffuse cones of a pixel may finish in any order.
+__device__ __forceinline__ unsigned long long fix32(float v) { return __float2ull_rn(v * 4294967296.0f); }
 
 // LDS of one workgroup of the cone marcher.
 template <class SH>
 struct ShadeLds {
-  float cone[SH::NSLOT][3][SH::NPIX];  // slot (diffuse 0..nd-1, specular nd, refraction nd+1) x pixel
-  float shadow[SH::NSHADOW][SH::NPIX];  // max(0, 1 - alpha) of the shadow cone per light and pixel
+  unsigned long long dsum[3][SH::NPIX];  // sum of the diffuse cones' rgb per pixel (fix32)
+  float cone[2][3][SH::NPIX];            // specular, refraction rgb per pixel
+  float shadow[SH::NSHADOW][SH::NPIX];   // max(0, 1 - alpha) of the shadow cone per light and pixel
   float pos[3][SH::NPIX], nrm[3][SH::NPIX], o1[3][SH::NPIX], o2[3][SH::NPIX], vdir[3][SH::NPIX];
   float spec_ap[SH::NPIX], eta[SH::NPIX];
   unsigned long long refr_mask[SH::NSUB];  // shading pixels with a transmissive material, per sub-block
@@ -572,6 +577,9 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
       o1 = normalize3(tangent_of(n));  // trace_diffuse (:146-147)
       o2 = normalize3(cross3(o1, n));
     }
+    L->dsum[0][me] = 0ull;
+    L->dsum[1][me] = 0ull;
+    L->dsum[2][me] = 0ull;
     const float3 vals[5] = {pos, n, o1, o2, vd};
     float (*dst[5])[SH::NPIX] = {L->pos, L->nrm, L->o1, L->o2, L->vdir};
 #pragma unroll
@@ -626,10 +634,16 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
     if (job >= kJobShadow) {
       L->shadow[job - kJobShadow][pix] = fmaxf(0.0f, 1.0f - k.c.w);
     } else {
-      const int slot = job < kJobSpecular ? job : (job == kJobSpecular ? nd : nd + 1);  // < SH::NSLOT
-      L->cone[slot][0][pix] = k.c.x;
-      L->cone[slot][1][pix] = k.c.y;
-      L->cone[slot][2][pix] = k.c.z;
+      if (job < kJobSpecular) {
+        atomicAdd(&L->dsum[0][pix], fix32(k.c.x));
+        atomicAdd(&L->dsum[1][pix], fix32(k.c.y));
+        atomicAdd(&L->dsum[2][pix], fix32(k.c.z));
+      } else {
+        const int slot = job == kJobSpecular ? 0 : 1;
+        L->cone[slot][0][pix] = k.c.x;
+        L->cone[slot][1][pix] = k.c.y;
+        L->cone[slot][2][pix] = k.c.z;
+      }
     }
   };
   for (;;) {
@@ -724,10 +738,11 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
   const bool do_refr = (L->refr_mask[grp] >> lane) & 1ull;
   float3 out = make_float3(0.f, 0.f, 0.f);
   if (nd > 0) {
-    float3 diff_acc = make_float3(0.f, 0.f, 0.f);
-    for (int q = 0; q < nd; ++q)
-      diff_acc = make_float3(diff_acc.x + L->cone[q][0][me], diff_acc.y + L->cone[q][1][me],
-                             diff_acc.z + L->cone[q][2][me]);
+    // the exact sum of the diffuse cones, rounded once (trace_diffuse adds
+    // them in cone order in FP32, :144-167)
+    const float u = 1.0f / 4294967296.0f;
+    const float3 diff_acc = make_float3(__ull2float_rn(L->dsum[0][me]) * u, __ull2float_rn(L->dsum[1][me]) * u,
+                                        __ull2float_rn(L->dsum[2][me]) * u);
     const float inv = 1.0f / (float)nd;
     out.x += m.diffuse[0] * (diff_acc.x * inv);
     out.y += m.diffuse[1] * (diff_acc.y * inv);
@@ -765,14 +780,14 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
     out.z += res.z + fminf(fmaxf(m.emission[2], 0.0f), 1.0f);
   }
   if (do_spec) {
-    out.x += m.specular[0] * L->cone[nd][0][me];
-    out.y += m.specular[1] * L->cone[nd][1][me];
-    out.z += m.specular[2] * L->cone[nd][2][me];
+    out.x += m.specular[0] * L->cone[0][0][me];
+    out.y += m.specular[1] * L->cone[0][1][me];
+    out.z += m.specular[2] * L->cone[0][2][me];
   }
   if (do_refr) {  // final colour override (:273-274)
-    const float3 tr = make_float3(m.transmittance[0] * L->cone[nd + 1][0][me],
-                                  m.transmittance[1] * L->cone[nd + 1][1][me],
-                                  m.transmittance[2] * L->cone[nd + 1][2][me]);
+    const float3 tr = make_float3(m.transmittance[0] * L->cone[1][0][me],
+                                  m.transmittance[1] * L->cone[1][1][me],
+                                  m.transmittance[2] * L->cone[1][2][me]);
     const float dv = m.dissolve;
     out = make_float3(tr.x * (1.0f - dv) + out.x * dv, tr.y * (1.0f - dv) + out.y * dv,
                       tr.z * (1.0f - dv) + out.z * dv);
@@ -801,7 +816,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void t
   sampler_init(A, &s_sampler);
   if (threadIdx.x == 0) s_shade.next_item = 0;
   __syncthreads();
-  const int px = blockIdx.x * (8 * SH::NSUB) + sub * 8 + (lane & 7), py = blockIdx.y * 8 + (lane >> 3);
+  const int px = blockIdx.x * (8 * SH::BX) + (sub % SH::BX) * 8 + (lane & 7);
+  const int py = blockIdx.y * (8 * (SH::NSUB / SH::BX)) + (sub / SH::BX) * 8 + (lane >> 3);
   shade_block<DT, SH>(A, px, py, (size_t)py * A.W + px, true, &s_shade, &s_sampler);
 }
 
@@ -1007,7 +1023,8 @@ int vct_trace_into(vct_context* ctx, const float4* gpos, const float4* gnrm, flo
 // per chunk of the largest the list can be.
 template <int DT, class SH>
 static void launch_shape(vct_context* ctx, const TraceArgs& A, const TraceSelect& sel, int W, int H) {
-  dim3 grid((W + 8 * SH::NSUB - 1) / (8 * SH::NSUB), (H + 7) / 8);
+  constexpr int bw = 8 * SH::BX, bh = 8 * (SH::NSUB / SH::BX);
+  dim3 grid((W + bw - 1) / bw, (H + bh - 1) / bh);
   if (sel.pix_list) {
     grid = dim3((sel.npix_list + SH::NPIX - 1) / SH::NPIX);
     hipLaunchKernelGGL((trace_list_kernel<DT, SH>), grid, dim3(256), 0, ctx->stream, A);
@@ -1082,9 +1099,9 @@ int vct_trace_select(vct_context* ctx, const float4* gpos, const float4* gnrm, f
     A.lights.l[i].kind = ctx->lights[i].kind;
   }
   A.lights.count = n;
-  // the wide shape holds up to 9 diffuse cones and shadows of up to 2 lights
+  // the wide shape holds the shadows of up to 2 lights
   const int nshadow = (ctx->en_direct && ctx->en_shadow) ? n : 0;
-  const bool wide = A.cones.n <= ShapeWide::NSLOT - 2 && nshadow <= ShapeWide::NSHADOW;
+  const bool wide = nshadow <= ShapeWide::NSHADOW;
   switch (ctx->dtype) {
     case VCT_RGBA8: rc = launch_trace<VCT_RGBA8>(ctx, A, sel, W, H, wide); break;
     case VCT_FP16: rc = launch_trace<VCT_FP16>(ctx, A, sel, W, H, wide); break;
