@@ -367,17 +367,17 @@ struct BrickBox {
 // Bricks overlapped by the voxel bounding box of a set-up triangle.  Fragment
 // positions are convex combinations of the vertices up to rounding; 0.01
 // voxel of margin covers it.
-__device__ __forceinline__ BrickBox brick_box(const Setup& s, uint32_t res, BrickGeom g) {
+__device__ __forceinline__ BrickBox brick_box(const float (&pw)[3][3], bool ok, uint32_t res, BrickGeom g) {
 #pragma clang fp contract(off)
   BrickBox bb;
   bb.n = 0;
-  if (!s.ok) return bb;
+  if (!ok) return bb;
   const float fres = (float)res;
   bb.n = 1;
   for (int k = 0; k < 3; ++k) {
     float lo = 3.0e38f, hi = -3.0e38f;
     for (int v = 0; v < 3; ++v) {
-      const float f = fres * (0.5f * s.w[v][k] + 0.5f);
+      const float f = fres * (0.5f * pw[v][k] + 0.5f);
       lo = fminf(lo, f);
       hi = fmaxf(hi, f);
     }
@@ -394,6 +394,10 @@ __device__ __forceinline__ BrickBox brick_box(const Setup& s, uint32_t res, Bric
     bb.n *= bb.nb[k];
   }
   return bb;
+}
+
+__device__ __forceinline__ BrickBox brick_box(const Setup& s, uint32_t res, BrickGeom g) {
+  return brick_box(s.w, s.ok, res, g);
 }
 
 __device__ __forceinline__ uint32_t brick_of(const BrickBox& bb, int k) {
@@ -416,6 +420,7 @@ __global__ __launch_bounds__(256) void vox_brick_count_kernel(DrawBatch b, VoxPa
     const uint32_t lt = t - dr.tri_base;
     tri_setup(dr, lt, vp.res, vp.cube_size, s);
     if (s.ok) recs[t] = make_rec(dr, lt, s);
+    else recs[t].area = 0;  // marks a skipped triangle (set-up triangles have area > 0)
   }
   const BrickBox bb = brick_box(s, vp.res, g);
   for (int k = 0; __ballot(k < bb.n) != 0ull; ++k) {
@@ -424,19 +429,23 @@ __global__ __launch_bounds__(256) void vox_brick_count_kernel(DrawBatch b, VoxPa
   }
 }
 
-// 3. list write (recomputes the same brick box from the same setup).
-__global__ __launch_bounds__(256) void vox_brick_write_kernel(DrawBatch b, VoxParams vp,
+// 3. list write: the same brick box, from the positions in the record.
+__global__ __launch_bounds__(256) void vox_brick_write_kernel(uint32_t ntri, VoxParams vp,
+                                                              const TriRec* __restrict__ recs,
                                                               const uint32_t* __restrict__ brick_off,
                                                               uint32_t* cursor, uint32_t* __restrict__ list) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const BrickGeom g = brick_geom(vp.res);
-  Setup s;
-  s.ok = false;
-  if (t < b.ntri) {
-    const int d = find_draw(b, t);
-    tri_setup(b.d[d], t - b.d[d].tri_base, vp.res, vp.cube_size, s);
+  float pw[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+  bool ok = false;
+  if (t < ntri) {
+    const TriRec* rp = recs + t;
+    ok = rp->area != 0;
+    if (ok)
+      for (int v = 0; v < 3; ++v)
+        for (int k = 0; k < 3; ++k) pw[v][k] = rp->w[v][k];
   }
-  const BrickBox bb = brick_box(s, vp.res, g);
+  const BrickBox bb = brick_box(pw, ok, vp.res, g);
   for (int k = 0; __ballot(k < bb.n) != 0ull; ++k) {
     const bool act = k < bb.n;
     const uint32_t key = act ? brick_of(bb, k) : 0u;
@@ -835,8 +844,8 @@ static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, cons
   const uint32_t total = ctx->h_readback[1];
   if (total) {
     if ((rc = vct_ensure(ctx->tri_off, (size_t)total * 4 + (size_t)total / 4 * 4, ctx))) return rc;
-    hipLaunchKernelGGL(vox_brick_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
-                       (const uint32_t*)boff, cursor, (uint32_t*)ctx->tri_off.p);
+    hipLaunchKernelGGL(vox_brick_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, ntri, vp,
+                       (const TriRec*)ctx->setup.p, (const uint32_t*)boff, cursor, (uint32_t*)ctx->tri_off.p);
     VCT_HIP(hipGetLastError());
   } else {
     hipLaunchKernelGGL(zero_u32_kernel, dim3((nbricks + 255) / 256), dim3(256), 0, ctx->stream, boff,
@@ -882,8 +891,8 @@ static int ref_batch_lists(vct_context* ctx, const DrawBatch& b, const VoxParams
   VCT_HIP(hipStreamSynchronize(ctx->stream));
   if (!pairs) return VCT_OK;
   if ((rc = vct_ensure(ctx->tri_off, (size_t)pairs * 4 + (size_t)pairs / 4 * 4, ctx))) return rc;
-  hipLaunchKernelGGL(vox_brick_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
-                     (const uint32_t*)boff, cursor, (uint32_t*)ctx->tri_off.p);
+  hipLaunchKernelGGL(vox_brick_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, ntri, vp,
+                     (const TriRec*)ctx->setup.p, (const uint32_t*)boff, cursor, (uint32_t*)ctx->tri_off.p);
   VCT_HIP(hipGetLastError());
   return VCT_OK;
 }
