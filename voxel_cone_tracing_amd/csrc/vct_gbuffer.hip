@@ -323,12 +323,24 @@ __device__ __forceinline__ void g_resolve_pixel(const GParams& p, unsigned long 
 // 4. one workgroup per 32x32 tile: the depth test runs on a tile depth buffer
 //    in LDS ((depth bits << 32 | original triangle id), 64-bit LDS atomicMin;
 //    equal depths resolve to the lower triangle id, the draw-order winner of
-//    GL_LESS).  The tile's triangles are taken 256 at a time, one per lane:
-//    a lane walks its triangle's box (clipped to the tile) with incremental
-//    edge functions; triangles whose clipped box exceeds kLaneBox pixels are
-//    queued and rasterised by the whole workgroup, one pixel per lane.  Then
+//    GL_LESS).  The tile's triangles are taken kChunk at a time: their
+//    records and edge functions at the first pixel of their box (clipped to
+//    the tile) go to LDS, and the rows of all their boxes are dealt over the
+//    lanes (a lane finds its triangle by a search over the prefix of the row
+//    counts).  A lane walks its row's span with incremental edge functions,
+//    so the work per lane is one row, not one triangle of any size.  Then
 //    every pixel of the tile is resolved and written once.
-constexpr int kLaneBox = 128;
+constexpr int kChunk = 128;
+
+// Row-walk data of one triangle of a chunk: edge values at (ci0, cj0) and
+// their per-column / per-row increments, reciprocals of the per-column ones
+// (0 when 0), the clipped box and the early-depth bound.
+struct GRowSetup {
+  int64_t w0[3], di[3], dj[3];
+  float inv[3];
+  int16_t ci0, ci1, cj0, pad;
+  uint32_t zb;
+};
 
 // Narrows [lo, hi] to the columns where w + d * (i - i0) >= 0 may hold, with
 // inv_d = 1 / d computed once per triangle (0 when d == 0).  The crossing
@@ -393,79 +405,108 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
                                                      float4* __restrict__ gpos, float4* __restrict__ gnrm) {
 #pragma clang fp contract(off)
   __shared__ unsigned long long s_depth[kTile * kTile];
-  __shared__ GRec s_big[256];
-  __shared__ uint32_t s_nbig;
-  const int tid = threadIdx.x;
+  __shared__ GRec s_rec[kChunk];
+  __shared__ GRowSetup s_row[kChunk];
+  __shared__ uint32_t s_end[kChunk];  // inclusive prefix of the row counts
+  __shared__ uint32_t s_wsum[kChunk / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int tile = blockIdx.x;
   const int x0 = (tile % tiles_x) * kTile, y0 = (tile / tiles_x) * kTile;
   for (int q = tid; q < kTile * kTile; q += 256) s_depth[q] = ~0ull;
-  if (tid == 0) s_nbig = 0;
   const uint32_t n = tile_count[tile], base = tile_off[tile];
   __syncthreads();
-  for (uint32_t b0 = 0; b0 < n; b0 += 256) {
-    if (b0 + tid < n) {
-      const GRec r = recs[list[base + b0 + tid]];
-      const int ci0 = max((int)r.i0, x0), ci1 = min((int)r.i1, x0 + kTile - 1);
-      const int cj0 = max((int)r.j0, y0), cj1 = min((int)r.j1, y0 + kTile - 1);
-      const int cw = ci1 - ci0 + 1, ch = cj1 - cj0 + 1;
-      if (cw > 0 && ch > 0) {
-        if (cw * ch > kLaneBox) {
-          s_big[atomicAdd(&s_nbig, 1u)] = r;
-        } else {
-          const GSetup s = g_setup_of(r);
-          const uint32_t zb = g_zmin_bits(r);
+  for (uint32_t b0 = 0; b0 < n; b0 += kChunk) {
+    if (tid < kChunk) {
+      uint32_t rows = 0;
+      if (b0 + tid < n) {
+        const GRec r = recs[list[base + b0 + tid]];
+        const int ci0 = max((int)r.i0, x0), ci1 = min((int)r.i1, x0 + kTile - 1);
+        const int cj0 = max((int)r.j0, y0), cj1 = min((int)r.j1, y0 + kTile - 1);
+        if (ci1 >= ci0 && cj1 >= cj0) {
+          rows = (uint32_t)(cj1 - cj0 + 1);
           GEdges e;
-          g_edges(s, ci0, cj0, e);
-          int64_t ra = e.w[0], rb = e.w[1], rc = e.w[2];
-          const float ia = e.di[0] ? 1.0f / (float)e.di[0] : 0.0f;
-          const float ib = e.di[1] ? 1.0f / (float)e.di[1] : 0.0f;
-          const float ic = e.di[2] ? 1.0f / (float)e.di[2] : 0.0f;
-          for (int j = cj0; j <= cj1; ++j) {
-            // the row's span: for each edge w(i) = w_row + di * (i - ci0) >= 0,
-            // bounded in float with a margin (the exact test below decides)
-            int lo = ci0, hi = ci1;
-            g_span(ra, e.di[0], ia, ci0, lo, hi);
-            g_span(rb, e.di[1], ib, ci0, lo, hi);
-            g_span(rc, e.di[2], ic, ci0, lo, hi);
-            if (lo <= hi) {
-              const int64_t k = lo - ci0;
-              int64_t wa = ra + e.di[0] * k, wb = rb + e.di[1] * k, wc = rc + e.di[2] * k;
-              for (int i = lo; i <= hi; ++i) {
-                float bw0, bw1, bw2;
-                if (wa >= 0 && wb >= 0 && wc >= 0 && !g_hidden(s_depth, i - x0, j - y0, zb) &&
-                    g_cover_w(s, e, wa, wb, wc, bw0, bw1, bw2))
-                  g_depth_test(s_depth, i - x0, j - y0, bw0, bw1, bw2, r);
-                wa += e.di[0];
-                wb += e.di[1];
-                wc += e.di[2];
-              }
-            }
-            ra += e.dj[0];
-            rb += e.dj[1];
-            rc += e.dj[2];
+          g_edges(g_setup_of(r), ci0, cj0, e);
+          GRowSetup g;
+          for (int k = 0; k < 3; ++k) {
+            g.w0[k] = e.w[k];
+            g.di[k] = e.di[k];
+            g.dj[k] = e.dj[k];
+            g.inv[k] = e.di[k] ? 1.0f / (float)e.di[k] : 0.0f;
           }
+          g.ci0 = (int16_t)ci0;
+          g.ci1 = (int16_t)ci1;
+          g.cj0 = (int16_t)cj0;
+          g.pad = 0;
+          g.zb = g_zmin_bits(r);
+          s_rec[tid] = r;
+          s_row[tid] = g;
+        }
+      }
+      uint32_t x = rows;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+      }
+      if (lane == 63) s_wsum[wave] = x;
+      s_end[tid] = x;  // within the wave; the waves before are added below
+    }
+    __syncthreads();
+    if (tid < kChunk) {
+      uint32_t add = 0;
+      for (int w = 0; w < wave; ++w) add += s_wsum[w];
+      s_end[tid] += add;
+    }
+    __syncthreads();
+    const uint32_t total = s_end[kChunk - 1];
+    for (uint32_t c = tid; c < total; c += 256) {
+      // the triangle of the wave's first row by binary search (the same for
+      // every lane), then each lane steps forward over the triangles its row
+      // lies past
+      const uint32_t c0 = c - (uint32_t)lane;
+      uint32_t lo = 0, hi = kChunk - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_end[mid] > c0) hi = mid;
+        else lo = mid + 1;
+      }
+      while (s_end[lo] <= c) ++lo;
+      const uint32_t row = c - (lo ? s_end[lo - 1] : 0u);
+      const GRowSetup& g = s_row[lo];
+      const GRec& r = s_rec[lo];
+      GSetup st;
+      st.area = r.area;
+      st.swap = r.swap != 0u;
+      GEdges e;
+      for (int k = 0; k < 3; ++k) {
+        e.di[k] = g.di[k];
+        e.ex[k] = g.dj[k] >> 8;   // dj = ex * 256, di = -ey * 256 (exact)
+        e.ey[k] = -(g.di[k] >> 8);
+      }
+      const int ci0 = g.ci0, ci1 = g.ci1, j = g.cj0 + (int)row;
+      const int64_t ra = g.w0[0] + g.dj[0] * (int64_t)row;
+      const int64_t rb = g.w0[1] + g.dj[1] * (int64_t)row;
+      const int64_t rc = g.w0[2] + g.dj[2] * (int64_t)row;
+      // the row's span: for each edge w(i) = w_row + di * (i - ci0) >= 0,
+      // bounded in float with a margin (the exact test below decides)
+      int lo_i = ci0, hi_i = ci1;
+      g_span(ra, g.di[0], g.inv[0], ci0, lo_i, hi_i);
+      g_span(rb, g.di[1], g.inv[1], ci0, lo_i, hi_i);
+      g_span(rc, g.di[2], g.inv[2], ci0, lo_i, hi_i);
+      if (lo_i <= hi_i) {
+        const int64_t k0 = lo_i - ci0;
+        int64_t wa = ra + g.di[0] * k0, wb = rb + g.di[1] * k0, wc = rc + g.di[2] * k0;
+        const uint32_t zb = g.zb;
+        for (int i = lo_i; i <= hi_i; ++i) {
+          float bw0, bw1, bw2;
+          if (wa >= 0 && wb >= 0 && wc >= 0 && !g_hidden(s_depth, i - x0, j - y0, zb) &&
+              g_cover_w(st, e, wa, wb, wc, bw0, bw1, bw2))
+            g_depth_test(s_depth, i - x0, j - y0, bw0, bw1, bw2, r);
+          wa += g.di[0];
+          wb += g.di[1];
+          wc += g.di[2];
         }
       }
     }
-    __syncthreads();
-    // large triangles: the whole workgroup, one pixel per lane
-    const uint32_t nbig = s_nbig;
-    for (uint32_t k = 0; k < nbig; ++k) {
-      const GRec& r = s_big[k];
-      const int ci0 = max((int)r.i0, x0), ci1 = min((int)r.i1, x0 + kTile - 1);
-      const int cj0 = max((int)r.j0, y0), cj1 = min((int)r.j1, y0 + kTile - 1);
-      const int cw = ci1 - ci0 + 1, npx = cw * (cj1 - cj0 + 1);
-      const GSetup s = g_setup_of(r);
-      const uint32_t zb = g_zmin_bits(r);
-      for (int q = tid; q < npx; q += 256) {
-        const int i = ci0 + q % cw, j = cj0 + q / cw;
-        float bw0, bw1, bw2;
-        if (!g_hidden(s_depth, i - x0, j - y0, zb) && g_cover(s, i, j, bw0, bw1, bw2))
-          g_depth_test(s_depth, i - x0, j - y0, bw0, bw1, bw2, r);
-      }
-    }
-    __syncthreads();
-    if (tid == 0) s_nbig = 0;
     __syncthreads();
   }
   for (int q = tid; q < kTile * kTile; q += 256) {
