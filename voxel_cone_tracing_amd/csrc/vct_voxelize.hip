@@ -487,8 +487,8 @@ __device__ __forceinline__ void store_mean(void* level0, uint32_t v, uint32_t cn
 }
 
 // 4. one workgroup per brick (blockIdx.x = the brick's Morton index).  The
-//    brick's triangles are taken 1024 at a time; the pixels of their boxes
-//    inside the brick's footprint (one pixel of margin) are spread over the
+//    brick's triangles are taken 256 at a time; the pixels of their boxes
+//    inside the brick's footprint on the projection plane are spread over the
 //    lanes (a lane finds its triangle by a binary search over the prefix of
 //    the box sizes), coverage is tested from a compact copy of the records in
 //    LDS, and only covered pixels read the rest of the record.
@@ -629,10 +629,16 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
       const int ia = axis == 0 ? 1 : 0, ib = axis == 2 ? 1 : 2;
       const int qh = (int)(fl >> 8);
       const int qx0 = rp->qx0, qy0 = rp->qy0, qw = rp->qw;
-      // footprint with one pixel of margin, clipped to the triangle's box
+      // the brick's footprint clipped to the triangle's box.  A covered
+      // pixel (i, j) lands in voxel (i >> 1, j >> 1) on the projection plane:
+      // its centre sits a quarter voxel inside that voxel, and the
+      // interpolated position is a convex combination of the vertices that
+      // differs from the centre by the 1/512-pixel vertex snap plus rounding,
+      // far below a quarter voxel.  So pixels outside the footprint never
+      // land in the brick.
       const int fx = (int)bc[ia] * span, fy = (int)bc[ib] * span;
-      const int i0 = max(qx0, fx - 1), i1 = min(min(qx0 + 2 * qw - 1, fx + span), lim);
-      const int j0 = max(qy0, fy - 1), j1 = min(min(qy0 + 2 * qh - 1, fy + span), lim);
+      const int i0 = max(qx0, fx), i1 = min(min(qx0 + 2 * qw - 1, fx + span - 1), lim);
+      const int j0 = max(qy0, fy), j1 = min(min(qy0 + 2 * qh - 1, fy + span - 1), lim);
       if (i1 >= i0 && j1 >= j0) {
         const uint32_t cw = (uint32_t)(i1 - i0 + 1);
         cnt = cw * (uint32_t)(j1 - j0 + 1);
