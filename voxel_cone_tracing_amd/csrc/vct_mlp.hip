@@ -29,6 +29,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIn = VCT_MLP_IN, kHid = VCT_MLP_HIDDEN, kOut = VCT_MLP_OUT;
 constexpr int kKP = 64;  // padded input width
+constexpr int kFbTile = 32;  // the fallback's tiles (pixels per edge)
 // flat master layout
 constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kHid * kIn, kOffW2 = kOffB1 + kHid, kOffB2 = kOffW2 + kHid * kHid,
               kOffW3 = kOffB2 + kHid, kOffB3 = kOffW3 + kOut * kHid, kParams = kOffB3 + kOut;
@@ -197,7 +198,13 @@ struct InferArgs {
   FeatArgs F;
   MlpDev P;
   float4* image;
-  uint32_t* trace_list;  // [0]: count (zero before the launch), then pixel indices
+  uint32_t* trace_list;  // [0]: count (zero before the launch), then pixel indices; null: not listed
+  // optional: the pixels of a list (256 per workgroup) instead of 16x16 blocks
+  const uint32_t* pix_list;
+  uint32_t npix_list;
+  // optional, blocks only: skip the 16x16 blocks of 32x32 tiles with skip_tile[t] == 1
+  const uint32_t* skip_tile;
+  int tiles_x;
 };
 
 __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs A) {
@@ -205,13 +212,29 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs A) {
   __shared__ float s_base[256][3];
   __shared__ float s_alb[256][3];
   __shared__ int s_st[256];
+  __shared__ int s_px[256], s_py[256];
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-  const int px = blockIdx.x * 16 + (t & 15), py = blockIdx.y * 16 + (t >> 4);
+  if (!A.pix_list && A.skip_tile &&
+      A.skip_tile[(blockIdx.y * 16 / kFbTile) * A.tiles_x + blockIdx.x * 16 / kFbTile] == 1u)
+    return;  // the whole workgroup: its tile is traced in full
+  int px = blockIdx.x * 16 + (t & 15), py = blockIdx.y * 16 + (t >> 4);
+  if (A.pix_list) {
+    const uint32_t k = blockIdx.x * 256u + (uint32_t)t;
+    px = py = -1;
+    if (k < A.npix_list) {
+      const uint32_t q = A.pix_list[k];
+      px = (int)(q % (uint32_t)A.F.W);
+      py = (int)(q / (uint32_t)A.F.W);
+    }
+  }
+  s_px[t] = px;
+  s_py[t] = py;
   float f[kIn], alb[3] = {0.f, 0.f, 0.f};
   float3 base = make_float3(0.f, 0.f, 0.f);
   int st = kBackground;
-  if (px < A.F.W && py < A.F.H) st = pixel_features(A.F, px, py, f, alb, base);
-  {  // pixels to trace: one atomic per wave
+  const bool inside = px >= 0 && py >= 0 && px < A.F.W && py < A.F.H;
+  if (inside) st = pixel_features(A.F, px, py, f, alb, base);
+  if (A.trace_list) {  // pixels to trace: one atomic per wave
     const unsigned long long m = __ballot(st == kTraceIt);
     uint32_t b0 = 0;
     if (lane == 0 && m) b0 = atomicAdd(A.trace_list, (uint32_t)__popcll(m));
@@ -245,8 +268,8 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs A) {
   if (lane < 16) {
     for (int nt = 0; nt < 4; ++nt) {
       const int s = wave * 64 + nt * 16 + lane;
-      const int qx = blockIdx.x * 16 + (s & 15), qy = blockIdx.y * 16 + (s >> 4);
-      if (qx >= A.F.W || qy >= A.F.H) continue;
+      const int qx = s_px[s], qy = s_py[s];
+      if (qx < 0 || qy < 0 || qx >= A.F.W || qy >= A.F.H) continue;
       float4 o;
       if (s_st[s] == kTraceIt) {  // left for tracing
         A.image[(size_t)qy * A.F.W + qx] = make_float4(0.f, 0.f, 0.f, -1.0f);
@@ -595,7 +618,6 @@ namespace vct {
 // pixels farthest from that grid), and is traced in full if it still fails.
 // Probe values are written into the image; a probe on a pixel that the
 // interpolation left for tracing (w < 0) counts as exact.
-constexpr int kFbTile = 32;
 constexpr int kProbes = 16;  // per tile: one or two 4x4 grids
 static_assert(kProbes == 16 || kProbes == 32, "probe grids");
 
@@ -663,7 +685,7 @@ __global__ __launch_bounds__(256) void tile_estimate_kernel(const float4* __rest
                                                             const uint32_t* __restrict__ probe_pix,
                                                             const float4* __restrict__ gpos, float4* image,
                                                             uint32_t ntiles, float* __restrict__ est,
-                                                            uint32_t* __restrict__ ncov) {
+                                                            uint32_t* __restrict__ ncov, int write_probes) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
   float sse = 0.0f;
@@ -679,7 +701,8 @@ __global__ __launch_bounds__(256) void tile_estimate_kernel(const float4* __rest
     const float dx = v.x - m.x, dy = v.y - m.y, dz = v.z - m.z;
     sse += dx * dx + dy * dy + dz * dz;
   }
-  for (int k = 0; k < kProbes; ++k) image[probe_pix[t * kProbes + k]] = probe_val[t * kProbes + k];
+  if (write_probes)
+    for (int k = 0; k < kProbes; ++k) image[probe_pix[t * kProbes + k]] = probe_val[t * kProbes + k];
   est[t] = sse;
   ncov[t] = nc;
 }
@@ -1001,16 +1024,19 @@ int trace_lowres(vct_context* ctx) {
 __global__ void zero_count_kernel(uint32_t* p) { *p = 0u; }
 
 // Upsampling, then the full-resolution trace of the pixels it listed.
-int infer_full(vct_context* ctx) {
+int infer_full(vct_context* ctx, const uint32_t* skip_tile = nullptr) {
   const size_t npix = (size_t)ctx->width * ctx->height;
   int rc;
   if ((rc = vct_ensure(ctx->fb_pix, (npix + 1) * 4, ctx))) return rc;
   hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, ctx->stream, (uint32_t*)ctx->fb_pix.p);
   InferArgs A;
+  memset(&A, 0, sizeof(A));
   A.F = feat_args(ctx);
   A.P = mlp_dev(ctx);
   A.image = (float4*)ctx->image.p;
   A.trace_list = (uint32_t*)ctx->fb_pix.p;
+  A.skip_tile = skip_tile;
+  A.tiles_x = (ctx->width + kFbTile - 1) / kFbTile;
   dim3 grid((ctx->width + 15) / 16, (ctx->height + 15) / 16);
   hipLaunchKernelGGL(mlp_infer_kernel, grid, dim3(256), 0, ctx->stream, A);
   VCT_HIP(hipGetLastError());
@@ -1064,9 +1090,56 @@ int refine_probes(vct_context* ctx) {
                           (float4*)ctx->probe_val.p, W, H, 0, nullptr, sel);
 }
 
+// Stage-1 selection of the frame budget: estimates from the probes and the
+// learned values at the probe pixels, then the tiles to trace in full (flag 1,
+// listed in fb_list).  write_probes puts the probe values into the image.
+int budget_select(vct_context* ctx, bool write_probes) {
+  const int W = ctx->width, H = ctx->height;
+  const int tx = (W + kFbTile - 1) / kFbTile, ty = (H + kFbTile - 1) / kFbTile;
+  const uint32_t ntiles = (uint32_t)(tx * ty);
+  int rc;
+  if ((rc = vct_ensure(ctx->fb_list, (size_t)(ntiles + 1) * 4, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->fb_flag, (size_t)ntiles * 4, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->fb_est, (size_t)ntiles * 8, ctx))) return rc;
+  const float max_mse = powf(10.0f, -ctx->fb_psnr_db / 10.0f);
+  float* est = (float*)ctx->fb_est.p;
+  uint32_t* ncov = (uint32_t*)(est + ntiles);
+  hipLaunchKernelGGL(tile_estimate_kernel, dim3((ntiles + 255) / 256), dim3(256), 0, ctx->stream,
+                     (const float4*)ctx->probe_val.p, (const uint32_t*)ctx->probe_pix.p, (const float4*)ctx->g_pos.p,
+                     (float4*)ctx->image.p, ntiles, est, ncov, write_probes ? 1 : 0);
+  hipLaunchKernelGGL(tile_budget_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const float*)est,
+                     (const uint32_t*)ncov, ntiles, max_mse, (uint32_t*)ctx->fb_list.p, (uint32_t*)ctx->fb_flag.p);
+  VCT_HIP(hipGetLastError());
+  return VCT_OK;
+}
+
+__global__ void probe_write_kernel(const float4* __restrict__ val, const uint32_t* __restrict__ pix, uint32_t n,
+                                   float4* __restrict__ image) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) image[pix[k]] = val[k];
+}
+
+// The learned values at the probe pixels only (no trace list).
+int infer_probes(vct_context* ctx) {
+  const int tx = (ctx->width + kFbTile - 1) / kFbTile, ty = (ctx->height + kFbTile - 1) / kFbTile;
+  const uint32_t n = (uint32_t)(tx * ty) * kProbes;
+  InferArgs A;
+  memset(&A, 0, sizeof(A));
+  A.F = feat_args(ctx);
+  A.P = mlp_dev(ctx);
+  A.image = (float4*)ctx->image.p;
+  A.pix_list = (const uint32_t*)ctx->probe_pix.p;
+  A.npix_list = n;
+  hipLaunchKernelGGL(mlp_infer_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, A);
+  VCT_HIP(hipGetLastError());
+  return VCT_OK;
+}
+
 // Per-tile check and refinement of the learned frame, after refine_probes
-// (see tile_select_kernel and tile_budget_kernel).
-int refine(vct_context* ctx) {
+// (see tile_select_kernel and tile_budget_kernel); `selected`: the frame
+// budget's selection already ran (budget_select) and the probes are in the
+// image.
+int refine(vct_context* ctx, bool selected) {
   const int W = ctx->width, H = ctx->height;
   const int tx = (W + kFbTile - 1) / kFbTile, ty = (H + kFbTile - 1) / kFbTile;
   const uint32_t ntiles = (uint32_t)(tx * ty);
@@ -1086,15 +1159,8 @@ int refine(vct_context* ctx) {
     hipLaunchKernelGGL(tile_select_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const float4*)ctx->probe_val.p,
                        (const uint32_t*)ctx->probe_pix.p, gpos, image, (const uint32_t*)nullptr, ntiles, max_mse, l1,
                        flags, 0u, 1u);
-  } else {  // frame budget: largest estimated errors first
-    if ((rc = vct_ensure(ctx->fb_est, (size_t)ntiles * 8, ctx))) return rc;
-    float* est = (float*)ctx->fb_est.p;
-    uint32_t* ncov = (uint32_t*)(est + ntiles);
-    hipLaunchKernelGGL(tile_estimate_kernel, dim3((ntiles + 255) / 256), dim3(256), 0, ctx->stream,
-                       (const float4*)ctx->probe_val.p, (const uint32_t*)ctx->probe_pix.p, gpos, image, ntiles, est,
-                       ncov);
-    hipLaunchKernelGGL(tile_budget_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const float*)est,
-                       (const uint32_t*)ncov, ntiles, max_mse, l1, flags);
+  } else if (!selected) {  // frame budget: largest estimated errors first
+    if ((rc = budget_select(ctx, true))) return rc;
   }
   VCT_HIP(hipGetLastError());
   // pixels the quarter-resolution stage left, in tiles that passed: on the
@@ -1395,14 +1461,33 @@ int vct_render_learned(vct_ctx ctx) {
   VCT_HIP(hipEventRecord(ctx->ev[7], ctx->stream));
   ctx->ev_recorded[3] = true;
   VCT_HIP(hipEventRecord(ctx->ev[8], ctx->stream));
-  if ((rc = infer_full(ctx))) return rc;
+  // frame budget: the learned values at the probes first, the tiles to trace
+  // in full picked from them, then the network only outside those tiles
+  const bool budget = fallback && !ctx->fb_half;
+  if (budget) {
+    if ((rc = infer_probes(ctx))) return rc;
+    VCT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+    if ((rc = budget_select(ctx, false))) return rc;
+    if ((rc = infer_full(ctx, (const uint32_t*)ctx->fb_flag.p))) return rc;
+  } else if ((rc = infer_full(ctx))) {
+    return rc;
+  }
   VCT_HIP(hipEventRecord(ctx->ev[9], ctx->stream));
   ctx->ev_recorded[4] = true;
   VCT_HIP(hipEventRecord(ctx->ev[10], ctx->stream));
   ctx->fb_ran = false;
-  if (fallback) {
+  if (budget) {
+    const uint32_t np = (uint32_t)(((ctx->width + kFbTile - 1) / kFbTile) * ((ctx->height + kFbTile - 1) / kFbTile)) *
+                        kProbes;
+    hipLaunchKernelGGL(probe_write_kernel, dim3((np + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const float4*)ctx->probe_val.p, (const uint32_t*)ctx->probe_pix.p, np,
+                       (float4*)ctx->image.p);
+    VCT_HIP(hipGetLastError());
+    if ((rc = refine(ctx, true))) return rc;
+    ctx->fb_ran = true;
+  } else if (fallback) {
     VCT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
-    if ((rc = refine(ctx))) return rc;
+    if ((rc = refine(ctx, false))) return rc;
     ctx->fb_ran = true;
   } else if ((rc = trace_listed_pixels(ctx))) {
     return rc;
