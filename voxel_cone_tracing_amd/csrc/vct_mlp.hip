@@ -677,34 +677,43 @@ __global__ __launch_bounds__(1024) void tile_select_kernel(const float4* __restr
   if (tid == 0) out[0] = s_base;
 }
 
-// Stage-1 estimate, one thread per tile: est[t] = sum over the tile's covered
-// probes of the squared error (3 channels) of the learned value, ncov[t] =
-// covered probes; the probe values go into the image.  A probe on a pixel
-// the interpolation left for tracing (w < 0) counts as exact.
+// Stage-1 estimate: est[t] = sum over the tile's covered probes of the
+// squared error (3 channels) of the learned value, ncov[t] = covered probes;
+// with write_probes the probe values then go into the image.  A probe on a
+// pixel the interpolation left for tracing (w < 0) counts as exact.
 __global__ __launch_bounds__(256) void tile_estimate_kernel(const float4* __restrict__ probe_val,
                                                             const uint32_t* __restrict__ probe_pix,
                                                             const float4* __restrict__ gpos, float4* image,
                                                             uint32_t ntiles, float* __restrict__ est,
                                                             uint32_t* __restrict__ ncov, int write_probes) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntiles) return;
-  float sse = 0.0f;
-  uint32_t nc = 0;
-#pragma unroll 4
-  for (int k = 0; k < kProbes; ++k) {
-    const uint32_t q = t * kProbes + k;
-    const uint32_t p = probe_pix[q];
-    const float4 v = probe_val[q], m = image[p];
-    if (gpos[p].w == 0.0f) continue;  // background
-    ++nc;
-    if (m.w < 0.0f) continue;  // left for tracing: exact
-    const float dx = v.x - m.x, dy = v.y - m.y, dz = v.z - m.z;
-    sse += dx * dx + dy * dy + dz * dz;
+  // one lane per probe, kProbes lanes per tile (a power of two dividing 64);
+  // the tile's sums are reduced across its lanes in a fixed order
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t t = g / kProbes;
+  float e = 0.0f;
+  uint32_t c = 0;
+  if (t < ntiles) {
+    const uint32_t p = probe_pix[g];
+    const float4 v = probe_val[g], m = image[p];
+    if (gpos[p].w != 0.0f) {  // covered
+      c = 1;
+      if (m.w >= 0.0f) {  // w < 0: left for tracing, exact
+        const float dx = v.x - m.x, dy = v.y - m.y, dz = v.z - m.z;
+        e = dx * dx + dy * dy + dz * dz;
+      }
+    }
   }
-  if (write_probes)
-    for (int k = 0; k < kProbes; ++k) image[probe_pix[t * kProbes + k]] = probe_val[t * kProbes + k];
-  est[t] = sse;
-  ncov[t] = nc;
+  for (int o = kProbes / 2; o; o >>= 1) {
+    e += __shfl_xor(e, o, 64);
+    c += __shfl_xor(c, o, 64);
+  }
+  if (t < ntiles) {
+    if (write_probes) image[probe_pix[g]] = probe_val[g];
+    if (g % kProbes == 0) {
+      est[t] = e;
+      ncov[t] = c;
+    }
+  }
 }
 
 // One workgroup: the tiles to trace in full, largest estimated error first,
@@ -1104,7 +1113,7 @@ int budget_select(vct_context* ctx, bool write_probes) {
   const float max_mse = powf(10.0f, -ctx->fb_psnr_db / 10.0f);
   float* est = (float*)ctx->fb_est.p;
   uint32_t* ncov = (uint32_t*)(est + ntiles);
-  hipLaunchKernelGGL(tile_estimate_kernel, dim3((ntiles + 255) / 256), dim3(256), 0, ctx->stream,
+  hipLaunchKernelGGL(tile_estimate_kernel, dim3((ntiles * kProbes + 255) / 256), dim3(256), 0, ctx->stream,
                      (const float4*)ctx->probe_val.p, (const uint32_t*)ctx->probe_pix.p, (const float4*)ctx->g_pos.p,
                      (float4*)ctx->image.p, ntiles, est, ncov, write_probes ? 1 : 0);
   hipLaunchKernelGGL(tile_budget_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const float*)est,
