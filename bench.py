@@ -20,7 +20,7 @@ G-buffer and trace in double precision) over the covered pixels of a
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload soup256]
                     [--weights file.npy] [--no-quality] [--no-cpu-baseline]
 
-Prints one JSON line.  --gpus above 1 exits: this build runs on one GPU.
+Prints one JSON line.  --gpus above 1 exits: the benchmark runs on one GPU.
 """
 from __future__ import annotations
 

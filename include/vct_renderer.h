@@ -117,7 +117,7 @@ struct Renderer {
   void queue_point_light(point_light_t& point_light);
   void render();
 
-  // Additions of this build
+  // Additions beyond the reference's Renderer
   model_id_t add_model(const float* positions, const float* normals, const uint32_t* tri_material, uint32_t ntri);
   void set_grid_format(unsigned res, int levels, int dtype, int average);
   std::vector<float> image();  // RGBA32F, row 0 = bottom

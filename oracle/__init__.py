@@ -5,8 +5,8 @@ this package, and only as the checker: the product never calls it.  The
 C++ restatement lives in oracle/vct_oracle.cpp (see its header for the
 reference file:line of every function); `make -C oracle` builds it.
 
-Parity status of each function and what pins it is listed at the top of
-DESIGN.md ("Oracle" section) and enforced by tests/test_oracle_*.py.
+What pins each function (golden vectors, closed forms, brute force, the
+reference shaders under llvmpipe) is enforced by tests/test_oracle_*.py.
 """
 from __future__ import annotations
 
