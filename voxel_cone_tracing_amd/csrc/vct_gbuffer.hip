@@ -614,7 +614,9 @@ int vct_launch_gbuffer_phase(vct_context* ctx, int phase) {
     }
   }
   if (!(phase & 2)) return VCT_OK;
-  if (ntri) VCT_HIP(hipStreamSynchronize(ctx->stream));
+  // also when nothing was counted: a copy an interrupted earlier frame left
+  // in flight must not land after the word is read
+  VCT_HIP(hipStreamSynchronize(ctx->stream));
   const uint32_t total = ctx->h_readback[0];
   if (total) {
     if ((rc = vct_ensure(ctx->g_list, (size_t)total * 4 + (size_t)total / 4 * 4, ctx))) return rc;

@@ -846,7 +846,9 @@ static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, cons
     }
   }
   if (!(phase & 2)) return VCT_OK;
-  if (ntri) VCT_HIP(hipStreamSynchronize(ctx->stream));
+  // also when nothing was counted: a copy an interrupted earlier frame left
+  // in flight must not land after the word is read
+  VCT_HIP(hipStreamSynchronize(ctx->stream));
   const uint32_t total = ctx->h_readback[1];
   if (total) {
     if ((rc = vct_ensure(ctx->tri_off, (size_t)total * 4 + (size_t)total / 4 * 4, ctx))) return rc;
