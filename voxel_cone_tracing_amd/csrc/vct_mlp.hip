@@ -750,14 +750,28 @@ __global__ __launch_bounds__(1024) void tile_budget_kernel(const float* __restri
     s_count = 0u;
   }
   __syncthreads();
-  for (uint32_t t = tid; t < ntiles; t += 1024) {
-    const float e = est[t];
-    const int b = budget_bin(e);
-    atomicAdd(&s_cov, ncov[t]);
-    if (b >= 0) {
-      const unsigned long long q = (unsigned long long)(e * kFix);
-      atomicAdd(&s_mass[b], q);
-      atomicAdd(&s_total, q);
+  {
+    // per-bin masses by LDS atomics; the totals summed per wave first (exact
+    // integers, so the order does not matter)
+    unsigned long long tot = 0ull;
+    uint32_t cov = 0u;
+    for (uint32_t t = tid; t < ntiles; t += 1024) {
+      const float e = est[t];
+      const int b = budget_bin(e);
+      cov += ncov[t];
+      if (b >= 0) {
+        const unsigned long long q = (unsigned long long)(e * kFix);
+        atomicAdd(&s_mass[b], q);
+        tot += q;
+      }
+    }
+    for (int o = 32; o; o >>= 1) {
+      tot += __shfl_xor(tot, o, 64);
+      cov += __shfl_xor(cov, o, 64);
+    }
+    if ((tid & 63) == 0) {
+      atomicAdd(&s_total, tot);
+      atomicAdd(&s_cov, cov);
     }
   }
   __syncthreads();
