@@ -202,26 +202,23 @@ __device__ __forceinline__ bool top_left(int64_t dx, int64_t dy) { return dy < 0
 // Edge functions of a record in CCW order (a, b, c) = (0, 1, 2) or (0, 2, 1)
 // at pixel (i, j), and their exact per-column / per-row increments.
 struct VEdges {
-  int64_t w[3], di[3], dj[3];
+  int64_t w[3];
   int64_t ex[3], ey[3];
 };
 
+// Differences of snapped coordinates fit 32 bits (vertices lie within 65
+// grid widths of the origin: |X|, |Y| < 2^25 in 1/256 pixel, and 256 i,
+// 256 j < 2^20), so every product is a 32 x 32 -> 64-bit multiply.
 template <typename Rec>
 __device__ __forceinline__ void vox_edges(const Rec& s, int i, int j, VEdges& e) {
-  const int64_t px = (int64_t)i * 256, py = (int64_t)j * 256;
+  const int32_t px = i * 256, py = j * 256;
   const bool sw = (s.flags & 1u) != 0u;
-  const int64_t ax = s.X[0], ay = s.Y[0];
-  const int64_t bx = sw ? s.X[2] : s.X[1], by = sw ? s.Y[2] : s.Y[1];
-  const int64_t cx = sw ? s.X[1] : s.X[2], cy = sw ? s.Y[1] : s.Y[2];
-  e.w[0] = (cx - bx) * (py - by) - (cy - by) * (px - bx);
-  e.w[1] = (ax - cx) * (py - cy) - (ay - cy) * (px - cx);
-  e.w[2] = (bx - ax) * (py - ay) - (by - ay) * (px - ax);
-  e.di[0] = -(cy - by) * 256;
-  e.di[1] = -(ay - cy) * 256;
-  e.di[2] = -(by - ay) * 256;
-  e.dj[0] = (cx - bx) * 256;
-  e.dj[1] = (ax - cx) * 256;
-  e.dj[2] = (bx - ax) * 256;
+  const int32_t ax = s.X[0], ay = s.Y[0];
+  const int32_t bx = sw ? s.X[2] : s.X[1], by = sw ? s.Y[2] : s.Y[1];
+  const int32_t cx = sw ? s.X[1] : s.X[2], cy = sw ? s.Y[1] : s.Y[2];
+  e.w[0] = (int64_t)(cx - bx) * (int64_t)(py - by) - (int64_t)(cy - by) * (int64_t)(px - bx);
+  e.w[1] = (int64_t)(ax - cx) * (int64_t)(py - cy) - (int64_t)(ay - cy) * (int64_t)(px - cx);
+  e.w[2] = (int64_t)(bx - ax) * (int64_t)(py - ay) - (int64_t)(by - ay) * (int64_t)(px - ax);
   e.ex[0] = cx - bx;
   e.ey[0] = cy - by;
   e.ex[1] = ax - cx;
