@@ -32,7 +32,11 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 extern "C" {
 
@@ -369,12 +373,6 @@ void frag_color(V3<T> pos, V3<T> nrm_interp, const orc_material& m, const orc_li
   for (int k = 0; k < 4; ++k) out[k] = std::min(std::max(out[k], (T)0), (T)1);
 }
 
-struct Frag {
-  uint32_t vox;   // linear index (z * res + y) * res + x
-  uint64_t order;  // canonical sequence number
-  float c32[4];
-  double c64[4];
-};
 
 // imageAtomicRGBA8Avg helpers (voxelize.frag:66-93)
 uint32_t conv_vec4_to_rgba8(const float v[4]) {
@@ -425,119 +423,166 @@ uint32_t orc_rgba8_avg_step(uint32_t stored, const float color[4]) {
 //  out_f64  (res^3 x 4 double, may be NULL): exact mean in double
 //  out_count (res^3 u32, may be NULL): fragments per voxel
 // Coordinates are linear, x fastest.  Returns the number of fragments.
+//
+// The canonical order of the fragments of a voxel is GL's submission order:
+// triangle index, then the pixel order of for_each_pixel.  The work is split
+// into z-slabs of the grid, one thread per slab: a thread walks every
+// triangle whose voxel z-range meets its slab, in index order, and keeps
+// only the fragments whose voxel lies in the slab, so each voxel sees its
+// fragments in the canonical order and the running average, the 16.16 sums
+// and the double sums are those of a single sequential pass.
 int64_t orc_voxelize(int res, float cube_size, const float* M, int ntri, const float* pos, const float* nrm,
                      const uint32_t* tri_mat, const orc_material* mats, const orc_light* lights, int nlights,
                      uint32_t* out_ref, float* out_q32, double* out_f64, uint32_t* out_count) {
   float N[9];
   normal_matrix(M, N);
   const int W = 2 * res;  // viewport 2res x 2res (renderer.cpp:339-340)
-  std::vector<Frag> frags;
-  uint64_t seq = 0;
-  for (int t = 0; t < ntri; ++t) {
-    V3<float> w[3];
-    V3<double> wd[3];
-    V3<float> n[3];
-    V3<double> nd[3];
-    for (int v = 0; v < 3; ++v) {
-      // voxelize.vert:26
-      V3<float> p = xform_point<float>(M, pos + t * 9 + v * 3);
-      w[v] = mk(p.x / cube_size, p.y / cube_size, p.z / cube_size);
-      V3<double> pd = xform_point<double>(M, pos + t * 9 + v * 3);
-      wd[v] = mk(pd.x / cube_size, pd.y / cube_size, pd.z / cube_size);
-      // voxelize.vert:28
-      n[v] = normalize(xform_normal<float>(N, nrm + t * 9 + v * 3));
-      nd[v] = normalize(xform_normal<double>(N, nrm + t * 9 + v * 3));
-    }
-    // voxelize.geom:27-51
-    const V3<float> e1 = sub(w[1], w[0]), e2 = sub(w[2], w[0]);
-    const V3<float> cr = cross(e1, e2);
-    const float px = fabsf(cr.x), py = fabsf(cr.y), pz = fabsf(cr.z);
-    float xw[3], yw[3];
-    bool inside = true;
-    for (int v = 0; v < 3; ++v) {
-      float a, b;
-      if (pz > px && pz > py) {
-        a = w[v].x;
-        b = w[v].y;
-      } else if (px > py && px > pz) {
-        a = w[v].y;
-        b = w[v].z;
-      } else {
-        a = w[v].x;
-        b = w[v].z;
-      }
-      if (!(fabsf(a) < 64.0f && fabsf(b) < 64.0f)) inside = false;
-      xw[v] = a * (float)res + (float)res;  // viewport transform
-      yw[v] = b * (float)res + (float)res;
-    }
-    if (!inside) continue;
-    Tri2D tri;
-    setup2d(xw, yw, W, W, tri);
-    const orc_material& m = mats[tri_mat ? tri_mat[t] : 0];
-    for_each_pixel(tri, W, W, [&](int, int, const int64_t* bw) {
-      const float fa = (float)tri.area;
-      const float b[3] = {(float)bw[0] / fa, (float)bw[1] / fa, (float)bw[2] / fa};
-      V3<float> p = mk(b[0] * w[0].x + b[1] * w[1].x + b[2] * w[2].x, b[0] * w[0].y + b[1] * w[1].y + b[2] * w[2].y,
-                       b[0] * w[0].z + b[1] * w[1].z + b[2] * w[2].z);
-      // voxelize.frag:156-157, ivec3() truncates toward zero
-      int vc[3];
-      const float pc[3] = {p.x, p.y, p.z};
-      for (int k = 0; k < 3; ++k) {
-        const float f = (float)res * (0.5f * pc[k] + 0.5f);
-        if (!(f > -1.0f && f < (float)res)) return;
-        vc[k] = (int)f;
-      }
-      Frag fr;
-      fr.vox = ((uint32_t)vc[2] * (uint32_t)res + (uint32_t)vc[1]) * (uint32_t)res + (uint32_t)vc[0];
-      fr.order = seq++;
-      V3<float> ni = mk(b[0] * n[0].x + b[1] * n[1].x + b[2] * n[2].x, b[0] * n[0].y + b[1] * n[1].y + b[2] * n[2].y,
-                        b[0] * n[0].z + b[1] * n[1].z + b[2] * n[2].z);
-      frag_color<float>(p, ni, m, lights, nlights, cube_size, fr.c32);
-      // FP64: same fragment, values in double
-      const double da = (double)tri.area;
-      const double bd[3] = {(double)bw[0] / da, (double)bw[1] / da, (double)bw[2] / da};
-      V3<double> pd = mk(bd[0] * wd[0].x + bd[1] * wd[1].x + bd[2] * wd[2].x,
-                         bd[0] * wd[0].y + bd[1] * wd[1].y + bd[2] * wd[2].y,
-                         bd[0] * wd[0].z + bd[1] * wd[1].z + bd[2] * wd[2].z);
-      V3<double> nid = mk(bd[0] * nd[0].x + bd[1] * nd[1].x + bd[2] * nd[2].x,
-                          bd[0] * nd[0].y + bd[1] * nd[1].y + bd[2] * nd[2].y,
-                          bd[0] * nd[0].z + bd[1] * nd[1].z + bd[2] * nd[2].z);
-      frag_color<double>(pd, nid, m, lights, nlights, (double)cube_size, fr.c64);
-      frags.push_back(fr);
-    });
-  }
   const size_t nvox = (size_t)res * res * res;
-  std::stable_sort(frags.begin(), frags.end(), [](const Frag& a, const Frag& b) { return a.vox < b.vox; });
+  const bool need32 = out_ref || out_q32, need64 = out_f64 != nullptr;
   if (out_ref) memset(out_ref, 0, nvox * 4);
   if (out_q32) memset(out_q32, 0, nvox * 16);
   if (out_f64) memset(out_f64, 0, nvox * 32);
-  if (out_count) memset(out_count, 0, nvox * 4);
-  for (size_t i = 0; i < frags.size();) {
-    size_t j = i;
-    while (j < frags.size() && frags[j].vox == frags[i].vox) ++j;
-    const uint32_t v = frags[i].vox;
-    if (out_ref) {
-      uint32_t s = 0;
-      for (size_t k = i; k < j; ++k) s = orc_rgba8_avg_step(s, frags[k].c32);
-      out_ref[v] = s;
-    }
-    if (out_q32) {
-      uint64_t q[4] = {0, 0, 0, 0};
-      for (size_t k = i; k < j; ++k)
-        for (int c = 0; c < 4; ++c) q[c] += (uint32_t)(frags[k].c32[c] * 65536.0f + 0.5f);
-      const float cnt = (float)(uint32_t)(j - i);
-      for (int c = 0; c < 4; ++c) out_q32[(size_t)v * 4 + c] = ((float)(uint32_t)q[c] / cnt) * (1.0f / 65536.0f);
-    }
-    if (out_f64) {
-      double s[4] = {0, 0, 0, 0};
-      for (size_t k = i; k < j; ++k)
-        for (int c = 0; c < 4; ++c) s[c] += frags[k].c64[c];
-      for (int c = 0; c < 4; ++c) out_f64[(size_t)v * 4 + c] = s[c] / (double)(j - i);
-    }
-    if (out_count) out_count[v] = (uint32_t)(j - i);
-    i = j;
+  std::vector<uint32_t> own_count;
+  uint32_t* cnt = out_count;
+  if (!cnt) {
+    own_count.assign(nvox, 0u);
+    cnt = own_count.data();
+  } else {
+    memset(cnt, 0, nvox * 4);
   }
-  return (int64_t)frags.size();
+  // 16.16 sums modulo 2^32, kept in the output's own storage until the means
+  uint32_t* qsum = reinterpret_cast<uint32_t*>(out_q32);
+  // voxel z-range of every triangle: a fragment's position is a convex
+  // combination of the vertices' (up to rounding), one voxel of margin
+  std::vector<int32_t> zr((size_t)ntri * 2);
+#pragma omp parallel for schedule(static)
+  for (int t = 0; t < ntri; ++t) {
+    float lo = INFINITY, hi = -INFINITY;
+    for (int v = 0; v < 3; ++v) {
+      const V3<float> p = xform_point<float>(M, pos + (size_t)t * 9 + v * 3);
+      const float f = (float)res * (0.5f * (p.z / cube_size) + 0.5f);
+      lo = std::min(lo, f);
+      hi = std::max(hi, f);
+    }
+    if (std::isfinite(lo) && std::isfinite(hi)) {
+      zr[2 * (size_t)t] = (int32_t)std::max(-1.0f, std::min((float)res, floorf(lo))) - 1;
+      zr[2 * (size_t)t + 1] = (int32_t)std::max(-1.0f, std::min((float)res, floorf(hi))) + 1;
+    } else {
+      zr[2 * (size_t)t] = -2;
+      zr[2 * (size_t)t + 1] = res + 1;
+    }
+  }
+  int nthreads = 1;
+#ifdef _OPENMP
+  nthreads = omp_get_max_threads();
+#endif
+  const int nslab = std::max(1, std::min(res, 8 * nthreads));
+  const int thick = (res + nslab - 1) / nslab;
+  int64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
+  for (int sl = 0; sl < nslab; ++sl) {
+    const int z0 = sl * thick, z1 = std::min(res, z0 + thick);
+    if (z0 >= z1) continue;
+    for (int t = 0; t < ntri; ++t) {
+      if (zr[2 * (size_t)t + 1] < z0 || zr[2 * (size_t)t] >= z1) continue;
+      V3<float> w[3];
+      V3<double> wd[3];
+      V3<float> n[3];
+      V3<double> nd[3];
+      for (int v = 0; v < 3; ++v) {
+        // voxelize.vert:26
+        V3<float> p = xform_point<float>(M, pos + (size_t)t * 9 + v * 3);
+        w[v] = mk(p.x / cube_size, p.y / cube_size, p.z / cube_size);
+        V3<double> pd = xform_point<double>(M, pos + (size_t)t * 9 + v * 3);
+        wd[v] = mk(pd.x / cube_size, pd.y / cube_size, pd.z / cube_size);
+        // voxelize.vert:28
+        n[v] = normalize(xform_normal<float>(N, nrm + (size_t)t * 9 + v * 3));
+        nd[v] = normalize(xform_normal<double>(N, nrm + (size_t)t * 9 + v * 3));
+      }
+      // voxelize.geom:27-51
+      const V3<float> e1 = sub(w[1], w[0]), e2 = sub(w[2], w[0]);
+      const V3<float> cr = cross(e1, e2);
+      const float px = fabsf(cr.x), py = fabsf(cr.y), pz = fabsf(cr.z);
+      float xw[3], yw[3];
+      bool inside = true;
+      for (int v = 0; v < 3; ++v) {
+        float a, b;
+        if (pz > px && pz > py) {
+          a = w[v].x;
+          b = w[v].y;
+        } else if (px > py && px > pz) {
+          a = w[v].y;
+          b = w[v].z;
+        } else {
+          a = w[v].x;
+          b = w[v].z;
+        }
+        if (!(fabsf(a) < 64.0f && fabsf(b) < 64.0f)) inside = false;
+        xw[v] = a * (float)res + (float)res;  // viewport transform
+        yw[v] = b * (float)res + (float)res;
+      }
+      if (!inside) continue;
+      Tri2D tri;
+      setup2d(xw, yw, W, W, tri);
+      const orc_material& m = mats[tri_mat ? tri_mat[t] : 0];
+      for_each_pixel(tri, W, W, [&](int, int, const int64_t* bw) {
+        const float fa = (float)tri.area;
+        const float b[3] = {(float)bw[0] / fa, (float)bw[1] / fa, (float)bw[2] / fa};
+        V3<float> p = mk(b[0] * w[0].x + b[1] * w[1].x + b[2] * w[2].x,
+                         b[0] * w[0].y + b[1] * w[1].y + b[2] * w[2].y,
+                         b[0] * w[0].z + b[1] * w[1].z + b[2] * w[2].z);
+        // voxelize.frag:156-157, ivec3() truncates toward zero
+        int vc[3];
+        const float pc[3] = {p.x, p.y, p.z};
+        for (int k = 0; k < 3; ++k) {
+          const float f = (float)res * (0.5f * pc[k] + 0.5f);
+          if (!(f > -1.0f && f < (float)res)) return;
+          vc[k] = (int)f;
+        }
+        if (vc[2] < z0 || vc[2] >= z1) return;  // another slab's voxel
+        const size_t v = ((size_t)vc[2] * (size_t)res + (size_t)vc[1]) * (size_t)res + (size_t)vc[0];
+        ++total;
+        ++cnt[v];
+        if (need32) {
+          V3<float> ni = mk(b[0] * n[0].x + b[1] * n[1].x + b[2] * n[2].x,
+                            b[0] * n[0].y + b[1] * n[1].y + b[2] * n[2].y,
+                            b[0] * n[0].z + b[1] * n[1].z + b[2] * n[2].z);
+          float c32[4];
+          frag_color<float>(p, ni, m, lights, nlights, cube_size, c32);
+          if (out_ref) out_ref[v] = orc_rgba8_avg_step(out_ref[v], c32);
+          if (qsum)
+            for (int c = 0; c < 4; ++c) qsum[v * 4 + c] += (uint32_t)(c32[c] * 65536.0f + 0.5f);
+        }
+        if (need64) {
+          // FP64: same fragment, values in double
+          const double da = (double)tri.area;
+          const double bd[3] = {(double)bw[0] / da, (double)bw[1] / da, (double)bw[2] / da};
+          V3<double> pd = mk(bd[0] * wd[0].x + bd[1] * wd[1].x + bd[2] * wd[2].x,
+                             bd[0] * wd[0].y + bd[1] * wd[1].y + bd[2] * wd[2].y,
+                             bd[0] * wd[0].z + bd[1] * wd[1].z + bd[2] * wd[2].z);
+          V3<double> nid = mk(bd[0] * nd[0].x + bd[1] * nd[1].x + bd[2] * nd[2].x,
+                              bd[0] * nd[0].y + bd[1] * nd[1].y + bd[2] * nd[2].y,
+                              bd[0] * nd[0].z + bd[1] * nd[1].z + bd[2] * nd[2].z);
+          double c64[4];
+          frag_color<double>(pd, nid, m, lights, nlights, (double)cube_size, c64);
+          for (int c = 0; c < 4; ++c) out_f64[v * 4 + c] += c64[c];
+        }
+      });
+    }
+  }
+  // the means
+#pragma omp parallel for schedule(static)
+  for (int64_t v = 0; v < (int64_t)nvox; ++v) {
+    const uint32_t k = cnt[v];
+    if (!k) continue;
+    if (out_q32) {
+      const float fk = (float)k;
+      for (int c = 0; c < 4; ++c) out_q32[v * 4 + c] = ((float)qsum[v * 4 + c] / fk) * (1.0f / 65536.0f);
+    }
+    if (out_f64)
+      for (int c = 0; c < 4; ++c) out_f64[v * 4 + c] = out_f64[v * 4 + c] / (double)k;
+  }
+  return total;
 }
 
 // Storage rounding of n floats in place (dtype 0..3) and packing helpers.
@@ -581,6 +626,7 @@ const int kPair[6][4][2] = {
 template <typename T>
 void mip_level(const T* src, int s2, T* dst, int dir) {
   const int s = s2 / 2;
+#pragma omp parallel for schedule(static)
   for (int z = 0; z < s; ++z)
     for (int y = 0; y < s; ++y)
       for (int x = 0; x < s; ++x) {
@@ -627,8 +673,9 @@ void orc_mip_f32(int res, int levels, int dtype, const float* level0, float* out
       const float* src = l == 1 ? level0 : out + loff[(l - 1) * 6 + d];
       float* dst = out + loff[l * 6 + d];
       mip_level<float>(src, res >> (l - 1), dst, d);
-      const size_t s = (size_t)(res >> l);
-      for (size_t i = 0; i < s * s * s * 4; ++i) dst[i] = store_round(dst[i], dtype);
+      const int64_t nv = (int64_t)(res >> l) * (res >> l) * (res >> l) * 4;
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < nv; ++i) dst[i] = store_round(dst[i], dtype);
     }
 }
 
@@ -672,6 +719,7 @@ void orc_gbuffer_raster(int W, int H, const float* view, const float* proj, cons
       s = s + proj[3 * 4 + r] * view[c * 4 + 3];
       PV[c * 4 + r] = s;
     }
+#pragma omp parallel for schedule(dynamic, 1024)
   for (int t = 0; t < ntri; ++t) {
     float clip[3][4];
     bool ok = true;
@@ -707,8 +755,12 @@ void orc_gbuffer_raster(int W, int H, const float* view, const float* proj, cons
       uint32_t db;
       memcpy(&db, &depth, 4);
       const uint64_t key = ((uint64_t)db << 32) | (uint64_t)(tri_base + (uint32_t)t);
-      uint64_t& dst = depth_key[(size_t)j * W + i];
-      if (key < dst) dst = key;  // GL_LESS; equal depth keeps the earlier triangle
+      // GL_LESS; equal depth keeps the earlier triangle (the minimum key,
+      // whatever order the threads reach the pixel in)
+      uint64_t* dst = &depth_key[(size_t)j * W + i];
+      uint64_t cur = __atomic_load_n(dst, __ATOMIC_RELAXED);
+      while (key < cur && !__atomic_compare_exchange_n(dst, &cur, key, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+      }
     });
   }
 }
@@ -731,7 +783,8 @@ void orc_gbuffer_resolve(int W, int H, const float* view, const float* proj, con
     }
   float N[9];
   normal_matrix(M, N);
-  for (size_t idx = 0; idx < (size_t)W * H; ++idx) {
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t idx = 0; idx < (int64_t)W * H; ++idx) {
     const uint64_t key = depth_key[idx];
     if (key == ~0ull) continue;
     const uint32_t gt = (uint32_t)(key & 0xffffffffu);
