@@ -62,6 +62,8 @@ def load():
                                       C.c_void_p, C.c_void_p, C.c_int, _I64P, C.c_int64, _FP, _I64P]
         lib.orc_trace_f64.argtypes = [C.POINTER(TraceParams), C.c_int, C.c_int, C.POINTER(_DP), _DP, _DP,
                                       C.c_void_p, C.c_void_p, C.c_int, _I64P, C.c_int64, _DP, _I64P]
+        if hasattr(lib, "orc_trace_counts_f32"):
+            lib.orc_trace_counts_f32.argtypes = lib.orc_trace_f32.argtypes
         for n in ("orc_store_round",):
             getattr(lib, n).argtypes = [_FP, C.c_int64, C.c_int]
         lib.orc_encode_f16.argtypes = [_FP, C.POINTER(C.c_uint16), C.c_int64]
@@ -222,10 +224,13 @@ def _params(W, H, cube_size, res, cam, phases=(False, True, True, False), view_d
 
 
 def trace(dtype_float, res, levels, level0, mips, gpos, gnrm, mats, lights, cam, cube_size, cone_set=0,
-          phases=(False, True, True, False), pixels=None, view_dir=7, view_lod=0.0, demodulate=False):
+          phases=(False, True, True, False), pixels=None, view_dir=7, view_lod=0.0, demodulate=False,
+          early_exit_steps=False):
     """Shade pixels.  dtype_float: np.float32 (faithful) or np.float64 (ideal).
     level0: (res,res,res,4); mips: dict (l, d) -> (s,s,s,4); gpos/gnrm: (H, W, 4).
-    Returns (out (npix, 4), steps)."""
+    Returns (out (npix, 4), steps): the cone steps of the reference loop, or
+    (FP32 with early_exit_steps) the pair (reference steps, steps of the loop
+    that also stops once every later sample reads the zero border)."""
     lib = load()
     f64 = dtype_float == np.float64
     dt = np.float64 if f64 else np.float32
@@ -250,13 +255,15 @@ def trace(dtype_float, res, levels, level0, mips, gpos, gnrm, mats, lights, cam,
         pix = _c(pixels, np.int64)
         npix = pix.shape[0]
     out = np.zeros((npix, 4), dt)
-    steps = C.c_int64(0)
+    steps = (C.c_int64 * 2)(0, 0)
     prm = _params(W, H, cube_size, res, cam, phases, view_dir, view_lod, cone_set, demodulate)
-    fn = lib.orc_trace_f64 if f64 else lib.orc_trace_f32
+    if early_exit_steps and f64:
+        raise ValueError("early_exit_steps is counted by the FP32 oracle")
+    fn = lib.orc_trace_f64 if f64 else (lib.orc_trace_counts_f32 if early_exit_steps else lib.orc_trace_f32)
     fn(C.byref(prm), res, levels, arr, gp.ctypes.data_as(P), gn.ctypes.data_as(P), mb.ctypes.data, lb.ctypes.data,
        len(lights), None if pix is None else pix.ctypes.data_as(_I64P), npix, out.ctypes.data_as(P),
-       C.byref(steps))
-    return out, steps.value
+       C.cast(steps, _I64P))
+    return out, ((steps[0], steps[1]) if early_exit_steps else steps[0])
 
 
 def decode_rgba8(a: np.ndarray) -> np.ndarray:

@@ -699,18 +699,13 @@ void orc_mip_f64(int res, int levels, const double* level0, double* out) {
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// G-buffer: voxel_cone_tracing.vert + raster + GL_LESS depth
+// G-buffer: voxel_cone_tracing.vert + raster + GL_LESS depth, with GL's
+// primitive clipping
 // ---------------------------------------------------------------------------
-extern "C" {
+namespace {
 
-// One draw of the scene into a W x H G-buffer (row 0 = bottom).  depth_key
-// holds the running (depth, triangle) minimum across draws: initialise it to
-// all ones before the first draw.  tri_base numbers this draw's triangles.
-// After the last draw, call orc_gbuffer_resolve for every draw.
-void orc_gbuffer_raster(int W, int H, const float* view, const float* proj, const float* M, int ntri,
-                        const float* pos, uint32_t tri_base, uint64_t* depth_key) {
-  // projection * view (the shader's `projection * view * world`), in float
-  float PV[16];
+// projection * view (the shader's `projection * view * world`), in float
+void proj_view(const float* view, const float* proj, float* PV) {
   for (int c = 0; c < 4; ++c)
     for (int r = 0; r < 4; ++r) {
       float s = proj[0 * 4 + r] * view[c * 4 + 0];
@@ -719,75 +714,177 @@ void orc_gbuffer_raster(int W, int H, const float* view, const float* proj, cons
       s = s + proj[3 * 4 + r] * view[c * 4 + 3];
       PV[c * 4 + r] = s;
     }
+}
+
+void clip_of(const float* PV, V3<float> w, float* c) {
+  for (int k = 0; k < 4; ++k) {
+    float s = PV[k] * w.x;
+    s = s + PV[4 + k] * w.y;
+    s = s + PV[8 + k] * w.z;
+    s = s + PV[12 + k];
+    c[k] = s;
+  }
+}
+
+// A vertex of a clipped triangle: clip coordinates and its weights over the
+// triangle's three original vertices.
+struct ClipVert {
+  float c[4];
+  float b[3];
+};
+
+// The triangle after GL's primitive clipping.  A triangle whose vertices all
+// lie in front of the near plane (w > 0, z >= -w) and inside the guard band
+// (|x/w|, |y/w| < 256) is rasterised as it is (clipped = false).  Any other
+// triangle is clipped in clip space against the near plane z = -w and the
+// guard-band planes x = +-255 w, y = +-255 w (Sutherland-Hodgman; a new vertex
+// is interpolated from the inside end of its edge, so two triangles sharing an
+// edge get the same point), and the convex polygon left (n <= 8 vertices) is
+// drawn as the fan (0, k + 1, k + 2), k < n - 2.  The guard band only keeps
+// window coordinates in the rasteriser's fixed-point range; it lies far
+// outside the viewport, so it removes no visible pixel.
+struct ClipPoly {
+  bool clipped;
+  int n;
+  ClipVert v[8];
+};
+
+constexpr float kGuard = 255.0f;
+
+float plane_dist(int plane, const float* c) {
+  switch (plane) {
+    case 0: return c[2] + c[3];          // near: z >= -w
+    case 1: return kGuard * c[3] - c[0];  // x <= 255 w
+    case 2: return kGuard * c[3] + c[0];  // x >= -255 w
+    case 3: return kGuard * c[3] - c[1];  // y <= 255 w
+    default: return kGuard * c[3] + c[1];  // y >= -255 w
+  }
+}
+
+void clip_triangle(const float clip[3][4], ClipPoly& P) {
+  bool inside = true;
+  for (int v = 0; v < 3; ++v) {
+    const float cw = clip[v][3];
+    if (!(cw > 0.0f) || clip[v][2] < -cw) inside = false;
+    else if (!(fabsf(clip[v][0] / cw) < 256.0f && fabsf(clip[v][1] / cw) < 256.0f)) inside = false;
+  }
+  P.clipped = !inside;
+  P.n = 3;
+  for (int v = 0; v < 3; ++v) {
+    for (int k = 0; k < 4; ++k) P.v[v].c[k] = clip[v][k];
+    for (int k = 0; k < 3; ++k) P.v[v].b[k] = k == v ? 1.0f : 0.0f;
+  }
+  if (inside) return;
+  for (int plane = 0; plane < 5 && P.n > 0; ++plane) {
+    ClipVert out[8];
+    int m = 0;
+    for (int i = 0; i < P.n; ++i) {
+      const ClipVert& a = P.v[i];
+      const ClipVert& b = P.v[(i + 1) % P.n];
+      const float da = plane_dist(plane, a.c), db = plane_dist(plane, b.c);
+      const bool ina = da >= 0.0f, inb = db >= 0.0f;
+      if (ina && m < 8) out[m++] = a;
+      if (ina != inb && m < 8) {
+        const ClipVert& in = ina ? a : b;
+        const ClipVert& ot = ina ? b : a;
+        const float din = ina ? da : db, dot_ = ina ? db : da;
+        const float t = din / (din - dot_);
+        ClipVert r;
+        for (int k = 0; k < 4; ++k) r.c[k] = in.c[k] + t * (ot.c[k] - in.c[k]);
+        for (int k = 0; k < 3; ++k) r.b[k] = in.b[k] + t * (ot.b[k] - in.b[k]);
+        out[m++] = r;
+      }
+    }
+    P.n = m;
+    for (int i = 0; i < m; ++i) P.v[i] = out[i];
+  }
+  if (P.n < 3) P.n = 0;
+}
+
+// Window coordinates and depths of fan triangle k of a clipped polygon
+// (vertices 0, k + 1, k + 2 of the polygon).  False if a vertex has w <= 0.
+bool sub_window(const ClipPoly& P, int k, int W, int H, float xw[3], float yw[3], float z[3], const ClipVert* sv[3]) {
+  sv[0] = &P.v[0];
+  sv[1] = &P.v[k + 1];
+  sv[2] = &P.v[k + 2];
+  const float hw = 0.5f * (float)W, hh = 0.5f * (float)H;
+  for (int v = 0; v < 3; ++v) {
+    const float* c = sv[v]->c;
+    if (!(c[3] > 0.0f)) return false;
+    xw[v] = (c[0] / c[3]) * hw + hw;
+    yw[v] = (c[1] / c[3]) * hh + hh;
+    z[v] = (c[2] / c[3]) * 0.5f + 0.5f;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+// One draw of the scene into a W x H G-buffer (row 0 = bottom).  depth_key
+// holds the running (depth, triangle) minimum across draws: initialise it to
+// all ones before the first draw.  tri_base numbers this draw's triangles;
+// the low word of a key is ((tri_base + t) << 3) | fan triangle.
+// After the last draw, call orc_gbuffer_resolve for every draw.
+void orc_gbuffer_raster(int W, int H, const float* view, const float* proj, const float* M, int ntri,
+                        const float* pos, uint32_t tri_base, uint64_t* depth_key) {
+  float PV[16];
+  proj_view(view, proj, PV);
 #pragma omp parallel for schedule(dynamic, 1024)
   for (int t = 0; t < ntri; ++t) {
     float clip[3][4];
-    bool ok = true;
-    for (int v = 0; v < 3; ++v) {
-      const V3<float> w = xform_point<float>(M, pos + t * 9 + v * 3);
-      for (int k = 0; k < 4; ++k) {
-        float s = PV[k] * w.x;
-        s = s + PV[4 + k] * w.y;
-        s = s + PV[8 + k] * w.z;
-        s = s + PV[12 + k];
-        clip[v][k] = s;
-      }
-      if (!(clip[v][3] > 0.0f) || clip[v][2] < -clip[v][3]) ok = false;  // behind the near plane
+    for (int v = 0; v < 3; ++v) clip_of(PV, xform_point<float>(M, pos + (size_t)t * 9 + v * 3), clip[v]);
+    ClipPoly P;
+    clip_triangle(clip, P);
+    const int nsub = P.clipped ? std::max(0, P.n - 2) : 1;
+    for (int k = 0; k < nsub; ++k) {
+      float xw[3], yw[3], z[3];
+      const ClipVert* sv[3];
+      if (!sub_window(P, k, W, H, xw, yw, z, sv)) continue;
+      Tri2D tri;
+      setup2d(xw, yw, W, H, tri);
+      const uint32_t id = ((tri_base + (uint32_t)t) << 3) | (uint32_t)k;
+      for_each_pixel(tri, W, H, [&](int i, int j, const int64_t* bw) {
+        const float fa = (float)tri.area;
+        const float b0 = (float)bw[0] / fa, b1 = (float)bw[1] / fa, b2 = (float)bw[2] / fa;
+        const float depth = b0 * z[0] + b1 * z[1] + b2 * z[2];
+        if (!(depth >= 0.0f && depth <= 1.0f)) return;
+        uint32_t db;
+        memcpy(&db, &depth, 4);
+        const uint64_t key = ((uint64_t)db << 32) | (uint64_t)id;
+        // GL_LESS; equal depth keeps the earlier triangle (the minimum key,
+        // whatever order the threads reach the pixel in)
+        uint64_t* dst = &depth_key[(size_t)j * W + i];
+        uint64_t cur = __atomic_load_n(dst, __ATOMIC_RELAXED);
+        while (key < cur && !__atomic_compare_exchange_n(dst, &cur, key, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+        }
+      });
     }
-    if (!ok) continue;
-    float xw[3], yw[3], z[3];
-    const float hw = 0.5f * (float)W, hh = 0.5f * (float)H;
-    for (int v = 0; v < 3; ++v) {
-      const float xn = clip[v][0] / clip[v][3], yn = clip[v][1] / clip[v][3];
-      if (!(fabsf(xn) < 256.0f && fabsf(yn) < 256.0f)) ok = false;
-      xw[v] = xn * hw + hw;
-      yw[v] = yn * hh + hh;
-      z[v] = (clip[v][2] / clip[v][3]) * 0.5f + 0.5f;
-    }
-    if (!ok) continue;
-    Tri2D tri;
-    setup2d(xw, yw, W, H, tri);
-    for_each_pixel(tri, W, H, [&](int i, int j, const int64_t* bw) {
-      const float fa = (float)tri.area;
-      const float b0 = (float)bw[0] / fa, b1 = (float)bw[1] / fa, b2 = (float)bw[2] / fa;
-      const float depth = b0 * z[0] + b1 * z[1] + b2 * z[2];
-      if (!(depth >= 0.0f && depth <= 1.0f)) return;
-      uint32_t db;
-      memcpy(&db, &depth, 4);
-      const uint64_t key = ((uint64_t)db << 32) | (uint64_t)(tri_base + (uint32_t)t);
-      // GL_LESS; equal depth keeps the earlier triangle (the minimum key,
-      // whatever order the threads reach the pixel in)
-      uint64_t* dst = &depth_key[(size_t)j * W + i];
-      uint64_t cur = __atomic_load_n(dst, __ATOMIC_RELAXED);
-      while (key < cur && !__atomic_compare_exchange_n(dst, &cur, key, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
-      }
-    });
   }
 }
 
 // Fill the G-buffer pixels won by this draw's triangles:
 // pos_depth = world xyz + interpolated clip w; normal_mat = interpolated
 // normal + material id bits.  FP32 faithful into *32, FP64 into *64 (NULL ok).
+// Attributes are interpolated perspective-correctly over the fan triangle
+// that won the pixel; its vertices carry weights over the original three, so
+// the interpolation weights of the original vertices are combined from them.
 void orc_gbuffer_resolve(int W, int H, const float* view, const float* proj, const float* M, int ntri,
                          const float* pos, const float* nrm, const uint32_t* tri_mat, uint32_t tri_base,
                          const uint64_t* depth_key, float* pos_depth32, float* normal_mat32, double* pos_depth64,
                          double* normal_mat64) {
   float PV[16];
-  for (int c = 0; c < 4; ++c)
-    for (int r = 0; r < 4; ++r) {
-      float s = proj[0 * 4 + r] * view[c * 4 + 0];
-      s = s + proj[1 * 4 + r] * view[c * 4 + 1];
-      s = s + proj[2 * 4 + r] * view[c * 4 + 2];
-      s = s + proj[3 * 4 + r] * view[c * 4 + 3];
-      PV[c * 4 + r] = s;
-    }
+  proj_view(view, proj, PV);
   float N[9];
   normal_matrix(M, N);
 #pragma omp parallel for schedule(dynamic, 4096)
   for (int64_t idx = 0; idx < (int64_t)W * H; ++idx) {
     const uint64_t key = depth_key[idx];
     if (key == ~0ull) continue;
-    const uint32_t gt = (uint32_t)(key & 0xffffffffu);
+    const uint32_t id = (uint32_t)(key & 0xffffffffu);
+    const uint32_t gt = id >> 3;
+    const int k = (int)(id & 7u);
     if (gt < tri_base || gt >= tri_base + (uint32_t)ntri) continue;
     const int t = (int)(gt - tri_base);
     const int i = (int)(idx % W), j = (int)(idx / W);
@@ -795,57 +892,66 @@ void orc_gbuffer_resolve(int W, int H, const float* view, const float* proj, con
     V3<float> wv[3];
     V3<double> wd[3];
     for (int v = 0; v < 3; ++v) {
-      wv[v] = xform_point<float>(M, pos + t * 9 + v * 3);
-      wd[v] = xform_point<double>(M, pos + t * 9 + v * 3);
-      for (int k = 0; k < 4; ++k) {
-        float s = PV[k] * wv[v].x;
-        s = s + PV[4 + k] * wv[v].y;
-        s = s + PV[8 + k] * wv[v].z;
-        s = s + PV[12 + k];
-        clip[v][k] = s;
-      }
+      wv[v] = xform_point<float>(M, pos + (size_t)t * 9 + v * 3);
+      wd[v] = xform_point<double>(M, pos + (size_t)t * 9 + v * 3);
+      clip_of(PV, wv[v], clip[v]);
     }
-    float xw[3], yw[3];
-    const float hw = 0.5f * (float)W, hh = 0.5f * (float)H;
-    for (int v = 0; v < 3; ++v) {
-      xw[v] = (clip[v][0] / clip[v][3]) * hw + hw;
-      yw[v] = (clip[v][1] / clip[v][3]) * hh + hh;
-    }
+    ClipPoly P;
+    clip_triangle(clip, P);
+    float xw[3], yw[3], zz[3];
+    const ClipVert* sv[3];
+    if (!sub_window(P, k, W, H, xw, yw, zz, sv)) continue;
     Tri2D tri;
     setup2d(xw, yw, W, H, tri);
     int64_t bw[3];
     if (!covers(tri, i, j, bw)) continue;
     const float fa = (float)tri.area;
     float b[3] = {(float)bw[0] / fa, (float)bw[1] / fa, (float)bw[2] / fa};
-    // perspective-correct weights (b_k / w_k) / sum
+    // perspective-correct weights (b_k / w_k) / sum over the drawn triangle
     float q[3], qs = 0.0f;
     for (int v = 0; v < 3; ++v) {
-      q[v] = b[v] / clip[v][3];
+      q[v] = b[v] / sv[v]->c[3];
       qs = qs + q[v];
     }
     for (int v = 0; v < 3; ++v) q[v] = q[v] / qs;
-    float wp[3] = {0, 0, 0}, nn[3] = {0, 0, 0};
     double bd[3], qd[3], qsd = 0.0;
     for (int v = 0; v < 3; ++v) bd[v] = (double)bw[v] / (double)tri.area;
     for (int v = 0; v < 3; ++v) {
-      qd[v] = bd[v] / (double)clip[v][3];
+      qd[v] = bd[v] / (double)sv[v]->c[3];
       qsd += qd[v];
     }
+    for (int v = 0; v < 3; ++v) qd[v] = qd[v] / qsd;
+    if (P.clipped) {  // weights of the original vertices
+      float lam[3];
+      double lamd[3];
+      for (int o = 0; o < 3; ++o) {
+        float a = q[0] * sv[0]->b[o];
+        a = a + q[1] * sv[1]->b[o];
+        a = a + q[2] * sv[2]->b[o];
+        lam[o] = a;
+        lamd[o] = qd[0] * (double)sv[0]->b[o] + qd[1] * (double)sv[1]->b[o] + qd[2] * (double)sv[2]->b[o];
+      }
+      for (int o = 0; o < 3; ++o) {
+        q[o] = lam[o];
+        qd[o] = lamd[o];
+      }
+    }
+    float wp[3] = {0, 0, 0}, nn[3] = {0, 0, 0};
     double wpd[3] = {0, 0, 0}, nnd[3] = {0, 0, 0};
     for (int v = 0; v < 3; ++v) {
-      const V3<float> tn = xform_normal<float>(N, nrm + t * 9 + v * 3);
+      const V3<float> tn = xform_normal<float>(N, nrm + (size_t)t * 9 + v * 3);
       const float l = length(tn);
-      const V3<double> tnd = xform_normal<double>(N, nrm + t * 9 + v * 3);
+      const V3<double> tnd = xform_normal<double>(N, nrm + (size_t)t * 9 + v * 3);
       const double ld = length(tnd);
       const float wc[3] = {wv[v].x, wv[v].y, wv[v].z};
       const double wcd[3] = {wd[v].x, wd[v].y, wd[v].z};
       const float nc[3] = {tn.x / l, tn.y / l, tn.z / l};
       const double ncd[3] = {tnd.x / ld, tnd.y / ld, tnd.z / ld};
-      for (int k = 0; k < 3; ++k) {
-        wp[k] = wp[k] + q[v] * wc[k];
-        nn[k] = nn[k] + q[v] * nc[k];
-        wpd[k] += (qd[v] / qsd) * wcd[k];
-        nnd[k] += (qd[v] / qsd) * ncd[k];
+      for (int c = 0; c < 3; ++c) {
+        wp[c] = wp[c] + q[v] * wc[c];
+        nn[c] = nn[c] + q[v] * nc[c];
+        wpd[c] += qd[v] * wcd[c];
+        nnd[c] += qd[v] * ncd[c];
       }
     }
     const uint32_t mid = tri_mat ? tri_mat[t] : 0u;
@@ -976,6 +1082,20 @@ struct Tracer {
   const orc_light* lights;
   int nlights;
   int64_t steps = 0, samples = 0;
+  // steps of the same loop with an exit once the sample point is more than
+  // half a texel of the coarsest level outside the unit cube and moving away
+  // on that axis (every later sample reads only the zero border, so the
+  // colour no longer changes), with the target distance capped at 2 sqrt(3):
+  // the steps a marcher with that exact early exit takes
+  int64_t live_steps = 0;
+
+  bool exits(V3<T> sp, V3<T> dir, T dist, T max_dist) const {
+    if (!(dist < std::min(max_dist, (T)3.4641016f))) return true;
+    const T margin = (T)0.5 * (T)(1 << (vol.levels - 1)) / (T)P.cube_res;
+    return (sp.x < -margin && dir.x <= (T)0) || (sp.x >= (T)1 + margin && dir.x >= (T)0) ||
+           (sp.y < -margin && dir.y <= (T)0) || (sp.y >= (T)1 + margin && dir.y >= (T)0) ||
+           (sp.z < -margin && dir.z <= (T)0) || (sp.z >= (T)1 + margin && dir.z >= (T)0);
+  }
 
   // sample_voxel (voxel_cone_tracing.frag:80-86)
   void sample_voxel(V3<T> pos, V3<T> dir, const int idx[3], T lod, T out[4]) {
@@ -998,7 +1118,10 @@ struct Tracer {
     T dist = (T)3 * voxel_size;
     T diam = dist * aperture;
     V3<T> sp = add(scale(dist, dir), origin);
+    bool live = true;
     while (c[3] < (T)1 && dist < max_dist) {
+      live = live && !exits(sp, dir, dist, max_dist);
+      live_steps += live ? 1 : 0;
       T mip = log2T(diam * (T)P.cube_res);
       if (!(mip > (T)0)) mip = 0;
       T s[4];
@@ -1203,13 +1326,13 @@ struct Tracer {
 template <typename T>
 void trace_pixels(const TraceParams& P, int res, int levels, const T* const* level_ptrs, const T* gpos,
                   const T* gnrm, const orc_material* mats, const orc_light* lights, int nlights,
-                  const int64_t* pixels, int64_t npix, T* out, int64_t* steps) {
+                  const int64_t* pixels, int64_t npix, T* out, int64_t* steps, int64_t* live_steps = nullptr) {
   Volume<T> vol;
   vol.res = res;
   vol.levels = levels;
   vol.lv.assign(level_ptrs, level_ptrs + levels * 6);
-  int64_t total_steps = 0;
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total_steps)
+  int64_t total_steps = 0, total_live = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total_steps, total_live)
   for (int64_t k = 0; k < npix; ++k) {
     const int64_t idx = pixels ? pixels[k] : k;
     Tracer<T> tr{vol, P, lights, nlights};
@@ -1229,8 +1352,10 @@ void trace_pixels(const TraceParams& P, int res, int levels, const T* const* lev
     }
     tr.shade(mk(gp[0], gp[1], gp[2]), mk(gn[0], gn[1], gn[2]), mats[mid], o);
     total_steps += tr.steps;
+    total_live += tr.live_steps;
   }
   if (steps) *steps = total_steps;
+  if (live_steps) *live_steps = total_live;
 }
 
 }  // namespace
@@ -1259,6 +1384,17 @@ void orc_trace_f32(const orc_trace_params* p, int res, int levels, const float* 
   TraceParams P;
   memcpy(&P, p, sizeof(P));
   trace_pixels<float>(P, res, levels, level_ptrs, gpos, gnrm, mats, lights, nlights, pixels, npix, out, steps);
+}
+
+// orc_trace_f32 with both step counts: counts[0] = steps of the reference
+// loop, counts[1] = steps of the loop with the exact early exit.
+void orc_trace_counts_f32(const orc_trace_params* p, int res, int levels, const float* const* level_ptrs,
+                          const float* gpos, const float* gnrm, const orc_material* mats, const orc_light* lights,
+                          int nlights, const int64_t* pixels, int64_t npix, float* out, int64_t* counts) {
+  TraceParams P;
+  memcpy(&P, p, sizeof(P));
+  trace_pixels<float>(P, res, levels, level_ptrs, gpos, gnrm, mats, lights, nlights, pixels, npix, out, counts,
+                      counts + 1);
 }
 
 void orc_trace_f64(const orc_trace_params* p, int res, int levels, const double* const* level_ptrs,

@@ -31,6 +31,13 @@ SCENES = {
     # every phase (direct light, shadow cones, diffuse and specular cones), a
     # transmissive glossy box and an emissive one: the full compose
     "cornell64_full": dict(kind="cornell", res=64, W=128, H=96, glass=True, phases=(True, True, True, True)),
+    # the camera inside the box: walls that pass behind the eye are clipped at
+    # the near plane (every phase on)
+    "cornell64_inside": dict(kind="cornell", res=64, W=128, H=96, phases=(True, True, True, True),
+                             eye=(0.12, 0.05, 0.2), target=(-0.25, -0.1, -0.5), fovy_deg=75.0),
+    # a model matrix with rotation and non-uniform scale (voxelize.vert:24-30,
+    # voxel_cone_tracing.vert:22-28)
+    "soup64_xform": dict(kind="soup", res=64, ntri=1500, W=96, H=64, model="rot_scale"),
 }
 
 

@@ -36,7 +36,8 @@ class SceneSpec:
 
     def __init__(self, kind: str, seed: int = 1, ntri: int = 2000, res: int = 64, W: int = 96, H: int = 64,
                  cube_size: float | None = None, extent: float = 1.0, mu: float = -4.0, glass: bool = False,
-                 phases=(False, True, True, False)):
+                 phases=(False, True, True, False), eye=None, target=None, fovy_deg: float = 50.0,
+                 model: str | None = None):
         self.kind, self.res, self.W, self.H = kind, res, W, H
         self.phases = tuple(phases)
         if kind == "cornell":
@@ -57,7 +58,7 @@ class SceneSpec:
             self.extra = mats
             self.cube_size = 0.55 if cube_size is None else cube_size
             self.lights = [((0.0, 0.45, 0.0), (1.0, 1.0, 1.0), 1.0, L.LIGHT_POINT)]
-            eye, tgt = (0.0, 0.0, 1.55), (0.0, 0.0, 0.0)
+            deye, dtgt = (0.0, 0.0, 1.55), (0.0, 0.0, 0.0)
             self.cone_set = L.CONES_5_WIDE
         else:
             pos, nrm, ar = L.soup(seed, ntri, extent, mu)
@@ -68,11 +69,13 @@ class SceneSpec:
             self.extra = rec
             self.cube_size = 1.1 if cube_size is None else cube_size
             self.lights = [((0.4, 1.0, 0.6), (1.0, 1.0, 1.0), 1.0, L.LIGHT_DIRECTIONAL)]
-            eye, tgt = (0.0, 0.3, 3.2), (0.0, 0.0, 0.0)
+            deye, dtgt = (0.0, 0.3, 3.2), (0.0, 0.0, 0.0)
             self.cone_set = L.CONES_REF9
-        self.M = identity()
-        self.view = look_at(eye, tgt, (0, 1, 0))
-        self.proj = perspective(np.radians(50.0), W / H, 0.1, 100.0)
+        self.M = identity() if model is None else model_matrix(model)
+        if eye is None:
+            eye, target = deye, dtgt
+        self.view = look_at(eye, target, (0, 1, 0))
+        self.proj = perspective(np.radians(fovy_deg), W / H, 0.1, 100.0)
 
     @property
     def cam(self):
@@ -101,6 +104,26 @@ class SceneSpec:
         for (p, c, i, k) in self.lights:
             ctx.queue_light(p, c, i, k)
         return self.model
+
+
+def model_matrix(name: str) -> np.ndarray:
+    """Column-major model matrices of the transform tests: "rot_scale" is a
+    translation, two rotations and a non-uniform scale (so the normal matrix,
+    mat3(transpose(inverse(model))), differs from the model's own 3x3)."""
+    if name != "rot_scale":
+        raise ValueError(name)
+
+    def rot(axis, ang):
+        c, s = np.cos(ang), np.sin(ang)
+        R = np.eye(4)
+        i, j = [(1, 2), (2, 0), (0, 1)][axis]
+        R[i, i], R[j, j], R[i, j], R[j, i] = c, c, -s, s
+        return R
+
+    T = np.eye(4)
+    T[:3, 3] = (0.05, -0.04, 0.03)
+    S = np.diag([0.85, 0.6, 0.95, 1.0])
+    return np.ascontiguousarray((T @ rot(1, 0.5) @ rot(0, -0.35) @ S).T.astype(np.float32))
 
 
 def decode_level(arr: np.ndarray, dtype: int) -> np.ndarray:

@@ -44,29 +44,83 @@ def soup():
     return sc, r
 
 
-@pytest.mark.parametrize("which", ["cornell", "soup"])
+@pytest.fixture(scope="module")
+def cornell_inside():
+    """The camera inside the box (walls clipped at the near plane), every phase on."""
+    sc = SceneSpec("cornell", res=64, W=128, H=96, phases=(True, True, True, True), eye=(0.12, 0.05, 0.2),
+                   target=(-0.25, -0.1, -0.5), fovy_deg=75.0)
+    r = glref.frame(64, sc.cube_size, sc.M, sc.view, sc.proj, sc.W, sc.H, sc.pos, sc.nrm, sc.tri_mat,
+                    sc.mat_records, sc.lights, phases=sc.phases)
+    return sc, r
+
+
+@pytest.fixture(scope="module")
+def soup_xform():
+    """A model matrix with rotation and non-uniform scale."""
+    sc = SceneSpec("soup", res=64, ntri=1500, W=96, H=64, model="rot_scale")
+    sc.lights = [((0.3, 1.2, 0.5), (1.0, 1.0, 1.0), 2.0, 0)]
+    r = glref.frame(64, sc.cube_size, sc.M, sc.view, sc.proj, sc.W, sc.H, sc.pos, sc.nrm, sc.tri_mat,
+                    sc.mat_records, sc.lights)
+    return sc, r
+
+
+SCENES = ["cornell", "soup", "cornell_inside", "soup_xform"]
+
+
+def moved_voxels(occ_ref, occ_orc):
+    """Voxels occupied on one side only, as (only llvmpipe, only oracle) lists."""
+    return np.argwhere(occ_ref & ~occ_orc), np.argwhere(occ_orc & ~occ_ref)
+
+
+def near(voxels, shape):
+    m = np.zeros(shape, bool)
+    for z, y, x in voxels:
+        m[max(z - 1, 0):z + 2, max(y - 1, 0):y + 2, max(x - 1, 0):x + 2] = True
+    return m
+
+
+@pytest.mark.parametrize("which", SCENES)
 def test_voxel_occupancy_and_coordinates_bit_exact(which, request):
+    """Occupancy and coordinates agree except at exact ties: a fragment whose
+    interpolated coordinate lies within rounding of a voxel boundary
+    (voxelize.frag:156-157 truncates) may land one voxel further on one side;
+    llvmpipe interpolates by plane equations, the oracle by barycentrics.  Such
+    a voxel then shows up as a pair of neighbours, one occupied on each side,
+    and is rare (seen: one pair in four of eight 1500-triangle soups)."""
     sc, r = request.getfixturevalue(which)
     v = oracle.voxelize(sc.res, sc.cube_size, sc.M, sc.pos, sc.nrm, sc.tri_mat, sc.mat_records, sc.lights,
                         want=("ref", "count"))
     occ_ref, occ_orc = r["level0"] != 0, v["ref"] != 0
     assert occ_orc.sum() > 100
-    assert np.array_equal(occ_ref, occ_orc)
+    only_ref, only_orc = moved_voxels(occ_ref, occ_orc)
+    assert len(only_ref) == len(only_orc) and len(only_ref) <= max(1, occ_orc.sum() // 200)
+    for a in only_ref:  # each stray voxel has its counterpart one voxel away
+        assert (np.abs(only_orc - a).sum(1) == 1).any(), a
+    if which in ("cornell", "soup"):
+        assert np.array_equal(occ_ref, occ_orc)
 
 
-@pytest.mark.parametrize("which", ["cornell", "soup"])
+@pytest.mark.parametrize("which", SCENES)
 def test_voxel_radiance_running_average(which, request):
     sc, r = request.getfixturevalue(which)
     v = oracle.voxelize(sc.res, sc.cube_size, sc.M, sc.pos, sc.nrm, sc.tri_mat, sc.mat_records, sc.lights,
                         want=("ref", "count"))
-    occ = v["ref"] != 0
+    occ_ref, occ_orc = r["level0"] != 0, v["ref"] != 0
+    moved = near(np.concatenate(moved_voxels(occ_ref, occ_orc)), occ_orc.shape)
+    occ = occ_orc & occ_ref & ~moved
     same = (r["level0"] == v["ref"])[occ]
     d = np.abs(_bytes(r["level0"][occ]) - _bytes(v["ref"][occ]))
     # voxels reached by a single triangle are bit-identical; voxels shared by
     # several triangles differ only where llvmpipe's tile order differs from
-    # the canonical triangle order
+    # the canonical triangle order.  With a model matrix the normal matrix
+    # mat3(transpose(inverse(model))) is inverted in float by llvmpipe and in
+    # double (then rounded) here, so a fragment colour at a quantisation
+    # boundary may truncate one quantum (2 LSB) apart.
     single = (v["count"][occ] <= 2)
-    assert same[single].all()
+    if which == "soup_xform":
+        assert d[single].max() <= 2 and same[single].mean() > 0.995
+    else:
+        assert same[single].all()
     assert same.mean() > 0.99
     assert d.max() <= 2
 
@@ -85,7 +139,7 @@ def test_glsl_round_ties_to_even_matches_llvmpipe(cornell):
     assert (r["level0"] == even)[occ].mean() > (r["level0"] == away)[occ].mean()
 
 
-@pytest.mark.parametrize("which", ["cornell", "soup"])
+@pytest.mark.parametrize("which", SCENES)
 def test_mip_within_one_lsb(which, request):
     sc, r = request.getfixturevalue(which)
     m = oracle.mip_f32(sc.res, 7, 0, oracle.decode_rgba8(r["level0"]))
@@ -100,7 +154,7 @@ def test_mip_within_one_lsb(which, request):
     assert n_diff < 0.02 * n
 
 
-@pytest.mark.parametrize("which", ["cornell", "soup"])
+@pytest.mark.parametrize("which", SCENES)
 def test_gbuffer_agrees(which, request):
     sc, r = request.getfixturevalue(which)
     p32, n32, _, _, _ = oracle.gbuffer(sc.W, sc.H, sc.view, sc.proj, [(sc.M, sc.pos, sc.nrm, sc.tri_mat)])
@@ -115,7 +169,7 @@ def test_gbuffer_agrees(which, request):
     assert np.abs(r["gnrm"][same][:, :3] - n32[same][:, :3]).max() < 2e-3
 
 
-@pytest.mark.parametrize("which", ["cornell", "soup"])
+@pytest.mark.parametrize("which", SCENES)
 def test_trace_image_matches_reference(which, request):
     """voxel_cone_tracing.frag under llvmpipe vs the oracle's FP32 trace, both
     fed with llvmpipe's own volume and G-buffer (indirect light only)."""
@@ -125,13 +179,19 @@ def test_trace_image_matches_reference(which, request):
     cov = r["gpos"][..., 3] != 0
     pix = np.flatnonzero(cov.reshape(-1)).astype(np.int64)
     out, _ = oracle.trace(np.float32, sc.res, 7, l0, mips, r["gpos"], r["gnrm"], sc.mat_records, sc.lights, sc.cam,
-                          sc.cube_size, cone_set=0, pixels=pix)
+                          sc.cube_size, cone_set=0, pixels=pix, phases=sc.phases)
     ref = r["image"].reshape(-1, 4)[pix]
     err = np.abs(out[:, :3] - ref[:, :3])
     psnr = oracle.psnr(out[:, :3], ref[:, :3])
     print(f"{which}: trace vs llvmpipe max-abs {err.max():.3e} PSNR {psnr:.1f} dB")
-    assert psnr > 45.0
-    assert np.quantile(err, 0.999) < 2e-2
+    if sc.phases == (False, True, True, False):
+        # indirect light only; measured 166.4 dB / 6.0e-8 (cornell), 168.8 dB /
+        # 5.2e-8 (soup), 166.9 dB / 3.0e-8 (soup with the model matrix)
+        assert psnr > 150.0 and err.max() < 1e-6
+    else:
+        # every phase: llvmpipe's pow() in the specular term of the direct
+        # light dominates; measured 116.9 dB / 7.9e-5 with the camera inside
+        assert psnr > 110.0 and err.max() < 2e-4
 
 
 @pytest.mark.parametrize("phases", [(True, False, False, False), (True, False, False, True),
