@@ -48,10 +48,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(wl, dtype, sample_tri=20000, sample_pix=1500):
-    """The oracle (C++ port of the reference shaders) on a bounded sample of
-    the same frame, scaled to ms/frame: voxelize on `sample_tri` triangles,
-    the full mip chain, and the trace of `sample_pix` pixels."""
+def cpu_baseline(wl, dtype):
+    """The oracle (the C++ restatement of the reference's shaders, FP32) on the
+    whole frame of the workload on the host's threads: voxelize of every
+    triangle into the storage type, the full mip chain, the G-buffer at the
+    full viewport and the trace of every covered pixel.  Nothing is sampled or
+    scaled."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle
     from voxel_cone_tracing_amd import _lib as L
@@ -59,38 +61,37 @@ def cpu_baseline(wl, dtype, sample_tri=20000, sample_pix=1500):
     from helpers import material_table
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    pos, nrm, ar = L.soup(wl.seed, sample_tri)
+    pos, nrm, ar = L.soup(wl.seed, wl.ntri)
     mats = material_table(soup_materials_fast(ar))
-    tm = np.arange(1, sample_tri + 1, dtype=np.uint32)
+    tm = np.arange(1, wl.ntri + 1, dtype=np.uint32)
     t0 = time.perf_counter()
     v = oracle.voxelize(wl.res, wl.cube_size, identity(), pos, nrm, tm, mats, wl.lights, want=("q32",))
-    t_vox = (time.perf_counter() - t0) * (wl.ntri / sample_tri)
     l0 = oracle.store_round(v["q32"], dtype)
+    t_vox = time.perf_counter() - t0
     t0 = time.perf_counter()
     mips = oracle.mip_f32(wl.res, wl.levels, dtype, l0)
     t_mip = time.perf_counter() - t0
-    # trace: G-buffer of the sample scene at a reduced size, pixels sampled from it
-    W, H = 320, 180
     view = look_at(wl.eye, wl.target, (0, 1, 0))
-    proj = perspective(wl.fovy, W / H, 0.1, 100.0)
-    p32, n32, _, _, _ = oracle.gbuffer(W, H, view, proj, [(identity(), pos, nrm, tm)])
-    cov = np.flatnonzero(p32[..., 3].reshape(-1) != 0)
-    rng = np.random.default_rng(0)
-    pix = rng.choice(cov, size=min(sample_pix, cov.size), replace=False).astype(np.int64)
-    cam = view.reshape(16)[12:15]
+    proj = perspective(wl.fovy, wl.width / wl.height, 0.1, 100.0)
     t0 = time.perf_counter()
-    oracle.trace(np.float32, wl.res, wl.levels, l0, mips, p32, n32, mats, wl.lights, cam, wl.cube_size,
-                 cone_set=wl.cone_set, pixels=pix)
-    t_trace = (time.perf_counter() - t0) * (wl.width * wl.height / max(1, pix.size))
-    total = t_vox + t_mip + t_trace
+    p32, n32, _, _, _ = oracle.gbuffer(wl.width, wl.height, view, proj, [(identity(), pos, nrm, tm)])
+    t_gb = time.perf_counter() - t0
+    pix = np.flatnonzero(p32[..., 3].reshape(-1) != 0).astype(np.int64)
+    t0 = time.perf_counter()
+    oracle.trace(np.float32, wl.res, wl.levels, l0, mips, p32, n32, mats, wl.lights, view.reshape(16)[12:15],
+                 wl.cube_size, cone_set=wl.cone_set, pixels=pix)
+    t_trace = time.perf_counter() - t0
+    total = t_vox + t_mip + t_gb + t_trace
     return {
         "value": round(total * 1000.0, 1),
         "unit": "ms/frame",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle FP32 port: voxelize {sample_tri} of {wl.ntri} triangles scaled x{wl.ntri // sample_tri}, "
-                   f"full {wl.levels}-level mip, trace {pix.size} sampled pixels scaled to {wl.width}x{wl.height}"),
-        "parts_ms": {"voxelize": round(t_vox * 1e3, 1), "mip": round(t_mip * 1e3, 1), "trace": round(t_trace * 1e3, 1)},
+        "sample": (f"the whole frame, nothing scaled: oracle FP32 port, voxelize of all {wl.ntri} triangles, "
+                   f"{wl.levels}-level mip, {wl.width}x{wl.height} G-buffer, trace of all {pix.size} covered "
+                   f"pixels"),
+        "parts_ms": {"voxelize": round(t_vox * 1e3, 1), "mip": round(t_mip * 1e3, 1), "gbuffer": round(t_gb * 1e3, 1),
+                     "trace": round(t_trace * 1e3, 1)},
     }
 
 
@@ -146,8 +147,8 @@ def algorithmic(name, wl, bpe, steps, fb_tiles):
     if name == "gbuffer":
         return 76 * wl.ntri + 32 * W * H, "B", "hbm", "76 B per triangle read + 32 B per pixel written"
     if name == "trace":
-        return 3 * steps * 16 * bpe, "B", "hbm", ("3 quadrilinear samples per cone step x 16 texels x bpp "
-                                                  "(served mostly from L2 / MALL)")
+        return 3 * steps * 16 * bpe, "B", "vl1d", ("3 quadrilinear samples per cone step x 16 texels x bpp "
+                                                   "(served from L1 / L2)")
     if name == "upsample":
         return 12928 * W * H, "FLOP", "mfma", "12,928 FLOP per output pixel (34-64-64-3 MLP)"
     return 0, "B", "hbm", ""
@@ -311,26 +312,37 @@ def main():
     # ---- roofline of the cone marcher, the kernel that dominates both forms: the
     # plain frame's trace pass is exactly one trace_frame_kernel launch, timed by
     # HIP events on the context stream around it (the learned form runs the same
-    # marcher over the quarter-resolution frame and over tile lists)
+    # marcher over the quarter-resolution frame and over tile lists).  Its texel
+    # gathers are served from L1/L2, so its ceiling is the rate at which the
+    # vector memory path delivers its 16-byte loads from L1, measured here by a
+    # calibration kernel of the same load form (vct_trace_load_peak).
     amount, unit, bound, definition = algorithmic("trace", wl, bpe, steps, 0)
     t_launch = float(plain_passes[3]) * 1e-3
     achieved = amount / t_launch / 1e9
+    load_peak = ctx.load_peak_gbps()
+    log(f"bench: L1-hit 16-byte load ceiling {load_peak:.0f} GB/s")
     # up to 9 diffuse cones without shadow lights: the 128-pixel (16x8) block shape
     kname = f"trace_frame_kernel<{dt},Shape<128,2>>"
     grid = (wl.width + 15) // 16 * 256 * ((wl.height + 7) // 8)
     traffic = l2_hit = None
-    if wl.name == "soup256" and args.dtype == "fp16":
-        from voxel_cone_tracing_amd.profile_summary import read_summary, read_traffic
+    from voxel_cone_tracing_amd.profile_summary import read_summary, read_traffic, summary_is_current
 
+    if wl.name == "soup256" and args.dtype == "fp16":
+        # counters of a profile taken with the kernel sources as they are now, else none
         traffic = read_traffic(TRAFFIC_PROFILE, kname, grid)
         l2_hit = read_summary(TRAFFIC_PROFILE, "l2_hit_rate", kname, grid)
+        if not summary_is_current(TRAFFIC_PROFILE):
+            log(f"bench: {TRAFFIC_PROFILE.name} was taken with other kernel sources: no traffic figure")
     roofline = {
-        "kernel": "trace_frame_kernel", "pass": "trace (plain frame, one launch)", "bound": bound,
-        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic, "l2_hit_rate": l2_hit, "algorithmic": definition,
-        "launch_ms": round(float(plain_passes[3]), 4),
-        "note": ("effective sample bandwidth: the gathers are served from L1/L2 (see traffic, the HBM bytes per "
-                 "launch from rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a copy-kernel calibration)"),
+        "kernel": "trace_frame_kernel", "pass": "trace (plain frame, one launch)", "bound": "vl1d",
+        "achieved": round(achieved, 1), "peak": round(load_peak, 1), "unit": "GB/s",
+        "frac": round(achieved / load_peak, 4), "traffic": traffic, "l2_hit_rate": l2_hit,
+        "hbm_GBps": round(traffic / t_launch / 1e9, 1) if traffic else None,
+        "algorithmic": definition, "launch_ms": round(float(plain_passes[3]), 4),
+        "note": ("bound vl1d: the texel gathers hit L1/L2; peak = 16-byte loads hitting L1, measured live by a "
+                 "calibration kernel of the same load form (no gather pattern of these loads is delivered "
+                 "faster); traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE (scaled by a "
+                 "copy-kernel calibration) of a profile stamped with the same kernel sources, else null"),
     }
     # HBM-bound passes against the HBM peak (algorithmic bytes / pass time), timed
     # in frames whose passes do not overlap
@@ -389,7 +401,7 @@ def main():
     out["roofline"] = roofline
     out["hbm_passes"] = hbm_passes
     if not args.no_cpu_baseline:
-        log("bench: cpu baseline (oracle port, bounded sample)...")
+        log("bench: cpu baseline (oracle port, the whole frame)...")
         out["cpu_baseline"] = cpu_baseline(wl, dt)
     ctx.close()
     print(json.dumps(out), flush=True)

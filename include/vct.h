@@ -224,6 +224,11 @@ int vct_get_voxelize_stats(vct_ctx ctx, uint64_t* fragments, uint64_t* candidate
  * voxel_cone_tracing.frag:80-86): the unit of the effective sample
  * bandwidth.  Diagnostics; not part of a frame. */
 int vct_trace_count_steps(vct_ctx ctx, uint64_t* steps);
+/* Diagnostics with no counterpart in the reference: the peak rate (GB/s) at
+ * which this GPU's vector memory path delivers 16-byte-per-lane loads that hit
+ * the L1 cache, measured with a calibration kernel in the marcher's load form.
+ * It is the ceiling of the marcher's texel gathers (their roofline). */
+int vct_trace_load_peak(vct_ctx ctx, double* gbps);
 
 /* ---- Texture3D (texture_3d.h:6-10) --------------------------------------- */
 /* create_tex_3d (texture_3d.cpp:3-25): levels of a w x h x d texture,
@@ -286,7 +291,9 @@ int vct_upsample_clear_samples(vct_ctx ctx);
 int vct_upsample_set_samples(vct_ctx ctx, const float* features, const float* albedo, const float* target,
                              uint32_t n);
 int vct_upsample_sample_count(vct_ctx ctx, uint64_t* n);
-/* Adam steps (beta 0.9 / 0.999) on random windows of `batch` samples. */
+/* Adam steps (beta 0.9 / 0.999) on minibatches of `batch` samples: step s
+ * takes samples s*batch .. s*batch + batch - 1 (mod n) of a random order of
+ * the n samples, drawn from `seed`. */
 int vct_upsample_train(vct_ctx ctx, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float* last_loss);
 /* The learned form of Renderer::render: voxelize, mip, full-resolution
  * G-buffer, quarter-resolution cone trace, MLP upsampling (surface-guided:

@@ -82,10 +82,18 @@ typedef struct {
 } material_data_t;
 #pragma pack(pop)
 
-// material_t (renderer.h:37-42): here the material's parameters
+#define MAX_TEXTURE_BINDINGS 32  // renderer.h:18
+
+// material_t (renderer.h:37-42) with the reference's members: `ubo` names the
+// material (its id: there is no GL uniform buffer, the parameters live in the
+// core's material table), `textures` and `num_textures` stay empty (materials
+// are not texture-mapped here).  Added member: `data`, the material's
+// parameters as last added or uploaded.
 typedef struct {
-  material_data_t data;
+  GLuint ubo;
+  GLuint textures[MAX_TEXTURE_BINDINGS];
   size_t num_textures;
+  material_data_t data;
 } material_t;
 
 struct Renderer {

@@ -50,6 +50,10 @@ def load():
         lib.orc_voxelize.restype = C.c_int64
         lib.orc_voxelize.argtypes = [C.c_int, C.c_float, _FP, C.c_int, _FP, _FP, _UP, C.c_void_p, C.c_void_p,
                                      C.c_int, _UP, _FP, _DP, _UP]
+        if hasattr(lib, "orc_voxelize_draws"):
+            lib.orc_voxelize_draws.restype = C.c_int64
+            lib.orc_voxelize_draws.argtypes = [C.c_int, C.c_float, C.c_int, _FP, C.c_void_p, _FP, _FP, _UP,
+                                               C.c_void_p, C.c_void_p, C.c_int, _UP, _FP, _DP, _UP]
         lib.orc_rgba8_avg_step.restype = C.c_uint32
         lib.orc_rgba8_avg_step.argtypes = [C.c_uint32, _FP]
         lib.orc_set_round_mode.argtypes = [C.c_int]
@@ -123,12 +127,22 @@ def rgba8_avg_step(stored: int, color) -> int:
 
 
 def voxelize(res, cube_size, M, pos, nrm, tri_mat, mats, lights, want=("ref", "q32", "f64", "count")):
+    """One draw (model matrix M) voxelized into level 0."""
+    return voxelize_draws(res, cube_size, [(M, pos, nrm, tri_mat)], mats, lights, want)
+
+
+def voxelize_draws(res, cube_size, draws, mats, lights, want=("ref", "q32", "f64", "count")):
+    """Draws [(M, pos, nrm, tri_mat or None), ...] in submission order
+    voxelized into level 0 (orc_voxelize_draws)."""
     lib = load()
-    M = _c(M, np.float32).reshape(16)
-    pos = _c(pos, np.float32)
-    nrm = _c(nrm, np.float32)
-    ntri = pos.reshape(-1, 9).shape[0]
-    tm = None if tri_mat is None else _c(tri_mat, np.uint32)
+    Ms = np.ascontiguousarray(np.stack([np.asarray(d[0], np.float32).reshape(16) for d in draws]))
+    counts = np.array([np.asarray(d[1]).reshape(-1, 9).shape[0] for d in draws], np.int32)
+    pos = _c(np.concatenate([np.asarray(d[1], np.float32).reshape(-1) for d in draws]), np.float32)
+    nrm = _c(np.concatenate([np.asarray(d[2], np.float32).reshape(-1) for d in draws]), np.float32)
+    tm = None
+    if any(d[3] is not None for d in draws):
+        tm = _c(np.concatenate([np.zeros(c, np.uint32) if d[3] is None else np.asarray(d[3], np.uint32).reshape(-1)
+                                for d, c in zip(draws, counts)]), np.uint32)
     mb = material_bytes(mats)
     lb = light_bytes(lights)
     n = res ** 3
@@ -137,10 +151,10 @@ def voxelize(res, cube_size, M, pos, nrm, tri_mat, mats, lights, want=("ref", "q
     q32 = np.zeros((n, 4), np.float32) if "q32" in want else None
     f64 = np.zeros((n, 4), np.float64) if "f64" in want else None
     cnt = np.zeros(n, np.uint32) if "count" in want else None
-    nf = lib.orc_voxelize(res, cube_size, _f(M), ntri, _f(pos), _f(nrm), None if tm is None else _u(tm),
-                          mb.ctypes.data, lb.ctypes.data, len(lights),
-                          None if ref is None else _u(ref), None if q32 is None else _f(q32),
-                          None if f64 is None else _d(f64), None if cnt is None else _u(cnt))
+    nf = lib.orc_voxelize_draws(res, cube_size, len(draws), _f(Ms), counts.ctypes.data, _f(pos), _f(nrm),
+                                None if tm is None else _u(tm), mb.ctypes.data, lb.ctypes.data, len(lights),
+                                None if ref is None else _u(ref), None if q32 is None else _f(q32),
+                                None if f64 is None else _d(f64), None if cnt is None else _u(cnt))
     s = (res, res, res)
     if ref is not None:
         out["ref"] = ref.reshape(s)

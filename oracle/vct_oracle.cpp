@@ -431,11 +431,23 @@ uint32_t orc_rgba8_avg_step(uint32_t stored, const float color[4]) {
 // only the fragments whose voxel lies in the slab, so each voxel sees its
 // fragments in the canonical order and the running average, the 16.16 sums
 // and the double sums are those of a single sequential pass.
-int64_t orc_voxelize(int res, float cube_size, const float* M, int ntri, const float* pos, const float* nrm,
-                     const uint32_t* tri_mat, const orc_material* mats, const orc_light* lights, int nlights,
-                     uint32_t* out_ref, float* out_q32, double* out_f64, uint32_t* out_count) {
-  float N[9];
-  normal_matrix(M, N);
+//
+// orc_voxelize_draws takes several draws in submission order (Renderer::render
+// queues them, renderer.cpp:242-256): Ms holds one model matrix per draw,
+// draw_ntri the triangle count of each, and pos / nrm / tri_mat the draws'
+// triangles one after the other; the canonical order runs over the draws.
+int64_t orc_voxelize_draws(int res, float cube_size, int ndraw, const float* Ms, const int* draw_ntri,
+                           const float* pos, const float* nrm, const uint32_t* tri_mat, const orc_material* mats,
+                           const orc_light* lights, int nlights, uint32_t* out_ref, float* out_q32, double* out_f64,
+                           uint32_t* out_count) {
+  int ntri = 0;
+  for (int d = 0; d < ndraw; ++d) ntri += draw_ntri[d];
+  std::vector<int> tdraw((size_t)ntri);
+  std::vector<float> Ns((size_t)ndraw * 9);
+  for (int d = 0, t = 0; d < ndraw; ++d) {
+    normal_matrix(Ms + 16 * (size_t)d, Ns.data() + 9 * (size_t)d);
+    for (int k = 0; k < draw_ntri[d]; ++k) tdraw[(size_t)t++] = d;
+  }
   const int W = 2 * res;  // viewport 2res x 2res (renderer.cpp:339-340)
   const size_t nvox = (size_t)res * res * res;
   const bool need32 = out_ref || out_q32, need64 = out_f64 != nullptr;
@@ -457,6 +469,7 @@ int64_t orc_voxelize(int res, float cube_size, const float* M, int ntri, const f
   std::vector<int32_t> zr((size_t)ntri * 2);
 #pragma omp parallel for schedule(static)
   for (int t = 0; t < ntri; ++t) {
+    const float* M = Ms + 16 * (size_t)tdraw[(size_t)t];
     float lo = INFINITY, hi = -INFINITY;
     for (int v = 0; v < 3; ++v) {
       const V3<float> p = xform_point<float>(M, pos + (size_t)t * 9 + v * 3);
@@ -485,6 +498,8 @@ int64_t orc_voxelize(int res, float cube_size, const float* M, int ntri, const f
     if (z0 >= z1) continue;
     for (int t = 0; t < ntri; ++t) {
       if (zr[2 * (size_t)t + 1] < z0 || zr[2 * (size_t)t] >= z1) continue;
+      const float* M = Ms + 16 * (size_t)tdraw[(size_t)t];
+      const float* N = Ns.data() + 9 * (size_t)tdraw[(size_t)t];
       V3<float> w[3];
       V3<double> wd[3];
       V3<float> n[3];
@@ -583,6 +598,13 @@ int64_t orc_voxelize(int res, float cube_size, const float* M, int ntri, const f
       for (int c = 0; c < 4; ++c) out_f64[v * 4 + c] = out_f64[v * 4 + c] / (double)k;
   }
   return total;
+}
+
+int64_t orc_voxelize(int res, float cube_size, const float* M, int ntri, const float* pos, const float* nrm,
+                     const uint32_t* tri_mat, const orc_material* mats, const orc_light* lights, int nlights,
+                     uint32_t* out_ref, float* out_q32, double* out_f64, uint32_t* out_count) {
+  return orc_voxelize_draws(res, cube_size, 1, M, &ntri, pos, nrm, tri_mat, mats, lights, nlights, out_ref, out_q32,
+                            out_f64, out_count);
 }
 
 // Storage rounding of n floats in place (dtype 0..3) and packing helpers.

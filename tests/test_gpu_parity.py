@@ -126,21 +126,107 @@ def test_voxelize_exact_mean_bit_exact(gpu_available, orc, dtype, kind, res, ntr
     assert err <= bound
 
 
+# ---- voxelize under a model matrix (voxelize.vert:24-30) ----------------------------
+@pytest.mark.parametrize("dtype,average", [(L.RGBA8, L.AVG_REFERENCE), (L.FP16, L.AVG_EXACT), (L.FP32, L.AVG_EXACT)])
+def test_voxelize_with_model_matrix_bit_exact(gpu_available, orc, dtype, average):
+    """Rotation, translation and non-uniform scale: world positions through
+    the model matrix, normals through its inverse transpose."""
+    say(f"voxelize with a model matrix dtype={dtype} average={average}")
+    sc = SceneSpec("soup", res=128, ntri=20000, model="rot_scale")
+    with Context(32, 32) as ctx:
+        sc.build(ctx, dtype, average)
+        ctx.voxelize()
+        got = ctx.download_level(0)
+    want = orc.voxelize(sc.res, sc.cube_size, sc.M, sc.pos, sc.nrm, sc.tri_mat, sc.mat_records, sc.lights,
+                        want=("ref", "q32"))
+    w = want["ref"] if average == L.AVG_REFERENCE else encode_level(want["q32"], dtype)
+    assert np.array_equal(bits(got), bits(w)), "level 0 differs"
+
+
+@pytest.mark.parametrize("dtype,average", [(L.RGBA8, L.AVG_REFERENCE), (L.FP16, L.AVG_EXACT)])
+def test_voxelize_many_models_bit_exact(gpu_available, orc, dtype, average):
+    """Any number of queued models (renderer.cpp:242-256): 37 draws, two of
+    them with a model matrix, in one frame; two frames in a row."""
+    say(f"voxelize over 37 models dtype={dtype} average={average}")
+    from helpers import model_matrix
+
+    sc = SceneSpec("soup", res=128, ntri=20000)
+    cuts = np.linspace(0, 20000, 38).astype(int)
+    draws = []
+    with Context(32, 32) as ctx:
+        sc.build(ctx, dtype, average)
+        ctx.clear_queues()
+        for k, (a, b) in enumerate(zip(cuts[:-1], cuts[1:])):
+            m = ctx.model_create(sc.pos[a:b], sc.nrm[a:b], sc.tri_mat[a:b])
+            M = model_matrix("rot_scale") if k in (3, 30) else sc.M
+            ctx.set_model_transform(m, M)
+            ctx.queue_model(m)
+            draws.append((M, sc.pos[a:b], sc.nrm[a:b], sc.tri_mat[a:b]))
+        for (p, c, i, k) in sc.lights:
+            ctx.queue_light(p, c, i, k)
+        for _ in range(2):  # the second frame reuses the draw table
+            ctx.voxelize()
+        got = ctx.download_level(0)
+    want = orc.voxelize_draws(sc.res, sc.cube_size, draws, sc.mat_records, sc.lights, want=("ref", "q32"))
+    w = want["ref"] if average == L.AVG_REFERENCE else encode_level(want["q32"], dtype)
+    assert np.array_equal(bits(got), bits(w)), "level 0 differs"
+
+
 # ---- G-buffer -----------------------------------------------------------------------
-@pytest.mark.parametrize("kind,ntri,W,H", [("cornell", 34, 96, 64), ("soup", 5000, 160, 90), ("cornell", 34, 512, 512),
-                                           ("soup", 40000, 640, 360), ("soup", 3000, 100, 70)])
-def test_gbuffer_bit_exact(gpu_available, orc, kind, ntri, W, H):
-    say(f"gbuffer parity kind={kind} {W}x{H}")
-    sc = SceneSpec(kind, ntri=ntri, W=W, H=H)
+INSIDE = dict(eye=(0.12, 0.05, 0.2), target=(-0.25, -0.1, -0.5), fovy_deg=75.0)
+
+
+@pytest.mark.parametrize("kind,ntri,W,H,extra", [
+    ("cornell", 34, 96, 64, {}), ("soup", 5000, 160, 90, {}), ("cornell", 34, 512, 512, {}),
+    ("soup", 40000, 640, 360, {}), ("soup", 3000, 100, 70, {}),
+    # the camera inside the box: walls crossing the near plane are clipped and drawn as fans
+    ("cornell", 34, 128, 96, INSIDE), ("cornell", 34, 640, 480, INSIDE),
+    # a camera among the soup's triangles: many cross the near plane
+    ("soup", 20000, 320, 180, dict(eye=(0.05, 0.02, 0.1), target=(0.4, -0.1, -1.0), fovy_deg=90.0)),
+    # a model matrix (voxel_cone_tracing.vert:22-28)
+    ("soup", 20000, 160, 90, dict(model="rot_scale"))],
+    ids=["cornell", "soup", "cornell512", "soup640", "ragged", "inside", "inside640", "in_soup", "model"])
+def test_gbuffer_bit_exact(gpu_available, orc, kind, ntri, W, H, extra):
+    say(f"gbuffer parity kind={kind} {W}x{H} {extra}")
+    sc = SceneSpec(kind, ntri=ntri, W=W, H=H, **extra)
     with Context(W, H) as ctx:
         sc.build(ctx, L.FP16)
         for _ in range(2):  # the second frame reuses the tile lists and counters
             ctx.gbuffer()
         gp, gn = ctx.download_gbuffer()
-    p32, n32, _, _, _ = orc.gbuffer(W, H, sc.view, sc.proj, [(sc.M, sc.pos, sc.nrm, sc.tri_mat)])
+    p32, n32, _, _, keys = orc.gbuffer(W, H, sc.view, sc.proj, [(sc.M, sc.pos, sc.nrm, sc.tri_mat)])
     cov_g, cov_w = gp[..., 3] != 0, p32[..., 3] != 0
     assert cov_w.any()
+    fans = int(((keys & np.uint64(7)) != 0).sum())
+    say(f"  covered {int(cov_w.sum())} pixels, {fans} of them by fan triangles beyond the first")
     assert np.array_equal(cov_g, cov_w)
+    assert np.array_equal(bits(gp), bits(p32))
+    assert np.array_equal(bits(gn), bits(n32))
+
+
+def test_gbuffer_many_models(gpu_available, orc):
+    """Any number of queued models (renderer.cpp:242-256): 37 draws, two of
+    them with a model matrix, camera inside the soup (clipped triangles)."""
+    say("gbuffer parity over 37 models")
+    from helpers import model_matrix
+
+    sc = SceneSpec("soup", ntri=6000, W=200, H=120, eye=(0.05, 0.02, 0.6), target=(0.0, 0.0, -1.0), fovy_deg=80.0)
+    cuts = np.linspace(0, 6000, 38).astype(int)
+    draws = []
+    with Context(sc.W, sc.H) as ctx:
+        sc.build(ctx, L.FP16)
+        ctx.clear_queues()
+        for k, (a, b) in enumerate(zip(cuts[:-1], cuts[1:])):
+            m = ctx.model_create(sc.pos[a:b], sc.nrm[a:b], sc.tri_mat[a:b])
+            M = model_matrix("rot_scale") if k in (3, 30) else sc.M
+            ctx.set_model_transform(m, M)
+            ctx.queue_model(m)
+            draws.append((M, sc.pos[a:b], sc.nrm[a:b], sc.tri_mat[a:b]))
+        for _ in range(2):  # the second frame reuses the draw table and the lists
+            ctx.gbuffer()
+        gp, gn = ctx.download_gbuffer()
+    p32, n32, _, _, _ = orc.gbuffer(sc.W, sc.H, sc.view, sc.proj, draws)
+    assert (p32[..., 3] != 0).any()
     assert np.array_equal(bits(gp), bits(p32))
     assert np.array_equal(bits(gn), bits(n32))
 

@@ -24,6 +24,11 @@ GOLD = Path(__file__).resolve().parent / "golden"
 SCENES = {
     "cornell64": dict(kind="cornell", res=64, W=128, H=96),
     "soup64": dict(kind="soup", res=64, ntri=1500, W=96, H=64),
+    # the camera inside the box (walls clipped at the near plane), every phase on
+    "cornell64_inside": dict(kind="cornell", res=64, W=128, H=96, phases=(True, True, True, True),
+                             eye=(0.12, 0.05, 0.2), target=(-0.25, -0.1, -0.5), fovy_deg=75.0),
+    # a model matrix with rotation and non-uniform scale
+    "soup64_xform": dict(kind="soup", res=64, ntri=1500, W=96, H=64, model="rot_scale"),
 }
 
 
@@ -33,6 +38,21 @@ def _scene(name):
         sc.lights = [((0.3, 1.2, 0.5), (1.0, 1.0, 1.0), 2.0, 0)]
     sc.cone_set = L.CONES_REF9  # the reference's trace_diffuse
     return sc, np.load(GOLD / f"glref_{name}.npz")
+
+
+def _moved_pairs_only(got_occ, ref_occ):
+    """Occupancy equal up to exact ties: a fragment whose coordinate lies
+    within rounding of a voxel boundary lands one voxel further on one side
+    (llvmpipe interpolates by plane equations); each such voxel shows up as a
+    pair of neighbours, rare."""
+    only_g, only_r = np.argwhere(got_occ & ~ref_occ), np.argwhere(ref_occ & ~got_occ)
+    assert len(only_g) == len(only_r) and len(only_g) <= max(1, ref_occ.sum() // 200)
+    for a in only_g:
+        assert (np.abs(only_r - a).sum(1) == 1).any(), a
+    near = np.zeros(ref_occ.shape, bool)
+    for z, y, x in np.concatenate([only_g, only_r]):
+        near[max(z - 1, 0):z + 2, max(y - 1, 0):y + 2, max(x - 1, 0):x + 2] = True
+    return near
 
 
 def _bytes(a):
@@ -58,8 +78,10 @@ def test_voxelize_against_reference_shader(gpu_available, name):
         ctx.voxelize()
         got = ctx.download_level(0)
     ref = g["level0"]
-    assert np.array_equal(got != 0, ref != 0), "occupancy / coordinates differ from the reference shader"
-    occ = ref != 0
+    if name in ("cornell64", "soup64"):
+        assert np.array_equal(got != 0, ref != 0), "occupancy / coordinates differ from the reference shader"
+    near = _moved_pairs_only(got != 0, ref != 0)
+    occ = (ref != 0) & (got != 0) & ~near
     same = (got == ref)[occ].mean()
     worst = np.abs(_bytes(got[occ]) - _bytes(ref[occ])).max()
     print(f"  bit-identical voxels {same:.4f}, max byte difference {worst}")
@@ -102,13 +124,20 @@ def test_trace_against_reference_shader(gpu_available, name):
         img = ctx.download_image()
         ctx.gbuffer()
         gp, _ = ctx.download_gbuffer()
+        ctx.render()  # the whole frame on the GPU
+        frame = ctx.download_image()
     cov = g["gpos"][..., 3] != 0
     e = img[cov][:, :3] - g["image"][cov][:, :3]
     psnr = 10 * np.log10(1.0 / np.mean(e.astype(np.float64) ** 2))
-    print(f"  max-abs {np.abs(e).max():.3e}, PSNR {psnr:.1f} dB")
+    ef = frame[cov][:, :3].astype(np.float64) - g["image"][cov][:, :3]
+    psnr_f = 10 * np.log10(1.0 / np.mean(ef ** 2))
+    agree = ((gp[..., 3] != 0) == cov).mean()
+    print(f"  max-abs {np.abs(e).max():.3e}, PSNR {psnr:.1f} dB; G-buffer coverage agreement {agree:.5f}; "
+          f"whole GPU frame vs llvmpipe {psnr_f:.1f} dB")
     assert psnr > 60.0 and np.abs(e).max() < 5e-3
-    # the G-buffer raster covers the same pixels as GL's
-    assert ((gp[..., 3] != 0) == cov).mean() > 0.995
+    # the G-buffer raster covers the same pixels as GL's (clipped triangles included)
+    assert agree > 0.995
+    assert psnr_f > 40.0
 
 
 def test_full_compose_against_reference_shader(gpu_available):

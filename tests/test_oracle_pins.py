@@ -356,3 +356,46 @@ def test_dominant_axis_and_ties_go_to_xz():
     assert oracle.dominant_axis([0, 0, 0, 1, 0, 0, 0, 1, -1]) == 1
     # equal in x and y: z is smaller, x > y fails -> XZ
     assert oracle.dominant_axis([0, 0, 0, 0, 0, 1, 1, -1, 0]) == 1
+
+
+# --------------------------------------------------------------------------------------
+# several draws in one frame (renderer.cpp:242-256: each queued model is one draw)
+# --------------------------------------------------------------------------------------
+
+
+def _soup_draws(n_draws, model_at=()):
+    from helpers import SceneSpec, model_matrix
+
+    sc = SceneSpec("soup", res=32, ntri=3000)
+    cuts = np.linspace(0, 3000, n_draws + 1).astype(int)
+    draws = [(model_matrix("rot_scale") if k in model_at else sc.M, sc.pos[a:b], sc.nrm[a:b], sc.tri_mat[a:b])
+             for k, (a, b) in enumerate(zip(cuts[:-1], cuts[1:]))]
+    return sc, draws
+
+
+def test_voxelize_split_into_draws_is_the_single_draw():
+    """Identity matrices: the draws' triangles in submission order are the
+    single draw's triangles in index order, so every output, the
+    order-dependent running average included, is the same."""
+    sc, draws = _soup_draws(7)
+    one = oracle.voxelize(sc.res, sc.cube_size, sc.M, sc.pos, sc.nrm, sc.tri_mat, sc.mat_records, sc.lights)
+    many = oracle.voxelize_draws(sc.res, sc.cube_size, draws, sc.mat_records, sc.lights)
+    assert one["fragments"] > 300
+    for k in ("ref", "q32", "count"):
+        assert np.array_equal(one[k], many[k]), k
+
+
+def test_voxelize_draws_each_take_their_own_matrix():
+    """Fragments of a draw depend only on that draw's matrix: counts add up
+    over the draws, and so do the double-precision colour sums; a draw
+    voxelized with another draw's matrix moves fragments and fails both."""
+    sc, draws = _soup_draws(3, model_at=(1,))
+    both = oracle.voxelize_draws(sc.res, sc.cube_size, draws, sc.mat_records, sc.lights)
+    parts = [oracle.voxelize_draws(sc.res, sc.cube_size, [d], sc.mat_records, sc.lights) for d in draws]
+    assert np.array_equal(both["count"], sum(p["count"] for p in parts))
+    s_both = both["f64"] * both["count"][..., None]
+    s_parts = sum(p["f64"] * p["count"][..., None] for p in parts)
+    assert np.allclose(s_both, s_parts, rtol=1e-12, atol=1e-12)
+    # the transformed draw lands elsewhere than it would untransformed
+    plain = oracle.voxelize_draws(sc.res, sc.cube_size, [(sc.M,) + draws[1][1:]], sc.mat_records, sc.lights)
+    assert not np.array_equal(plain["count"], parts[1]["count"])

@@ -141,6 +141,7 @@ PROTOTYPES = {
     "vct_mlp_set_params": (_I, [_P, _FP, _U]),
     "vct_mlp_get_params": (_I, [_P, _FP, _U]),
     "vct_mlp_forward": (_I, [_P, _FP, _U, _FP]),
+    "vct_trace_load_peak": (_I, [_P, C.POINTER(C.c_double)]),
     "vct_mlp_loss_grad": (_I, [_P, _FP, _FP, _FP, _U, _FP, _FP]),
     "vct_upsample_collect": (_I, [_P, _U, C.c_uint64]),
     "vct_upsample_clear_samples": (_I, [_P]),

@@ -68,14 +68,15 @@ def build(verbose: bool = True) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
-    # headless driver of the Renderer API (links libvct.so next to it)
-    demo_src = PKG / "apps" / "vct_demo.cpp"
-    if demo_src.exists():
-        cmd = ["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", str(demo_src), "-o", str(DEMO), f"-L{PKG}", "-lvct",
+    # host programs on the Renderer API (vct_demo: the headless frame loop;
+    # vct_reference_app: the reference application's scene), next to libvct.so
+    for app in sorted((PKG / "apps").glob("*.cpp")):
+        exe = PKG / app.stem
+        cmd = ["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", str(app), "-o", str(exe), f"-L{PKG}", "-lvct",
                "-Wl,-rpath,$ORIGIN"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
-            raise RuntimeError(f"vct_demo build failed:\n{r.stdout}\n{r.stderr}")
+            raise RuntimeError(f"{app.stem} build failed:\n{r.stdout}\n{r.stderr}")
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources")
     return LIB

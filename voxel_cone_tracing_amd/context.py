@@ -274,6 +274,12 @@ class Context:
         L.check(self.lib.vct_trace_count_steps(self.h, C.byref(n)))
         return n.value
 
+    def load_peak_gbps(self) -> float:
+        """Ceiling of the marcher's texel loads (L1-hit 16-byte loads, GB/s)."""
+        g = C.c_double()
+        L.check(self.lib.vct_trace_load_peak(self.h, C.byref(g)))
+        return g.value
+
     def voxelize_stats(self):
         f, c = C.c_uint64(), C.c_uint64()
         L.check(self.lib.vct_get_voxelize_stats(self.h, C.byref(f), C.byref(c)))

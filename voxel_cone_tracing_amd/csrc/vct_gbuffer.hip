@@ -5,17 +5,26 @@
 // projection * view * world, normal by the inverse-transpose model matrix)
 // rasterised at W x H with GL_LESS depth (renderer.cpp:355-390).  Here, a
 // tiled rasteriser with the depth buffer in LDS:
-//  1. setup + bin count (lane per triangle): clip, near-plane reject,
-//     1/256-pixel snap, pixel bounding box; count into the 32x32 tiles the
-//     box overlaps (one atomic per distinct tile per wave).
+//  1. setup + bin count (lane per triangle): clip coordinates, GL's primitive
+//     clipping (below), 1/256-pixel snap, pixel bounding box; count into the
+//     32x32 tiles the box overlaps (one atomic per distinct tile per wave).
 //  2. exclusive scan of the tile counts.
-//  3. bin write: triangle indices into per-tile lists.
+//  3. bin write: (triangle, fan triangle) entries into per-tile lists.
 //  4. tile pass (workgroup per tile): exact 64-bit edge functions with the
 //     top-left rule, screen-linear depth, LDS atomicMin of
-//     (depth bits << 32 | triangle id); then perspective-correct world
-//     position and normal of each pixel's winner, linear view depth and
-//     material id, written once per pixel.
-// Triangles with a vertex behind the near plane are skipped (not clipped).
+//     (depth bits << 32 | triangle << 3 | fan triangle); then
+//     perspective-correct world position and normal of each pixel's winner,
+//     linear view depth and material id, written once per pixel.
+// Clipping: GL clips every primitive against the view volume.  A triangle
+// whose vertices all lie in front of the near plane (w > 0, z >= -w) and
+// inside a guard band of +-256 NDC units is drawn as it is; any other is
+// clipped in clip space against the near plane and the guard-band planes
+// x, y = +-255 w (Sutherland-Hodgman, new vertices interpolated from the
+// inside end of their edge) and the polygon left is drawn as the fan
+// (0, k + 1, k + 2).  Its vertices carry weights over the original three, so
+// attributes resolve against the original vertices.  Clipped triangles are
+// rare (they pass near the eye), so their polygon is recomputed where it is
+// needed instead of stored.
 #include <cmath>
 #include <cstring>
 
@@ -35,13 +44,12 @@ struct GDraw {
   float N[9];
 };
 
-constexpr int kMaxGDraws = 16;
-
-// Passed by value: no host staging buffer outlives the launch.
+// Passed by value; the draw table lives in device memory (any number of
+// queued models).
 struct GParams {
-  GDraw draws[kMaxGDraws];
-  float PV[16];  // projection * view
+  const GDraw* draws;
   int ndraw;
+  float PV[16];  // projection * view
   uint32_t ntri;
   int W, H;
 };
@@ -54,11 +62,16 @@ struct GSetup {
   int i0, j0, i1, j1;  // pixel bounding box, inclusive
 };
 
+// The draw of frame triangle t (stored or original numbering: both count
+// from the draw's tri_base): the last draw whose tri_base is <= t.
 __device__ __forceinline__ int g_find_draw(const GParams& p, uint32_t t) {
-  int d = 0;
-  for (int i = 1; i < p.ndraw; ++i)
-    if (t >= p.draws[i].tri_base) d = i;
-  return d;
+  int lo = 0, hi = p.ndraw - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (p.draws[mid].tri_base <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
 }
 
 __device__ __forceinline__ void world_of(const GDraw& dr, const float* p, float* w) {
@@ -72,10 +85,10 @@ __device__ __forceinline__ void world_of(const GDraw& dr, const float* p, float*
   }
 }
 
-__device__ __forceinline__ void g_setup(const GDraw& dr, const float* PV, uint32_t lt, int W, int H, GSetup& s) {
+// Clip coordinates of the three vertices of stored triangle lt of a draw.
+__device__ __forceinline__ void g_clip_coords(const GDraw& dr, const float* PV, uint32_t lt, float (&clip)[3][4]) {
 #pragma clang fp contract(off)
   const float* p = dr.pos + (size_t)lt * 9;
-  bool ok = true;
   for (int v = 0; v < 3; ++v) {
     float w[3];
     world_of(dr, p + 3 * v, w);
@@ -84,16 +97,35 @@ __device__ __forceinline__ void g_setup(const GDraw& dr, const float* PV, uint32
       t = t + PV[4 + k] * w[1];
       t = t + PV[8 + k] * w[2];
       t = t + PV[12 + k];
-      s.clip[v][k] = t;
+      clip[v][k] = t;
     }
-    const float cw = s.clip[v][3];
-    if (!(cw > 0.0f) || s.clip[v][2] < -cw) ok = false;
   }
+}
+
+// True if the triangle is drawn unclipped: every vertex in front of the near
+// plane and inside the +-256 NDC guard band.
+__device__ __forceinline__ bool g_unclipped(const float (&clip)[3][4]) {
+#pragma clang fp contract(off)
+  bool in = true;
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    const float cw = clip[v][3];
+    if (!(cw > 0.0f) || clip[v][2] < -cw) in = false;
+    else if (!(fabsf(clip[v][0] / cw) < 256.0f && fabsf(clip[v][1] / cw) < 256.0f)) in = false;
+  }
+  return in;
+}
+
+// Snap, orientation and pixel bounding box of a triangle given by the clip
+// coordinates of its vertices (every w > 0, NDC within the guard band).
+__device__ __forceinline__ void g_setup_clip(GSetup& s, int W, int H) {
+#pragma clang fp contract(off)
   const float hw = 0.5f * (float)W, hh = 0.5f * (float)H;
+  bool ok = true;
 #pragma unroll
   for (int v = 0; v < 3; ++v) {
     float xn = s.clip[v][0] / s.clip[v][3], yn = s.clip[v][1] / s.clip[v][3];
-    if (!(fabsf(xn) < 256.0f && fabsf(yn) < 256.0f)) {
+    if (!(s.clip[v][3] > 0.0f) || !(fabsf(xn) < 256.0f && fabsf(yn) < 256.0f)) {
       ok = false;
       xn = 0.0f;
       yn = 0.0f;
@@ -121,6 +153,83 @@ __device__ __forceinline__ void g_setup(const GDraw& dr, const float* PV, uint32
   s.i1 = (int)imax;
   s.j1 = (int)jmax;
   s.ok = true;
+}
+
+// A vertex of a clipped triangle: clip coordinates and its weights over the
+// triangle's three original vertices.
+struct ClipVert {
+  float c[4];
+  float b[3];
+};
+
+__device__ __forceinline__ float g_plane_dist(int plane, const float* c) {
+#pragma clang fp contract(off)
+  const float g = 255.0f;
+  switch (plane) {
+    case 0: return c[2] + c[3];  // near: z >= -w
+    case 1: return g * c[3] - c[0];
+    case 2: return g * c[3] + c[0];
+    case 3: return g * c[3] - c[1];
+    default: return g * c[3] + c[1];
+  }
+}
+
+// Clips the triangle (clip coordinates) against the near plane and the
+// guard-band planes; returns the polygon's vertex count (0 or 3..8).
+__device__ __noinline__ int g_clip_polygon(const float (&clip)[3][4], ClipVert (&P)[8]) {
+#pragma clang fp contract(off)
+  int n = 3;
+  for (int v = 0; v < 3; ++v) {
+    for (int k = 0; k < 4; ++k) P[v].c[k] = clip[v][k];
+    for (int k = 0; k < 3; ++k) P[v].b[k] = k == v ? 1.0f : 0.0f;
+  }
+  for (int plane = 0; plane < 5 && n > 0; ++plane) {
+    ClipVert out[8];
+    int m = 0;
+    for (int i = 0; i < n; ++i) {
+      const ClipVert a = P[i];
+      const ClipVert b = P[i + 1 < n ? i + 1 : 0];
+      const float da = g_plane_dist(plane, a.c), db = g_plane_dist(plane, b.c);
+      const bool ina = da >= 0.0f, inb = db >= 0.0f;
+      if (ina && m < 8) out[m++] = a;
+      if (ina != inb && m < 8) {
+        const ClipVert& in = ina ? a : b;
+        const ClipVert& ot = ina ? b : a;
+        const float din = ina ? da : db, dout = ina ? db : da;
+        const float t = din / (din - dout);
+        ClipVert r;
+        for (int k = 0; k < 4; ++k) r.c[k] = in.c[k] + t * (ot.c[k] - in.c[k]);
+        for (int k = 0; k < 3; ++k) r.b[k] = in.b[k] + t * (ot.b[k] - in.b[k]);
+        out[m++] = r;
+      }
+    }
+    n = m;
+    for (int i = 0; i < m; ++i) P[i] = out[i];
+  }
+  return n < 3 ? 0 : n;
+}
+
+// Setup of fan triangle k of the clipped triangle; wts receives the weights
+// of its three vertices over the original ones.
+__device__ __noinline__ void g_setup_fan(const float (&clip)[3][4], int k, int W, int H, GSetup& s,
+                                         float (&wts)[3][3]) {
+  ClipVert P[8];
+  const int n = g_clip_polygon(clip, P);
+  s.ok = false;
+  if (k + 2 >= n) return;
+  const int idx[3] = {0, k + 1, k + 2};
+  for (int v = 0; v < 3; ++v) {
+    for (int c = 0; c < 4; ++c) s.clip[v][c] = P[idx[v]].c[c];
+    for (int c = 0; c < 3; ++c) wts[v][c] = P[idx[v]].b[c];
+  }
+  g_setup_clip(s, W, H);
+}
+
+// Number of fan triangles of a clipped triangle.
+__device__ __noinline__ int g_fan_count(const float (&clip)[3][4]) {
+  ClipVert P[8];
+  const int n = g_clip_polygon(clip, P);
+  return n ? n - 2 : 0;
 }
 
 __device__ __forceinline__ bool g_top_left(int64_t dx, int64_t dy) { return dy < 0 || (dy == 0 && dx < 0); }
@@ -186,13 +295,34 @@ struct alignas(16) GRec {
   int32_t X[3], Y[3];
   float z[3];                // window depth of each vertex
   int16_t i0, j0, i1, j1;    // pixel bounding box, inclusive; i0 > i1 when culled
-  uint32_t swap;
-  uint32_t otri;             // original triangle index in the frame (GL draw order)
-  uint32_t pad;
+  uint32_t swap;             // bit 0: vertices 1 and 2 swapped for CCW; bit 1: clipped (drawn as its fan records)
+  uint32_t otri;             // original triangle index in the frame (GL draw order) << 3 | fan triangle
+  uint32_t nfan;             // clipped: fan triangles, stored from fan record `area`
 };
 static_assert(sizeof(GRec) == 64, "GRec must be 64 bytes");
+constexpr uint32_t kClipped = 2u;
+
+// What the resolve of a fan triangle needs besides its record: the clip w of
+// its vertices and their weights over the original three.
+struct alignas(16) GFan {
+  float w[3];
+  float wts[3][3];
+  float pad[4];
+};
+static_assert(sizeof(GFan) == 64, "GFan must be 64 bytes");
 
 constexpr int kTile = 32;  // screen tile edge in pixels (one workgroup per tile)
+
+__device__ __forceinline__ GSetup g_setup_of(const GRec& r) {
+  GSetup s;
+  s.area = r.area;
+  s.swap = (r.swap & 1u) != 0u;
+  for (int v = 0; v < 3; ++v) {
+    s.X[v] = r.X[v];
+    s.Y[v] = r.Y[v];
+  }
+  return s;
+}
 
 struct TileRange {
   int tx0, ty0, tw, n;
@@ -210,10 +340,57 @@ __device__ __forceinline__ TileRange tile_range(const GRec& r) {
   return t;
 }
 
+__device__ __forceinline__ void g_rec_from(const GSetup& s, GRec& r) {
+#pragma clang fp contract(off)
+  r.area = s.area;
+  for (int v = 0; v < 3; ++v) {
+    r.X[v] = (int32_t)s.X[v];
+    r.Y[v] = (int32_t)s.Y[v];
+    r.z[v] = s.ok ? (s.clip[v][2] / s.clip[v][3]) * 0.5f + 0.5f : 0.0f;
+  }
+  r.i0 = 1;
+  r.i1 = 0;
+  r.j0 = 1;
+  r.j1 = 0;
+  if (s.ok) {
+    r.i0 = (int16_t)s.i0;
+    r.j0 = (int16_t)s.j0;
+    r.i1 = (int16_t)s.i1;
+    r.j1 = (int16_t)s.j1;
+  }
+  r.swap = s.swap ? 1u : 0u;
+}
+
+// The record (and resolve data) of fan triangle k of clipped stored triangle t.
+__device__ __noinline__ GRec g_fan_record(const GParams& p, uint32_t t, uint32_t k, GFan* fan) {
+  const int d = g_find_draw(p, t);
+  const GDraw& dr = p.draws[d];
+  float clip[3][4];
+  g_clip_coords(dr, p.PV, t - dr.tri_base, clip);
+  GSetup s;
+  float wts[3][3];
+  g_setup_fan(clip, (int)k, p.W, p.H, s, wts);
+  GRec r;
+  g_rec_from(s, r);
+  r.otri = ((dr.tri_base + dr.perm[t - dr.tri_base]) << 3) | k;
+  r.nfan = 0;
+  if (fan) {
+    for (int v = 0; v < 3; ++v) {
+      fan->w[v] = s.clip[v][3];
+      for (int o = 0; o < 3; ++o) fan->wts[v][o] = wts[v][o];
+    }
+    for (int q = 0; q < 4; ++q) fan->pad[q] = 0.0f;
+  }
+  return r;
+}
+
 // 1. setup (lane per stored triangle): clip, snap, bounding box; count the
-//    triangle into every 32x32 tile its bounding box overlaps.
+//    triangle into every 32x32 tile its bounding box overlaps.  A triangle that
+//    needs clipping is only flagged and listed here: its fans are counted and
+//    written by g_fan_count_kernel / g_fan_write_kernel, so that the polygon
+//    code (private arrays, calls) stays out of this kernel.
 __global__ __launch_bounds__(256) void g_bin_count_kernel(GParams p, GRec* __restrict__ recs, uint32_t* tile_count,
-                                                          int tiles_x) {
+                                                          int tiles_x, uint32_t* clip_list, uint32_t* clip_count) {
 #pragma clang fp contract(off)
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   GRec r;
@@ -221,27 +398,29 @@ __global__ __launch_bounds__(256) void g_bin_count_kernel(GParams p, GRec* __res
   r.i1 = 0;
   r.j0 = 1;
   r.j1 = 0;
+  bool clipped = false;
   if (t < p.ntri) {
     const int d = g_find_draw(p, t);
+    const GDraw& dr = p.draws[d];
+    float clip[3][4];
+    g_clip_coords(dr, p.PV, t - dr.tri_base, clip);
     GSetup s;
-    g_setup(p.draws[d], p.PV, t - p.draws[d].tri_base, p.W, p.H, s);
-    r.area = s.area;
-    for (int v = 0; v < 3; ++v) {
-      r.X[v] = (int32_t)s.X[v];
-      r.Y[v] = (int32_t)s.Y[v];
-      r.z[v] = s.ok ? (s.clip[v][2] / s.clip[v][3]) * 0.5f + 0.5f : 0.0f;
+    if (g_unclipped(clip)) {
+      for (int v = 0; v < 3; ++v)
+        for (int c = 0; c < 4; ++c) s.clip[v][c] = clip[v][c];
+      g_setup_clip(s, p.W, p.H);
+      g_rec_from(s, r);
+    } else {
+      clipped = true;
+      r.area = 0;
+      r.swap = kClipped;
     }
-    if (s.ok) {
-      r.i0 = (int16_t)s.i0;
-      r.j0 = (int16_t)s.j0;
-      r.i1 = (int16_t)s.i1;
-      r.j1 = (int16_t)s.j1;
-    }
-    r.swap = s.swap ? 1u : 0u;
-    r.otri = p.draws[d].tri_base + p.draws[d].perm[t - p.draws[d].tri_base];
-    r.pad = 0;
+    r.nfan = 0;
+    r.otri = (dr.tri_base + dr.perm[t - dr.tri_base]) << 3;
     recs[t] = r;
   }
+  const uint32_t slot = wave_add_one(clip_count, 0u, clipped);
+  if (clipped) clip_list[slot] = t;
   const TileRange tr = tile_range(r);
   for (int k = 0; __ballot(k < tr.n) != 0ull; ++k) {
     const bool act = k < tr.n;
@@ -250,7 +429,30 @@ __global__ __launch_bounds__(256) void g_bin_count_kernel(GParams p, GRec* __res
   }
 }
 
-// 3. list write: the stored triangle index into each overlapped tile's list.
+// 1b. the listed clipped triangles (a fixed grid striding over the device-side
+//     count): each takes its fan records' range and counts its fans' tiles.
+__global__ __launch_bounds__(256) void g_fan_count_kernel(GParams p, GRec* __restrict__ recs,
+                                                          const uint32_t* __restrict__ clip_list,
+                                                          const uint32_t* __restrict__ clip_count,
+                                                          uint32_t* tile_count, int tiles_x, uint32_t* fan_total) {
+  const uint32_t n = *clip_count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t t = clip_list[i];
+    const int d = g_find_draw(p, t);
+    float clip[3][4];
+    g_clip_coords(p.draws[d], p.PV, t - p.draws[d].tri_base, clip);
+    const int nfan = g_fan_count(clip);
+    recs[t].area = nfan ? (int64_t)atomicAdd(fan_total, (uint32_t)nfan) : 0;  // first fan record
+    recs[t].nfan = (uint32_t)nfan;
+    for (int f = 0; f < nfan; ++f) {
+      const TileRange tr = tile_range(g_fan_record(p, t, (uint32_t)f, nullptr));
+      for (int k = 0; k < tr.n; ++k) atomicAdd(tile_count + (tr.ty0 + k / tr.tw) * tiles_x + tr.tx0 + k % tr.tw, 1u);
+    }
+  }
+}
+
+// 3. list write: the record index of each stored triangle into each overlapped
+//    tile's list (clipped triangles have no tiles here).
 __global__ __launch_bounds__(256) void g_bin_write_kernel(uint32_t ntri, const GRec* __restrict__ recs,
                                                           const uint32_t* __restrict__ tile_off, uint32_t* cursor,
                                                           int tiles_x, uint32_t* __restrict__ list) {
@@ -268,10 +470,38 @@ __global__ __launch_bounds__(256) void g_bin_write_kernel(uint32_t ntri, const G
   }
 }
 
+// 3b. the fan records of the clipped triangles and their list entries
+//     (ntri + fan record).
+__global__ __launch_bounds__(256) void g_fan_write_kernel(GParams p, const GRec* __restrict__ recs,
+                                                          const uint32_t* __restrict__ clip_list,
+                                                          const uint32_t* __restrict__ clip_count,
+                                                          const uint32_t* __restrict__ tile_off, uint32_t* cursor,
+                                                          int tiles_x, uint32_t* __restrict__ list,
+                                                          GRec* __restrict__ fan_recs, GFan* __restrict__ fans) {
+  const uint32_t n = *clip_count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t t = clip_list[i];
+    const uint32_t base = (uint32_t)recs[t].area, nfan = recs[t].nfan;
+    for (uint32_t f = 0; f < nfan; ++f) {
+      GFan fd;
+      const GRec fr = g_fan_record(p, t, f, &fd);
+      fan_recs[base + f] = fr;
+      fans[base + f] = fd;
+      const TileRange tr = tile_range(fr);
+      for (int k = 0; k < tr.n; ++k) {
+        const uint32_t key = (uint32_t)((tr.ty0 + k / tr.tw) * tiles_x + tr.tx0 + k % tr.tw);
+        list[tile_off[key] + atomicAdd(cursor + key, 1u)] = p.ntri + base + f;
+      }
+    }
+  }
+}
+
 // Perspective-correct world position, normal, linear depth and material id of
 // pixel (i, j) from its winning key; writes the G-buffer texel.
-__device__ __forceinline__ void g_resolve_pixel(const GParams& p, unsigned long long key, int i, int j,
-                                                float4* __restrict__ gpos, float4* __restrict__ gnrm) {
+__device__ __forceinline__ void g_resolve_pixel(const GParams& p, const GRec* __restrict__ recs,
+                                                const GRec* __restrict__ fan_recs, const GFan* __restrict__ fans,
+                                                unsigned long long key, int i, int j, float4* __restrict__ gpos,
+                                                float4* __restrict__ gnrm) {
 #pragma clang fp contract(off)
   const size_t idx = (size_t)j * p.W + i;
   if (key == ~0ull) {
@@ -279,23 +509,50 @@ __device__ __forceinline__ void g_resolve_pixel(const GParams& p, unsigned long 
     gnrm[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
     return;
   }
-  const uint32_t t = (uint32_t)(key & 0xffffffffu);  // original index
+  const uint32_t id = (uint32_t)(key & 0xffffffffu);
+  const uint32_t t = id >> 3;  // original index
   const int d = g_find_draw(p, t);
   const GDraw& dr = p.draws[d];
-  const uint32_t lt = dr.inv[t - dr.tri_base];       // stored index
+  const uint32_t lt = dr.inv[t - dr.tri_base];  // stored index
+  float clip[3][4];
+  g_clip_coords(dr, p.PV, lt, clip);
+  const bool fan = !g_unclipped(clip);
+  uint32_t fi = 0;  // fan record of a clipped triangle
   GSetup s;
-  g_setup(dr, p.PV, lt, p.W, p.H, s);
+  float cw[3];
+  if (!fan) {
+    for (int v = 0; v < 3; ++v) {
+      for (int c = 0; c < 4; ++c) s.clip[v][c] = clip[v][c];
+      cw[v] = clip[v][3];
+    }
+    g_setup_clip(s, p.W, p.H);
+  } else {
+    fi = (uint32_t)recs[dr.tri_base + lt].area + (id & 7u);
+    s = g_setup_of(fan_recs[fi]);
+    for (int v = 0; v < 3; ++v) cw[v] = fans[fi].w[v];
+  }
   float b[3];
   if (!g_cover(s, i, j, b[0], b[1], b[2])) {  // cannot happen: same arithmetic as the raster
     b[0] = b[1] = b[2] = 1.0f / 3.0f;
   }
-  // perspective-correct weights: (b_i / w_i) / sum(b_k / w_k)
+  // perspective-correct weights: (b_i / w_i) / sum(b_k / w_k) over the drawn triangle
   float q[3], qs = 0.0f;
   for (int v = 0; v < 3; ++v) {
-    q[v] = b[v] / s.clip[v][3];
+    q[v] = b[v] / cw[v];
     qs = qs + q[v];
   }
   for (int v = 0; v < 3; ++v) q[v] = q[v] / qs;
+  if (fan) {  // weights of the original vertices
+    const GFan& fd = fans[fi];
+    float lam[3];
+    for (int o = 0; o < 3; ++o) {
+      float a = q[0] * fd.wts[0][o];
+      a = a + q[1] * fd.wts[1][o];
+      a = a + q[2] * fd.wts[2][o];
+      lam[o] = a;
+    }
+    for (int o = 0; o < 3; ++o) q[o] = lam[o];
+  }
   const float* pp = dr.pos + (size_t)lt * 9;
   const float* nn = dr.nrm + (size_t)lt * 9;
   float wp[3] = {0.f, 0.f, 0.f}, n[3] = {0.f, 0.f, 0.f};
@@ -359,16 +616,6 @@ __device__ __forceinline__ void g_span(int64_t w, int64_t d, float inv_d, int i0
   else hi = min(hi, i0 + (int)floorf(t + m));
 }
 
-__device__ __forceinline__ GSetup g_setup_of(const GRec& r) {
-  GSetup s;
-  s.area = r.area;
-  s.swap = r.swap != 0u;
-  for (int v = 0; v < 3; ++v) {
-    s.X[v] = r.X[v];
-    s.Y[v] = r.Y[v];
-  }
-  return s;
-}
 
 // Early depth rejection: a covered pixel's depth is a convex combination of
 // the vertex depths, computed with b0 + b1 + b2 within 4 * 2^-23 of 1, so it
@@ -399,6 +646,8 @@ __device__ __forceinline__ void g_depth_test(unsigned long long* s_depth, int li
 }
 
 __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __restrict__ recs,
+                                                     const GRec* __restrict__ fan_recs,
+                                                     const GFan* __restrict__ fans,
                                                      const uint32_t* __restrict__ tile_count,
                                                      const uint32_t* __restrict__ tile_off,
                                                      const uint32_t* __restrict__ list, int tiles_x,
@@ -419,7 +668,8 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
     if (tid < kChunk) {
       uint32_t rows = 0;
       if (b0 + tid < n) {
-        const GRec r = recs[list[base + b0 + tid]];
+        const uint32_t e = list[base + b0 + tid];
+        const GRec r = e < p.ntri ? recs[e] : fan_recs[e - p.ntri];
         const int ci0 = max((int)r.i0, x0), ci1 = min((int)r.i1, x0 + kTile - 1);
         const int cj0 = max((int)r.j0, y0), cj1 = min((int)r.j1, y0 + kTile - 1);
         if (ci1 >= ci0 && cj1 >= cj0) {
@@ -475,7 +725,7 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
       const GRec& r = s_rec[lo];
       GSetup st;
       st.area = r.area;
-      st.swap = r.swap != 0u;
+      st.swap = (r.swap & 1u) != 0u;
       GEdges e;
       for (int k = 0; k < 3; ++k) {
         e.di[k] = g.di[k];
@@ -511,9 +761,12 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
   }
   for (int q = tid; q < kTile * kTile; q += 256) {
     const int i = x0 + (q % kTile), j = y0 + (q / kTile);
-    if (i < p.W && j < p.H) g_resolve_pixel(p, s_depth[q], i, j, gpos, gnrm);
+    if (i < p.W && j < p.H) g_resolve_pixel(p, recs, fan_recs, fans, s_depth[q], i, j, gpos, gnrm);
   }
 }
+
+// workgroups of the kernels that stride over the clipped triangles
+constexpr int kFanGrid = 64;
 
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
@@ -569,25 +822,36 @@ int vct_launch_gbuffer_phase(vct_context* ctx, int phase) {
 
   GParams p;
   memset(&p, 0, sizeof(p));
-  uint32_t ntri = 0;
+  uint64_t ntri = 0;
   mat4_mul(ctx->proj, ctx->view, p.PV);
+  // every queued model is one draw, in queue order (renderer.cpp:242-256)
+  std::vector<GDraw> draws;
+  draws.reserve(ctx->queue.size());
   for (uint32_t id : ctx->queue) {
     if (id >= ctx->models.size() || !ctx->models[id].ntri) continue;
-    if (p.ndraw == kMaxGDraws) return vct_set_error(VCT_ERR_LIMIT, "at most 16 queued models per frame");
     const VctModel& m = ctx->models[id];
-    GDraw& d = p.draws[p.ndraw++];
+    draws.emplace_back();
+    GDraw& d = draws.back();
+    memset(&d, 0, sizeof(d));
     d.pos = m.d_pos;
     d.nrm = m.d_nrm;
     d.mat = m.d_mat;
     d.perm = m.d_perm;
     d.inv = m.d_inv;
-    d.tri_base = ntri;
+    d.tri_base = (uint32_t)ntri;
     d.ntri = m.ntri;
     memcpy(d.M, m.xform, 64);
     normal_matrix3(m.xform, d.N);
     ntri += m.ntri;
+    // depth keys and tile-list entries hold triangle << 3 | fan triangle
+    if (ntri >= (1ull << 29)) return vct_set_error(VCT_ERR_LIMIT, "at most 2^29 triangles per frame");
   }
-  p.ntri = ntri;
+  p.ndraw = (int)draws.size();
+  if ((phase & 1) && p.ndraw) {
+    if ((rc = vct_upload(ctx, ctx->g_draws, draws.data(), draws.size() * sizeof(GDraw)))) return rc;
+  }
+  p.draws = (const GDraw*)ctx->g_draws.dev.p;
+  p.ntri = (uint32_t)ntri;
   p.W = W;
   p.H = H;
   const int tiles_x = (W + kTile - 1) / kTile, tiles_y = (H + kTile - 1) / kTile;
@@ -603,14 +867,21 @@ int vct_launch_gbuffer_phase(vct_context* ctx, int phase) {
                        2 * ntiles, 0u);
     VCT_HIP(hipGetLastError());
     ctx->h_readback[0] = 0u;
+    ctx->h_readback[1] = 0u;
     if (ntri) {
       if ((rc = vct_ensure(ctx->g_recs, (size_t)ntri * sizeof(GRec), ctx))) return rc;
+      if ((rc = vct_ensure(ctx->g_clip, (size_t)ntri * 4, ctx))) return rc;
+      hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, ctx->stream, counters + 5, 2u, 0u);
       hipLaunchKernelGGL(g_bin_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, p,
-                         (GRec*)ctx->g_recs.p, tile_count, tiles_x);
+                         (GRec*)ctx->g_recs.p, tile_count, tiles_x, (uint32_t*)ctx->g_clip.p, counters + 6);
+      hipLaunchKernelGGL(g_fan_count_kernel, dim3(kFanGrid), dim3(256), 0, ctx->stream, p, (GRec*)ctx->g_recs.p,
+                         (const uint32_t*)ctx->g_clip.p, (const uint32_t*)(counters + 6), tile_count, tiles_x,
+                         counters + 5);
       VCT_HIP(hipGetLastError());
       if ((rc = vct_launch_scan_u32(ctx, tile_count, tile_off, ntiles, counters + 4, &ctx->g_scan_tmp))) return rc;
-      // the list size is known on the device only
-      VCT_HIP(hipMemcpyAsync(ctx->h_readback, counters + 4, 4, hipMemcpyDeviceToHost, ctx->stream));
+      // the list size and the number of fan triangles of clipped triangles
+      // are known on the device only
+      VCT_HIP(hipMemcpyAsync(ctx->h_readback, counters + 4, 8, hipMemcpyDeviceToHost, ctx->stream));
     }
   }
   if (!(phase & 2)) return VCT_OK;
@@ -618,11 +889,19 @@ int vct_launch_gbuffer_phase(vct_context* ctx, int phase) {
   // in flight must not land after the word is read
   VCT_HIP(hipStreamSynchronize(ctx->stream));
   const uint32_t total = ctx->h_readback[0];
+  const uint32_t nfans = ctx->h_readback[1];
+  if ((rc = vct_ensure(ctx->g_fans, (size_t)nfans * (sizeof(GRec) + sizeof(GFan)) + 64, ctx))) return rc;
+  GRec* fan_recs = (GRec*)ctx->g_fans.p;
+  GFan* fans = (GFan*)((char*)ctx->g_fans.p + (size_t)nfans * sizeof(GRec));
   if (total) {
     if ((rc = vct_ensure(ctx->g_list, (size_t)total * 4 + (size_t)total / 4 * 4, ctx))) return rc;
-    hipLaunchKernelGGL(g_bin_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, ntri,
+    hipLaunchKernelGGL(g_bin_write_kernel, dim3((p.ntri + 255) / 256), dim3(256), 0, ctx->stream, p.ntri,
                        (const GRec*)ctx->g_recs.p, (const uint32_t*)tile_off, cursor, tiles_x,
                        (uint32_t*)ctx->g_list.p);
+    if (nfans)
+      hipLaunchKernelGGL(g_fan_write_kernel, dim3(kFanGrid), dim3(256), 0, ctx->stream, p, (const GRec*)ctx->g_recs.p,
+                         (const uint32_t*)ctx->g_clip.p, (const uint32_t*)(counters + 6), (const uint32_t*)tile_off,
+                         cursor, tiles_x, (uint32_t*)ctx->g_list.p, fan_recs, fans);
     VCT_HIP(hipGetLastError());
   } else {
     hipLaunchKernelGGL(fill_u32_kernel, dim3((ntiles + 255) / 256), dim3(256), 0, ctx->stream, tile_off, ntiles,
@@ -632,7 +911,7 @@ int vct_launch_gbuffer_phase(vct_context* ctx, int phase) {
   }
   if ((rc = vct_ensure(ctx->g_recs, 64, ctx))) return rc;
   hipLaunchKernelGGL(g_tile_kernel, dim3(ntiles), dim3(256), 0, ctx->stream, p, (const GRec*)ctx->g_recs.p,
-                     (const uint32_t*)tile_count, (const uint32_t*)tile_off, (const uint32_t*)ctx->g_list.p,
+                     (const GRec*)fan_recs, (const GFan*)fans, (const uint32_t*)tile_count, (const uint32_t*)tile_off, (const uint32_t*)ctx->g_list.p,
                      tiles_x, (float4*)ctx->g_pos.p, (float4*)ctx->g_nrm.p);
   VCT_HIP(hipGetLastError());
   ctx->last_gbuffer_refs = total;

@@ -182,13 +182,24 @@ struct VctBuf {
   size_t cap = 0;
 };
 
+// A per-frame table (the draws of a pass) copied to the device on a stream:
+// pinned host staging and the device copy.  The staging area is rewritten
+// only once the copy that last read it has completed (its event).
+struct VctUpload {
+  void* host = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;
+  VctBuf dev;
+};
+
 struct vct_context {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;          // second stream (default priority): the G-buffer beside voxelize + mip
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // pinned host words the list sizes are copied into: [0] G-buffer
-  // (triangle, tile) pairs, [1] voxelize (triangle, brick) pairs
+  // (triangle, tile) pairs, [1] G-buffer fan triangles of clipped triangles,
+  // [2] voxelize (triangle, brick) pairs
   uint32_t* h_readback = nullptr;
   int width = 0, height = 0;
   unsigned res = 128;
@@ -230,6 +241,10 @@ struct vct_context {
   // G-buffer + image
   VctBuf g_list, g_pos, g_nrm, image;
   VctBuf g_count, g_off, g_recs;  // per-tile counts + cursors, per-tile offsets, per-triangle records
+  VctUpload g_draws;               // the G-buffer's draw table
+  VctUpload v_draws;               // voxelize's draw table
+  VctBuf g_fans;                   // records and resolve data of clipped triangles' fan triangles
+  VctBuf g_clip;                   // the frame's triangles that need clipping
   uint64_t last_gbuffer_refs = 0;  // (triangle, tile) pairs of the last G-buffer
 
   std::vector<VctTex3D> textures;
@@ -240,6 +255,7 @@ struct vct_context {
   int mlp_step = 0;
   VctBuf low_gpos, low_gnrm, low_img, full_target, sample_pix;
   VctBuf train_X, train_A, train_Y, train_valid;
+  VctBuf train_perm, batch_X, batch_A, batch_Y;  // minibatch order and the gathered minibatch
   size_t train_n = 0, train_cap = 0;
   // per-tile fallback of the learned frame
   VctBuf probe_pix, probe_val, fb_list;  // fb_list: [count][tiles...]
@@ -262,6 +278,9 @@ struct vct_context {
 int vct_set_error(int code, const std::string& msg);
 int vct_hip_check(hipError_t e, const char* what);
 int vct_ensure(VctBuf& b, size_t bytes, vct_context* ctx);
+// Copies bytes of a host table to u.dev on ctx->stream (asynchronously).
+int vct_upload(vct_context* ctx, VctUpload& u, const void* data, size_t bytes);
+void vct_upload_free(VctUpload& u);
 #define VCT_HIP(call)                                  \
   do {                                                 \
     hipError_t _e = (call);                            \
@@ -305,4 +324,6 @@ int vct_trace_into(vct_context* ctx, const float4* gpos, const float4* gnrm, flo
 int vct_launch_scan_u32(vct_context* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
                         uint32_t* total_dev, VctBuf* tmp = nullptr);
 int vct_tex3d_mip_launch(vct_context* ctx, VctTex3D& t);
+// Device address of the no-allocation entry point's staging array, taken once
+// per device (the current one) at context creation.
 int vct_tex3d_clear_launch(vct_context* ctx, VctTex3D& t, const float rgba[4]);

@@ -1265,6 +1265,43 @@ int refine(vct_context* ctx, bool selected) {
   return VCT_OK;
 }
 
+// Minibatch k of a training run: rows perm[(k*batch + i) mod n] of the
+// sample set, gathered into contiguous batch buffers.  Eight threads per row
+// move its 128-byte BF16 feature row as 16-byte pieces; the first of them
+// also moves albedo and target.
+__global__ __launch_bounds__(256) void mlp_batch_gather_kernel(const uint32_t* __restrict__ perm, uint32_t n,
+                                                               uint64_t first, uint32_t batch,
+                                                               const uint16_t* __restrict__ X,
+                                                               const float* __restrict__ A,
+                                                               const float* __restrict__ Y, uint16_t* __restrict__ bx,
+                                                               float* __restrict__ ba, float* __restrict__ by) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = t >> 3, c = t & 7;
+  if (i >= batch) return;
+  const uint32_t r = perm[(first + i) % n];
+  reinterpret_cast<uint4*>(bx + (size_t)i * 64)[c] = reinterpret_cast<const uint4*>(X + (size_t)r * 64)[c];
+  if (c == 0) {
+    for (int q = 0; q < 3; ++q) {
+      ba[(size_t)i * 3 + q] = A[(size_t)r * 3 + q];
+      by[(size_t)i * 3 + q] = Y[(size_t)r * 3 + q];
+    }
+  }
+}
+
+// A uniformly random order of the n training rows (Fisher-Yates, the
+// library's splitmix64 stream of `seed`), on the device in ctx->train_perm.
+int shuffle_rows(vct_context* ctx, uint32_t n, uint64_t seed) {
+  std::vector<uint32_t> perm(n);
+  for (uint32_t i = 0; i < n; ++i) perm[i] = i;
+  HostRng rng(seed);
+  for (uint32_t i = n - 1; i > 0; --i) std::swap(perm[i], perm[rng.next() % ((uint64_t)i + 1)]);
+  int rc;
+  if ((rc = vct_ensure(ctx->train_perm, (size_t)n * 4, ctx))) return rc;
+  VCT_HIP(hipMemcpyAsync(ctx->train_perm.p, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+  VCT_HIP(hipStreamSynchronize(ctx->stream));  // `perm` is pageable and goes out of scope
+  return VCT_OK;
+}
+
 }  // namespace
 
 #define MLP_CTX(ctx)                                                          \
@@ -1446,14 +1483,24 @@ int vct_upsample_train(vct_ctx ctx, uint32_t steps, uint32_t batch, float lr, ui
   if ((rc = need_mlp(ctx))) return rc;
   if (ctx->train_n < 64) return vct_set_error(VCT_ERR_STATE, "no training samples (vct_upsample_collect first)");
   if (batch < 64 || !(lr > 0.0f)) return vct_set_error(VCT_ERR_INVALID_ARG, "batch >= 64 and lr > 0 required");
-  batch = (uint32_t)std::min<size_t>(batch, ctx->train_n) & ~63u;
-  HostRng rng(seed);
+  if (ctx->train_n > 0xffffffffull) return vct_set_error(VCT_ERR_LIMIT, "more than 2^32 - 1 training samples");
+  const uint32_t n = (uint32_t)ctx->train_n;
+  batch = std::min<uint32_t>(batch, n) & ~63u;
+  // Minibatches walk one random order of the whole sample set (samples are
+  // stored frame by frame, so a contiguous window would hold one camera pose).
+  if ((rc = shuffle_rows(ctx, n, seed))) return rc;
+  if ((rc = vct_ensure(ctx->batch_X, (size_t)batch * 128, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->batch_A, (size_t)batch * 12, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->batch_Y, (size_t)batch * 12, ctx))) return rc;
   for (uint32_t s = 0; s < steps; ++s) {
-    const size_t span = ctx->train_n - batch + 1;
-    const size_t off = (size_t)(rng.next() % span);
+    hipLaunchKernelGGL(mlp_batch_gather_kernel, dim3((batch * 8 + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const uint32_t*)ctx->train_perm.p, n, (uint64_t)s * batch, batch,
+                       (const uint16_t*)ctx->train_X.p, (const float*)ctx->train_A.p, (const float*)ctx->train_Y.p,
+                       (uint16_t*)ctx->batch_X.p, (float*)ctx->batch_A.p, (float*)ctx->batch_Y.p);
+    VCT_HIP(hipGetLastError());
     const bool last = s + 1 == steps;
-    if ((rc = mlp_train_on(ctx, (const uint16_t*)ctx->train_X.p + off * 64, (const float*)ctx->train_A.p + off * 3,
-                           (const float*)ctx->train_Y.p + off * 3, batch, lr, last ? last_loss : nullptr, nullptr)))
+    if ((rc = mlp_train_on(ctx, (const uint16_t*)ctx->batch_X.p, (const float*)ctx->batch_A.p,
+                           (const float*)ctx->batch_Y.p, batch, lr, last ? last_loss : nullptr, nullptr)))
       return rc;
   }
   return VCT_OK;

@@ -37,13 +37,20 @@ material_data_t default_material() {
 
 }  // namespace
 
+// A table entry of material `id` with parameters d.
+static material_t material_entry(size_t id, const material_data_t& d) {
+  material_t mt;
+  memset(&mt, 0, sizeof mt);
+  mt.ubo = (GLuint)id;
+  mt.num_textures = 0;
+  mt.data = d;
+  return mt;
+}
+
 Renderer::Renderer(int width, int height) : m_width(width), m_height(height) {
   int rc = vct_create(width, height, &m_ctx);
   if (rc != VCT_OK) throw std::runtime_error(std::string("Renderer: ") + vct_last_error());
-  material_t def;
-  def.data = default_material();
-  def.num_textures = 0;
-  m_materials.push_back(def);  // material 0, as in renderer.cpp:115-128
+  m_materials.push_back(material_entry(0, default_material()));  // material 0, as in renderer.cpp:115-128
   t_current = m_ctx;
   set_grid_resolution(128);  // renderer.cpp:146
 }
@@ -61,10 +68,9 @@ model_id_t Renderer::load_model(const char* filename) {
   // mirror the materials the loader appended
   vct_material m;
   for (size_t i = m_materials.size(); vct_get_material(m_ctx, (uint32_t)i, &m) == VCT_OK; ++i) {
-    material_t mt;
-    memcpy(&mt.data, &m, sizeof m);
-    mt.num_textures = 0;
-    m_materials.push_back(mt);
+    material_data_t d;
+    memcpy(&d, &m, sizeof m);
+    m_materials.push_back(material_entry(i, d));
   }
   return id;
 }
@@ -75,10 +81,7 @@ material_id_t Renderer::add_material(material_data_t& material_data) {
   vct_material m = to_c(material_data);
   uint32_t id = 0;
   report("add_material", vct_add_material(m_ctx, &m, &id));
-  material_t mt;
-  mt.data = material_data;
-  mt.num_textures = 0;
-  m_materials.push_back(mt);
+  m_materials.push_back(material_entry(id, material_data));
   return id;
 }
 

@@ -54,6 +54,37 @@ static void free_buf(VctBuf& b) {
   b.cap = 0;
 }
 
+int vct_upload(vct_context* ctx, VctUpload& u, const void* data, size_t bytes) {
+  if (!u.ev) VCT_HIP(hipEventCreateWithFlags(&u.ev, hipEventDisableTiming));
+  else VCT_HIP(hipEventSynchronize(u.ev));  // the last copy out of the staging area has completed
+  if (bytes > u.cap) {
+    if (u.host) VCT_HIP(hipHostFree(u.host));
+    u.host = nullptr;
+    u.cap = 0;
+    const size_t want = bytes < 4096 ? 4096 : bytes;
+    VCT_HIP(hipHostMalloc(&u.host, want, hipHostMallocDefault));
+    u.cap = want;
+  }
+  int rc = vct_ensure(u.dev, bytes, ctx);
+  if (rc) return rc;
+  memcpy(u.host, data, bytes);
+  VCT_HIP(hipMemcpyAsync(u.dev.p, u.host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  VCT_HIP(hipEventRecord(u.ev, ctx->stream));
+  return VCT_OK;
+}
+
+void vct_upload_free(VctUpload& u) {
+  if (u.ev) {
+    (void)hipEventSynchronize(u.ev);
+    (void)hipEventDestroy(u.ev);
+  }
+  if (u.host) (void)hipHostFree(u.host);
+  free_buf(u.dev);
+  u.ev = nullptr;
+  u.host = nullptr;
+  u.cap = 0;
+}
+
 // ---- scan (exclusive, u32) --------------------------------------------------
 // Three-phase scan: per-block reduce, scan of block sums (recursive), per-block
 // scan plus block offset.  1024 elements per 256-thread block.
@@ -398,8 +429,11 @@ int vct_destroy(vct_ctx ctx) {
                     &ctx->full_target, &ctx->sample_pix, &ctx->train_X, &ctx->train_A, &ctx->train_Y,
                     &ctx->train_valid, &ctx->probe_pix, &ctx->probe_val, &ctx->fb_list, &ctx->fb_pix,
                     &ctx->fb_list2, &ctx->fb_pix2, &ctx->fb_flag, &ctx->half_img, &ctx->probe_pix2,
-                    &ctx->probe_val2, &ctx->pairs, &ctx->fb_est};
+                    &ctx->probe_val2, &ctx->pairs, &ctx->fb_est, &ctx->g_fans, &ctx->g_clip,
+                    &ctx->train_perm, &ctx->batch_X, &ctx->batch_A, &ctx->batch_Y};
   for (VctBuf* b : bufs) free_buf(*b);
+  vct_upload_free(ctx->g_draws);
+  vct_upload_free(ctx->v_draws);
   for (int i = 0; i < 12; ++i)
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
   if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);

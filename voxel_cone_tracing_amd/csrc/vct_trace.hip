@@ -453,59 +453,71 @@ struct ShadeLds {
   float spec_ap[SH::NPIX], eta[SH::NPIX];
   unsigned long long refr_mask[SH::NSUB];  // shading pixels with a transmissive material, per sub-block
   int sub_count[SH::NSUB];                 // shading pixels per sub-block
+  // each set-up thread's G-buffer index (~0: none) and output index, kept
+  // here over the cone queue instead of in registers
+  uint32_t gidx[SH::NPIX], oidx[SH::NPIX];
   int npx, njobs, next_item;
   uint8_t px_of[SH::NPIX];  // the block's shading pixels, compacted
   int8_t jobs[kMaxJobs];
 };
 
 // The march state of one lane's cone (trace_cone, voxel_cone_tracing.frag:88-119).
+// Registers are what bounds the marcher's occupancy, so only what cannot be
+// derived is kept: |dir| and the three maps a sample reads follow from dir
+// (abs is a free operand modifier), the diameter is dist * aperture, and the
+// origin is the pixel's surface point, read from LDS where it is needed.
 struct Cone {
-  float3 origin, dir, ad;
-  int mx, my, mz;
-  float ap, max_dist, dist, diam;
+  float3 dir;
+  float ap, max_dist, dist;
   float4 c;
 };
 
-__device__ __forceinline__ void cone_begin(Cone& k, float3 origin, float3 dir, float aperture, float max_dist,
-                                           float vs) {
+// The origin of the cone of block pixel `pix`, read from LDS where it is
+// used.  A volatile read (forcing a re-read at every step) measured 0.11 ms
+// slower on the 2560x1440 frame than letting the compiler schedule the reads.
+template <int NPIX>
+__device__ __forceinline__ float3 cone_origin(const float (*pos)[NPIX], int pix) {
+  const float* p = &pos[0][0];
+  return make_float3(p[pix], p[NPIX + pix], p[2 * NPIX + pix]);
+}
+
+__device__ __forceinline__ void cone_begin(Cone& k, float3 dir, float aperture, float max_dist, float vs) {
   dir = normalize3(dir);
-  k.origin = origin;
   k.dir = dir;
-  k.mx = dir.x < 0.0f ? 0 : 1;
-  k.my = dir.y < 0.0f ? 2 : 3;
-  k.mz = dir.z < 0.0f ? 4 : 5;
-  k.ad = make_float3(fabsf(dir.x), fabsf(dir.y), fabsf(dir.z));
   // A cone never needs to leave the volume by more than its diagonal; the
   // bound also keeps a non-finite target distance from looping forever.
   k.max_dist = fminf(max_dist, 3.4641016f);
   k.ap = aperture;
   k.c = f4(0.0f);
   k.dist = 3.0f * vs;
-  k.diam = k.dist * aperture;
 }
 
 // The loop condition of trace_cone, plus the exact early exit: a sample more
 // than half a texel of the coarsest level outside the unit cube reads only
 // the zero border, so once the cone is that far out and moving away on that
 // axis every later sample is zero and c no longer changes.
-__device__ __forceinline__ bool cone_alive(const Cone& k, float margin) {
+__device__ __forceinline__ bool cone_alive(const Cone& k, float3 origin, float margin) {
   if (!(k.c.w < 1.0f && k.dist < k.max_dist)) return false;
-  const float3 sp = make_float3(k.dir.x * k.dist + k.origin.x, k.dir.y * k.dist + k.origin.y,
-                                k.dir.z * k.dist + k.origin.z);
+  const float3 sp = make_float3(k.dir.x * k.dist + origin.x, k.dir.y * k.dist + origin.y,
+                                k.dir.z * k.dist + origin.z);
   return !((sp.x < -margin && k.dir.x <= 0.0f) || (sp.x >= 1.0f + margin && k.dir.x >= 0.0f) ||
            (sp.y < -margin && k.dir.y <= 0.0f) || (sp.y >= 1.0f + margin && k.dir.y >= 0.0f) ||
            (sp.z < -margin && k.dir.z <= 0.0f) || (sp.z >= 1.0f + margin && k.dir.z >= 0.0f));
 }
 
 template <int DT>
-__device__ __forceinline__ void cone_step(const TraceArgs& A, const SamplerLds* S, Cone& k, float vs, float fres) {
-  const float3 sp = make_float3(k.dir.x * k.dist + k.origin.x, k.dir.y * k.dist + k.origin.y,
-                                k.dir.z * k.dist + k.origin.z);
-  const float mip = fmaxf(log2f(k.diam * fres), 0.0f);
-  const float4 s = sample_voxel<DT>(A, S, sp, k.ad, k.mx, k.my, k.mz, mip);
+__device__ __forceinline__ void cone_step(const TraceArgs& A, const SamplerLds* S, Cone& k, float3 origin, float vs,
+                                          float fres) {
+  const float3 sp = make_float3(k.dir.x * k.dist + origin.x, k.dir.y * k.dist + origin.y,
+                                k.dir.z * k.dist + origin.z);
+  const float diam = k.dist * k.ap;
+  const float mip = fmaxf(log2f(diam * fres), 0.0f);
+  // face_indices (voxel_cone_tracing.frag:71-78): the map of each axis by the sign of dir
+  const float3 ad = make_float3(fabsf(k.dir.x), fabsf(k.dir.y), fabsf(k.dir.z));
+  const float4 s = sample_voxel<DT>(A, S, sp, ad, k.dir.x < 0.0f ? 0 : 1, k.dir.y < 0.0f ? 2 : 3,
+                                    k.dir.z < 0.0f ? 4 : 5, mip);
   k.c = k.c + (1.0f - k.c.w) * s;
-  k.dist += fmaxf(k.diam * 0.5f, vs);
-  k.diam = k.dist * k.ap;
+  k.dist += fmaxf(diam * 0.5f, vs);
 }
 
 // Direction of light li seen from pos (biased coordinates) and the distance
@@ -532,10 +544,11 @@ __device__ __forceinline__ float3 ld3(const float (*a)[N], int p) { return make_
 template <int DT, class SH>
 __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, size_t oidx, bool valid,
                                             ShadeLds<SH>* L, const SamplerLds* S) {
-  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int grp = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave-uniform: a scalar register
   const int me = grp * 64 + lane;  // the thread's pixel in the block (set-up and composition waves)
   valid = valid && px < A.W && py < A.H;
-  const size_t idx = valid ? (size_t)py * A.W + px : 0;
+  size_t idx = valid ? (size_t)py * A.W + px : 0;
   const float sqrt3 = 1.73205080757f;
   const int nd = A.en_diffuse ? A.cones.n : 0;
   const bool do_spec = A.en_specular != 0;
@@ -559,6 +572,8 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
   unsigned long long shade = 0ull;
   int smode = 0;
   if (grp < SH::NSUB) {
+    L->gidx[me] = valid ? (uint32_t)idx : ~0u;
+    L->oidx[me] = (uint32_t)oidx;
     float4 gp, gn;
     const DevMaterial* m;
     float3 pos;
@@ -626,11 +641,12 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
   const int nitems = npx * L->njobs;
   // it / npx as (it * ceil(2^20 / npx)) >> 20: exact for it < kMaxJobs * 128 and npx <= 128
   const uint32_t npx_inv = npx ? (1u << 20) / (uint32_t)npx + (((1u << 20) % (uint32_t)npx) ? 1u : 0u) : 0u;
-  uint32_t steps = 0;
-  int item = -1, job = 0, pix = 0;
+  // the lane's cone: job << 8 | pixel while it marches, -1 while it idles
+  int cur = -1;
+  unsigned long long wave_steps = 0;  // cone steps of the wave (uniform)
   bool drained = nitems == 0;
   Cone k;
-  auto finish = [&]() {
+  auto finish = [&](int job, int pix) {
     if (job >= kJobShadow) {
       L->shadow[job - kJobShadow][pix] = fmaxf(0.0f, 1.0f - k.c.w);
     } else {
@@ -649,9 +665,9 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
   for (;;) {
     // lanes without a cone take the next items of the queue, once enough of
     // them idle (or none is busy): a refill costs the whole wave its set-up
-    const unsigned long long busy = __ballot(item >= 0);
+    const unsigned long long busy = __ballot(cur >= 0);
     for (;;) {
-      const bool need = item < 0 && !drained;
+      const bool need = cur < 0 && !drained;
       const unsigned long long want = __ballot(need);
       if (!want || (busy && __popcll(want) < kRefillMin)) break;
       const int first = __ffsll((long long)want) - 1;
@@ -665,8 +681,8 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
         continue;
       }
       const int j = (int)(((uint32_t)it * npx_inv) >> 20);
-      job = L->jobs[j];
-      pix = L->px_of[it - j * npx];
+      const int job = L->jobs[j];
+      const int pix = L->px_of[it - j * npx];
       if (job == kJobRefraction && !((L->refr_mask[pix >> 6] >> (pix & 63)) & 1ull)) continue;  // opaque: no item
       const float3 pos = ld3(L->pos, pix), n = ld3(L->nrm, pix);
       float3 d;
@@ -690,24 +706,34 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
         d = light_dir(A, job - kJobShadow, pos, md);  // shadow cone of direct_light
         ap = 0.1f;
       }
-      cone_begin(k, pos, d, ap, md, vs);
-      if (cone_alive(k, margin)) item = it;
-      else finish();  // no step at all
+      cone_begin(k, d, ap, md, vs);
+      if (cone_alive(k, pos, margin)) cur = (job << 8) | pix;
+      else finish(job, pix);  // no step at all
     }
-    if (!__ballot(item >= 0)) break;
-    if (item >= 0) {
-      cone_step<DT>(A, S, k, vs, fres);
-      ++steps;
-      if (!cone_alive(k, margin)) {
-        finish();
-        item = -1;
+    const unsigned long long active = __ballot(cur >= 0);
+    if (!active) break;
+    wave_steps += (unsigned long long)__popcll(active);
+    if (cur >= 0) {
+      cone_step<DT>(A, S, k, cone_origin(L->pos, cur & 255), vs, fres);
+      if (!cone_alive(k, cone_origin(L->pos, cur & 255), margin)) {
+        finish(cur >> 8, cur & 255);
+        cur = -1;
       }
     }
   }
-  if (A.step_counter && steps) atomicAdd(A.step_counter, (unsigned long long)steps);
+  if (A.step_counter && lane == 0 && wave_steps) atomicAdd(A.step_counter, wave_steps);
   __syncthreads();
   // ---- composition (waves 0..NSUB-1, lane = pixel) -----------------------------------
   if (grp >= SH::NSUB) return;
+  // the thread's slot again, from the scalar wave index and the hardware lane
+  // id: nothing per lane is held over the cone queue
+  const int lane_c = (int)__lane_id(), me_c = grp * 64 + lane_c;
+  {
+    const uint32_t g = ((const volatile uint32_t*)L->gidx)[me_c];
+    valid = g != ~0u;
+    idx = valid ? (size_t)g : 0;
+    oidx = ((const volatile uint32_t*)L->oidx)[me_c];
+  }
   float4 gp, gn;
   const DevMaterial* mp;
   float3 pos;
@@ -734,15 +760,15 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
   }
   const DevMaterial& m = *mp;
   const float3 n = make_float3(gn.x, gn.y, gn.z);
-  const float3 view_dir = ld3(L->vdir, me);
-  const bool do_refr = (L->refr_mask[grp] >> lane) & 1ull;
+  const float3 view_dir = ld3(L->vdir, me_c);
+  const bool do_refr = (L->refr_mask[grp] >> lane_c) & 1ull;
   float3 out = make_float3(0.f, 0.f, 0.f);
   if (nd > 0) {
     // the exact sum of the diffuse cones, rounded once (trace_diffuse adds
     // them in cone order in FP32, :144-167)
     const float u = 1.0f / 4294967296.0f;
-    const float3 diff_acc = make_float3(__ull2float_rn(L->dsum[0][me]) * u, __ull2float_rn(L->dsum[1][me]) * u,
-                                        __ull2float_rn(L->dsum[2][me]) * u);
+    const float3 diff_acc = make_float3(__ull2float_rn(L->dsum[0][me_c]) * u, __ull2float_rn(L->dsum[1][me_c]) * u,
+                                        __ull2float_rn(L->dsum[2][me_c]) * u);
     const float inv = 1.0f / (float)nd;
     out.x += m.diffuse[0] * (diff_acc.x * inv);
     out.y += m.diffuse[1] * (diff_acc.y * inv);
@@ -759,7 +785,7 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
       const float kk = att * cos_surf;
       const float3 lc = make_float3(kk * Lt.color[0] * Lt.intensity, kk * Lt.color[1] * Lt.intensity,
                                     kk * Lt.color[2] * Lt.intensity);
-      const float shadow = A.en_shadow ? L->shadow[li][me] : 1.0f;
+      const float shadow = A.en_shadow ? L->shadow[li][me_c] : 1.0f;
       const float lambert = fmaxf(dot3(ldir, n), 0.0f);
       float refr_angle = 0.0f;
       if (m.dissolve <= 0.1f) {
@@ -780,14 +806,14 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
     out.z += res.z + fminf(fmaxf(m.emission[2], 0.0f), 1.0f);
   }
   if (do_spec) {
-    out.x += m.specular[0] * L->cone[0][0][me];
-    out.y += m.specular[1] * L->cone[0][1][me];
-    out.z += m.specular[2] * L->cone[0][2][me];
+    out.x += m.specular[0] * L->cone[0][0][me_c];
+    out.y += m.specular[1] * L->cone[0][1][me_c];
+    out.z += m.specular[2] * L->cone[0][2][me_c];
   }
   if (do_refr) {  // final colour override (:273-274)
-    const float3 tr = make_float3(m.transmittance[0] * L->cone[1][0][me],
-                                  m.transmittance[1] * L->cone[1][1][me],
-                                  m.transmittance[2] * L->cone[1][2][me]);
+    const float3 tr = make_float3(m.transmittance[0] * L->cone[1][0][me_c],
+                                  m.transmittance[1] * L->cone[1][1][me_c],
+                                  m.transmittance[2] * L->cone[1][2][me_c]);
     const float dv = m.dissolve;
     out = make_float3(tr.x * (1.0f - dv) + out.x * dv, tr.y * (1.0f - dv) + out.y * dv,
                       tr.z * (1.0f - dv) + out.z * dv);
@@ -935,9 +961,79 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs P) {
   }
 }
 
+// Ceiling of the marcher's texel loads: the rate at which the vector memory
+// path delivers 16-byte-per-lane loads (global_load_dwordx4, the form of the
+// marcher's paired-texel gathers) when every access hits the L1 cache.  All
+// workgroups read one 16 KiB region (L1 holds 32 KiB), each wave's lanes
+// 1 KiB of consecutive addresses per load (8 full cache lines: the fewest
+// lines a 64 x 16-byte load can touch), 8 loads of distinct rows in flight
+// per lane; the row advances by a run-time stride, so no load can be merged
+// with another or moved out of the loop.  A gather touches more lines per
+// load, so no access pattern of these loads is delivered faster.
+__global__ __launch_bounds__(256) void load_peak_kernel(const uint4* __restrict__ src, uint32_t iters,
+                                                        uint32_t stride, uint4* __restrict__ sink) {
+  uint4 acc[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) acc[u] = make_uint4(0u, 0u, 0u, 0u);
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  for (uint32_t i = 0; i < iters; i += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t row = (wave + i * stride + (uint32_t)u) & 15u;  // the 16 KiB region in 1 KiB rows
+      const uint4 v = src[row * 64u + lane];
+      acc[u].x ^= v.x;
+      acc[u].y ^= v.y;
+      acc[u].z ^= v.z;
+      acc[u].w ^= v.w;
+    }
+  }
+  uint4 r = acc[0];
+#pragma unroll
+  for (int u = 1; u < 8; ++u) {
+    r.x ^= acc[u].x;
+    r.y ^= acc[u].y;
+    r.z ^= acc[u].z;
+    r.w ^= acc[u].w;
+  }
+  sink[(size_t)blockIdx.x * 256u + threadIdx.x] = r;  // keeps every load live
+}
+
 }  // namespace vct
 
 using namespace vct;
+
+extern "C" int vct_trace_load_peak(vct_ctx ctx, double* gbps) {
+  if (!ctx) return vct_set_error(VCT_ERR_BAD_HANDLE, "null context");
+  if (!gbps) return vct_set_error(VCT_ERR_INVALID_ARG, "null out");
+  VCT_HIP(hipSetDevice(ctx->device));
+  int cus = 0;
+  VCT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  const uint32_t blocks = (uint32_t)std::max(cus, 1) * 8u, iters = 4096;
+  int rc;
+  VctBuf src, sink;
+  if ((rc = vct_ensure(src, 16384, ctx))) return rc;
+  if ((rc = vct_ensure(sink, (size_t)blocks * 256 * 16, ctx))) return rc;
+  VCT_HIP(hipMemsetAsync(src.p, 0x5a, 16384, ctx->stream));
+  float best = 0.0f;
+  for (int rep = 0; rep < 4; ++rep) {  // the first launch also warms the caches
+    VCT_HIP(hipEventRecord(ctx->ev[10], ctx->stream));
+    hipLaunchKernelGGL(load_peak_kernel, dim3(blocks), dim3(256), 0, ctx->stream, (const uint4*)src.p, iters, 1u,
+                       (uint4*)sink.p);
+    VCT_HIP(hipGetLastError());
+    VCT_HIP(hipEventRecord(ctx->ev[11], ctx->stream));
+    VCT_HIP(hipEventSynchronize(ctx->ev[11]));
+    float ms = 0.0f;
+    VCT_HIP(hipEventElapsedTime(&ms, ctx->ev[10], ctx->ev[11]));
+    const double bytes = (double)blocks * 256.0 * iters * 16.0;
+    const float g = (float)(bytes / (ms * 1e-3) / 1e9);
+    if (rep > 0 && g > best) best = g;
+  }
+  VCT_HIP(hipStreamSynchronize(ctx->stream));
+  (void)hipFree(sink.p);
+  (void)hipFree(src.p);
+  *gbps = best;
+  return VCT_OK;
+}
 
 // Cone direction tables in the tangent frame (n, o1, o2).
 void vct_cone_set_table(int set, ConeSet& cs) {

@@ -48,10 +48,10 @@ struct Draw {
   float N[9];   // mat3(transpose(inverse(model))), column-major
 };
 
-constexpr int kMaxDraws = 16;
-
-struct DrawBatch {
-  Draw d[kMaxDraws];
+// The queued draws of a frame, in queue order, in device memory (no limit on
+// their number).
+struct DrawTable {
+  const Draw* d;
   int ndraw;
   uint32_t ntri;
 };
@@ -93,11 +93,15 @@ struct alignas(16) TriRec {
 };
 static_assert(sizeof(TriRec) == 128, "TriRec must be 128 bytes");
 
-__device__ __forceinline__ int find_draw(const DrawBatch& b, uint32_t t) {
-  int d = 0;
-  for (int i = 1; i < b.ndraw; ++i)
-    if (t >= b.d[i].tri_base) d = i;
-  return d;
+// The draw holding stored triangle t: the last one with tri_base <= t.
+__device__ __forceinline__ int find_draw(const DrawTable& b, uint32_t t) {
+  int lo = 0, hi = b.ndraw - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (b.d[mid].tri_base <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
 }
 
 // voxelize.vert:26 + voxelize.geom:27-51 + viewport + snap.
@@ -322,8 +326,6 @@ struct RefPass {
   float4* s_color;               // colour per slot
   uint32_t nfrag;                // slots
   uint32_t nvox;                 // voxels of level 0
-  uint32_t replay;               // place pass: replay the brick's voxels into level 0 afterwards
-  uint32_t carry;                // place pass: earlier batches placed fragments, counted in vcount
 };
 
 // ---- exact mean: bricks of the grid accumulated in LDS ----------------------
@@ -405,7 +407,7 @@ __device__ __forceinline__ uint32_t brick_of(const BrickBox& bb, int k) {
 }
 
 // 1. setup (lane per stored triangle): record + count into overlapped bricks.
-__global__ __launch_bounds__(256) void vox_brick_count_kernel(DrawBatch b, VoxParams vp, TriRec* __restrict__ recs,
+__global__ __launch_bounds__(256) void vox_brick_count_kernel(DrawTable b, VoxParams vp, TriRec* __restrict__ recs,
                                                               uint32_t* brick_count) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const BrickGeom g = brick_geom(vp.res);
@@ -575,10 +577,9 @@ __device__ __forceinline__ uint32_t replay_voxel(unsigned long long* __restrict_
 // reference running average takes two passes over the same candidates:
 // kRefCount counts the fragments per voxel (into r.vcount), and after the
 // counts are scanned into r.vox_off, kRefPlace shades them and stores
-// (canonical key, colour) in the voxel's slots, in no particular order.  With
-// r.replay (the last batch of draws, when every earlier fragment is placed)
-// the workgroup then replays its brick's voxels and writes their texels.  No
-// global counter is shared between workgroups.
+// (canonical key, colour) in the voxel's slots, in no particular order, and
+// then replays its brick's voxels and writes their texels.  No global counter
+// is shared between workgroups.
 enum { kExactMean = 0, kRefCount = 1, kRefPlace = 2 };
 template <int DT, int kMode>
 __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, const TriRec* __restrict__ recs,
@@ -618,7 +619,7 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
       s_sum[3][v] = 0u;
     }
     if constexpr (kMode == kRefPlace)
-      s_base[v] = r.vox_off[brick * nvox + v] + (r.carry ? r.vcount[brick * nvox + v] : 0u);
+      s_base[v] = r.vox_off[brick * nvox + v];
     s_cnt[v] = 0u;
   }
   const uint32_t n = brick_count[brick], base = brick_off[brick];
@@ -721,10 +722,9 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
   if constexpr (kMode == kExactMean) {
     for (uint32_t v = tid; v < nvox; v += kBrickThreads)
       store_mean<DT>(level0, brick * nvox + v, s_cnt[v], s_sum[0][v], s_sum[1][v], s_sum[2][v], s_sum[3][v]);
-  } else if (kMode == kRefCount || !r.replay) {
-    // one workgroup per brick per launch: a plain read-modify-write suffices
-    for (uint32_t v = tid; v < nvox; v += kBrickThreads)
-      if (s_cnt[v]) r.vcount[brick * nvox + v] += s_cnt[v];
+  } else if (kMode == kRefCount) {
+    // one workgroup per brick: it writes the count of every voxel it owns
+    for (uint32_t v = tid; v < nvox; v += kBrickThreads) r.vcount[brick * nvox + v] = s_cnt[v];
   } else {
     // the barrier makes the workgroup's slot stores visible to all its lanes
     // (the loop above ended with one)
@@ -786,12 +786,17 @@ static LightSet make_lights(const vct_context* ctx) {
   return ls;
 }
 
-static void fill_batch(vct_context* ctx, const std::vector<uint32_t>& q, size_t qi, uint32_t& canon, DrawBatch& b) {
+// Every queued draw into the device draw table (through pinned staging, in
+// stream order with the kernels that read the previous frame's table).
+static int fill_draws(vct_context* ctx, const std::vector<uint32_t>& q, DrawTable& b, bool upload = true) {
   memset(&b, 0, sizeof(b));
-  uint32_t ntri = 0;
-  for (size_t k = qi; k < q.size() && k < qi + kMaxDraws; ++k) {
+  if (q.size() > (size_t)0x7fffffff) return vct_set_error(VCT_ERR_LIMIT, "too many queued models");
+  std::vector<Draw> draws(q.size());
+  uint64_t ntri = 0;
+  uint32_t canon = 0;
+  for (size_t k = 0; k < q.size(); ++k) {
     const VctModel& m = ctx->models[q[k]];
-    Draw& d = b.d[b.ndraw++];
+    Draw& d = draws[k];
     d.pos = m.d_pos;
     d.nrm = m.d_nrm;
     d.mat = m.d_mat;
@@ -804,7 +809,14 @@ static void fill_batch(vct_context* ctx, const std::vector<uint32_t>& q, size_t 
     normal_matrix(m.xform, d.N);
     ntri += m.ntri;
   }
-  b.ntri = ntri;
+  if (ntri > (1ull << 31)) return vct_set_error(VCT_ERR_LIMIT, "more than 2^31 triangles queued");
+  b.ndraw = (int)draws.size();
+  b.ntri = (uint32_t)ntri;
+  int rc;
+  if (upload && b.ndraw && (rc = vct_upload(ctx, ctx->v_draws, draws.data(), draws.size() * sizeof(Draw))))
+    return rc;
+  b.d = (const Draw*)ctx->v_draws.dev.p;
+  return VCT_OK;
 }
 
 template <int DT, int kMode = kExactMean>
@@ -832,30 +844,28 @@ static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, cons
   uint32_t* bcount = (uint32_t*)ctx->vox_count.p;
   uint32_t* cursor = bcount + nbricks;
   uint32_t* boff = (uint32_t*)ctx->vox_off.p;
-  if (q.size() > (size_t)kMaxDraws) return vct_set_error(VCT_ERR_LIMIT, "at most 16 queued models per frame");
-  DrawBatch b;
-  uint32_t canon = 0;
-  fill_batch(ctx, q, 0, canon, b);
+  DrawTable b;
+  if ((rc = fill_draws(ctx, q, b, (phase & 1) != 0))) return rc;
   const uint32_t ntri = b.ntri;
   if (phase & 1) {
     hipLaunchKernelGGL(zero_u32_kernel, dim3((2 * nbricks + 255) / 256), dim3(256), 0, ctx->stream, bcount,
                        (uint64_t)2 * nbricks);
     VCT_HIP(hipGetLastError());
-    ctx->h_readback[1] = 0u;
+    ctx->h_readback[2] = 0u;
     if (ntri) {
       if ((rc = vct_ensure(ctx->setup, (size_t)ntri * sizeof(TriRec), ctx))) return rc;
       hipLaunchKernelGGL(vox_brick_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
                          (TriRec*)ctx->setup.p, bcount);
       VCT_HIP(hipGetLastError());
       if ((rc = vct_launch_scan_u32(ctx, bcount, boff, nbricks, counters + 2))) return rc;
-      VCT_HIP(hipMemcpyAsync(ctx->h_readback + 1, counters + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
+      VCT_HIP(hipMemcpyAsync(ctx->h_readback + 2, counters + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
     }
   }
   if (!(phase & 2)) return VCT_OK;
   // also when nothing was counted: a copy an interrupted earlier frame left
   // in flight must not land after the word is read
   VCT_HIP(hipStreamSynchronize(ctx->stream));
-  const uint32_t total = ctx->h_readback[1];
+  const uint32_t total = ctx->h_readback[2];
   if (total) {
     if ((rc = vct_ensure(ctx->tri_off, (size_t)total * 4 + (size_t)total / 4 * 4, ctx))) return rc;
     hipLaunchKernelGGL(vox_brick_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, ntri, vp,
@@ -880,11 +890,10 @@ static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, cons
   return VCT_OK;
 }
 
-// Brick lists of one batch of draws for the reference running average:
+// Brick lists of the queued draws for the reference running average:
 // records, per-brick triangle counts (bcount), offsets (boff) and lists
 // (tri_off).  Returns the number of (triangle, brick) pairs in `pairs`.
-static int ref_batch_lists(vct_context* ctx, const DrawBatch& b, const VoxParams& vp, uint32_t nbricks,
-                           uint32_t& pairs) {
+static int ref_lists(vct_context* ctx, const DrawTable& b, const VoxParams& vp, uint32_t nbricks, uint32_t& pairs) {
   uint32_t* counters = (uint32_t*)ctx->counters.p;
   uint32_t* bcount = (uint32_t*)ctx->tri_count.p;
   uint32_t* cursor = bcount + nbricks;
@@ -893,6 +902,7 @@ static int ref_batch_lists(vct_context* ctx, const DrawBatch& b, const VoxParams
   int rc;
   pairs = 0;
   if ((rc = vct_ensure(ctx->setup, (size_t)(ntri ? ntri : 1) * sizeof(TriRec), ctx))) return rc;
+  if ((rc = vct_ensure(ctx->tri_off, 16, ctx))) return rc;
   hipLaunchKernelGGL(zero_u32_kernel, dim3((2 * nbricks + 255) / 256), dim3(256), 0, ctx->stream, bcount,
                      (uint64_t)2 * nbricks);
   VCT_HIP(hipGetLastError());
@@ -901,8 +911,9 @@ static int ref_batch_lists(vct_context* ctx, const DrawBatch& b, const VoxParams
                      (TriRec*)ctx->setup.p, bcount);
   VCT_HIP(hipGetLastError());
   if ((rc = vct_launch_scan_u32(ctx, bcount, boff, nbricks, counters + 2))) return rc;
-  VCT_HIP(hipMemcpyAsync(&pairs, counters + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
+  VCT_HIP(hipMemcpyAsync(ctx->h_readback + 2, counters + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
   VCT_HIP(hipStreamSynchronize(ctx->stream));
+  pairs = ctx->h_readback[2];
   if (!pairs) return VCT_OK;
   if ((rc = vct_ensure(ctx->tri_off, (size_t)pairs * 4 + (size_t)pairs / 4 * 4, ctx))) return rc;
   hipLaunchKernelGGL(vox_brick_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, ntri, vp,
@@ -913,10 +924,9 @@ static int ref_batch_lists(vct_context* ctx, const DrawBatch& b, const VoxParams
 
 // Reference running average: the brick pipeline of the exact mean finds the
 // same fragments (each covered pixel of a triangle once, in the brick of its
-// voxel).  A count pass adds them up per voxel, a scan gives every voxel its
-// slots, a place pass stores (canonical key, colour) in them, and the replay
-// sorts each voxel's slots and runs the RGBA8 recurrence.  Batches of more
-// than kMaxDraws draws rebuild their brick lists for the place pass.
+// voxel).  A count pass writes the fragments of every voxel, a scan gives
+// every voxel its slots, a place pass stores (canonical key, colour) in them,
+// and the replay sorts each voxel's slots and runs the RGBA8 recurrence.
 static int voxelize_reference(vct_context* ctx, const std::vector<uint32_t>& q, const VoxParams& vp, void* level0) {
   const uint32_t res = ctx->res;
   const uint64_t nvox = (uint64_t)res * res * res;
@@ -933,27 +943,27 @@ static int voxelize_reference(vct_context* ctx, const std::vector<uint32_t>& q, 
   uint32_t* vcount = (uint32_t*)ctx->vox_count.p;
   const uint32_t* bcount = (const uint32_t*)ctx->tri_count.p;
   const uint32_t* boff = (const uint32_t*)ctx->tri_base.p;
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(2048), dim3(256), 0, ctx->stream, vcount, nvox);
-  VCT_HIP(hipGetLastError());
-  const size_t nbatch = (q.size() + kMaxDraws - 1) / kMaxDraws;
+  DrawTable b;
+  if ((rc = fill_draws(ctx, q, b))) return rc;
+  uint32_t pairs = 0;
+  if ((rc = ref_lists(ctx, b, vp, nbricks, pairs))) return rc;
+  ctx->last_candidates += pairs;
   RefPass r;
   memset(&r, 0, sizeof(r));
   r.vcount = vcount;
-  uint32_t canon = 0, pairs = 0;
-  for (size_t qi = 0; qi < q.size(); qi += kMaxDraws) {
-    DrawBatch b;
-    fill_batch(ctx, q, qi, canon, b);
-    if ((rc = ref_batch_lists(ctx, b, vp, nbricks, pairs))) return rc;
-    ctx->last_candidates += pairs;
-    if (pairs)
-      launch_brick<VCT_RGBA8, kRefCount>(ctx, nbricks, vp, bcount, boff, (const uint32_t*)ctx->tri_off.p, nullptr, r);
-    VCT_HIP(hipGetLastError());
-  }
+  // every brick's workgroup writes its voxels' counts, zeros included
+  launch_brick<VCT_RGBA8, kRefCount>(ctx, nbricks, vp, bcount, boff, (const uint32_t*)ctx->tri_off.p, nullptr, r);
+  VCT_HIP(hipGetLastError());
   if ((rc = vct_launch_scan_u32(ctx, vcount, (uint32_t*)ctx->vox_off.p, (uint32_t)nvox, counters + 1))) return rc;
-  uint32_t nfrag = 0;
-  VCT_HIP(hipMemcpyAsync(&nfrag, counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+  VCT_HIP(hipMemcpyAsync(ctx->h_readback + 2, counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
   VCT_HIP(hipStreamSynchronize(ctx->stream));
+  const uint32_t nfrag = ctx->h_readback[2];
   ctx->last_fragments = nfrag;
+  if (!nfrag) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(2048), dim3(256), 0, ctx->stream, (uint32_t*)level0, nvox);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+  }
   const size_t ns = ((size_t)nfrag + 4) & ~(size_t)3;
   if ((rc = vct_ensure(ctx->frags_sorted, ns * 24, ctx))) return rc;
   r.vox_off = (const uint32_t*)ctx->vox_off.p;
@@ -961,30 +971,8 @@ static int voxelize_reference(vct_context* ctx, const std::vector<uint32_t>& q, 
   r.s_color = (float4*)((char*)ctx->frags_sorted.p + ns * 8);
   r.nfrag = nfrag;
   r.nvox = (uint32_t)nvox;
-  if (!nfrag) {
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(2048), dim3(256), 0, ctx->stream, (uint32_t*)level0, nvox);
-    VCT_HIP(hipGetLastError());
-    return VCT_OK;
-  }
-  if (nbatch > 1) {
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(2048), dim3(256), 0, ctx->stream, vcount, nvox);
-    VCT_HIP(hipGetLastError());
-  }
-  canon = 0;
-  for (size_t qi = 0; qi < q.size(); qi += kMaxDraws) {
-    if (nbatch > 1) {
-      DrawBatch b;
-      fill_batch(ctx, q, qi, canon, b);
-      if ((rc = ref_batch_lists(ctx, b, vp, nbricks, pairs))) return rc;
-    }
-    // the last batch runs even without pairs: its workgroups write level 0
-    r.replay = qi + kMaxDraws >= q.size() ? 1u : 0u;
-    r.carry = qi > 0 ? 1u : 0u;
-    if (pairs || r.replay)
-      launch_brick<VCT_RGBA8, kRefPlace>(ctx, nbricks, vp, bcount, boff, (const uint32_t*)ctx->tri_off.p, level0,
-                                         r);
-    VCT_HIP(hipGetLastError());
-  }
+  launch_brick<VCT_RGBA8, kRefPlace>(ctx, nbricks, vp, bcount, boff, (const uint32_t*)ctx->tri_off.p, level0, r);
+  VCT_HIP(hipGetLastError());
   return VCT_OK;
 }
 

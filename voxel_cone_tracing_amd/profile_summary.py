@@ -17,15 +17,28 @@ Kernels are keyed by name and grid size, so that launches of one kernel over
 different work (the full frame, a quarter-resolution frame, tile lists) are
 kept apart.  The traffic summary ends with one machine-readable line per
 (kernel, grid):  `traffic_bytes_per_launch <kernel> <grid> <bytes>`, which
-bench.py reads for its roofline.
+bench.py reads for its roofline, and a `kernel_source_sha256 <hash>` line: the
+hash of the kernel sources the counters were taken with.  bench.py uses the
+counters only while the sources still hash the same.
 """
 from __future__ import annotations
 
 import argparse
 import collections
 import csv
+import hashlib
 import re
 from pathlib import Path
+
+CSRC = Path(__file__).resolve().parent / "csrc"
+
+
+def source_hash() -> str:
+    """SHA-256 over the kernel sources (names and contents of csrc/*.hip, *.h, *.cpp)."""
+    h = hashlib.sha256()
+    for f in sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.h")) + list(CSRC.glob("*.cpp"))):
+        h.update(f.name.encode() + b"\0" + f.read_bytes() + b"\0")
+    return h.hexdigest()
 
 
 def short_name(name: str) -> str:
@@ -113,15 +126,29 @@ def cmd_traffic(a):
             hit = f"{h / max(h + m, 1.0):.1%}"
             machine.append(f"l2_hit_rate {k} {g} {h / max(h + m, 1.0):.4f}")
         lines.append(f"| `{k}` | {g} | {len(fv)} | {f:.0f} | {w:.0f} | {traffic:.4g} | {hit} |")
-    lines += ["", "Machine-readable (kernel, grid, scaled HBM bytes per launch):", "", "```"] + machine + ["```"]
+    machine.append(f"kernel_source_sha256 {source_hash()}")
+    lines += ["", "Machine-readable (kernel, grid, scaled HBM bytes per launch; the hash of the kernel sources "
+              "profiled):", "", "```"] + machine + ["```"]
     Path(a.out).write_text("\n".join(lines) + "\n")
     print(f"wrote {a.out}")
 
 
+def summary_is_current(path: Path) -> bool:
+    """True if the summary was taken with the kernel sources as they are now."""
+    if not Path(path).exists():
+        return False
+    for line in Path(path).read_text().splitlines():
+        p = line.split()
+        if len(p) == 2 and p[0] == "kernel_source_sha256":
+            return p[1] == source_hash()
+    return False
+
+
 def read_summary(path: Path, what: str, kernel: str, grid: int | None = None):
     """A machine-readable value of a traffic summary: what = "traffic_bytes_per_launch"
-    (scaled HBM bytes per launch) or "l2_hit_rate", for `kernel` (and grid), or None."""
-    if not Path(path).exists():
+    (scaled HBM bytes per launch) or "l2_hit_rate", for `kernel` (and grid), or
+    None; None also when the summary is stale (taken with other kernel sources)."""
+    if not summary_is_current(path):
         return None
     for line in Path(path).read_text().splitlines():
         p = line.split()
