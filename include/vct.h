@@ -276,6 +276,18 @@ int vct_mlp_get_params(vct_ctx ctx, float* params, uint32_t n);
 /* Network output for n feature rows (n x 34 floats, rounded to BF16 on the
  * device) -> n x 3 floats. */
 int vct_mlp_forward(vct_ctx ctx, const float* features, uint32_t n, float* out);
+/* vct_mlp_forward on device buffers, allocating nothing.  It has no
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  features (n x 34 floats) and out (n x 3 floats) are
+ * device pointers; the work is queued on `stream` (a hipStream_t; NULL: the
+ * context's stream), after vct_mlp_set_params / vct_mlp_init has completed on
+ * it (vct_sync).  The BF16 rows are staged in a fixed per-GPU array of
+ * VCT_MLP_NOALLOC_MAX_ROWS rows (32 MiB), held until the process ends: larger
+ * n is rejected with VCT_ERR_LIMIT.  Every call on a GPU shares that array, so
+ * calls that overlap on one GPU race.  Outputs equal vct_mlp_forward's. */
+#define VCT_MLP_NOALLOC_MAX_ROWS 262144
+int vct_mlp_forward_noalloc(vct_ctx ctx, const float* features, uint32_t n, float* out, void* stream);
 /* Loss mean((albedo * (y - target))^2) over n samples and 3 channels and its
  * gradient w.r.t. every parameter (the training kernels, no update). */
 int vct_mlp_loss_grad(vct_ctx ctx, const float* features, const float* albedo, const float* target, uint32_t n,

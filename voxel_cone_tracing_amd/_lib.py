@@ -30,6 +30,7 @@ AVG_REFERENCE, AVG_EXACT = 0, 1
 CONES_REF9, CONES_5_WIDE, CONES_16 = 0, 1, 2
 LIGHT_POINT, LIGHT_DIRECTIONAL = 0, 1
 MLP_IN, MLP_HIDDEN, MLP_OUT = 34, 64, 3
+MLP_NOALLOC_MAX_ROWS = 262144
 MLP_PARAMS = MLP_IN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN * MLP_OUT + MLP_OUT
 
 
@@ -141,6 +142,7 @@ PROTOTYPES = {
     "vct_mlp_set_params": (_I, [_P, _FP, _U]),
     "vct_mlp_get_params": (_I, [_P, _FP, _U]),
     "vct_mlp_forward": (_I, [_P, _FP, _U, _FP]),
+    "vct_mlp_forward_noalloc": (_I, [_P, _P, _U, _P, _P]),
     "vct_trace_load_peak": (_I, [_P, C.POINTER(C.c_double)]),
     "vct_mlp_loss_grad": (_I, [_P, _FP, _FP, _FP, _U, _FP, _FP]),
     "vct_upsample_collect": (_I, [_P, _U, C.c_uint64]),

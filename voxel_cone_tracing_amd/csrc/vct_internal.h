@@ -326,4 +326,5 @@ int vct_launch_scan_u32(vct_context* ctx, const uint32_t* in, uint32_t* out, uin
 int vct_tex3d_mip_launch(vct_context* ctx, VctTex3D& t);
 // Device address of the no-allocation entry point's staging array, taken once
 // per device (the current one) at context creation.
+int vct_noalloc_setup(int device);
 int vct_tex3d_clear_launch(vct_context* ctx, VctTex3D& t, const float rgba[4]);

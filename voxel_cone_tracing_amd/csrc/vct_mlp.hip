@@ -19,6 +19,7 @@
 // albedo * prediction.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "vct_internal.h"
 
@@ -1310,6 +1311,50 @@ int shuffle_rows(vct_context* ctx, uint32_t n, uint64_t seed) {
     VCT_HIP(hipSetDevice((ctx)->device));                                     \
   } while (0)
 
+// ---- the forward pass that allocates nothing ----------------------------------------
+// The BF16 feature rows of vct_mlp_forward_noalloc are staged in this fixed
+// array: each GPU the library runs on holds its own copy until the process
+// ends, at its own address.  Host code never passes the array's name to a
+// kernel (on the host it is only a stand-in symbol): the device address comes
+// from hipGetSymbolAddress, taken once per device when a context is created.
+static __device__ __align__(256) uint16_t scratch[(size_t)VCT_MLP_NOALLOC_MAX_ROWS * 64];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+uint16_t* g_scratch_ptr[kScratchDevices];  // device address of `scratch` per device (null: not set up)
+}  // namespace
+
+// Sets up the device address of the staging array of the current device.
+int vct_noalloc_setup(int device) {
+  if (device < 0 || device >= kScratchDevices) return vct_set_error(VCT_ERR_LIMIT, "device index above 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return VCT_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return vct_hip_check(e, "hipGetSymbolAddress(scratch)");
+  if (!ptr) return vct_set_error(VCT_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+  g_scratch_ptr[device] = (uint16_t*)ptr;
+  return VCT_OK;
+}
+
+// Feature rows (n x 34 floats) -> BF16 rows of 64 (columns 34..63 zero), the
+// layout the forward kernel reads; one thread per (row, 8 columns).
+__global__ __launch_bounds__(256) void mlp_stage_kernel(const float* __restrict__ features, uint32_t n,
+                                                        uint16_t* __restrict__ X) {
+  const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  if (g >= (uint64_t)n * 8u) return;
+  const uint32_t row = (uint32_t)(g >> 3), q = (uint32_t)(g & 7u);
+  uint32_t w[4];
+  for (int e = 0; e < 8; e += 2) {
+    const int c0 = (int)q * 8 + e, c1 = c0 + 1;
+    const uint32_t lo = c0 < kIn ? f32_to_bf16(features[(size_t)row * kIn + c0]) : 0u;
+    const uint32_t hi = c1 < kIn ? f32_to_bf16(features[(size_t)row * kIn + c1]) : 0u;
+    w[e >> 1] = lo | (hi << 16);
+  }
+  reinterpret_cast<uint4*>(X + (size_t)row * 64)[q] = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 extern "C" {
 
 int vct_mlp_init(vct_ctx ctx, uint64_t seed) {
@@ -1373,6 +1418,24 @@ int vct_mlp_forward(vct_ctx ctx, const float* features, uint32_t n, float* out) 
   VCT_HIP(hipMemcpyAsync(out, ctx->train_Y.p, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream));
   VCT_HIP(hipStreamSynchronize(ctx->stream));
   ctx->train_n = 0;  // the training set buffers were reused
+  return VCT_OK;
+}
+
+int vct_mlp_forward_noalloc(vct_ctx ctx, const float* features, uint32_t n, float* out, void* stream) {
+  MLP_CTX(ctx);
+  if (!features || !out || n == 0) return vct_set_error(VCT_ERR_INVALID_ARG, "bad arguments");
+  if (n > (uint32_t)VCT_MLP_NOALLOC_MAX_ROWS)
+    return vct_set_error(VCT_ERR_LIMIT, "vct_mlp_forward_noalloc takes at most VCT_MLP_NOALLOC_MAX_ROWS rows");
+  int rc;
+  if ((rc = need_mlp(ctx))) return rc;
+  uint16_t* X = ctx->device < kScratchDevices ? g_scratch_ptr[ctx->device] : nullptr;
+  if (!X) return vct_set_error(VCT_ERR_STATE, "staging array of the device not set up");
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  hipLaunchKernelGGL(mlp_stage_kernel, dim3((unsigned)(((uint64_t)n * 8u + 255u) / 256u)), dim3(256), 0, st,
+                     features, n, X);
+  hipLaunchKernelGGL(mlp_forward_kernel, dim3((n + 255) / 256), dim3(256), 0, st, mlp_dev(ctx),
+                     (const uint16_t*)X, n, out);
+  VCT_HIP(hipGetLastError());
   return VCT_OK;
 }
 

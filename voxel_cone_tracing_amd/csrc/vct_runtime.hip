@@ -398,6 +398,7 @@ int vct_create(int width, int height, vct_ctx* out) {
   dm.illum = 0;
   ctx->materials.push_back(dm);
   int rc = alloc_volume(ctx);
+  if (!rc) rc = vct_noalloc_setup(dev);  // the staging array of vct_mlp_forward_noalloc
   if (rc) {
     vct_destroy(ctx);
     return rc;
