@@ -142,12 +142,17 @@ def _bilinear(low, gn_alb, W, H):
     return (top * (1 - av) + bot * av) * gn_alb
 
 
-def test_learned_upsampler_beats_its_interpolation_and_bilinear(gpu_available):
-    """What the trained network adds: the learned frame (per-tile fallback off)
-    against the same frame with a zero network (the surface-guided
-    interpolation alone, same per-pixel tracing) and against bilinear
-    upsampling of the same quarter-resolution buffer; then the per-tile
-    fallback at 40 dB."""
+def test_upsampler_trained_in_place_beats_bilinear_and_holds_40db(gpu_available):
+    """Collect and train on the device end to end on a small scene, then the
+    learned frame (per-tile fallback off) against bilinear upsampling of the
+    same quarter-resolution buffer, and the per-tile fallback at 40 dB.
+
+    The gain of the network over the surface-guided interpolation alone is
+    tested at the benchmark workload with the committed network
+    (test_c4_network_beats_guided_interpolation).  A network trained here in
+    1,500 steps on 24 poses of this 4,000-triangle scene does not beat the
+    interpolation on a held-out pose (measured 34.88 against 35.18 dB); the
+    bound below only keeps it from falling further behind."""
     W, H = 256, 192
     # triangles of ~e^-2.5 = 0.08 units (several quarter-resolution samples
     # each at this viewport), so that most covered pixels are interpolated
@@ -202,7 +207,7 @@ def test_learned_upsampler_beats_its_interpolation_and_bilinear(gpu_available):
         f"traced pixels {traced}/{int(cov.sum())}; with tile fallback {p_c:.2f} dB ({fb}/{tiles} tiles)")
     assert np.isfinite(learned).all()
     assert traced < 0.5 * cov.sum()  # the comparison is about interpolated pixels
-    assert p_l > p_g - 0.5  # the correction must not cost quality (it gains ~0.3 dB at 2560x1440)
+    assert p_l > p_g - 0.5  # regression bound only (see the docstring)
     assert p_l > p_b + 1.0
     assert p_c >= 40.0
 
