@@ -231,6 +231,15 @@ __device__ __forceinline__ void vox_edges(const Rec& s, int i, int j, VEdges& e)
   e.ey[2] = by - ay;
 }
 
+// Coverage of a pixel from its edge values (top-left rule).
+__device__ __forceinline__ bool vox_covers(const VEdges& e) {
+  if (e.w[0] < 0 || e.w[1] < 0 || e.w[2] < 0) return false;
+  if (e.w[0] == 0 && !top_left(e.ex[0], e.ey[0])) return false;
+  if (e.w[1] == 0 && !top_left(e.ex[1], e.ey[1])) return false;
+  if (e.w[2] == 0 && !top_left(e.ex[2], e.ey[2])) return false;
+  return true;
+}
+
 // Fragment at pixel (i, j) of the triangle's 2res x 2res raster given its
 // edge values (wa, wb, wc): coverage, voxel (Morton index) and colour.
 // Returns false if the pixel is not covered or the voxel falls outside the
@@ -597,6 +606,7 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
   __shared__ uint32_t s_org[kBrickThreads];  // clipped box origin: i | j << 16
   __shared__ uint32_t s_div[kBrickThreads];  // ceil(65536 / width) | width << 20
   __shared__ uint32_t s_wsum[kBrickThreads / 64];
+  __shared__ uint32_t s_q[kBrickThreads / 64][128];  // per-wave queue of covered candidates
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const BrickGeom g = brick_geom(vp.res);
   // Workgroups are dealt to the 8 XCDs round-robin in dispatch order; XCD k
@@ -673,24 +683,19 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
     s_end[tid] = x;
     __syncthreads();
     const uint32_t total = s_end[kBrickThreads - 1];
-    for (uint32_t c = tid; c < total; c += kBrickThreads) {
-      // the triangle of the wave's first candidate by binary search (the same
-      // for every lane), then each lane steps forward over the few triangles
-      // its candidate lies past
-      const uint32_t c0 = c - (uint32_t)lane;
-      uint32_t lo = 0, hi = kBrickThreads - 1;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (s_end[mid] > c0) hi = mid;
-        else lo = mid + 1;
-      }
-      while (s_end[lo] <= c) ++lo;
-      const uint32_t local = c - (lo ? s_end[lo - 1] : 0u);
-      const uint32_t dv = s_div[lo], cw = dv >> 20;
-      const uint32_t row = (local * (dv & 0xfffffu)) >> 16;  // exact: local < 35 * 35, cw <= 35
-      const uint32_t col = local - row * cw;
-      const uint32_t org = s_org[lo];
-      const int i = (int)(org & 0xffffu) + (int)col, j = (int)(org >> 16) + (int)row;
+    // Candidates in two phases.  1: the wave's 64 candidates are tested for
+    // coverage only (exact edge functions, top-left rule); the covered ones
+    // are appended to the wave's queue in LDS.  2: whenever the queue holds
+    // 64, every lane takes one and runs the exact fragment path (voxel
+    // address, brick test, shading, accumulation), so the expensive part
+    // runs with all lanes busy instead of on the covered ~40 % of them.  The
+    // set of fragments, and every value computed for one, is the same as
+    // testing each candidate in full.
+    uint32_t* q = s_q[wave];
+    uint32_t qn = 0;  // entries in the queue (the same in every lane)
+    auto fragment = [&](uint32_t d) {
+      const uint32_t lo = d & 255u;
+      const int i = (int)((d >> 8) & 0x7ffu), j = (int)(d >> 19);
       const CovRec cr = s_cov[lo];
       VEdges e;
       vox_edges(cr, i, j, e);
@@ -698,7 +703,7 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
       float4 color;
       if (!shade_w<true, kMode == kRefCount>(cr, recs + cr.tri, e, i, j, e.w[0], e.w[1], e.w[2], vp, mats, key,
                                              color, bshift, brick))
-        continue;
+        return;
       const uint32_t lv = key & (nvox - 1u);
       if constexpr (kMode == kExactMean) {
         brick_add(s_sum, s_cnt, lv, color);
@@ -716,7 +721,43 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
           r.s_color[slot] = color;
         }
       }
+    };
+    for (uint32_t cb = (uint32_t)wave * 64u; cb < total; cb += kBrickThreads) {
+      const uint32_t c = cb + (uint32_t)lane;
+      bool hit = false;
+      uint32_t desc = 0;
+      if (c < total) {
+        // the triangle of the wave's first candidate by binary search (the
+        // same for every lane), then each lane steps forward over the few
+        // triangles its candidate lies past
+        uint32_t lo = 0, hi = kBrickThreads - 1;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s_end[mid] > cb) hi = mid;
+          else lo = mid + 1;
+        }
+        while (s_end[lo] <= c) ++lo;
+        const uint32_t local = c - (lo ? s_end[lo - 1] : 0u);
+        const uint32_t dv = s_div[lo], cw = dv >> 20;
+        const uint32_t row = (local * (dv & 0xfffffu)) >> 16;  // exact: local < 35 * 35, cw <= 35
+        const uint32_t col = local - row * cw;
+        const uint32_t org = s_org[lo];
+        const int i = (int)(org & 0xffffu) + (int)col, j = (int)(org >> 16) + (int)row;
+        const CovRec cr = s_cov[lo];
+        VEdges e;
+        vox_edges(cr, i, j, e);
+        hit = vox_covers(e);
+        desc = lo | ((uint32_t)i << 8) | ((uint32_t)j << 19);
+      }
+      const unsigned long long m = __ballot(hit);
+      if (hit) q[qn + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = desc;
+      qn += (uint32_t)__popcll(m);
+      if (qn >= 64u) {
+        qn -= 64u;
+        fragment(q[qn + (uint32_t)lane]);
+      }
     }
+    if ((uint32_t)lane < qn) fragment(q[lane]);
     __syncthreads();
   }
   if constexpr (kMode == kExactMean) {
