@@ -514,20 +514,29 @@ __device__ __forceinline__ void g_resolve_pixel(const GParams& p, const GRec* __
   const int d = g_find_draw(p, t);
   const GDraw& dr = p.draws[d];
   const uint32_t lt = dr.inv[t - dr.tri_base];  // stored index
-  float clip[3][4];
-  g_clip_coords(dr, p.PV, lt, clip);
-  const bool fan = !g_unclipped(clip);
+  const float* pp = dr.pos + (size_t)lt * 9;
+  const float* nn = dr.nrm + (size_t)lt * 9;
+  // the snapped vertices, orientation and area the raster used, from the
+  // triangle's record (or its fan triangle's)
+  const GRec& rec = recs[dr.tri_base + lt];
+  const bool fan = (rec.swap & kClipped) != 0u;
   uint32_t fi = 0;  // fan record of a clipped triangle
   GSetup s;
   float cw[3];
   if (!fan) {
+    s = g_setup_of(rec);
+    // clip w of each vertex, as g_clip_coords computes it
     for (int v = 0; v < 3; ++v) {
-      for (int c = 0; c < 4; ++c) s.clip[v][c] = clip[v][c];
-      cw[v] = clip[v][3];
+      float w[3];
+      world_of(dr, pp + 3 * v, w);
+      float t4 = p.PV[3] * w[0];
+      t4 = t4 + p.PV[7] * w[1];
+      t4 = t4 + p.PV[11] * w[2];
+      t4 = t4 + p.PV[15];
+      cw[v] = t4;
     }
-    g_setup_clip(s, p.W, p.H);
   } else {
-    fi = (uint32_t)recs[dr.tri_base + lt].area + (id & 7u);
+    fi = (uint32_t)rec.area + (id & 7u);
     s = g_setup_of(fan_recs[fi]);
     for (int v = 0; v < 3; ++v) cw[v] = fans[fi].w[v];
   }
@@ -553,8 +562,6 @@ __device__ __forceinline__ void g_resolve_pixel(const GParams& p, const GRec* __
     }
     for (int o = 0; o < 3; ++o) q[o] = lam[o];
   }
-  const float* pp = dr.pos + (size_t)lt * 9;
-  const float* nn = dr.nrm + (size_t)lt * 9;
   float wp[3] = {0.f, 0.f, 0.f}, n[3] = {0.f, 0.f, 0.f};
   for (int v = 0; v < 3; ++v) {
     float w[3];

@@ -234,6 +234,7 @@ struct vct_context {
 
   // voxelize scratch
   VctBuf tri_count, tri_off, tri_base;  // per-brick triangle counts + cursors, triangle lists, per-brick offsets
+  VctBuf vox_box;                      // voxelize: packed brick box per triangle
   VctBuf frags_sorted, vox_count, vox_off, scan_tmp, g_scan_tmp, counters;
   VctBuf setup;  // per-frame triangle list (model index, transform)
   uint64_t last_fragments = 0, last_candidates = 0;

@@ -431,7 +431,7 @@ int vct_destroy(vct_ctx ctx) {
                     &ctx->train_valid, &ctx->probe_pix, &ctx->probe_val, &ctx->fb_list, &ctx->fb_pix,
                     &ctx->fb_list2, &ctx->fb_pix2, &ctx->fb_flag, &ctx->half_img, &ctx->probe_pix2,
                     &ctx->probe_val2, &ctx->pairs, &ctx->fb_est, &ctx->g_fans, &ctx->g_clip,
-                    &ctx->train_perm, &ctx->batch_X, &ctx->batch_A, &ctx->batch_Y};
+                    &ctx->train_perm, &ctx->batch_X, &ctx->batch_A, &ctx->batch_Y, &ctx->vox_box};
   for (VctBuf* b : bufs) free_buf(*b);
   vct_upload_free(ctx->g_draws);
   vct_upload_free(ctx->v_draws);

@@ -426,8 +426,8 @@ constexpr int kMaxJobs = kJobShadow + VCT_MAX_LIGHTS;
 constexpr int kRefillMin = 24;
 
 // Block shapes: pixels per workgroup and the shadow-light slots in LDS.
-// Cones are short in cluttered scenes (3 steps on average on a 1M-triangle
-// soup), so a queue of 64 pixels x 10 cones gives each of the 256 lanes only
+// Cones are short in cluttered scenes (6.2 steps per cone of a covered pixel
+// on a 1M-triangle soup), so a queue of 64 pixels x 10 cones gives each of the 256 lanes only
 // 2.5 items and the drain at its end leaves a third of the lanes idle; more
 // pixels per workgroup shorten the drain relative to the work.  The wide
 // shape holds shadow results of up to 2 lights; more lights use the full one.

@@ -415,9 +415,27 @@ __device__ __forceinline__ uint32_t brick_of(const BrickBox& bb, int k) {
   return morton3((uint32_t)x, (uint32_t)y, (uint32_t)z);
 }
 
+// The brick box of a triangle packed into 8 bytes for the list write: first
+// brick per axis, then bricks per axis (0: no brick), 8 bits each (at most
+// 128 bricks per axis).
+__device__ __forceinline__ uint2 pack_box(const BrickBox& bb) {
+  if (bb.n == 0) return make_uint2(0u, 0u);
+  return make_uint2((uint32_t)bb.b0[0] | ((uint32_t)bb.b0[1] << 8) | ((uint32_t)bb.b0[2] << 16),
+                    (uint32_t)bb.nb[0] | ((uint32_t)bb.nb[1] << 8) | ((uint32_t)bb.nb[2] << 16));
+}
+__device__ __forceinline__ BrickBox unpack_box(uint2 p) {
+  BrickBox bb;
+  for (int k = 0; k < 3; ++k) {
+    bb.b0[k] = (int)((p.x >> (8 * k)) & 255u);
+    bb.nb[k] = (int)((p.y >> (8 * k)) & 255u);
+  }
+  bb.n = bb.nb[0] * bb.nb[1] * bb.nb[2];
+  return bb;
+}
+
 // 1. setup (lane per stored triangle): record + count into overlapped bricks.
 __global__ __launch_bounds__(256) void vox_brick_count_kernel(DrawTable b, VoxParams vp, TriRec* __restrict__ recs,
-                                                              uint32_t* brick_count) {
+                                                              uint2* __restrict__ boxes, uint32_t* brick_count) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const BrickGeom g = brick_geom(vp.res);
   Setup s;
@@ -431,29 +449,19 @@ __global__ __launch_bounds__(256) void vox_brick_count_kernel(DrawTable b, VoxPa
     else recs[t].area = 0;  // marks a skipped triangle (set-up triangles have area > 0)
   }
   const BrickBox bb = brick_box(s, vp.res, g);
+  if (t < b.ntri) boxes[t] = pack_box(bb);
   for (int k = 0; __ballot(k < bb.n) != 0ull; ++k) {
     const bool act = k < bb.n;
     wave_add_one(brick_count, act ? brick_of(bb, k) : 0u, act);
   }
 }
 
-// 3. list write: the same brick box, from the positions in the record.
-__global__ __launch_bounds__(256) void vox_brick_write_kernel(uint32_t ntri, VoxParams vp,
-                                                              const TriRec* __restrict__ recs,
+// 3. list write: the same brick boxes, as the setup packed them.
+__global__ __launch_bounds__(256) void vox_brick_write_kernel(uint32_t ntri, const uint2* __restrict__ boxes,
                                                               const uint32_t* __restrict__ brick_off,
                                                               uint32_t* cursor, uint32_t* __restrict__ list) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const BrickGeom g = brick_geom(vp.res);
-  float pw[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-  bool ok = false;
-  if (t < ntri) {
-    const TriRec* rp = recs + t;
-    ok = rp->area != 0;
-    if (ok)
-      for (int v = 0; v < 3; ++v)
-        for (int k = 0; k < 3; ++k) pw[v][k] = rp->w[v][k];
-  }
-  const BrickBox bb = brick_box(pw, ok, vp.res, g);
+  const BrickBox bb = unpack_box(t < ntri ? boxes[t] : make_uint2(0u, 0u));
   for (int k = 0; __ballot(k < bb.n) != 0ull; ++k) {
     const bool act = k < bb.n;
     const uint32_t key = act ? brick_of(bb, k) : 0u;
@@ -598,9 +606,9 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
                                                                   const uint32_t* __restrict__ list,
                                                                   void* level0, RefPass r) {
   constexpr int kVox = kBrick * kBrick * kBrick;
-  __shared__ uint32_t s_sum[4][kVox];
+  __shared__ uint32_t s_sum[4][kMode == kExactMean ? kVox : 1];
   __shared__ uint32_t s_cnt[kVox];
-  __shared__ uint32_t s_base[kVox];
+  __shared__ uint32_t s_base[kMode == kRefPlace ? kVox : 1];
   __shared__ CovRec s_cov[kBrickThreads];
   __shared__ uint32_t s_end[kBrickThreads];  // inclusive prefix of candidate counts
   __shared__ uint32_t s_org[kBrickThreads];  // clipped box origin: i | j << 16
@@ -895,8 +903,9 @@ static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, cons
     ctx->h_readback[2] = 0u;
     if (ntri) {
       if ((rc = vct_ensure(ctx->setup, (size_t)ntri * sizeof(TriRec), ctx))) return rc;
+      if ((rc = vct_ensure(ctx->vox_box, (size_t)ntri * 8, ctx))) return rc;
       hipLaunchKernelGGL(vox_brick_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
-                         (TriRec*)ctx->setup.p, bcount);
+                         (TriRec*)ctx->setup.p, (uint2*)ctx->vox_box.p, bcount);
       VCT_HIP(hipGetLastError());
       if ((rc = vct_launch_scan_u32(ctx, bcount, boff, nbricks, counters + 2))) return rc;
       VCT_HIP(hipMemcpyAsync(ctx->h_readback + 2, counters + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -909,8 +918,8 @@ static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, cons
   const uint32_t total = ctx->h_readback[2];
   if (total) {
     if ((rc = vct_ensure(ctx->tri_off, (size_t)total * 4 + (size_t)total / 4 * 4, ctx))) return rc;
-    hipLaunchKernelGGL(vox_brick_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, ntri, vp,
-                       (const TriRec*)ctx->setup.p, (const uint32_t*)boff, cursor, (uint32_t*)ctx->tri_off.p);
+    hipLaunchKernelGGL(vox_brick_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, ntri,
+                       (const uint2*)ctx->vox_box.p, (const uint32_t*)boff, cursor, (uint32_t*)ctx->tri_off.p);
     VCT_HIP(hipGetLastError());
   } else {
     hipLaunchKernelGGL(zero_u32_kernel, dim3((nbricks + 255) / 256), dim3(256), 0, ctx->stream, boff,
@@ -948,8 +957,9 @@ static int ref_lists(vct_context* ctx, const DrawTable& b, const VoxParams& vp, 
                      (uint64_t)2 * nbricks);
   VCT_HIP(hipGetLastError());
   if (!ntri) return VCT_OK;
+  if ((rc = vct_ensure(ctx->vox_box, (size_t)ntri * 8, ctx))) return rc;
   hipLaunchKernelGGL(vox_brick_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
-                     (TriRec*)ctx->setup.p, bcount);
+                     (TriRec*)ctx->setup.p, (uint2*)ctx->vox_box.p, bcount);
   VCT_HIP(hipGetLastError());
   if ((rc = vct_launch_scan_u32(ctx, bcount, boff, nbricks, counters + 2))) return rc;
   VCT_HIP(hipMemcpyAsync(ctx->h_readback + 2, counters + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -957,8 +967,8 @@ static int ref_lists(vct_context* ctx, const DrawTable& b, const VoxParams& vp, 
   pairs = ctx->h_readback[2];
   if (!pairs) return VCT_OK;
   if ((rc = vct_ensure(ctx->tri_off, (size_t)pairs * 4 + (size_t)pairs / 4 * 4, ctx))) return rc;
-  hipLaunchKernelGGL(vox_brick_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, ntri, vp,
-                     (const TriRec*)ctx->setup.p, (const uint32_t*)boff, cursor, (uint32_t*)ctx->tri_off.p);
+  hipLaunchKernelGGL(vox_brick_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, ntri,
+                     (const uint2*)ctx->vox_box.p, (const uint32_t*)boff, cursor, (uint32_t*)ctx->tri_off.p);
   VCT_HIP(hipGetLastError());
   return VCT_OK;
 }
