@@ -5,6 +5,12 @@ Kernel times (from `rocprofv3 --kernel-trace --stats --output-format csv`):
     python -m voxel_cone_tracing_amd.profile_summary times --trace DIR/run_kernel_trace.csv \\
         --cmd "<the profiled command>" --out profiles/kernel_times_soup256.md
 
+Counter utilisation (VALU / TA busy, stalls, instructions per wave) from
+`rocprofv3 --pmc` runs of SQ, TA and GRBM counters:
+
+    python -m voxel_cone_tracing_amd.profile_summary counters --dirs DIR1 DIR2 ... \\
+        --cmd "<the profiled commands>" --out profiles/kernel_counters_soup256.md
+
 HBM bytes per launch (from two `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE`
 runs of a program that also copied a buffer of known size with the runtime's
 copy kernel, bench.py --copy-calibration BYTES), optionally with a third run
@@ -133,6 +139,61 @@ def cmd_traffic(a):
     print(f"wrote {a.out}")
 
 
+def cmd_counters(a):
+    """Utilisation of the listed kernels from SQ / TA / GRBM counter runs.
+
+    SQ counters are sums over the shader engines, GRBM_GUI_ACTIVE a sum over
+    the XCDs; with N CUs and X XCDs:
+      VALU busy = SQ_ACTIVE_INST_VALU / N / (GRBM_GUI_ACTIVE / X)
+      TA busy   = TA_TA_BUSY_sum / N / (GRBM_GUI_ACTIVE / X)
+      wait      = SQ_WAIT_ANY / SQ_WAVE_CYCLES (share of wave time stalled)
+    and instructions per wave are the SQ_INSTS_* counts over SQ_WAVES."""
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    for d in a.dirs:
+        for k, c in _counters(Path(d)).items():
+            for n, v in c.items():
+                agg[k][n].extend(v)
+    lines = [f"# Kernel counters: {a.title}", "", "Commands (from the repository root), one per counter set:", "",
+             "```", a.cmd, "```", "",
+             "Summarised by `python -m voxel_cone_tracing_amd.profile_summary counters --dirs <dir>...`, means "
+             f"over launches, {a.cus} CUs in {a.xcds} XCDs.  VALU busy = SQ_ACTIVE_INST_VALU / CUs / "
+             "(GRBM_GUI_ACTIVE / XCDs); TA busy = TA_TA_BUSY_sum / CUs / (GRBM_GUI_ACTIVE / XCDs); stalled = "
+             "SQ_WAIT_ANY / SQ_WAVE_CYCLES; per-wave counts = SQ_INSTS_* / SQ_WAVES.", "",
+             "| kernel | grid (threads) | VALU busy | TA busy | stalled | VALU / wave | SALU / wave | LDS / wave "
+             "| VMEM rd / wave | LDS bank conflicts / LDS instr |",
+             "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+
+    def mean(c, n):
+        v = c.get(n)
+        return sum(v) / len(v) if v else None
+
+    for key in sorted(agg, key=lambda k: -(mean(agg[k], "GRBM_GUI_ACTIVE") or 0.0)):
+        k, g = key
+        if a.kernels and not any(k.startswith(x) for x in a.kernels):
+            continue
+        c = agg[key]
+        gui = mean(c, "GRBM_GUI_ACTIVE")
+        waves = mean(c, "SQ_WAVES")
+
+        def frac(n):
+            v = mean(c, n)
+            return f"{v / a.cus / (gui / a.xcds):.0%}" if v is not None and gui else ""
+
+        def per_wave(n):
+            v = mean(c, n)
+            return f"{v / waves:.0f}" if v is not None and waves else ""
+
+        wc, wa = mean(c, "SQ_WAVE_CYCLES"), mean(c, "SQ_WAIT_ANY")
+        li, bc = mean(c, "SQ_INSTS_LDS"), mean(c, "SQ_LDS_BANK_CONFLICT")
+        lines.append(f"| `{k}` | {g} | {frac('SQ_ACTIVE_INST_VALU')} | {frac('TA_TA_BUSY_sum')} | "
+                     f"{f'{wa / wc:.0%}' if wc and wa is not None else ''} | {per_wave('SQ_INSTS_VALU')} | "
+                     f"{per_wave('SQ_INSTS_SALU')} | {per_wave('SQ_INSTS_LDS')} | {per_wave('SQ_INSTS_VMEM_RD')} | "
+                     f"{f'{bc / li:.2f}' if li and bc is not None else ''} |")
+    lines += ["", f"kernel_source_sha256 {source_hash()}"]
+    Path(a.out).write_text("\n".join(lines) + "\n")
+    print(f"wrote {a.out}")
+
+
 def summary_is_current(path: Path) -> bool:
     """True if the summary was taken with the kernel sources as they are now."""
     if not Path(path).exists():
@@ -179,8 +240,16 @@ def main():
     r.add_argument("--cmd", required=True)
     r.add_argument("--title", default="soup256 (1M triangles, 256^3, 2560x1440)")
     r.add_argument("--out", required=True)
+    c = sub.add_parser("counters")
+    c.add_argument("--dirs", nargs="+", required=True)
+    c.add_argument("--kernels", nargs="*", default=None, help="kernel name prefixes to list (default: all)")
+    c.add_argument("--cus", type=int, default=256)
+    c.add_argument("--xcds", type=int, default=8)
+    c.add_argument("--cmd", required=True)
+    c.add_argument("--title", default="soup256 (1M triangles, 256^3, 2560x1440)")
+    c.add_argument("--out", required=True)
     a = ap.parse_args()
-    (cmd_times if a.what == "times" else cmd_traffic)(a)
+    {"times": cmd_times, "traffic": cmd_traffic, "counters": cmd_counters}[a.what](a)
 
 
 if __name__ == "__main__":
