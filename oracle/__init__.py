@@ -54,6 +54,9 @@ def load():
             lib.orc_voxelize_draws.restype = C.c_int64
             lib.orc_voxelize_draws.argtypes = [C.c_int, C.c_float, C.c_int, _FP, C.c_void_p, _FP, _FP, _UP,
                                                C.c_void_p, C.c_void_p, C.c_int, _UP, _FP, _DP, _UP]
+        if hasattr(lib, "orc_voxel_view_f32"):
+            lib.orc_voxel_view_f32.argtypes = [C.c_int, C.c_int, _FP, _FP, C.c_int, _FP, C.c_void_p, _FP, C.c_int,
+                                               C.c_int, C.c_void_p, C.c_float, C.c_int, C.c_float, _FP]
         lib.orc_rgba8_avg_step.restype = C.c_uint32
         lib.orc_rgba8_avg_step.argtypes = [C.c_uint32, _FP]
         lib.orc_set_round_mode.argtypes = [C.c_int]
@@ -278,6 +281,29 @@ def trace(dtype_float, res, levels, level0, mips, gpos, gnrm, mats, lights, cam,
        len(lights), None if pix is None else pix.ctypes.data_as(_I64P), npix, out.ctypes.data_as(P),
        C.cast(steps, _I64P))
     return out, ((steps[0], steps[1]) if early_exit_steps else steps[0])
+
+
+def voxel_view(W, H, view, proj, draws, res, levels, level0, mips, cube_size, view_dir, view_lod):
+    """The voxel view with GL's in-order SRC_ALPHA blending (orc_voxel_view_f32):
+    draws [(M, pos), ...] in submission order; level0 (res,res,res,4) and mips
+    dict (l, d) -> (s,s,s,4) decoded volume.  Returns the (H, W, 4) image."""
+    lib = load()
+    keep = [_c(level0, np.float32)]
+    ptrs = []
+    P = C.POINTER(C.c_float)
+    for l in range(levels):
+        for d in range(6):
+            a = keep[0] if l == 0 else _c(mips[(l, d)], np.float32)
+            keep.append(a)
+            ptrs.append(a.ctypes.data_as(P))
+    arr = (P * len(ptrs))(*ptrs)
+    Ms = np.ascontiguousarray(np.stack([np.asarray(d[0], np.float32).reshape(16) for d in draws]))
+    counts = np.array([np.asarray(d[1]).reshape(-1, 9).shape[0] for d in draws], np.int32)
+    pos = _c(np.concatenate([np.asarray(d[1], np.float32).reshape(-1) for d in draws]), np.float32)
+    out = np.zeros((H, W, 4), np.float32)
+    lib.orc_voxel_view_f32(W, H, _f(_c(view, np.float32)), _f(_c(proj, np.float32)), len(draws), _f(Ms),
+                           counts.ctypes.data, _f(pos), res, levels, arr, cube_size, view_dir, view_lod, _f(out))
+    return out
 
 
 def decode_rgba8(a: np.ndarray) -> np.ndarray:

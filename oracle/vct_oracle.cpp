@@ -1430,6 +1430,116 @@ void orc_trace_f64(const orc_trace_params* p, int res, int levels, const double*
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
+// the voxel view with GL's in-order blending
+// ---------------------------------------------------------------------------
+extern "C" {
+
+// Renderer::visualize with view_voxel_dir < 7 (renderer.cpp:355-390,
+// voxel_cone_tracing.frag:246-257): the draws in submission order (Ms: one
+// model matrix per draw, draw_ntri: triangles per draw, pos: their triangles
+// one draw after the other, each in index order), every triangle clipped and
+// drawn as its fan like the G-buffer raster.  A covered pixel's fragment is
+// depth tested GL_LESS against the depth written so far (equal depth fails;
+// depths outside [0, 1] are clipped); when it passes, its depth is written
+// and its colour textureLod(tex3D[dir], pos, lod) is blended
+// SRC_ALPHA / ONE_MINUS_SRC_ALPHA over the pixel, alpha included.  A
+// fragment outside within_cube (:250) writes its depth and returns before
+// writing a colour: its unwritten output blends with alpha 0, leaving the
+// pixel's colour.  The frame starts at the clear colour (renderer.cpp:398).
+// image: W x H x 4, row 0 at the bottom.  Rows are split over threads; each
+// thread walks every triangle in order over its own rows.
+void orc_voxel_view_f32(int W, int H, const float* view, const float* proj, int ndraw, const float* Ms,
+                        const int* draw_ntri, const float* pos, int res, int levels, const float* const* level_ptrs,
+                        float cube_size, int view_dir, float view_lod, float* image) {
+  float PV[16];
+  proj_view(view, proj, PV);
+  Volume<float> vol;
+  vol.res = res;
+  vol.levels = levels;
+  vol.lv.assign(level_ptrs, level_ptrs + levels * 6);
+  const int map = std::min(std::max(view_dir, 0), 5);
+  const float bg[4] = {0.15f, 0.25f, 0.25f, 1.0f};
+  std::vector<float> depth((size_t)W * H, INFINITY);
+  for (int64_t k = 0; k < (int64_t)W * H; ++k)
+    for (int c = 0; c < 4; ++c) image[k * 4 + c] = bg[c];
+  int nthreads = 1;
+#ifdef _OPENMP
+  nthreads = omp_get_max_threads();
+#endif
+  const int nband = std::max(1, std::min(H, 4 * nthreads));
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int band = 0; band < nband; ++band) {
+    const int j0 = (int)((int64_t)H * band / nband), j1 = (int)((int64_t)H * (band + 1) / nband);
+    const float* tp = pos;
+    for (int d = 0; d < ndraw; ++d) {
+      const float* M = Ms + 16 * (size_t)d;
+      for (int t = 0; t < draw_ntri[d]; ++t, tp += 9) {
+        float clip[3][4];
+        V3<float> wv[3];
+        for (int v = 0; v < 3; ++v) {
+          wv[v] = xform_point<float>(M, tp + v * 3);
+          clip_of(PV, wv[v], clip[v]);
+        }
+        ClipPoly P;
+        clip_triangle(clip, P);
+        const int nsub = P.clipped ? std::max(0, P.n - 2) : 1;
+        for (int k = 0; k < nsub; ++k) {
+          float xw[3], yw[3], z[3];
+          const ClipVert* sv[3];
+          if (!sub_window(P, k, W, H, xw, yw, z, sv)) continue;
+          Tri2D tri;
+          setup2d(xw, yw, W, H, tri);
+          if (!tri.ok || tri.jmax < j0 || tri.jmin >= j1) continue;
+          for_each_pixel(tri, W, H, [&](int i, int j, const int64_t* bw) {
+            if (j < j0 || j >= j1) return;
+            const float fa = (float)tri.area;
+            const float b[3] = {(float)bw[0] / fa, (float)bw[1] / fa, (float)bw[2] / fa};
+            const float dz = b[0] * z[0] + b[1] * z[1] + b[2] * z[2];
+            if (!(dz >= 0.0f && dz <= 1.0f)) return;
+            const size_t idx = (size_t)j * W + i;
+            if (!(dz < depth[idx])) return;  // GL_LESS
+            depth[idx] = dz;
+            // the world position as the G-buffer resolve interpolates it
+            float q[3], qs = 0.0f;
+            for (int v = 0; v < 3; ++v) {
+              q[v] = b[v] / sv[v]->c[3];
+              qs = qs + q[v];
+            }
+            for (int v = 0; v < 3; ++v) q[v] = q[v] / qs;
+            if (P.clipped) {
+              float lam[3];
+              for (int o = 0; o < 3; ++o) {
+                float a = q[0] * sv[0]->b[o];
+                a = a + q[1] * sv[1]->b[o];
+                a = a + q[2] * sv[2]->b[o];
+                lam[o] = a;
+              }
+              for (int o = 0; o < 3; ++o) q[o] = lam[o];
+            }
+            float wp[3] = {0, 0, 0};
+            for (int v = 0; v < 3; ++v) {
+              const float wc[3] = {wv[v].x, wv[v].y, wv[v].z};
+              for (int c = 0; c < 3; ++c) wp[c] = wp[c] + q[v] * wc[c];
+            }
+            const V3<float> bp = mk(wp[0] / cube_size * 0.5f + 0.5f, wp[1] / cube_size * 0.5f + 0.5f,
+                                    wp[2] / cube_size * 0.5f + 0.5f);
+            if (!(fabsf(bp.x) < 1.0f && fabsf(bp.y) < 1.0f && fabsf(bp.z) < 1.0f)) return;  // within_cube
+            float sm[4];
+            texture_lod(vol, map, bp, view_lod, sm);
+            float* o = image + idx * 4;
+            const float a = sm[3];
+            for (int c = 0; c < 3; ++c) o[c] = sm[c] * a + o[c] * (1.0f - a);
+            o[3] = a * a + o[3] * (1.0f - a);
+          });
+        }
+      }
+    }
+  }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
 // probes of single oracle functions (used by the pinning tests)
 // ---------------------------------------------------------------------------
 extern "C" {

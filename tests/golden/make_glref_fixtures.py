@@ -2,10 +2,10 @@
 
 Runs the reference's own shaders (read from /root/reference/shader at run
 time, never copied) under Mesa llvmpipe via oracle/glref, single-threaded so
-the output is reproducible, for two small scenes built from seeds by
+the output is reproducible, for small scenes built from seeds by
 tests/helpers.SceneSpec, and stores the outputs only: level 0 and the mip
-chain (RGBA8), the indirect-light image and the G-buffer (RGBA32F), with
-the llvmpipe pass times.
+chain (RGBA8), the frame and the G-buffer (RGBA32F), with the llvmpipe pass
+times; for the voxel views (view_voxel_dir < 7) the frame only.
 
     LP_NUM_THREADS=0 python tests/golden/make_glref_fixtures.py [scene ...]
 """
@@ -38,6 +38,15 @@ SCENES = {
     # a model matrix with rotation and non-uniform scale (voxelize.vert:24-30,
     # voxel_cone_tracing.vert:22-28)
     "soup64_xform": dict(kind="soup", res=64, ntri=1500, W=96, H=64, model="rot_scale"),
+}
+
+
+# The voxel view (view_voxel_dir < 7) of scenes above, image only (the volume
+# is the base scene's): name -> (base scene, view_voxel_dir, view_voxel_lod)
+VOXEL_VIEWS = {
+    "vview_cornell64_d0": ("cornell64", 0, 0.0),
+    "vview_cornell64_d3": ("cornell64", 3, 1.5),
+    "vview_cornell64_inside_d1": ("cornell64_inside", 1, 0.5),
 }
 
 
@@ -117,6 +126,19 @@ def main():
         np.savez_compressed(out, level0=r["level0"], mips=mips, image=r["image"], gpos=r["gpos"], gnrm=r["gnrm"],
                             times=r["times"], renderer=np.array(glref.renderer()))
         print(f"{out.name}: {out.stat().st_size / 1024:.0f} KiB, llvmpipe times {r['times']}")
+    for name, (base, vd, lod) in VOXEL_VIEWS.items():
+        if only and name not in only:
+            continue
+        kw = SCENES[base]
+        sc = SceneSpec(**kw)
+        if kw["kind"] == "soup":
+            soup_point_light(sc)
+        r = glref.frame(sc.res, sc.cube_size, sc.M, sc.view, sc.proj, sc.W, sc.H, sc.pos, sc.nrm, sc.tri_mat,
+                        sc.mat_records, sc.lights, phases=sc.phases, view_voxel_dir=vd, view_voxel_lod=lod)
+        out = HERE / f"glref_{name}.npz"
+        np.savez_compressed(out, image=r["image"], view_voxel_dir=vd, view_voxel_lod=lod, base=base,
+                            renderer=np.array(glref.renderer()))
+        print(f"{out.name}: {out.stat().st_size / 1024:.0f} KiB")
 
 
 if __name__ == "__main__":

@@ -80,10 +80,8 @@ def test_voxelize_reference_average_bit_exact(gpu_available, orc, kind, res, ntr
 
 
 def test_voxelize_reference_average_many_models(gpu_available, orc):
-    """37 queued models (more than one batch of 16 draws, so fragments of later
-    batches join voxels filled by earlier ones): the texels equal the oracle's
-    for the same triangles as one list, since the canonical order runs over
-    the queue."""
+    """37 queued models: the texels equal the oracle's for the same triangles
+    as one list, since the canonical order runs over the queue."""
     say("voxelize reference average over 37 models")
     sc = SceneSpec("soup", res=64, ntri=3000)
     cuts = np.linspace(0, 3000, 38).astype(int)
@@ -340,25 +338,68 @@ def test_trace_ragged_viewport_matches_oracle(gpu_available, orc):
     assert err <= 2e-3
 
 
-@pytest.mark.parametrize("view_dir,lod", [(0, 0.0), (3, 1.5), (5, 4.0)])
-def test_voxel_view_matches_oracle(gpu_available, orc, view_dir, lod):
-    """The voxel-view debug path (voxel_cone_tracing.frag:253-257)."""
-    say(f"voxel view dir={view_dir} lod={lod}")
+@pytest.mark.parametrize("view_dir,lod,extra,dtype", [
+    (0, 0.0, {}, L.RGBA8), (3, 1.5, {}, L.RGBA8), (5, 4.0, {}, L.FP16),
+    # the camera inside the box: clipped walls drawn as fans, in draw order
+    (1, 0.5, dict(eye=(0.12, 0.05, 0.2), target=(-0.25, -0.1, -0.5), fovy_deg=75.0), L.RGBA8)])
+def test_voxel_view_matches_oracle(gpu_available, orc, view_dir, lod, extra, dtype):
+    """The voxel-view debug path (voxel_cone_tracing.frag:253-257) with GL's
+    in-order SRC_ALPHA blending of every fragment that passes the depth test
+    (renderer.cpp:384-388), against the oracle's sequential draw."""
+    say(f"voxel view dir={view_dir} lod={lod} {extra} dtype={dtype}")
     W, H, res = 96, 64, 64
-    sc = SceneSpec("cornell", res=res, W=W, H=H)
+    sc = SceneSpec("cornell", res=res, W=W, H=H, **extra)
     with Context(W, H) as ctx:
-        sc.build(ctx, L.RGBA8)
+        sc.build(ctx, dtype)
         ctx.set_voxel_view(view_dir, lod)
         ctx.render()
+        # a second frame reuses the sort buffers
+        ctx.queue_model(sc.model)
+        for (p, c, i, k) in sc.lights:
+            ctx.queue_light(p, c, i, k)
+        ctx.render()
         img = ctx.download_image()
-        gp, gn = ctx.download_gbuffer()
+        l0 = decode_level(ctx.download_level(0), dtype)
+        mips = {(l, d): decode_level(ctx.download_level(l, d), dtype) for l in range(1, 7) for d in range(6)}
+    want = orc.voxel_view(W, H, sc.view, sc.proj, [(sc.M, sc.pos)], res, 7, l0, mips, sc.cube_size, view_dir, lod)
+    err = np.abs(img - want).max()
+    say(f"  max-abs vs FP32 oracle = {err:.3e}")
+    assert err == 0.0  # the same float operations in the same order (measured bit-exact)
+
+
+def test_voxel_view_over_many_models(gpu_available, orc):
+    """Draw order runs over the queued models (renderer.cpp:242-256): the
+    Cornell box split into 5 models queued in reverse, two of them moved."""
+    say("voxel view over 5 models")
+    from helpers import model_matrix
+
+    W, H, res = 96, 64, 64
+    sc = SceneSpec("cornell", res=res, W=W, H=H)
+    cuts = np.linspace(0, sc.pos.shape[0], 6).astype(int)
+    draws = []
+    with Context(W, H) as ctx:
+        sc.build(ctx, L.RGBA8)
+        ctx.clear_queues()
+        models = []
+        for k, (a, b) in enumerate(zip(cuts[:-1], cuts[1:])):
+            m = ctx.model_create(sc.pos[a:b], sc.nrm[a:b], sc.tri_mat[a:b])
+            M = model_matrix("rot_scale") if k in (1, 3) else sc.M
+            ctx.set_model_transform(m, M)
+            models.append((m, M, sc.pos[a:b]))
+        for m, M, pos in reversed(models):
+            ctx.queue_model(m)
+            draws.append((M, pos))
+        for (p, c, i, k) in sc.lights:
+            ctx.queue_light(p, c, i, k)
+        ctx.set_voxel_view(2, 0.7)
+        ctx.render()
+        img = ctx.download_image()
         l0 = decode_level(ctx.download_level(0), L.RGBA8)
         mips = {(l, d): decode_level(ctx.download_level(l, d), L.RGBA8) for l in range(1, 7) for d in range(6)}
-    want, _ = orc.trace(np.float32, res, 7, l0, mips, gp, gn, sc.mat_records, sc.lights, sc.cam, sc.cube_size,
-                        cone_set=sc.cone_set, phases=sc.phases, view_dir=view_dir, view_lod=lod)
-    err = np.abs(img.reshape(-1, 4) - want).max()
+    want = orc.voxel_view(W, H, sc.view, sc.proj, draws, res, 7, l0, mips, sc.cube_size, 2, 0.7)
+    err = np.abs(img - want).max()
     say(f"  max-abs vs FP32 oracle = {err:.3e}")
-    assert err <= 1e-4
+    assert err == 0.0  # the same float operations in the same order (measured bit-exact)
 
 
 @pytest.mark.parametrize("dtype", [L.RGBA8, L.FP16, L.BF16, L.FP32])

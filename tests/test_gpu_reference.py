@@ -107,6 +107,34 @@ def test_mip_against_reference_shader(gpu_available, name):
     assert worst <= 1 and ndiff < 0.02 * n
 
 
+@pytest.mark.parametrize("name", ["vview_cornell64_d0", "vview_cornell64_d3", "vview_cornell64_inside_d1"])
+def test_voxel_view_against_reference_shader(gpu_available, name):
+    """The voxel view (view_voxel_dir < 7): the GPU rasterises the scene and
+    blends every fragment that passes the depth test in draw order
+    (renderer.cpp:384-388), sampling the reference's own volume, against the
+    reference's frame under llvmpipe.  llvmpipe filters RGBA8 texels with
+    fixed-point weights; the oracle on the same inputs measured max 6.0e-3,
+    99.9th percentile 2.0e-3."""
+    v = np.load(GOLD / f"glref_{name}.npz")
+    base = str(v["base"])
+    sc, g = _scene(base)
+    vd, lod = int(v["view_voxel_dir"]), float(v["view_voxel_lod"])
+    print(f"voxel view vs llvmpipe: {name} (dir {vd}, lod {lod})", flush=True)
+    ref_mips = _split_mips(g["mips"], sc.res)
+    with Context(sc.W, sc.H) as ctx:
+        sc.build(ctx, L.RGBA8, L.AVG_REFERENCE)
+        ctx.gbuffer()
+        ctx.upload_level(0, 0, g["level0"])
+        for (l, d), a in ref_mips.items():
+            ctx.upload_level(l, d, a)
+        ctx.set_voxel_view(vd, lod)
+        ctx.trace()
+        img = ctx.download_image()
+    d = np.abs(img - v["image"]).reshape(-1, 4).max(1)
+    print(f"  max {d.max():.2e}, 99.9th percentile {np.percentile(d, 99.9):.2e}, above 1e-3 {(d > 1e-3).mean():.4f}")
+    assert d.max() <= 1e-2 and np.percentile(d, 99.9) <= 3e-3 and (d > 1e-3).mean() <= 0.01
+
+
 @pytest.mark.parametrize("name", list(SCENES))
 def test_trace_against_reference_shader(gpu_available, name):
     """voxel_cone_tracing.frag (indirect light) on the reference's own volume

@@ -28,6 +28,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <hipcub/hipcub.hpp>
+
 #include "vct_internal.h"
 
 namespace vct {
@@ -496,26 +498,19 @@ __global__ __launch_bounds__(256) void g_fan_write_kernel(GParams p, const GRec*
   }
 }
 
-// Perspective-correct world position, normal, linear depth and material id of
-// pixel (i, j) from its winning key; writes the G-buffer texel.
-__device__ __forceinline__ void g_resolve_pixel(const GParams& p, const GRec* __restrict__ recs,
-                                                const GRec* __restrict__ fan_recs, const GFan* __restrict__ fans,
-                                                unsigned long long key, int i, int j, float4* __restrict__ gpos,
-                                                float4* __restrict__ gnrm) {
+// Perspective-correct interpolation weights q of the original vertices of
+// triangle id (frame triangle << 3 | fan triangle) at pixel (i, j), as the
+// raster drew it: its draw, stored index, and sum(b_k / w_k) (1 / qs is the
+// interpolated clip w).
+__device__ __forceinline__ const GDraw& g_weights(const GParams& p, const GRec* __restrict__ recs,
+                                                  const GRec* __restrict__ fan_recs, const GFan* __restrict__ fans,
+                                                  uint32_t id, int i, int j, uint32_t& lt, float (&q)[3], float& qs) {
 #pragma clang fp contract(off)
-  const size_t idx = (size_t)j * p.W + i;
-  if (key == ~0ull) {
-    gpos[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
-    gnrm[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
-    return;
-  }
-  const uint32_t id = (uint32_t)(key & 0xffffffffu);
   const uint32_t t = id >> 3;  // original index
   const int d = g_find_draw(p, t);
   const GDraw& dr = p.draws[d];
-  const uint32_t lt = dr.inv[t - dr.tri_base];  // stored index
+  lt = dr.inv[t - dr.tri_base];  // stored index
   const float* pp = dr.pos + (size_t)lt * 9;
-  const float* nn = dr.nrm + (size_t)lt * 9;
   // the snapped vertices, orientation and area the raster used, from the
   // triangle's record (or its fan triangle's)
   const GRec& rec = recs[dr.tri_base + lt];
@@ -545,7 +540,7 @@ __device__ __forceinline__ void g_resolve_pixel(const GParams& p, const GRec* __
     b[0] = b[1] = b[2] = 1.0f / 3.0f;
   }
   // perspective-correct weights: (b_i / w_i) / sum(b_k / w_k) over the drawn triangle
-  float q[3], qs = 0.0f;
+  qs = 0.0f;
   for (int v = 0; v < 3; ++v) {
     q[v] = b[v] / cw[v];
     qs = qs + q[v];
@@ -562,6 +557,27 @@ __device__ __forceinline__ void g_resolve_pixel(const GParams& p, const GRec* __
     }
     for (int o = 0; o < 3; ++o) q[o] = lam[o];
   }
+  return dr;
+}
+
+// Perspective-correct world position, normal, linear depth and material id of
+// pixel (i, j) from its winning key; writes the G-buffer texel.
+__device__ __forceinline__ void g_resolve_pixel(const GParams& p, const GRec* __restrict__ recs,
+                                                const GRec* __restrict__ fan_recs, const GFan* __restrict__ fans,
+                                                unsigned long long key, int i, int j, float4* __restrict__ gpos,
+                                                float4* __restrict__ gnrm) {
+#pragma clang fp contract(off)
+  const size_t idx = (size_t)j * p.W + i;
+  if (key == ~0ull) {
+    gpos[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+    gnrm[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+  uint32_t lt;
+  float q[3], qs;
+  const GDraw& dr = g_weights(p, recs, fan_recs, fans, (uint32_t)(key & 0xffffffffu), i, j, lt, q, qs);
+  const float* pp = dr.pos + (size_t)lt * 9;
+  const float* nn = dr.nrm + (size_t)lt * 9;
   float wp[3] = {0.f, 0.f, 0.f}, n[3] = {0.f, 0.f, 0.f};
   for (int v = 0; v < 3; ++v) {
     float w[3];
@@ -772,6 +788,170 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
   }
 }
 
+// ---- the voxel view: every fragment blended in draw order ---------------------
+// Renderer::visualize with view_voxel_dir < 7 draws the models with GL_LESS
+// depth and SRC_ALPHA / ONE_MINUS_SRC_ALPHA blending (renderer.cpp:355-390);
+// the fragment shader returns textureLod(tex3D[dir], pos, lod)
+// (voxel_cone_tracing.frag:253-257).  So a pixel shows, blended in draw order
+// over the clear colour, every fragment nearer than all drawn before it.  The
+// tile lists of the G-buffer raster are sorted into draw order (the records'
+// triangle << 3 | fan key), then one workgroup per tile walks its triangles
+// in that order; each lane owns 4 pixels of the tile (depth and colour in
+// registers), so no ordering between lanes is needed.
+struct VViewVol {
+  const char* base;  // the volume (plain texels, Morton order per level)
+  uint64_t off[16][6];
+  int res, levels, dtype;
+};
+
+__device__ __forceinline__ float4 vv_texel(const VViewVol& V, int l, int map, int x, int y, int z) {
+  const char* lv = V.base + V.off[l][l == 0 ? 0 : map];
+  const uint32_t e = brick_index((uint32_t)x, (uint32_t)y, (uint32_t)z);
+  if (V.dtype == VCT_RGBA8) {
+    const uint32_t v = reinterpret_cast<const uint32_t*>(lv)[e];
+    return make_float4(unorm8_exact(v & 255u), unorm8_exact((v >> 8) & 255u), unorm8_exact((v >> 16) & 255u),
+                       unorm8_exact(v >> 24));
+  }
+  if (V.dtype == VCT_FP16 || V.dtype == VCT_BF16) {
+    const uint2 v = reinterpret_cast<const uint2*>(lv)[e];
+    if (V.dtype == VCT_FP16)
+      return make_float4(f16_to_f32((uint16_t)(v.x & 0xffffu)), f16_to_f32((uint16_t)(v.x >> 16)),
+                         f16_to_f32((uint16_t)(v.y & 0xffffu)), f16_to_f32((uint16_t)(v.y >> 16)));
+    return make_float4(bf16_to_f32((uint16_t)(v.x & 0xffffu)), bf16_to_f32((uint16_t)(v.x >> 16)),
+                       bf16_to_f32((uint16_t)(v.y & 0xffffu)), bf16_to_f32((uint16_t)(v.y >> 16)));
+  }
+  return reinterpret_cast<const float4*>(lv)[e];
+}
+
+// One level, trilinear with a zero border (CLAMP_TO_BORDER, texture_3d.cpp:10-15).
+__device__ __forceinline__ float4 vv_trilinear(const VViewVol& V, int l, int map, float px, float py, float pz) {
+#pragma clang fp contract(off)
+  const int s = V.res >> l;
+  const float u = px * (float)s - 0.5f, v = py * (float)s - 0.5f, w = pz * (float)s - 0.5f;
+  const float fu = floorf(u), fv = floorf(v), fw = floorf(w);
+  const int i0 = (int)fu, j0 = (int)fv, k0 = (int)fw;
+  const float a = u - fu, b = v - fv, c = w - fw;
+  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx) {
+        const int x = i0 + dx, y = j0 + dy, z = k0 + dz;
+        if (x < 0 || y < 0 || z < 0 || x >= s || y >= s || z >= s) continue;
+        const float wt = (dx ? a : 1.0f - a) * (dy ? b : 1.0f - b) * (dz ? c : 1.0f - c);
+        const float4 t = vv_texel(V, l, map, x, y, z);
+        out.x = out.x + wt * t.x;
+        out.y = out.y + wt * t.y;
+        out.z = out.z + wt * t.z;
+        out.w = out.w + wt * t.w;
+      }
+  return out;
+}
+
+// textureLod with LINEAR_MIPMAP_LINEAR, lod clamped to [0, levels - 1].
+__device__ __forceinline__ float4 vv_texture_lod(const VViewVol& V, int map, float px, float py, float pz,
+                                                 float lod) {
+#pragma clang fp contract(off)
+  const float lmax = (float)(V.levels - 1);
+  if (!(lod > 0.0f)) lod = 0.0f;
+  if (lod > lmax) lod = lmax;
+  const int d1 = (int)floorf(lod);
+  const float f = lod - (float)d1;
+  const float4 a = vv_trilinear(V, d1, map, px, py, pz);
+  if (!(f > 0.0f && d1 + 1 < V.levels)) return a;
+  const float4 b = vv_trilinear(V, d1 + 1, map, px, py, pz);
+  const float g = 1.0f - f;
+  return make_float4(g * a.x + f * b.x, g * a.y + f * b.y, g * a.z + f * b.z, g * a.w + f * b.w);
+}
+
+// Keys (the record's triangle << 3 | fan) and values (list entries) of every
+// tile's list, and the end of each tile's segment, for the sort.
+__global__ __launch_bounds__(256) void g_vview_keys_kernel(const uint32_t* __restrict__ tile_count,
+                                                           const uint32_t* __restrict__ tile_off,
+                                                           const uint32_t* __restrict__ list,
+                                                           const GRec* __restrict__ recs,
+                                                           const GRec* __restrict__ fan_recs, uint32_t ntri,
+                                                           uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                           uint32_t* __restrict__ ends) {
+  const uint32_t tile = blockIdx.x, n = tile_count[tile], base = tile_off[tile];
+  if (threadIdx.x == 0) ends[tile] = base + n;
+  for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+    const uint32_t e = list[base + k];
+    keys[base + k] = (e < ntri ? recs[e] : fan_recs[e - ntri]).otri;
+    vals[base + k] = e;
+  }
+}
+
+struct VViewArgs {
+  GParams p;
+  const GRec* recs;
+  const GRec* fan_recs;
+  const GFan* fans;
+  const uint32_t* tile_count;
+  const uint32_t* tile_off;
+  const uint32_t* order;  // each tile's list entries in draw order
+  int tiles_x;
+  VViewVol V;
+  int map;
+  float lod, cube_size;
+  float4* image;
+};
+
+__global__ __launch_bounds__(256) void g_vview_kernel(VViewArgs A) {
+#pragma clang fp contract(off)
+  const int tid = threadIdx.x;
+  const int tile = blockIdx.x;
+  const int x0 = (tile % A.tiles_x) * kTile, y0 = (tile / A.tiles_x) * kTile;
+  constexpr int kPer = kTile * kTile / 256;  // pixels per lane
+  float dep[kPer];
+  float4 col[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    dep[q] = __builtin_inff();
+    col[q] = make_float4(0.15f, 0.25f, 0.25f, 1.0f);  // clear colour, renderer.cpp:398
+  }
+  const uint32_t n = A.tile_count[tile], base = A.tile_off[tile];
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t e = A.order[base + k];
+    const GRec r = e < A.p.ntri ? A.recs[e] : A.fan_recs[e - A.p.ntri];
+    if (r.i0 > r.i1) continue;
+    const GSetup s = g_setup_of(r);
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int li = (tid + 256 * q) % kTile, lj = (tid + 256 * q) / kTile;
+      const int i = x0 + li, j = y0 + lj;
+      if (i < r.i0 || i > r.i1 || j < r.j0 || j > r.j1) continue;
+      float b0, b1, b2;
+      if (!g_cover(s, i, j, b0, b1, b2)) continue;
+      const float depth = b0 * r.z[0] + b1 * r.z[1] + b2 * r.z[2];
+      if (!(depth >= 0.0f && depth <= 1.0f) || !(depth < dep[q])) continue;  // GL_LESS
+      dep[q] = depth;
+      uint32_t lt;
+      float w[3], qs;
+      const GDraw& dr = g_weights(A.p, A.recs, A.fan_recs, A.fans, r.otri, i, j, lt, w, qs);
+      const float* pp = dr.pos + (size_t)lt * 9;
+      float wp[3] = {0.f, 0.f, 0.f};
+      for (int v = 0; v < 3; ++v) {
+        float wv[3];
+        world_of(dr, pp + 3 * v, wv);
+        for (int c = 0; c < 3; ++c) wp[c] = wp[c] + w[v] * wv[c];
+      }
+      const float bx = wp[0] / A.cube_size * 0.5f + 0.5f, by = wp[1] / A.cube_size * 0.5f + 0.5f,
+                  bz = wp[2] / A.cube_size * 0.5f + 0.5f;
+      // within_cube (:250): outside, the fragment writes its depth and no colour
+      if (!(fabsf(bx) < 1.0f && fabsf(by) < 1.0f && fabsf(bz) < 1.0f)) continue;
+      const float4 sm = vv_texture_lod(A.V, A.map, bx, by, bz, A.lod);
+      const float a = sm.w, g = 1.0f - a;
+      col[q] = make_float4(sm.x * a + col[q].x * g, sm.y * a + col[q].y * g, sm.z * a + col[q].z * g,
+                           a * a + col[q].w * g);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int i = x0 + (tid + 256 * q) % kTile, j = y0 + (tid + 256 * q) / kTile;
+    if (i < A.p.W && j < A.p.H) A.image[(size_t)j * A.p.W + i] = col[q];
+  }
+}
+
 // workgroups of the kernels that stride over the clipped triangles
 constexpr int kFanGrid = 64;
 
@@ -922,5 +1102,75 @@ int vct_launch_gbuffer_phase(vct_context* ctx, int phase) {
                      tiles_x, (float4*)ctx->g_pos.p, (float4*)ctx->g_nrm.p);
   VCT_HIP(hipGetLastError());
   ctx->last_gbuffer_refs = total;
+  ctx->g_frame.valid = true;
+  ctx->g_frame.W = W;
+  ctx->g_frame.H = H;
+  ctx->g_frame.ndraw = p.ndraw;
+  ctx->g_frame.ntri = p.ntri;
+  ctx->g_frame.nfans = nfans;
+  ctx->g_frame.total = total;
+  memcpy(ctx->g_frame.PV, p.PV, sizeof(p.PV));
+  return VCT_OK;
+}
+
+// The voxel view of the last rasterised G-buffer, blended in draw order: the
+// tile lists sorted by draw order (a segmented radix sort, one segment per
+// tile), then one workgroup per tile.
+int vct_launch_voxel_view(vct_context* ctx) {
+  const auto& F = ctx->g_frame;
+  const int W = F.W, H = F.H;
+  const int tiles_x = (W + kTile - 1) / kTile, tiles_y = (H + kTile - 1) / kTile;
+  const uint32_t ntiles = (uint32_t)(tiles_x * tiles_y);
+  const uint64_t total = F.total;
+  if (total > 0xffffffffull) return vct_set_error(VCT_ERR_LIMIT, "voxel view: more than 2^32 - 1 tile entries");
+  int rc;
+  const size_t n4 = (size_t)(total ? total : 1) * 4;
+  if ((rc = vct_ensure(ctx->vv_keys, 2 * n4, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->vv_vals, 2 * n4, ctx))) return rc;
+  if ((rc = vct_ensure(ctx->vv_ends, (size_t)ntiles * 4, ctx))) return rc;
+  uint32_t* keys = (uint32_t*)ctx->vv_keys.p;
+  uint32_t* vals = (uint32_t*)ctx->vv_vals.p;
+  const GRec* fan_recs = (const GRec*)ctx->g_fans.p;
+  const GFan* fans = (const GFan*)((const char*)ctx->g_fans.p + (size_t)F.nfans * sizeof(GRec));
+  hipLaunchKernelGGL(g_vview_keys_kernel, dim3(ntiles), dim3(256), 0, ctx->stream, (const uint32_t*)ctx->g_count.p,
+                     (const uint32_t*)ctx->g_off.p, (const uint32_t*)ctx->g_list.p, (const GRec*)ctx->g_recs.p,
+                     fan_recs, F.ntri, keys, vals, (uint32_t*)ctx->vv_ends.p);
+  VCT_HIP(hipGetLastError());
+  size_t temp = 0;
+  VCT_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, temp, keys, keys + total, vals, vals + total,
+                                                       (int)total, (int)ntiles, (const uint32_t*)ctx->g_off.p,
+                                                       (const uint32_t*)ctx->vv_ends.p, 0, 32, ctx->stream));
+  if ((rc = vct_ensure(ctx->vv_temp, temp ? temp : 16, ctx))) return rc;
+  if (total)
+    VCT_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(ctx->vv_temp.p, temp, keys, keys + total, vals, vals + total,
+                                                         (int)total, (int)ntiles, (const uint32_t*)ctx->g_off.p,
+                                                         (const uint32_t*)ctx->vv_ends.p, 0, 32, ctx->stream));
+  VViewArgs A;
+  memset(&A, 0, sizeof(A));
+  A.p.draws = (const GDraw*)ctx->g_draws.dev.p;
+  A.p.ndraw = F.ndraw;
+  memcpy(A.p.PV, F.PV, sizeof(A.p.PV));
+  A.p.ntri = F.ntri;
+  A.p.W = W;
+  A.p.H = H;
+  A.recs = (const GRec*)ctx->g_recs.p;
+  A.fan_recs = fan_recs;
+  A.fans = fans;
+  A.tile_count = (const uint32_t*)ctx->g_count.p;
+  A.tile_off = (const uint32_t*)ctx->g_off.p;
+  A.order = vals + total;
+  A.tiles_x = tiles_x;
+  A.V.base = (const char*)ctx->volume.p;
+  for (int l = 0; l < ctx->levels && l < 16; ++l)
+    for (int d = 0; d < 6; ++d) A.V.off[l][d] = ctx->lvl_off[l][d];
+  A.V.res = (int)ctx->res;
+  A.V.levels = ctx->levels;
+  A.V.dtype = ctx->dtype;
+  A.map = ctx->view_voxel_dir < 0 ? 0 : (ctx->view_voxel_dir > 5 ? 5 : ctx->view_voxel_dir);
+  A.lod = ctx->view_voxel_lod;
+  A.cube_size = ctx->cube_size;
+  A.image = (float4*)ctx->image.p;
+  hipLaunchKernelGGL(g_vview_kernel, dim3(ntiles), dim3(256), 0, ctx->stream, A);
+  VCT_HIP(hipGetLastError());
   return VCT_OK;
 }

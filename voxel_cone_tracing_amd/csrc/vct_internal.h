@@ -247,6 +247,17 @@ struct vct_context {
   VctBuf g_fans;                   // records and resolve data of clipped triangles' fan triangles
   VctBuf g_clip;                   // the frame's triangles that need clipping
   uint64_t last_gbuffer_refs = 0;  // (triangle, tile) pairs of the last G-buffer
+  // what the last rasterised G-buffer left for the voxel view's in-order
+  // composite (its tile lists, records and draw table); invalid after an
+  // uploaded G-buffer
+  struct {
+    bool valid = false;
+    int W = 0, H = 0, ndraw = 0;
+    uint32_t ntri = 0, nfans = 0;
+    uint64_t total = 0;
+    float PV[16];
+  } g_frame;
+  VctBuf vv_keys, vv_vals, vv_ends, vv_temp;  // the voxel view's draw-order sort of the tile lists
 
   std::vector<VctTex3D> textures;
 
@@ -291,6 +302,7 @@ void vct_upload_free(VctUpload& u);
 // kernels launchers
 int vct_launch_mip(vct_context* ctx);
 int vct_ensure_pairs(vct_context* ctx);  // x-paired copy of the volume for the marcher
+int vct_launch_voxel_view(vct_context* ctx);  // the voxel view blended in draw order (last G-buffer's lists)
 int vct_frame_front(vct_context* ctx);   // voxelize + mip beside the G-buffer, joined
 int vct_launch_voxelize(vct_context* ctx);
 int vct_launch_gbuffer(vct_context* ctx);

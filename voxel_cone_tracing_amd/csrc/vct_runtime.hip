@@ -431,7 +431,8 @@ int vct_destroy(vct_ctx ctx) {
                     &ctx->train_valid, &ctx->probe_pix, &ctx->probe_val, &ctx->fb_list, &ctx->fb_pix,
                     &ctx->fb_list2, &ctx->fb_pix2, &ctx->fb_flag, &ctx->half_img, &ctx->probe_pix2,
                     &ctx->probe_val2, &ctx->pairs, &ctx->fb_est, &ctx->g_fans, &ctx->g_clip,
-                    &ctx->train_perm, &ctx->batch_X, &ctx->batch_A, &ctx->batch_Y, &ctx->vox_box};
+                    &ctx->train_perm, &ctx->batch_X, &ctx->batch_A, &ctx->batch_Y, &ctx->vox_box,
+                    &ctx->vv_keys, &ctx->vv_vals, &ctx->vv_ends, &ctx->vv_temp};
   for (VctBuf* b : bufs) free_buf(*b);
   vct_upload_free(ctx->g_draws);
   vct_upload_free(ctx->v_draws);
@@ -832,6 +833,7 @@ int vct_gbuffer_upload(vct_ctx ctx, const float* pos_depth, const float* normal_
   VCT_HIP(hipMemcpyAsync(ctx->g_pos.p, pos_depth, pixels * 16, hipMemcpyHostToDevice, ctx->stream));
   VCT_HIP(hipMemcpyAsync(ctx->g_nrm.p, normal_mat, pixels * 16, hipMemcpyHostToDevice, ctx->stream));
   VCT_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->g_frame.valid = false;  // no triangles behind this surface buffer
   return VCT_OK;
 }
 

@@ -1229,6 +1229,12 @@ int vct_launch_trace(vct_context* ctx) {
     return vct_set_error(VCT_ERR_STATE, "trace needs a G-buffer of the current viewport (run vct_gbuffer first)");
   int rc;
   if ((rc = vct_ensure(ctx->image, npix * 16, ctx))) return rc;
+  // the voxel view blends every fragment in draw order (renderer.cpp:384-388);
+  // from a rasterised G-buffer its triangles are known, from an uploaded one
+  // only the surface is, which is then blended over the background
+  if (ctx->view_voxel_dir < 7 && ctx->g_frame.valid && ctx->g_frame.W == ctx->width &&
+      ctx->g_frame.H == ctx->height)
+    return vct_launch_voxel_view(ctx);
   return vct_trace_into(ctx, (const float4*)ctx->g_pos.p, (const float4*)ctx->g_nrm.p, (float4*)ctx->image.p,
                         ctx->width, ctx->height, 0, nullptr);
 }
