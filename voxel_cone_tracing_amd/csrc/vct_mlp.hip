@@ -154,8 +154,14 @@ __device__ int pixel_features(const FeatArgs& F, int px, int py, float* f, float
   return kFeatures;
 }
 
+// Row stride (BF16 elements) of the activations a wave keeps in LDS: 64
+// values padded to 72, so the 16 rows one MFMA operand load reads start 16
+// banks apart instead of all on the same bank (a 128-byte stride put the
+// inference kernel at 30 LDS bank conflicts per LDS instruction).
+constexpr int kXS = 72;
+
 // ---- forward of one wave's 64 samples held in LDS ---------------------------------
-// xs: [64 samples][64] BF16 (features zero padded).  Layers 1 and 2 are
+// xs: [64 samples][kXS] BF16 (features zero padded to 64).  Layers 1 and 2 are
 // computed per 16-sample column tile and written back over that tile's own
 // input rows, which no other tile reads (the tile's MFMAs have consumed them
 // before the write; one wave, in order).  Returns y (C layout of the 16-row
@@ -170,7 +176,7 @@ __device__ __forceinline__ void forward_wave(const MlpDev& P, uint16_t* xs, floa
       floatx4 acc[4];
       for (int mt = 0; mt < 4; ++mt) {
         acc[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
-        acc[mt] = mfma_tile(Wp + mt * 16 * 64, 64, xs + nt * 16 * 64, 64, 64, acc[mt]);
+        acc[mt] = mfma_tile(Wp + mt * 16 * 64, 64, xs + nt * 16 * kXS, kXS, 64, acc[mt]);
       }
       for (int mt = 0; mt < 4; ++mt) {
         uint16_t o[4];
@@ -179,13 +185,13 @@ __device__ __forceinline__ void forward_wave(const MlpDev& P, uint16_t* xs, floa
           o[i] = bf16_bits(h);
         }
         uint2 pk = make_uint2((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16));
-        *reinterpret_cast<uint2*>(xs + (nt * 16 + col) * 64 + mt * 16 + rq) = pk;
+        *reinterpret_cast<uint2*>(xs + (nt * 16 + col) * kXS + mt * 16 + rq) = pk;
       }
     }
   }
   for (int nt = 0; nt < 4; ++nt) {
     floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
-    acc = mfma_tile(P.packed + kPW3, 64, xs + nt * 16 * 64, 64, 64, acc);
+    acc = mfma_tile(P.packed + kPW3, 64, xs + nt * 16 * kXS, kXS, 64, acc);
     for (int i = 0; i < 4; ++i) acc[i] += (rq + i < kOut) ? P.b3[rq + i] : 0.0f;
     y[nt] = acc;
   }
@@ -209,7 +215,7 @@ struct InferArgs {
 };
 
 __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs A) {
-  __shared__ __attribute__((aligned(16))) uint16_t s_x[256 * 64];
+  __shared__ __attribute__((aligned(16))) uint16_t s_x[256 * kXS];
   __shared__ float s_base[256][3];
   __shared__ float s_alb[256][3];
   __shared__ int s_st[256];
@@ -243,7 +249,7 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs A) {
     if (st == kTraceIt)
       A.trace_list[1 + b0 + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)(py * A.F.W + px);
   }
-  uint16_t* row = s_x + t * 64;
+  uint16_t* row = s_x + t * kXS;
   const bool feat = st == kFeatures;
   for (int k = 0; k < 64; k += 8) {
     uint32_t w[4];
@@ -264,7 +270,7 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs A) {
   s_st[t] = st;
   __syncthreads();
   floatx4 y[4];
-  forward_wave(A.P, s_x + wave * 64 * 64, y);
+  forward_wave(A.P, s_x + wave * 64 * kXS, y);
   // lanes 0..15 hold rows 0..3 (outputs 0..2) of sample column lane
   if (lane < 16) {
     for (int nt = 0; nt < 4; ++nt) {
@@ -321,19 +327,19 @@ __global__ __launch_bounds__(256) void mlp_sample_kernel(FeatArgs F, const float
 // ---- forward on given features (tests and evaluation) -------------------------------
 __global__ __launch_bounds__(256) void mlp_forward_kernel(MlpDev P, const uint16_t* __restrict__ X, uint32_t n,
                                                           float* __restrict__ Y) {
-  __shared__ __attribute__((aligned(16))) uint16_t s_x[256 * 64];
+  __shared__ __attribute__((aligned(16))) uint16_t s_x[256 * kXS];
   const int t = threadIdx.x;
   const uint32_t base = blockIdx.x * 256u;
   {
     const uint32_t k = base + t;
     const uint4* src = reinterpret_cast<const uint4*>(X + (size_t)k * 64);
-    uint4* dst = reinterpret_cast<uint4*>(s_x + t * 64);
+    uint4* dst = reinterpret_cast<uint4*>(s_x + t * kXS);
     for (int q = 0; q < 8; ++q) dst[q] = k < n ? src[q] : make_uint4(0, 0, 0, 0);
   }
   __syncthreads();
   const int wave = t >> 6, lane = t & 63;
   floatx4 y[4];
-  forward_wave(P, s_x + wave * 64 * 64, y);
+  forward_wave(P, s_x + wave * 64 * kXS, y);
   if (lane < 16)
     for (int nt = 0; nt < 4; ++nt) {
       const uint32_t k = base + wave * 64 + nt * 16 + lane;
@@ -719,14 +725,19 @@ __global__ __launch_bounds__(256) void tile_estimate_kernel(const float4* __rest
 
 // One workgroup: the tiles to trace in full, largest estimated error first,
 // until the error left in the others fits the frame's budget,
-//   sum_{not picked} est <= 3 * max_mse * kBudgetShare * sum ncov
+//   sum_{not picked} est <= 3 * max_mse * share * sum ncov
 // (3 channels; each covered probe stands for the same share of the tile's
-// covered pixels).  The estimates are binned by their float exponent and two
+// covered pixels).  The rest of the budget is margin for the estimate: 16
+// probes per tile estimate a tile's error loosely, and the sum over n tiles
+// with covered probes errs relatively by ~1/sqrt(n), so
+//   share = clamp(0.55 - 1.7 / sqrt(n), 0.25, 0.55).
+// The constant is set by the smallest frame measured: at ~40 covered tiles a
+// share of 0.5 ended at 38.6 dB and 0.3 at 40.9 dB for a 40 dB bound.  The estimates are binned by their float exponent and two
 // mantissa bits (4 bins per octave) and summed in 2^-40 fixed point, so the
 // pick is deterministic; every tile of the threshold bin is picked.
 // flags[t] = 1 picked, 0 not; out = [count, picked tiles...].
 constexpr int kBudgetBins = 256;
-constexpr float kBudgetShare = 0.3f;  // 16 probes per tile estimate the error loosely: most of the budget is margin
+constexpr float kShareMax = 0.55f, kShareMin = 0.25f, kShareSpread = 1.7f;
 
 __device__ __forceinline__ int budget_bin(float e) {
   if (!(e > 0.0f)) return -1;  // no error: never picked
@@ -740,7 +751,7 @@ __global__ __launch_bounds__(1024) void tile_budget_kernel(const float* __restri
                                                            uint32_t* __restrict__ flags) {
   __shared__ unsigned long long s_mass[kBudgetBins];
   __shared__ unsigned long long s_total;
-  __shared__ unsigned int s_cov, s_count;
+  __shared__ unsigned int s_cov, s_count, s_tiles;
   __shared__ int s_thr;
   const int tid = threadIdx.x;
   const float kFix = 1099511627776.0f;  // 2^40
@@ -749,17 +760,19 @@ __global__ __launch_bounds__(1024) void tile_budget_kernel(const float* __restri
     s_total = 0ull;
     s_cov = 0u;
     s_count = 0u;
+    s_tiles = 0u;
   }
   __syncthreads();
   {
     // per-bin masses by LDS atomics; the totals summed per wave first (exact
     // integers, so the order does not matter)
     unsigned long long tot = 0ull;
-    uint32_t cov = 0u;
+    uint32_t cov = 0u, tiles = 0u;
     for (uint32_t t = tid; t < ntiles; t += 1024) {
       const float e = est[t];
       const int b = budget_bin(e);
       cov += ncov[t];
+      tiles += ncov[t] ? 1u : 0u;
       if (b >= 0) {
         const unsigned long long q = (unsigned long long)(e * kFix);
         atomicAdd(&s_mass[b], q);
@@ -769,15 +782,19 @@ __global__ __launch_bounds__(1024) void tile_budget_kernel(const float* __restri
     for (int o = 32; o; o >>= 1) {
       tot += __shfl_xor(tot, o, 64);
       cov += __shfl_xor(cov, o, 64);
+      tiles += __shfl_xor(tiles, o, 64);
     }
     if ((tid & 63) == 0) {
       atomicAdd(&s_total, tot);
       atomicAdd(&s_cov, cov);
+      atomicAdd(&s_tiles, tiles);
     }
   }
   __syncthreads();
   if (tid == 0) {
-    const double budget = 3.0 * (double)max_mse * (double)kBudgetShare * (double)s_cov * (double)kFix;
+    const double share = fmin(fmax((double)kShareMax - (double)kShareSpread / sqrt(fmax((double)s_tiles, 1.0)),
+                                   (double)kShareMin), (double)kShareMax);
+    const double budget = 3.0 * (double)max_mse * share * (double)s_cov * (double)kFix;
     double left = (double)s_total;
     int thr = kBudgetBins;  // pick bins >= thr
     while (thr > 0 && left > budget) {
