@@ -313,10 +313,12 @@ int vct_upsample_train(vct_ctx ctx, uint32_t steps, uint32_t batch, float lr, ui
  * traced at full resolution instead), then the per-tile fallback: every
  * 32x32 tile is compared with a full-resolution trace at 16 probe pixels,
  * which estimates its error, and the tiles with the largest estimated errors
- * are traced again in full until the error left in the others is within 30%
- * of the frame's budget for the bound (with the half-resolution refinement on,
- * every tile whose own estimate is below the bound is refined instead).
- * Pass times [voxelize, mip, gbuffer, low-res trace, upsample, fallback]. */
+ * are traced again in full until the error left in the others is within a
+ * share of the frame's budget for the bound (0.55 - 1.7/sqrt(covered tiles),
+ * at least 0.25: margin for the estimate's noise; with the half-resolution
+ * refinement on, every tile whose own estimate is below the bound is refined
+ * instead).  Pass times [voxelize, mip, gbuffer, low-res trace, upsample,
+ * fallback]. */
 int vct_render_learned(vct_ctx ctx);
 /* The PSNR bound in dB (peak 1, RGB over covered pixels, against the plain
  * full-resolution frame); default 40; <= 0 turns the fallback off. */
