@@ -701,6 +701,7 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
     // testing each candidate in full.
     uint32_t* q = s_q[wave];
     uint32_t qn = 0;  // entries in the queue (the same in every lane)
+    uint32_t own0 = 0;  // the entry of the wave's first candidate (the same in every lane; never decreases)
     auto fragment = [&](uint32_t d) {
       const uint32_t lo = d & 255u;
       const int i = (int)((d >> 8) & 0x7ffu), j = (int)(d >> 19);
@@ -732,19 +733,32 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
     };
     for (uint32_t cb = (uint32_t)wave * 64u; cb < total; cb += kBrickThreads) {
       const uint32_t c = cb + (uint32_t)lane;
+      // The entry (triangle) of each lane's candidate, for the window
+      // [cb, cb + 64) of consecutive candidates.  own0 moves to the first
+      // entry ending after cb: lane k tests entry own0 + k and the ballot
+      // counts those ending at or before cb (a prefix, the ends being
+      // sorted).  Lane k then holds the end of entry own0 + k, and each lane
+      // finds how many of these end at or before its candidate by a binary
+      // search over the lanes (register shuffles): that count from own0 is
+      // its entry.  A window that spans more than 64 entries (entries with
+      // one candidate or none) finishes with a per-lane walk.
+      for (;;) {
+        const uint32_t ek = own0 + (uint32_t)lane < kBrickThreads ? s_end[own0 + lane] : 0xffffffffu;
+        const uint32_t k = (uint32_t)__popcll(__ballot(ek <= cb));
+        own0 += k;
+        if (k < 64u) break;
+      }
+      const uint32_t ek = own0 + (uint32_t)lane < kBrickThreads ? s_end[own0 + lane] : 0xffffffffu;
+      uint32_t past = 0;
+      for (uint32_t st = 32; st; st >>= 1)
+        if ((uint32_t)__shfl(ek, (int)(past + st - 1u), 64) <= c) past += st;
+      if ((uint32_t)__shfl(ek, (int)past, 64) <= c) ++past;
       bool hit = false;
       uint32_t desc = 0;
       if (c < total) {
-        // the triangle of the wave's first candidate by binary search (the
-        // same for every lane), then each lane steps forward over the few
-        // triangles its candidate lies past
-        uint32_t lo = 0, hi = kBrickThreads - 1;
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (s_end[mid] > cb) hi = mid;
-          else lo = mid + 1;
-        }
-        while (s_end[lo] <= c) ++lo;
+        uint32_t lo = own0 + past;
+        if (past == 64u)
+          while (s_end[lo] <= c) ++lo;
         const uint32_t local = c - (lo ? s_end[lo - 1] : 0u);
         const uint32_t dv = s_div[lo], cw = dv >> 20;
         const uint32_t row = (local * (dv & 0xfffffu)) >> 16;  // exact: local < 35 * 35, cw <= 35
