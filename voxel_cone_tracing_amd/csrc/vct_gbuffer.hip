@@ -731,18 +731,32 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
     }
     __syncthreads();
     const uint32_t total = s_end[kChunk - 1];
-    for (uint32_t c = tid; c < total; c += 256) {
-      // the triangle of the wave's first row by binary search (the same for
-      // every lane), then each lane steps forward over the triangles its row
-      // lies past
-      const uint32_t c0 = c - (uint32_t)lane;
-      uint32_t lo = 0, hi = kChunk - 1;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (s_end[mid] > c0) hi = mid;
-        else lo = mid + 1;
+    uint32_t own0 = 0;  // the entry of the wave's first row (the same in every lane; never decreases)
+    for (uint32_t c0 = (uint32_t)wave * 64u; c0 < total; c0 += 256) {
+      // The entry (triangle) of each lane's row, for the window [c0, c0 + 64)
+      // of consecutive rows.  own0 moves to the first entry ending after c0:
+      // lane k tests entry own0 + k and the ballot counts those ending at or
+      // before c0 (a prefix, the ends being sorted).  Lane k then holds the
+      // end of entry own0 + k, and each lane finds how many of these end at
+      // or before its row by a binary search over the lanes (register
+      // shuffles): that count from own0 is its entry.  A window that spans
+      // more than 64 entries finishes with a per-lane walk.
+      const uint32_t c = c0 + (uint32_t)lane;
+      for (;;) {
+        const uint32_t ek = own0 + (uint32_t)lane < (uint32_t)kChunk ? s_end[own0 + lane] : 0xffffffffu;
+        const uint32_t k = (uint32_t)__popcll(__ballot(ek <= c0));
+        own0 += k;
+        if (k < 64u) break;
       }
-      while (s_end[lo] <= c) ++lo;
+      const uint32_t ek = own0 + (uint32_t)lane < (uint32_t)kChunk ? s_end[own0 + lane] : 0xffffffffu;
+      uint32_t past = 0;
+      for (uint32_t st = 32; st; st >>= 1)
+        if ((uint32_t)__shfl(ek, (int)(past + st - 1u), 64) <= c) past += st;
+      if ((uint32_t)__shfl(ek, (int)past, 64) <= c) ++past;
+      if (c >= total) continue;
+      uint32_t lo = own0 + past;
+      if (past == 64u)
+        while (s_end[lo] <= c) ++lo;
       const uint32_t row = c - (lo ? s_end[lo - 1] : 0u);
       const GRowSetup& g = s_row[lo];
       const GRec& r = s_rec[lo];
