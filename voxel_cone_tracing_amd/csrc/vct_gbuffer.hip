@@ -681,6 +681,7 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
   __shared__ GRowSetup s_row[kChunk];
   __shared__ uint32_t s_end[kChunk];  // inclusive prefix of the row counts
   __shared__ uint32_t s_wsum[kChunk / 64];
+  __shared__ uint32_t s_q[4][128];  // per-wave queue of pixels that passed the edge signs and early depth test
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int tile = blockIdx.x;
   const int x0 = (tile % tiles_x) * kTile, y0 = (tile / tiles_x) * kTile;
@@ -731,6 +732,39 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
     }
     __syncthreads();
     const uint32_t total = s_end[kChunk - 1];
+    // Pixels in two phases.  1: each lane walks the span of its row with the
+    // edge signs and the early depth rejection only, and appends the pixels
+    // that pass to its wave's queue in LDS (ballot + popcount; no barrier:
+    // only the wave touches it).  2: whenever the queue holds 64, every lane
+    // takes one and runs the exact test (top-left rule), the barycentrics and
+    // the depth test, so the expensive part runs with all lanes busy instead
+    // of on the lanes whose rows are still long.  A pixel's key reaches its
+    // atomic minimum whatever the order, so the result is the same.
+    uint32_t* q = s_q[wave];
+    uint32_t qn = 0;  // entries in the queue (the same in every lane)
+    // queue entry: entry lo (7 bits) | row (5 bits) << 7 | column in the tile << 12
+    auto pixel = [&](uint32_t d) {
+      const uint32_t lo = d & 127u, row = (d >> 7) & 31u;
+      const int li = (int)(d >> 12);
+      const GRowSetup& g = s_row[lo];
+      const GRec& r = s_rec[lo];
+      GSetup st;
+      st.area = r.area;
+      st.swap = (r.swap & 1u) != 0u;
+      GEdges e;
+      for (int k = 0; k < 3; ++k) {
+        e.di[k] = g.di[k];
+        e.ex[k] = g.dj[k] >> 8;   // dj = ex * 256, di = -ey * 256 (exact)
+        e.ey[k] = -(g.di[k] >> 8);
+      }
+      const int64_t col = (int64_t)(x0 + li - (int)g.ci0);
+      const int64_t wa = g.w0[0] + g.dj[0] * (int64_t)row + g.di[0] * col;
+      const int64_t wb = g.w0[1] + g.dj[1] * (int64_t)row + g.di[1] * col;
+      const int64_t wc = g.w0[2] + g.dj[2] * (int64_t)row + g.di[2] * col;
+      float bw0, bw1, bw2;
+      if (g_cover_w(st, e, wa, wb, wc, bw0, bw1, bw2))
+        g_depth_test(s_depth, li, (int)g.cj0 + (int)row - y0, bw0, bw1, bw2, r);
+    };
     uint32_t own0 = 0;  // the entry of the wave's first row (the same in every lane; never decreases)
     for (uint32_t c0 = (uint32_t)wave * 64u; c0 < total; c0 += 256) {
       // The entry (triangle) of each lane's row, for the window [c0, c0 + 64)
@@ -753,47 +787,61 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
       for (uint32_t st = 32; st; st >>= 1)
         if ((uint32_t)__shfl(ek, (int)(past + st - 1u), 64) <= c) past += st;
       if ((uint32_t)__shfl(ek, (int)past, 64) <= c) ++past;
-      if (c >= total) continue;
-      uint32_t lo = own0 + past;
-      if (past == 64u)
-        while (s_end[lo] <= c) ++lo;
-      const uint32_t row = c - (lo ? s_end[lo - 1] : 0u);
-      const GRowSetup& g = s_row[lo];
-      const GRec& r = s_rec[lo];
-      GSetup st;
-      st.area = r.area;
-      st.swap = (r.swap & 1u) != 0u;
-      GEdges e;
-      for (int k = 0; k < 3; ++k) {
-        e.di[k] = g.di[k];
-        e.ex[k] = g.dj[k] >> 8;   // dj = ex * 256, di = -ey * 256 (exact)
-        e.ey[k] = -(g.di[k] >> 8);
+      // the lane's row and its span: for each edge w(i) = w_row + di * (i - ci0)
+      // >= 0, bounded in float with a margin (the exact tests decide)
+      int len = 0, lo_i = 0, lj = 0;
+      uint32_t lo = 0, row = 0, zb = 0;
+      int64_t wa = 0, wb = 0, wc = 0, da = 0, db = 0, dc = 0;
+      if (c < total) {
+        lo = own0 + past;
+        if (past == 64u)
+          while (s_end[lo] <= c) ++lo;
+        row = c - (lo ? s_end[lo - 1] : 0u);
+        const GRowSetup& g = s_row[lo];
+        const int ci0 = g.ci0, ci1 = g.ci1;
+        lj = (int)g.cj0 + (int)row - y0;
+        const int64_t ra = g.w0[0] + g.dj[0] * (int64_t)row;
+        const int64_t rb = g.w0[1] + g.dj[1] * (int64_t)row;
+        const int64_t rc = g.w0[2] + g.dj[2] * (int64_t)row;
+        int hi_i = ci1;
+        lo_i = ci0;
+        g_span(ra, g.di[0], g.inv[0], ci0, lo_i, hi_i);
+        g_span(rb, g.di[1], g.inv[1], ci0, lo_i, hi_i);
+        g_span(rc, g.di[2], g.inv[2], ci0, lo_i, hi_i);
+        if (lo_i <= hi_i) {
+          len = hi_i - lo_i + 1;
+          const int64_t k0 = lo_i - ci0;
+          da = g.di[0];
+          db = g.di[1];
+          dc = g.di[2];
+          wa = ra + da * k0;
+          wb = rb + db * k0;
+          wc = rc + dc * k0;
+          zb = g.zb;
+        }
       }
-      const int ci0 = g.ci0, ci1 = g.ci1, j = g.cj0 + (int)row;
-      const int64_t ra = g.w0[0] + g.dj[0] * (int64_t)row;
-      const int64_t rb = g.w0[1] + g.dj[1] * (int64_t)row;
-      const int64_t rc = g.w0[2] + g.dj[2] * (int64_t)row;
-      // the row's span: for each edge w(i) = w_row + di * (i - ci0) >= 0,
-      // bounded in float with a margin (the exact test below decides)
-      int lo_i = ci0, hi_i = ci1;
-      g_span(ra, g.di[0], g.inv[0], ci0, lo_i, hi_i);
-      g_span(rb, g.di[1], g.inv[1], ci0, lo_i, hi_i);
-      g_span(rc, g.di[2], g.inv[2], ci0, lo_i, hi_i);
-      if (lo_i <= hi_i) {
-        const int64_t k0 = lo_i - ci0;
-        int64_t wa = ra + g.di[0] * k0, wb = rb + g.di[1] * k0, wc = rc + g.di[2] * k0;
-        const uint32_t zb = g.zb;
-        for (int i = lo_i; i <= hi_i; ++i) {
-          float bw0, bw1, bw2;
-          if (wa >= 0 && wb >= 0 && wc >= 0 && !g_hidden(s_depth, i - x0, j - y0, zb) &&
-              g_cover_w(st, e, wa, wb, wc, bw0, bw1, bw2))
-            g_depth_test(s_depth, i - x0, j - y0, bw0, bw1, bw2, r);
-          wa += g.di[0];
-          wb += g.di[1];
-          wc += g.di[2];
+      // the wave walks its longest span; each lane stops at the end of its own
+      for (int k = 0; __ballot(k < len) != 0ull; ++k) {
+        bool hit = false;
+        uint32_t desc = 0;
+        if (k < len) {
+          const int li = lo_i + k - x0;
+          hit = wa >= 0 && wb >= 0 && wc >= 0 && !g_hidden(s_depth, li, lj, zb);
+          desc = lo | (row << 7) | ((uint32_t)li << 12);
+          wa += da;
+          wb += db;
+          wc += dc;
+        }
+        const unsigned long long m = __ballot(hit);
+        if (hit) q[qn + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = desc;
+        qn += (uint32_t)__popcll(m);
+        if (qn >= 64u) {
+          qn -= 64u;
+          pixel(q[qn + (uint32_t)lane]);
         }
       }
     }
+    if ((uint32_t)lane < qn) pixel(q[lane]);
     __syncthreads();
   }
   for (int q = tid; q < kTile * kTile; q += 256) {
