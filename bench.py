@@ -198,6 +198,10 @@ def run_form(ctx, sc, learned, steps, warmup):
         else:
             ctx.render()
 
+    # the timed frames record no pass-timing events (each is a packet the
+    # stream processes between kernels); the per-pass HIP event times come
+    # from the same frames run again with timing on, one synchronisation each
+    ctx.set_pass_timing(0)
     for _ in range(warmup):
         frame()
     ctx.sync()
@@ -206,8 +210,7 @@ def run_form(ctx, sc, learned, steps, warmup):
         frame()
     ctx.sync()
     ms = (time.perf_counter() - t0) * 1000.0 / steps
-    # per-pass HIP event times (recorded on the context stream), in separate
-    # frames so that the timed loop above has no host synchronisation inside
+    ctx.set_pass_timing(0x3F)
     passes = np.zeros(6)
     for _ in range(steps):
         frame()

@@ -215,8 +215,14 @@ int vct_gbuffer_upload(vct_ctx ctx, const float* pos_depth, const float* normal_
 int vct_image_download(vct_ctx ctx, float* rgba, size_t pixels);
 /* Milliseconds of the last frame's passes measured with HIP events on the
  * context stream: [voxelize, mip, gbuffer, trace, upsample, fallback]
- * (n <= 6 entries are written). */
+ * (n <= 6 entries are written); 0 for a pass that ran untimed. */
 int vct_get_pass_times(vct_ctx ctx, float* ms, int n);
+/* Which passes record timing events: bit k for pass k of vct_get_pass_times
+ * (default 0x3f, all).  Each event is a packet the stream processes between
+ * kernels (~3-4 us of idle GPU each on the 1M-triangle frame: 28 us of the
+ * plain frame, 40 us of the learned one); 0 turns the diagnostic off.  No
+ * counterpart in the reference, which does not time its passes. */
+int vct_set_pass_timing(vct_ctx ctx, uint32_t mask);
 /* Fragments and candidate pixels of the last voxelize. */
 int vct_get_voxelize_stats(vct_ctx ctx, uint64_t* fragments, uint64_t* candidates);
 /* Runs vct_trace once more with a counter and returns the number of cone

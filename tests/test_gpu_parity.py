@@ -313,6 +313,37 @@ def test_full_compose_matches_oracle(gpu_available, orc, dtype):
     assert err <= 2e-3
 
 
+def test_pass_timing_mask(gpu_available):
+    """vct_set_pass_timing: with timing off the frame is the same bit for bit
+    and every pass reads 0 ms; a pass left out of the mask reads 0 while the
+    others are timed; a mask past pass 5 is an error."""
+    say("pass timing mask")
+    W, H, res = 96, 64, 64
+    sc = SceneSpec("cornell", res=res, W=W, H=H)
+
+    def frame(ctx):
+        ctx.clear_queues()
+        ctx.queue_model(sc.model)
+        for (p, c, i, k) in sc.lights:
+            ctx.queue_light(p, c, i, k)
+        ctx.render()
+        return ctx.download_image(), ctx.pass_times()
+
+    with Context(W, H) as ctx:
+        sc.build(ctx, L.FP16)
+        img_on, t_on = frame(ctx)
+        ctx.set_pass_timing(0)
+        img_off, t_off = frame(ctx)
+        ctx.set_pass_timing(0x3F & ~(1 << 3))  # all but the trace
+        _, t_some = frame(ctx)
+        with pytest.raises(L.VctError):
+            ctx.set_pass_timing(0x40)
+    say(f"  on {np.round(t_on[:4], 4)}, off {t_off[:4]}, no trace {np.round(t_some[:4], 4)}")
+    assert np.array_equal(img_on, img_off)
+    assert (t_on[:4] > 0).all() and (t_off == 0).all()
+    assert t_some[3] == 0 and (t_some[[0, 1, 2]] > 0).all()
+
+
 def test_trace_ragged_viewport_matches_oracle(gpu_available, orc):
     """A viewport whose block grid is ragged: 88x100 is 6 x 13 blocks of
     16x8 pixels, partial on the right and bottom edges, 8 block rows in the

@@ -129,6 +129,7 @@ PROTOTYPES = {
     "vct_gbuffer_upload": (_I, [_P, _FP, _FP, _SZ]),
     "vct_image_download": (_I, [_P, _FP, _SZ]),
     "vct_get_pass_times": (_I, [_P, _FP, _I]),
+    "vct_set_pass_timing": (_I, [_P, _U]),
     "vct_get_voxelize_stats": (_I, [_P, _U64P, _U64P]),
     "vct_trace_count_steps": (_I, [_P, _U64P]),
     "vct_tex3d_create": (_I, [_P, _I, _I, _I, _I, _I, _UP]),

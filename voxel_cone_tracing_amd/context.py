@@ -183,6 +183,10 @@ class Context:
         L.check(self.lib.vct_get_pass_times(self.h, L.fptr(t), t.shape[0]))
         return t
 
+    def set_pass_timing(self, mask: int) -> None:
+        """Passes that record timing events (bit k: pass k of pass_times; 0x3f all, 0 none)."""
+        L.check(self.lib.vct_set_pass_timing(self.h, mask))
+
     # -- learned upsampler --------------------------------------------------------
     def mlp_init(self, seed: int = 1) -> None:
         L.check(self.lib.vct_mlp_init(self.h, seed))

@@ -284,6 +284,7 @@ struct vct_context {
   hipEvent_t ev[12];  // start / end of 6 passes
   float pass_ms[6] = {0, 0, 0, 0, 0, 0};
   bool ev_recorded[6] = {false, false, false, false, false, false};
+  uint32_t time_mask = 0x3fu;  // passes timed with events (bit k: pass k; vct_set_pass_timing)
 };
 
 // ---- internal helpers (vct_runtime.hip) ---------------------------------
@@ -298,6 +299,19 @@ void vct_upload_free(VctUpload& u);
     hipError_t _e = (call);                            \
     if (_e != hipSuccess) return vct_hip_check(_e, #call); \
   } while (0)
+
+// Records the start (end = false) or the end event of pass `slot` on stream
+// s when pass timing covers that pass; an untimed pass reads 0 ms.
+inline int vct_pass_event(vct_context* ctx, int slot, bool end, hipStream_t s) {
+  if (!((ctx->time_mask >> slot) & 1u)) {
+    ctx->ev_recorded[slot] = false;
+    ctx->pass_ms[slot] = 0.0f;
+    return VCT_OK;
+  }
+  VCT_HIP(hipEventRecord(ctx->ev[2 * slot + (end ? 1 : 0)], s));
+  if (end) ctx->ev_recorded[slot] = true;
+  return VCT_OK;
+}
 
 // kernels launchers
 int vct_launch_mip(vct_context* ctx);

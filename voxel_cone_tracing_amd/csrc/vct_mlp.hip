@@ -1606,11 +1606,9 @@ int vct_render_learned(vct_ctx ctx) {
     if (rc) return rc;
     VCT_HIP(hipEventRecord(ctx->ev_join, ctx->side));
   }
-  VCT_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
-  if ((rc = trace_lowres(ctx))) return rc;
-  VCT_HIP(hipEventRecord(ctx->ev[7], ctx->stream));
-  ctx->ev_recorded[3] = true;
-  VCT_HIP(hipEventRecord(ctx->ev[8], ctx->stream));
+  if ((rc = vct_pass_event(ctx, 3, false, ctx->stream)) || (rc = trace_lowres(ctx)) ||
+      (rc = vct_pass_event(ctx, 3, true, ctx->stream)) || (rc = vct_pass_event(ctx, 4, false, ctx->stream)))
+    return rc;
   // frame budget: the learned values at the probes first, the tiles to trace
   // in full picked from them, then the network only outside those tiles
   const bool budget = fallback && !ctx->fb_half;
@@ -1622,9 +1620,8 @@ int vct_render_learned(vct_ctx ctx) {
   } else if ((rc = infer_full(ctx))) {
     return rc;
   }
-  VCT_HIP(hipEventRecord(ctx->ev[9], ctx->stream));
-  ctx->ev_recorded[4] = true;
-  VCT_HIP(hipEventRecord(ctx->ev[10], ctx->stream));
+  if ((rc = vct_pass_event(ctx, 4, true, ctx->stream)) || (rc = vct_pass_event(ctx, 5, false, ctx->stream)))
+    return rc;
   ctx->fb_ran = false;
   if (budget) {
     const uint32_t np = (uint32_t)(((ctx->width + kFbTile - 1) / kFbTile) * ((ctx->height + kFbTile - 1) / kFbTile)) *
@@ -1642,8 +1639,7 @@ int vct_render_learned(vct_ctx ctx) {
   } else if ((rc = trace_listed_pixels(ctx))) {
     return rc;
   }
-  VCT_HIP(hipEventRecord(ctx->ev[11], ctx->stream));
-  ctx->ev_recorded[5] = true;
+  if ((rc = vct_pass_event(ctx, 5, true, ctx->stream))) return rc;
   return vct_clear_queues(ctx);
 }
 

@@ -307,17 +307,15 @@ int vct_frame_front(vct_context* ctx) {
   };
   VCT_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
   VCT_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-  VCT_HIP(hipEventRecord(ctx->ev[4], ctx->side));
+  if ((rc = vct_pass_event(ctx, 2, false, ctx->side))) return rc;
   if ((rc = on_side([ctx] { return vct_launch_gbuffer_phase(ctx, 1); }))) return rc;
   ++ctx->volume_version;
-  VCT_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+  if ((rc = vct_pass_event(ctx, 0, false, ctx->stream))) return rc;
   if ((rc = vct_launch_voxelize_phase(ctx, 1))) return rc;
   if ((rc = on_side([ctx] { return vct_launch_gbuffer_phase(ctx, 2); }))) return rc;
-  VCT_HIP(hipEventRecord(ctx->ev[5], ctx->side));
-  ctx->ev_recorded[2] = true;
+  if ((rc = vct_pass_event(ctx, 2, true, ctx->side))) return rc;
   if ((rc = vct_launch_voxelize_phase(ctx, 2))) return rc;
-  VCT_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
-  ctx->ev_recorded[0] = true;
+  if ((rc = vct_pass_event(ctx, 0, true, ctx->stream))) return rc;
   VCT_HIP(hipEventRecord(ctx->ev_join, ctx->side));
   if ((rc = vct_mip_aniso(ctx))) return rc;
   VCT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
@@ -714,12 +712,9 @@ int vct_set_cone_set(vct_ctx ctx, int cone_set) {
 }
 
 static int timed(vct_context* ctx, int slot, int (*fn)(vct_context*)) {
-  VCT_HIP(hipEventRecord(ctx->ev[2 * slot], ctx->stream));
-  int rc = fn(ctx);
-  if (rc) return rc;
-  VCT_HIP(hipEventRecord(ctx->ev[2 * slot + 1], ctx->stream));
-  ctx->ev_recorded[slot] = true;
-  return VCT_OK;
+  int rc = vct_pass_event(ctx, slot, false, ctx->stream);
+  if (rc || (rc = fn(ctx))) return rc;
+  return vct_pass_event(ctx, slot, true, ctx->stream);
 }
 
 int vct_voxelize(vct_ctx ctx) {
@@ -844,6 +839,13 @@ int vct_image_download(vct_ctx ctx, float* rgba, size_t pixels) {
   if (!ctx->image.p) return vct_set_error(VCT_ERR_STATE, "no image yet");
   VCT_HIP(hipMemcpyAsync(rgba, ctx->image.p, pixels * 16, hipMemcpyDeviceToHost, ctx->stream));
   VCT_HIP(hipStreamSynchronize(ctx->stream));
+  return VCT_OK;
+}
+
+int vct_set_pass_timing(vct_ctx ctx, uint32_t mask) {
+  CTX_CHECK(ctx);
+  if (mask > 0x3fu) return vct_set_error(VCT_ERR_INVALID_ARG, "pass timing mask has bits past pass 5");
+  ctx->time_mask = mask;
   return VCT_OK;
 }
 
