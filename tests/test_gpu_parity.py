@@ -124,6 +124,38 @@ def test_voxelize_exact_mean_bit_exact(gpu_available, orc, dtype, kind, res, ntr
     assert err <= bound
 
 
+def test_voxelize_lists_past_their_capacity_bit_exact(gpu_available, orc):
+    """Brick lists sized from an earlier frame's (triangle, brick) count: a
+    frame whose geometry outgrew them (the same triangles, grown tenfold by
+    the model matrix) has its overflowing bricks walk every triangle's brick
+    box and stays bit-exact, and so does the frame after it (sized from the
+    grown count)."""
+    say("voxelize lists past their capacity")
+    res = 256
+    sc = SceneSpec("cornell", res=res)
+    ntri = sc.pos.reshape(-1, 9).shape[0]
+    small = np.diag([0.1, 0.1, 0.1, 1.0]).astype(np.float32)
+    with Context(32, 32) as ctx:
+        sc.build(ctx, L.FP16, L.AVG_EXACT)
+        ctx.set_model_transform(sc.model, small)
+        ctx.voxelize()  # the first frame waits for its count: the lists are sized from it
+        _, small_pairs = ctx.voxelize_stats()
+        ctx.set_model_transform(sc.model, sc.M)
+        got = []
+        for _ in range(2):
+            ctx.voxelize()
+            got.append(ctx.download_level(0))
+        _, pairs = ctx.voxelize_stats()
+    cap = max(small_pairs + small_pairs // 4, 2 * ntri + 1024)
+    say(f"  pairs: shrunk box {small_pairs}, full box {pairs}, capacity of the second frame {cap}")
+    assert pairs > cap
+    want = orc.voxelize(res, sc.cube_size, sc.M, sc.pos, sc.nrm, sc.tri_mat, sc.mat_records, sc.lights,
+                        want=("q32",))
+    w = encode_level(want["q32"], L.FP16)
+    for k, g in enumerate(got):
+        assert np.array_equal(bits(g), bits(w)), f"frame {k + 2}: level 0 differs"
+
+
 # ---- voxelize under a model matrix (voxelize.vert:24-30) ----------------------------
 @pytest.mark.parametrize("dtype,average", [(L.RGBA8, L.AVG_REFERENCE), (L.FP16, L.AVG_EXACT), (L.FP32, L.AVG_EXACT)])
 def test_voxelize_with_model_matrix_bit_exact(gpu_available, orc, dtype, average):

@@ -1132,11 +1132,12 @@ int vct_launch_gbuffer_phase(vct_context* ctx, int phase) {
       // are known on the device only
       VCT_HIP(hipMemcpyAsync(ctx->h_readback, counters + 4, 8, hipMemcpyDeviceToHost, ctx->stream));
     }
+    // also when nothing was counted: a copy an interrupted earlier frame left
+    // in flight must not land after the words are read
+    VCT_HIP(hipEventRecord(ctx->g_count_ev, ctx->stream));
   }
   if (!(phase & 2)) return VCT_OK;
-  // also when nothing was counted: a copy an interrupted earlier frame left
-  // in flight must not land after the word is read
-  VCT_HIP(hipStreamSynchronize(ctx->stream));
+  VCT_HIP(hipEventSynchronize(ctx->g_count_ev));
   const uint32_t total = ctx->h_readback[0];
   const uint32_t nfans = ctx->h_readback[1];
   if ((rc = vct_ensure(ctx->g_fans, (size_t)nfans * (sizeof(GRec) + sizeof(GFan)) + 64, ctx))) return rc;

@@ -238,6 +238,16 @@ struct vct_context {
   VctBuf frags_sorted, vox_count, vox_off, scan_tmp, g_scan_tmp, counters;
   VctBuf setup;  // per-frame triangle list (model index, transform)
   uint64_t last_fragments = 0, last_candidates = 0;
+  // Exact-mean voxelize lists sized without a host wait: once a total is
+  // known for the queued triangle count, the list holds vox_cap entries and a
+  // brick whose range runs past them walks every triangle's brick box
+  // instead.  Each frame's (triangle, brick) pair count comes back into
+  // h_readback[3] asynchronously, complete once vox_total_ev is.
+  uint32_t vox_known_total = 0, vox_known_ntri = 0, vox_cap = 0;
+  hipEvent_t vox_total_ev = nullptr;
+  bool vox_total_pending = false;
+  bool vox_sized = false;            // this frame's list was launched against vox_cap
+  bool vox_pairs_on_device = false;  // last_candidates still to be read from counters[2]
 
   // G-buffer + image
   VctBuf g_list, g_pos, g_nrm, image;
@@ -285,6 +295,7 @@ struct vct_context {
   float pass_ms[6] = {0, 0, 0, 0, 0, 0};
   bool ev_recorded[6] = {false, false, false, false, false, false};
   uint32_t time_mask = 0x3fu;  // passes timed with events (bit k: pass k; vct_set_pass_timing)
+  hipEvent_t g_count_ev = nullptr;    // the G-buffer's list sizes copied to h_readback[0..1]
 };
 
 // ---- internal helpers (vct_runtime.hip) ---------------------------------

@@ -291,10 +291,13 @@ static void identity(float* m) {
 // The passes before the trace of a frame: the G-buffer on the second stream
 // beside voxelize and mip on the context stream (they read the same geometry
 // and write disjoint buffers; each pass is timed on its own stream, so the
-// pass times overlap); the context stream then waits for the G-buffer.  Both
-// the G-buffer and voxelize size a triangle list on the device and read it
-// back: their first phases are launched before either waits, so neither
-// stream idles while the host waits for the other's count.
+// pass times overlap); the context stream then waits for the G-buffer.
+// Voxelize sizes its triangle list from earlier frames' counts and is
+// launched whole without a host wait (after the first frame, which reads its
+// count back); the G-buffer reads its list size and fan count back, the host
+// waiting on the copy's event.  Starting the G-buffer's raster only after
+// voxelize's brick kernel, beside the HBM-bound mip, measured slower: its
+// long workgroups take the slots the mip's 512-lane ones need.
 int vct_frame_front(vct_context* ctx) {
   int rc;
   if ((rc = sync_materials(ctx))) return rc;
@@ -373,6 +376,8 @@ int vct_create(int width, int height, vct_ctx* out) {
   for (int i = 0; i < 12 && e == hipSuccess; ++i) e = hipEventCreate(&ctx->ev[i]);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->vox_total_ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->g_count_ev, hipEventDisableTiming);
   if (e != hipSuccess) {
     vct_destroy(ctx);
     return vct_hip_check(e, "hipEventCreate");
@@ -438,6 +443,8 @@ int vct_destroy(vct_ctx ctx) {
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
   if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
   if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+  if (ctx->vox_total_ev) (void)hipEventDestroy(ctx->vox_total_ev);
+  if (ctx->g_count_ev) (void)hipEventDestroy(ctx->g_count_ev);
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->h_readback) (void)hipHostFree(ctx->h_readback);
@@ -866,6 +873,13 @@ int vct_get_pass_times(vct_ctx ctx, float* ms, int n) {
 
 int vct_get_voxelize_stats(vct_ctx ctx, uint64_t* fragments, uint64_t* candidates) {
   CTX_CHECK(ctx);
+  if (ctx->vox_pairs_on_device) {  // a list sized without a host wait: its total is on the device
+    uint32_t total = 0;
+    VCT_HIP(hipMemcpyAsync(&total, (const uint32_t*)ctx->counters.p + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
+    VCT_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->last_candidates = total;
+    ctx->vox_pairs_on_device = false;
+  }
   if (fragments) *fragments = ctx->last_fragments;
   if (candidates) *candidates = ctx->last_candidates;
   return VCT_OK;

@@ -32,6 +32,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <algorithm>
+
 #include "vct_internal.h"
 
 namespace vct {
@@ -456,17 +458,19 @@ __global__ __launch_bounds__(256) void vox_brick_count_kernel(DrawTable b, VoxPa
   }
 }
 
-// 3. list write: the same brick boxes, as the setup packed them.
+// 3. list write: the same brick boxes, as the setup packed them.  Entries at
+//    or past `cap` are not written (their bricks walk the boxes instead).
 __global__ __launch_bounds__(256) void vox_brick_write_kernel(uint32_t ntri, const uint2* __restrict__ boxes,
                                                               const uint32_t* __restrict__ brick_off,
-                                                              uint32_t* cursor, uint32_t* __restrict__ list) {
+                                                              uint32_t* cursor, uint32_t* __restrict__ list,
+                                                              uint32_t cap) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const BrickBox bb = unpack_box(t < ntri ? boxes[t] : make_uint2(0u, 0u));
   for (int k = 0; __ballot(k < bb.n) != 0ull; ++k) {
     const bool act = k < bb.n;
     const uint32_t key = act ? brick_of(bb, k) : 0u;
     const uint32_t slot = wave_add_one(cursor, key, act);
-    if (act) list[brick_off[key] + slot] = t;
+    if (act && (uint64_t)brick_off[key] + slot < (uint64_t)cap) list[brick_off[key] + slot] = t;
   }
 }
 
@@ -597,6 +601,15 @@ __device__ __forceinline__ uint32_t replay_voxel(unsigned long long* __restrict_
 // (canonical key, colour) in the voxel's slots, in no particular order, and
 // then replays its brick's voxels and writes their texels.  No global counter
 // is shared between workgroups.
+// The brick lists' capacity: a brick whose range [off, off + count) runs past
+// `cap` entries walks the packed brick boxes of all `ntri` triangles and keeps
+// those that contain it (the same triangles its list would hold).
+struct ListCap {
+  const uint2* boxes;
+  uint32_t ntri;
+  uint32_t cap;
+};
+
 enum { kExactMean = 0, kRefCount = 1, kRefPlace = 2 };
 template <int DT, int kMode>
 __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, const TriRec* __restrict__ recs,
@@ -604,7 +617,7 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
                                                                   const uint32_t* __restrict__ brick_count,
                                                                   const uint32_t* __restrict__ brick_off,
                                                                   const uint32_t* __restrict__ list,
-                                                                  void* level0, RefPass r) {
+                                                                  ListCap lc, void* level0, RefPass r) {
   constexpr int kVox = kBrick * kBrick * kBrick;
   __shared__ uint32_t s_sum[4][kMode == kExactMean ? kVox : 1];
   __shared__ uint32_t s_cnt[kVox];
@@ -641,13 +654,25 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
     s_cnt[v] = 0u;
   }
   const uint32_t n = brick_count[brick], base = brick_off[brick];
+  // a brick whose list runs past the lists' capacity (the geometry grew past
+  // what the lists were sized for) walks every triangle's brick box instead
+  const bool walk = (uint64_t)base + n > (uint64_t)lc.cap;
+  const uint32_t nitems = walk ? lc.ntri : n;
   const int lim = 2 * (int)vp.res - 1;
   const int span = 2 << g.bits;  // footprint of the brick on the raster, pixels per axis
   __syncthreads();
-  for (uint32_t b0 = 0; b0 < n; b0 += kBrickThreads) {
+  for (uint32_t b0 = 0; b0 < nitems; b0 += kBrickThreads) {
     uint32_t cnt = 0;
-    if (b0 + tid < n) {
-      const uint32_t t = list[base + b0 + tid];
+    uint32_t t = b0 + (uint32_t)tid;
+    bool in = t < nitems;
+    if (in && walk) {  // the box as pack_box stores it: first brick, bricks per axis
+      const uint2 pb = lc.boxes[t];
+      for (int k = 0; k < 3; ++k)
+        in = in && bc[k] - ((pb.x >> (8 * k)) & 255u) < ((pb.y >> (8 * k)) & 255u);
+    } else if (in) {
+      t = list[base + t];
+    }
+    if (in) {
       const TriRec* rp = recs + t;
       const uint32_t fl = rp->flags;
       const int axis = (int)((fl >> 1) & 3u);
@@ -884,15 +909,21 @@ static int fill_draws(vct_context* ctx, const std::vector<uint32_t>& q, DrawTabl
 
 template <int DT, int kMode = kExactMean>
 static void launch_brick(vct_context* ctx, uint32_t nbricks, const VoxParams& vp, const uint32_t* count,
-                         const uint32_t* off, const uint32_t* list, void* level0, const RefPass& r = RefPass{}) {
+                         const uint32_t* off, const uint32_t* list, void* level0, const RefPass& r = RefPass{},
+                         ListCap lc = ListCap{nullptr, 0u, 0xffffffffu}) {
   hipLaunchKernelGGL((vox_brick_kernel<DT, kMode>), dim3(nbricks), dim3(kBrickThreads), 0, ctx->stream, vp,
-                     (const TriRec*)ctx->setup.p, (const DevMaterial*)ctx->d_materials.p, count, off, list, level0,
-                     r);
+                     (const TriRec*)ctx->setup.p, (const DevMaterial*)ctx->d_materials.p, count, off, list, lc,
+                     level0, r);
 }
 
 // Exact mean: all queued draws in one batch (brick lists span every draw).
-// Phase 1 launches up to the copy of the list size to pinned host memory,
-// phase 2 waits for it and launches the rest.
+// Once a (triangle, brick) pair count is known for this many triangles, phase
+// 1 launches everything: the list is sized from that count with a margin
+// (a brick whose range runs past it walks the triangles' boxes, so a frame
+// whose geometry grew is still exact), and this frame's count comes back
+// asynchronously for the next.  Otherwise (the first frame, or another
+// triangle count) phase 1 launches up to the copy of the count to pinned
+// host memory and phase 2 waits for it and launches the rest.
 static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, const VoxParams& vp, void* level0,
                           int phase) {
   const uint32_t res = ctx->res;
@@ -910,11 +941,32 @@ static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, cons
   DrawTable b;
   if ((rc = fill_draws(ctx, q, b, (phase & 1) != 0))) return rc;
   const uint32_t ntri = b.ntri;
+  auto bricks = [&](const uint32_t* list, ListCap lc) {
+    switch (ctx->dtype) {
+      case VCT_RGBA8: launch_brick<VCT_RGBA8>(ctx, nbricks, vp, bcount, boff, list, level0, RefPass{}, lc); break;
+      case VCT_FP16: launch_brick<VCT_FP16>(ctx, nbricks, vp, bcount, boff, list, level0, RefPass{}, lc); break;
+      case VCT_BF16: launch_brick<VCT_BF16>(ctx, nbricks, vp, bcount, boff, list, level0, RefPass{}, lc); break;
+      default: launch_brick<VCT_FP32>(ctx, nbricks, vp, bcount, boff, list, level0, RefPass{}, lc); break;
+    }
+  };
   if (phase & 1) {
+    // the count an earlier frame copied back, if it has landed (never waited for)
+    if (ctx->vox_total_pending) {
+      const hipError_t qe = hipEventQuery(ctx->vox_total_ev);
+      if (qe == hipSuccess) {
+        ctx->vox_total_pending = false;
+        ctx->vox_known_total = ctx->h_readback[3];
+      } else if (qe == hipErrorNotReady) {
+        (void)hipGetLastError();  // not an error: the copy is still in flight
+      } else {
+        return vct_hip_check(qe, "hipEventQuery");
+      }
+    }
     hipLaunchKernelGGL(zero_u32_kernel, dim3((2 * nbricks + 255) / 256), dim3(256), 0, ctx->stream, bcount,
                        (uint64_t)2 * nbricks);
     VCT_HIP(hipGetLastError());
     ctx->h_readback[2] = 0u;
+    ctx->vox_sized = ntri > 0 && ctx->vox_known_total > 0 && ctx->vox_known_ntri == ntri;
     if (ntri) {
       if ((rc = vct_ensure(ctx->setup, (size_t)ntri * sizeof(TriRec), ctx))) return rc;
       if ((rc = vct_ensure(ctx->vox_box, (size_t)ntri * 8, ctx))) return rc;
@@ -922,10 +974,29 @@ static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, cons
                          (TriRec*)ctx->setup.p, (uint2*)ctx->vox_box.p, bcount);
       VCT_HIP(hipGetLastError());
       if ((rc = vct_launch_scan_u32(ctx, bcount, boff, nbricks, counters + 2))) return rc;
-      VCT_HIP(hipMemcpyAsync(ctx->h_readback + 2, counters + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
+      if (!ctx->vox_sized) {
+        VCT_HIP(hipMemcpyAsync(ctx->h_readback + 2, counters + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
+      } else {
+        // a quarter more than the last count, at least two pairs per triangle
+        const uint64_t want = std::max<uint64_t>((uint64_t)ctx->vox_known_total + ctx->vox_known_total / 4,
+                                                 2ull * ntri + 1024u);
+        const uint32_t cap = (uint32_t)std::min<uint64_t>(want, 0xffffffffull);
+        if ((rc = vct_ensure(ctx->tri_off, (size_t)cap * 4, ctx))) return rc;
+        ctx->vox_cap = cap;
+        hipLaunchKernelGGL(vox_brick_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, ntri,
+                           (const uint2*)ctx->vox_box.p, (const uint32_t*)boff, cursor, (uint32_t*)ctx->tri_off.p,
+                           cap);
+        VCT_HIP(hipGetLastError());
+        VCT_HIP(hipMemcpyAsync(ctx->h_readback + 3, counters + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
+        VCT_HIP(hipEventRecord(ctx->vox_total_ev, ctx->stream));
+        ctx->vox_total_pending = true;
+        ctx->vox_pairs_on_device = true;
+        bricks((const uint32_t*)ctx->tri_off.p, ListCap{(const uint2*)ctx->vox_box.p, ntri, cap});
+        VCT_HIP(hipGetLastError());
+      }
     }
   }
-  if (!(phase & 2)) return VCT_OK;
+  if (!(phase & 2) || ctx->vox_sized) return VCT_OK;
   // also when nothing was counted: a copy an interrupted earlier frame left
   // in flight must not land after the word is read
   VCT_HIP(hipStreamSynchronize(ctx->stream));
@@ -933,7 +1004,8 @@ static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, cons
   if (total) {
     if ((rc = vct_ensure(ctx->tri_off, (size_t)total * 4 + (size_t)total / 4 * 4, ctx))) return rc;
     hipLaunchKernelGGL(vox_brick_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, ntri,
-                       (const uint2*)ctx->vox_box.p, (const uint32_t*)boff, cursor, (uint32_t*)ctx->tri_off.p);
+                       (const uint2*)ctx->vox_box.p, (const uint32_t*)boff, cursor, (uint32_t*)ctx->tri_off.p,
+                       total);
     VCT_HIP(hipGetLastError());
   } else {
     hipLaunchKernelGGL(zero_u32_kernel, dim3((nbricks + 255) / 256), dim3(256), 0, ctx->stream, boff,
@@ -943,13 +1015,9 @@ static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, cons
   }
   if ((rc = vct_ensure(ctx->setup, sizeof(TriRec), ctx))) return rc;
   ctx->last_candidates = total;  // (triangle, brick) pairs
-  const uint32_t* list = (const uint32_t*)ctx->tri_off.p;
-  switch (ctx->dtype) {
-    case VCT_RGBA8: launch_brick<VCT_RGBA8>(ctx, nbricks, vp, bcount, boff, list, level0); break;
-    case VCT_FP16: launch_brick<VCT_FP16>(ctx, nbricks, vp, bcount, boff, list, level0); break;
-    case VCT_BF16: launch_brick<VCT_BF16>(ctx, nbricks, vp, bcount, boff, list, level0); break;
-    default: launch_brick<VCT_FP32>(ctx, nbricks, vp, bcount, boff, list, level0); break;
-  }
+  ctx->vox_known_total = total;
+  ctx->vox_known_ntri = ntri;
+  bricks((const uint32_t*)ctx->tri_off.p, ListCap{nullptr, 0u, 0xffffffffu});
   VCT_HIP(hipGetLastError());
   return VCT_OK;
 }
@@ -982,7 +1050,7 @@ static int ref_lists(vct_context* ctx, const DrawTable& b, const VoxParams& vp, 
   if (!pairs) return VCT_OK;
   if ((rc = vct_ensure(ctx->tri_off, (size_t)pairs * 4 + (size_t)pairs / 4 * 4, ctx))) return rc;
   hipLaunchKernelGGL(vox_brick_write_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, ntri,
-                     (const uint2*)ctx->vox_box.p, (const uint32_t*)boff, cursor, (uint32_t*)ctx->tri_off.p);
+                     (const uint2*)ctx->vox_box.p, (const uint32_t*)boff, cursor, (uint32_t*)ctx->tri_off.p, pairs);
   VCT_HIP(hipGetLastError());
   return VCT_OK;
 }
@@ -1072,6 +1140,7 @@ int vct_launch_voxelize_phase(vct_context* ctx, int phase) {
   if (phase & 1) {
     ctx->last_candidates = 0;
     ctx->last_fragments = 0;
+    ctx->vox_pairs_on_device = false;
   }
   if (ctx->average == VCT_AVG_EXACT) return voxelize_exact(ctx, q, vp, level0, phase);
   return (phase & 2) ? voxelize_reference(ctx, q, vp, level0) : VCT_OK;
