@@ -833,9 +833,8 @@ __device__ __forceinline__ void shade_block(const TraceArgs& A, int px, int py, 
 // (NPIX per chunk) or tiles (16 sub-blocks of 8x8 pixels per tile, or 4 of
 // 8x8 odd-odd grid points, NSUB per chunk) for trace_list_kernel.  5 waves per
 // SIMD (96 VGPRs; the 8- and 16-bit storage paths fit without scratch, the
-// FP32 path spills 20 bytes per lane, which measured faster than 4 waves
-// without spills): the 12 paired loads of a three-map level are in flight
-// together.
+// FP32 path keeps 20 bytes per lane in scratch at this occupancy): the 12
+// paired loads of a three-map level are in flight together.
 template <int DT, class SH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void trace_frame_kernel(TraceArgs A) {
   __shared__ ShadeLds<SH> s_shade;
