@@ -88,6 +88,16 @@ VCT_HD uint32_t dtype_bytes(int dt) {
   return dt == VCT_RGBA8 ? 4u : (dt == VCT_FP32 ? 16u : 8u);
 }
 
+// One entry of the marcher's x-paired copy: texel a and its +x neighbour b.
+// 16-bit storage pairs channel by channel (four words of (c(a), c(b))), so a
+// two-term dot product weights both x-corners of a channel at once; RGBA8
+// texels pair as two words.
+VCT_HD uint4 pair_pack(uint2 a, uint2 b) {
+  return make_uint4((a.x & 0xffffu) | (b.x << 16), (a.x >> 16) | (b.x & 0xffff0000u), (a.y & 0xffffu) | (b.y << 16),
+                    (a.y >> 16) | (b.y & 0xffff0000u));
+}
+VCT_HD uint2 pair_pack(uint32_t a, uint32_t b) { return make_uint2(a, b); }
+
 // Round-to-nearest-even float -> bf16 bits (NaN kept quiet).
 VCT_HD uint16_t f32_to_bf16(float f) {
   uint32_t u = __builtin_bit_cast(uint32_t, f);
@@ -327,6 +337,10 @@ inline int vct_pass_event(vct_context* ctx, int slot, bool end, hipStream_t s) {
 // kernels launchers
 int vct_launch_mip(vct_context* ctx);
 int vct_ensure_pairs(vct_context* ctx);  // x-paired copy of the volume for the marcher
+// Offsets of the paired copy's maps (ctx->pair_off) and its buffer.
+int vct_pair_layout(vct_context* ctx);
+// The paired copy of levels first.. (the mip kernel writes levels 0 and 1).
+int vct_pairs_from(vct_context* ctx, int first);
 int vct_launch_voxel_view(vct_context* ctx);  // the voxel view blended in draw order (last G-buffer's lists)
 int vct_frame_front(vct_context* ctx);   // voxelize + mip beside the G-buffer, joined
 int vct_launch_voxelize(vct_context* ctx);

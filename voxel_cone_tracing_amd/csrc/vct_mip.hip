@@ -121,15 +121,21 @@ struct MipArgs {
   uint32_t n_src;      // texels in the source level
   int nout;            // output levels produced by this launch (1..4)
   int shared_src;      // 1: one source for all six maps (level 0)
+  // level 0 -> 1 launch, paired storage: the marcher's x-paired copy of the
+  // source level and of the first output level per map (null: not written)
+  void* pair0;
+  void* pair1[6];
+  uint32_t res;        // source level edge
 };
 
 constexpr int kThreads = 512;  // one output texel of the first level per lane
 
-// Eight consecutive stored elements as decoded texels in voxel_offsets order.
+// Eight consecutive stored elements as decoded texels in voxel_offsets order
+// (raw: as stored, in Morton order x | y << 1 | z << 2).
 template <int DT>
-__device__ __forceinline__ void load_block(const typename Codec<DT>::T* p, float4 (&v)[8]) {
+__device__ __forceinline__ void load_block(const typename Codec<DT>::T* p, float4 (&v)[8],
+                                           typename Codec<DT>::T (&raw)[8]) {
   using T = typename Codec<DT>::T;
-  T raw[8];
   if constexpr (sizeof(T) == 4) {
     const uint4* q = reinterpret_cast<const uint4*>(p);
     const uint4 a = q[0], b = q[1];
@@ -159,6 +165,8 @@ __global__ __launch_bounds__(kThreads) void mip_kernel(MipArgs a) {
   __shared__ T s1[6][512];
   __shared__ T s2[6][64];
   __shared__ T s3[6][8];
+  // x = 0 face of each source block (the paired copy, level 0 -> 1 launch)
+  __shared__ T s0[sizeof(T) < 16 ? 512 : 1][4];
 
   const int dir_lo = a.shared_src ? 0 : (int)blockIdx.y;
   const int ndir = a.shared_src ? 6 : 1;
@@ -167,23 +175,84 @@ __global__ __launch_bounds__(kThreads) void mip_kernel(MipArgs a) {
   const uint32_t t = threadIdx.x;
 
   // ---- first level: straight from the source, coalesced ----------------------
+  T raw[8];
   if (t < n1) {
     const uint32_t m = blockIdx.x * n1 + t;
     float4 v[8];
     if (a.shared_src) {
-      load_block<DT>(reinterpret_cast<const T*>(a.src[0]) + 8ull * m, v);
+      load_block<DT>(reinterpret_cast<const T*>(a.src[0]) + 8ull * m, v, raw);
 #pragma unroll
       for (int d = 0; d < 6; ++d) {
         const T e = C::enc(dir_texel_rt(d, v));
         reinterpret_cast<T*>(a.dst[0][d])[m] = e;
         s1[d][t] = e;
       }
+      if constexpr (sizeof(T) < 16) {
+        if (a.pair0) {  // the x = 0 face of this 2x2x2 block, for the -x neighbour's pairs
+#pragma unroll
+          for (int k = 0; k < 4; ++k) s0[t][k] = raw[2 * k];
+        }
+      }
     } else {
       const int d = dir_lo;
-      load_block<DT>(reinterpret_cast<const T*>(a.src[d]) + 8ull * m, v);
+      load_block<DT>(reinterpret_cast<const T*>(a.src[d]) + 8ull * m, v, raw);
       const T e = C::enc(dir_texel_rt(d, v));
       reinterpret_cast<T*>(a.dst[0][d])[m] = e;
       s1[0][t] = e;
+    }
+  }
+  // ---- the x-paired copy of the source and first output levels -----------------
+  // Entry i of a paired map holds texel i and its +x neighbour (zero past the
+  // last column).  Lane t holds the 2x2x2 source block t of the block's cube
+  // (edge: 16 texels, or the whole level when smaller) and output texel t:
+  // the +x neighbours inside the cube come from the neighbouring lane through
+  // LDS, those of the cube's last block column from the first block of the
+  // next cube in x, loaded again (its output texels recomputed from it).
+  if constexpr (sizeof(T) < 16) {
+    if (a.pair0) {
+      __syncthreads();
+      if (t < n1) {
+        const uint32_t edge = a.res < 16u ? a.res : 16u;  // cube edge in source texels
+        const uint32_t nb = edge >> 1;                     // 2x2x2 blocks per cube edge
+        const uint32_t bx = compact3(t), by = compact3(t >> 1), bz = compact3(t >> 2);
+        T nb0[4];   // x = 0 face of the +x neighbour block
+        T nb1[6];   // the +x neighbour output texel per map
+        if (bx + 1u < nb) {
+          const uint32_t tn = morton3(bx + 1u, by, bz);
+#pragma unroll
+          for (int k = 0; k < 4; ++k) nb0[k] = s0[tn][k];
+#pragma unroll
+          for (int d = 0; d < 6; ++d) nb1[d] = s1[d][tn];
+        } else {
+          const uint32_t c = blockIdx.x, cubes = a.res / edge;
+          const uint32_t cx = compact3(c) + 1u, cy = compact3(c >> 1), cz = compact3(c >> 2);
+          if (cx < cubes) {
+            const uint32_t mn = morton3(cx, cy, cz) * n1 + morton3(0u, by, bz);
+            float4 vn[8];
+            T rn[8];
+            load_block<DT>(reinterpret_cast<const T*>(a.src[0]) + 8ull * mn, vn, rn);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) nb0[k] = rn[2 * k];
+#pragma unroll
+            for (int d = 0; d < 6; ++d) nb1[d] = C::enc(dir_texel_rt(d, vn));
+          } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) memset(&nb0[k], 0, sizeof(T));
+#pragma unroll
+            for (int d = 0; d < 6; ++d) memset(&nb1[d], 0, sizeof(T));
+          }
+        }
+        using PR = decltype(pair_pack(raw[0], raw[0]));
+        const uint32_t m = blockIdx.x * n1 + t;
+        PR* p0 = reinterpret_cast<PR*>(a.pair0) + 8ull * m;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {  // (y, z) = (k & 1, k >> 1): texels 2k (x = 0) and 2k + 1 (x = 1)
+          p0[2 * k] = pair_pack(raw[2 * k], raw[2 * k + 1]);
+          p0[2 * k + 1] = pair_pack(raw[2 * k + 1], nb0[k]);
+        }
+#pragma unroll
+        for (int d = 0; d < 6; ++d) reinterpret_cast<PR*>(a.pair1[d])[m] = pair_pack(s1[d][t], nb1[d]);
+      }
     }
   }
   // ---- further levels from LDS --------------------------------------------------
@@ -208,8 +277,10 @@ __global__ __launch_bounds__(kThreads) void mip_kernel(MipArgs a) {
   }
 }
 
+// paired: the level 0 -> 1 launch also writes the x-paired copy of levels 0
+// and 1 (ctx->pair_off laid out beforehand).
 template <int DT>
-int launch_chain(vct_context* ctx) {
+int launch_chain(vct_context* ctx, bool paired) {
   const int L = ctx->levels;
   int src_level = 0;
   char* base = reinterpret_cast<char*>(ctx->volume.p);
@@ -228,6 +299,14 @@ int launch_chain(vct_context* ctx) {
     a.n_src = n_src;
     a.nout = nout;
     a.shared_src = src_level == 0 ? 1 : 0;
+    a.res = s;
+    a.pair0 = nullptr;
+    for (int d = 0; d < 6; ++d) a.pair1[d] = nullptr;
+    if (paired && src_level == 0) {
+      char* pb = reinterpret_cast<char*>(ctx->pairs.p);
+      a.pair0 = pb + ctx->pair_off[0][0];
+      for (int d = 0; d < 6; ++d) a.pair1[d] = pb + ctx->pair_off[1][d];
+    }
     dim3 grid(n_src / per_blk, a.shared_src ? 1 : 6);
     hipLaunchKernelGGL(mip_kernel<DT>, grid, dim3(kThreads), 0, ctx->stream, a);
     VCT_HIP(hipGetLastError());
@@ -239,16 +318,23 @@ int launch_chain(vct_context* ctx) {
 }  // namespace
 }  // namespace vct
 
-// The mip chain, then the cone marcher's x-paired copy of the finished volume.
+// The mip chain, then the cone marcher's x-paired copy of the finished volume:
+// the first launch of the chain writes the copy of levels 0 and 1 from the
+// blocks it holds, a copy pass the rest (1/17 of the texels).
 int vct_launch_mip(vct_context* ctx) {
   int rc;
+  const bool paired = ctx->dtype != VCT_FP32 && ctx->levels >= 2;
+  if (paired && (rc = vct_pair_layout(ctx))) return rc;
   switch (ctx->dtype) {
-    case VCT_RGBA8: rc = vct::launch_chain<VCT_RGBA8>(ctx); break;
-    case VCT_FP16: rc = vct::launch_chain<VCT_FP16>(ctx); break;
-    case VCT_BF16: rc = vct::launch_chain<VCT_BF16>(ctx); break;
-    case VCT_FP32: rc = vct::launch_chain<VCT_FP32>(ctx); break;
+    case VCT_RGBA8: rc = vct::launch_chain<VCT_RGBA8>(ctx, paired); break;
+    case VCT_FP16: rc = vct::launch_chain<VCT_FP16>(ctx, paired); break;
+    case VCT_BF16: rc = vct::launch_chain<VCT_BF16>(ctx, paired); break;
+    case VCT_FP32: rc = vct::launch_chain<VCT_FP32>(ctx, false); break;
     default: return vct_set_error(VCT_ERR_INVALID_ARG, "bad dtype");
   }
   if (rc) return rc;
-  return vct_ensure_pairs(ctx);
+  if (!paired) return vct_ensure_pairs(ctx);
+  if ((rc = vct_pairs_from(ctx, 2))) return rc;
+  ctx->pairs_version = ctx->volume_version;
+  return VCT_OK;
 }

@@ -123,7 +123,7 @@ template <>
 struct Pair<VCT_RGBA8> {
   static constexpr bool kOn = true;
   using Raw = uint2;
-  static __device__ __forceinline__ Raw make(uint32_t a, uint32_t b) { return make_uint2(a, b); }
+  static __device__ __forceinline__ Raw make(uint32_t a, uint32_t b) { return pair_pack(a, b); }
   static __device__ __forceinline__ float4 first(Raw v) { return Tex<VCT_RGBA8>::decode(v.x); }
   static __device__ __forceinline__ float4 second(Raw v) { return Tex<VCT_RGBA8>::decode(v.y); }
 };
@@ -131,10 +131,7 @@ template <int DT>
 struct Pair16 {
   static constexpr bool kOn = true;
   using Raw = uint4;
-  static __device__ __forceinline__ Raw make(uint2 a, uint2 b) {
-    return make_uint4((a.x & 0xffffu) | (b.x << 16), (a.x >> 16) | (b.x & 0xffff0000u), (a.y & 0xffffu) | (b.y << 16),
-                      (a.y >> 16) | (b.y & 0xffff0000u));
-  }
+  static __device__ __forceinline__ Raw make(uint2 a, uint2 b) { return pair_pack(a, b); }
   static __device__ __forceinline__ float4 first(Raw v) {
     return Tex<DT>::decode(make_uint2((v.x & 0xffffu) | (v.y << 16), (v.z & 0xffffu) | (v.w << 16)));
   }
@@ -926,9 +923,9 @@ struct PairArgs {
   const char* src;
   char* dst;
   uint64_t src_off[16][6], dst_off[16][6];
-  int levels;
+  int first, levels;  // levels first .. levels - 1
   uint32_t res;
-  uint64_t total;  // texels over all maps
+  uint64_t total;  // texels over all their maps
 };
 
 template <int DT>
@@ -937,12 +934,15 @@ __global__ __launch_bounds__(256) void pair_kernel(PairArgs P) {
   using PR = typename Pair<DT>::Raw;
   for (uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x; g < P.total; g += (uint64_t)gridDim.x * 256u) {
     uint64_t r = g;
-    int l = 0, d = 0;
-    uint32_t s = P.res;
+    int l = P.first, d = 0;
+    uint32_t s = P.res >> l;
     uint64_t n = (uint64_t)s * s * s;
-    if (r >= n) {
-      r -= n;
-      for (l = 1; l < P.levels; ++l) {
+    if (l > 0 || r >= n) {
+      if (l == 0) {
+        r -= n;
+        l = 1;
+      }
+      for (; l < P.levels; ++l) {
         s = P.res >> l;
         n = (uint64_t)s * s * s;
         if (r < 6 * n) break;
@@ -1086,29 +1086,37 @@ void vct_cone_set_table(int set, ConeSet& cs) {
 // Rebuilds the x-paired copy of the volume when the volume has changed since
 // it was last built (16-bit and RGBA8 storage).  The mip pass calls it last;
 // the marcher calls it again for volumes written since (uploads).
-int vct_ensure_pairs(vct_context* ctx) {
-  if (ctx->dtype == VCT_FP32 || ctx->pairs_version == ctx->volume_version) return VCT_OK;
-  PairArgs P;
-  memset(&P, 0, sizeof(P));
+int vct_pair_layout(vct_context* ctx) {
   const uint64_t bpe = dtype_bytes(ctx->dtype);
-  uint64_t bytes = 0, total = 0;
+  uint64_t bytes = 0;
   for (int l = 0; l < ctx->levels; ++l) {
-    const uint32_t s = ctx->res >> l;
-    const uint64_t n = level_elems(s);
-    const uint64_t map_bytes = (n * 2 * bpe + 255) & ~255ull;
+    const uint64_t map_bytes = (level_elems(ctx->res >> l) * 2 * bpe + 255) & ~255ull;
     for (int d = 0; d < 6; ++d) {
-      P.src_off[l][d] = ctx->lvl_off[l][d];
-      P.dst_off[l][d] = bytes;
       ctx->pair_off[l][d] = bytes;
       if (l > 0) bytes += map_bytes;
     }
     if (l == 0) bytes += map_bytes;
-    total += (l == 0 ? 1 : 6) * n;
   }
+  return vct_ensure(ctx->pairs, bytes, ctx);
+}
+
+int vct_pairs_from(vct_context* ctx, int first) {
   int rc;
-  if ((rc = vct_ensure(ctx->pairs, bytes, ctx))) return rc;
+  if ((rc = vct_pair_layout(ctx))) return rc;
+  PairArgs P;
+  memset(&P, 0, sizeof(P));
+  uint64_t total = 0;
+  for (int l = 0; l < ctx->levels; ++l) {
+    for (int d = 0; d < 6; ++d) {
+      P.src_off[l][d] = ctx->lvl_off[l][d];
+      P.dst_off[l][d] = ctx->pair_off[l][d];
+    }
+    if (l >= first) total += (l == 0 ? 1 : 6) * level_elems(ctx->res >> l);
+  }
+  if (total == 0) return VCT_OK;
   P.src = (const char*)ctx->volume.p;
   P.dst = (char*)ctx->pairs.p;
+  P.first = first;
   P.levels = ctx->levels;
   P.res = ctx->res;
   P.total = total;
@@ -1119,6 +1127,13 @@ int vct_ensure_pairs(vct_context* ctx) {
     default: hipLaunchKernelGGL(pair_kernel<VCT_BF16>, grid, dim3(256), 0, ctx->stream, P); break;
   }
   VCT_HIP(hipGetLastError());
+  return VCT_OK;
+}
+
+int vct_ensure_pairs(vct_context* ctx) {
+  if (ctx->dtype == VCT_FP32 || ctx->pairs_version == ctx->volume_version) return VCT_OK;
+  int rc;
+  if ((rc = vct_pairs_from(ctx, 0))) return rc;
   ctx->pairs_version = ctx->volume_version;
   return VCT_OK;
 }
