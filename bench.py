@@ -141,9 +141,9 @@ def algorithmic(name, wl, bpe, steps, fb_tiles):
         wr = sum(6 * (wl.res >> l) ** 3 * bpe for l in range(1, wl.levels))
         if bpe == 16:
             return rd + wr, "B", "hbm", "each source level read once, every mip texel written once"
-        vol = wl.res ** 3 * bpe + wr  # the x-paired copy: every level read once, written twice over
-        return rd + wr + 3 * vol, "B", "hbm", ("each source level read once, every mip texel written once, then the "
-                                                "whole volume read once and its x-paired copy (2x) written")
+        vol = wl.res ** 3 * bpe + wr  # the x-paired copy: the whole volume written twice over
+        return rd + wr + 2 * vol, "B", "hbm", ("each source level read once, every mip texel written once, and the "
+                                                "x-paired copy (2x the volume) written")
     if name == "gbuffer":
         return 76 * wl.ntri + 32 * W * H, "B", "hbm", "76 B per triangle read + 32 B per pixel written"
     if name == "trace":
