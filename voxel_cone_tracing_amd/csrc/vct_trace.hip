@@ -152,22 +152,20 @@ __device__ __forceinline__ float4 operator*(float s, float4 a) {
   return make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
 }
 
-// Per-workgroup sampler tables in LDS: spread3 (brick_index(x, y, z) =
-// spread[x] | spread[y] << 1 | spread[z] << 2) and the byte offset of every
-// (level, map) of the volume (a lane's level varies, so the table is indexed
-// per lane; LDS reads are far cheaper than per-lane reads of kernel
-// arguments).  Offsets are 32-bit: the volume stays below 4 GiB (checked
-// when it is allocated), so every texel load is the uniform volume pointer
-// plus one 32-bit lane offset.
+// Per-workgroup sampler tables in LDS: the byte offset of a texel within a
+// map, brick_index(x, y, z) << shift = sx[x] + sy[y] + sz[z] (spread3 of each
+// coordinate, pre-shifted by the axis and the entry size, for edges up to
+// 512: the volume's 4 GiB bound rules out 1024^3 in every storage type, and
+// the launch checks it), and the byte offset of every (level, map) of the
+// volume (a lane's level varies, so the table is indexed per lane; LDS reads
+// are far cheaper than per-lane reads of kernel arguments).  Offsets are
+// 32-bit: the volume stays below 4 GiB (checked when it is allocated), so
+// every texel load is the uniform volume pointer plus one 32-bit lane offset.
+constexpr int kMaxEdge = 512;
 struct SamplerLds {
-  uint32_t spread[1024];
+  uint32_t sx[kMaxEdge], sy[kMaxEdge], sz[kMaxEdge];
   uint32_t off[16][6];
 };
-
-__device__ __forceinline__ void sampler_init(const TraceArgs& A, SamplerLds* S) {
-  for (int q = threadIdx.x; q < 1024; q += blockDim.x) S->spread[q] = spread3((uint32_t)q);
-  for (int q = threadIdx.x; q < 16 * 6; q += blockDim.x) S->off[q / 6][q % 6] = (uint32_t)A.off[q / 6][q % 6];
-}
 
 template <int DT>
 struct Fetch {  // the unit one texel load returns: a pair where the volume is paired
@@ -188,6 +186,18 @@ struct Fetch<VCT_BF16> {
 template <int DT>
 constexpr int kTexShift = sizeof(typename Fetch<DT>::Raw) == 16 ? 4 : (sizeof(typename Fetch<DT>::Raw) == 8 ? 3 : 2);
 
+template <int DT>
+__device__ __forceinline__ void sampler_init(const TraceArgs& A, SamplerLds* S) {
+  constexpr int sh = kTexShift<DT>;
+  for (int q = threadIdx.x; q < kMaxEdge; q += blockDim.x) {
+    const uint32_t v = spread3((uint32_t)q);
+    S->sx[q] = v << sh;
+    S->sy[q] = v << (sh + 1);
+    S->sz[q] = v << (sh + 2);
+  }
+  for (int q = threadIdx.x; q < 16 * 6; q += blockDim.x) S->off[q / 6][q % 6] = (uint32_t)A.off[q / 6][q % 6];
+}
+
 // Corner byte offsets within a map: spread parts have disjoint bits, so
 // brick_index(x, y, z) << shift = ix[dx] + iyz[dy + 2 dz].  Paired volumes
 // use ix[0] only: the entry of the clamped x0 holds both x-corners, weighted
@@ -199,23 +209,24 @@ struct Corners {
 
 template <int DT>
 __device__ __forceinline__ void corners(const SamplerLds* S, float3 p, int s, Corners& c) {
-  constexpr int sh = kTexShift<DT>;
   const float fs = (float)s;
   const float u = p.x * fs - 0.5f, v = p.y * fs - 0.5f, w = p.z * fs - 0.5f;
   const float fu = floorf(u), fv = floorf(v), fw = floorf(w);
   const float au = u - fu, av = v - fv, aw = w - fw;
   const int x0 = (int)fu, y0 = (int)fv, z0 = (int)fw;
-  // Out-of-range texels take the zero border colour: weight 0, index clamped.
-  float wx0 = (x0 >= 0 && x0 < s) ? 1.0f - au : 0.0f;
-  float wx1 = (x0 + 1 >= 0 && x0 + 1 < s) ? au : 0.0f;
+  // Out-of-range texels take the zero border colour: weight 0, any valid
+  // index (0 <= i < s as one unsigned compare).
+  const uint32_t us = (uint32_t)s;
+  float wx0 = (uint32_t)x0 < us ? 1.0f - au : 0.0f;
+  float wx1 = (uint32_t)(x0 + 1) < us ? au : 0.0f;
   if (Pair<DT>::kOn && x0 == -1) {  // the pair at column 0 holds x1 = 0 first
     wx0 = au;
     wx1 = 0.0f;
   }
-  const float wy0 = (y0 >= 0 && y0 < s) ? 1.0f - av : 0.0f;
-  const float wy1 = (y0 + 1 >= 0 && y0 + 1 < s) ? av : 0.0f;
-  const float wz0 = (z0 >= 0 && z0 < s) ? 1.0f - aw : 0.0f;
-  const float wz1 = (z0 + 1 >= 0 && z0 + 1 < s) ? aw : 0.0f;
+  const float wy0 = (uint32_t)y0 < us ? 1.0f - av : 0.0f;
+  const float wy1 = (uint32_t)(y0 + 1) < us ? av : 0.0f;
+  const float wz0 = (uint32_t)z0 < us ? 1.0f - aw : 0.0f;
+  const float wz1 = (uint32_t)(z0 + 1) < us ? aw : 0.0f;
   const float w00 = wy0 * wz0, w10 = wy1 * wz0, w01 = wy0 * wz1, w11 = wy1 * wz1;
   c.w[0] = wx0 * w00;
   c.w[1] = wx1 * w00;
@@ -225,13 +236,16 @@ __device__ __forceinline__ void corners(const SamplerLds* S, float3 p, int s, Co
   c.w[5] = wx1 * w01;
   c.w[6] = wx0 * w11;
   c.w[7] = wx1 * w11;
+  // Level edges are powers of two, so i & (s - 1) is a valid index for any i;
+  // it is the texel itself whenever the weight is not zero.  Only x0 = -1
+  // must map to column 0 (the paired entry there holds its right corner).
   const int m = s - 1;
-  c.ix[0] = S->spread[min(max(x0, 0), m)] << sh;
-  c.ix[1] = S->spread[min(max(x0 + 1, 0), m)] << sh;
-  const uint32_t y0p = S->spread[min(max(y0, 0), m)] << (sh + 1);
-  const uint32_t y1p = S->spread[min(max(y0 + 1, 0), m)] << (sh + 1);
-  const uint32_t z0p = S->spread[min(max(z0, 0), m)] << (sh + 2);
-  const uint32_t z1p = S->spread[min(max(z0 + 1, 0), m)] << (sh + 2);
+  c.ix[0] = S->sx[max(x0, 0) & m];
+  c.ix[1] = S->sx[(x0 + 1) & m];
+  const uint32_t y0p = S->sy[y0 & m];
+  const uint32_t y1p = S->sy[(y0 + 1) & m];
+  const uint32_t z0p = S->sz[z0 & m];
+  const uint32_t z1p = S->sz[(z0 + 1) & m];
   c.iyz[0] = y0p + z0p;
   c.iyz[1] = y1p + z0p;
   c.iyz[2] = y0p + z1p;
@@ -837,7 +851,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void t
   __shared__ ShadeLds<SH> s_shade;
   __shared__ SamplerLds s_sampler;
   const int lane = threadIdx.x & 63, sub = (int)(threadIdx.x >> 6) % SH::NSUB;
-  sampler_init(A, &s_sampler);
+  sampler_init<DT>(A, &s_sampler);
   if (threadIdx.x == 0) s_shade.next_item = 0;
   __syncthreads();
   // Workgroups are dealt to the 8 XCDs round-robin in dispatch order.  XCD k
@@ -905,7 +919,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void t
   __shared__ ShadeLds<SH> s_shade;
   __shared__ SamplerLds s_sampler;
   if (blockIdx.x >= list_chunks(A, SH::NPIX)) return;
-  sampler_init(A, &s_sampler);
+  sampler_init<DT>(A, &s_sampler);
   if (threadIdx.x == 0) s_shade.next_item = 0;
   __syncthreads();
   int px, py;
@@ -1194,6 +1208,7 @@ int vct_trace_select(vct_context* ctx, const float4* gpos, const float4* gnrm, f
   A.grid_stride = sel.grid_stride;
   A.half_w = sel.half_w;
   int rc;
+  if (ctx->res > (uint32_t)kMaxEdge) return vct_set_error(VCT_ERR_LIMIT, "the marcher samples grids of at most 512^3");
   if ((rc = vct_ensure_pairs(ctx))) return rc;
   const bool paired = ctx->dtype != VCT_FP32;
   A.vol = (const char*)(paired ? ctx->pairs.p : ctx->volume.p);
