@@ -438,17 +438,38 @@ __device__ __forceinline__ BrickBox unpack_box(uint2 p) {
 // 1. setup (lane per stored triangle): record + count into overlapped bricks.
 __global__ __launch_bounds__(256) void vox_brick_count_kernel(DrawTable b, VoxParams vp, TriRec* __restrict__ recs,
                                                               uint2* __restrict__ boxes, uint32_t* brick_count) {
+  // each wave's 64 records (128 B each, consecutive in memory) go out through
+  // LDS as lane-contiguous 16-byte stores instead of one record per lane
+  // (stride 9: a padded record, so the lanes' 16-byte pieces spread over banks)
+  __shared__ uint4 s_out[4][64 * 9];
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const BrickGeom g = brick_geom(vp.res);
   Setup s;
   s.ok = false;
+  TriRec rec;
+  memset(&rec, 0, sizeof(rec));  // a skipped triangle keeps area 0 (set-up triangles have area > 0)
   if (t < b.ntri) {
     const int d = find_draw(b, t);
     const Draw& dr = b.d[d];
     const uint32_t lt = t - dr.tri_base;
     tri_setup(dr, lt, vp.res, vp.cube_size, s);
-    if (s.ok) recs[t] = make_rec(dr, lt, s);
-    else recs[t].area = 0;  // marks a skipped triangle (set-up triangles have area > 0)
+    if (s.ok) rec = make_rec(dr, lt, s);
+  }
+  {
+    uint4* w = s_out[wave];
+    const uint4* rp = reinterpret_cast<const uint4*>(&rec);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) w[lane * 9 + q] = rp[q];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const uint32_t t0 = t - (uint32_t)lane;
+    uint4* out = reinterpret_cast<uint4*>(recs + t0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int c = k * 64 + lane;  // 16-byte piece c of the wave's 8 KB: record c / 8, piece c % 8
+      if (t0 + (uint32_t)(c >> 3) < b.ntri) out[c] = w[(c >> 3) * 9 + (c & 7)];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
   const BrickBox bb = brick_box(s, vp.res, g);
   if (t < b.ntri) boxes[t] = pack_box(bb);
