@@ -210,6 +210,10 @@ __global__ __launch_bounds__(kThreads) void mip_kernel(MipArgs a) {
   // next cube in x, loaded again (its output texels recomputed from it).
   if constexpr (sizeof(T) < 16) {
     if (a.pair0) {
+      using PR = decltype(pair_pack(raw[0], raw[0]));
+      // 16-byte entries of whole waves go out through LDS (below)
+      const bool staged = sizeof(PR) == 16 && n1 >= 64u;
+      PR e0[8];  // (y, z) = (k & 1, k >> 1): entries 2k (x = 0) and 2k + 1 (x = 1)
       __syncthreads();
       if (t < n1) {
         const uint32_t edge = a.res < 16u ? a.res : 16u;  // cube edge in source texels
@@ -242,16 +246,43 @@ __global__ __launch_bounds__(kThreads) void mip_kernel(MipArgs a) {
             for (int d = 0; d < 6; ++d) memset(&nb1[d], 0, sizeof(T));
           }
         }
-        using PR = decltype(pair_pack(raw[0], raw[0]));
         const uint32_t m = blockIdx.x * n1 + t;
-        PR* p0 = reinterpret_cast<PR*>(a.pair0) + 8ull * m;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {  // (y, z) = (k & 1, k >> 1): texels 2k (x = 0) and 2k + 1 (x = 1)
-          p0[2 * k] = pair_pack(raw[2 * k], raw[2 * k + 1]);
-          p0[2 * k + 1] = pair_pack(raw[2 * k + 1], nb0[k]);
+        for (int k = 0; k < 4; ++k) {
+          e0[2 * k] = pair_pack(raw[2 * k], raw[2 * k + 1]);
+          e0[2 * k + 1] = pair_pack(raw[2 * k + 1], nb0[k]);
         }
 #pragma unroll
         for (int d = 0; d < 6; ++d) reinterpret_cast<PR*>(a.pair1[d])[m] = pair_pack(s1[d][t], nb1[d]);
+        if (!staged) {
+          PR* p0 = reinterpret_cast<PR*>(a.pair0) + 8ull * m;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) p0[q] = e0[q];
+        }
+      }
+      if (staged) {
+        // A lane's 8 entries are 128 contiguous bytes, so lane-by-lane
+        // stores touch 64 cache lines each.  Once every wave has read its
+        // neighbours from s0, each wave's 512 entries go out through its own
+        // 2 KB of s0 in 4 rounds of 128: 16 lanes write 8 entries each, then
+        // every lane stores 2 of them, 1 KB contiguous per store.
+        __syncthreads();
+        if (t < n1) {  // n1 is a multiple of 64 here: whole waves
+          const uint32_t ln = t & 63u;
+          PR* buf = reinterpret_cast<PR*>(&s0[t - ln][0]);
+          PR* w0 = reinterpret_cast<PR*>(a.pair0) + 8ull * (blockIdx.x * n1 + t - ln);
+#pragma unroll
+          for (uint32_t r = 0; r < 4; ++r) {
+            if ((ln >> 4) == r) {
+#pragma unroll
+              for (int q = 0; q < 8; ++q) buf[(ln & 15u) * 8u + q] = e0[q];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            w0[128u * r + ln] = buf[ln];
+            w0[128u * r + 64u + ln] = buf[64u + ln];
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          }
+        }
       }
     }
   }
