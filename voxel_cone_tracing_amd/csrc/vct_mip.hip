@@ -265,21 +265,25 @@ __global__ __launch_bounds__(kThreads) void mip_kernel(MipArgs a) {
         // stores touch 64 cache lines each.  Once every wave has read its
         // neighbours from s0, each wave's 512 entries go out through its own
         // 2 KB of s0 in 4 rounds of 128: 16 lanes write 8 entries each, then
-        // every lane stores 2 of them, 1 KB contiguous per store.
+        // every lane stores 2 of them, 1 KB contiguous per store.  Entry q of
+        // lane j sits in slot q ^ (j & 7) of its 128 bytes, so the 16 writing
+        // lanes spread over the banks instead of sharing one 16-byte column.
         __syncthreads();
         if (t < n1) {  // n1 is a multiple of 64 here: whole waves
           const uint32_t ln = t & 63u;
           PR* buf = reinterpret_cast<PR*>(&s0[t - ln][0]);
           PR* w0 = reinterpret_cast<PR*>(a.pair0) + 8ull * (blockIdx.x * n1 + t - ln);
+          const uint32_t jw = ln & 15u;  // writing lane within its round
+          const uint32_t ra = ln ^ ((ln >> 3) & 7u), rb = (64u + ln) ^ ((ln >> 3) & 7u);
 #pragma unroll
           for (uint32_t r = 0; r < 4; ++r) {
             if ((ln >> 4) == r) {
 #pragma unroll
-              for (int q = 0; q < 8; ++q) buf[(ln & 15u) * 8u + q] = e0[q];
+              for (uint32_t q = 0; q < 8; ++q) buf[jw * 8u + (q ^ (jw & 7u))] = e0[q];
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            w0[128u * r + ln] = buf[ln];
-            w0[128u * r + 64u + ln] = buf[64u + ln];
+            w0[128u * r + ln] = buf[ra];
+            w0[128u * r + 64u + ln] = buf[rb];
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
           }
         }
