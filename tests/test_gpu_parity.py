@@ -156,6 +156,61 @@ def test_voxelize_lists_past_their_capacity_bit_exact(gpu_available, orc):
         assert np.array_equal(bits(g), bits(w)), f"frame {k + 2}: level 0 differs"
 
 
+@pytest.mark.parametrize("dtype,average", [(L.FP16, L.AVG_EXACT), (L.RGBA8, L.AVG_REFERENCE)])
+def test_voxelize_follows_material_changes_bit_exact(gpu_available, orc, dtype, average):
+    """The set-up keeps the queued models' per-triangle material terms from
+    frame to frame; a material upload (upload_material_data), a model's
+    material assignment (set_model_material) and another model in the queue
+    with the same triangle count each rebuild them, so every frame matches
+    the oracle for the materials then in force."""
+    import ctypes as C
+
+    say(f"voxelize after material changes dtype={dtype} average={average}")
+    sc = SceneSpec("cornell", res=64)
+    recs = sc.mat_records.copy()
+    with Context(32, 32) as ctx:
+        sc.build(ctx, dtype, average)
+
+        def check(tri_mat, tag):
+            ctx.voxelize()
+            got = ctx.download_level(0)
+            want = orc.voxelize(sc.res, sc.cube_size, sc.M, sc.pos, sc.nrm, tri_mat, recs, sc.lights,
+                                want=("ref", "q32"))
+            w = want["ref"] if average == L.AVG_REFERENCE else encode_level(want["q32"], dtype)
+            assert np.array_equal(bits(got), bits(w)), f"{tag}: level 0 differs"
+
+        check(sc.tri_mat, "first frame")
+        check(sc.tri_mat, "second frame")
+        # a wall (material 2) turns green, emissive and transmissive
+        r = recs[2].copy()
+        r[4:7] = (0.1, 0.7, 0.3)
+        r[12:15] = (0.5, 0.6, 0.7)
+        r[16:19] = (0.2, 0.0, 0.05)
+        r[21] = 0.4
+        r[22] = np.array([4], np.int32).view(np.float32)[0]
+        recs[2] = r
+        m = L.Material.from_buffer_copy(np.ascontiguousarray(r).tobytes())
+        L.check(ctx.lib.vct_upload_material_data(ctx.h, C.byref(m), 2))
+        check(sc.tri_mat, "after the upload")
+        check(sc.tri_mat, "the frame after the upload")
+        ctx.set_model_material(2, sc.model)
+        every2 = np.full_like(sc.tri_mat, 2)
+        check(every2, "after set_model_material")
+        # another model with the same triangles and count, in the original materials
+        other = ctx.model_create(sc.pos, sc.nrm, sc.tri_mat)
+        ctx.set_model_transform(other, sc.M)
+        ctx.clear_queues()
+        ctx.queue_model(other)
+        for (p, c, i, k) in sc.lights:
+            ctx.queue_light(p, c, i, k)
+        check(sc.tri_mat, "another model queued")
+        ctx.clear_queues()
+        ctx.queue_model(sc.model)
+        for (p, c, i, k) in sc.lights:
+            ctx.queue_light(p, c, i, k)
+        check(every2, "the first model queued again")
+
+
 # ---- voxelize under a model matrix (voxelize.vert:24-30) ----------------------------
 @pytest.mark.parametrize("dtype,average", [(L.RGBA8, L.AVG_REFERENCE), (L.FP16, L.AVG_EXACT), (L.FP32, L.AVG_EXACT)])
 def test_voxelize_with_model_matrix_bit_exact(gpu_available, orc, dtype, average):

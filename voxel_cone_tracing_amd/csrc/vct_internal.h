@@ -247,6 +247,14 @@ struct vct_context {
   VctBuf vox_box;                      // voxelize: packed brick box per triangle
   VctBuf frags_sorted, vox_count, vox_off, scan_tmp, g_scan_tmp, counters;
   VctBuf setup;  // per-frame triangle list (model index, transform)
+  // voxelize: the material terms of the queued models' triangles, in frame
+  // order; rebuilt only when shade_epoch (bumped by every material change and
+  // model material assignment) or the queued models differ from those it
+  // was built for
+  VctBuf vox_shade;
+  uint64_t shade_epoch = 0, vox_shade_epoch = ~0ull;
+  std::vector<uint32_t> vox_shade_models;
+  void* vox_shade_ptr = nullptr;
   uint64_t last_fragments = 0, last_candidates = 0;
   // Exact-mean voxelize lists sized without a host wait: once a total is
   // known for the queued triangle count, the list holds vox_cap entries and a

@@ -434,7 +434,7 @@ int vct_destroy(vct_ctx ctx) {
                     &ctx->train_valid, &ctx->probe_pix, &ctx->probe_val, &ctx->fb_list, &ctx->fb_pix,
                     &ctx->fb_list2, &ctx->fb_pix2, &ctx->fb_flag, &ctx->half_img, &ctx->probe_pix2,
                     &ctx->probe_val2, &ctx->pairs, &ctx->fb_est, &ctx->g_fans, &ctx->g_clip,
-                    &ctx->train_perm, &ctx->batch_X, &ctx->batch_A, &ctx->batch_Y, &ctx->vox_box,
+                    &ctx->train_perm, &ctx->batch_X, &ctx->batch_A, &ctx->batch_Y, &ctx->vox_box, &ctx->vox_shade,
                     &ctx->vv_keys, &ctx->vv_vals, &ctx->vv_ends, &ctx->vv_temp};
   for (VctBuf* b : bufs) free_buf(*b);
   vct_upload_free(ctx->g_draws);
@@ -526,6 +526,7 @@ int vct_add_materials(vct_ctx ctx, const vct_material* m, uint32_t n, uint32_t* 
   *first_id = (uint32_t)ctx->materials.size();
   ctx->materials.insert(ctx->materials.end(), m, m + n);
   ctx->materials_dirty = true;
+  ++ctx->shade_epoch;
   return VCT_OK;
 }
 
@@ -535,6 +536,7 @@ int vct_upload_material_data(vct_ctx ctx, const vct_material* m, uint32_t id) {
   if (id >= ctx->materials.size()) return vct_set_error(VCT_ERR_BAD_HANDLE, "material id out of range");
   ctx->materials[id] = *m;
   ctx->materials_dirty = true;
+  ++ctx->shade_epoch;
   return VCT_OK;
 }
 
@@ -634,6 +636,7 @@ int vct_set_model_material(vct_ctx ctx, uint32_t material_id, uint32_t model_id)
   if (material_id >= ctx->materials.size()) return vct_set_error(VCT_ERR_BAD_HANDLE, "material id out of range");
   VctModel& m = ctx->models[model_id];
   std::fill(m.h_mat.begin(), m.h_mat.end(), material_id);
+  ++ctx->shade_epoch;
   if (m.ntri) {
     VCT_HIP(hipMemcpyAsync(m.d_mat, m.h_mat.data(), (size_t)m.ntri * 4, hipMemcpyHostToDevice, ctx->stream));
     VCT_HIP(hipStreamSynchronize(ctx->stream));

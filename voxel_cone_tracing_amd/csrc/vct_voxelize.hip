@@ -95,6 +95,33 @@ struct alignas(16) TriRec {
 };
 static_assert(sizeof(TriRec) == 128, "TriRec must be 128 bytes");
 
+// The material terms a fragment of the triangle needs (voxelize.frag:139-155),
+// gathered per triangle by the set-up kernel in frame order, so a fragment
+// reads them by triangle index rather than through its record's material id
+// (a dependent gather into the material table; on a soup of 1M triangles
+// with a material each the brick pass went from 360 to 336 us).  They depend
+// on the materials and the queued models only, so frames that change
+// neither keep them.
+struct alignas(16) VoxShade {
+  float diffuse[3], alpha;   // alpha: the dissolve of a transmissive material, else 1
+  float emission[3], pad0;
+  float tr[3], pad1;         // the transmittance of a transmissive material, else 1 (1 * c == c)
+};
+static_assert(sizeof(VoxShade) == 48, "VoxShade must be 48 bytes");
+
+__device__ __forceinline__ VoxShade make_shade(const DevMaterial& m) {
+  VoxShade h;
+  const bool t = illum_transmissive(m.illum);
+  for (int k = 0; k < 3; ++k) {
+    h.diffuse[k] = m.diffuse[k];
+    h.emission[k] = m.emission[k];
+    h.tr[k] = t ? m.transmittance[k] : 1.0f;
+  }
+  h.alpha = t ? m.dissolve : 1.0f;
+  h.pad0 = h.pad1 = 0.0f;
+  return h;
+}
+
 // The draw holding stored triangle t: the last one with tri_base <= t.
 __device__ __forceinline__ int find_draw(const DrawTable& b, uint32_t t) {
   int lo = 0, hi = b.ndraw - 1;
@@ -248,10 +275,10 @@ __device__ __forceinline__ bool vox_covers(const VEdges& e) {
 // grid (with kBrick: also when the voxel's Morton index >> bshift is not
 // bwant, before any shading).  kKeyOnly stops once the voxel is known.
 // `c` holds the coverage fields (X, Y, area, flags), `sp` the whole record
-// (read only for covered pixels).
+// and `sh` the triangle's material terms (both read only for covered pixels).
 template <bool kBrick = false, bool kKeyOnly = false, typename Rec>
-__device__ __forceinline__ bool shade_w(const Rec& c, const TriRec* sp, const VEdges& e, int i, int j, int64_t wa,
-                                        int64_t wb, int64_t wc, const VoxParams& vp, const DevMaterial* mats,
+__device__ __forceinline__ bool shade_w(const Rec& c, const TriRec* sp, const VoxShade* sh, const VEdges& e, int i,
+                                        int j, int64_t wa, int64_t wb, int64_t wc, const VoxParams& vp,
                                         uint32_t& key, float4& color, uint32_t bshift = 0, uint32_t bwant = 0) {
 #pragma clang fp contract(off)
   const int lim = 2 * (int)vp.res;
@@ -309,21 +336,13 @@ __device__ __forceinline__ bool shade_w(const Rec& c, const TriRec* sp, const VE
     float ca = c * att;
     for (int k = 0; k < 3; ++k) col[k] = col[k] + ca * L.color[k] * L.intensity;
   }
-  const DevMaterial& m = mats[s.mat];
-  float tr[3] = {1.0f, 1.0f, 1.0f};
-  float alpha = 1.0f;
-  if (illum_transmissive(m.illum)) {
-    tr[0] = m.transmittance[0];
-    tr[1] = m.transmittance[1];
-    tr[2] = m.transmittance[2];
-    alpha = m.dissolve;
-  }
+  const VoxShade m = *sh;
   float out[4];
   for (int k = 0; k < 3; ++k) {
     float c = m.diffuse[k] * col[k] + m.emission[k];
-    out[k] = tr[k] * c;
+    out[k] = m.tr[k] * c;
   }
-  out[3] = alpha;
+  out[3] = m.alpha;
   for (int k = 0; k < 4; ++k) out[k] = fminf(fmaxf(out[k], 0.0f), 1.0f);
   color = make_float4(out[0], out[1], out[2], out[3]);
   return true;
@@ -437,6 +456,8 @@ __device__ __forceinline__ BrickBox unpack_box(uint2 p) {
 
 // 1. setup (lane per stored triangle): record + count into overlapped bricks.
 __global__ __launch_bounds__(256) void vox_brick_count_kernel(DrawTable b, VoxParams vp, TriRec* __restrict__ recs,
+                                                              const DevMaterial* __restrict__ mats,
+                                                              VoxShade* __restrict__ shade, bool build_shade,
                                                               uint2* __restrict__ boxes, uint32_t* brick_count) {
   // each wave's 64 records (128 B each, consecutive in memory) go out through
   // LDS as lane-contiguous 16-byte stores instead of one record per lane
@@ -449,12 +470,22 @@ __global__ __launch_bounds__(256) void vox_brick_count_kernel(DrawTable b, VoxPa
   s.ok = false;
   TriRec rec;
   memset(&rec, 0, sizeof(rec));  // a skipped triangle keeps area 0 (set-up triangles have area > 0)
+  VoxShade h;
+  memset(&h, 0, sizeof(h));
   if (t < b.ntri) {
     const int d = find_draw(b, t);
     const Draw& dr = b.d[d];
     const uint32_t lt = t - dr.tri_base;
+    // every stored triangle's material id indexes the table (a default
+    // material always exists), so the gather goes out before the set-up's
+    // arithmetic instead of after its result
+    DevMaterial m;
+    if (build_shade) m = mats[dr.mat[lt]];
     tri_setup(dr, lt, vp.res, vp.cube_size, s);
-    if (s.ok) rec = make_rec(dr, lt, s);
+    if (s.ok) {
+      rec = make_rec(dr, lt, s);
+      if (build_shade) h = make_shade(m);
+    }
   }
   {
     uint4* w = s_out[wave];
@@ -470,6 +501,21 @@ __global__ __launch_bounds__(256) void vox_brick_count_kernel(DrawTable b, VoxPa
       if (t0 + (uint32_t)(c >> 3) < b.ntri) out[c] = w[(c >> 3) * 9 + (c & 7)];
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (build_shade) {
+      // the material terms the same way (48 B per lane: a stride of three
+      // 16-byte pieces spreads over the banks unpadded)
+      const uint4* hp = reinterpret_cast<const uint4*>(&h);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) w[lane * 3 + q] = hp[q];
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      uint4* hout = reinterpret_cast<uint4*>(shade + t0);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int c = k * 64 + lane;  // piece c of the wave's 3 KB: triangle c / 3
+        if (t0 + (uint32_t)c / 3u < b.ntri) hout[c] = w[c];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
   }
   const BrickBox bb = brick_box(s, vp.res, g);
   if (t < b.ntri) boxes[t] = pack_box(bb);
@@ -634,7 +680,7 @@ struct ListCap {
 enum { kExactMean = 0, kRefCount = 1, kRefPlace = 2 };
 template <int DT, int kMode>
 __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, const TriRec* __restrict__ recs,
-                                                                  const DevMaterial* __restrict__ mats,
+                                                                  const VoxShade* __restrict__ shade,
                                                                   const uint32_t* __restrict__ brick_count,
                                                                   const uint32_t* __restrict__ brick_off,
                                                                   const uint32_t* __restrict__ list,
@@ -756,7 +802,7 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
       vox_edges(cr, i, j, e);
       uint32_t key;
       float4 color;
-      if (!shade_w<true, kMode == kRefCount>(cr, recs + cr.tri, e, i, j, e.w[0], e.w[1], e.w[2], vp, mats, key,
+      if (!shade_w<true, kMode == kRefCount>(cr, recs + cr.tri, shade + cr.tri, e, i, j, e.w[0], e.w[1], e.w[2], vp, key,
                                              color, bshift, brick))
         return;
       const uint32_t lv = key & (nvox - 1u);
@@ -928,13 +974,35 @@ static int fill_draws(vct_context* ctx, const std::vector<uint32_t>& q, DrawTabl
   return VCT_OK;
 }
 
+static size_t setup_bytes(uint32_t ntri) { return (size_t)(ntri ? ntri : 1) * sizeof(TriRec); }
+
+// The set-up launch; it also writes the material terms of the frame's
+// triangles unless those of the last build still hold: the same queued
+// models (so the same frame order) at the same material epoch, in the same
+// buffer.
+static int launch_setup(vct_context* ctx, const std::vector<uint32_t>& q, const DrawTable& b, const VoxParams& vp,
+                        uint32_t* bcount) {
+  int rc;
+  if ((rc = vct_ensure(ctx->vox_shade, (size_t)(b.ntri ? b.ntri : 1) * sizeof(VoxShade), ctx))) return rc;
+  const bool build = !(ctx->vox_shade_epoch == ctx->shade_epoch && ctx->vox_shade_models == q &&
+                       ctx->vox_shade_ptr == ctx->vox_shade.p);
+  hipLaunchKernelGGL(vox_brick_count_kernel, dim3((b.ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
+                     (TriRec*)ctx->setup.p, (const DevMaterial*)ctx->d_materials.p, (VoxShade*)ctx->vox_shade.p,
+                     build, (uint2*)ctx->vox_box.p, bcount);
+  VCT_HIP(hipGetLastError());
+  ctx->vox_shade_epoch = ctx->shade_epoch;
+  ctx->vox_shade_models = q;
+  ctx->vox_shade_ptr = ctx->vox_shade.p;
+  return VCT_OK;
+}
+
 template <int DT, int kMode = kExactMean>
-static void launch_brick(vct_context* ctx, uint32_t nbricks, const VoxParams& vp, const uint32_t* count,
-                         const uint32_t* off, const uint32_t* list, void* level0, const RefPass& r = RefPass{},
-                         ListCap lc = ListCap{nullptr, 0u, 0xffffffffu}) {
+static void launch_brick(vct_context* ctx, uint32_t nbricks, uint32_t ntri, const VoxParams& vp,
+                         const uint32_t* count, const uint32_t* off, const uint32_t* list, void* level0,
+                         const RefPass& r = RefPass{}, ListCap lc = ListCap{nullptr, 0u, 0xffffffffu}) {
   hipLaunchKernelGGL((vox_brick_kernel<DT, kMode>), dim3(nbricks), dim3(kBrickThreads), 0, ctx->stream, vp,
-                     (const TriRec*)ctx->setup.p, (const DevMaterial*)ctx->d_materials.p, count, off, list, lc,
-                     level0, r);
+                     (const TriRec*)ctx->setup.p, (const VoxShade*)ctx->vox_shade.p, count, off, list, lc, level0,
+                     r);
 }
 
 // Exact mean: all queued draws in one batch (brick lists span every draw).
@@ -964,10 +1032,10 @@ static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, cons
   const uint32_t ntri = b.ntri;
   auto bricks = [&](const uint32_t* list, ListCap lc) {
     switch (ctx->dtype) {
-      case VCT_RGBA8: launch_brick<VCT_RGBA8>(ctx, nbricks, vp, bcount, boff, list, level0, RefPass{}, lc); break;
-      case VCT_FP16: launch_brick<VCT_FP16>(ctx, nbricks, vp, bcount, boff, list, level0, RefPass{}, lc); break;
-      case VCT_BF16: launch_brick<VCT_BF16>(ctx, nbricks, vp, bcount, boff, list, level0, RefPass{}, lc); break;
-      default: launch_brick<VCT_FP32>(ctx, nbricks, vp, bcount, boff, list, level0, RefPass{}, lc); break;
+      case VCT_RGBA8: launch_brick<VCT_RGBA8>(ctx, nbricks, ntri, vp, bcount, boff, list, level0, RefPass{}, lc); break;
+      case VCT_FP16: launch_brick<VCT_FP16>(ctx, nbricks, ntri, vp, bcount, boff, list, level0, RefPass{}, lc); break;
+      case VCT_BF16: launch_brick<VCT_BF16>(ctx, nbricks, ntri, vp, bcount, boff, list, level0, RefPass{}, lc); break;
+      default: launch_brick<VCT_FP32>(ctx, nbricks, ntri, vp, bcount, boff, list, level0, RefPass{}, lc); break;
     }
   };
   if (phase & 1) {
@@ -989,11 +1057,9 @@ static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, cons
     ctx->h_readback[2] = 0u;
     ctx->vox_sized = ntri > 0 && ctx->vox_known_total > 0 && ctx->vox_known_ntri == ntri;
     if (ntri) {
-      if ((rc = vct_ensure(ctx->setup, (size_t)ntri * sizeof(TriRec), ctx))) return rc;
+      if ((rc = vct_ensure(ctx->setup, setup_bytes(ntri), ctx))) return rc;
       if ((rc = vct_ensure(ctx->vox_box, (size_t)ntri * 8, ctx))) return rc;
-      hipLaunchKernelGGL(vox_brick_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
-                         (TriRec*)ctx->setup.p, (uint2*)ctx->vox_box.p, bcount);
-      VCT_HIP(hipGetLastError());
+      if ((rc = launch_setup(ctx, q, b, vp, bcount))) return rc;
       if ((rc = vct_launch_scan_u32(ctx, bcount, boff, nbricks, counters + 2))) return rc;
       if (!ctx->vox_sized) {
         VCT_HIP(hipMemcpyAsync(ctx->h_readback + 2, counters + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -1034,7 +1100,7 @@ static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, cons
     VCT_HIP(hipGetLastError());
     if ((rc = vct_ensure(ctx->tri_off, 16, ctx))) return rc;
   }
-  if ((rc = vct_ensure(ctx->setup, sizeof(TriRec), ctx))) return rc;
+  if ((rc = vct_ensure(ctx->setup, setup_bytes(ntri), ctx))) return rc;
   ctx->last_candidates = total;  // (triangle, brick) pairs
   ctx->vox_known_total = total;
   ctx->vox_known_ntri = ntri;
@@ -1046,7 +1112,8 @@ static int voxelize_exact(vct_context* ctx, const std::vector<uint32_t>& q, cons
 // Brick lists of the queued draws for the reference running average:
 // records, per-brick triangle counts (bcount), offsets (boff) and lists
 // (tri_off).  Returns the number of (triangle, brick) pairs in `pairs`.
-static int ref_lists(vct_context* ctx, const DrawTable& b, const VoxParams& vp, uint32_t nbricks, uint32_t& pairs) {
+static int ref_lists(vct_context* ctx, const std::vector<uint32_t>& q, const DrawTable& b, const VoxParams& vp,
+                     uint32_t nbricks, uint32_t& pairs) {
   uint32_t* counters = (uint32_t*)ctx->counters.p;
   uint32_t* bcount = (uint32_t*)ctx->tri_count.p;
   uint32_t* cursor = bcount + nbricks;
@@ -1054,16 +1121,14 @@ static int ref_lists(vct_context* ctx, const DrawTable& b, const VoxParams& vp, 
   const uint32_t ntri = b.ntri;
   int rc;
   pairs = 0;
-  if ((rc = vct_ensure(ctx->setup, (size_t)(ntri ? ntri : 1) * sizeof(TriRec), ctx))) return rc;
+  if ((rc = vct_ensure(ctx->setup, setup_bytes(ntri), ctx))) return rc;
   if ((rc = vct_ensure(ctx->tri_off, 16, ctx))) return rc;
   hipLaunchKernelGGL(zero_u32_kernel, dim3((2 * nbricks + 255) / 256), dim3(256), 0, ctx->stream, bcount,
                      (uint64_t)2 * nbricks);
   VCT_HIP(hipGetLastError());
   if (!ntri) return VCT_OK;
   if ((rc = vct_ensure(ctx->vox_box, (size_t)ntri * 8, ctx))) return rc;
-  hipLaunchKernelGGL(vox_brick_count_kernel, dim3((ntri + 255) / 256), dim3(256), 0, ctx->stream, b, vp,
-                     (TriRec*)ctx->setup.p, (uint2*)ctx->vox_box.p, bcount);
-  VCT_HIP(hipGetLastError());
+  if ((rc = launch_setup(ctx, q, b, vp, bcount))) return rc;
   if ((rc = vct_launch_scan_u32(ctx, bcount, boff, nbricks, counters + 2))) return rc;
   VCT_HIP(hipMemcpyAsync(ctx->h_readback + 2, counters + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
   VCT_HIP(hipStreamSynchronize(ctx->stream));
@@ -1100,13 +1165,14 @@ static int voxelize_reference(vct_context* ctx, const std::vector<uint32_t>& q, 
   DrawTable b;
   if ((rc = fill_draws(ctx, q, b))) return rc;
   uint32_t pairs = 0;
-  if ((rc = ref_lists(ctx, b, vp, nbricks, pairs))) return rc;
+  if ((rc = ref_lists(ctx, q, b, vp, nbricks, pairs))) return rc;
   ctx->last_candidates += pairs;
   RefPass r;
   memset(&r, 0, sizeof(r));
   r.vcount = vcount;
   // every brick's workgroup writes its voxels' counts, zeros included
-  launch_brick<VCT_RGBA8, kRefCount>(ctx, nbricks, vp, bcount, boff, (const uint32_t*)ctx->tri_off.p, nullptr, r);
+  launch_brick<VCT_RGBA8, kRefCount>(ctx, nbricks, b.ntri, vp, bcount, boff, (const uint32_t*)ctx->tri_off.p, nullptr,
+                                     r);
   VCT_HIP(hipGetLastError());
   if ((rc = vct_launch_scan_u32(ctx, vcount, (uint32_t*)ctx->vox_off.p, (uint32_t)nvox, counters + 1))) return rc;
   VCT_HIP(hipMemcpyAsync(ctx->h_readback + 2, counters + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -1125,7 +1191,8 @@ static int voxelize_reference(vct_context* ctx, const std::vector<uint32_t>& q, 
   r.s_color = (float4*)((char*)ctx->frags_sorted.p + ns * 8);
   r.nfrag = nfrag;
   r.nvox = (uint32_t)nvox;
-  launch_brick<VCT_RGBA8, kRefPlace>(ctx, nbricks, vp, bcount, boff, (const uint32_t*)ctx->tri_off.p, level0, r);
+  launch_brick<VCT_RGBA8, kRefPlace>(ctx, nbricks, b.ntri, vp, bcount, boff, (const uint32_t*)ctx->tri_off.p, level0,
+                                     r);
   VCT_HIP(hipGetLastError());
   return VCT_OK;
 }
