@@ -374,6 +374,11 @@ struct RefPass {
 // than they save on triangles shared by neighbouring bricks.
 constexpr int kBrick = 8;
 constexpr int kBrickThreads = 256;
+// (triangle, brick) items staged per round: a brick holds ~58 items on
+// average (1.9 M over 32,768 bricks of a 1M-triangle soup at 256^3), so
+// rounds of 128 rarely repeat, and the smaller stage leaves LDS for a 7th
+// workgroup per CU (the registers' limit) where 256 allowed 6
+constexpr int kBrickItems = 128;
 
 struct BrickGeom {
   uint32_t bits;    // log2(brick edge)
@@ -678,8 +683,12 @@ struct ListCap {
 };
 
 enum { kExactMean = 0, kRefCount = 1, kRefPlace = 2 };
+// Its phases wait on LDS and L2 more than they compute (fewer VALU
+// instructions per fragment or candidate measured no faster; more waves
+// did), so it runs at 8 waves per SIMD: 64 registers, no spills, and LDS for
+// 8 workgroups per CU.
 template <int DT, int kMode>
-__global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, const TriRec* __restrict__ recs,
+__global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8))) void vox_brick_kernel(VoxParams vp, const TriRec* __restrict__ recs,
                                                                   const VoxShade* __restrict__ shade,
                                                                   const uint32_t* __restrict__ brick_count,
                                                                   const uint32_t* __restrict__ brick_off,
@@ -689,11 +698,11 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
   __shared__ uint32_t s_sum[4][kMode == kExactMean ? kVox : 1];
   __shared__ uint32_t s_cnt[kVox];
   __shared__ uint32_t s_base[kMode == kRefPlace ? kVox : 1];
-  __shared__ CovRec s_cov[kBrickThreads];
-  __shared__ uint32_t s_end[kBrickThreads];  // inclusive prefix of candidate counts
-  __shared__ uint32_t s_org[kBrickThreads];  // clipped box origin: i | j << 16
-  __shared__ uint32_t s_div[kBrickThreads];  // ceil(65536 / width) | width << 20
-  __shared__ uint32_t s_wsum[kBrickThreads / 64];
+  __shared__ CovRec s_cov[kBrickItems];
+  __shared__ uint32_t s_end[kBrickItems];  // inclusive prefix of candidate counts
+  __shared__ uint32_t s_org[kBrickItems];  // clipped box origin: i | j << 16
+  __shared__ uint32_t s_div[kBrickItems];  // ceil(65536 / width) | width << 20
+  __shared__ uint32_t s_wsum[kBrickItems / 64];
   __shared__ uint32_t s_q[kBrickThreads / 64][128];  // per-wave queue of covered candidates
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const BrickGeom g = brick_geom(vp.res);
@@ -728,10 +737,10 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
   const int lim = 2 * (int)vp.res - 1;
   const int span = 2 << g.bits;  // footprint of the brick on the raster, pixels per axis
   __syncthreads();
-  for (uint32_t b0 = 0; b0 < nitems; b0 += kBrickThreads) {
+  for (uint32_t b0 = 0; b0 < nitems; b0 += kBrickItems) {
     uint32_t cnt = 0;
     uint32_t t = b0 + (uint32_t)tid;
-    bool in = t < nitems;
+    bool in = tid < kBrickItems && t < nitems;
     if (in && walk) {  // the box as pack_box stores it: first brick, bricks per axis
       const uint2 pb = lc.boxes[t];
       for (int k = 0; k < 3; ++k)
@@ -773,16 +782,20 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
       }
     }
     uint32_t x = cnt;
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
+    if (tid < kBrickItems) {
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+      }
+      if (lane == 63) s_wsum[wave] = x;
     }
-    if (lane == 63) s_wsum[wave] = x;
     __syncthreads();
-    for (int w = 0; w < wave; ++w) x += s_wsum[w];
-    s_end[tid] = x;
+    if (tid < kBrickItems) {
+      for (int w = 0; w < wave; ++w) x += s_wsum[w];
+      s_end[tid] = x;
+    }
     __syncthreads();
-    const uint32_t total = s_end[kBrickThreads - 1];
+    const uint32_t total = s_end[kBrickItems - 1];
     // Candidates in two phases.  1: the wave's 64 candidates are tested for
     // coverage only (exact edge functions, top-left rule); the covered ones
     // are appended to the wave's queue in LDS.  2: whenever the queue holds
@@ -835,12 +848,12 @@ __global__ __launch_bounds__(kBrickThreads) void vox_brick_kernel(VoxParams vp, 
       // its entry.  A window that spans more than 64 entries (entries with
       // one candidate or none) finishes with a per-lane walk.
       for (;;) {
-        const uint32_t ek = own0 + (uint32_t)lane < kBrickThreads ? s_end[own0 + lane] : 0xffffffffu;
+        const uint32_t ek = own0 + (uint32_t)lane < kBrickItems ? s_end[own0 + lane] : 0xffffffffu;
         const uint32_t k = (uint32_t)__popcll(__ballot(ek <= cb));
         own0 += k;
         if (k < 64u) break;
       }
-      const uint32_t ek = own0 + (uint32_t)lane < kBrickThreads ? s_end[own0 + lane] : 0xffffffffu;
+      const uint32_t ek = own0 + (uint32_t)lane < kBrickItems ? s_end[own0 + lane] : 0xffffffffu;
       uint32_t past = 0;
       for (uint32_t st = 32; st; st >>= 1)
         if ((uint32_t)__shfl(ek, (int)(past + st - 1u), 64) <= c) past += st;

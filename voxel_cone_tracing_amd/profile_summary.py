@@ -5,7 +5,7 @@ Kernel times (from `rocprofv3 --kernel-trace --stats --output-format csv`):
     python -m voxel_cone_tracing_amd.profile_summary times --trace DIR/run_kernel_trace.csv \\
         --cmd "<the profiled command>" --out profiles/kernel_times_soup256.md
 
-Counter utilisation (VALU / TA busy, stalls, instructions per wave) from
+Counter ratios (VALU active, TA busy, stalls, instructions per wave) from
 `rocprofv3 --pmc` runs of SQ, TA and GRBM counters:
 
     python -m voxel_cone_tracing_amd.profile_summary counters --dirs DIR1 DIR2 ... \\
@@ -144,10 +144,13 @@ def cmd_counters(a):
 
     SQ counters are sums over the shader engines, GRBM_GUI_ACTIVE a sum over
     the XCDs; with N CUs and X XCDs:
-      VALU busy = SQ_ACTIVE_INST_VALU / N / (GRBM_GUI_ACTIVE / X)
+      VALU active = SQ_ACTIVE_INST_VALU / N / (GRBM_GUI_ACTIVE / X)
       TA busy   = TA_TA_BUSY_sum / N / (GRBM_GUI_ACTIVE / X)
       wait      = SQ_WAIT_ANY / SQ_WAVE_CYCLES (share of wave time stalled)
-    and instructions per wave are the SQ_INSTS_* counts over SQ_WAVES."""
+    and instructions per wave are the SQ_INSTS_* counts over SQ_WAVES.
+    VALU active is a ratio of counter sums, not a utilisation bounded by 1:
+    it passes 100 % on kernels whose waves keep VALU work in flight on every
+    SIMD, and such a kernel can still be bound by latency."""
     agg = collections.defaultdict(lambda: collections.defaultdict(list))
     for d in a.dirs:
         for k, c in _counters(Path(d)).items():
@@ -156,10 +159,12 @@ def cmd_counters(a):
     lines = [f"# Kernel counters: {a.title}", "", "Commands (from the repository root), one per counter set:", "",
              "```", a.cmd, "```", "",
              "Summarised by `python -m voxel_cone_tracing_amd.profile_summary counters --dirs <dir>...`, means "
-             f"over launches, {a.cus} CUs in {a.xcds} XCDs.  VALU busy = SQ_ACTIVE_INST_VALU / CUs / "
-             "(GRBM_GUI_ACTIVE / XCDs); TA busy = TA_TA_BUSY_sum / CUs / (GRBM_GUI_ACTIVE / XCDs); stalled = "
-             "SQ_WAIT_ANY / SQ_WAVE_CYCLES; per-wave counts = SQ_INSTS_* / SQ_WAVES.", "",
-             "| kernel | grid (threads) | VALU busy | TA busy | stalled | VALU / wave | SALU / wave | LDS / wave "
+             f"over launches, {a.cus} CUs in {a.xcds} XCDs.  VALU active = SQ_ACTIVE_INST_VALU / CUs / "
+             "(GRBM_GUI_ACTIVE / XCDs), a ratio of counter sums that can pass 100 %: it shows where VALU work "
+             "fills the issue slots, not that the ALUs bound the kernel; TA busy = TA_TA_BUSY_sum / CUs / "
+             "(GRBM_GUI_ACTIVE / XCDs); stalled = SQ_WAIT_ANY / SQ_WAVE_CYCLES; per-wave counts = "
+             "SQ_INSTS_* / SQ_WAVES.", "",
+             "| kernel | grid (threads) | VALU active | TA busy | stalled | VALU / wave | SALU / wave | LDS / wave "
              "| VMEM rd / wave | LDS bank conflicts / LDS instr |",
              "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
 
