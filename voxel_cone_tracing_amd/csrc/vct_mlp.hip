@@ -214,12 +214,11 @@ struct InferArgs {
   int tiles_x;
 };
 
+// Each lane keeps its pixel's data in registers over the network and writes
+// its own pixel, taking the three outputs of its sample from the lane that
+// holds them (36 KB of LDS per workgroup: 4 workgroups per CU).
 __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs A) {
   __shared__ __attribute__((aligned(16))) uint16_t s_x[256 * kXS];
-  __shared__ float s_base[256][3];
-  __shared__ float s_alb[256][3];
-  __shared__ int s_st[256];
-  __shared__ int s_px[256], s_py[256];
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   if (!A.pix_list && A.skip_tile &&
       A.skip_tile[(blockIdx.y * 16 / kFbTile) * A.tiles_x + blockIdx.x * 16 / kFbTile] == 1u)
@@ -234,8 +233,6 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs A) {
       py = (int)(q / (uint32_t)A.F.W);
     }
   }
-  s_px[t] = px;
-  s_py[t] = py;
   float f[kIn], alb[3] = {0.f, 0.f, 0.f};
   float3 base = make_float3(0.f, 0.f, 0.f);
   int st = kBackground;
@@ -261,41 +258,34 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs A) {
     }
     *reinterpret_cast<uint4*>(row + k) = make_uint4(w[0], w[1], w[2], w[3]);
   }
-  s_alb[t][0] = alb[0];
-  s_alb[t][1] = alb[1];
-  s_alb[t][2] = alb[2];
-  s_base[t][0] = base.x;
-  s_base[t][1] = base.y;
-  s_base[t][2] = base.z;
-  s_st[t] = st;
   __syncthreads();
   floatx4 y[4];
   forward_wave(A.P, s_x + wave * 64 * kXS, y);
-  // lanes 0..15 hold rows 0..3 (outputs 0..2) of sample column lane
-  if (lane < 16) {
-    for (int nt = 0; nt < 4; ++nt) {
-      const int s = wave * 64 + nt * 16 + lane;
-      const int qx = s_px[s], qy = s_py[s];
-      if (qx < 0 || qy < 0 || qx >= A.F.W || qy >= A.F.H) continue;
-      float4 o;
-      if (s_st[s] == kTraceIt) {  // left for tracing
-        A.image[(size_t)qy * A.F.W + qx] = make_float4(0.f, 0.f, 0.f, -1.0f);
-        continue;
-      }
-      if (s_st[s] == kFeatures) {
-        float3 v = make_float3(s_base[s][0] + y[nt][0], s_base[s][1] + y[nt][1], s_base[s][2] + y[nt][2]);
-        if ((qx & 3) == 1 && (qy & 3) == 1 && (qx >> 2) < A.F.wl && (qy >> 2) < A.F.hl) {
-          // this pixel was traced at quarter resolution: keep the traced value
-          const float4 tv = A.F.low[(size_t)(qy >> 2) * A.F.wl + (qx >> 2)];
-          v = make_float3(tv.x, tv.y, tv.z);
-        }
-        o = make_float4(s_alb[s][0] * v.x, s_alb[s][1] * v.y, s_alb[s][2] * v.z, 1.0f);
-      } else {
-        o = make_float4(0.15f, 0.25f, 0.25f, 1.0f);  // clear colour, renderer.cpp:398
-      }
-      A.image[(size_t)qy * A.F.W + qx] = o;
-    }
+  // sample t is column (lane & 15) of the wave's column tile lane >> 4; lanes
+  // 0..15 hold rows 0..3 (outputs 0..2) of each tile's columns
+  const int src = lane & 15, tile = lane >> 4;
+  float out[3];
+  for (int i = 0; i < 3; ++i) {
+    const float v0 = __shfl(y[0][i], src, 64), v1 = __shfl(y[1][i], src, 64);
+    const float v2 = __shfl(y[2][i], src, 64), v3 = __shfl(y[3][i], src, 64);
+    out[i] = tile == 0 ? v0 : (tile == 1 ? v1 : (tile == 2 ? v2 : v3));
   }
+  if (!inside) return;
+  float4 o;
+  if (st == kTraceIt) {  // left for tracing
+    o = make_float4(0.f, 0.f, 0.f, -1.0f);
+  } else if (st == kFeatures) {
+    float3 v = make_float3(base.x + out[0], base.y + out[1], base.z + out[2]);
+    if ((px & 3) == 1 && (py & 3) == 1 && (px >> 2) < A.F.wl && (py >> 2) < A.F.hl) {
+      // this pixel was traced at quarter resolution: keep the traced value
+      const float4 tv = A.F.low[(size_t)(py >> 2) * A.F.wl + (px >> 2)];
+      v = make_float3(tv.x, tv.y, tv.z);
+    }
+    o = make_float4(alb[0] * v.x, alb[1] * v.y, alb[2] * v.z, 1.0f);
+  } else {
+    o = make_float4(0.15f, 0.25f, 0.25f, 1.0f);  // clear colour, renderer.cpp:398
+  }
+  A.image[(size_t)py * A.F.W + px] = o;
 }
 
 // ---- training samples: features, albedo and residual target at listed pixels ------
