@@ -269,15 +269,14 @@ __device__ __forceinline__ void g_edges(const GSetup& s, int i, int j, GEdges& e
 
 // Coverage (top-left rule) and barycentric weights of vertices 0, 1, 2 from
 // the edge values (wa, wb, wc) of a pixel.
-__device__ __forceinline__ bool g_cover_w(const GSetup& s, const GEdges& e, int64_t wa, int64_t wb, int64_t wc,
+__device__ __forceinline__ bool g_cover_w(bool swap, float fa, const GEdges& e, int64_t wa, int64_t wb, int64_t wc,
                                           float& b0, float& b1, float& b2) {
 #pragma clang fp contract(off)
   if (wa < 0 || wb < 0 || wc < 0) return false;
   if (wa == 0 && !g_top_left(e.ex[0], e.ey[0])) return false;
   if (wb == 0 && !g_top_left(e.ex[1], e.ey[1])) return false;
   if (wc == 0 && !g_top_left(e.ex[2], e.ey[2])) return false;
-  const int64_t w0 = wa, w1 = s.swap ? wc : wb, w2 = s.swap ? wb : wc;
-  const float fa = (float)s.area;
+  const int64_t w0 = wa, w1 = swap ? wc : wb, w2 = swap ? wb : wc;
   b0 = (float)w0 / fa;
   b1 = (float)w1 / fa;
   b2 = (float)w2 / fa;
@@ -288,7 +287,7 @@ __device__ __forceinline__ bool g_cover_w(const GSetup& s, const GEdges& e, int6
 __device__ __forceinline__ bool g_cover(const GSetup& s, int i, int j, float& b0, float& b1, float& b2) {
   GEdges e;
   g_edges(s, i, j, e);
-  return g_cover_w(s, e, e.w[0], e.w[1], e.w[2], b0, b1, b2);
+  return g_cover_w(s.swap, (float)s.area, e, e.w[0], e.w[1], e.w[2], b0, b1, b2);
 }
 
 // Per-triangle record of the bin pass, read by every tile the triangle overlaps.
@@ -617,10 +616,24 @@ constexpr int kChunk = 128;
 // (0 when 0), the clipped box and the early-depth bound.
 struct GRowSetup {
   int64_t w0[3], di[3], dj[3];
-  float inv[3];
   int16_t ci0, ci1, cj0, pad;
-  uint32_t zb;
 };
+static_assert(sizeof(GRowSetup) == 80, "GRowSetup must be 80 bytes");
+
+// What else the tile kernel reads of a chunk's triangle, beside its rows:
+// the barycentrics' divisor, the vertex depths and the depth key's triangle
+// id (the exact phase), the span reciprocals and the early-depth bound (the
+// walk).  40 B instead of the 64-B record: with the 80-B row data a chunk of
+// 128 takes 15 KB, and the workgroup 25.5 KB of LDS, so 6 fit on a CU.
+struct GTileRec {
+  float fa;       // (float)area
+  float z[3];     // window depth of each vertex
+  uint32_t otri;  // the depth key's triangle id
+  uint32_t swap;  // bit 0: vertices 1 and 2 swapped for CCW
+  uint32_t zb;    // the early-depth bound, g_zmin_bits
+  float inv[3];   // 1 / di per edge (0 when di == 0), for g_span
+};
+static_assert(sizeof(GTileRec) == 40, "GTileRec must be 40 bytes");
 
 // Narrows [lo, hi] to the columns where w + d * (i - i0) >= 0 may hold, with
 // inv_d = 1 / d computed once per triangle (0 when d == 0).  The crossing
@@ -658,7 +671,7 @@ __device__ __forceinline__ bool g_hidden(const unsigned long long* s_depth, int 
 }
 
 __device__ __forceinline__ void g_depth_test(unsigned long long* s_depth, int li, int lj, float b0, float b1,
-                                             float b2, const GRec& r) {
+                                             float b2, const GTileRec& r) {
 #pragma clang fp contract(off)
   const float depth = b0 * r.z[0] + b1 * r.z[1] + b2 * r.z[2];
   if (depth >= 0.0f && depth <= 1.0f) {
@@ -668,7 +681,9 @@ __device__ __forceinline__ void g_depth_test(unsigned long long* s_depth, int li
   }
 }
 
-__global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __restrict__ recs,
+// 6 waves per SIMD: the LDS of 6 workgroups fits a CU, and 80 registers
+// hold the kernel without spills.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void g_tile_kernel(GParams p, const GRec* __restrict__ recs,
                                                      const GRec* __restrict__ fan_recs,
                                                      const GFan* __restrict__ fans,
                                                      const uint32_t* __restrict__ tile_count,
@@ -677,7 +692,7 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
                                                      float4* __restrict__ gpos, float4* __restrict__ gnrm) {
 #pragma clang fp contract(off)
   __shared__ unsigned long long s_depth[kTile * kTile];
-  __shared__ GRec s_rec[kChunk];
+  __shared__ GTileRec s_tr[kChunk];
   __shared__ GRowSetup s_row[kChunk];
   __shared__ uint32_t s_end[kChunk];  // inclusive prefix of the row counts
   __shared__ uint32_t s_wsum[kChunk / 64];
@@ -701,18 +716,23 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
           GEdges e;
           g_edges(g_setup_of(r), ci0, cj0, e);
           GRowSetup g;
+          GTileRec tr;
           for (int k = 0; k < 3; ++k) {
             g.w0[k] = e.w[k];
             g.di[k] = e.di[k];
             g.dj[k] = e.dj[k];
-            g.inv[k] = e.di[k] ? 1.0f / (float)e.di[k] : 0.0f;
+            tr.inv[k] = e.di[k] ? 1.0f / (float)e.di[k] : 0.0f;
+            tr.z[k] = r.z[k];
           }
           g.ci0 = (int16_t)ci0;
           g.ci1 = (int16_t)ci1;
           g.cj0 = (int16_t)cj0;
           g.pad = 0;
-          g.zb = g_zmin_bits(r);
-          s_rec[tid] = r;
+          tr.fa = (float)r.area;
+          tr.otri = r.otri;
+          tr.swap = r.swap;
+          tr.zb = g_zmin_bits(r);
+          s_tr[tid] = tr;
           s_row[tid] = g;
         }
       }
@@ -747,10 +767,7 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
       const uint32_t lo = d & 127u, row = (d >> 7) & 31u;
       const int li = (int)(d >> 12);
       const GRowSetup& g = s_row[lo];
-      const GRec& r = s_rec[lo];
-      GSetup st;
-      st.area = r.area;
-      st.swap = (r.swap & 1u) != 0u;
+      const GTileRec& tr = s_tr[lo];
       GEdges e;
       for (int k = 0; k < 3; ++k) {
         e.di[k] = g.di[k];
@@ -762,8 +779,8 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
       const int64_t wb = g.w0[1] + g.dj[1] * (int64_t)row + g.di[1] * col;
       const int64_t wc = g.w0[2] + g.dj[2] * (int64_t)row + g.di[2] * col;
       float bw0, bw1, bw2;
-      if (g_cover_w(st, e, wa, wb, wc, bw0, bw1, bw2))
-        g_depth_test(s_depth, li, (int)g.cj0 + (int)row - y0, bw0, bw1, bw2, r);
+      if (g_cover_w((tr.swap & 1u) != 0u, tr.fa, e, wa, wb, wc, bw0, bw1, bw2))
+        g_depth_test(s_depth, li, (int)g.cj0 + (int)row - y0, bw0, bw1, bw2, tr);
     };
     uint32_t own0 = 0;  // the entry of the wave's first row (the same in every lane; never decreases)
     for (uint32_t c0 = (uint32_t)wave * 64u; c0 < total; c0 += 256) {
@@ -805,9 +822,10 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
         const int64_t rc = g.w0[2] + g.dj[2] * (int64_t)row;
         int hi_i = ci1;
         lo_i = ci0;
-        g_span(ra, g.di[0], g.inv[0], ci0, lo_i, hi_i);
-        g_span(rb, g.di[1], g.inv[1], ci0, lo_i, hi_i);
-        g_span(rc, g.di[2], g.inv[2], ci0, lo_i, hi_i);
+        const GTileRec& tr = s_tr[lo];
+        g_span(ra, g.di[0], tr.inv[0], ci0, lo_i, hi_i);
+        g_span(rb, g.di[1], tr.inv[1], ci0, lo_i, hi_i);
+        g_span(rc, g.di[2], tr.inv[2], ci0, lo_i, hi_i);
         if (lo_i <= hi_i) {
           len = hi_i - lo_i + 1;
           const int64_t k0 = lo_i - ci0;
@@ -817,7 +835,7 @@ __global__ __launch_bounds__(256) void g_tile_kernel(GParams p, const GRec* __re
           wa = ra + da * k0;
           wb = rb + db * k0;
           wc = rc + dc * k0;
-          zb = g.zb;
+          zb = tr.zb;
         }
       }
       // the wave walks its longest span; each lane stops at the end of its own
