@@ -854,10 +854,20 @@ __global__ __launch_bounds__(kBrickThreads) __attribute__((amdgpu_waves_per_eu(8
         if (k < 64u) break;
       }
       const uint32_t ek = own0 + (uint32_t)lane < kBrickItems ? s_end[own0 + lane] : 0xffffffffu;
+      // Only entries ending inside the window can end at or before a
+      // candidate: nend of them, the first nend lanes (the ends are sorted).  A
+      // window usually spans a few entries, and a scan of their ends read
+      // into scalar registers has no LDS round trip in its chain, where the
+      // binary search takes seven dependent shuffles; the count is the same.
+      const uint32_t nend = (uint32_t)__popcll(__ballot(ek < cb + 64u));
       uint32_t past = 0;
-      for (uint32_t st = 32; st; st >>= 1)
-        if ((uint32_t)__shfl(ek, (int)(past + st - 1u), 64) <= c) past += st;
-      if ((uint32_t)__shfl(ek, (int)past, 64) <= c) ++past;
+      if (nend <= 8u) {
+        for (uint32_t k = 0; k < nend; ++k) past += (uint32_t)__builtin_amdgcn_readlane((int)ek, (int)k) <= c ? 1u : 0u;
+      } else {
+        for (uint32_t st = 32; st; st >>= 1)
+          if ((uint32_t)__shfl(ek, (int)(past + st - 1u), 64) <= c) past += st;
+        if ((uint32_t)__shfl(ek, (int)past, 64) <= c) ++past;
+      }
       bool hit = false;
       uint32_t desc = 0;
       if (c < total) {
