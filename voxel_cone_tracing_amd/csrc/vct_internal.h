@@ -251,6 +251,9 @@ struct vct_context {
   // order; rebuilt only when shade_epoch (bumped by every material change and
   // model material assignment) or the queued models differ from those it
   // was built for
+  // vct_frame_front also launches the quarter-resolution G-buffer gather on
+  // the G-buffer's stream, right after the G-buffer (the learned frame)
+  bool front_lowres = false;
   VctBuf vox_shade;
   uint64_t shade_epoch = 0, vox_shade_epoch = ~0ull;
   std::vector<uint32_t> vox_shade_models;
@@ -359,6 +362,9 @@ int vct_launch_gbuffer(vct_context* ctx);
 // frame front runs phase 1 of both passes before either waits.
 int vct_launch_voxelize_phase(vct_context* ctx, int phase);
 int vct_launch_gbuffer_phase(vct_context* ctx, int phase);
+// The quarter-resolution G-buffer of the learned frame (its buffers already
+// sized), on the context stream.
+int vct_launch_lowres_gather(vct_context* ctx);
 int vct_launch_trace(vct_context* ctx);
 // Trace of a W x H G-buffer into out; demodulate: final / max(Kd, eps) with
 // the pixel's linear depth in w (the learned path's buffers).

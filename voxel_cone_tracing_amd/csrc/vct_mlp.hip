@@ -1033,23 +1033,37 @@ FeatArgs feat_args(vct_context* ctx) {
   return F;
 }
 
-// Quarter-resolution G-buffer (pixel (4i+1, 4j+1) of every 4x4 block) and the
-// albedo-demodulated indirect light traced there (w = linear depth).
-int trace_lowres(vct_context* ctx) {
+// The quarter-resolution buffers (pixel (4i+1, 4j+1) of every 4x4 block).
+int lowres_buffers(vct_context* ctx) {
   const int W = ctx->width, H = ctx->height;
   if (W % 4 || H % 4) return vct_set_error(VCT_ERR_STATE, "the learned path needs a viewport divisible by 4");
-  const int wl = W / 4, hl = H / 4;
-  const size_t nl = (size_t)wl * hl;
+  const size_t nl = (size_t)(W / 4) * (H / 4);
   int rc;
   if ((rc = vct_ensure(ctx->low_gpos, nl * 16, ctx))) return rc;
   if ((rc = vct_ensure(ctx->low_gnrm, nl * 16, ctx))) return rc;
-  if ((rc = vct_ensure(ctx->low_img, nl * 16, ctx))) return rc;
+  return vct_ensure(ctx->low_img, nl * 16, ctx);
+}
+
+int launch_lowres_gather(vct_context* ctx) {
+  const int W = ctx->width, H = ctx->height, wl = W / 4, hl = H / 4;
+  const size_t nl = (size_t)wl * hl;
   hipLaunchKernelGGL(lowres_gather_kernel, dim3((uint32_t)((nl + 255) / 256)), dim3(256), 0, ctx->stream,
                      (const float4*)ctx->g_pos.p, (const float4*)ctx->g_nrm.p, W, H, (float4*)ctx->low_gpos.p,
                      (float4*)ctx->low_gnrm.p, wl, hl);
   VCT_HIP(hipGetLastError());
+  return VCT_OK;
+}
+
+// Quarter-resolution G-buffer and the albedo-demodulated indirect light
+// traced there (w = linear depth); `gathered`: the frame front already
+// gathered the G-buffer (on the G-buffer's stream, joined before the mip's
+// end), so only the trace is left.
+int trace_lowres(vct_context* ctx, bool gathered = false) {
+  int rc;
+  if ((rc = lowres_buffers(ctx))) return rc;
+  if (!gathered && (rc = launch_lowres_gather(ctx))) return rc;
   return vct_trace_into(ctx, (const float4*)ctx->low_gpos.p, (const float4*)ctx->low_gnrm.p,
-                        (float4*)ctx->low_img.p, wl, hl, 1, nullptr);
+                        (float4*)ctx->low_img.p, ctx->width / 4, ctx->height / 4, 1, nullptr);
 }
 
 __global__ void zero_count_kernel(uint32_t* p) { *p = 0u; }
@@ -1311,6 +1325,8 @@ int shuffle_rows(vct_context* ctx, uint32_t n, uint64_t seed) {
 }
 
 }  // namespace
+
+int vct_launch_lowres_gather(vct_context* ctx) { return launch_lowres_gather(ctx); }
 
 #define MLP_CTX(ctx)                                                          \
   do {                                                                        \
@@ -1583,7 +1599,11 @@ int vct_render_learned(vct_ctx ctx) {
   MLP_CTX(ctx);
   int rc;
   if ((rc = need_mlp(ctx))) return rc;
-  if ((rc = vct_frame_front(ctx))) return rc;
+  if ((rc = lowres_buffers(ctx))) return rc;
+  ctx->front_lowres = true;  // the quarter-resolution G-buffer is gathered beside voxelize + mip
+  rc = vct_frame_front(ctx);
+  ctx->front_lowres = false;
+  if (rc) return rc;
   const size_t npix = (size_t)ctx->width * ctx->height;
   if ((rc = vct_ensure(ctx->image, npix * 16, ctx))) return rc;
   const bool fallback = ctx->fb_psnr_db > 0.0f;
@@ -1596,7 +1616,7 @@ int vct_render_learned(vct_ctx ctx) {
     if (rc) return rc;
     VCT_HIP(hipEventRecord(ctx->ev_join, ctx->side));
   }
-  if ((rc = vct_pass_event(ctx, 3, false, ctx->stream)) || (rc = trace_lowres(ctx)) ||
+  if ((rc = vct_pass_event(ctx, 3, false, ctx->stream)) || (rc = trace_lowres(ctx, true)) ||
       (rc = vct_pass_event(ctx, 3, true, ctx->stream)) || (rc = vct_pass_event(ctx, 4, false, ctx->stream)))
     return rc;
   // frame budget: the learned values at the probes first, the tiles to trace

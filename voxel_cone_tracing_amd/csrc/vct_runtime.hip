@@ -316,6 +316,7 @@ int vct_frame_front(vct_context* ctx) {
   if ((rc = vct_pass_event(ctx, 0, false, ctx->stream))) return rc;
   if ((rc = vct_launch_voxelize_phase(ctx, 1))) return rc;
   if ((rc = on_side([ctx] { return vct_launch_gbuffer_phase(ctx, 2); }))) return rc;
+  if (ctx->front_lowres && (rc = on_side([ctx] { return vct_launch_lowres_gather(ctx); }))) return rc;
   if ((rc = vct_pass_event(ctx, 2, true, ctx->side))) return rc;
   if ((rc = vct_launch_voxelize_phase(ctx, 2))) return rc;
   if ((rc = vct_pass_event(ctx, 0, true, ctx->stream))) return rc;
